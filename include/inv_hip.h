@@ -1,0 +1,196 @@
+/*
+ * inv_hip.h — C ABI of libinv_hip.so, the MI355X (gfx950) implementation of
+ * inv.cu's forward+adjoint gradient path.
+ *
+ * The reference exposes this path as C++ plugin classes (Solver / Misfit /
+ * Filter / Optimizer / Source, lib/index.h:32-65).  This header is the flat,
+ * FFI-friendly boundary underneath them: plain pointers, sizes and status
+ * codes, no C++ or torch types.  The C++ plugin classes in
+ * inv.cu_amd/host/ (same names and ids as the reference) sit on top of it.
+ *
+ * Conventions
+ *   - every function returns int: INV_OK (0) or a negative inv_status;
+ *     inv_last_error(ctx) gives the message (HIP error text preserved).
+ *   - a context owns one device, one HIP stream and all device memory it
+ *     allocates.  Functions are thread-compatible per context, not
+ *     thread-safe; different contexts may run concurrently.
+ *   - fields are float32 [nx][nz], z fastest (k = i*nz + j), the reference's
+ *     Dim layout (lib/device.h:303-308).  Traces are [nrec][nt] per
+ *     component; components are ordered y (if sh) then x, z (if psv).
+ *   - "host" pointers are caller-owned host memory copied in or out;
+ *     "dev" pointers are device addresses (from inv_field_ptr / inv_alloc).
+ *   - calls that return data to the host synchronise the context stream;
+ *     the others are asynchronous on it.
+ */
+#ifndef INV_HIP_H
+#define INV_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct inv_ctx inv_ctx;
+
+enum inv_status {
+    INV_OK = 0,
+    INV_ERR_ARG = -1,         /* invalid argument or size */
+    INV_ERR_HIP = -2,         /* HIP runtime error (text in inv_last_error) */
+    INV_ERR_STATE = -3,       /* call out of order (e.g. forward before model) */
+    INV_ERR_NOMEM = -4,       /* device allocation failed */
+    INV_ERR_UNSUPPORTED = -5  /* feature/combination not supported */
+};
+
+/* Field ids for inv_field_ptr. */
+enum inv_field {
+    INV_LAMBDA = 0, INV_MU = 1, INV_RHO = 2,
+    INV_K_LAMBDA = 3, INV_K_MU = 4, INV_K_RHO = 5,
+    INV_OUT = 6,     /* receiver traces of batch slot 0: [ncomp][nrec][nt] */
+    INV_ADSTF = 7,   /* adjoint sources of batch slot 0: [ncomp][nrec][nt] */
+    INV_OBS = 8      /* resident observed data: [nsrc][ncomp][nrec][nt] */
+};
+
+/* Snapshot storage formats (arithmetic is always FP32). */
+enum inv_snap_format { INV_SNAP_F32 = 0, INV_SNAP_F16 = 1, INV_SNAP_BF16 = 2 };
+
+/* Forward flags. */
+enum inv_forward_flags {
+    INV_FWD_SNAPSHOTS = 1  /* store forward snapshots for a later adjoint */
+};
+
+/* Solver description: the FdmSolver parameters read in Solver::init
+ * (solver/solver.h:158-224) and FdmSolver::init (solver/fdm.h:350-462). */
+typedef struct inv_grid_desc {
+    int nx, nz;            /* grid nodes */
+    float dx, dz, dt;      /* spacing (m) and time step (s) */
+    int nt;                /* time steps */
+    int sh, psv;           /* wave types (config keys sh, psv) */
+    int adj;               /* 1 in modes 0/2: record u, keep snapshots */
+    int trace_type;        /* 0 velocity, 1 displacement traces */
+    int sfe;               /* snapshot interval (key sfe); nt % sfe must be 0 when adj */
+    int abs_width;         /* absorbing strip width in cells */
+    float abs_alpha;       /* damping parameter */
+    int abs_left, abs_right, abs_top, abs_bottom;
+    int inv_lambda, inv_mu, inv_rho;  /* kernels accumulated by the adjoint */
+    int batch;             /* shot slots per launch (1 = plain one-shot path) */
+} inv_grid_desc;
+
+/* ---- context ---------------------------------------------------------- */
+int inv_device_count(int *count);
+int inv_ctx_create(int device, inv_ctx **out);
+int inv_ctx_destroy(inv_ctx *ctx);
+const char *inv_last_error(const inv_ctx *ctx);
+int inv_sync(inv_ctx *ctx);
+/* free/total device memory in bytes (lib/query.h:3-22) */
+int inv_mem_info(inv_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
+
+/* ---- host-side setup helpers (no device work) --------------------------- */
+/* Grid derivation of FdmSolver::init (solver/fdm.h:356-389): nx from the
+ * model point count and the aspect ratio, nz = npt / nx, dx = xmax/(nx-1). */
+int inv_grid_dims(int npt, float xmax, float zmax, int *nx, int *nz, float *dx, float *dz);
+/* calcIdx (solver/fdm.h:44-47): nearest grid index of a coordinate. */
+int inv_calc_idx(float coord, float length, int n);
+/* RickerSource::create + Source::init (source/ricker.h:5-21,
+ * source/source.h:17-37): stf = -t exp(-(pi f0 t)^2) peak-normalised, then
+ * stf_x = amp stf cos(angle), stf_y = amp stf, stf_z = amp stf sin(angle). */
+int inv_source_ricker(int nt, float dt, float f0, float t0, float angle, float amp,
+                      float *stf_x, float *stf_y, float *stf_z);
+
+/* ---- grid, model, acquisition ------------------------------------------ */
+/* FdmSolver::init (solver/fdm.h:350-462): allocates every field for
+ * desc->batch slots and builds the absorbing-boundary profile
+ * (initAbsbound, solver/fdm.h:48-74). */
+int inv_grid_set(inv_ctx *ctx, const inv_grid_desc *desc);
+/* Solver::importModel + mesh2grid + vps2lm (solver/solver.h:225-252,
+ * solver/fdm.h:4-36,463-476) for on-grid input: host [N] arrays. */
+int inv_model_upload_lmr(inv_ctx *ctx, const float *lambda, const float *mu, const float *rho);
+int inv_model_download_lmr(inv_ctx *ctx, float *lambda, float *mu, float *rho);
+/* mesh2grid (solver/fdm.h:4-24): nearest model point, strict <, first wins;
+ * then vps2lm (fdm.h:25-36) when model_type == 0.  Host point arrays [npt]. */
+int inv_model_from_points(inv_ctx *ctx, int npt, const float *px, const float *pz,
+                          const float *p1, const float *p2, const float *prho, float xmax,
+                          float zmax, int model_type);
+/* lm2vps (solver/fdm.h:37-43) into host arrays (export in vp/vs form). */
+int inv_model_to_vps(inv_ctx *ctx, float *vp, float *vs, float *rho);
+/* Source positions (grid indices from calcIdx, fdm.h:44-47,454-457) and
+ * wavelets stf_{x,y,z} [nsrc][nt] (Source::init, source/source.h:17-37). */
+int inv_sources_set(inv_ctx *ctx, int nsrc, const int *ix, const int *iz, const float *stf_x,
+                    const float *stf_y, const float *stf_z);
+int inv_receivers_set(inv_ctx *ctx, int nrec, const int *ix, const int *iz);
+
+/* ---- propagation -------------------------------------------------------- */
+/* FdmSolver::runForward (solver/fdm.h:515-570) for shots first..first+count-1
+ * in batch slots 0..count-1 (count <= batch).  first = -1, count = 1 fires
+ * every source in slot 0 (runForward(-1), modes 1 and 4 of inv.cu:46-54).
+ * Traces land in the slots' OUT buffers. */
+int inv_forward(inv_ctx *ctx, int first, int count, int flags);
+/* FdmSolver::runAdjoint (solver/fdm.h:571-612) for the same slots, driven
+ * by their ADSTF buffers; kernels accumulate into K_LAMBDA/K_MU/K_RHO. */
+int inv_adjoint(inv_ctx *ctx, int first, int count);
+/* initKernels (solver/fdm.h:495-499) */
+int inv_kernels_zero(inv_ctx *ctx);
+int inv_kernels_get(inv_ctx *ctx, float *k_lambda, float *k_mu, float *k_rho);
+int inv_kernels_set(inv_ctx *ctx, const float *k_lambda, const float *k_mu, const float *k_rho);
+
+/* Host copies of one slot's traces / adjoint sources, [ncomp][nrec][nt]. */
+int inv_traces_get(inv_ctx *ctx, int slot, float *host);
+int inv_adstf_set(inv_ctx *ctx, int slot, const float *host);
+int inv_adstf_get(inv_ctx *ctx, int slot, float *host);
+/* Observed data resident on the device (Misfit::init, misfit/misfit.h:91-151):
+ * set from host [ncomp][nrec][nt] for one shot, or copy slots' traces in. */
+int inv_obs_set(inv_ctx *ctx, int isrc, const float *host);
+int inv_obs_get(inv_ctx *ctx, int isrc, float *host);
+int inv_obs_from_traces(inv_ctx *ctx, int first, int count);
+
+/* ---- misfit ------------------------------------------------------------- */
+/* Misfit of slots 0..count-1 (shots first..) against resident obs:
+ * type 0 waveform (misfit/waveform.h:4-50), 1 envelope (misfit/envelope.h).
+ * Writes the slots' ADSTF and, if chi != NULL, per-trace misfits
+ * chi[count][nrec][ncomp] (receiver-major as Misfit::run sums them,
+ * misfit/misfit.h:31-40).  *total (FP64) = their sum. */
+int inv_misfit(inv_ctx *ctx, int first, int count, int type, double *chi, double *total);
+
+/* ---- whole gradient: Misfit::run(kernel) (misfit/misfit.h:15-51) ---------
+ * Forward every shot in batches of `batch`, misfit against resident obs,
+ * adjoint (if kernel), then the Gaussian filter (sigma > 0) on the three
+ * kernels.  *misfit = summed misfit (FP64). */
+int inv_gradient(inv_ctx *ctx, int misfit_type, int sigma, int kernel, double *misfit);
+
+/* ---- filter ------------------------------------------------------------- */
+/* GaussianFilter::apply (filter/gaussian.h:45-56), one-sided weights, in
+ * place on a device field [nx][nz]. */
+int inv_filter_gauss1s(inv_ctx *ctx, float *dev_field, int nx, int nz, int sigma);
+
+/* ---- vector ops on device arrays (lib/device.h:157-192, the p_* helpers of
+ * optimizer/optimizer.h:49-107) --------------------------------------- */
+int inv_vec_axpby(inv_ctx *ctx, float *c, float ka, const float *a, float kb, const float *b,
+                  size_t n);                                    /* c = ka*a + kb*b */
+int inv_vec_scale(inv_ctx *ctx, float *c, float ka, const float *a, size_t n); /* c = ka*a */
+int inv_vec_mul(inv_ctx *ctx, float *c, const float *a, const float *b, size_t n); /* c = a*b */
+int inv_vec_copy(inv_ctx *ctx, float *dst, const float *src, size_t n);
+int inv_vec_fill(inv_ctx *ctx, float *dst, float value, size_t n);
+int inv_vec_dot_f64(inv_ctx *ctx, const float *a, const float *b, size_t n, double *out);
+int inv_vec_amax(inv_ctx *ctx, const float *a, size_t n, float *out);  /* max |a| */
+
+/* ---- device memory ------------------------------------------------------ */
+int inv_field_ptr(inv_ctx *ctx, int field, float **out);
+int inv_alloc(inv_ctx *ctx, size_t bytes, void **out);
+int inv_free(inv_ctx *ctx, void *dev);
+int inv_memcpy_h2d(inv_ctx *ctx, void *dev, const void *host, size_t bytes);
+int inv_memcpy_d2h(inv_ctx *ctx, void *host, const void *dev, size_t bytes);
+
+/* ---- test hooks ---------------------------------------------------------
+ * Wavefields of one slot after the last forward/adjoint:
+ * SH {vy, uy, sxy, szy} then P-SV {vx, vz, ux, uz, sxx, szz, sxz}, each [N].
+ * Snapshots of one slot: [nsfe][nf][N], SH {uy, vy} then P-SV {ux, uz, vx, vz},
+ * decoded to float. */
+int inv_state_get(inv_ctx *ctx, int slot, float *host);
+int inv_snapshots_get(inv_ctx *ctx, int slot, float *host);
+/* Run only the first nsteps steps of the next forward (0 = all). */
+int inv_set_step_limit(inv_ctx *ctx, int nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,759 @@
+// extern "C" boundary of libinv_hip.so (declared in include/inv_hip.h) and the
+// per-context resource management behind it.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <new>
+
+#include "internal.h"
+
+namespace inv {
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+void DevBuf::ensure(size_t nbytes) {
+    if (nbytes <= bytes && p) return;
+    release();
+    if (nbytes == 0) nbytes = 16;
+    hipError_t e = hipMalloc(&p, nbytes);
+    if (e != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        throw Error{INV_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(nbytes) +
+                                       " bytes failed: " + hipGetErrorString(e)};
+    }
+    bytes = nbytes;
+}
+
+// Absorbing profile of initAbsbound (solver/fdm.h:48-74) as 1-D factor
+// tables: ab(i,j) = ((L(i) * R(i)) * B(j)) * T(j), the reference's product
+// order, with factor exp(-(alpha d)^2) in FP32 (absent sides give 1).
+static void build_absbound(Ctx &c) {
+    const inv_grid_desc &g = c.g;
+    const size_t w = (size_t)g.abs_width;
+    auto f = [&](long d) {
+        const float x = g.abs_alpha * (float)d;
+        return expf(-(x * x));
+    };
+    std::vector<float> ax(g.nx), bz(g.nz), tz(g.nz);
+    for (int i = 0; i < g.nx; i++) {
+        float v = 1;
+        if (g.abs_left && (size_t)i + 1 < w) v *= f((long)w - i - 1);
+        if (g.abs_right && (size_t)i > (size_t)g.nx - w) v *= f((long)w + i - g.nx);
+        ax[i] = v;
+    }
+    for (int j = 0; j < g.nz; j++) {
+        bz[j] = (g.abs_bottom && (size_t)j > (size_t)g.nz - w) ? f((long)w + j - g.nz) : 1.0f;
+        tz[j] = (g.abs_top && (size_t)j + 1 < w) ? f((long)w - j - 1) : 1.0f;
+    }
+    c.abx.ensure(g.nx * sizeof(float));
+    c.abzb.ensure(g.nz * sizeof(float));
+    c.abzt.ensure(g.nz * sizeof(float));
+    INV_HIP(hipMemcpy(c.abx.p, ax.data(), g.nx * sizeof(float), hipMemcpyHostToDevice));
+    INV_HIP(hipMemcpy(c.abzb.p, bz.data(), g.nz * sizeof(float), hipMemcpyHostToDevice));
+    INV_HIP(hipMemcpy(c.abzt.p, tz.data(), g.nz * sizeof(float), hipMemcpyHostToDevice));
+}
+
+// taper of misfit/waveform.h:9-27 in FP32 (with its double literals 0.5)
+static void build_taper(Ctx &c) {
+    const int nt = c.g.nt;
+    const float dt = c.g.dt;
+    const float pi = 3.1415927f;
+    std::vector<float> tw(nt);
+    float t_end = (nt - 1) * dt;
+    float taper_width = t_end / 10;
+    float t_min = taper_width;
+    float t_max = t_end - taper_width;
+    for (int it = 0; it < nt; it++) {
+        float t = it * dt;
+        if (t <= t_min) tw[it] = (float)(0.5 + 0.5 * cosf(pi * (t_min - t) / (taper_width)));
+        else if (t >= t_max) tw[it] = (float)(0.5 + 0.5 * cosf(pi * (t_max - t) / (taper_width)));
+        else tw[it] = 1;
+    }
+    c.taper.ensure(nt * sizeof(float));
+    INV_HIP(hipMemcpy(c.taper.p, tw.data(), nt * sizeof(float), hipMemcpyHostToDevice));
+}
+
+static void alloc_slots(Ctx &c) {
+    const size_t B = (size_t)c.g.batch, N = (size_t)c.N, f = sizeof(float);
+    auto st = [&](DevBuf &b, bool on) {
+        if (on) b.ensure(B * N * f);
+        else b.release();
+    };
+    st(c.vy, c.g.sh); st(c.uy, c.g.sh); st(c.sxy, c.g.sh); st(c.szy, c.g.sh);
+    st(c.vx, c.g.psv); st(c.vz, c.g.psv); st(c.ux, c.g.psv); st(c.uz, c.g.psv);
+    st(c.sxx, c.g.psv); st(c.szz, c.g.psv); st(c.sxz, c.g.psv);
+    c.lam.ensure(N * f);
+    c.mu.ensure(N * f);
+    c.rho.ensure(N * f);
+    c.k_lambda.ensure(N * f);
+    c.k_mu.ensure(N * f);
+    c.k_rho.ensure(N * f);
+    INV_HIP(hipMemset(c.k_lambda.p, 0, N * f));
+    INV_HIP(hipMemset(c.k_mu.p, 0, N * f));
+    INV_HIP(hipMemset(c.k_rho.p, 0, N * f));
+    if (c.g.adj && c.g.batch > 1) c.kslot.ensure(B * 3 * N * f);
+    else c.kslot.release();
+    if (c.g.adj && c.nsfe > 0) c.snap.ensure(B * (size_t)c.nsfe * c.nsnapf * N * f);
+    else c.snap.release();
+}
+
+static void alloc_traces(Ctx &c) {
+    if (!c.have_grid) return;
+    const size_t tr = (size_t)c.ncomp * std::max(c.nrec, 1) * c.g.nt;
+    c.out.ensure((size_t)c.g.batch * tr * sizeof(float));
+    c.adstf.ensure((size_t)c.g.batch * tr * sizeof(float));
+    INV_HIP(hipMemset(c.out.p, 0, (size_t)c.g.batch * tr * sizeof(float)));
+    INV_HIP(hipMemset(c.adstf.p, 0, (size_t)c.g.batch * tr * sizeof(float)));
+    c.chi.ensure((size_t)c.g.batch * std::max(c.nrec, 1) * c.ncomp * sizeof(double));
+    if (c.nsrc > 0) {
+        c.obs.ensure((size_t)c.nsrc * tr * sizeof(float));
+        INV_HIP(hipMemset(c.obs.p, 0, (size_t)c.nsrc * tr * sizeof(float)));
+    }
+}
+
+static void build_receivers(Ctx &c) {
+    std::vector<std::vector<int>> cells(1), rows(1);
+    for (int r = 0; r < c.nrec; r++) {
+        cells[0].push_back(c.rec_ix[r] * c.g.nz + c.rec_iz[r]);
+        rows[0].push_back(r);
+    }
+    build_point_list(c, c.rec_list, 1, cells, rows);
+}
+
+static void require_ready(Ctx &c) {
+    INV_REQUIRE(c.have_grid, INV_ERR_STATE, "grid not set (inv_grid_set)");
+    INV_REQUIRE(c.have_model, INV_ERR_STATE, "model not set");
+    INV_REQUIRE(c.have_src && c.have_rec, INV_ERR_STATE, "sources/receivers not set");
+}
+
+static void check_slots(Ctx &c, int first, int count) {
+    INV_REQUIRE(count >= 1 && count <= c.g.batch, INV_ERR_ARG, "count must be in [1, batch]");
+    if (first == -1) {
+        INV_REQUIRE(count == 1, INV_ERR_ARG, "first = -1 (all sources) needs count = 1");
+    } else {
+        INV_REQUIRE(first >= 0 && first + count <= c.nsrc, INV_ERR_ARG, "shot range out of bounds");
+    }
+}
+
+static void sync(Ctx &c) { INV_HIP(hipStreamSynchronize(c.stream)); }
+
+}  // namespace inv
+
+using namespace inv;
+
+struct inv_ctx : public inv::Ctx {};
+
+template <typename F>
+static int guard(inv_ctx *ctx, F &&f) {
+    if (!ctx) return INV_ERR_ARG;
+    try {
+        INV_HIP(hipSetDevice(ctx->device));
+        f(*ctx);
+        ctx->err.clear();
+        return INV_OK;
+    } catch (const inv::Error &e) {
+        ctx->err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        ctx->err = "host allocation failed";
+        return INV_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return INV_ERR_ARG;
+    }
+}
+
+extern "C" {
+
+int inv_grid_dims(int npt, float xmax, float zmax, int *nx, int *nz, float *dx, float *dz) {
+    if (!nx || !nz || !dx || !dz || npt < 4 || !(xmax > 0) || !(zmax > 0)) return INV_ERR_ARG;
+    // solver/fdm.h:382-389, in FP32 as the reference evaluates it
+    float a = zmax / xmax;
+    float b = a + 1;
+    float c = 1 - npt;
+    const size_t gx = (size_t)std::round((-b + std::sqrt(b * b - 4 * a * c)) / (2 * a)) + 1;
+    const size_t gz = (size_t)npt / gx;
+    if (gx < 2 || gz < 2) return INV_ERR_ARG;
+    *nx = (int)gx;
+    *nz = (int)gz;
+    *dx = xmax / (float)(gx - 1);
+    *dz = zmax / (float)(gz - 1);
+    return INV_OK;
+}
+
+int inv_calc_idx(float coord, float length, int n) {
+    // (int)(coord / L * (n - 1) + 0.5): FP32 product, double-literal rounding
+    return (int)((double)(coord / length * (float)(n - 1)) + 0.5);
+}
+
+int inv_source_ricker(int nt, float dt, float f0, float t0, float angle, float amp,
+                      float *stf_x, float *stf_y, float *stf_z) {
+    if (nt < 1 || !stf_x || !stf_y || !stf_z) return INV_ERR_ARG;
+    const float pi = 3.1415927f;  // host::pi, lib/device.h:336
+    std::vector<float> stf(nt);
+    float amax = 0;
+    for (int it = 0; it < nt; it++) {
+        const float t = (float)it * dt - t0;
+        const float pft = pi * f0 * t;
+        stf[it] = -t * expf(-pft * pft);
+        if (fabsf(stf[it]) > amax) amax = fabsf(stf[it]);
+    }
+    if (amax > 0)
+        for (int it = 0; it < nt; it++) stf[it] /= amax;
+    for (int it = 0; it < nt; it++) {
+        stf_x[it] = amp * stf[it] * cosf(angle);
+        stf_y[it] = amp * stf[it];
+        stf_z[it] = amp * stf[it] * sinf(angle);
+    }
+    return INV_OK;
+}
+
+int inv_device_count(int *count) {
+    if (!count) return INV_ERR_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        n = 0;
+    }
+    *count = n;
+    return INV_OK;
+}
+
+int inv_ctx_create(int device, inv_ctx **out) {
+    if (!out) return INV_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+        (void)hipGetLastError();
+        return INV_ERR_ARG;
+    }
+    inv_ctx *c = new (std::nothrow) inv_ctx();
+    if (!c) return INV_ERR_NOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        return INV_ERR_HIP;
+    }
+    *out = c;
+    return INV_OK;
+}
+
+int inv_ctx_destroy(inv_ctx *ctx) {
+    if (!ctx) return INV_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    Ctx &c = *ctx;
+    DevBuf *bufs[] = {&c.lam, &c.mu, &c.rho, &c.abx, &c.abzb, &c.abzt, &c.k_lambda, &c.k_mu,
+                      &c.k_rho, &c.kslot, &c.vx, &c.vz, &c.ux, &c.uz, &c.sxx, &c.szz, &c.sxz,
+                      &c.vy, &c.uy, &c.sxy, &c.szy, &c.snap, &c.out, &c.adstf, &c.obs, &c.chi,
+                      &c.taper, &c.red, &c.scal, &c.stf, &c.rec_list.ptr, &c.rec_list.cell,
+                      &c.rec_list.row, &c.src_list.ptr, &c.src_list.cell, &c.src_list.row,
+                      &c.gauss_w, &c.gauss_tmp, &c.gauss_sx, &c.gauss_sz};
+    for (DevBuf *b : bufs) b->release();
+    for (void *p : c.user_allocs) (void)hipFree(p);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return INV_OK;
+}
+
+const char *inv_last_error(const inv_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int inv_sync(inv_ctx *ctx) {
+    return guard(ctx, [](Ctx &c) { sync(c); });
+}
+
+int inv_mem_info(inv_ctx *ctx, size_t *free_bytes, size_t *total_bytes) {
+    return guard(ctx, [&](Ctx &) {
+        INV_REQUIRE(free_bytes && total_bytes, INV_ERR_ARG, "null output");
+        INV_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    });
+}
+
+int inv_grid_set(inv_ctx *ctx, const inv_grid_desc *d) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(d, INV_ERR_ARG, "null descriptor");
+        INV_REQUIRE(d->nx >= 5 && d->nz >= 5, INV_ERR_ARG, "grid must be at least 5x5");
+        INV_REQUIRE(d->sh || d->psv, INV_ERR_ARG, "enable sh and/or psv");
+        INV_REQUIRE(d->nt >= 1 && d->dt > 0 && d->dx > 0 && d->dz > 0, INV_ERR_ARG,
+                    "bad nt/dt/dx/dz");
+        INV_REQUIRE(d->batch >= 1, INV_ERR_ARG, "batch must be >= 1");
+        INV_REQUIRE((long)d->nx * d->nz < (1L << 31), INV_ERR_ARG, "grid too large");
+        if (d->adj) {
+            // nt % sfe != 0 makes the reference's adjoint read past the last
+            // snapshot slot (solver/fdm.h:583-585); reject it here.
+            INV_REQUIRE(d->sfe >= 1 && d->nt % d->sfe == 0, INV_ERR_ARG,
+                        "adj needs sfe >= 1 with nt % sfe == 0");
+        }
+        sync(c);
+        c.g = *d;
+        c.N = (long)d->nx * d->nz;
+        c.nsfe = (d->adj && d->sfe > 0) ? d->nt / d->sfe : 0;
+        c.ncomp = (d->sh ? 1 : 0) + (d->psv ? 2 : 0);
+        c.nsnapf = (d->sh ? 2 : 0) + (d->psv ? 4 : 0);
+        c.have_grid = true;
+        c.have_model = false;
+        build_absbound(c);
+        build_taper(c);
+        alloc_slots(c);
+        alloc_traces(c);
+        c.src_list_first = -2;
+        if (c.have_rec) build_receivers(c);
+    });
+}
+
+int inv_model_upload_lmr(inv_ctx *ctx, const float *lambda, const float *mu, const float *rho) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_grid, INV_ERR_STATE, "grid not set");
+        INV_REQUIRE(lambda && mu && rho, INV_ERR_ARG, "null model array");
+        const size_t b = c.N * sizeof(float);
+        INV_HIP(hipMemcpy(c.lam.p, lambda, b, hipMemcpyHostToDevice));
+        INV_HIP(hipMemcpy(c.mu.p, mu, b, hipMemcpyHostToDevice));
+        INV_HIP(hipMemcpy(c.rho.p, rho, b, hipMemcpyHostToDevice));
+        c.have_model = true;
+    });
+}
+
+int inv_model_download_lmr(inv_ctx *ctx, float *lambda, float *mu, float *rho) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_grid, INV_ERR_STATE, "grid not set");
+        sync(c);
+        const size_t b = c.N * sizeof(float);
+        if (lambda) INV_HIP(hipMemcpy(lambda, c.lam.p, b, hipMemcpyDeviceToHost));
+        if (mu) INV_HIP(hipMemcpy(mu, c.mu.p, b, hipMemcpyDeviceToHost));
+        if (rho) INV_HIP(hipMemcpy(rho, c.rho.p, b, hipMemcpyDeviceToHost));
+    });
+}
+
+int inv_model_from_points(inv_ctx *ctx, int npt, const float *px, const float *pz,
+                          const float *p1, const float *p2, const float *prho, float xmax,
+                          float zmax, int model_type) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_grid, INV_ERR_STATE, "grid not set");
+        INV_REQUIRE(npt > 0 && px && pz && p1 && p2 && prho, INV_ERR_ARG, "bad model points");
+        const int nx = c.g.nx, nz = c.g.nz;
+        const float dx = c.g.dx, dz = c.g.dz;
+        std::vector<float> l(c.N), m(c.N), r(c.N);
+        // Fast path: every grid node has a model point exactly on it.  Then
+        // the brute-force nearest search of mesh2grid (strict <, first wins)
+        // picks the first point at distance 0, which a lookup finds directly.
+        std::vector<int> hit(c.N, -1);
+        for (int p = 0; p < npt; p++) {
+            const float fi = px[p] / dx, fj = pz[p] / dz;
+            const long i = std::lround(fi), j = std::lround(fj);
+            if (i < 0 || i >= nx || j < 0 || j >= nz) continue;
+            if ((float)i * dx != px[p] || (float)j * dz != pz[p]) continue;
+            const long k = i * nz + j;
+            if (hit[k] < 0) hit[k] = p;
+        }
+        bool all = true;
+        for (long k = 0; k < c.N; k++) all = all && hit[k] >= 0;
+        if (all) {
+            for (long k = 0; k < c.N; k++) {
+                l[k] = p1[hit[k]];
+                m[k] = p2[hit[k]];
+                r[k] = prho[hit[k]];
+            }
+        } else {
+            // general case: nearest point by brute force, as mesh2grid
+            const float dmax = xmax * xmax + zmax * zmax;
+            for (int i = 0; i < nx; i++)
+                for (int j = 0; j < nz; j++) {
+                    const long k = (long)i * nz + j;
+                    const float x = i * dx, z = j * dz;
+                    float dmin = dmax;
+                    for (int p = 0; p < npt; p++) {
+                        const float ex = x - px[p], ez = z - pz[p];
+                        const float d = ex * ex + ez * ez;
+                        if (d < dmin) {
+                            dmin = d;
+                            l[k] = p1[p];
+                            m[k] = p2[p];
+                            r[k] = prho[p];
+                        }
+                    }
+                }
+        }
+        if (model_type == 0) {
+            // vps2lm, solver/fdm.h:25-36
+            for (long k = 0; k < c.N; k++) {
+                const float vp = l[k], vs = m[k];
+                l[k] = vp > vs ? r[k] * (vp * vp - 2 * vs * vs) : 0.0f;
+                m[k] = r[k] * vs * vs;
+            }
+        }
+        const size_t b = c.N * sizeof(float);
+        INV_HIP(hipMemcpy(c.lam.p, l.data(), b, hipMemcpyHostToDevice));
+        INV_HIP(hipMemcpy(c.mu.p, m.data(), b, hipMemcpyHostToDevice));
+        INV_HIP(hipMemcpy(c.rho.p, r.data(), b, hipMemcpyHostToDevice));
+        c.have_model = true;
+    });
+}
+
+int inv_model_to_vps(inv_ctx *ctx, float *vp, float *vs, float *rho) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_grid && vp && vs && rho, INV_ERR_ARG, "bad arguments");
+        sync(c);
+        const size_t b = c.N * sizeof(float);
+        INV_HIP(hipMemcpy(vp, c.lam.p, b, hipMemcpyDeviceToHost));
+        INV_HIP(hipMemcpy(vs, c.mu.p, b, hipMemcpyDeviceToHost));
+        INV_HIP(hipMemcpy(rho, c.rho.p, b, hipMemcpyDeviceToHost));
+        // lm2vps, solver/fdm.h:37-43
+        for (long k = 0; k < c.N; k++) {
+            const float l = vp[k], m = vs[k];
+            vp[k] = sqrtf((l + 2 * m) / rho[k]);
+            vs[k] = sqrtf(m / rho[k]);
+        }
+    });
+}
+
+int inv_sources_set(inv_ctx *ctx, int nsrc, const int *ix, const int *iz, const float *stf_x,
+                    const float *stf_y, const float *stf_z) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_grid, INV_ERR_STATE, "grid not set");
+        INV_REQUIRE(nsrc >= 1 && ix && iz && stf_x && stf_y && stf_z, INV_ERR_ARG, "bad sources");
+        for (int s = 0; s < nsrc; s++)
+            INV_REQUIRE(ix[s] >= 0 && ix[s] < c.g.nx && iz[s] >= 0 && iz[s] < c.g.nz,
+                        INV_ERR_ARG, "source outside the grid");
+        sync(c);
+        c.nsrc = nsrc;
+        c.src_ix.assign(ix, ix + nsrc);
+        c.src_iz.assign(iz, iz + nsrc);
+        const size_t b = (size_t)nsrc * c.g.nt * sizeof(float);
+        c.stf.ensure(3 * b);
+        INV_HIP(hipMemcpy(c.stf.as<char>(), stf_x, b, hipMemcpyHostToDevice));
+        INV_HIP(hipMemcpy(c.stf.as<char>() + b, stf_y, b, hipMemcpyHostToDevice));
+        INV_HIP(hipMemcpy(c.stf.as<char>() + 2 * b, stf_z, b, hipMemcpyHostToDevice));
+        c.have_src = true;
+        c.src_list_first = -2;
+        alloc_traces(c);
+    });
+}
+
+int inv_receivers_set(inv_ctx *ctx, int nrec, const int *ix, const int *iz) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_grid, INV_ERR_STATE, "grid not set");
+        INV_REQUIRE(nrec >= 1 && ix && iz, INV_ERR_ARG, "bad receivers");
+        for (int r = 0; r < nrec; r++)
+            INV_REQUIRE(ix[r] >= 0 && ix[r] < c.g.nx && iz[r] >= 0 && iz[r] < c.g.nz,
+                        INV_ERR_ARG, "receiver outside the grid");
+        sync(c);
+        c.nrec = nrec;
+        c.rec_ix.assign(ix, ix + nrec);
+        c.rec_iz.assign(iz, iz + nrec);
+        build_receivers(c);
+        c.have_rec = true;
+        alloc_traces(c);
+    });
+}
+
+int inv_forward(inv_ctx *ctx, int first, int count, int flags) {
+    return guard(ctx, [&](Ctx &c) {
+        require_ready(c);
+        check_slots(c, first, count);
+        fdm_forward(c, first, count, flags);
+    });
+}
+
+int inv_adjoint(inv_ctx *ctx, int first, int count) {
+    return guard(ctx, [&](Ctx &c) {
+        require_ready(c);
+        check_slots(c, first < 0 ? 0 : first, count);
+        INV_REQUIRE(c.g.adj && c.snap.p, INV_ERR_STATE, "adjoint needs adj = 1 (snapshots)");
+        fdm_adjoint(c, first, count);
+    });
+}
+
+int inv_kernels_zero(inv_ctx *ctx) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_grid, INV_ERR_STATE, "grid not set");
+        vec_fill(c, c.k_lambda.as<float>(), 0.0f, c.N);
+        vec_fill(c, c.k_mu.as<float>(), 0.0f, c.N);
+        vec_fill(c, c.k_rho.as<float>(), 0.0f, c.N);
+    });
+}
+
+int inv_kernels_get(inv_ctx *ctx, float *kl, float *km, float *kr) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_grid, INV_ERR_STATE, "grid not set");
+        sync(c);
+        const size_t b = c.N * sizeof(float);
+        if (kl) INV_HIP(hipMemcpy(kl, c.k_lambda.p, b, hipMemcpyDeviceToHost));
+        if (km) INV_HIP(hipMemcpy(km, c.k_mu.p, b, hipMemcpyDeviceToHost));
+        if (kr) INV_HIP(hipMemcpy(kr, c.k_rho.p, b, hipMemcpyDeviceToHost));
+    });
+}
+
+int inv_kernels_set(inv_ctx *ctx, const float *kl, const float *km, const float *kr) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_grid, INV_ERR_STATE, "grid not set");
+        sync(c);
+        const size_t b = c.N * sizeof(float);
+        if (kl) INV_HIP(hipMemcpy(c.k_lambda.p, kl, b, hipMemcpyHostToDevice));
+        if (km) INV_HIP(hipMemcpy(c.k_mu.p, km, b, hipMemcpyHostToDevice));
+        if (kr) INV_HIP(hipMemcpy(c.k_rho.p, kr, b, hipMemcpyHostToDevice));
+    });
+}
+
+static size_t trace_bytes(const Ctx &c) { return (size_t)c.ncomp * c.nrec * c.g.nt * sizeof(float); }
+
+int inv_traces_get(inv_ctx *ctx, int slot, float *host) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_rec && host && slot >= 0 && slot < c.g.batch, INV_ERR_ARG, "bad slot");
+        sync(c);
+        INV_HIP(hipMemcpy(host, c.out.as<char>() + slot * trace_bytes(c), trace_bytes(c),
+                          hipMemcpyDeviceToHost));
+    });
+}
+
+int inv_adstf_set(inv_ctx *ctx, int slot, const float *host) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_rec && host && slot >= 0 && slot < c.g.batch, INV_ERR_ARG, "bad slot");
+        sync(c);
+        INV_HIP(hipMemcpy(c.adstf.as<char>() + slot * trace_bytes(c), host, trace_bytes(c),
+                          hipMemcpyHostToDevice));
+    });
+}
+
+int inv_adstf_get(inv_ctx *ctx, int slot, float *host) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_rec && host && slot >= 0 && slot < c.g.batch, INV_ERR_ARG, "bad slot");
+        sync(c);
+        INV_HIP(hipMemcpy(host, c.adstf.as<char>() + slot * trace_bytes(c), trace_bytes(c),
+                          hipMemcpyDeviceToHost));
+    });
+}
+
+int inv_obs_set(inv_ctx *ctx, int isrc, const float *host) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_src && c.have_rec && host && isrc >= 0 && isrc < c.nsrc, INV_ERR_ARG,
+                    "bad shot");
+        sync(c);
+        INV_HIP(hipMemcpy(c.obs.as<char>() + isrc * trace_bytes(c), host, trace_bytes(c),
+                          hipMemcpyHostToDevice));
+    });
+}
+
+int inv_obs_get(inv_ctx *ctx, int isrc, float *host) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_src && c.have_rec && host && isrc >= 0 && isrc < c.nsrc, INV_ERR_ARG,
+                    "bad shot");
+        sync(c);
+        INV_HIP(hipMemcpy(host, c.obs.as<char>() + isrc * trace_bytes(c), trace_bytes(c),
+                          hipMemcpyDeviceToHost));
+    });
+}
+
+int inv_obs_from_traces(inv_ctx *ctx, int first, int count) {
+    return guard(ctx, [&](Ctx &c) {
+        require_ready(c);
+        check_slots(c, first, count);
+        INV_REQUIRE(first >= 0, INV_ERR_ARG, "first must name a shot");
+        INV_HIP(hipMemcpyAsync(c.obs.as<char>() + first * trace_bytes(c), c.out.p,
+                               count * trace_bytes(c), hipMemcpyDeviceToDevice, c.stream));
+    });
+}
+
+static double run_misfit(Ctx &c, int first, int count, int type, double *chi_host) {
+    if (type == 0) misfit_waveform(c, count, first);
+    else if (type == 1) misfit_envelope(c, count, first);
+    else throw Error{INV_ERR_ARG, "misfit type must be 0 (waveform) or 1 (envelope)"};
+    const size_t ntr = (size_t)count * c.nrec * c.ncomp;
+    c.scal.ensure(8 * sizeof(double));
+    sum_f64(c, c.chi.as<double>(), ntr, c.scal.as<double>());
+    double total = 0;
+    INV_HIP(hipMemcpyAsync(&total, c.scal.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (chi_host)
+        INV_HIP(hipMemcpyAsync(chi_host, c.chi.p, ntr * sizeof(double), hipMemcpyDeviceToHost,
+                               c.stream));
+    sync(c);
+    return total;
+}
+
+int inv_misfit(inv_ctx *ctx, int first, int count, int type, double *chi, double *total) {
+    return guard(ctx, [&](Ctx &c) {
+        require_ready(c);
+        check_slots(c, first, count);
+        INV_REQUIRE(first >= 0, INV_ERR_ARG, "first must name a shot");
+        const double t = run_misfit(c, first, count, type, chi);
+        if (total) *total = t;
+    });
+}
+
+int inv_gradient(inv_ctx *ctx, int misfit_type, int sigma, int kernel, double *misfit) {
+    return guard(ctx, [&](Ctx &c) {
+        require_ready(c);
+        INV_REQUIRE(c.g.adj, INV_ERR_STATE, "gradient needs adj = 1");
+        if (kernel) {
+            vec_fill(c, c.k_lambda.as<float>(), 0.0f, c.N);
+            vec_fill(c, c.k_mu.as<float>(), 0.0f, c.N);
+            vec_fill(c, c.k_rho.as<float>(), 0.0f, c.N);
+        }
+        double total = 0;
+        for (int first = 0; first < c.nsrc; first += c.g.batch) {
+            const int count = std::min(c.g.batch, c.nsrc - first);
+            fdm_forward(c, first, count, kernel ? INV_FWD_SNAPSHOTS : 0);
+            total += run_misfit(c, first, count, misfit_type, nullptr);
+            if (kernel) fdm_adjoint(c, first, count);
+        }
+        if (kernel && sigma > 0) {
+            gauss_filter(c, c.k_lambda.as<float>(), c.g.nx, c.g.nz, sigma);
+            gauss_filter(c, c.k_mu.as<float>(), c.g.nx, c.g.nz, sigma);
+            gauss_filter(c, c.k_rho.as<float>(), c.g.nx, c.g.nz, sigma);
+        }
+        sync(c);
+        if (misfit) *misfit = total;
+    });
+}
+
+int inv_filter_gauss1s(inv_ctx *ctx, float *dev_field, int nx, int nz, int sigma) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(dev_field, INV_ERR_ARG, "null field");
+        gauss_filter(c, dev_field, nx, nz, sigma);
+    });
+}
+
+int inv_vec_axpby(inv_ctx *ctx, float *cc, float ka, const float *a, float kb, const float *b,
+                  size_t n) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(cc && a && b, INV_ERR_ARG, "null vector");
+        vec_axpby(c, cc, ka, a, kb, b, n);
+    });
+}
+int inv_vec_scale(inv_ctx *ctx, float *cc, float ka, const float *a, size_t n) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(cc && a, INV_ERR_ARG, "null vector");
+        vec_scale(c, cc, ka, a, n);
+    });
+}
+int inv_vec_mul(inv_ctx *ctx, float *cc, const float *a, const float *b, size_t n) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(cc && a && b, INV_ERR_ARG, "null vector");
+        vec_mul(c, cc, a, b, n);
+    });
+}
+int inv_vec_copy(inv_ctx *ctx, float *dst, const float *src, size_t n) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(dst && src, INV_ERR_ARG, "null vector");
+        vec_copy(c, dst, src, n);
+    });
+}
+int inv_vec_fill(inv_ctx *ctx, float *dst, float value, size_t n) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(dst, INV_ERR_ARG, "null vector");
+        vec_fill(c, dst, value, n);
+    });
+}
+int inv_vec_dot_f64(inv_ctx *ctx, const float *a, const float *b, size_t n, double *out) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(a && b && out, INV_ERR_ARG, "null argument");
+        *out = vec_dot(c, a, b, n);
+    });
+}
+int inv_vec_amax(inv_ctx *ctx, const float *a, size_t n, float *out) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(a && out, INV_ERR_ARG, "null argument");
+        *out = vec_amax(c, a, n);
+    });
+}
+
+int inv_field_ptr(inv_ctx *ctx, int field, float **out) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(out, INV_ERR_ARG, "null output");
+        INV_REQUIRE(c.have_grid, INV_ERR_STATE, "grid not set");
+        float *p = nullptr;
+        switch (field) {
+            case INV_LAMBDA: p = c.lam.as<float>(); break;
+            case INV_MU: p = c.mu.as<float>(); break;
+            case INV_RHO: p = c.rho.as<float>(); break;
+            case INV_K_LAMBDA: p = c.k_lambda.as<float>(); break;
+            case INV_K_MU: p = c.k_mu.as<float>(); break;
+            case INV_K_RHO: p = c.k_rho.as<float>(); break;
+            case INV_OUT: p = c.out.as<float>(); break;
+            case INV_ADSTF: p = c.adstf.as<float>(); break;
+            case INV_OBS: p = c.obs.as<float>(); break;
+            default: throw Error{INV_ERR_ARG, "unknown field id"};
+        }
+        INV_REQUIRE(p, INV_ERR_STATE, "field not allocated");
+        *out = p;
+    });
+}
+
+int inv_alloc(inv_ctx *ctx, size_t bytes, void **out) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(out && bytes > 0, INV_ERR_ARG, "bad allocation request");
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error{INV_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e)};
+        }
+        c.user_allocs.push_back(p);
+        *out = p;
+    });
+}
+
+int inv_free(inv_ctx *ctx, void *dev) {
+    return guard(ctx, [&](Ctx &c) {
+        auto it = std::find(c.user_allocs.begin(), c.user_allocs.end(), dev);
+        INV_REQUIRE(it != c.user_allocs.end(), INV_ERR_ARG, "pointer not from inv_alloc");
+        sync(c);
+        INV_HIP(hipFree(dev));
+        c.user_allocs.erase(it);
+    });
+}
+
+int inv_memcpy_h2d(inv_ctx *ctx, void *dev, const void *host, size_t bytes) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(dev && host, INV_ERR_ARG, "null pointer");
+        sync(c);
+        INV_HIP(hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice));
+    });
+}
+
+int inv_memcpy_d2h(inv_ctx *ctx, void *host, const void *dev, size_t bytes) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(dev && host, INV_ERR_ARG, "null pointer");
+        sync(c);
+        INV_HIP(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int inv_state_get(inv_ctx *ctx, int slot, float *host) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_grid && host && slot >= 0 && slot < c.g.batch, INV_ERR_ARG, "bad slot");
+        sync(c);
+        const size_t b = c.N * sizeof(float);
+        int f = 0;
+        auto get = [&](DevBuf &d) {
+            INV_HIP(hipMemcpy(reinterpret_cast<char *>(host) + (size_t)f++ * b,
+                              d.as<char>() + (size_t)slot * b, b, hipMemcpyDeviceToHost));
+        };
+        if (c.g.sh) { get(c.vy); get(c.uy); get(c.sxy); get(c.szy); }
+        if (c.g.psv) { get(c.vx); get(c.vz); get(c.ux); get(c.uz); get(c.sxx); get(c.szz); get(c.sxz); }
+    });
+}
+
+int inv_snapshots_get(inv_ctx *ctx, int slot, float *host) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.snap.p && host && slot >= 0 && slot < c.g.batch, INV_ERR_ARG, "bad slot");
+        sync(c);
+        const size_t b = (size_t)c.nsfe * c.nsnapf * c.N * sizeof(float);
+        INV_HIP(hipMemcpy(host, c.snap.as<char>() + slot * b, b, hipMemcpyDeviceToHost));
+    });
+}
+
+int inv_set_step_limit(inv_ctx *ctx, int nsteps) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(nsteps >= 0, INV_ERR_ARG, "nsteps must be >= 0");
+        c.step_limit = nsteps;
+    });
+}
+
+}  // extern "C"

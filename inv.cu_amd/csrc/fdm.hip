@@ -1,0 +1,547 @@
+// Velocity-stress propagation on gfx950: the forward and adjoint time loops of
+// FdmSolver (solver/fdm.h:515-612) as fused per-half-step kernels.
+//
+// Per time step the reference launches divS -> addSTF -> updateV -> divV ->
+// updateS -> saveRec (six to eight kernels through scratch fields dsx.., dvxdx..).
+// Here one kernel does the velocity half-step (stress divergence, source
+// injection, velocity/displacement update, receiver recording, snapshot store)
+// and one the stress half-step (velocity gradient and Hooke update); the
+// divergence scratch never touches HBM.  Every shot slot of a batch is one
+// grid z-layer of the same launch.  The arithmetic keeps the reference's
+// expression order and is compiled with -ffp-contract=off, so the FP32 result
+// of each cell is the same as the reference's formula evaluated in IEEE FP32.
+#include <algorithm>
+
+#include "internal.h"
+
+namespace inv {
+
+// 4th-order staggered differences (solver/fdm.h:79,85,106,112):
+//   D-f = 9 (f[k] - f[k-s]) / (8h) - (f[k+s] - f[k-2s]) / (24h)
+//   D+f = 9 (f[k+s] - f[k]) / (8h) - (f[k+2s] - f[k-s]) / (24h)
+// c8 = 8h and c24 = 24h are formed in FP32 on the host, as the reference forms
+// them per use.
+__device__ __forceinline__ float d_minus(const float *__restrict__ f, long k, long s, float c8,
+                                         float c24) {
+    return 9.0f * (f[k] - f[k - s]) / c8 - (f[k + s] - f[k - 2 * s]) / c24;
+}
+__device__ __forceinline__ float d_plus(const float *__restrict__ f, long k, long s, float c8,
+                                        float c24) {
+    return 9.0f * (f[k + s] - f[k]) / c8 - (f[k + 2 * s] - f[k - s]) / c24;
+}
+
+struct Geo {
+    int nx, nz;
+    long N;
+    float dt, c8dx, c24dx, c8dz, c24dz;
+};
+
+struct VelArgs {
+    Geo g;
+    const float *rho, *abx, *abzb, *abzt;
+    float *vx, *vz, *ux, *uz, *vy, *uy;
+    const float *sxx, *szz, *sxz, *sxy, *szy;
+    // point-source injection (addSTF, fdm.h:163-180)
+    const float *inj_x, *inj_y, *inj_z;
+    long inj_slot_stride;
+    const int *inj_ptr, *inj_cell, *inj_row;
+    int inj_per_slot, ntiles, nt, it;
+    // receiver recording (saveRec, fdm.h:181-196)
+    float *out_x, *out_y, *out_z;
+    long out_slot_stride;
+    const int *rec_ptr, *rec_cell, *rec_row;
+    int rec_disp;
+    // snapshot store: u before the update, v after it (fdm.h:521-560)
+    float *snap;  // slot-0 base of the current snapshot slot, or null
+    long snap_slot_stride;
+};
+
+struct StrArgs {
+    Geo g;
+    const float *lam, *mu;
+    const float *vx, *vz, *vy;
+    float *sxx, *szz, *sxz, *sxy, *szy;
+};
+
+struct IntArgs {
+    Geo g;
+    const float *vx, *vz, *ux, *uz, *vy, *uy;
+    const float *snap;  // slot-0 base of the snapshot slot being read
+    long snap_slot_stride;
+    float *kl, *km, *kr;
+    long k_slot_stride;  // 0: every slot accumulates into the same kernels
+    float ndt;
+    int inv_lambda, inv_mu, inv_rho;
+};
+
+template <bool SH, bool PSV>
+__global__ __launch_bounds__(TILE_Z *TILE_X) void k_velocity(const VelArgs a) {
+    const int j = blockIdx.x * TILE_Z + threadIdx.x;
+    const int i = blockIdx.y * TILE_X + threadIdx.y;
+    const int b = blockIdx.z;
+    const Geo &g = a.g;
+    if (i >= g.nx || j >= g.nz) return;
+    const long nz = g.nz;
+    const long k = (long)i * nz + j;
+    const long o = (long)b * g.N + k;
+    const bool xin = i >= 2 && i < g.nx - 2;
+    const bool zin = j >= 2 && j < g.nz - 2;
+    // initAbsbound product order: left, right (abx), bottom, top
+    const float ab = (a.abx[i] * a.abzb[j]) * a.abzt[j];
+    const float rho = a.rho[k];
+    const int tile = blockIdx.y * gridDim.x + blockIdx.x;
+    int p0 = 0, p1 = 0;
+    if (a.inj_ptr) {
+        const int set = a.inj_per_slot ? b : 0;
+        p0 = a.inj_ptr[set * a.ntiles + tile];
+        p1 = a.inj_ptr[set * a.ntiles + tile + 1];
+    }
+    int r0 = 0, r1 = 0;
+    if (a.rec_ptr) {
+        r0 = a.rec_ptr[tile];
+        r1 = a.rec_ptr[tile + 1];
+    }
+    const long it = a.it;
+    if (SH) {
+        const float *sxy = a.sxy + (long)b * g.N, *szy = a.szy + (long)b * g.N;
+        float d = xin ? d_minus(sxy, k, nz, g.c8dx, g.c24dx) : 0.0f;
+        if (zin) d += d_minus(szy, k, 1, g.c8dz, g.c24dz);
+        for (int p = p0; p < p1; p++)
+            if (a.inj_cell[p] == k)
+                d += a.inj_y[(long)b * a.inj_slot_stride + (long)a.inj_row[p] * a.nt + it];
+        float v = a.vy[o], u = a.uy[o];
+        if (a.snap) a.snap[(long)b * a.snap_slot_stride + k] = u;
+        v = ab * (v + g.dt * d / rho);
+        u += v * g.dt;
+        a.vy[o] = v;
+        a.uy[o] = u;
+        if (a.snap) a.snap[(long)b * a.snap_slot_stride + g.N + k] = v;
+        for (int p = r0; p < r1; p++)
+            if (a.rec_cell[p] == k)
+                a.out_y[(long)b * a.out_slot_stride + (long)a.rec_row[p] * a.nt + it] =
+                    a.rec_disp ? u : v;
+    }
+    if (PSV) {
+        const long so = (long)b * g.N;
+        const float *sxx = a.sxx + so, *szz = a.szz + so, *sxz = a.sxz + so;
+        float dx_, dz_;
+        if (xin) {
+            dx_ = d_minus(sxx, k, nz, g.c8dx, g.c24dx);
+            dz_ = d_minus(sxz, k, nz, g.c8dx, g.c24dx);
+        } else {
+            dx_ = 0.0f;
+            dz_ = 0.0f;
+        }
+        if (zin) {
+            dx_ += d_minus(sxz, k, 1, g.c8dz, g.c24dz);
+            dz_ += d_minus(szz, k, 1, g.c8dz, g.c24dz);
+        }
+        for (int p = p0; p < p1; p++)
+            if (a.inj_cell[p] == k) {
+                const long ks = (long)b * a.inj_slot_stride + (long)a.inj_row[p] * a.nt + it;
+                dx_ += a.inj_x[ks];
+                dz_ += a.inj_z[ks];
+            }
+        float vx = a.vx[o], vz = a.vz[o], ux = a.ux[o], uz = a.uz[o];
+        const int f0 = SH ? 2 : 0;
+        float *sp = a.snap ? a.snap + (long)b * a.snap_slot_stride + (long)f0 * g.N + k : nullptr;
+        if (sp) {
+            sp[0] = ux;
+            sp[g.N] = uz;
+        }
+        vx = ab * (vx + g.dt * dx_ / rho);
+        vz = ab * (vz + g.dt * dz_ / rho);
+        ux += vx * g.dt;
+        uz += vz * g.dt;
+        a.vx[o] = vx;
+        a.vz[o] = vz;
+        a.ux[o] = ux;
+        a.uz[o] = uz;
+        if (sp) {
+            sp[2 * g.N] = vx;
+            sp[3 * g.N] = vz;
+        }
+        for (int p = r0; p < r1; p++)
+            if (a.rec_cell[p] == k) {
+                const long kr = (long)b * a.out_slot_stride + (long)a.rec_row[p] * a.nt + it;
+                a.out_x[kr] = a.rec_disp ? ux : vx;
+                a.out_z[kr] = a.rec_disp ? uz : vz;
+            }
+    }
+}
+
+template <bool SH, bool PSV>
+__global__ __launch_bounds__(TILE_Z *TILE_X) void k_stress(const StrArgs a) {
+    const int j = blockIdx.x * TILE_Z + threadIdx.x;
+    const int i = blockIdx.y * TILE_X + threadIdx.y;
+    const int b = blockIdx.z;
+    const Geo &g = a.g;
+    if (i >= g.nx || j >= g.nz) return;
+    const long nz = g.nz;
+    const long k = (long)i * nz + j;
+    const long so = (long)b * g.N;
+    const long o = so + k;
+    const bool xin = i >= 1 && i < g.nx - 2;
+    const bool zin = j >= 1 && j < g.nz - 2;
+    const float mu = a.mu[k];
+    if (SH) {
+        const float *vy = a.vy + so;
+        const float dx_ = xin ? d_plus(vy, k, nz, g.c8dx, g.c24dx) : 0.0f;
+        const float dz_ = zin ? d_plus(vy, k, 1, g.c8dz, g.c24dz) : 0.0f;
+        a.sxy[o] += g.dt * mu * dx_;
+        a.szy[o] += g.dt * mu * dz_;
+    }
+    if (PSV) {
+        const float *vx = a.vx + so, *vz = a.vz + so;
+        float xx, zx, xz, zz;
+        if (xin) {
+            xx = d_plus(vx, k, nz, g.c8dx, g.c24dx);
+            zx = d_plus(vz, k, nz, g.c8dx, g.c24dx);
+        } else {
+            xx = 0.0f;
+            zx = 0.0f;
+        }
+        if (zin) {
+            xz = d_plus(vx, k, 1, g.c8dz, g.c24dz);
+            zz = d_plus(vz, k, 1, g.c8dz, g.c24dz);
+        } else {
+            xz = 0.0f;
+            zz = 0.0f;
+        }
+        const float lam = a.lam[k];
+        a.sxx[o] += g.dt * ((lam + 2 * mu) * xx + lam * zz);
+        a.szz[o] += g.dt * ((lam + 2 * mu) * zz + lam * xx);
+        a.sxz[o] += g.dt * (mu * (xz + zx));
+    }
+}
+
+// interaction* kernels of solver/fdm.h:198-219 with the D+ derivatives of the
+// adjoint and the forward displacement (fdm.h:586-605) computed in place.
+template <bool SH, bool PSV>
+__global__ __launch_bounds__(TILE_Z *TILE_X) void k_interact(const IntArgs a) {
+    const int j = blockIdx.x * TILE_Z + threadIdx.x;
+    const int i = blockIdx.y * TILE_X + threadIdx.y;
+    const int b = blockIdx.z;
+    const Geo &g = a.g;
+    if (i >= g.nx || j >= g.nz) return;
+    const long nz = g.nz;
+    const long k = (long)i * nz + j;
+    const long so = (long)b * g.N;
+    const bool xin = i >= 1 && i < g.nx - 2;
+    const bool zin = j >= 1 && j < g.nz - 2;
+    const float *sp = a.snap + (long)b * a.snap_slot_stride;
+    float *kl = a.kl + (long)b * a.k_slot_stride;
+    float *km = a.km + (long)b * a.k_slot_stride;
+    float *kr = a.kr + (long)b * a.k_slot_stride;
+    const float ndt = a.ndt;
+    if (SH) {
+        const float *uy = a.uy + so;
+        const float *uy_fw = sp, *vy_fw = sp + g.N;
+        const float dx_ = xin ? d_plus(uy, k, nz, g.c8dx, g.c24dx) : 0.0f;
+        const float dz_ = zin ? d_plus(uy, k, 1, g.c8dz, g.c24dz) : 0.0f;
+        const float dxf = xin ? d_plus(uy_fw, k, nz, g.c8dx, g.c24dx) : 0.0f;
+        const float dzf = zin ? d_plus(uy_fw, k, 1, g.c8dz, g.c24dz) : 0.0f;
+        if (a.inv_rho) kr[k] -= vy_fw[k] * a.vy[so + k] * ndt;
+        if (a.inv_mu) km[k] -= (dx_ * dxf + dz_ * dzf) * ndt;
+    }
+    if (PSV) {
+        const float *ux = a.ux + so, *uz = a.uz + so;
+        const long f0 = SH ? 2 : 0;
+        const float *ux_fw = sp + f0 * g.N, *uz_fw = sp + (f0 + 1) * g.N;
+        const float *vx_fw = sp + (f0 + 2) * g.N, *vz_fw = sp + (f0 + 3) * g.N;
+        float xx, xz, zx, zz, xxf, xzf, zxf, zzf;
+        if (xin) {
+            xx = d_plus(ux, k, nz, g.c8dx, g.c24dx);
+            zx = d_plus(uz, k, nz, g.c8dx, g.c24dx);
+            xxf = d_plus(ux_fw, k, nz, g.c8dx, g.c24dx);
+            zxf = d_plus(uz_fw, k, nz, g.c8dx, g.c24dx);
+        } else {
+            xx = zx = xxf = zxf = 0.0f;
+        }
+        if (zin) {
+            xz = d_plus(ux, k, 1, g.c8dz, g.c24dz);
+            zz = d_plus(uz, k, 1, g.c8dz, g.c24dz);
+            xzf = d_plus(ux_fw, k, 1, g.c8dz, g.c24dz);
+            zzf = d_plus(uz_fw, k, 1, g.c8dz, g.c24dz);
+        } else {
+            xz = zz = xzf = zzf = 0.0f;
+        }
+        if (a.inv_rho) kr[k] -= (vx_fw[k] * a.vx[so + k] + vz_fw[k] * a.vz[so + k]) * ndt;
+        if (a.inv_mu)
+            km[k] -= (2 * xx * xxf + 2 * zz * zzf + (xz + zx) * (zxf + xzf)) * ndt;
+        if (a.inv_lambda) kl[k] -= ((xx + zz) * (xxf + zzf)) * ndt;
+    }
+}
+
+// k_total[k] += sum over slots (in slot order) of the per-slot accumulators.
+__global__ void k_kslot_reduce(float *__restrict__ kl, float *__restrict__ km,
+                               float *__restrict__ kr, const float *__restrict__ ks, long N,
+                               int count) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    float a = kl[k], m = km[k], r = kr[k];
+    for (int b = 0; b < count; b++) {
+        const float *s = ks + (long)b * 3 * N;
+        a += s[k];
+        m += s[N + k];
+        r += s[2 * N + k];
+    }
+    kl[k] = a;
+    km[k] = m;
+    kr[k] = r;
+}
+
+__global__ void k_fill(float *__restrict__ p, float v, long n) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static Geo make_geo(const Ctx &c) {
+    Geo g;
+    g.nx = c.g.nx;
+    g.nz = c.g.nz;
+    g.N = c.N;
+    g.dt = c.g.dt;
+    g.c8dx = 8 * c.g.dx;
+    g.c24dx = 24 * c.g.dx;
+    g.c8dz = 8 * c.g.dz;
+    g.c24dz = 24 * c.g.dz;
+    return g;
+}
+
+static dim3 tile_grid(const Ctx &c, int count) {
+    return dim3((c.g.nz + TILE_Z - 1) / TILE_Z, (c.g.nx + TILE_X - 1) / TILE_X, count);
+}
+
+static int tile_of(const Ctx &c, long cell) {
+    const int tz = (c.g.nz + TILE_Z - 1) / TILE_Z;
+    const int i = (int)(cell / c.g.nz), j = (int)(cell % c.g.nz);
+    return (i / TILE_X) * tz + (j / TILE_Z);
+}
+
+void build_point_list(Ctx &c, PointList &pl, int nsets, const std::vector<std::vector<int>> &cells,
+                      const std::vector<std::vector<int>> &rows) {
+    const int tz = (c.g.nz + TILE_Z - 1) / TILE_Z;
+    const int tx = (c.g.nx + TILE_X - 1) / TILE_X;
+    const int ntiles = tz * tx;
+    std::vector<int> count((size_t)nsets * ntiles + 1, 0);
+    int total = 0;
+    for (int s = 0; s < nsets; s++)
+        for (int cell : cells[s]) {
+            count[(size_t)s * ntiles + tile_of(c, cell) + 1]++;
+            total++;
+        }
+    for (size_t t = 1; t < count.size(); t++) count[t] += count[t - 1];
+    std::vector<int> fill(count.begin(), count.end() - 1);
+    std::vector<int> hc(std::max(total, 1)), hr(std::max(total, 1));
+    for (int s = 0; s < nsets; s++)
+        for (size_t p = 0; p < cells[s].size(); p++) {
+            const int slot = fill[(size_t)s * ntiles + tile_of(c, cells[s][p])]++;
+            hc[slot] = cells[s][p];
+            hr[slot] = rows[s][p];
+        }
+    pl.ptr.ensure(count.size() * sizeof(int));
+    pl.cell.ensure(hc.size() * sizeof(int));
+    pl.row.ensure(hr.size() * sizeof(int));
+    INV_HIP(hipMemcpyAsync(pl.ptr.p, count.data(), count.size() * sizeof(int),
+                           hipMemcpyHostToDevice, c.stream));
+    INV_HIP(hipMemcpyAsync(pl.cell.p, hc.data(), hc.size() * sizeof(int), hipMemcpyHostToDevice,
+                           c.stream));
+    INV_HIP(hipMemcpyAsync(pl.row.p, hr.data(), hr.size() * sizeof(int), hipMemcpyHostToDevice,
+                           c.stream));
+    // the host vectors die at return: wait for the copies
+    INV_HIP(hipStreamSynchronize(c.stream));
+    pl.nsets = nsets;
+    pl.ntiles = ntiles;
+    pl.npoints = total;
+}
+
+static void fill(Ctx &c, void *p, long n, float v = 0.0f) {
+    if (n <= 0 || !p) return;
+    k_fill<<<(unsigned)((n + 255) / 256), 256, 0, c.stream>>>(static_cast<float *>(p), v, n);
+    INV_HIP(hipGetLastError());
+}
+
+void fdm_zero_state(Ctx &c, int count) {
+    const long n = (long)count * c.N;
+    if (c.g.sh) {
+        fill(c, c.vy.p, n); fill(c, c.uy.p, n); fill(c, c.sxy.p, n); fill(c, c.szy.p, n);
+    }
+    if (c.g.psv) {
+        fill(c, c.vx.p, n); fill(c, c.vz.p, n); fill(c, c.ux.p, n); fill(c, c.uz.p, n);
+        fill(c, c.sxx.p, n); fill(c, c.szz.p, n); fill(c, c.sxz.p, n);
+    }
+}
+
+template <bool SH, bool PSV>
+static void launch_step(Ctx &c, const VelArgs &va, const StrArgs &sa, int count) {
+    const dim3 grid = tile_grid(c, count), block(TILE_Z, TILE_X);
+    k_velocity<SH, PSV><<<grid, block, 0, c.stream>>>(va);
+    k_stress<SH, PSV><<<grid, block, 0, c.stream>>>(sa);
+}
+
+static void step(Ctx &c, const VelArgs &va, const StrArgs &sa, int count) {
+    if (c.g.sh && c.g.psv) launch_step<true, true>(c, va, sa, count);
+    else if (c.g.sh) launch_step<true, false>(c, va, sa, count);
+    else launch_step<false, true>(c, va, sa, count);
+    INV_HIP(hipGetLastError());
+}
+
+static void fill_state_ptrs(Ctx &c, VelArgs &va, StrArgs &sa) {
+    va.vx = c.vx.as<float>(); va.vz = c.vz.as<float>();
+    va.ux = c.ux.as<float>(); va.uz = c.uz.as<float>();
+    va.vy = c.vy.as<float>(); va.uy = c.uy.as<float>();
+    va.sxx = c.sxx.as<float>(); va.szz = c.szz.as<float>(); va.sxz = c.sxz.as<float>();
+    va.sxy = c.sxy.as<float>(); va.szy = c.szy.as<float>();
+    sa.vx = va.vx; sa.vz = va.vz; sa.vy = va.vy;
+    sa.sxx = c.sxx.as<float>(); sa.szz = c.szz.as<float>(); sa.sxz = c.sxz.as<float>();
+    sa.sxy = c.sxy.as<float>(); sa.szy = c.szy.as<float>();
+}
+
+static void make_common(Ctx &c, VelArgs &va, StrArgs &sa) {
+    va = VelArgs{};
+    sa = StrArgs{};
+    va.g = sa.g = make_geo(c);
+    va.rho = c.rho.as<float>();
+    va.abx = c.abx.as<float>();
+    va.abzb = c.abzb.as<float>();
+    va.abzt = c.abzt.as<float>();
+    sa.lam = c.lam.as<float>();
+    sa.mu = c.mu.as<float>();
+    fill_state_ptrs(c, va, sa);
+    va.nt = c.g.nt;
+    va.ntiles = c.rec_list.ntiles;
+}
+
+void fdm_forward(Ctx &c, int first, int count, int flags) {
+    // point lists of this batch's sources: set b = shot first+b (or every
+    // source in set 0 for first == -1, runForward(-1) of fdm.h:171)
+    if (c.src_list_first != first || c.src_list_count != count) {
+        std::vector<std::vector<int>> cells(count), rows(count);
+        for (int b = 0; b < count; b++) {
+            for (int is = 0; is < c.nsrc; is++) {
+                if (first < 0 || is == first + b) {
+                    cells[b].push_back(c.src_ix[is] * c.g.nz + c.src_iz[is]);
+                    rows[b].push_back(is);
+                }
+            }
+        }
+        build_point_list(c, c.src_list, count, cells, rows);
+        c.src_list_first = first;
+        c.src_list_count = count;
+    }
+    VelArgs va;
+    StrArgs sa;
+    make_common(c, va, sa);
+    const long nt = c.g.nt, nsrc = c.nsrc;
+    va.inj_x = c.stf.as<float>();
+    va.inj_y = c.stf.as<float>() + nsrc * nt;
+    va.inj_z = c.stf.as<float>() + 2 * nsrc * nt;
+    va.inj_slot_stride = 0;
+    va.inj_ptr = c.src_list.ptr.as<int>();
+    va.inj_cell = c.src_list.cell.as<int>();
+    va.inj_row = c.src_list.row.as<int>();
+    va.inj_per_slot = 1;
+    va.ntiles = c.src_list.ntiles;
+    const long tr = (long)c.nrec * nt;
+    const int cy = 0, cx = c.g.sh ? 1 : 0, cz = cx + 1;
+    va.out_y = c.out.as<float>() + cy * tr;
+    va.out_x = c.out.as<float>() + cx * tr;
+    va.out_z = c.out.as<float>() + cz * tr;
+    va.out_slot_stride = (long)c.ncomp * tr;
+    va.rec_ptr = c.nrec ? c.rec_list.ptr.as<int>() : nullptr;
+    va.rec_cell = c.rec_list.cell.as<int>();
+    va.rec_row = c.rec_list.row.as<int>();
+    va.rec_disp = (c.g.adj || c.g.trace_type == 1) ? 1 : 0;
+    va.snap_slot_stride = (long)c.nsfe * c.nsnapf * c.N;
+    const bool store = (flags & INV_FWD_SNAPSHOTS) && c.g.adj && c.g.sfe > 0 && c.snap.p;
+
+    fdm_zero_state(c, count);
+    int nsteps = c.g.nt;
+    if (c.step_limit > 0 && c.step_limit < nsteps) nsteps = c.step_limit;
+    for (int it = 0; it < nsteps; it++) {
+        va.it = it;
+        va.snap = nullptr;
+        if (store && (it + 1) % c.g.sfe == 0) {
+            const long isfe = c.nsfe - (it + 1) / c.g.sfe;
+            va.snap = c.snap.as<float>() + isfe * c.nsnapf * c.N;
+        }
+        step(c, va, sa, count);
+    }
+}
+
+template <bool SH, bool PSV>
+static void launch_interact(Ctx &c, const IntArgs &ia, int count) {
+    k_interact<SH, PSV><<<tile_grid(c, count), dim3(TILE_Z, TILE_X), 0, c.stream>>>(ia);
+}
+
+void fdm_adjoint(Ctx &c, int first, int count) {
+    (void)first;
+    VelArgs va;
+    StrArgs sa;
+    make_common(c, va, sa);
+    const long nt = c.g.nt;
+    const long tr = (long)c.nrec * nt;
+    const int cy = 0, cx = c.g.sh ? 1 : 0, cz = cx + 1;
+    va.inj_y = c.adstf.as<float>() + cy * tr;
+    va.inj_x = c.adstf.as<float>() + cx * tr;
+    va.inj_z = c.adstf.as<float>() + cz * tr;
+    va.inj_slot_stride = (long)c.ncomp * tr;
+    va.inj_ptr = c.nrec ? c.rec_list.ptr.as<int>() : nullptr;
+    va.inj_cell = c.rec_list.cell.as<int>();
+    va.inj_row = c.rec_list.row.as<int>();
+    va.inj_per_slot = 0;
+    va.ntiles = c.rec_list.ntiles;
+    va.rec_ptr = nullptr;
+    va.snap = nullptr;
+
+    IntArgs ia{};
+    ia.g = va.g;
+    ia.vx = c.vx.as<float>(); ia.vz = c.vz.as<float>();
+    ia.ux = c.ux.as<float>(); ia.uz = c.uz.as<float>();
+    ia.vy = c.vy.as<float>(); ia.uy = c.uy.as<float>();
+    ia.snap_slot_stride = (long)c.nsfe * c.nsnapf * c.N;
+    ia.ndt = (float)c.g.sfe * c.g.dt;
+    ia.inv_lambda = c.g.inv_lambda;
+    ia.inv_mu = c.g.inv_mu;
+    ia.inv_rho = c.g.inv_rho;
+    const bool direct = c.g.batch == 1;
+    if (direct) {
+        ia.kl = c.k_lambda.as<float>();
+        ia.km = c.k_mu.as<float>();
+        ia.kr = c.k_rho.as<float>();
+        ia.k_slot_stride = 0;
+    } else {
+        ia.kl = c.kslot.as<float>();
+        ia.km = c.kslot.as<float>() + c.N;
+        ia.kr = c.kslot.as<float>() + 2 * c.N;
+        ia.k_slot_stride = 3 * c.N;
+        fill(c, c.kslot.p, 3L * count * c.N);
+    }
+
+    fdm_zero_state(c, count);
+    for (int it = 0; it < c.g.nt; it++) {
+        va.it = it;
+        step(c, va, sa, count);
+        if (it % c.g.sfe == 0) {
+            ia.snap = c.snap.as<float>() + (long)(it / c.g.sfe) * c.nsnapf * c.N;
+            if (c.g.sh && c.g.psv) launch_interact<true, true>(c, ia, count);
+            else if (c.g.sh) launch_interact<true, false>(c, ia, count);
+            else launch_interact<false, true>(c, ia, count);
+            INV_HIP(hipGetLastError());
+        }
+    }
+    if (!direct) fdm_kslot_reduce(c, count);
+}
+
+void fdm_kslot_reduce(Ctx &c, int count) {
+    k_kslot_reduce<<<(unsigned)((c.N + 255) / 256), 256, 0, c.stream>>>(
+        c.k_lambda.as<float>(), c.k_mu.as<float>(), c.k_rho.as<float>(), c.kslot.as<float>(), c.N,
+        count);
+    INV_HIP(hipGetLastError());
+}
+
+}  // namespace inv

@@ -1,0 +1,129 @@
+// Internal declarations shared by the libinv_hip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/inv_hip.h"
+
+namespace inv {
+
+// Status helpers -------------------------------------------------------------
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define INV_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            throw ::inv::Error{e_ == hipErrorOutOfMemory ? INV_ERR_NOMEM : INV_ERR_HIP, \
+                               std::string(#call) + ": " + hipGetErrorString(e_)};     \
+    } while (0)
+
+#define INV_REQUIRE(cond, code, text)                 \
+    do {                                              \
+        if (!(cond)) throw ::inv::Error{(code), (text)}; \
+    } while (0)
+
+// Device buffer owned by a context.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    void release();
+    void ensure(size_t nbytes);  // (re)allocates when too small; contents undefined
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// Sparse point set (sources or receivers) bucketed by propagation tile, so a
+// fused stencil kernel finds the points of its own tile without scanning the
+// whole list.  ptr has nsets*ntiles+1 entries; entries sorted by (set, tile,
+// original order).
+struct PointList {
+    DevBuf ptr, cell, row;
+    int nsets = 0, ntiles = 0, npoints = 0;
+};
+
+// Kernel launch geometry of the propagation tiles.
+constexpr int TILE_Z = 64;  // one wavefront along z (the contiguous axis)
+constexpr int TILE_X = 4;   // wavefronts per block along x
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // grid
+    inv_grid_desc g{};
+    bool have_grid = false, have_model = false, have_src = false, have_rec = false;
+    long N = 0;
+    int nsfe = 0;
+    int ncomp = 0;  // trace components: y (sh) then x, z (psv)
+    int nsnapf = 0;  // snapshot fields per slot
+    int step_limit = 0;
+    int snap_format = INV_SNAP_F32;
+
+    // model and absorbing profile (1-D factor tables: ab = ((ax*bz)*tz))
+    DevBuf lam, mu, rho, abx, abzb, abzt;
+    // kernels
+    DevBuf k_lambda, k_mu, k_rho;
+    DevBuf kslot;  // per-slot kernel accumulators [batch][3][N] (batched adjoint)
+
+    // per-slot wavefields [batch][N]
+    DevBuf vx, vz, ux, uz, sxx, szz, sxz, vy, uy, sxy, szy;
+    DevBuf snap;   // [batch][nsfe][nsnapf][N]
+    DevBuf out;    // [batch][ncomp][nrec][nt]
+    DevBuf adstf;  // [batch][ncomp][nrec][nt]
+    DevBuf obs;    // [nsrc][ncomp][nrec][nt]
+    DevBuf chi;    // per-trace misfit [batch][nrec][ncomp] (double)
+    DevBuf taper;  // [nt] waveform taper
+    DevBuf red;    // reduction scratch
+    DevBuf scal;   // a few device scalars (FP64 results)
+
+    // acquisition
+    int nsrc = 0, nrec = 0;
+    std::vector<int> src_ix, src_iz, rec_ix, rec_iz;
+    DevBuf stf;  // [3][nsrc][nt]  x, y, z
+    PointList rec_list;  // receivers, one set shared by all slots
+    PointList src_list;  // sources of the current forward, one set per slot
+    int src_list_first = -2, src_list_count = -1;
+
+    // filter
+    DevBuf gauss_w, gauss_tmp, gauss_sx, gauss_sz;
+    int gauss_sigma = -1, gauss_nx = -1, gauss_nz = -1, gauss_m = 0;
+
+    std::vector<void *> user_allocs;
+
+    int slot_offset_traces() const { return ncomp * nrec * g.nt; }
+};
+
+// fdm.hip -------------------------------------------------------------------
+void build_point_list(Ctx &c, PointList &pl, int nsets, const std::vector<std::vector<int>> &cells,
+                      const std::vector<std::vector<int>> &rows);
+void fdm_forward(Ctx &c, int first, int count, int flags);
+void fdm_adjoint(Ctx &c, int first, int count);
+void fdm_zero_state(Ctx &c, int count);
+void fdm_kslot_reduce(Ctx &c, int count);
+
+// misfit.hip ----------------------------------------------------------------
+void misfit_waveform(Ctx &c, int count, int first);
+void misfit_envelope(Ctx &c, int count, int first);
+
+// linalg.hip ----------------------------------------------------------------
+void gauss_filter(Ctx &c, float *field, int nx, int nz, int sigma);
+void vec_axpby(Ctx &c, float *out, float ka, const float *a, float kb, const float *b, size_t n);
+void vec_scale(Ctx &c, float *out, float ka, const float *a, size_t n);
+void vec_mul(Ctx &c, float *out, const float *a, const float *b, size_t n);
+void vec_copy(Ctx &c, float *out, const float *a, size_t n);
+void vec_fill(Ctx &c, float *out, float v, size_t n);
+double vec_dot(Ctx &c, const float *a, const float *b, size_t n);
+float vec_amax(Ctx &c, const float *a, size_t n);
+void sum_f64(Ctx &c, const double *x, size_t n, double *dev_out);
+
+}  // namespace inv

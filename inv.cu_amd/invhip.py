@@ -1,0 +1,358 @@
+"""ctypes binding of libinv_hip.so (include/inv_hip.h).
+
+Every call goes through the C ABI; device memory is owned by the library's
+context.  If the shared library is missing this module raises at load time:
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libinv_hip.so"
+
+INV_OK = 0
+INV_LAMBDA, INV_MU, INV_RHO, INV_K_LAMBDA, INV_K_MU, INV_K_RHO, INV_OUT, INV_ADSTF, INV_OBS = range(9)
+INV_SNAP_F32, INV_SNAP_F16, INV_SNAP_BF16 = 0, 1, 2
+INV_FWD_SNAPSHOTS = 1
+
+STATUS = {0: "INV_OK", -1: "INV_ERR_ARG", -2: "INV_ERR_HIP", -3: "INV_ERR_STATE",
+          -4: "INV_ERR_NOMEM", -5: "INV_ERR_UNSUPPORTED"}
+
+
+class GridDesc(C.Structure):
+    """inv_grid_desc of include/inv_hip.h."""
+
+    _fields_ = [
+        ("nx", C.c_int), ("nz", C.c_int),
+        ("dx", C.c_float), ("dz", C.c_float), ("dt", C.c_float),
+        ("nt", C.c_int), ("sh", C.c_int), ("psv", C.c_int), ("adj", C.c_int),
+        ("trace_type", C.c_int), ("sfe", C.c_int), ("abs_width", C.c_int),
+        ("abs_alpha", C.c_float), ("abs_left", C.c_int), ("abs_right", C.c_int),
+        ("abs_top", C.c_int), ("abs_bottom", C.c_int),
+        ("inv_lambda", C.c_int), ("inv_mu", C.c_int), ("inv_rho", C.c_int),
+        ("batch", C.c_int),
+    ]
+
+
+# Every exported function and its argument types (checked against the header
+# by tests/test_capi_exports.py).
+_vp = C.c_void_p
+_i = C.c_int
+_f = C.c_float
+_sz = C.c_size_t
+SIGNATURES = {
+    "inv_grid_dims": [_i, _f, _f, _vp, _vp, _vp, _vp],
+    "inv_calc_idx": [_f, _f, _i],
+    "inv_source_ricker": [_i, _f, _f, _f, _f, _f, _vp, _vp, _vp],
+    "inv_device_count": [_vp],
+    "inv_ctx_create": [_i, _vp],
+    "inv_ctx_destroy": [_vp],
+    "inv_last_error": [_vp],
+    "inv_sync": [_vp],
+    "inv_mem_info": [_vp, _vp, _vp],
+    "inv_grid_set": [_vp, _vp],
+    "inv_model_upload_lmr": [_vp, _vp, _vp, _vp],
+    "inv_model_download_lmr": [_vp, _vp, _vp, _vp],
+    "inv_model_from_points": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _f, _f, _i],
+    "inv_model_to_vps": [_vp, _vp, _vp, _vp],
+    "inv_sources_set": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
+    "inv_receivers_set": [_vp, _i, _vp, _vp],
+    "inv_forward": [_vp, _i, _i, _i],
+    "inv_adjoint": [_vp, _i, _i],
+    "inv_kernels_zero": [_vp],
+    "inv_kernels_get": [_vp, _vp, _vp, _vp],
+    "inv_kernels_set": [_vp, _vp, _vp, _vp],
+    "inv_traces_get": [_vp, _i, _vp],
+    "inv_adstf_set": [_vp, _i, _vp],
+    "inv_adstf_get": [_vp, _i, _vp],
+    "inv_obs_set": [_vp, _i, _vp],
+    "inv_obs_get": [_vp, _i, _vp],
+    "inv_obs_from_traces": [_vp, _i, _i],
+    "inv_misfit": [_vp, _i, _i, _i, _vp, _vp],
+    "inv_gradient": [_vp, _i, _i, _i, _vp],
+    "inv_filter_gauss1s": [_vp, _vp, _i, _i, _i],
+    "inv_vec_axpby": [_vp, _vp, _f, _vp, _f, _vp, _sz],
+    "inv_vec_scale": [_vp, _vp, _f, _vp, _sz],
+    "inv_vec_mul": [_vp, _vp, _vp, _vp, _sz],
+    "inv_vec_copy": [_vp, _vp, _vp, _sz],
+    "inv_vec_fill": [_vp, _vp, _f, _sz],
+    "inv_vec_dot_f64": [_vp, _vp, _vp, _sz, _vp],
+    "inv_vec_amax": [_vp, _vp, _sz, _vp],
+    "inv_field_ptr": [_vp, _i, _vp],
+    "inv_alloc": [_vp, _sz, _vp],
+    "inv_free": [_vp, _vp],
+    "inv_memcpy_h2d": [_vp, _vp, _vp, _sz],
+    "inv_memcpy_d2h": [_vp, _vp, _vp, _sz],
+    "inv_state_get": [_vp, _i, _vp],
+    "inv_snapshots_get": [_vp, _i, _vp],
+    "inv_set_step_limit": [_vp, _i],
+}
+
+_lib = None
+
+
+def load():
+    """Load libinv_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                               "(make -C inv.cu_amd)")
+        L = C.CDLL(str(LIB_PATH))
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_char_p if name == "inv_last_error" else C.c_int
+        _lib = L
+    return _lib
+
+
+def grid_dims(npt, xmax, zmax):
+    nx, nz, dx, dz = C.c_int(), C.c_int(), C.c_float(), C.c_float()
+    rc = load().inv_grid_dims(npt, xmax, zmax, C.byref(nx), C.byref(nz), C.byref(dx), C.byref(dz))
+    if rc != INV_OK:
+        raise InvError(rc, "invalid model geometry")
+    return nx.value, nz.value, np.float32(dx.value), np.float32(dz.value)
+
+
+def calc_idx(coord, length, n):
+    return load().inv_calc_idx(float(coord), float(length), int(n))
+
+
+def source_ricker(nt, dt, f0, t0=0.0, angle=0.0, amp=1.0):
+    """Wavelets (stf_x, stf_y, stf_z) of one source, each float32 [nt]."""
+    out = [np.empty(nt, np.float32) for _ in range(3)]
+    rc = load().inv_source_ricker(nt, dt, f0, t0, angle, amp, *(_p(a) for a in out))
+    if rc != INV_OK:
+        raise InvError(rc, "bad source parameters")
+    return out
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    load().inv_device_count(C.byref(n))
+    return n.value
+
+
+class InvError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One device context of the library (inv_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        h = C.c_void_p()
+        rc = self.L.inv_ctx_create(device, C.byref(h))
+        if rc != INV_OK:
+            raise InvError(rc, f"inv_ctx_create({device}) failed")
+        self.h = h
+        self.desc = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.inv_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != INV_OK:
+            raise InvError(rc, self.L.inv_last_error(self.h).decode())
+
+    # ---- setup ----
+    def grid_set(self, **kw):
+        d = GridDesc()
+        defaults = dict(sh=0, psv=0, adj=1, trace_type=0, sfe=10, abs_width=20, abs_alpha=0.015,
+                        abs_left=1, abs_right=1, abs_top=1, abs_bottom=1, inv_lambda=1, inv_mu=1,
+                        inv_rho=1, batch=1)
+        defaults.update(kw)
+        for k, v in defaults.items():
+            setattr(d, k, v)
+        self._chk(self.L.inv_grid_set(self.h, C.byref(d)))
+        self.desc = d
+        return d
+
+    @property
+    def N(self):
+        return self.desc.nx * self.desc.nz
+
+    @property
+    def ncomp(self):
+        return (1 if self.desc.sh else 0) + (2 if self.desc.psv else 0)
+
+    @property
+    def nsnap(self):
+        return (2 if self.desc.sh else 0) + (4 if self.desc.psv else 0)
+
+    @property
+    def nstate(self):
+        return (4 if self.desc.sh else 0) + (7 if self.desc.psv else 0)
+
+    def model_upload(self, lam, mu, rho):
+        f = lambda a: np.ascontiguousarray(a, np.float32).ravel()
+        lam, mu, rho = f(lam), f(mu), f(rho)
+        self._chk(self.L.inv_model_upload_lmr(self.h, _p(lam), _p(mu), _p(rho)))
+
+    def model_download(self):
+        out = [np.empty(self.N, np.float32) for _ in range(3)]
+        self._chk(self.L.inv_model_download_lmr(self.h, *(_p(a) for a in out)))
+        return out
+
+    def sources_set(self, ix, iz, stf_x, stf_y, stf_z):
+        ix = np.ascontiguousarray(ix, np.int32)
+        iz = np.ascontiguousarray(iz, np.int32)
+        f = lambda a: np.ascontiguousarray(a, np.float32).reshape(-1)
+        sx, sy, sz = f(stf_x), f(stf_y), f(stf_z)
+        self.nsrc = len(ix)
+        self._chk(self.L.inv_sources_set(self.h, len(ix), _p(ix), _p(iz), _p(sx), _p(sy), _p(sz)))
+
+    def receivers_set(self, ix, iz):
+        ix = np.ascontiguousarray(ix, np.int32)
+        iz = np.ascontiguousarray(iz, np.int32)
+        self.nrec = len(ix)
+        self._chk(self.L.inv_receivers_set(self.h, len(ix), _p(ix), _p(iz)))
+
+    # ---- propagation ----
+    def forward(self, first, count=1, snapshots=True):
+        self._chk(self.L.inv_forward(self.h, first, count, INV_FWD_SNAPSHOTS if snapshots else 0))
+
+    def adjoint(self, first, count=1):
+        self._chk(self.L.inv_adjoint(self.h, first, count))
+
+    def kernels_zero(self):
+        self._chk(self.L.inv_kernels_zero(self.h))
+
+    def kernels_get(self):
+        out = [np.empty(self.N, np.float32) for _ in range(3)]
+        self._chk(self.L.inv_kernels_get(self.h, *(_p(a) for a in out)))
+        return out
+
+    def traces_get(self, slot=0):
+        a = np.empty((self.ncomp, self.nrec, self.desc.nt), np.float32)
+        self._chk(self.L.inv_traces_get(self.h, slot, _p(a)))
+        return a
+
+    def adstf_set(self, slot, a):
+        a = np.ascontiguousarray(a, np.float32)
+        assert a.size == self.ncomp * self.nrec * self.desc.nt
+        self._chk(self.L.inv_adstf_set(self.h, slot, _p(a)))
+
+    def adstf_get(self, slot=0):
+        a = np.empty((self.ncomp, self.nrec, self.desc.nt), np.float32)
+        self._chk(self.L.inv_adstf_get(self.h, slot, _p(a)))
+        return a
+
+    def obs_set(self, isrc, a):
+        a = np.ascontiguousarray(a, np.float32)
+        assert a.size == self.ncomp * self.nrec * self.desc.nt
+        self._chk(self.L.inv_obs_set(self.h, isrc, _p(a)))
+
+    def obs_get(self, isrc):
+        a = np.empty((self.ncomp, self.nrec, self.desc.nt), np.float32)
+        self._chk(self.L.inv_obs_get(self.h, isrc, _p(a)))
+        return a
+
+    def obs_from_traces(self, first, count=1):
+        self._chk(self.L.inv_obs_from_traces(self.h, first, count))
+
+    def misfit(self, first, count=1, kind=0):
+        chi = np.empty((count, self.nrec, self.ncomp), np.float64)
+        tot = C.c_double()
+        self._chk(self.L.inv_misfit(self.h, first, count, kind, _p(chi), C.byref(tot)))
+        return tot.value, chi
+
+    def gradient(self, misfit_type=0, sigma=0, kernel=True):
+        f = C.c_double()
+        self._chk(self.L.inv_gradient(self.h, misfit_type, sigma, 1 if kernel else 0, C.byref(f)))
+        return f.value
+
+    def sync(self):
+        self._chk(self.L.inv_sync(self.h))
+
+    # ---- device memory / vectors ----
+    def field_ptr(self, field):
+        p = C.c_void_p()
+        self._chk(self.L.inv_field_ptr(self.h, field, C.byref(p)))
+        return p.value
+
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        self._chk(self.L.inv_alloc(self.h, nbytes, C.byref(p)))
+        return p.value
+
+    def free(self, ptr):
+        self._chk(self.L.inv_free(self.h, ptr))
+
+    def h2d(self, dev, a):
+        a = np.ascontiguousarray(a)
+        self._chk(self.L.inv_memcpy_h2d(self.h, dev, _p(a), a.nbytes))
+
+    def d2h(self, dev, shape, dtype=np.float32):
+        a = np.empty(shape, dtype)
+        self._chk(self.L.inv_memcpy_d2h(self.h, _p(a), dev, a.nbytes))
+        return a
+
+    def filter_gauss(self, dev, nx, nz, sigma):
+        self._chk(self.L.inv_filter_gauss1s(self.h, dev, nx, nz, sigma))
+
+    def vec_axpby(self, c, ka, a, kb, b, n):
+        self._chk(self.L.inv_vec_axpby(self.h, c, ka, a, kb, b, n))
+
+    def vec_scale(self, c, ka, a, n):
+        self._chk(self.L.inv_vec_scale(self.h, c, ka, a, n))
+
+    def vec_mul(self, c, a, b, n):
+        self._chk(self.L.inv_vec_mul(self.h, c, a, b, n))
+
+    def vec_copy(self, dst, src, n):
+        self._chk(self.L.inv_vec_copy(self.h, dst, src, n))
+
+    def vec_fill(self, dst, v, n):
+        self._chk(self.L.inv_vec_fill(self.h, dst, v, n))
+
+    def vec_dot(self, a, b, n):
+        out = C.c_double()
+        self._chk(self.L.inv_vec_dot_f64(self.h, a, b, n, C.byref(out)))
+        return out.value
+
+    def vec_amax(self, a, n):
+        out = C.c_float()
+        self._chk(self.L.inv_vec_amax(self.h, a, n, C.byref(out)))
+        return out.value
+
+    # ---- test hooks ----
+    def state_get(self, slot=0):
+        a = np.empty((self.nstate, self.N), np.float32)
+        self._chk(self.L.inv_state_get(self.h, slot, _p(a)))
+        return a
+
+    def snapshots_get(self, slot=0):
+        nsfe = self.desc.nt // self.desc.sfe
+        a = np.empty((nsfe, self.nsnap, self.N), np.float32)
+        self._chk(self.L.inv_snapshots_get(self.h, slot, _p(a)))
+        return a
+
+    def set_step_limit(self, n):
+        self._chk(self.L.inv_set_step_limit(self.h, n))

@@ -1,0 +1,262 @@
+"""Synthetic inversion setups (model, acquisition, time stepping).
+
+Each generator returns a `Problem`: true and initial models on the grid
+(lambda, mu, rho, z fastest), point sources with Ricker wavelets from the
+library's own RickerSource (inv_source_ricker), receivers, and the solver
+flags.  The checkerboard follows the semantics of the reference's
+util/generate_checker.cu:4-39 (square cells of +-dv around a background); the
+layered/faulted generator is a Marmousi-shaped model of our own.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+from . import invhip
+
+
+def vps2lm(vp, vs, rho):
+    """vps2lm of solver/fdm.h:25-36 in FP32 (no fused multiply-add)."""
+    vp = np.asarray(vp, np.float32)
+    vs = np.asarray(vs, np.float32)
+    rho = np.asarray(rho, np.float32)
+    lam = np.where(vp > vs, rho * (vp * vp - np.float32(2) * vs * vs), np.float32(0))
+    mu = rho * vs * vs
+    return lam.astype(np.float32), mu.astype(np.float32)
+
+
+@dataclass
+class Problem:
+    name: str
+    nx: int
+    nz: int
+    dx: float
+    dz: float
+    dt: float
+    nt: int
+    sh: int
+    psv: int
+    lam: np.ndarray          # true model
+    mu: np.ndarray
+    rho: np.ndarray
+    lam0: np.ndarray         # initial model
+    mu0: np.ndarray
+    rho0: np.ndarray
+    src_ix: np.ndarray
+    src_iz: np.ndarray
+    stf_x: np.ndarray        # [nsrc][nt]
+    stf_y: np.ndarray
+    stf_z: np.ndarray
+    rec_ix: np.ndarray
+    rec_iz: np.ndarray
+    sfe: int = 10
+    abs_width: int = 20
+    abs_alpha: float = 0.015
+    abs_left: int = 1
+    abs_right: int = 1
+    abs_top: int = 1
+    abs_bottom: int = 1
+    inv_lambda: int = 1
+    inv_mu: int = 1
+    inv_rho: int = 1
+    trace_type: int = 0
+    misfit: int = 0
+    sigma: int = 0
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def N(self):
+        return self.nx * self.nz
+
+    @property
+    def nsrc(self):
+        return len(self.src_ix)
+
+    @property
+    def nrec(self):
+        return len(self.rec_ix)
+
+    @property
+    def ncomp(self):
+        return (1 if self.sh else 0) + (2 if self.psv else 0)
+
+    def cell_updates(self, nshots=None, adjoint=True):
+        """Cell-updates of one gradient: sum over shots of nx*nz*nt*(2 with adjoint)."""
+        ns = self.nsrc if nshots is None else nshots
+        return ns * self.nx * self.nz * self.nt * (2 if adjoint else 1)
+
+    def grid_kwargs(self, batch=1, adj=1):
+        return dict(nx=self.nx, nz=self.nz, dx=self.dx, dz=self.dz, dt=self.dt, nt=self.nt,
+                    sh=self.sh, psv=self.psv, adj=adj, trace_type=self.trace_type, sfe=self.sfe,
+                    abs_width=self.abs_width, abs_alpha=self.abs_alpha, abs_left=self.abs_left,
+                    abs_right=self.abs_right, abs_top=self.abs_top,
+                    abs_bottom=self.abs_bottom, inv_lambda=self.inv_lambda, inv_mu=self.inv_mu,
+                    inv_rho=self.inv_rho, batch=batch)
+
+    def context(self, batch=1, adj=1, model="init", device=0):
+        """A library context with grid, model and acquisition set."""
+        ctx = invhip.Context(device)
+        ctx.grid_set(**self.grid_kwargs(batch=batch, adj=adj))
+        self.load_model(ctx, model)
+        ctx.sources_set(self.src_ix, self.src_iz, self.stf_x, self.stf_y, self.stf_z)
+        ctx.receivers_set(self.rec_ix, self.rec_iz)
+        return ctx
+
+    def load_model(self, ctx, model="init"):
+        if model == "true":
+            ctx.model_upload(self.lam, self.mu, self.rho)
+        else:
+            ctx.model_upload(self.lam0, self.mu0, self.rho0)
+
+    def with_(self, **kw):
+        return replace(self, **kw)
+
+
+def _wavelets(nt, dt, f0, t0, amp, angle, nsrc):
+    sx, sy, sz = invhip.source_ricker(nt, dt, f0, t0, angle, amp)
+    rep = lambda a: np.repeat(a[None, :], nsrc, axis=0).astype(np.float32)
+    return rep(sx), rep(sy), rep(sz)
+
+
+def _smooth(a, sigma):
+    """Symmetric separable Gaussian smoothing (edge-reflecting), float64."""
+    if sigma <= 0:
+        return a.astype(np.float64)
+    r = int(np.ceil(3 * sigma))
+    w = np.exp(-0.5 * (np.arange(-r, r + 1) / sigma) ** 2)
+    w /= w.sum()
+    out = a.astype(np.float64)
+    for ax in (0, 1):
+        pad = [(0, 0), (0, 0)]
+        pad[ax] = (r, r)
+        p = np.pad(out, pad, mode="reflect")
+        acc = np.zeros_like(out)
+        for m, wm in enumerate(w):
+            sl = [slice(None), slice(None)]
+            sl[ax] = slice(m, m + out.shape[ax])
+            acc += wm * p[tuple(sl)]
+        out = acc
+    return out
+
+
+# ---------------------------------------------------------------------------
+# generators
+# ---------------------------------------------------------------------------
+def checker(nx=200, nz=200, dx=10.0, dt=1e-3, nt=2000, cell=20, frac=0.10, vp0=3000.0,
+            vs0=1732.0, rho0=2200.0, nsrc=4, nrec=40, f0=10.0, t0=0.15, amp=1e10, sh=0,
+            psv=1, src_depth=20, rec_depth=10, sigma=5, name="checker"):
+    """P-SV checkerboard gradient setup: background +-frac on vp, vs, rho in
+    squares of `cell` cells, sign (-1)^(floor(i/cell)+floor(j/cell)); the
+    initial model is the background.  Sources evenly spaced in x at
+    `src_depth` cells, receivers at `rec_depth` cells."""
+    i = np.arange(nx)[:, None]
+    j = np.arange(nz)[None, :]
+    sign = np.where(((i // cell) + (j // cell)) % 2 == 0, 1.0, -1.0)
+    f32 = np.float32
+    vp = (vp0 * (1 + frac * sign)).astype(f32)
+    vs = (vs0 * (1 + frac * sign)).astype(f32)
+    rho = (rho0 * (1 + frac * sign)).astype(f32)
+    lam, mu = vps2lm(vp.ravel(), vs.ravel(), rho.ravel())
+    b = np.ones(nx * nz, f32)
+    lam0, mu0 = vps2lm(b * f32(vp0), b * f32(vs0), b * f32(rho0))
+    xmax = (nx - 1) * dx
+    src_x = [(s + 1) * xmax / (nsrc + 1) for s in range(nsrc)]
+    rec_x = [(r + 0.5) * xmax / nrec for r in range(nrec)]
+    src_ix = np.array([invhip.calc_idx(x, xmax, nx) for x in src_x], np.int32)
+    rec_ix = np.array([invhip.calc_idx(x, xmax, nx) for x in rec_x], np.int32)
+    sx, sy, sz = _wavelets(nt, dt, f0, t0, amp, 0.0, nsrc)
+    return Problem(name, nx, nz, f32(dx), f32(dx), f32(dt), nt, sh, psv, lam, mu,
+                   rho.ravel(), lam0, mu0, b * f32(rho0), src_ix,
+                   np.full(nsrc, src_depth, np.int32), sx, sy, sz, rec_ix,
+                   np.full(nrec, rec_depth, np.int32), sigma=sigma,
+                   meta=dict(f0=f0, t0=t0, amp=amp))
+
+
+def layered_faulted(nx=737, nz=240, dx=12.5, seed=1, nwater=12, vmin=1500.0, vmax=4700.0,
+                    nfaults=3):
+    """Marmousi-shaped vp/vs/rho: vp rising with depth below a water layer,
+    8-15 seeded dipping layers, `nfaults` normal faults with 50-200 m throw;
+    vs = vp/1.73 and Gardner density 310 vp^0.25 below the water."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(nx)[:, None] * dx
+    z = np.arange(nz)[None, :] * dx
+    xmax, zmax = (nx - 1) * dx, (nz - 1) * dx
+    zw = nwater * dx
+    zeff = np.broadcast_to(z, (nx, nz)).astype(np.float64).copy()
+    for _ in range(nfaults):
+        x0 = rng.uniform(0.15, 0.85) * xmax
+        dip = np.deg2rad(rng.uniform(55.0, 75.0))
+        throw = rng.uniform(50.0, 200.0)
+        hanging = x > x0 + z / np.tan(dip)
+        zeff = np.where(hanging, zeff - throw, zeff)
+    nl = int(rng.integers(8, 16))
+    bases = np.sort(rng.uniform(zw, zmax, nl))
+    slopes = rng.uniform(-0.12, 0.12, nl)
+    layer = np.zeros((nx, nz), np.int64)
+    for b0, s in zip(bases, slopes):
+        layer += (zeff > b0 + s * (x - xmax / 2)).astype(np.int64)
+    trend = vmin + (vmax - vmin) * np.clip((zeff - zw) / (zmax - zw), 0, 1)
+    jitter = rng.uniform(-250.0, 250.0, nl + 1)
+    vp = np.clip(trend + jitter[layer], vmin, vmax)
+    vs = vp / 1.73
+    rho = 310.0 * vp ** 0.25
+    water = np.broadcast_to(np.arange(nz)[None, :] < nwater, (nx, nz))
+    vp = np.where(water, 1500.0, vp)
+    vs = np.where(water, 0.0, vs)
+    rho = np.where(water, 1000.0, rho)
+    return vp.astype(np.float32), vs.astype(np.float32), rho.astype(np.float32), water
+
+
+def marmousi(nx=737, nz=240, dx=12.5, dt=1e-3, nt=4000, nsrc=16, nrec=368, f0=5.0, t0=0.3,
+             amp=1e10, seed=1, src_depth=2, rec_depth=0, rec_step=2, smooth=10.0, sh=0, psv=1,
+             name="marmousi", sigma=0):
+    """Batched P-SV inversion setup on the Marmousi-shaped model.  The
+    initial model is the true one smoothed with a symmetric Gaussian
+    (`smooth` cells), water rows kept."""
+    vp, vs, rho, water = layered_faulted(nx, nz, dx, seed)
+    f32 = np.float32
+    sm = lambda a: np.where(water, a, _smooth(a, smooth)).astype(f32)
+    vp0, vs0, rho0 = sm(vp), sm(vs), sm(rho)
+    vs0 = np.where(water, f32(0), vs0).astype(f32)
+    lam, mu = vps2lm(vp.ravel(), vs.ravel(), rho.ravel())
+    lam0, mu0 = vps2lm(vp0.ravel(), vs0.ravel(), rho0.ravel())
+    xmax = (nx - 1) * dx
+    src_x = [(s + 0.5) * xmax / nsrc for s in range(nsrc)]
+    src_ix = np.array([invhip.calc_idx(xx, xmax, nx) for xx in src_x], np.int32)
+    rec_ix = (np.arange(nrec) * rec_step).astype(np.int32)
+    rec_ix = np.minimum(rec_ix, nx - 1).astype(np.int32)
+    sx, sy, sz = _wavelets(nt, dt, f0, t0, amp, 0.0, nsrc)
+    return Problem(name, nx, nz, f32(dx), f32(dx), f32(dt), nt, sh, psv, lam, mu, rho.ravel(),
+                   lam0, mu0, rho0.ravel(), src_ix, np.full(nsrc, src_depth, np.int32), sx, sy,
+                   sz, rec_ix, np.full(nrec, rec_depth, np.int32), abs_top=0, sigma=sigma,
+                   meta=dict(f0=f0, t0=t0, amp=amp, seed=seed))
+
+
+def tiny(nx=48, nz=40, dx=10.0, dt=1e-3, nt=300, nsrc=3, nrec=12, sh=1, psv=1, seed=7,
+         f0=25.0, t0=0.04, sfe=10, abs_width=8, name="tiny"):
+    """Small random layered setup for parity tests (seconds on the oracle)."""
+    rng = np.random.default_rng(seed)
+    f32 = np.float32
+    z = np.arange(nz)[None, :]
+    base = 2000.0 + 1500.0 * z / (nz - 1) + np.zeros((nx, 1))
+    vp = (base * (1 + 0.05 * rng.standard_normal((nx, nz)))).astype(f32)
+    vs = (vp / 1.8).astype(f32)
+    rho = (1800.0 + 0.2 * vp).astype(f32)
+    lam, mu = vps2lm(vp.ravel(), vs.ravel(), rho.ravel())
+    vp0 = _smooth(vp, 3.0).astype(f32)
+    vs0 = (vp0 / 1.8).astype(f32)
+    rho0 = (1800.0 + 0.2 * vp0).astype(f32)
+    lam0, mu0 = vps2lm(vp0.ravel(), vs0.ravel(), rho0.ravel())
+    src_ix = rng.integers(abs_width, nx - abs_width, nsrc).astype(np.int32)
+    src_iz = rng.integers(3, nz // 3, nsrc).astype(np.int32)
+    rec_ix = np.linspace(2, nx - 3, nrec).astype(np.int32)
+    rec_iz = np.full(nrec, 4, np.int32)
+    angle = 0.7
+    stf = [invhip.source_ricker(nt, dt, f0, t0, angle, 1e9) for _ in range(nsrc)]
+    sx = np.stack([s[0] for s in stf])
+    sy = np.stack([s[1] for s in stf])
+    sz = np.stack([s[2] for s in stf])
+    return Problem(name, nx, nz, f32(dx), f32(dx), f32(dt), nt, sh, psv, lam, mu, rho.ravel(),
+                   lam0, mu0, rho0.ravel(), src_ix, src_iz, sx, sy, sz, rec_ix, rec_iz, sfe=sfe,
+                   abs_width=abs_width, meta=dict(f0=f0, t0=t0, seed=seed))

@@ -1,0 +1,106 @@
+"""GPU parity of the whole gradient evaluation (Misfit::run(true),
+misfit/misfit.h:15-51): forward, waveform misfit, adjoint, Gaussian filter."""
+import numpy as np
+import pytest
+
+from conftest import inv, oracle_problem, oracle_traces, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _obs_from_true(P):
+    """Observed data: forward on the true model (Misfit::init, misfit.h:135-148)."""
+    O = oracle_problem(P, model="true")
+    obs = []
+    for isrc in range(P.nsrc):
+        tr, _, _ = O.forward(isrc)
+        obs.append(oracle_traces(tr, P))
+    return np.stack(obs).astype(np.float32)
+
+
+@pytest.mark.parametrize("batch", [1, 3])
+@pytest.mark.parametrize("sigma", [0, 3])
+def test_gradient_matches_oracle(gpu, batch, sigma):
+    P = inv.problems.tiny(sh=1, psv=1, nsrc=3)
+    obs = _obs_from_true(P)
+    print(f"gradient batch={batch} sigma={sigma}", flush=True)
+    ctx = P.context(batch=batch)
+    for s in range(P.nsrc):
+        ctx.obs_set(s, obs[s])
+    f = ctx.gradient(misfit_type=0, sigma=sigma)
+    kl, km, kr = ctx.kernels_get()
+    O = oracle_problem(P)
+    f32, okl, okm, okr = O.gradient(obs, 0, sigma, dtype=np.float32)
+    f64, dkl, dkm, dkr = O.gradient(obs.astype(np.float64), 0, sigma, dtype=np.float64)
+    assert abs(f - f64) / abs(f64) < 1e-5
+    for a, b, c, name in ((kl, okl, dkl, "k_lambda"), (km, okm, dkm, "k_mu"),
+                          (kr, okr, dkr, "k_rho")):
+        e32, e64 = rel(a, b), rel(a, c)
+        print(f"  {name}: vs f32 oracle {e32:.2e}, vs f64 oracle {e64:.2e}")
+        if batch == 1:
+            assert np.array_equal(a, b), f"{name} plain path not bit-exact ({e32:.3e})"
+        assert e32 < 1e-4 and e64 < 1e-4
+    ctx.close()
+
+
+def test_misfit_waveform_matches_oracle(gpu):
+    P = inv.problems.tiny(sh=1, psv=1, nsrc=2)
+    obs = _obs_from_true(P)
+    ctx = P.context(batch=2)
+    for s in range(P.nsrc):
+        ctx.obs_set(s, obs[s])
+    ctx.forward(0, 2, snapshots=False)
+    tot, chi = ctx.misfit(0, 2, kind=0)
+    import pyoracle
+    ref_tot = 0.0
+    for s in range(2):
+        syn = ctx.traces_get(s)
+        ad = ctx.adstf_get(s)
+        for c in range(P.ncomp):
+            for r in range(P.nrec):
+                x, a = pyoracle.waveform(syn[c, r], obs[s, c, r], P.dt)
+                assert np.array_equal(a, ad[c, r])
+                assert chi[s, r, c] == np.float64(np.float32(x))
+                ref_tot += x
+    assert abs(tot - ref_tot) <= 1e-12 * abs(ref_tot)
+    ctx.close()
+
+
+def test_filter_matches_oracle(gpu):
+    import pyoracle
+    rng = np.random.default_rng(3)
+    nx, nz = 70, 53
+    a = rng.standard_normal(nx * nz).astype(np.float32)
+    ctx = inv.invhip.Context(0)
+    dev = ctx.alloc(a.nbytes)
+    for sigma in (1, 5, 20):
+        ctx.h2d(dev, a)
+        ctx.filter_gauss(dev, nx, nz, sigma)
+        out = ctx.d2h(dev, a.shape)
+        ref = pyoracle.gauss(a, nx, nz, sigma)
+        assert np.array_equal(out, ref), f"sigma {sigma}: rel {rel(out, ref):.3e}"
+    ctx.close()
+
+
+def test_vector_ops(gpu):
+    rng = np.random.default_rng(4)
+    n = 100003
+    a = rng.standard_normal(n).astype(np.float32)
+    b = rng.standard_normal(n).astype(np.float32)
+    ctx = inv.invhip.Context(0)
+    da, db, dc = (ctx.alloc(4 * n) for _ in range(3))
+    ctx.h2d(da, a)
+    ctx.h2d(db, b)
+    ctx.vec_axpby(dc, 0.5, da, -2.0, db, n)
+    assert np.array_equal(ctx.d2h(dc, n), np.float32(0.5) * a + np.float32(-2.0) * b)
+    ctx.vec_scale(dc, 3.0, da, n)
+    assert np.array_equal(ctx.d2h(dc, n), np.float32(3.0) * a)
+    ctx.vec_mul(dc, da, db, n)
+    assert np.array_equal(ctx.d2h(dc, n), a * b)
+    ctx.vec_copy(dc, db, n)
+    assert np.array_equal(ctx.d2h(dc, n), b)
+    d = ctx.vec_dot(da, db, n)
+    ref = float(np.dot(a.astype(np.float64), b.astype(np.float64)))
+    assert abs(d - ref) <= 1e-12 * np.sqrt(n) * np.linalg.norm(a) * np.linalg.norm(b)
+    assert ctx.vec_amax(da, n) == np.abs(a).max()
+    ctx.close()
