@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark: forward+adjoint gradient throughput on one MI355X.
+
+A step is one full gradient evaluation of the batched P-SV Marmousi-shaped
+workload (737x240 cells at 12.5 m, 16 shots, 368 receivers, dt 1 ms,
+nt 4000): forward of every shot with snapshots, waveform misfit and adjoint
+sources, adjoint with lambda/mu/rho kernel accumulation, one-sided Gaussian
+smoothing of the three kernels.  All 16 shots run as one batch per launch.
+Inputs (model, observed data) are resident in HBM before the timed region.
+
+Prints one JSON line (rank 0).  Extra keys: roofline (stencil step vs HBM
+peak, timed live with HIP events inside the library), cpu_baseline (the CPU
+oracle on a bounded sample of the same workload), plain (the one-shot-per-
+launch GPU path on the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Gcell-updates/s fwd+adjoint; gradients/hr; p50/p99 shot latency vs plain"
+HBM_PEAK_GBPS = 8000.0
+# Algorithmic HBM bytes per cell-update of one P-SV stencil time step: each of
+# the 7 wavefields (vx, vz, ux, uz, sxx, szz, sxz) read and written once and
+# the 3 material fields read once, FP32.
+PSV_STEP_BYTES = 4 * (2 * 7 + 3)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=16, help="shots per launch")
+    ap.add_argument("--nt", type=int, default=4000)
+    ap.add_argument("--sigma", type=int, default=5, help="Gaussian filter sigma (cells)")
+    ap.add_argument("--prof-every", type=int, default=16,
+                    help="time every n-th step's stencil launches with HIP events (0 = off)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the one-shot GPU path")
+    ap.add_argument("--cpu-shots", type=int, default=1)
+    return ap.parse_args()
+
+
+def cpu_baseline(P, shots):
+    """The CPU oracle (FP32, OpenMP) on `shots` shots of the same workload."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle  # the checker, used here only as the timed CPU baseline
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    pyoracle.set_threads(threads)
+    Ps = P.with_(src_ix=P.src_ix[:shots], src_iz=P.src_iz[:shots], stf_x=P.stf_x[:shots],
+                 stf_y=P.stf_y[:shots], stf_z=P.stf_z[:shots])
+    ab = pyoracle.absbound(P.nx, P.nz, P.abs_width, P.abs_alpha, P.abs_left, P.abs_right,
+                           P.abs_bottom, P.abs_top)
+    O = pyoracle.Problem(P.nx, P.nz, P.dx, P.dz, P.dt, P.nt, P.lam0, P.mu0, P.rho0, ab,
+                         Ps.src_ix, Ps.src_iz, Ps.stf_x, Ps.stf_y, Ps.stf_z, P.rec_ix, P.rec_iz,
+                         sh=P.sh, psv=P.psv, sfe=P.sfe, adj=1)
+    obs = np.zeros((shots, P.ncomp, P.nrec, P.nt), np.float32)
+    t0 = time.perf_counter()
+    O.gradient(obs, 0, P.sigma, dtype=np.float32)
+    dt = time.perf_counter() - t0
+    work = Ps.cell_updates(shots)
+    return {"value": work / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle FP32 gradient (forward+misfit+adjoint+filter) of {shots} of the "
+                      f"16 shots, full nt={P.nt}, {P.nx}x{P.nz}; {dt:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
+              "shot sharding is described in DESIGN.md but not built.", file=sys.stderr)
+        sys.exit(2)
+    import __graft_entry__ as graft
+
+    inv = graft.load_package()
+    P = inv.problems.marmousi(nt=a.nt, sigma=a.sigma)
+    B = min(a.batch, P.nsrc)
+    print(f"# workload {P.name} {P.nx}x{P.nz} shots={P.nsrc} nt={P.nt} batch={B}",
+          file=sys.stderr, flush=True)
+    ctx = P.context(batch=B, model="true")
+    # observed data: forward on the true model (Misfit::init, misfit.h:135-148)
+    for first in range(0, P.nsrc, B):
+        n = min(B, P.nsrc - first)
+        ctx.forward(first, n, snapshots=False)
+        ctx.obs_from_traces(first, n)
+    P.load_model(ctx, "init")
+    ctx.sync()
+    print("# observed data ready", file=sys.stderr, flush=True)
+
+    for _ in range(a.warmup):
+        ctx.gradient(P.misfit, P.sigma)
+    ctx.sync()
+    ctx.prof_reset()
+    ctx.prof_enable(a.prof_every)
+    t0 = time.perf_counter()
+    f = 0.0
+    for _ in range(a.steps):
+        f = ctx.gradient(P.misfit, P.sigma)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.prof_enable(0)
+    ms = (t1 - t0) / a.steps * 1e3
+    work = P.cell_updates()
+    value = work / (ms / 1e3) / 1e9
+    print(f"# timed {a.steps} gradients: {ms:.1f} ms each, misfit {f:.6e}", file=sys.stderr,
+          flush=True)
+
+    nv, tv = ctx.prof_read(inv.invhip.INV_PROF_VELOCITY)
+    ns, ts = ctx.prof_read(inv.invhip.INV_PROF_STRESS)
+    roof = None
+    if nv and ns:
+        step_s = (tv / nv + ts / ns) / 1e3
+        cells = B * P.N
+        achieved = PSV_STEP_BYTES * cells / step_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "kernel": "P-SV stencil step (velocity + stress launch pair)",
+                "alg_bytes_per_launch": PSV_STEP_BYTES * cells,
+                "avg_us": {"velocity": round(tv / nv * 1e3, 2), "stress": round(ts / ns * 1e3, 2)},
+                "samples": {"velocity": nv, "stress": ns}}
+    ni, ti = ctx.prof_read(inv.invhip.INV_PROF_INTERACT)
+    ctx.close()
+
+    plain = None
+    if not a.no_plain:
+        pctx = P.context(batch=1, model="true")
+        for s in range(P.nsrc):
+            pctx.forward(s, 1, snapshots=False)
+            pctx.obs_from_traces(s, 1)
+        P.load_model(pctx, "init")
+        pctx.gradient(P.misfit, P.sigma)
+        pctx.sync()
+        t0 = time.perf_counter()
+        pctx.gradient(P.misfit, P.sigma)
+        pctx.sync()
+        pms = (time.perf_counter() - t0) * 1e3
+        pctx.close()
+        plain = {"ms_per_gradient": round(pms, 2), "gcell_per_s": round(work / pms / 1e6, 2),
+                 "gradients_per_hr": round(3.6e6 / pms, 1),
+                 "note": "one shot per launch, same kernels, no batching"}
+        print(f"# plain path: {pms:.1f} ms per gradient", file=sys.stderr, flush=True)
+
+    cpu = None if a.no_cpu else cpu_baseline(P, a.cpu_shots)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Marmousi-shaped model; observed data modelled on the true model)",
+        "config": {"workload": "batched P-SV gradient, Marmousi-shaped 737x240 @ 12.5 m, "
+                               "16 shots, 368 receivers, dt 1 ms, nt 4000",
+                   "nx": P.nx, "nz": P.nz, "shots": P.nsrc, "receivers": P.nrec, "nt": P.nt,
+                   "batch": B, "sfe": P.sfe, "filter_sigma": P.sigma, "misfit": "waveform",
+                   "snapshots": "fp32 in HBM"},
+        "gradients_per_hr": round(3.6e6 / ms, 1),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "plain": plain,
+    }
+    if ni:
+        out["interaction_avg_us"] = round(ti / ni * 1e3, 2)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

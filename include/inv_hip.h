@@ -187,6 +187,17 @@ int inv_memcpy_d2h(inv_ctx *ctx, void *host, const void *dev, size_t bytes);
  * decoded to float. */
 int inv_state_get(inv_ctx *ctx, int slot, float *host);
 int inv_snapshots_get(inv_ctx *ctx, int slot, float *host);
+/* ---- measurement -------------------------------------------------------
+ * Sampled kernel timing: with every > 0, each propagation kernel launched on
+ * a time step it with it % every == 0 is bracketed by HIP events on the
+ * context stream; inv_prof_read returns the number of sampled launches and
+ * their summed duration.  No counterpart in the reference (which times only
+ * the whole run, optimizer/optimizer.h:432,456). */
+enum inv_prof_kernel { INV_PROF_VELOCITY = 0, INV_PROF_STRESS = 1, INV_PROF_INTERACT = 2 };
+int inv_prof_enable(inv_ctx *ctx, int every);
+int inv_prof_reset(inv_ctx *ctx);
+int inv_prof_read(inv_ctx *ctx, int kernel, long long *count, double *total_ms);
+
 /* Run only the first nsteps steps of the next forward (0 = all). */
 int inv_set_step_limit(inv_ctx *ctx, int nsteps);
 

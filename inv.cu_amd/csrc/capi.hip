@@ -258,6 +258,7 @@ int inv_ctx_destroy(inv_ctx *ctx) {
                       &c.gauss_w, &c.gauss_tmp, &c.gauss_sx, &c.gauss_sz};
     for (DevBuf *b : bufs) b->release();
     for (void *p : c.user_allocs) (void)hipFree(p);
+    prof_release(c);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return INV_OK;
@@ -746,6 +747,33 @@ int inv_snapshots_get(inv_ctx *ctx, int slot, float *host) {
         sync(c);
         const size_t b = (size_t)c.nsfe * c.nsnapf * c.N * sizeof(float);
         INV_HIP(hipMemcpy(host, c.snap.as<char>() + slot * b, b, hipMemcpyDeviceToHost));
+    });
+}
+
+int inv_prof_enable(inv_ctx *ctx, int every) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(every >= 0, INV_ERR_ARG, "every must be >= 0");
+        c.prof.every = every;
+    });
+}
+
+int inv_prof_reset(inv_ctx *ctx) {
+    return guard(ctx, [&](Ctx &c) {
+        prof_resolve(c);
+        for (int k = 0; k < PROF_NKIND; k++) {
+            c.prof.count[k] = 0;
+            c.prof.ms[k] = 0;
+        }
+    });
+}
+
+int inv_prof_read(inv_ctx *ctx, int kernel, long long *count, double *total_ms) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(kernel >= 0 && kernel < PROF_NKIND && count && total_ms, INV_ERR_ARG,
+                    "bad kernel id");
+        prof_resolve(c);
+        *count = c.prof.count[kernel];
+        *total_ms = c.prof.ms[kernel];
     });
 }
 

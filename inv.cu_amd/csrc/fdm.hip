@@ -376,17 +376,75 @@ void fdm_zero_state(Ctx &c, int count) {
     }
 }
 
-template <bool SH, bool PSV>
-static void launch_step(Ctx &c, const VelArgs &va, const StrArgs &sa, int count) {
-    const dim3 grid = tile_grid(c, count), block(TILE_Z, TILE_X);
-    k_velocity<SH, PSV><<<grid, block, 0, c.stream>>>(va);
-    k_stress<SH, PSV><<<grid, block, 0, c.stream>>>(sa);
+// ---- sampled event timing ----
+static constexpr size_t PROF_MAX_PENDING = 1 << 15;
+
+static hipEvent_t prof_event(Ctx &c) {
+    if (!c.prof.pool.empty()) {
+        hipEvent_t e = c.prof.pool.back();
+        c.prof.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    INV_HIP(hipEventCreate(&e));
+    return e;
 }
 
-static void step(Ctx &c, const VelArgs &va, const StrArgs &sa, int count) {
-    if (c.g.sh && c.g.psv) launch_step<true, true>(c, va, sa, count);
-    else if (c.g.sh) launch_step<true, false>(c, va, sa, count);
-    else launch_step<false, true>(c, va, sa, count);
+bool prof_begin(Ctx &c, int it, hipEvent_t *a) {
+    if (c.prof.every <= 0 || it % c.prof.every != 0) return false;
+    if (c.prof.pending.size() >= PROF_MAX_PENDING) return false;
+    *a = prof_event(c);
+    INV_HIP(hipEventRecord(*a, c.stream));
+    return true;
+}
+
+void prof_end(Ctx &c, int kind, hipEvent_t a) {
+    hipEvent_t b = prof_event(c);
+    INV_HIP(hipEventRecord(b, c.stream));
+    c.prof.pending.push_back({kind, a, b});
+}
+
+void prof_resolve(Ctx &c) {
+    if (c.prof.pending.empty()) return;
+    INV_HIP(hipStreamSynchronize(c.stream));
+    for (auto &r : c.prof.pending) {
+        float ms = 0;
+        INV_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        c.prof.count[r.kind] += 1;
+        c.prof.ms[r.kind] += ms;
+        c.prof.pool.push_back(r.a);
+        c.prof.pool.push_back(r.b);
+    }
+    c.prof.pending.clear();
+}
+
+void prof_release(Ctx &c) {
+    (void)hipStreamSynchronize(c.stream);
+    for (auto &r : c.prof.pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : c.prof.pool) (void)hipEventDestroy(e);
+    c.prof.pending.clear();
+    c.prof.pool.clear();
+}
+
+template <bool SH, bool PSV>
+static void launch_step(Ctx &c, const VelArgs &va, const StrArgs &sa, int count, int it) {
+    const dim3 grid = tile_grid(c, count), block(TILE_Z, TILE_X);
+    hipEvent_t a;
+    bool p = prof_begin(c, it, &a);
+    k_velocity<SH, PSV><<<grid, block, 0, c.stream>>>(va);
+    if (p) prof_end(c, PROF_VELOCITY, a);
+    p = prof_begin(c, it, &a);
+    k_stress<SH, PSV><<<grid, block, 0, c.stream>>>(sa);
+    if (p) prof_end(c, PROF_STRESS, a);
+}
+
+static void step(Ctx &c, const VelArgs &va, const StrArgs &sa, int count, int it) {
+    if (c.g.sh && c.g.psv) launch_step<true, true>(c, va, sa, count, it);
+    else if (c.g.sh) launch_step<true, false>(c, va, sa, count, it);
+    else launch_step<false, true>(c, va, sa, count, it);
     INV_HIP(hipGetLastError());
 }
 
@@ -469,7 +527,7 @@ void fdm_forward(Ctx &c, int first, int count, int flags) {
             const long isfe = c.nsfe - (it + 1) / c.g.sfe;
             va.snap = c.snap.as<float>() + isfe * c.nsnapf * c.N;
         }
-        step(c, va, sa, count);
+        step(c, va, sa, count, it);
     }
 }
 
@@ -525,13 +583,16 @@ void fdm_adjoint(Ctx &c, int first, int count) {
     fdm_zero_state(c, count);
     for (int it = 0; it < c.g.nt; it++) {
         va.it = it;
-        step(c, va, sa, count);
+        step(c, va, sa, count, it);
         if (it % c.g.sfe == 0) {
             ia.snap = c.snap.as<float>() + (long)(it / c.g.sfe) * c.nsnapf * c.N;
+            hipEvent_t a;
+            const bool p = prof_begin(c, it / c.g.sfe, &a);
             if (c.g.sh && c.g.psv) launch_interact<true, true>(c, ia, count);
             else if (c.g.sh) launch_interact<true, false>(c, ia, count);
             else launch_interact<false, true>(c, ia, count);
             INV_HIP(hipGetLastError());
+            if (p) prof_end(c, PROF_INTERACT, a);
         }
     }
     if (!direct) fdm_kslot_reduce(c, count);

@@ -54,6 +54,20 @@ struct PointList {
 constexpr int TILE_Z = 64;  // one wavefront along z (the contiguous axis)
 constexpr int TILE_X = 4;   // wavefronts per block along x
 
+// Sampled kernel timing with HIP events on the context stream.
+enum ProfKind { PROF_VELOCITY = 0, PROF_STRESS = 1, PROF_INTERACT = 2, PROF_NKIND = 3 };
+struct Prof {
+    int every = 0;  // sample every n-th time step (0 = off)
+    struct Rec {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    long long count[PROF_NKIND] = {};
+    double ms[PROF_NKIND] = {};
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -99,6 +113,7 @@ struct Ctx {
     int gauss_sigma = -1, gauss_nx = -1, gauss_nz = -1, gauss_m = 0;
 
     std::vector<void *> user_allocs;
+    Prof prof;
 
     int slot_offset_traces() const { return ncomp * nrec * g.nt; }
 };
@@ -110,6 +125,10 @@ void fdm_forward(Ctx &c, int first, int count, int flags);
 void fdm_adjoint(Ctx &c, int first, int count);
 void fdm_zero_state(Ctx &c, int count);
 void fdm_kslot_reduce(Ctx &c, int count);
+bool prof_begin(Ctx &c, int it, hipEvent_t *a);
+void prof_end(Ctx &c, int kind, hipEvent_t a);
+void prof_resolve(Ctx &c);
+void prof_release(Ctx &c);
 
 // misfit.hip ----------------------------------------------------------------
 void misfit_waveform(Ctx &c, int count, int first);

@@ -18,6 +18,7 @@ INV_OK = 0
 INV_LAMBDA, INV_MU, INV_RHO, INV_K_LAMBDA, INV_K_MU, INV_K_RHO, INV_OUT, INV_ADSTF, INV_OBS = range(9)
 INV_SNAP_F32, INV_SNAP_F16, INV_SNAP_BF16 = 0, 1, 2
 INV_FWD_SNAPSHOTS = 1
+INV_PROF_VELOCITY, INV_PROF_STRESS, INV_PROF_INTERACT = 0, 1, 2
 
 STATUS = {0: "INV_OK", -1: "INV_ERR_ARG", -2: "INV_ERR_HIP", -3: "INV_ERR_STATE",
           -4: "INV_ERR_NOMEM", -5: "INV_ERR_UNSUPPORTED"}
@@ -90,6 +91,9 @@ SIGNATURES = {
     "inv_state_get": [_vp, _i, _vp],
     "inv_snapshots_get": [_vp, _i, _vp],
     "inv_set_step_limit": [_vp, _i],
+    "inv_prof_enable": [_vp, _i],
+    "inv_prof_reset": [_vp],
+    "inv_prof_read": [_vp, _i, _vp, _vp],
 }
 
 _lib = None
@@ -356,3 +360,16 @@ class Context:
 
     def set_step_limit(self, n):
         self._chk(self.L.inv_set_step_limit(self.h, n))
+
+    # ---- measurement ----
+    def prof_enable(self, every):
+        self._chk(self.L.inv_prof_enable(self.h, every))
+
+    def prof_reset(self):
+        self._chk(self.L.inv_prof_reset(self.h))
+
+    def prof_read(self, kernel):
+        n = C.c_longlong()
+        ms = C.c_double()
+        self._chk(self.L.inv_prof_read(self.h, kernel, C.byref(n), C.byref(ms)))
+        return n.value, ms.value

@@ -209,11 +209,17 @@ def layered_faulted(nx=737, nz=240, dx=12.5, seed=1, nwater=12, vmin=1500.0, vma
 
 
 def marmousi(nx=737, nz=240, dx=12.5, dt=1e-3, nt=4000, nsrc=16, nrec=368, f0=5.0, t0=0.3,
-             amp=1e10, seed=1, src_depth=2, rec_depth=0, rec_step=2, smooth=10.0, sh=0, psv=1,
+             amp=1e10, seed=1, src_depth=2, rec_depth=2, rec_step=2, smooth=10.0, sh=0, psv=1,
              name="marmousi", sigma=0):
     """Batched P-SV inversion setup on the Marmousi-shaped model.  The
     initial model is the true one smoothed with a symmetric Gaussian
-    (`smooth` cells), water rows kept."""
+    (`smooth` cells), water rows kept.
+
+    Receivers sit on row 2, the shallowest row the stencil couples to the
+    interior: the z-terms of the divergence exist only for 2 <= j < nz-2
+    (solver/fdm.h:84,98) and those of the gradient for 1 <= j < nz-2
+    (fdm.h:111,128), so rows 0 and 1 never receive energy from below and a
+    receiver there records zeros."""
     vp, vs, rho, water = layered_faulted(nx, nz, dx, seed)
     f32 = np.float32
     sm = lambda a: np.where(water, a, _smooth(a, smooth)).astype(f32)
