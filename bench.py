@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-shot GPU path")
     ap.add_argument("--cpu-shots", type=int, default=1)
+    ap.add_argument("--variant", type=int, default=0, help="0 fused step kernel, 1 kernel pair")
+    ap.add_argument("--tile-x", type=int, default=16, help="fused tile rows (8, 16, 32)")
     return ap.parse_args()
 
 
@@ -90,6 +92,7 @@ def main():
     print(f"# workload {P.name} {P.nx}x{P.nz} shots={P.nsrc} nt={P.nt} batch={B}",
           file=sys.stderr, flush=True)
     ctx = P.context(batch=B, model="true")
+    ctx.set_variant(a.variant, a.tile_x)
     # observed data: forward on the true model (Misfit::init, misfit.h:135-148)
     for first in range(0, P.nsrc, B):
         n = min(B, P.nsrc - first)
@@ -119,23 +122,32 @@ def main():
 
     nv, tv = ctx.prof_read(inv.invhip.INV_PROF_VELOCITY)
     ns, ts = ctx.prof_read(inv.invhip.INV_PROF_STRESS)
+    nf, tf = ctx.prof_read(inv.invhip.INV_PROF_STEP)
     roof = None
-    if nv and ns:
+    cells = B * P.N
+    if nf:
+        step_s = tf / nf / 1e3
+        kernel = "k_step<false,true,TX> (fused P-SV time step)"
+        avg = {"step": round(tf / nf * 1e3, 2)}
+        samples = {"step": nf}
+    elif nv and ns:
         step_s = (tv / nv + ts / ns) / 1e3
-        cells = B * P.N
+        kernel = "P-SV stencil step (velocity + stress launch pair)"
+        avg = {"velocity": round(tv / nv * 1e3, 2), "stress": round(ts / ns * 1e3, 2)}
+        samples = {"velocity": nv, "stress": ns}
+    if nf or (nv and ns):
         achieved = PSV_STEP_BYTES * cells / step_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": "P-SV stencil step (velocity + stress launch pair)",
-                "alg_bytes_per_launch": PSV_STEP_BYTES * cells,
-                "avg_us": {"velocity": round(tv / nv * 1e3, 2), "stress": round(ts / ns * 1e3, 2)},
-                "samples": {"velocity": nv, "stress": ns}}
+                "kernel": kernel, "alg_bytes_per_launch": PSV_STEP_BYTES * cells,
+                "avg_us": avg, "samples": samples}
     ni, ti = ctx.prof_read(inv.invhip.INV_PROF_INTERACT)
     ctx.close()
 
     plain = None
     if not a.no_plain:
         pctx = P.context(batch=1, model="true")
+        pctx.set_variant(a.variant, a.tile_x)
         for s in range(P.nsrc):
             pctx.forward(s, 1, snapshots=False)
             pctx.obs_from_traces(s, 1)

@@ -193,10 +193,22 @@ int inv_snapshots_get(inv_ctx *ctx, int slot, float *host);
  * context stream; inv_prof_read returns the number of sampled launches and
  * their summed duration.  No counterpart in the reference (which times only
  * the whole run, optimizer/optimizer.h:432,456). */
-enum inv_prof_kernel { INV_PROF_VELOCITY = 0, INV_PROF_STRESS = 1, INV_PROF_INTERACT = 2 };
+enum inv_prof_kernel {
+    INV_PROF_VELOCITY = 0,  /* velocity half-step (kernel-pair variant) */
+    INV_PROF_STRESS = 1,    /* stress half-step (kernel-pair variant) */
+    INV_PROF_INTERACT = 2,  /* adjoint kernel accumulation */
+    INV_PROF_STEP = 3       /* whole time step (fused variant) */
+};
 int inv_prof_enable(inv_ctx *ctx, int every);
 int inv_prof_reset(inv_ctx *ctx);
 int inv_prof_read(inv_ctx *ctx, int kernel, long long *count, double *total_ms);
+
+/* Propagation kernel variant: INV_VARIANT_FUSED (default) runs one kernel per
+ * time step on tiles of 64 (z) x tile_x (x) cells, tile_x in {8, 16, 32}
+ * (0 keeps the current value); INV_VARIANT_SPLIT runs a velocity and a stress
+ * kernel per step.  Results are bit-identical between variants. */
+enum inv_variant { INV_VARIANT_FUSED = 0, INV_VARIANT_SPLIT = 1 };
+int inv_set_variant(inv_ctx *ctx, int variant, int tile_x);
 
 /* Run only the first nsteps steps of the next forward (0 = all). */
 int inv_set_step_limit(inv_ctx *ctx, int nsteps);

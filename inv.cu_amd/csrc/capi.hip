@@ -255,7 +255,11 @@ int inv_ctx_destroy(inv_ctx *ctx) {
                       &c.vy, &c.uy, &c.sxy, &c.szy, &c.snap, &c.out, &c.adstf, &c.obs, &c.chi,
                       &c.taper, &c.red, &c.scal, &c.stf, &c.rec_list.ptr, &c.rec_list.cell,
                       &c.rec_list.row, &c.src_list.ptr, &c.src_list.cell, &c.src_list.row,
-                      &c.gauss_w, &c.gauss_tmp, &c.gauss_sx, &c.gauss_sz};
+                      &c.gauss_w, &c.gauss_tmp, &c.gauss_sx, &c.gauss_sz, &c.vx_b, &c.vz_b,
+                      &c.sxx_b, &c.szz_b, &c.sxz_b, &c.vy_b, &c.sxy_b, &c.szy_b,
+                      &c.f_src_v.ptr, &c.f_src_v.cell, &c.f_src_v.row, &c.f_rec_v.ptr,
+                      &c.f_rec_v.cell, &c.f_rec_v.row, &c.f_rec_t.ptr, &c.f_rec_t.cell,
+                      &c.f_rec_t.row};
     for (DevBuf *b : bufs) b->release();
     for (void *p : c.user_allocs) (void)hipFree(p);
     prof_release(c);
@@ -305,6 +309,8 @@ int inv_grid_set(inv_ctx *ctx, const inv_grid_desc *d) {
         alloc_slots(c);
         alloc_traces(c);
         c.src_list_first = -2;
+        c.parity = 0;
+        fused_invalidate_lists(c);
         if (c.have_rec) build_receivers(c);
     });
 }
@@ -433,6 +439,7 @@ int inv_sources_set(inv_ctx *ctx, int nsrc, const int *ix, const int *iz, const 
         INV_HIP(hipMemcpy(c.stf.as<char>() + 2 * b, stf_z, b, hipMemcpyHostToDevice));
         c.have_src = true;
         c.src_list_first = -2;
+        fused_invalidate_lists(c);
         alloc_traces(c);
     });
 }
@@ -449,6 +456,7 @@ int inv_receivers_set(inv_ctx *ctx, int nrec, const int *ix, const int *iz) {
         c.rec_ix.assign(ix, ix + nrec);
         c.rec_iz.assign(iz, iz + nrec);
         build_receivers(c);
+        fused_invalidate_lists(c);
         c.have_rec = true;
         alloc_traces(c);
     });
@@ -458,7 +466,8 @@ int inv_forward(inv_ctx *ctx, int first, int count, int flags) {
     return guard(ctx, [&](Ctx &c) {
         require_ready(c);
         check_slots(c, first, count);
-        fdm_forward(c, first, count, flags);
+        if (c.variant == INV_VARIANT_SPLIT) fdm_forward(c, first, count, flags);
+        else fused_forward(c, first, count, flags);
     });
 }
 
@@ -467,7 +476,8 @@ int inv_adjoint(inv_ctx *ctx, int first, int count) {
         require_ready(c);
         check_slots(c, first < 0 ? 0 : first, count);
         INV_REQUIRE(c.g.adj && c.snap.p, INV_ERR_STATE, "adjoint needs adj = 1 (snapshots)");
-        fdm_adjoint(c, first, count);
+        if (c.variant == INV_VARIANT_SPLIT) fdm_adjoint(c, first, count);
+        else fused_adjoint(c, first, count);
     });
 }
 
@@ -599,9 +609,14 @@ int inv_gradient(inv_ctx *ctx, int misfit_type, int sigma, int kernel, double *m
         double total = 0;
         for (int first = 0; first < c.nsrc; first += c.g.batch) {
             const int count = std::min(c.g.batch, c.nsrc - first);
-            fdm_forward(c, first, count, kernel ? INV_FWD_SNAPSHOTS : 0);
+            const bool split = c.variant == INV_VARIANT_SPLIT;
+            if (split) fdm_forward(c, first, count, kernel ? INV_FWD_SNAPSHOTS : 0);
+            else fused_forward(c, first, count, kernel ? INV_FWD_SNAPSHOTS : 0);
             total += run_misfit(c, first, count, misfit_type, nullptr);
-            if (kernel) fdm_adjoint(c, first, count);
+            if (kernel) {
+                if (split) fdm_adjoint(c, first, count);
+                else fused_adjoint(c, first, count);
+            }
         }
         if (kernel && sigma > 0) {
             gauss_filter(c, c.k_lambda.as<float>(), c.g.nx, c.g.nz, sigma);
@@ -732,12 +747,21 @@ int inv_state_get(inv_ctx *ctx, int slot, float *host) {
         sync(c);
         const size_t b = c.N * sizeof(float);
         int f = 0;
-        auto get = [&](DevBuf &d) {
+        auto get = [&](const float *d) {
             INV_HIP(hipMemcpy(reinterpret_cast<char *>(host) + (size_t)f++ * b,
-                              d.as<char>() + (size_t)slot * b, b, hipMemcpyDeviceToHost));
+                              reinterpret_cast<const char *>(d) + (size_t)slot * b, b,
+                              hipMemcpyDeviceToHost));
         };
-        if (c.g.sh) { get(c.vy); get(c.uy); get(c.sxy); get(c.szy); }
-        if (c.g.psv) { get(c.vx); get(c.vz); get(c.ux); get(c.uz); get(c.sxx); get(c.szz); get(c.sxz); }
+        auto cur = [&](DevBuf &x, DevBuf &y) { return cur_field(c, x, y); };
+        if (c.g.sh) {
+            get(cur(c.vy, c.vy_b)); get(c.uy.as<float>());
+            get(cur(c.sxy, c.sxy_b)); get(cur(c.szy, c.szy_b));
+        }
+        if (c.g.psv) {
+            get(cur(c.vx, c.vx_b)); get(cur(c.vz, c.vz_b)); get(c.ux.as<float>());
+            get(c.uz.as<float>()); get(cur(c.sxx, c.sxx_b)); get(cur(c.szz, c.szz_b));
+            get(cur(c.sxz, c.sxz_b));
+        }
     });
 }
 
@@ -774,6 +798,18 @@ int inv_prof_read(inv_ctx *ctx, int kernel, long long *count, double *total_ms) 
         prof_resolve(c);
         *count = c.prof.count[kernel];
         *total_ms = c.prof.ms[kernel];
+    });
+}
+
+int inv_set_variant(inv_ctx *ctx, int variant, int tile_x) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(variant == INV_VARIANT_FUSED || variant == INV_VARIANT_SPLIT, INV_ERR_ARG,
+                    "unknown variant");
+        INV_REQUIRE(tile_x == 0 || tile_x == 8 || tile_x == 16 || tile_x == 32, INV_ERR_ARG,
+                    "tile_x must be 0, 8, 16 or 32");
+        sync(c);
+        c.variant = variant;
+        if (tile_x) c.tile_x = tile_x;
     });
 }
 

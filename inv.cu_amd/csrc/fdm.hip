@@ -13,28 +13,9 @@
 #include <algorithm>
 
 #include "internal.h"
+#include "stencil.h"
 
 namespace inv {
-
-// 4th-order staggered differences (solver/fdm.h:79,85,106,112):
-//   D-f = 9 (f[k] - f[k-s]) / (8h) - (f[k+s] - f[k-2s]) / (24h)
-//   D+f = 9 (f[k+s] - f[k]) / (8h) - (f[k+2s] - f[k-s]) / (24h)
-// c8 = 8h and c24 = 24h are formed in FP32 on the host, as the reference forms
-// them per use.
-__device__ __forceinline__ float d_minus(const float *__restrict__ f, long k, long s, float c8,
-                                         float c24) {
-    return 9.0f * (f[k] - f[k - s]) / c8 - (f[k + s] - f[k - 2 * s]) / c24;
-}
-__device__ __forceinline__ float d_plus(const float *__restrict__ f, long k, long s, float c8,
-                                        float c24) {
-    return 9.0f * (f[k + s] - f[k]) / c8 - (f[k + 2 * s] - f[k - s]) / c24;
-}
-
-struct Geo {
-    int nx, nz;
-    long N;
-    float dt, c8dx, c24dx, c8dz, c24dz;
-};
 
 struct VelArgs {
     Geo g;
@@ -299,7 +280,7 @@ __global__ void k_fill(float *__restrict__ p, float v, long n) {
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-static Geo make_geo(const Ctx &c) {
+Geo make_geo(const Ctx &c) {
     Geo g;
     g.nx = c.g.nx;
     g.nz = c.g.nz;
@@ -359,7 +340,7 @@ void build_point_list(Ctx &c, PointList &pl, int nsets, const std::vector<std::v
     pl.npoints = total;
 }
 
-static void fill(Ctx &c, void *p, long n, float v = 0.0f) {
+void fill_dev(Ctx &c, void *p, long n, float v) {
     if (n <= 0 || !p) return;
     k_fill<<<(unsigned)((n + 255) / 256), 256, 0, c.stream>>>(static_cast<float *>(p), v, n);
     INV_HIP(hipGetLastError());
@@ -368,11 +349,11 @@ static void fill(Ctx &c, void *p, long n, float v = 0.0f) {
 void fdm_zero_state(Ctx &c, int count) {
     const long n = (long)count * c.N;
     if (c.g.sh) {
-        fill(c, c.vy.p, n); fill(c, c.uy.p, n); fill(c, c.sxy.p, n); fill(c, c.szy.p, n);
+        fill_dev(c, c.vy.p, n); fill_dev(c, c.uy.p, n); fill_dev(c, c.sxy.p, n); fill_dev(c, c.szy.p, n);
     }
     if (c.g.psv) {
-        fill(c, c.vx.p, n); fill(c, c.vz.p, n); fill(c, c.ux.p, n); fill(c, c.uz.p, n);
-        fill(c, c.sxx.p, n); fill(c, c.szz.p, n); fill(c, c.sxz.p, n);
+        fill_dev(c, c.vx.p, n); fill_dev(c, c.vz.p, n); fill_dev(c, c.ux.p, n); fill_dev(c, c.uz.p, n);
+        fill_dev(c, c.sxx.p, n); fill_dev(c, c.szz.p, n); fill_dev(c, c.sxz.p, n);
     }
 }
 
@@ -517,6 +498,7 @@ void fdm_forward(Ctx &c, int first, int count, int flags) {
     va.snap_slot_stride = (long)c.nsfe * c.nsnapf * c.N;
     const bool store = (flags & INV_FWD_SNAPSHOTS) && c.g.adj && c.g.sfe > 0 && c.snap.p;
 
+    c.parity = 0;
     fdm_zero_state(c, count);
     int nsteps = c.g.nt;
     if (c.step_limit > 0 && c.step_limit < nsteps) nsteps = c.step_limit;
@@ -534,6 +516,46 @@ void fdm_forward(Ctx &c, int first, int count, int flags) {
 template <bool SH, bool PSV>
 static void launch_interact(Ctx &c, const IntArgs &ia, int count) {
     k_interact<SH, PSV><<<tile_grid(c, count), dim3(TILE_Z, TILE_X), 0, c.stream>>>(ia);
+}
+
+void fdm_interact(Ctx &c, int count, int it, const float *vx, const float *vz, const float *vy) {
+    IntArgs ia{};
+    ia.g = make_geo(c);
+    ia.vx = vx;
+    ia.vz = vz;
+    ia.vy = vy;
+    ia.ux = c.ux.as<float>();
+    ia.uz = c.uz.as<float>();
+    ia.uy = c.uy.as<float>();
+    ia.snap_slot_stride = (long)c.nsfe * c.nsnapf * c.N;
+    ia.snap = c.snap.as<float>() + (long)(it / c.g.sfe) * c.nsnapf * c.N;
+    ia.ndt = (float)c.g.sfe * c.g.dt;  // sfe * dt, fdm.h:585
+    ia.inv_lambda = c.g.inv_lambda;
+    ia.inv_mu = c.g.inv_mu;
+    ia.inv_rho = c.g.inv_rho;
+    if (c.g.batch == 1) {
+        // one-shot path: accumulate straight into the kernels, shot after
+        // shot, as Misfit::run does (misfit/misfit.h:29-44)
+        ia.kl = c.k_lambda.as<float>();
+        ia.km = c.k_mu.as<float>();
+        ia.kr = c.k_rho.as<float>();
+        ia.k_slot_stride = 0;
+    } else {
+        // batched path: one accumulator per slot, summed in slot order after
+        // the adjoint (fdm_kslot_reduce), so a shot's contribution does not
+        // depend on which shots share its launch
+        ia.kl = c.kslot.as<float>();
+        ia.km = c.kslot.as<float>() + c.N;
+        ia.kr = c.kslot.as<float>() + 2 * c.N;
+        ia.k_slot_stride = 3 * c.N;
+    }
+    hipEvent_t a;
+    const bool p = prof_begin(c, it / c.g.sfe, &a);
+    if (c.g.sh && c.g.psv) launch_interact<true, true>(c, ia, count);
+    else if (c.g.sh) launch_interact<true, false>(c, ia, count);
+    else launch_interact<false, true>(c, ia, count);
+    INV_HIP(hipGetLastError());
+    if (p) prof_end(c, PROF_INTERACT, a);
 }
 
 void fdm_adjoint(Ctx &c, int first, int count) {
@@ -555,45 +577,16 @@ void fdm_adjoint(Ctx &c, int first, int count) {
     va.ntiles = c.rec_list.ntiles;
     va.rec_ptr = nullptr;
     va.snap = nullptr;
-
-    IntArgs ia{};
-    ia.g = va.g;
-    ia.vx = c.vx.as<float>(); ia.vz = c.vz.as<float>();
-    ia.ux = c.ux.as<float>(); ia.uz = c.uz.as<float>();
-    ia.vy = c.vy.as<float>(); ia.uy = c.uy.as<float>();
-    ia.snap_slot_stride = (long)c.nsfe * c.nsnapf * c.N;
-    ia.ndt = (float)c.g.sfe * c.g.dt;
-    ia.inv_lambda = c.g.inv_lambda;
-    ia.inv_mu = c.g.inv_mu;
-    ia.inv_rho = c.g.inv_rho;
     const bool direct = c.g.batch == 1;
-    if (direct) {
-        ia.kl = c.k_lambda.as<float>();
-        ia.km = c.k_mu.as<float>();
-        ia.kr = c.k_rho.as<float>();
-        ia.k_slot_stride = 0;
-    } else {
-        ia.kl = c.kslot.as<float>();
-        ia.km = c.kslot.as<float>() + c.N;
-        ia.kr = c.kslot.as<float>() + 2 * c.N;
-        ia.k_slot_stride = 3 * c.N;
-        fill(c, c.kslot.p, 3L * count * c.N);
-    }
+    if (!direct) fill_dev(c, c.kslot.p, 3L * count * c.N);
 
+    c.parity = 0;
     fdm_zero_state(c, count);
     for (int it = 0; it < c.g.nt; it++) {
         va.it = it;
         step(c, va, sa, count, it);
-        if (it % c.g.sfe == 0) {
-            ia.snap = c.snap.as<float>() + (long)(it / c.g.sfe) * c.nsnapf * c.N;
-            hipEvent_t a;
-            const bool p = prof_begin(c, it / c.g.sfe, &a);
-            if (c.g.sh && c.g.psv) launch_interact<true, true>(c, ia, count);
-            else if (c.g.sh) launch_interact<true, false>(c, ia, count);
-            else launch_interact<false, true>(c, ia, count);
-            INV_HIP(hipGetLastError());
-            if (p) prof_end(c, PROF_INTERACT, a);
-        }
+        if (it % c.g.sfe == 0)
+            fdm_interact(c, count, it, c.vx.as<float>(), c.vz.as<float>(), c.vy.as<float>());
     }
     if (!direct) fdm_kslot_reduce(c, count);
 }

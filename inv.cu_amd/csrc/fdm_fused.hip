@@ -1,0 +1,472 @@
+// One kernel per time step: the whole velocity-stress step of FdmSolver
+// (divS -> addSTF -> updateV -> divV -> updateS -> saveRec, solver/fdm.h:
+// 534-551) for a tile of FZ x TX cells of one shot slot.
+//
+// Data flow of a tile (rows i0..i0+TX-1 along x, columns j0..j0+FZ-1 along z):
+//   1. the stress of step n is staged in LDS on the tile plus a 3-cell halo;
+//   2. the velocity of step n+1 is computed on the tile plus a halo of 1 cell
+//      before and 2 after in each direction (what the forward-staggered
+//      gradient of the tile needs), from that LDS stress, v^n, rho and the
+//      absorbing profile; halo cells are recomputed, not exchanged, and get
+//      the same bits as in their owner tile.  Tile cells also update u, record
+//      receivers and store snapshots;
+//   3. the stress of step n+1 is computed on the tile from LDS velocities.
+// Every wavefield is read and written once per step from HBM (halo re-reads
+// come from L2), 68 B per P-SV cell-update instead of the 88 B of a
+// velocity/stress kernel pair.  v and s are double-buffered because
+// neighbouring tiles still read step-n values of their halo.
+//
+// Tiles are ordered slot-major, then x, then z; the hardware deals workgroups
+// to the 8 XCDs round-robin, so the block index is remapped to give each XCD
+// one contiguous run of tiles and the halo rows of consecutive tiles hit the
+// same L2.
+#include <algorithm>
+
+#include "internal.h"
+
+namespace inv {
+
+constexpr int FZ = 64;        // tile width along z: one wavefront
+constexpr int FBLOCK = 256;   // four wavefronts per tile
+
+struct StepArgs {
+    Geo g;
+    int tiles_x, tiles_z;
+    const float *rho, *lam, *mu, *abx, *abzb, *abzt;
+    const float *vx0, *vz0, *sxx0, *szz0, *sxz0, *vy0, *sxy0, *szy0;  // step n
+    float *vx1, *vz1, *sxx1, *szz1, *sxz1, *vy1, *sxy1, *szy1;        // step n+1
+    float *ux, *uz, *uy;                                              // in place
+    // point injection (addSTF, fdm.h:163-180), bucketed by velocity region
+    const float *inj_x, *inj_y, *inj_z;
+    long inj_slot_stride;
+    const int *inj_ptr, *inj_cell, *inj_row;
+    int inj_per_slot;
+    int nt, it;
+    // receiver recording (saveRec, fdm.h:181-196), bucketed by tile
+    float *out_x, *out_y, *out_z;
+    long out_slot_stride;
+    const int *rec_ptr, *rec_cell, *rec_row;
+    int rec_disp;
+    // snapshot slot base (u before, v after the step, fdm.h:521-560) or null
+    float *snap;
+    long snap_slot_stride;
+};
+
+template <bool SH, bool PSV, int TX>
+__global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
+    constexpr int SW = FZ + 6, SR = TX + 6;  // stress tile: halo 3 on each side
+    constexpr int VW = FZ + 3, VR = TX + 3;  // velocity tile: halo 1 before, 2 after
+    __shared__ float l_sxx[PSV ? SR * SW : 1], l_szz[PSV ? SR * SW : 1], l_sxz[PSV ? SR * SW : 1];
+    __shared__ float l_sxy[SH ? SR * SW : 1], l_szy[SH ? SR * SW : 1];
+    __shared__ float l_vx[PSV ? VR * VW : 1], l_vz[PSV ? VR * VW : 1], l_vy[SH ? VR * VW : 1];
+
+    // XCD-aware tile order
+    const int nw = gridDim.x, w = blockIdx.x;
+    const int q = nw >> 3, rmd = nw & 7, xcd = w & 7;
+    const int L = xcd * q + (xcd < rmd ? xcd : rmd) + (w >> 3);
+    const int per = a.tiles_x * a.tiles_z;
+    const int b = L / per;
+    const int t = L - b * per;
+    const int tx = t / a.tiles_z, tz = t - tx * a.tiles_z;
+    const int i0 = tx * TX, j0 = tz * FZ;
+    const Geo &g = a.g;
+    const int nx = g.nx, nz = g.nz;
+    const long so = (long)b * g.N;
+
+    // 1. stress of step n on the tile plus halo
+    for (int e = threadIdx.x; e < SR * SW; e += FBLOCK) {
+        const int rr = e / SW, cc = e - rr * SW;
+        const int i = i0 - 3 + rr, j = j0 - 3 + cc;
+        const bool in = i >= 0 && i < nx && j >= 0 && j < nz;
+        const long k = so + (long)i * nz + j;
+        if (PSV) {
+            l_sxx[e] = in ? a.sxx0[k] : 0.0f;
+            l_szz[e] = in ? a.szz0[k] : 0.0f;
+            l_sxz[e] = in ? a.sxz0[k] : 0.0f;
+        }
+        if (SH) {
+            l_sxy[e] = in ? a.sxy0[k] : 0.0f;
+            l_szy[e] = in ? a.szy0[k] : 0.0f;
+        }
+    }
+    __syncthreads();
+
+    int p0 = 0, p1 = 0;
+    if (a.inj_ptr) {
+        const int set = a.inj_per_slot ? b : 0;
+        p0 = a.inj_ptr[set * per + t];
+        p1 = a.inj_ptr[set * per + t + 1];
+    }
+    int r0 = 0, r1 = 0;
+    if (a.rec_ptr) {
+        r0 = a.rec_ptr[t];
+        r1 = a.rec_ptr[t + 1];
+    }
+    const long it = a.it;
+
+    // 2. velocity of step n+1 on the tile plus halo (updateV*, fdm.h:150-161)
+    for (int e = threadIdx.x; e < VR * VW; e += FBLOCK) {
+        const int vr = e / VW, vc = e - vr * VW;
+        const int i = i0 - 1 + vr, j = j0 - 1 + vc;
+        if (i < 0 || i >= nx || j < 0 || j >= nz) {
+            if (PSV) {
+                l_vx[e] = 0.0f;
+                l_vz[e] = 0.0f;
+            }
+            if (SH) l_vy[e] = 0.0f;
+            continue;
+        }
+        const long kc = (long)i * nz + j;
+        const long o = so + kc;
+        const bool xin = i >= 2 && i < nx - 2;
+        const bool zin = j >= 2 && j < nz - 2;
+        const bool own = vr >= 1 && vr <= TX && vc >= 1 && vc <= FZ;
+        const int se = (vr + 2) * SW + (vc + 2);
+        const float ab = (a.abx[i] * a.abzb[j]) * a.abzt[j];
+        const float rho = a.rho[kc];
+        if (SH) {
+            float d = xin ? d_minus(l_sxy, se, SW, g.c8dx, g.c24dx) : 0.0f;
+            if (zin) d += d_minus(l_szy, se, 1, g.c8dz, g.c24dz);
+            for (int p = p0; p < p1; p++)
+                if (a.inj_cell[p] == kc)
+                    d += a.inj_y[(long)b * a.inj_slot_stride + (long)a.inj_row[p] * a.nt + it];
+            float v = a.vy0[o];
+            v = ab * (v + g.dt * d / rho);
+            l_vy[e] = v;
+            if (own) {
+                float u = a.uy[o];
+                if (a.snap) a.snap[(long)b * a.snap_slot_stride + kc] = u;
+                u += v * g.dt;
+                a.vy1[o] = v;
+                a.uy[o] = u;
+                if (a.snap) a.snap[(long)b * a.snap_slot_stride + g.N + kc] = v;
+                for (int p = r0; p < r1; p++)
+                    if (a.rec_cell[p] == kc)
+                        a.out_y[(long)b * a.out_slot_stride + (long)a.rec_row[p] * a.nt + it] =
+                            a.rec_disp ? u : v;
+            }
+        }
+        if (PSV) {
+            float dx_, dz_;
+            if (xin) {
+                dx_ = d_minus(l_sxx, se, SW, g.c8dx, g.c24dx);
+                dz_ = d_minus(l_sxz, se, SW, g.c8dx, g.c24dx);
+            } else {
+                dx_ = 0.0f;
+                dz_ = 0.0f;
+            }
+            if (zin) {
+                dx_ += d_minus(l_sxz, se, 1, g.c8dz, g.c24dz);
+                dz_ += d_minus(l_szz, se, 1, g.c8dz, g.c24dz);
+            }
+            for (int p = p0; p < p1; p++)
+                if (a.inj_cell[p] == kc) {
+                    const long ks = (long)b * a.inj_slot_stride + (long)a.inj_row[p] * a.nt + it;
+                    dx_ += a.inj_x[ks];
+                    dz_ += a.inj_z[ks];
+                }
+            float vx = a.vx0[o], vz = a.vz0[o];
+            vx = ab * (vx + g.dt * dx_ / rho);
+            vz = ab * (vz + g.dt * dz_ / rho);
+            l_vx[e] = vx;
+            l_vz[e] = vz;
+            if (own) {
+                float ux = a.ux[o], uz = a.uz[o];
+                const int f0 = SH ? 2 : 0;
+                float *sp = a.snap ? a.snap + (long)b * a.snap_slot_stride + (long)f0 * g.N + kc
+                                   : nullptr;
+                if (sp) {
+                    sp[0] = ux;
+                    sp[g.N] = uz;
+                }
+                ux += vx * g.dt;
+                uz += vz * g.dt;
+                a.vx1[o] = vx;
+                a.vz1[o] = vz;
+                a.ux[o] = ux;
+                a.uz[o] = uz;
+                if (sp) {
+                    sp[2 * g.N] = vx;
+                    sp[3 * g.N] = vz;
+                }
+                for (int p = r0; p < r1; p++)
+                    if (a.rec_cell[p] == kc) {
+                        const long kr =
+                            (long)b * a.out_slot_stride + (long)a.rec_row[p] * a.nt + it;
+                        a.out_x[kr] = a.rec_disp ? ux : vx;
+                        a.out_z[kr] = a.rec_disp ? uz : vz;
+                    }
+            }
+        }
+    }
+    __syncthreads();
+
+    // 3. stress of step n+1 on the tile (divV* + updateS*, fdm.h:103-149)
+    for (int e = threadIdx.x; e < TX * FZ; e += FBLOCK) {
+        const int tr = e / FZ, tc = e - tr * FZ;
+        const int i = i0 + tr, j = j0 + tc;
+        if (i >= nx || j >= nz) continue;
+        const long kc = (long)i * nz + j;
+        const long o = so + kc;
+        const bool xin = i >= 1 && i < nx - 2;
+        const bool zin = j >= 1 && j < nz - 2;
+        const int ve = (tr + 1) * VW + (tc + 1);
+        const int se = (tr + 3) * SW + (tc + 3);
+        const float mu = a.mu[kc];
+        if (SH) {
+            const float dx_ = xin ? d_plus(l_vy, ve, VW, g.c8dx, g.c24dx) : 0.0f;
+            const float dz_ = zin ? d_plus(l_vy, ve, 1, g.c8dz, g.c24dz) : 0.0f;
+            a.sxy1[o] = l_sxy[se] + g.dt * mu * dx_;
+            a.szy1[o] = l_szy[se] + g.dt * mu * dz_;
+        }
+        if (PSV) {
+            float xx, zx, xz, zz;
+            if (xin) {
+                xx = d_plus(l_vx, ve, VW, g.c8dx, g.c24dx);
+                zx = d_plus(l_vz, ve, VW, g.c8dx, g.c24dx);
+            } else {
+                xx = 0.0f;
+                zx = 0.0f;
+            }
+            if (zin) {
+                xz = d_plus(l_vx, ve, 1, g.c8dz, g.c24dz);
+                zz = d_plus(l_vz, ve, 1, g.c8dz, g.c24dz);
+            } else {
+                xz = 0.0f;
+                zz = 0.0f;
+            }
+            const float lam = a.lam[kc];
+            a.sxx1[o] = l_sxx[se] + g.dt * ((lam + 2 * mu) * xx + lam * zz);
+            a.szz1[o] = l_szz[se] + g.dt * ((lam + 2 * mu) * zz + lam * xx);
+            a.sxz1[o] = l_sxz[se] + g.dt * (mu * (xz + zx));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static int tiles_x_of(const Ctx &c, int TX) { return (c.g.nx + TX - 1) / TX; }
+static int tiles_z_of(const Ctx &c) { return (c.g.nz + FZ - 1) / FZ; }
+
+// CSR of points by tile: a point (i, j) belongs to tile (tx, tz) when
+// tx*TX - lo <= i < (tx+1)*TX + hi and tz*FZ - lo <= j < (tz+1)*FZ + hi.
+static void build_tiled(Ctx &c, PointList &pl, int nsets, const std::vector<std::vector<int>> &cells,
+                        const std::vector<std::vector<int>> &rows, int TX, int lo, int hi) {
+    const int ntx = tiles_x_of(c, TX), ntz = tiles_z_of(c);
+    const int ntiles = ntx * ntz;
+    auto tiles_of = [&](int cell, std::vector<int> &out) {
+        out.clear();
+        const int i = cell / c.g.nz, j = cell % c.g.nz;
+        for (int tx = std::max(0, (i - hi) / TX - 1); tx <= std::min(ntx - 1, (i + lo) / TX); tx++) {
+            if (!(tx * TX - lo <= i && i < (tx + 1) * TX + hi)) continue;
+            for (int tz = std::max(0, (j - hi) / FZ - 1); tz <= std::min(ntz - 1, (j + lo) / FZ);
+                 tz++) {
+                if (!(tz * FZ - lo <= j && j < (tz + 1) * FZ + hi)) continue;
+                out.push_back(tx * ntz + tz);
+            }
+        }
+    };
+    std::vector<int> count((size_t)nsets * ntiles + 1, 0), tl;
+    long total = 0;
+    for (int s = 0; s < nsets; s++)
+        for (int cell : cells[s]) {
+            tiles_of(cell, tl);
+            for (int t : tl) count[(size_t)s * ntiles + t + 1]++;
+            total += (long)tl.size();
+        }
+    for (size_t t = 1; t < count.size(); t++) count[t] += count[t - 1];
+    std::vector<int> fillp(count.begin(), count.end() - 1);
+    std::vector<int> hc(std::max(total, 1L)), hr(std::max(total, 1L));
+    for (int s = 0; s < nsets; s++)
+        for (size_t p = 0; p < cells[s].size(); p++) {
+            tiles_of(cells[s][p], tl);
+            for (int t : tl) {
+                const int slot = fillp[(size_t)s * ntiles + t]++;
+                hc[slot] = cells[s][p];
+                hr[slot] = rows[s][p];
+            }
+        }
+    pl.ptr.ensure(count.size() * sizeof(int));
+    pl.cell.ensure(hc.size() * sizeof(int));
+    pl.row.ensure(hr.size() * sizeof(int));
+    INV_HIP(hipMemcpy(pl.ptr.p, count.data(), count.size() * sizeof(int), hipMemcpyHostToDevice));
+    INV_HIP(hipMemcpy(pl.cell.p, hc.data(), hc.size() * sizeof(int), hipMemcpyHostToDevice));
+    INV_HIP(hipMemcpy(pl.row.p, hr.data(), hr.size() * sizeof(int), hipMemcpyHostToDevice));
+    pl.nsets = nsets;
+    pl.ntiles = ntiles;
+    pl.npoints = (int)total;
+}
+
+void fused_invalidate_lists(Ctx &c) {
+    c.f_src_first = -2;
+    c.f_src_tx = -1;
+    c.f_rec_tx = -1;
+}
+
+static void ensure_rec_lists(Ctx &c) {
+    if (c.f_rec_tx == c.tile_x) return;
+    std::vector<std::vector<int>> cells(1), rows(1);
+    for (int r = 0; r < c.nrec; r++) {
+        cells[0].push_back(c.rec_ix[r] * c.g.nz + c.rec_iz[r]);
+        rows[0].push_back(r);
+    }
+    build_tiled(c, c.f_rec_v, 1, cells, rows, c.tile_x, 1, 2);
+    build_tiled(c, c.f_rec_t, 1, cells, rows, c.tile_x, 0, 0);
+    c.f_rec_tx = c.tile_x;
+}
+
+static void ensure_src_list(Ctx &c, int first, int count) {
+    if (c.f_src_first == first && c.f_src_count == count && c.f_src_tx == c.tile_x) return;
+    std::vector<std::vector<int>> cells(count), rows(count);
+    for (int b = 0; b < count; b++)
+        for (int is = 0; is < c.nsrc; is++)
+            if (first < 0 || is == first + b) {
+                cells[b].push_back(c.src_ix[is] * c.g.nz + c.src_iz[is]);
+                rows[b].push_back(is);
+            }
+    build_tiled(c, c.f_src_v, count, cells, rows, c.tile_x, 1, 2);
+    c.f_src_first = first;
+    c.f_src_count = count;
+    c.f_src_tx = c.tile_x;
+}
+
+static void ensure_second_set(Ctx &c) {
+    const size_t b = (size_t)c.g.batch * c.N * sizeof(float);
+    if (c.g.psv) {
+        c.vx_b.ensure(b); c.vz_b.ensure(b);
+        c.sxx_b.ensure(b); c.szz_b.ensure(b); c.sxz_b.ensure(b);
+    }
+    if (c.g.sh) {
+        c.vy_b.ensure(b); c.sxy_b.ensure(b); c.szy_b.ensure(b);
+    }
+}
+
+static void set_state_ptrs(Ctx &c, StepArgs &a, int par) {
+    auto pick = [&](DevBuf &x, DevBuf &y, int p) { return (p & 1) ? y.as<float>() : x.as<float>(); };
+    a.vx0 = pick(c.vx, c.vx_b, par); a.vx1 = pick(c.vx, c.vx_b, par + 1);
+    a.vz0 = pick(c.vz, c.vz_b, par); a.vz1 = pick(c.vz, c.vz_b, par + 1);
+    a.sxx0 = pick(c.sxx, c.sxx_b, par); a.sxx1 = pick(c.sxx, c.sxx_b, par + 1);
+    a.szz0 = pick(c.szz, c.szz_b, par); a.szz1 = pick(c.szz, c.szz_b, par + 1);
+    a.sxz0 = pick(c.sxz, c.sxz_b, par); a.sxz1 = pick(c.sxz, c.sxz_b, par + 1);
+    a.vy0 = pick(c.vy, c.vy_b, par); a.vy1 = pick(c.vy, c.vy_b, par + 1);
+    a.sxy0 = pick(c.sxy, c.sxy_b, par); a.sxy1 = pick(c.sxy, c.sxy_b, par + 1);
+    a.szy0 = pick(c.szy, c.szy_b, par); a.szy1 = pick(c.szy, c.szy_b, par + 1);
+}
+
+template <bool SH, bool PSV>
+static void launch_tx(Ctx &c, const StepArgs &a, unsigned blocks) {
+    switch (c.tile_x) {
+        case 8: k_step<SH, PSV, 8><<<blocks, FBLOCK, 0, c.stream>>>(a); break;
+        case 32: k_step<SH, PSV, 32><<<blocks, FBLOCK, 0, c.stream>>>(a); break;
+        default: k_step<SH, PSV, 16><<<blocks, FBLOCK, 0, c.stream>>>(a); break;
+    }
+}
+
+static void fused_step(Ctx &c, StepArgs &a, int count, int it) {
+    set_state_ptrs(c, a, c.parity);
+    a.it = it;
+    const unsigned blocks = (unsigned)(count * a.tiles_x * a.tiles_z);
+    hipEvent_t ev;
+    const bool p = prof_begin(c, it, &ev);
+    if (c.g.sh && c.g.psv) launch_tx<true, true>(c, a, blocks);
+    else if (c.g.sh) launch_tx<true, false>(c, a, blocks);
+    else launch_tx<false, true>(c, a, blocks);
+    INV_HIP(hipGetLastError());
+    if (p) prof_end(c, PROF_STEP, ev);
+    c.parity ^= 1;
+}
+
+static void common_args(Ctx &c, StepArgs &a) {
+    a = StepArgs{};
+    a.g = make_geo(c);
+    a.tiles_x = tiles_x_of(c, c.tile_x);
+    a.tiles_z = tiles_z_of(c);
+    a.rho = c.rho.as<float>();
+    a.lam = c.lam.as<float>();
+    a.mu = c.mu.as<float>();
+    a.abx = c.abx.as<float>();
+    a.abzb = c.abzb.as<float>();
+    a.abzt = c.abzt.as<float>();
+    a.ux = c.ux.as<float>();
+    a.uz = c.uz.as<float>();
+    a.uy = c.uy.as<float>();
+    a.nt = c.g.nt;
+}
+
+void fused_forward(Ctx &c, int first, int count, int flags) {
+    ensure_second_set(c);
+    ensure_rec_lists(c);
+    ensure_src_list(c, first, count);
+    StepArgs a;
+    common_args(c, a);
+    const long nt = c.g.nt, nsrc = c.nsrc;
+    a.inj_x = c.stf.as<float>();
+    a.inj_y = c.stf.as<float>() + nsrc * nt;
+    a.inj_z = c.stf.as<float>() + 2 * nsrc * nt;
+    a.inj_slot_stride = 0;
+    a.inj_ptr = c.f_src_v.ptr.as<int>();
+    a.inj_cell = c.f_src_v.cell.as<int>();
+    a.inj_row = c.f_src_v.row.as<int>();
+    a.inj_per_slot = 1;
+    const long tr = (long)c.nrec * nt;
+    const int cy = 0, cx = c.g.sh ? 1 : 0, cz = cx + 1;
+    a.out_y = c.out.as<float>() + cy * tr;
+    a.out_x = c.out.as<float>() + cx * tr;
+    a.out_z = c.out.as<float>() + cz * tr;
+    a.out_slot_stride = (long)c.ncomp * tr;
+    a.rec_ptr = c.nrec ? c.f_rec_t.ptr.as<int>() : nullptr;
+    a.rec_cell = c.f_rec_t.cell.as<int>();
+    a.rec_row = c.f_rec_t.row.as<int>();
+    a.rec_disp = (c.g.adj || c.g.trace_type == 1) ? 1 : 0;
+    a.snap_slot_stride = (long)c.nsfe * c.nsnapf * c.N;
+    const bool store = (flags & INV_FWD_SNAPSHOTS) && c.g.adj && c.g.sfe > 0 && c.snap.p;
+
+    c.parity = 0;
+    fdm_zero_state(c, count);
+    int nsteps = c.g.nt;
+    if (c.step_limit > 0 && c.step_limit < nsteps) nsteps = c.step_limit;
+    for (int it = 0; it < nsteps; it++) {
+        a.snap = nullptr;
+        if (store && (it + 1) % c.g.sfe == 0) {
+            const long isfe = c.nsfe - (it + 1) / c.g.sfe;
+            a.snap = c.snap.as<float>() + isfe * c.nsnapf * c.N;
+        }
+        fused_step(c, a, count, it);
+    }
+}
+
+void fused_adjoint(Ctx &c, int first, int count) {
+    (void)first;
+    ensure_second_set(c);
+    ensure_rec_lists(c);
+    StepArgs a;
+    common_args(c, a);
+    const long nt = c.g.nt;
+    const long tr = (long)c.nrec * nt;
+    const int cy = 0, cx = c.g.sh ? 1 : 0, cz = cx + 1;
+    a.inj_y = c.adstf.as<float>() + cy * tr;
+    a.inj_x = c.adstf.as<float>() + cx * tr;
+    a.inj_z = c.adstf.as<float>() + cz * tr;
+    a.inj_slot_stride = (long)c.ncomp * tr;
+    a.inj_ptr = c.nrec ? c.f_rec_v.ptr.as<int>() : nullptr;
+    a.inj_cell = c.f_rec_v.cell.as<int>();
+    a.inj_row = c.f_rec_v.row.as<int>();
+    a.inj_per_slot = 0;
+    a.rec_ptr = nullptr;
+    a.snap = nullptr;
+    const bool direct = c.g.batch == 1;
+    if (!direct) fill_dev(c, c.kslot.p, 3L * count * c.N);
+
+    c.parity = 0;
+    fdm_zero_state(c, count);
+    for (int it = 0; it < c.g.nt; it++) {
+        fused_step(c, a, count, it);
+        if (it % c.g.sfe == 0)
+            fdm_interact(c, count, it, cur_field(c, c.vx, c.vx_b), cur_field(c, c.vz, c.vz_b),
+                         cur_field(c, c.vy, c.vy_b));
+    }
+    if (!direct) fdm_kslot_reduce(c, count);
+}
+
+}  // namespace inv

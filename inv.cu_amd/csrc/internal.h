@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/inv_hip.h"
+#include "stencil.h"
 
 namespace inv {
 
@@ -55,7 +56,7 @@ constexpr int TILE_Z = 64;  // one wavefront along z (the contiguous axis)
 constexpr int TILE_X = 4;   // wavefronts per block along x
 
 // Sampled kernel timing with HIP events on the context stream.
-enum ProfKind { PROF_VELOCITY = 0, PROF_STRESS = 1, PROF_INTERACT = 2, PROF_NKIND = 3 };
+enum ProfKind { PROF_VELOCITY = 0, PROF_STRESS = 1, PROF_INTERACT = 2, PROF_STEP = 3, PROF_NKIND = 4 };
 struct Prof {
     int every = 0;  // sample every n-th time step (0 = off)
     struct Rec {
@@ -89,8 +90,15 @@ struct Ctx {
     DevBuf k_lambda, k_mu, k_rho;
     DevBuf kslot;  // per-slot kernel accumulators [batch][3][N] (batched adjoint)
 
-    // per-slot wavefields [batch][N]
+    // per-slot wavefields [batch][N].  The fused step kernel reads v and s at
+    // step n from one buffer set and writes step n+1 to the other (a
+    // neighbouring tile still reads the old values of its halo); `parity`
+    // names the set holding the current state.  u is updated in place.
     DevBuf vx, vz, ux, uz, sxx, szz, sxz, vy, uy, sxy, szy;
+    DevBuf vx_b, vz_b, sxx_b, szz_b, sxz_b, vy_b, sxy_b, szy_b;
+    int parity = 0;
+    int variant = 0;   // 0 fused step kernel, 1 velocity/stress kernel pair
+    int tile_x = 16;   // rows per fused tile (8, 16 or 32)
     DevBuf snap;   // [batch][nsfe][nsnapf][N]
     DevBuf out;    // [batch][ncomp][nrec][nt]
     DevBuf adstf;  // [batch][ncomp][nrec][nt]
@@ -107,6 +115,10 @@ struct Ctx {
     PointList rec_list;  // receivers, one set shared by all slots
     PointList src_list;  // sources of the current forward, one set per slot
     int src_list_first = -2, src_list_count = -1;
+    // the same sets bucketed by fused tile: injection points by the tile's
+    // velocity region (tile plus halo), recording points by the tile itself
+    PointList f_src_v, f_rec_v, f_rec_t;
+    int f_src_first = -2, f_src_count = -1, f_src_tx = -1, f_rec_tx = -1;
 
     // filter
     DevBuf gauss_w, gauss_tmp, gauss_sx, gauss_sz;
@@ -125,6 +137,18 @@ void fdm_forward(Ctx &c, int first, int count, int flags);
 void fdm_adjoint(Ctx &c, int first, int count);
 void fdm_zero_state(Ctx &c, int count);
 void fdm_kslot_reduce(Ctx &c, int count);
+Geo make_geo(const Ctx &c);
+void fill_dev(Ctx &c, void *p, long n, float v = 0.0f);
+// interaction kernels for adjoint step it (it % sfe == 0) with the adjoint
+// velocities vx/vz/vy of the current state set
+void fdm_interact(Ctx &c, int count, int it, const float *vx, const float *vz, const float *vy);
+void fused_forward(Ctx &c, int first, int count, int flags);
+void fused_adjoint(Ctx &c, int first, int count);
+void fused_invalidate_lists(Ctx &c);
+// device pointer of the current buffer of a ping-pong field
+inline float *cur_field(Ctx &c, DevBuf &a, DevBuf &b) {
+    return (c.parity & 1) ? b.as<float>() : a.as<float>();
+}
 bool prof_begin(Ctx &c, int it, hipEvent_t *a);
 void prof_end(Ctx &c, int kind, hipEvent_t a);
 void prof_resolve(Ctx &c);

@@ -18,7 +18,8 @@ INV_OK = 0
 INV_LAMBDA, INV_MU, INV_RHO, INV_K_LAMBDA, INV_K_MU, INV_K_RHO, INV_OUT, INV_ADSTF, INV_OBS = range(9)
 INV_SNAP_F32, INV_SNAP_F16, INV_SNAP_BF16 = 0, 1, 2
 INV_FWD_SNAPSHOTS = 1
-INV_PROF_VELOCITY, INV_PROF_STRESS, INV_PROF_INTERACT = 0, 1, 2
+INV_PROF_VELOCITY, INV_PROF_STRESS, INV_PROF_INTERACT, INV_PROF_STEP = 0, 1, 2, 3
+INV_VARIANT_FUSED, INV_VARIANT_SPLIT = 0, 1
 
 STATUS = {0: "INV_OK", -1: "INV_ERR_ARG", -2: "INV_ERR_HIP", -3: "INV_ERR_STATE",
           -4: "INV_ERR_NOMEM", -5: "INV_ERR_UNSUPPORTED"}
@@ -94,6 +95,7 @@ SIGNATURES = {
     "inv_prof_enable": [_vp, _i],
     "inv_prof_reset": [_vp],
     "inv_prof_read": [_vp, _i, _vp, _vp],
+    "inv_set_variant": [_vp, _i, _i],
 }
 
 _lib = None
@@ -360,6 +362,9 @@ class Context:
 
     def set_step_limit(self, n):
         self._chk(self.L.inv_set_step_limit(self.h, n))
+
+    def set_variant(self, variant, tile_x=0):
+        self._chk(self.L.inv_set_variant(self.h, variant, tile_x))
 
     # ---- measurement ----
     def prof_enable(self, every):

@@ -12,17 +12,37 @@ from conftest import inv, oracle_problem, oracle_traces, rel
 pytestmark = pytest.mark.gpu
 
 CASES = [dict(sh=1, psv=0), dict(sh=0, psv=1), dict(sh=1, psv=1)]
+# (variant, tile_x): fused step kernel with each tile height, and the
+# velocity/stress kernel pair
+VARIANTS = [(0, 8), (0, 16), (0, 32), (1, 0)]
+
+
+def _vid(v):
+    return "split" if v[0] == 1 else f"fused{v[1]}"
+
+
+def boundary_problem(**kw):
+    """A grid with several tiles in both directions and sources/receivers on
+    tile edges (rows 15/16/31/32, columns 63/64/65/127/128)."""
+    P = inv.problems.tiny(nx=70, nz=150, nt=160, nsrc=4, nrec=10, **kw)
+    src_ix = np.array([15, 16, 31, 32], np.int32)
+    src_iz = np.array([63, 64, 65, 128], np.int32)
+    rec_ix = np.array([8, 15, 16, 17, 31, 32, 33, 47, 48, 60], np.int32)
+    rec_iz = np.array([63, 64, 65, 127, 128, 3, 64, 63, 64, 146], np.int32)
+    return P.with_(src_ix=src_ix, src_iz=src_iz, rec_ix=rec_ix, rec_iz=rec_iz)
 
 
 def _ids(c):
     return "sh" if c["sh"] and not c["psv"] else ("psv" if not c["sh"] else "sh+psv")
 
 
+@pytest.mark.parametrize("variant", VARIANTS, ids=_vid)
 @pytest.mark.parametrize("case", CASES, ids=_ids)
-def test_forward_plain_matches_oracle(gpu, case):
+def test_forward_plain_matches_oracle(gpu, case, variant):
     P = inv.problems.tiny(**case)
-    print(f"forward plain {_ids(case)} {P.nx}x{P.nz} nt={P.nt}", flush=True)
+    print(f"forward plain {_ids(case)} {_vid(variant)} {P.nx}x{P.nz} nt={P.nt}", flush=True)
     ctx = P.context(batch=1)
+    ctx.set_variant(*variant)
     O = oracle_problem(P)
     for isrc in range(P.nsrc):
         ctx.forward(isrc, 1, snapshots=True)
@@ -62,12 +82,33 @@ def test_displacement_traces(gpu):
     ctx.close()
 
 
+@pytest.mark.parametrize("variant", VARIANTS, ids=_vid)
 @pytest.mark.parametrize("case", CASES, ids=_ids)
-def test_batched_forward_equals_plain(gpu, case):
+def test_tile_edges_match_oracle(gpu, case, variant):
+    """Sources and receivers on tile edges and in tile halos, several tiles."""
+    P = boundary_problem(**case)
+    print(f"tile edges {_ids(case)} {_vid(variant)}", flush=True)
+    ctx = P.context(batch=2)
+    ctx.set_variant(*variant)
+    O = oracle_problem(P)
+    ctx.forward(1, 2, snapshots=True)
+    for s in range(2):
+        otr, osn, ost = O.forward(1 + s, snapshots=True)
+        assert np.array_equal(ctx.traces_get(s), oracle_traces(otr, P))
+        assert np.array_equal(ctx.state_get(s), ost)
+        assert np.array_equal(ctx.snapshots_get(s), osn)
+    ctx.close()
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=_vid)
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_batched_forward_equals_plain(gpu, case, variant):
     """Each slot of a batched launch gives the same bits as the one-shot path."""
     P = inv.problems.tiny(nsrc=4, **case)
     plain = P.context(batch=1)
     batch = P.context(batch=4)
+    plain.set_variant(*variant)
+    batch.set_variant(*variant)
     batch.forward(0, 4, snapshots=True)
     for s in range(4):
         plain.forward(s, 1, snapshots=True)
@@ -78,10 +119,12 @@ def test_batched_forward_equals_plain(gpu, case):
     batch.close()
 
 
+@pytest.mark.parametrize("variant", VARIANTS, ids=_vid)
 @pytest.mark.parametrize("case", CASES, ids=_ids)
-def test_adjoint_plain_matches_oracle(gpu, case):
-    P = inv.problems.tiny(**case)
+def test_adjoint_plain_matches_oracle(gpu, case, variant):
+    P = boundary_problem(**case)
     ctx = P.context(batch=1)
+    ctx.set_variant(*variant)
     O = oracle_problem(P)
     ctx.kernels_zero()
     ks = [np.zeros(P.N, np.float32) for _ in range(3)]

@@ -18,13 +18,15 @@ def _obs_from_true(P):
     return np.stack(obs).astype(np.float32)
 
 
+@pytest.mark.parametrize("variant", [0, 1], ids=["fused", "split"])
 @pytest.mark.parametrize("batch", [1, 3])
 @pytest.mark.parametrize("sigma", [0, 3])
-def test_gradient_matches_oracle(gpu, batch, sigma):
+def test_gradient_matches_oracle(gpu, batch, sigma, variant):
     P = inv.problems.tiny(sh=1, psv=1, nsrc=3)
     obs = _obs_from_true(P)
-    print(f"gradient batch={batch} sigma={sigma}", flush=True)
+    print(f"gradient batch={batch} sigma={sigma} variant={variant}", flush=True)
     ctx = P.context(batch=batch)
+    ctx.set_variant(variant)
     for s in range(P.nsrc):
         ctx.obs_set(s, obs[s])
     f = ctx.gradient(misfit_type=0, sigma=sigma)
