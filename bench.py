@@ -49,7 +49,10 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the one-shot GPU path")
     ap.add_argument("--cpu-shots", type=int, default=1)
     ap.add_argument("--variant", type=int, default=0, help="0 fused step kernel, 1 kernel pair")
-    ap.add_argument("--tile-x", type=int, default=16, help="fused tile rows (8, 16, 32)")
+    ap.add_argument("--tile-x", type=int, default=8, help="fused tile rows (8, 16, 32)")
+    ap.add_argument("--math", type=int, default=1, help="0 exact (IEEE divisions), 1 fast")
+    ap.add_argument("--copy-calib-mb", type=int, default=0,
+                    help="also run one plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
 
 
@@ -93,6 +96,7 @@ def main():
           file=sys.stderr, flush=True)
     ctx = P.context(batch=B, model="true")
     ctx.set_variant(a.variant, a.tile_x)
+    ctx.set_math(a.math)
     # observed data: forward on the true model (Misfit::init, misfit.h:135-148)
     for first in range(0, P.nsrc, B):
         n = min(B, P.nsrc - first)
@@ -102,6 +106,14 @@ def main():
     ctx.sync()
     print("# observed data ready", file=sys.stderr, flush=True)
 
+    if a.copy_calib_mb:
+        nb = a.copy_calib_mb << 20
+        src, dst = ctx.alloc(nb), ctx.alloc(nb)
+        ctx.vec_fill(src, 1.0, nb // 4)
+        ctx.vec_copy(dst, src, nb // 4)
+        ctx.sync()
+        ctx.free(src)
+        ctx.free(dst)
     for _ in range(a.warmup):
         ctx.gradient(P.misfit, P.sigma)
     ctx.sync()
@@ -127,7 +139,7 @@ def main():
     cells = B * P.N
     if nf:
         step_s = tf / nf / 1e3
-        kernel = "k_step<false,true,TX> (fused P-SV time step)"
+        kernel = "k_col<false,true,TX,FAST> (fused P-SV time step)"
         avg = {"step": round(tf / nf * 1e3, 2)}
         samples = {"step": nf}
     elif nv and ns:
@@ -148,6 +160,7 @@ def main():
     if not a.no_plain:
         pctx = P.context(batch=1, model="true")
         pctx.set_variant(a.variant, a.tile_x)
+        pctx.set_math(a.math)
         for s in range(P.nsrc):
             pctx.forward(s, 1, snapshots=False)
             pctx.obs_from_traces(s, 1)

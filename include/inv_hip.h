@@ -128,6 +128,14 @@ int inv_forward(inv_ctx *ctx, int first, int count, int flags);
 /* FdmSolver::runAdjoint (solver/fdm.h:571-612) for the same slots, driven
  * by their ADSTF buffers; kernels accumulate into K_LAMBDA/K_MU/K_RHO. */
 int inv_adjoint(inv_ctx *ctx, int first, int count);
+/* Incremental runs: begin a forward (or adjoint) of the given slots, then
+ * advance it nsteps at a time; *position returns the steps done.  Between
+ * advances the wavefields can be read (inv_state_get), which is how the
+ * wavefield exports every wfe / wae steps of exportSnapshot
+ * (solver/fdm.h:312-347) are served. */
+int inv_forward_begin(inv_ctx *ctx, int first, int count, int flags);
+int inv_adjoint_begin(inv_ctx *ctx, int first, int count);
+int inv_run_advance(inv_ctx *ctx, int nsteps, int *position);
 /* initKernels (solver/fdm.h:495-499) */
 int inv_kernels_zero(inv_ctx *ctx);
 int inv_kernels_get(inv_ctx *ctx, float *k_lambda, float *k_mu, float *k_rho);
@@ -150,6 +158,11 @@ int inv_obs_from_traces(inv_ctx *ctx, int first, int count);
  * chi[count][nrec][ncomp] (receiver-major as Misfit::run sums them,
  * misfit/misfit.h:31-40).  *total (FP64) = their sum. */
 int inv_misfit(inv_ctx *ctx, int first, int count, int type, double *chi, double *total);
+
+/* One trace on device pointers (the per-trace Misfit::run(syn, obs, adstf) of
+ * misfit/misfit.h:11): writes adstf [nt], returns the trace misfit. */
+int inv_misfit_trace(inv_ctx *ctx, int type, const float *syn, const float *obs, float *adstf,
+                     double *chi);
 
 /* ---- whole gradient: Misfit::run(kernel) (misfit/misfit.h:15-51) ---------
  * Forward every shot in batches of `batch`, misfit against resident obs,
@@ -204,11 +217,20 @@ int inv_prof_reset(inv_ctx *ctx);
 int inv_prof_read(inv_ctx *ctx, int kernel, long long *count, double *total_ms);
 
 /* Propagation kernel variant: INV_VARIANT_FUSED (default) runs one kernel per
- * time step on tiles of 64 (z) x tile_x (x) cells, tile_x in {8, 16, 32}
- * (0 keeps the current value); INV_VARIANT_SPLIT runs a velocity and a stress
- * kernel per step.  Results are bit-identical between variants. */
-enum inv_variant { INV_VARIANT_FUSED = 0, INV_VARIANT_SPLIT = 1 };
+ * time step on tiles of 58 (z) x tile_x (x) cells, one lane per column;
+ * INV_VARIANT_FUSED_LINEAR the same step on 64 x tile_x tiles with a linear
+ * cell index; tile_x in {8, 16, 32} (0 keeps the current value).
+ * INV_VARIANT_SPLIT runs a velocity and a stress kernel per step.  Results
+ * are identical between variants. */
+enum inv_variant { INV_VARIANT_FUSED = 0, INV_VARIANT_SPLIT = 1, INV_VARIANT_FUSED_LINEAR = 2 };
 int inv_set_variant(inv_ctx *ctx, int variant, int tile_x);
+/* Stencil arithmetic: INV_MATH_EXACT evaluates the reference's expressions
+ * (solver/fdm.h:76-161) with IEEE FP32 divisions, bitwise equal to them;
+ * INV_MATH_FAST (default) multiplies by FP32-rounded coefficients 9/(8h),
+ * 1/(24h) and dt/rho, the same FP32 accuracy against FP64 at a fraction of
+ * the instructions. */
+enum inv_math { INV_MATH_EXACT = 0, INV_MATH_FAST = 1 };
+int inv_set_math(inv_ctx *ctx, int mode);
 
 /* Run only the first nsteps steps of the next forward (0 = all). */
 int inv_set_step_limit(inv_ctx *ctx, int nsteps);

@@ -18,6 +18,8 @@ void DevBuf::release() {
 
 void DevBuf::ensure(size_t nbytes) {
     if (nbytes <= bytes && p) return;
+    // queued kernels may still use the old buffer
+    if (p) (void)hipDeviceSynchronize();
     release();
     if (nbytes == 0) nbytes = 16;
     hipError_t e = hipMalloc(&p, nbytes);
@@ -91,6 +93,9 @@ static void alloc_slots(Ctx &c) {
     c.lam.ensure(N * f);
     c.mu.ensure(N * f);
     c.rho.ensure(N * f);
+    // dt / rho of the fast math mode: allocated here, with the model, so that
+    // launch arguments built before a run never see a null pointer
+    c.bdt.ensure(N * f);
     c.k_lambda.ensure(N * f);
     c.k_mu.ensure(N * f);
     c.k_rho.ensure(N * f);
@@ -259,7 +264,7 @@ int inv_ctx_destroy(inv_ctx *ctx) {
                       &c.sxx_b, &c.szz_b, &c.sxz_b, &c.vy_b, &c.sxy_b, &c.szy_b,
                       &c.f_src_v.ptr, &c.f_src_v.cell, &c.f_src_v.row, &c.f_rec_v.ptr,
                       &c.f_rec_v.cell, &c.f_rec_v.row, &c.f_rec_t.ptr, &c.f_rec_t.cell,
-                      &c.f_rec_t.row};
+                      &c.f_rec_t.row, &c.env_w, &c.env_fac, &c.env_scratch, &c.bdt};
     for (DevBuf *b : bufs) b->release();
     for (void *p : c.user_allocs) (void)hipFree(p);
     prof_release(c);
@@ -319,6 +324,9 @@ int inv_model_upload_lmr(inv_ctx *ctx, const float *lambda, const float *mu, con
     return guard(ctx, [&](Ctx &c) {
         INV_REQUIRE(c.have_grid, INV_ERR_STATE, "grid not set");
         INV_REQUIRE(lambda && mu && rho, INV_ERR_ARG, "null model array");
+        // hipMemcpy does not order against the non-blocking context stream:
+        // wait for kernels that may still read the model
+        sync(c);
         const size_t b = c.N * sizeof(float);
         INV_HIP(hipMemcpy(c.lam.p, lambda, b, hipMemcpyHostToDevice));
         INV_HIP(hipMemcpy(c.mu.p, mu, b, hipMemcpyHostToDevice));
@@ -387,6 +395,7 @@ int inv_model_from_points(inv_ctx *ctx, int npt, const float *px, const float *p
                     }
                 }
         }
+        sync(c);
         if (model_type == 0) {
             // vps2lm, solver/fdm.h:25-36
             for (long k = 0; k < c.N; k++) {
@@ -462,12 +471,27 @@ int inv_receivers_set(inv_ctx *ctx, int nrec, const int *ix, const int *iz) {
     });
 }
 
+static void run_steps(Ctx &c, int kind, int first, int count, int flags, int it0, int it1) {
+    const bool split = c.variant == INV_VARIANT_SPLIT;
+    if (kind == 1) {
+        if (split) fdm_forward(c, first, count, flags, it0, it1);
+        else fused_forward(c, first, count, flags, it0, it1);
+    } else {
+        if (split) fdm_adjoint(c, first, count, it0, it1);
+        else fused_adjoint(c, first, count, it0, it1);
+    }
+}
+
+static int forward_end(const Ctx &c) {
+    return (c.step_limit > 0 && c.step_limit < c.g.nt) ? c.step_limit : c.g.nt;
+}
+
 int inv_forward(inv_ctx *ctx, int first, int count, int flags) {
     return guard(ctx, [&](Ctx &c) {
         require_ready(c);
         check_slots(c, first, count);
-        if (c.variant == INV_VARIANT_SPLIT) fdm_forward(c, first, count, flags);
-        else fused_forward(c, first, count, flags);
+        run_steps(c, 1, first, count, flags, 0, forward_end(c));
+        c.run_kind = 0;
     });
 }
 
@@ -476,8 +500,47 @@ int inv_adjoint(inv_ctx *ctx, int first, int count) {
         require_ready(c);
         check_slots(c, first < 0 ? 0 : first, count);
         INV_REQUIRE(c.g.adj && c.snap.p, INV_ERR_STATE, "adjoint needs adj = 1 (snapshots)");
-        if (c.variant == INV_VARIANT_SPLIT) fdm_adjoint(c, first, count);
-        else fused_adjoint(c, first, count);
+        run_steps(c, 2, first, count, 0, 0, c.g.nt);
+        c.run_kind = 0;
+    });
+}
+
+int inv_forward_begin(inv_ctx *ctx, int first, int count, int flags) {
+    return guard(ctx, [&](Ctx &c) {
+        require_ready(c);
+        check_slots(c, first, count);
+        c.run_kind = 1;
+        c.run_first = first;
+        c.run_count = count;
+        c.run_flags = flags;
+        c.run_it = 0;
+    });
+}
+
+int inv_adjoint_begin(inv_ctx *ctx, int first, int count) {
+    return guard(ctx, [&](Ctx &c) {
+        require_ready(c);
+        check_slots(c, first < 0 ? 0 : first, count);
+        INV_REQUIRE(c.g.adj && c.snap.p, INV_ERR_STATE, "adjoint needs adj = 1 (snapshots)");
+        c.run_kind = 2;
+        c.run_first = first;
+        c.run_count = count;
+        c.run_flags = 0;
+        c.run_it = 0;
+    });
+}
+
+int inv_run_advance(inv_ctx *ctx, int nsteps, int *position) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.run_kind != 0, INV_ERR_STATE, "no run begun (inv_forward_begin)");
+        INV_REQUIRE(nsteps >= 0, INV_ERR_ARG, "nsteps must be >= 0");
+        const int end = c.run_kind == 1 ? forward_end(c) : c.g.nt;
+        const int it1 = std::min(end, c.run_it + nsteps);
+        if (it1 > c.run_it)
+            run_steps(c, c.run_kind, c.run_first, c.run_count, c.run_flags, c.run_it, it1);
+        c.run_it = it1;
+        if (position) *position = it1;
+        if (it1 == end) c.run_kind = 0;
     });
 }
 
@@ -597,6 +660,16 @@ int inv_misfit(inv_ctx *ctx, int first, int count, int type, double *chi, double
     });
 }
 
+int inv_misfit_trace(inv_ctx *ctx, int type, const float *syn, const float *obs, float *adstf,
+                     double *chi) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_grid, INV_ERR_STATE, "grid not set");
+        INV_REQUIRE(syn && obs && adstf && chi, INV_ERR_ARG, "null argument");
+        INV_REQUIRE(type == 0 || type == 1, INV_ERR_ARG, "misfit type must be 0 or 1");
+        *chi = misfit_trace(c, type, syn, obs, adstf);
+    });
+}
+
 int inv_gradient(inv_ctx *ctx, int misfit_type, int sigma, int kernel, double *misfit) {
     return guard(ctx, [&](Ctx &c) {
         require_ready(c);
@@ -609,14 +682,9 @@ int inv_gradient(inv_ctx *ctx, int misfit_type, int sigma, int kernel, double *m
         double total = 0;
         for (int first = 0; first < c.nsrc; first += c.g.batch) {
             const int count = std::min(c.g.batch, c.nsrc - first);
-            const bool split = c.variant == INV_VARIANT_SPLIT;
-            if (split) fdm_forward(c, first, count, kernel ? INV_FWD_SNAPSHOTS : 0);
-            else fused_forward(c, first, count, kernel ? INV_FWD_SNAPSHOTS : 0);
+            run_steps(c, 1, first, count, kernel ? INV_FWD_SNAPSHOTS : 0, 0, c.g.nt);
             total += run_misfit(c, first, count, misfit_type, nullptr);
-            if (kernel) {
-                if (split) fdm_adjoint(c, first, count);
-                else fused_adjoint(c, first, count);
-            }
+            if (kernel) run_steps(c, 2, first, count, 0, 0, c.g.nt);
         }
         if (kernel && sigma > 0) {
             gauss_filter(c, c.k_lambda.as<float>(), c.g.nx, c.g.nz, sigma);
@@ -803,13 +871,24 @@ int inv_prof_read(inv_ctx *ctx, int kernel, long long *count, double *total_ms) 
 
 int inv_set_variant(inv_ctx *ctx, int variant, int tile_x) {
     return guard(ctx, [&](Ctx &c) {
-        INV_REQUIRE(variant == INV_VARIANT_FUSED || variant == INV_VARIANT_SPLIT, INV_ERR_ARG,
-                    "unknown variant");
+        INV_REQUIRE(variant == INV_VARIANT_FUSED || variant == INV_VARIANT_SPLIT ||
+                        variant == INV_VARIANT_FUSED_LINEAR,
+                    INV_ERR_ARG, "unknown variant");
         INV_REQUIRE(tile_x == 0 || tile_x == 8 || tile_x == 16 || tile_x == 32, INV_ERR_ARG,
                     "tile_x must be 0, 8, 16 or 32");
         sync(c);
         c.variant = variant;
         if (tile_x) c.tile_x = tile_x;
+        fused_invalidate_lists(c);
+    });
+}
+
+int inv_set_math(inv_ctx *ctx, int mode) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(mode == INV_MATH_EXACT || mode == INV_MATH_FAST, INV_ERR_ARG,
+                    "unknown math mode");
+        sync(c);
+        c.math = mode;
     });
 }
 

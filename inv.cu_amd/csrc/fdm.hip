@@ -55,7 +55,7 @@ struct IntArgs {
     int inv_lambda, inv_mu, inv_rho;
 };
 
-template <bool SH, bool PSV>
+template <bool SH, bool PSV, bool FAST>
 __global__ __launch_bounds__(TILE_Z *TILE_X) void k_velocity(const VelArgs a) {
     const int j = blockIdx.x * TILE_Z + threadIdx.x;
     const int i = blockIdx.y * TILE_X + threadIdx.y;
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_velocity(const VelArgs a) {
     const bool zin = j >= 2 && j < g.nz - 2;
     // initAbsbound product order: left, right (abx), bottom, top
     const float ab = (a.abx[i] * a.abzb[j]) * a.abzt[j];
-    const float rho = a.rho[k];
+    const float rho = a.rho[k];  // dt / rho in fast mode
     const int tile = blockIdx.y * gridDim.x + blockIdx.x;
     int p0 = 0, p1 = 0;
     if (a.inj_ptr) {
@@ -85,14 +85,14 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_velocity(const VelArgs a) {
     const long it = a.it;
     if (SH) {
         const float *sxy = a.sxy + (long)b * g.N, *szy = a.szy + (long)b * g.N;
-        float d = xin ? d_minus(sxy, k, nz, g.c8dx, g.c24dx) : 0.0f;
-        if (zin) d += d_minus(szy, k, 1, g.c8dz, g.c24dz);
+        float d = xin ? d_minus<FAST>(sxy, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
+        if (zin) d += d_minus<FAST>(szy, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
         for (int p = p0; p < p1; p++)
             if (a.inj_cell[p] == k)
                 d += a.inj_y[(long)b * a.inj_slot_stride + (long)a.inj_row[p] * a.nt + it];
         float v = a.vy[o], u = a.uy[o];
         if (a.snap) a.snap[(long)b * a.snap_slot_stride + k] = u;
-        v = ab * (v + g.dt * d / rho);
+        v = vel_update<FAST>(ab, v, g.dt, d, rho);
         u += v * g.dt;
         a.vy[o] = v;
         a.uy[o] = u;
@@ -107,15 +107,15 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_velocity(const VelArgs a) {
         const float *sxx = a.sxx + so, *szz = a.szz + so, *sxz = a.sxz + so;
         float dx_, dz_;
         if (xin) {
-            dx_ = d_minus(sxx, k, nz, g.c8dx, g.c24dx);
-            dz_ = d_minus(sxz, k, nz, g.c8dx, g.c24dx);
+            dx_ = d_minus<FAST>(sxx, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x);
+            dz_ = d_minus<FAST>(sxz, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x);
         } else {
             dx_ = 0.0f;
             dz_ = 0.0f;
         }
         if (zin) {
-            dx_ += d_minus(sxz, k, 1, g.c8dz, g.c24dz);
-            dz_ += d_minus(szz, k, 1, g.c8dz, g.c24dz);
+            dx_ += d_minus<FAST>(sxz, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
+            dz_ += d_minus<FAST>(szz, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
         }
         for (int p = p0; p < p1; p++)
             if (a.inj_cell[p] == k) {
@@ -130,8 +130,8 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_velocity(const VelArgs a) {
             sp[0] = ux;
             sp[g.N] = uz;
         }
-        vx = ab * (vx + g.dt * dx_ / rho);
-        vz = ab * (vz + g.dt * dz_ / rho);
+        vx = vel_update<FAST>(ab, vx, g.dt, dx_, rho);
+        vz = vel_update<FAST>(ab, vz, g.dt, dz_, rho);
         ux += vx * g.dt;
         uz += vz * g.dt;
         a.vx[o] = vx;
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_velocity(const VelArgs a) {
     }
 }
 
-template <bool SH, bool PSV>
+template <bool SH, bool PSV, bool FAST>
 __global__ __launch_bounds__(TILE_Z *TILE_X) void k_stress(const StrArgs a) {
     const int j = blockIdx.x * TILE_Z + threadIdx.x;
     const int i = blockIdx.y * TILE_X + threadIdx.y;
@@ -167,8 +167,8 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_stress(const StrArgs a) {
     const float mu = a.mu[k];
     if (SH) {
         const float *vy = a.vy + so;
-        const float dx_ = xin ? d_plus(vy, k, nz, g.c8dx, g.c24dx) : 0.0f;
-        const float dz_ = zin ? d_plus(vy, k, 1, g.c8dz, g.c24dz) : 0.0f;
+        const float dx_ = xin ? d_plus<FAST>(vy, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
+        const float dz_ = zin ? d_plus<FAST>(vy, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z) : 0.0f;
         a.sxy[o] += g.dt * mu * dx_;
         a.szy[o] += g.dt * mu * dz_;
     }
@@ -176,15 +176,15 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_stress(const StrArgs a) {
         const float *vx = a.vx + so, *vz = a.vz + so;
         float xx, zx, xz, zz;
         if (xin) {
-            xx = d_plus(vx, k, nz, g.c8dx, g.c24dx);
-            zx = d_plus(vz, k, nz, g.c8dx, g.c24dx);
+            xx = d_plus<FAST>(vx, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x);
+            zx = d_plus<FAST>(vz, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x);
         } else {
             xx = 0.0f;
             zx = 0.0f;
         }
         if (zin) {
-            xz = d_plus(vx, k, 1, g.c8dz, g.c24dz);
-            zz = d_plus(vz, k, 1, g.c8dz, g.c24dz);
+            xz = d_plus<FAST>(vx, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
+            zz = d_plus<FAST>(vz, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
         } else {
             xz = 0.0f;
             zz = 0.0f;
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_stress(const StrArgs a) {
 
 // interaction* kernels of solver/fdm.h:198-219 with the D+ derivatives of the
 // adjoint and the forward displacement (fdm.h:586-605) computed in place.
-template <bool SH, bool PSV>
+template <bool SH, bool PSV, bool FAST>
 __global__ __launch_bounds__(TILE_Z *TILE_X) void k_interact(const IntArgs a) {
     const int j = blockIdx.x * TILE_Z + threadIdx.x;
     const int i = blockIdx.y * TILE_X + threadIdx.y;
@@ -218,10 +218,10 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_interact(const IntArgs a) {
     if (SH) {
         const float *uy = a.uy + so;
         const float *uy_fw = sp, *vy_fw = sp + g.N;
-        const float dx_ = xin ? d_plus(uy, k, nz, g.c8dx, g.c24dx) : 0.0f;
-        const float dz_ = zin ? d_plus(uy, k, 1, g.c8dz, g.c24dz) : 0.0f;
-        const float dxf = xin ? d_plus(uy_fw, k, nz, g.c8dx, g.c24dx) : 0.0f;
-        const float dzf = zin ? d_plus(uy_fw, k, 1, g.c8dz, g.c24dz) : 0.0f;
+        const float dx_ = xin ? d_plus<FAST>(uy, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
+        const float dz_ = zin ? d_plus<FAST>(uy, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z) : 0.0f;
+        const float dxf = xin ? d_plus<FAST>(uy_fw, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
+        const float dzf = zin ? d_plus<FAST>(uy_fw, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z) : 0.0f;
         if (a.inv_rho) kr[k] -= vy_fw[k] * a.vy[so + k] * ndt;
         if (a.inv_mu) km[k] -= (dx_ * dxf + dz_ * dzf) * ndt;
     }
@@ -232,18 +232,18 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_interact(const IntArgs a) {
         const float *vx_fw = sp + (f0 + 2) * g.N, *vz_fw = sp + (f0 + 3) * g.N;
         float xx, xz, zx, zz, xxf, xzf, zxf, zzf;
         if (xin) {
-            xx = d_plus(ux, k, nz, g.c8dx, g.c24dx);
-            zx = d_plus(uz, k, nz, g.c8dx, g.c24dx);
-            xxf = d_plus(ux_fw, k, nz, g.c8dx, g.c24dx);
-            zxf = d_plus(uz_fw, k, nz, g.c8dx, g.c24dx);
+            xx = d_plus<FAST>(ux, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x);
+            zx = d_plus<FAST>(uz, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x);
+            xxf = d_plus<FAST>(ux_fw, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x);
+            zxf = d_plus<FAST>(uz_fw, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x);
         } else {
             xx = zx = xxf = zxf = 0.0f;
         }
         if (zin) {
-            xz = d_plus(ux, k, 1, g.c8dz, g.c24dz);
-            zz = d_plus(uz, k, 1, g.c8dz, g.c24dz);
-            xzf = d_plus(ux_fw, k, 1, g.c8dz, g.c24dz);
-            zzf = d_plus(uz_fw, k, 1, g.c8dz, g.c24dz);
+            xz = d_plus<FAST>(ux, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
+            zz = d_plus<FAST>(uz, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
+            xzf = d_plus<FAST>(ux_fw, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
+            zzf = d_plus<FAST>(uz_fw, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
         } else {
             xz = zz = xzf = zzf = 0.0f;
         }
@@ -272,6 +272,11 @@ __global__ void k_kslot_reduce(float *__restrict__ kl, float *__restrict__ km,
     kr[k] = r;
 }
 
+__global__ void k_bdt(float *__restrict__ bdt, const float *__restrict__ rho, float dt, long n) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) bdt[k] = dt / rho[k];
+}
+
 __global__ void k_fill(float *__restrict__ p, float v, long n) {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = v;
@@ -290,6 +295,10 @@ Geo make_geo(const Ctx &c) {
     g.c24dx = 24 * c.g.dx;
     g.c8dz = 8 * c.g.dz;
     g.c24dz = 24 * c.g.dz;
+    g.a1x = 9.0f / g.c8dx;
+    g.a2x = 1.0f / g.c24dx;
+    g.a1z = 9.0f / g.c8dz;
+    g.a2z = 1.0f / g.c24dz;
     return g;
 }
 
@@ -324,6 +333,7 @@ void build_point_list(Ctx &c, PointList &pl, int nsets, const std::vector<std::v
             hc[slot] = cells[s][p];
             hr[slot] = rows[s][p];
         }
+    INV_HIP(hipStreamSynchronize(c.stream));
     pl.ptr.ensure(count.size() * sizeof(int));
     pl.cell.ensure(hc.size() * sizeof(int));
     pl.row.ensure(hr.size() * sizeof(int));
@@ -343,6 +353,18 @@ void build_point_list(Ctx &c, PointList &pl, int nsets, const std::vector<std::v
 void fill_dev(Ctx &c, void *p, long n, float v) {
     if (n <= 0 || !p) return;
     k_fill<<<(unsigned)((n + 255) / 256), 256, 0, c.stream>>>(static_cast<float *>(p), v, n);
+    INV_HIP(hipGetLastError());
+}
+
+void fdm_prepare_model(Ctx &c) {
+    if (c.math != INV_MATH_FAST) return;
+    // the model may have changed through its device pointers since the last
+    // run (the optimizer updates it in place): rebuild dt / rho.  The buffer
+    // is allocated by inv_grid_set with the model.
+    INV_REQUIRE(c.bdt.p && c.bdt.bytes >= c.N * sizeof(float), INV_ERR_STATE,
+                "dt/rho buffer missing");
+    k_bdt<<<(unsigned)((c.N + 255) / 256), 256, 0, c.stream>>>(c.bdt.as<float>(),
+                                                               c.rho.as<float>(), c.g.dt, c.N);
     INV_HIP(hipGetLastError());
 }
 
@@ -410,22 +432,28 @@ void prof_release(Ctx &c) {
     c.prof.pool.clear();
 }
 
-template <bool SH, bool PSV>
+template <bool SH, bool PSV, bool FAST>
 static void launch_step(Ctx &c, const VelArgs &va, const StrArgs &sa, int count, int it) {
     const dim3 grid = tile_grid(c, count), block(TILE_Z, TILE_X);
     hipEvent_t a;
     bool p = prof_begin(c, it, &a);
-    k_velocity<SH, PSV><<<grid, block, 0, c.stream>>>(va);
+    k_velocity<SH, PSV, FAST><<<grid, block, 0, c.stream>>>(va);
     if (p) prof_end(c, PROF_VELOCITY, a);
     p = prof_begin(c, it, &a);
-    k_stress<SH, PSV><<<grid, block, 0, c.stream>>>(sa);
+    k_stress<SH, PSV, FAST><<<grid, block, 0, c.stream>>>(sa);
     if (p) prof_end(c, PROF_STRESS, a);
 }
 
+template <bool FAST>
+static void step_m(Ctx &c, const VelArgs &va, const StrArgs &sa, int count, int it) {
+    if (c.g.sh && c.g.psv) launch_step<true, true, FAST>(c, va, sa, count, it);
+    else if (c.g.sh) launch_step<true, false, FAST>(c, va, sa, count, it);
+    else launch_step<false, true, FAST>(c, va, sa, count, it);
+}
+
 static void step(Ctx &c, const VelArgs &va, const StrArgs &sa, int count, int it) {
-    if (c.g.sh && c.g.psv) launch_step<true, true>(c, va, sa, count, it);
-    else if (c.g.sh) launch_step<true, false>(c, va, sa, count, it);
-    else launch_step<false, true>(c, va, sa, count, it);
+    if (c.math == INV_MATH_FAST) step_m<true>(c, va, sa, count, it);
+    else step_m<false>(c, va, sa, count, it);
     INV_HIP(hipGetLastError());
 }
 
@@ -444,18 +472,20 @@ static void make_common(Ctx &c, VelArgs &va, StrArgs &sa) {
     va = VelArgs{};
     sa = StrArgs{};
     va.g = sa.g = make_geo(c);
-    va.rho = c.rho.as<float>();
+    va.rho = c.math == INV_MATH_FAST ? c.bdt.as<float>() : c.rho.as<float>();
     va.abx = c.abx.as<float>();
     va.abzb = c.abzb.as<float>();
     va.abzt = c.abzt.as<float>();
     sa.lam = c.lam.as<float>();
     sa.mu = c.mu.as<float>();
+    INV_REQUIRE(va.rho && va.abx && va.abzb && va.abzt && sa.lam && sa.mu, INV_ERR_STATE,
+                "model or absorbing profile not allocated");
     fill_state_ptrs(c, va, sa);
     va.nt = c.g.nt;
     va.ntiles = c.rec_list.ntiles;
 }
 
-void fdm_forward(Ctx &c, int first, int count, int flags) {
+void fdm_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
     // point lists of this batch's sources: set b = shot first+b (or every
     // source in set 0 for first == -1, runForward(-1) of fdm.h:171)
     if (c.src_list_first != first || c.src_list_count != count) {
@@ -498,11 +528,12 @@ void fdm_forward(Ctx &c, int first, int count, int flags) {
     va.snap_slot_stride = (long)c.nsfe * c.nsnapf * c.N;
     const bool store = (flags & INV_FWD_SNAPSHOTS) && c.g.adj && c.g.sfe > 0 && c.snap.p;
 
-    c.parity = 0;
-    fdm_zero_state(c, count);
-    int nsteps = c.g.nt;
-    if (c.step_limit > 0 && c.step_limit < nsteps) nsteps = c.step_limit;
-    for (int it = 0; it < nsteps; it++) {
+    if (it0 == 0) {
+        c.parity = 0;
+        fdm_zero_state(c, count);
+        fdm_prepare_model(c);
+    }
+    for (int it = it0; it < it1; it++) {
         va.it = it;
         va.snap = nullptr;
         if (store && (it + 1) % c.g.sfe == 0) {
@@ -515,7 +546,10 @@ void fdm_forward(Ctx &c, int first, int count, int flags) {
 
 template <bool SH, bool PSV>
 static void launch_interact(Ctx &c, const IntArgs &ia, int count) {
-    k_interact<SH, PSV><<<tile_grid(c, count), dim3(TILE_Z, TILE_X), 0, c.stream>>>(ia);
+    if (c.math == INV_MATH_FAST)
+        k_interact<SH, PSV, true><<<tile_grid(c, count), dim3(TILE_Z, TILE_X), 0, c.stream>>>(ia);
+    else
+        k_interact<SH, PSV, false><<<tile_grid(c, count), dim3(TILE_Z, TILE_X), 0, c.stream>>>(ia);
 }
 
 void fdm_interact(Ctx &c, int count, int it, const float *vx, const float *vz, const float *vy) {
@@ -558,7 +592,7 @@ void fdm_interact(Ctx &c, int count, int it, const float *vx, const float *vz, c
     if (p) prof_end(c, PROF_INTERACT, a);
 }
 
-void fdm_adjoint(Ctx &c, int first, int count) {
+void fdm_adjoint(Ctx &c, int first, int count, int it0, int it1) {
     (void)first;
     VelArgs va;
     StrArgs sa;
@@ -578,17 +612,19 @@ void fdm_adjoint(Ctx &c, int first, int count) {
     va.rec_ptr = nullptr;
     va.snap = nullptr;
     const bool direct = c.g.batch == 1;
-    if (!direct) fill_dev(c, c.kslot.p, 3L * count * c.N);
-
-    c.parity = 0;
-    fdm_zero_state(c, count);
-    for (int it = 0; it < c.g.nt; it++) {
+    if (it0 == 0) {
+        if (!direct) fill_dev(c, c.kslot.p, 3L * count * c.N);
+        c.parity = 0;
+        fdm_zero_state(c, count);
+        fdm_prepare_model(c);
+    }
+    for (int it = it0; it < it1; it++) {
         va.it = it;
         step(c, va, sa, count, it);
         if (it % c.g.sfe == 0)
             fdm_interact(c, count, it, c.vx.as<float>(), c.vz.as<float>(), c.vy.as<float>());
     }
-    if (!direct) fdm_kslot_reduce(c, count);
+    if (!direct && it1 == c.g.nt) fdm_kslot_reduce(c, count);
 }
 
 void fdm_kslot_reduce(Ctx &c, int count) {

@@ -50,12 +50,15 @@ struct StepArgs {
     // snapshot slot base (u before, v after the step, fdm.h:521-560) or null
     float *snap;
     long snap_slot_stride;
+    long psv_snap_off;  // offset of the P-SV fields in a snapshot slot
 };
 
-template <bool SH, bool PSV, int TX>
+template <bool SH, bool PSV, int TX, bool FAST>
 __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
     constexpr int SW = FZ + 6, SR = TX + 6;  // stress tile: halo 3 on each side
     constexpr int VW = FZ + 3, VR = TX + 3;  // velocity tile: halo 1 before, 2 after
+    constexpr int NV = (VR * VW + FBLOCK - 1) / FBLOCK;  // velocity cells per thread
+    constexpr int NT = TX * FZ / FBLOCK;                 // tile cells per thread
     __shared__ float l_sxx[PSV ? SR * SW : 1], l_szz[PSV ? SR * SW : 1], l_sxz[PSV ? SR * SW : 1];
     __shared__ float l_sxy[SH ? SR * SW : 1], l_szy[SH ? SR * SW : 1];
     __shared__ float l_vx[PSV ? VR * VW : 1], l_vz[PSV ? VR * VW : 1], l_vy[SH ? VR * VW : 1];
@@ -72,6 +75,41 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
     const Geo &g = a.g;
     const int nx = g.nx, nz = g.nz;
     const long so = (long)b * g.N;
+
+    // 0. issue the global loads of phases 2 and 3 first, so that they are in
+    //    flight together with the stress loads of phase 1 (the barriers below
+    //    wait for LDS traffic only)
+    float pvx[NV], pvz[NV], pvy[NV], prho[NV], pux[NV], puz[NV], puy[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const int e = threadIdx.x + k * FBLOCK;
+        const int vr = e / VW, vc = e - vr * VW;
+        const int i = i0 - 1 + vr, j = j0 - 1 + vc;
+        const bool in = e < VR * VW && i >= 0 && i < nx && j >= 0 && j < nz;
+        const bool own = in && vr >= 1 && vr <= TX && vc >= 1 && vc <= FZ;
+        const long kc = (long)i * nz + j;
+        prho[k] = in ? a.rho[kc] : 1.0f;
+        if (PSV) {
+            pvx[k] = in ? a.vx0[so + kc] : 0.0f;
+            pvz[k] = in ? a.vz0[so + kc] : 0.0f;
+            pux[k] = own ? a.ux[so + kc] : 0.0f;
+            puz[k] = own ? a.uz[so + kc] : 0.0f;
+        }
+        if (SH) {
+            pvy[k] = in ? a.vy0[so + kc] : 0.0f;
+            puy[k] = own ? a.uy[so + kc] : 0.0f;
+        }
+    }
+    float plam[NT], pmu[NT];
+#pragma unroll
+    for (int k = 0; k < NT; k++) {
+        const int e = threadIdx.x + k * FBLOCK;
+        const int i = i0 + e / FZ, j = j0 + (e % FZ);
+        const bool in = i < nx && j < nz;
+        const long kc = (long)i * nz + j;
+        pmu[k] = in ? a.mu[kc] : 0.0f;
+        if (PSV) plam[k] = in ? a.lam[kc] : 0.0f;
+    }
 
     // 1. stress of step n on the tile plus halo
     for (int e = threadIdx.x; e < SR * SW; e += FBLOCK) {
@@ -105,7 +143,10 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
     const long it = a.it;
 
     // 2. velocity of step n+1 on the tile plus halo (updateV*, fdm.h:150-161)
-    for (int e = threadIdx.x; e < VR * VW; e += FBLOCK) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const int e = threadIdx.x + k * FBLOCK;
+        if (e >= VR * VW) break;
         const int vr = e / VW, vc = e - vr * VW;
         const int i = i0 - 1 + vr, j = j0 - 1 + vc;
         if (i < 0 || i >= nx || j < 0 || j >= nz) {
@@ -123,18 +164,17 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
         const bool own = vr >= 1 && vr <= TX && vc >= 1 && vc <= FZ;
         const int se = (vr + 2) * SW + (vc + 2);
         const float ab = (a.abx[i] * a.abzb[j]) * a.abzt[j];
-        const float rho = a.rho[kc];
+        const float rho = prho[k];  // dt / rho in fast mode
         if (SH) {
-            float d = xin ? d_minus(l_sxy, se, SW, g.c8dx, g.c24dx) : 0.0f;
-            if (zin) d += d_minus(l_szy, se, 1, g.c8dz, g.c24dz);
+            float d = xin ? d_minus<FAST>(l_sxy, se, SW, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
+            if (zin) d += d_minus<FAST>(l_szy, se, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
             for (int p = p0; p < p1; p++)
                 if (a.inj_cell[p] == kc)
                     d += a.inj_y[(long)b * a.inj_slot_stride + (long)a.inj_row[p] * a.nt + it];
-            float v = a.vy0[o];
-            v = ab * (v + g.dt * d / rho);
+            const float v = vel_update<FAST>(ab, pvy[k], g.dt, d, rho);
             l_vy[e] = v;
             if (own) {
-                float u = a.uy[o];
+                float u = puy[k];
                 if (a.snap) a.snap[(long)b * a.snap_slot_stride + kc] = u;
                 u += v * g.dt;
                 a.vy1[o] = v;
@@ -149,15 +189,15 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
         if (PSV) {
             float dx_, dz_;
             if (xin) {
-                dx_ = d_minus(l_sxx, se, SW, g.c8dx, g.c24dx);
-                dz_ = d_minus(l_sxz, se, SW, g.c8dx, g.c24dx);
+                dx_ = d_minus<FAST>(l_sxx, se, SW, g.c8dx, g.c24dx, g.a1x, g.a2x);
+                dz_ = d_minus<FAST>(l_sxz, se, SW, g.c8dx, g.c24dx, g.a1x, g.a2x);
             } else {
                 dx_ = 0.0f;
                 dz_ = 0.0f;
             }
             if (zin) {
-                dx_ += d_minus(l_sxz, se, 1, g.c8dz, g.c24dz);
-                dz_ += d_minus(l_szz, se, 1, g.c8dz, g.c24dz);
+                dx_ += d_minus<FAST>(l_sxz, se, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
+                dz_ += d_minus<FAST>(l_szz, se, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
             }
             for (int p = p0; p < p1; p++)
                 if (a.inj_cell[p] == kc) {
@@ -165,13 +205,12 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
                     dx_ += a.inj_x[ks];
                     dz_ += a.inj_z[ks];
                 }
-            float vx = a.vx0[o], vz = a.vz0[o];
-            vx = ab * (vx + g.dt * dx_ / rho);
-            vz = ab * (vz + g.dt * dz_ / rho);
+            const float vx = vel_update<FAST>(ab, pvx[k], g.dt, dx_, rho);
+            const float vz = vel_update<FAST>(ab, pvz[k], g.dt, dz_, rho);
             l_vx[e] = vx;
             l_vz[e] = vz;
             if (own) {
-                float ux = a.ux[o], uz = a.uz[o];
+                float ux = pux[k], uz = puz[k];
                 const int f0 = SH ? 2 : 0;
                 float *sp = a.snap ? a.snap + (long)b * a.snap_slot_stride + (long)f0 * g.N + kc
                                    : nullptr;
@@ -202,7 +241,9 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
     __syncthreads();
 
     // 3. stress of step n+1 on the tile (divV* + updateS*, fdm.h:103-149)
-    for (int e = threadIdx.x; e < TX * FZ; e += FBLOCK) {
+#pragma unroll
+    for (int k = 0; k < NT; k++) {
+        const int e = threadIdx.x + k * FBLOCK;
         const int tr = e / FZ, tc = e - tr * FZ;
         const int i = i0 + tr, j = j0 + tc;
         if (i >= nx || j >= nz) continue;
@@ -212,30 +253,30 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
         const bool zin = j >= 1 && j < nz - 2;
         const int ve = (tr + 1) * VW + (tc + 1);
         const int se = (tr + 3) * SW + (tc + 3);
-        const float mu = a.mu[kc];
+        const float mu = pmu[k];
         if (SH) {
-            const float dx_ = xin ? d_plus(l_vy, ve, VW, g.c8dx, g.c24dx) : 0.0f;
-            const float dz_ = zin ? d_plus(l_vy, ve, 1, g.c8dz, g.c24dz) : 0.0f;
+            const float dx_ = xin ? d_plus<FAST>(l_vy, ve, VW, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
+            const float dz_ = zin ? d_plus<FAST>(l_vy, ve, 1, g.c8dz, g.c24dz, g.a1z, g.a2z) : 0.0f;
             a.sxy1[o] = l_sxy[se] + g.dt * mu * dx_;
             a.szy1[o] = l_szy[se] + g.dt * mu * dz_;
         }
         if (PSV) {
             float xx, zx, xz, zz;
             if (xin) {
-                xx = d_plus(l_vx, ve, VW, g.c8dx, g.c24dx);
-                zx = d_plus(l_vz, ve, VW, g.c8dx, g.c24dx);
+                xx = d_plus<FAST>(l_vx, ve, VW, g.c8dx, g.c24dx, g.a1x, g.a2x);
+                zx = d_plus<FAST>(l_vz, ve, VW, g.c8dx, g.c24dx, g.a1x, g.a2x);
             } else {
                 xx = 0.0f;
                 zx = 0.0f;
             }
             if (zin) {
-                xz = d_plus(l_vx, ve, 1, g.c8dz, g.c24dz);
-                zz = d_plus(l_vz, ve, 1, g.c8dz, g.c24dz);
+                xz = d_plus<FAST>(l_vx, ve, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
+                zz = d_plus<FAST>(l_vz, ve, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
             } else {
                 xz = 0.0f;
                 zz = 0.0f;
             }
-            const float lam = a.lam[kc];
+            const float lam = plam[k];
             a.sxx1[o] = l_sxx[se] + g.dt * ((lam + 2 * mu) * xx + lam * zz);
             a.szz1[o] = l_szz[se] + g.dt * ((lam + 2 * mu) * zz + lam * xx);
             a.sxz1[o] = l_sxz[se] + g.dt * (mu * (xz + zx));
@@ -244,16 +285,287 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Column kernel: the same step with one lane per column.  The stress tile is
+// exactly one wavefront wide (CZ + 6 = 64 columns), the velocity tile uses
+// lanes 0..CZ+2 and the output tile lanes 0..CZ-1; the four wavefronts stride
+// over rows.  Addresses advance by nz per row, with no index division and no
+// row wrap-around inside a wavefront.
+// ---------------------------------------------------------------------------
+constexpr int CZ = 58;
+
+template <bool SH, bool PSV, int TX, bool FAST>
+__global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
+    static_assert(SH != PSV, "one wave type per column-kernel launch");
+    constexpr int W = 64;                 // LDS row = wavefront width
+    constexpr int SR = TX + 6, VR = TX + 3;
+    constexpr int NWAVE = FBLOCK / 64;
+    constexpr int PS = (SR + NWAVE - 1) / NWAVE;  // stress rows per wave
+    constexpr int PV = (VR + NWAVE - 1) / NWAVE;  // velocity rows per wave
+    constexpr int PT = TX / NWAVE;                // output rows per wave
+    __shared__ float l_sxx[PSV ? SR * W : 1], l_szz[PSV ? SR * W : 1], l_sxz[PSV ? SR * W : 1];
+    __shared__ float l_sxy[SH ? SR * W : 1], l_szy[SH ? SR * W : 1];
+    __shared__ float l_vx[PSV ? VR * W : 1], l_vz[PSV ? VR * W : 1], l_vy[SH ? VR * W : 1];
+
+    const int nw = gridDim.x, w = blockIdx.x;
+    const int q = nw >> 3, rmd = nw & 7, xcd = w & 7;
+    const int L = xcd * q + (xcd < rmd ? xcd : rmd) + (w >> 3);
+    const int per = a.tiles_x * a.tiles_z;
+    const int b = L / per;
+    const int t = L - b * per;
+    const int tx = t / a.tiles_z, tz = t - tx * a.tiles_z;
+    const int i0 = tx * TX, j0 = tz * CZ;
+    const Geo &g = a.g;
+    const int nx = g.nx, nz = g.nz;
+    const long so = (long)b * g.N;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
+
+    // All global loads of the three phases are issued first, at clamped (so
+    // always valid) addresses; values outside the grid are replaced by 0
+    // afterwards.  Branch-free loads let the whole batch be in flight at once.
+    const int js = j0 - 3 + lane, jv = j0 - 1 + lane, jt = j0 + lane;
+    const bool jsin = js >= 0 && js < nz;
+    const bool jvin = lane < CZ + 3 && jv >= 0 && jv < nz;
+    const bool jtin = lane < CZ && jt < nz;
+    const int jsc = clampi(js, 0, nz - 1), jvc = clampi(jv, 0, nz - 1), jtc = clampi(jt, 0, nz - 1);
+
+    float s1[PS], s2[PS], s3[PS];
+#pragma unroll
+    for (int p = 0; p < PS; p++) {
+        const int r = wave + p * NWAVE;
+        const long k = so + (long)clampi(i0 - 3 + r, 0, nx - 1) * nz + jsc;
+        if (PSV) {
+            s1[p] = a.sxx0[k];
+            s2[p] = a.szz0[k];
+            s3[p] = a.sxz0[k];
+        } else {
+            s1[p] = a.sxy0[k];
+            s2[p] = a.szy0[k];
+        }
+    }
+    float qv1[PV], qv2[PV], qr[PV], qu1[PV], qu2[PV];
+#pragma unroll
+    for (int p = 0; p < PV; p++) {
+        const int vr = wave + p * NWAVE;
+        const long kc = (long)clampi(i0 - 1 + vr, 0, nx - 1) * nz + jvc;
+        qr[p] = a.rho[kc];
+        if (PSV) {
+            qv1[p] = a.vx0[so + kc];
+            qv2[p] = a.vz0[so + kc];
+            qu1[p] = a.ux[so + kc];
+            qu2[p] = a.uz[so + kc];
+        } else {
+            qv1[p] = a.vy0[so + kc];
+            qu1[p] = a.uy[so + kc];
+        }
+    }
+    float qmu[PT], qlam[PT];
+#pragma unroll
+    for (int p = 0; p < PT; p++) {
+        const int tr = wave + p * NWAVE;
+        const long kc = (long)clampi(i0 + tr, 0, nx - 1) * nz + jtc;
+        qmu[p] = a.mu[kc];
+        if (PSV) qlam[p] = a.lam[kc];
+    }
+    const float abz = a.abzb[jvc], abt = a.abzt[jvc];
+
+    // 1. stress of step n, rows i0-3 .. i0+TX+2, columns j0-3 .. j0+60
+#pragma unroll
+    for (int p = 0; p < PS; p++) {
+        const int r = wave + p * NWAVE;
+        if (r < SR) {
+            const int i = i0 - 3 + r;
+            const bool in = jsin && i >= 0 && i < nx;
+            if (PSV) {
+                l_sxx[r * W + lane] = in ? s1[p] : 0.0f;
+                l_szz[r * W + lane] = in ? s2[p] : 0.0f;
+                l_sxz[r * W + lane] = in ? s3[p] : 0.0f;
+            } else {
+                l_sxy[r * W + lane] = in ? s1[p] : 0.0f;
+                l_szy[r * W + lane] = in ? s2[p] : 0.0f;
+            }
+        }
+    }
+    __syncthreads();
+
+    int p0 = 0, p1 = 0;
+    if (a.inj_ptr) {
+        const int set = a.inj_per_slot ? b : 0;
+        p0 = a.inj_ptr[set * per + t];
+        p1 = a.inj_ptr[set * per + t + 1];
+    }
+    int r0 = 0, r1 = 0;
+    if (a.rec_ptr) {
+        r0 = a.rec_ptr[t];
+        r1 = a.rec_ptr[t + 1];
+    }
+    const long it = a.it;
+
+    // 2. velocity of step n+1, rows i0-1 .. i0+TX+1, columns j0-1 .. j0+CZ+1
+    {
+        const int j = jv;
+        const bool zin = j >= 2 && j < nz - 2;
+        const bool jown = lane >= 1 && lane <= CZ;
+#pragma unroll
+        for (int p = 0; p < PV; p++) {
+            const int vr = wave + p * NWAVE;
+            if (vr >= VR) break;
+            const int i = i0 - 1 + vr;
+            const int e = vr * W + lane;
+            if (!(jvin && i >= 0 && i < nx)) {
+                if (PSV) {
+                    l_vx[e] = 0.0f;
+                    l_vz[e] = 0.0f;
+                } else {
+                    l_vy[e] = 0.0f;
+                }
+                continue;
+            }
+            const long kc = (long)i * nz + j;
+            const long o = so + kc;
+            const bool xin = i >= 2 && i < nx - 2;
+            const bool own = jown && vr >= 1 && vr <= TX;
+            const int se = (vr + 2) * W + (lane + 2);
+            const float ab = (a.abx[i] * abz) * abt;
+            const float rho = qr[p];  // dt / rho in fast mode
+            if (!PSV) {
+                float d = xin ? d_minus<FAST>(l_sxy, se, W, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
+                if (zin) d += d_minus<FAST>(l_szy, se, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
+                for (int pp = p0; pp < p1; pp++)
+                    if (a.inj_cell[pp] == kc)
+                        d += a.inj_y[(long)b * a.inj_slot_stride + (long)a.inj_row[pp] * a.nt + it];
+                const float v = vel_update<FAST>(ab, qv1[p], g.dt, d, rho);
+                l_vy[e] = v;
+                if (own) {
+                    float u = qu1[p];
+                    if (a.snap) a.snap[(long)b * a.snap_slot_stride + kc] = u;
+                    u += v * g.dt;
+                    a.vy1[o] = v;
+                    a.uy[o] = u;
+                    if (a.snap) a.snap[(long)b * a.snap_slot_stride + g.N + kc] = v;
+                    for (int pp = r0; pp < r1; pp++)
+                        if (a.rec_cell[pp] == kc)
+                            a.out_y[(long)b * a.out_slot_stride + (long)a.rec_row[pp] * a.nt + it] =
+                                a.rec_disp ? u : v;
+                }
+            } else {
+                float dx_, dz_;
+                if (xin) {
+                    dx_ = d_minus<FAST>(l_sxx, se, W, g.c8dx, g.c24dx, g.a1x, g.a2x);
+                    dz_ = d_minus<FAST>(l_sxz, se, W, g.c8dx, g.c24dx, g.a1x, g.a2x);
+                } else {
+                    dx_ = 0.0f;
+                    dz_ = 0.0f;
+                }
+                if (zin) {
+                    dx_ += d_minus<FAST>(l_sxz, se, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
+                    dz_ += d_minus<FAST>(l_szz, se, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
+                }
+                for (int pp = p0; pp < p1; pp++)
+                    if (a.inj_cell[pp] == kc) {
+                        const long ks =
+                            (long)b * a.inj_slot_stride + (long)a.inj_row[pp] * a.nt + it;
+                        dx_ += a.inj_x[ks];
+                        dz_ += a.inj_z[ks];
+                    }
+                const float vx = vel_update<FAST>(ab, qv1[p], g.dt, dx_, rho);
+                const float vz = vel_update<FAST>(ab, qv2[p], g.dt, dz_, rho);
+                l_vx[e] = vx;
+                l_vz[e] = vz;
+                if (own) {
+                    float ux = qu1[p], uz = qu2[p];
+                    float *sp = a.snap ? a.snap + (long)b * a.snap_slot_stride + a.psv_snap_off + kc
+                                       : nullptr;
+                    if (sp) {
+                        sp[0] = ux;
+                        sp[g.N] = uz;
+                    }
+                    ux += vx * g.dt;
+                    uz += vz * g.dt;
+                    a.vx1[o] = vx;
+                    a.vz1[o] = vz;
+                    a.ux[o] = ux;
+                    a.uz[o] = uz;
+                    if (sp) {
+                        sp[2 * g.N] = vx;
+                        sp[3 * g.N] = vz;
+                    }
+                    for (int pp = r0; pp < r1; pp++)
+                        if (a.rec_cell[pp] == kc) {
+                            const long kr =
+                                (long)b * a.out_slot_stride + (long)a.rec_row[pp] * a.nt + it;
+                            a.out_x[kr] = a.rec_disp ? ux : vx;
+                            a.out_z[kr] = a.rec_disp ? uz : vz;
+                        }
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // 3. stress of step n+1, rows i0 .. i0+TX-1, columns j0 .. j0+CZ-1
+    if (!jtin) return;
+    {
+        const int j = jt;
+        const bool zin = j >= 1 && j < nz - 2;
+#pragma unroll
+        for (int p = 0; p < PT; p++) {
+            const int tr = wave + p * NWAVE;
+            const int i = i0 + tr;
+            if (i >= nx) break;
+            const long kc = (long)i * nz + j;
+            const long o = so + kc;
+            const bool xin = i >= 1 && i < nx - 2;
+            const int ve = (tr + 1) * W + (lane + 1);
+            const int se = (tr + 3) * W + (lane + 3);
+            const float mu = qmu[p];
+            if (!PSV) {
+                const float dx_ = xin ? d_plus<FAST>(l_vy, ve, W, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
+                const float dz_ = zin ? d_plus<FAST>(l_vy, ve, 1, g.c8dz, g.c24dz, g.a1z, g.a2z) : 0.0f;
+                a.sxy1[o] = l_sxy[se] + g.dt * mu * dx_;
+                a.szy1[o] = l_szy[se] + g.dt * mu * dz_;
+            } else {
+                float xx, zx, xz, zz;
+                if (xin) {
+                    xx = d_plus<FAST>(l_vx, ve, W, g.c8dx, g.c24dx, g.a1x, g.a2x);
+                    zx = d_plus<FAST>(l_vz, ve, W, g.c8dx, g.c24dx, g.a1x, g.a2x);
+                } else {
+                    xx = 0.0f;
+                    zx = 0.0f;
+                }
+                if (zin) {
+                    xz = d_plus<FAST>(l_vx, ve, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
+                    zz = d_plus<FAST>(l_vz, ve, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
+                } else {
+                    xz = 0.0f;
+                    zz = 0.0f;
+                }
+                const float lam = qlam[p];
+                a.sxx1[o] = l_sxx[se] + g.dt * ((lam + 2 * mu) * xx + lam * zz);
+                a.szz1[o] = l_szz[se] + g.dt * ((lam + 2 * mu) * zz + lam * xx);
+                a.sxz1[o] = l_sxz[se] + g.dt * (mu * (xz + zx));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// output columns per tile along z: 64 for the linear-index kernel, 58 for the
+// column kernel (whose stress tile, 58 + 6 halo columns, is one wavefront wide)
+static int fz_of(const Ctx &c) { return c.variant == INV_VARIANT_FUSED_LINEAR ? FZ : CZ; }
 static int tiles_x_of(const Ctx &c, int TX) { return (c.g.nx + TX - 1) / TX; }
-static int tiles_z_of(const Ctx &c) { return (c.g.nz + FZ - 1) / FZ; }
+static int tiles_z_of(const Ctx &c) { return (c.g.nz + fz_of(c) - 1) / fz_of(c); }
+static int list_key(const Ctx &c) { return c.tile_x * 1000 + fz_of(c); }
 
 // CSR of points by tile: a point (i, j) belongs to tile (tx, tz) when
-// tx*TX - lo <= i < (tx+1)*TX + hi and tz*FZ - lo <= j < (tz+1)*FZ + hi.
+// tx*TX - lo <= i < (tx+1)*TX + hi and tz*W - lo <= j < (tz+1)*W + hi, W the
+// tile width along z.
 static void build_tiled(Ctx &c, PointList &pl, int nsets, const std::vector<std::vector<int>> &cells,
                         const std::vector<std::vector<int>> &rows, int TX, int lo, int hi) {
     const int ntx = tiles_x_of(c, TX), ntz = tiles_z_of(c);
+    const int FZ = fz_of(c);
     const int ntiles = ntx * ntz;
     auto tiles_of = [&](int cell, std::vector<int> &out) {
         out.clear();
@@ -287,6 +599,8 @@ static void build_tiled(Ctx &c, PointList &pl, int nsets, const std::vector<std:
                 hr[slot] = rows[s][p];
             }
         }
+    // queued launches may still read the old lists: wait before replacing them
+    INV_HIP(hipStreamSynchronize(c.stream));
     pl.ptr.ensure(count.size() * sizeof(int));
     pl.cell.ensure(hc.size() * sizeof(int));
     pl.row.ensure(hr.size() * sizeof(int));
@@ -305,7 +619,7 @@ void fused_invalidate_lists(Ctx &c) {
 }
 
 static void ensure_rec_lists(Ctx &c) {
-    if (c.f_rec_tx == c.tile_x) return;
+    if (c.f_rec_tx == list_key(c)) return;
     std::vector<std::vector<int>> cells(1), rows(1);
     for (int r = 0; r < c.nrec; r++) {
         cells[0].push_back(c.rec_ix[r] * c.g.nz + c.rec_iz[r]);
@@ -313,11 +627,11 @@ static void ensure_rec_lists(Ctx &c) {
     }
     build_tiled(c, c.f_rec_v, 1, cells, rows, c.tile_x, 1, 2);
     build_tiled(c, c.f_rec_t, 1, cells, rows, c.tile_x, 0, 0);
-    c.f_rec_tx = c.tile_x;
+    c.f_rec_tx = list_key(c);
 }
 
 static void ensure_src_list(Ctx &c, int first, int count) {
-    if (c.f_src_first == first && c.f_src_count == count && c.f_src_tx == c.tile_x) return;
+    if (c.f_src_first == first && c.f_src_count == count && c.f_src_tx == list_key(c)) return;
     std::vector<std::vector<int>> cells(count), rows(count);
     for (int b = 0; b < count; b++)
         for (int is = 0; is < c.nsrc; is++)
@@ -328,7 +642,7 @@ static void ensure_src_list(Ctx &c, int first, int count) {
     build_tiled(c, c.f_src_v, count, cells, rows, c.tile_x, 1, 2);
     c.f_src_first = first;
     c.f_src_count = count;
-    c.f_src_tx = c.tile_x;
+    c.f_src_tx = list_key(c);
 }
 
 static void ensure_second_set(Ctx &c) {
@@ -354,12 +668,34 @@ static void set_state_ptrs(Ctx &c, StepArgs &a, int par) {
     a.szy0 = pick(c.szy, c.szy_b, par); a.szy1 = pick(c.szy, c.szy_b, par + 1);
 }
 
-template <bool SH, bool PSV>
-static void launch_tx(Ctx &c, const StepArgs &a, unsigned blocks) {
+template <bool SH, bool PSV, bool FAST>
+static void launch_lin(Ctx &c, const StepArgs &a, unsigned blocks) {
     switch (c.tile_x) {
-        case 8: k_step<SH, PSV, 8><<<blocks, FBLOCK, 0, c.stream>>>(a); break;
-        case 32: k_step<SH, PSV, 32><<<blocks, FBLOCK, 0, c.stream>>>(a); break;
-        default: k_step<SH, PSV, 16><<<blocks, FBLOCK, 0, c.stream>>>(a); break;
+        case 8: k_step<SH, PSV, 8, FAST><<<blocks, FBLOCK, 0, c.stream>>>(a); break;
+        case 32: k_step<SH, PSV, 32, FAST><<<blocks, FBLOCK, 0, c.stream>>>(a); break;
+        default: k_step<SH, PSV, 16, FAST><<<blocks, FBLOCK, 0, c.stream>>>(a); break;
+    }
+}
+
+template <bool SH, bool PSV, bool FAST>
+static void launch_col(Ctx &c, const StepArgs &a, unsigned blocks) {
+    switch (c.tile_x) {
+        case 8: k_col<SH, PSV, 8, FAST><<<blocks, FBLOCK, 0, c.stream>>>(a); break;
+        case 32: k_col<SH, PSV, 32, FAST><<<blocks, FBLOCK, 0, c.stream>>>(a); break;
+        default: k_col<SH, PSV, 16, FAST><<<blocks, FBLOCK, 0, c.stream>>>(a); break;
+    }
+}
+
+template <bool FAST>
+static void launch_m(Ctx &c, const StepArgs &a, unsigned blocks) {
+    if (c.variant == INV_VARIANT_FUSED_LINEAR) {
+        if (c.g.sh && c.g.psv) launch_lin<true, true, FAST>(c, a, blocks);
+        else if (c.g.sh) launch_lin<true, false, FAST>(c, a, blocks);
+        else launch_lin<false, true, FAST>(c, a, blocks);
+    } else {
+        // SH and P-SV are independent systems: one launch each per step
+        if (c.g.sh) launch_col<true, false, FAST>(c, a, blocks);
+        if (c.g.psv) launch_col<false, true, FAST>(c, a, blocks);
     }
 }
 
@@ -369,9 +705,8 @@ static void fused_step(Ctx &c, StepArgs &a, int count, int it) {
     const unsigned blocks = (unsigned)(count * a.tiles_x * a.tiles_z);
     hipEvent_t ev;
     const bool p = prof_begin(c, it, &ev);
-    if (c.g.sh && c.g.psv) launch_tx<true, true>(c, a, blocks);
-    else if (c.g.sh) launch_tx<true, false>(c, a, blocks);
-    else launch_tx<false, true>(c, a, blocks);
+    if (c.math == INV_MATH_FAST) launch_m<true>(c, a, blocks);
+    else launch_m<false>(c, a, blocks);
     INV_HIP(hipGetLastError());
     if (p) prof_end(c, PROF_STEP, ev);
     c.parity ^= 1;
@@ -382,7 +717,7 @@ static void common_args(Ctx &c, StepArgs &a) {
     a.g = make_geo(c);
     a.tiles_x = tiles_x_of(c, c.tile_x);
     a.tiles_z = tiles_z_of(c);
-    a.rho = c.rho.as<float>();
+    a.rho = c.math == INV_MATH_FAST ? c.bdt.as<float>() : c.rho.as<float>();
     a.lam = c.lam.as<float>();
     a.mu = c.mu.as<float>();
     a.abx = c.abx.as<float>();
@@ -392,9 +727,14 @@ static void common_args(Ctx &c, StepArgs &a) {
     a.uz = c.uz.as<float>();
     a.uy = c.uy.as<float>();
     a.nt = c.g.nt;
+    INV_REQUIRE(a.rho && a.lam && a.mu && a.abx && a.abzb && a.abzt, INV_ERR_STATE,
+                "model or absorbing profile not allocated");
+    INV_REQUIRE((!c.g.psv || (a.ux && a.uz && c.vx.p && c.vx_b.p && c.sxx_b.p)) &&
+                    (!c.g.sh || (a.uy && c.vy.p && c.vy_b.p && c.sxy_b.p)),
+                INV_ERR_STATE, "wavefields not allocated");
 }
 
-void fused_forward(Ctx &c, int first, int count, int flags) {
+void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
     ensure_second_set(c);
     ensure_rec_lists(c);
     ensure_src_list(c, first, count);
@@ -420,13 +760,15 @@ void fused_forward(Ctx &c, int first, int count, int flags) {
     a.rec_row = c.f_rec_t.row.as<int>();
     a.rec_disp = (c.g.adj || c.g.trace_type == 1) ? 1 : 0;
     a.snap_slot_stride = (long)c.nsfe * c.nsnapf * c.N;
+    a.psv_snap_off = (c.g.sh ? 2L : 0L) * c.N;
     const bool store = (flags & INV_FWD_SNAPSHOTS) && c.g.adj && c.g.sfe > 0 && c.snap.p;
 
-    c.parity = 0;
-    fdm_zero_state(c, count);
-    int nsteps = c.g.nt;
-    if (c.step_limit > 0 && c.step_limit < nsteps) nsteps = c.step_limit;
-    for (int it = 0; it < nsteps; it++) {
+    if (it0 == 0) {
+        c.parity = 0;
+        fdm_zero_state(c, count);
+        fdm_prepare_model(c);
+    }
+    for (int it = it0; it < it1; it++) {
         a.snap = nullptr;
         if (store && (it + 1) % c.g.sfe == 0) {
             const long isfe = c.nsfe - (it + 1) / c.g.sfe;
@@ -436,7 +778,7 @@ void fused_forward(Ctx &c, int first, int count, int flags) {
     }
 }
 
-void fused_adjoint(Ctx &c, int first, int count) {
+void fused_adjoint(Ctx &c, int first, int count, int it0, int it1) {
     (void)first;
     ensure_second_set(c);
     ensure_rec_lists(c);
@@ -456,17 +798,19 @@ void fused_adjoint(Ctx &c, int first, int count) {
     a.rec_ptr = nullptr;
     a.snap = nullptr;
     const bool direct = c.g.batch == 1;
-    if (!direct) fill_dev(c, c.kslot.p, 3L * count * c.N);
-
-    c.parity = 0;
-    fdm_zero_state(c, count);
-    for (int it = 0; it < c.g.nt; it++) {
+    if (it0 == 0) {
+        if (!direct) fill_dev(c, c.kslot.p, 3L * count * c.N);
+        c.parity = 0;
+        fdm_zero_state(c, count);
+        fdm_prepare_model(c);
+    }
+    for (int it = it0; it < it1; it++) {
         fused_step(c, a, count, it);
         if (it % c.g.sfe == 0)
             fdm_interact(c, count, it, cur_field(c, c.vx, c.vx_b), cur_field(c, c.vz, c.vz_b),
                          cur_field(c, c.vy, c.vy_b));
     }
-    if (!direct) fdm_kslot_reduce(c, count);
+    if (!direct && it1 == c.g.nt) fdm_kslot_reduce(c, count);
 }
 
 }  // namespace inv

@@ -98,7 +98,9 @@ struct Ctx {
     DevBuf vx_b, vz_b, sxx_b, szz_b, sxz_b, vy_b, sxy_b, szy_b;
     int parity = 0;
     int variant = 0;   // 0 fused step kernel, 1 velocity/stress kernel pair
-    int tile_x = 16;   // rows per fused tile (8, 16 or 32)
+    int tile_x = 8;    // rows per fused tile (8, 16 or 32)
+    int math = INV_MATH_FAST;
+    DevBuf bdt;        // dt / rho, rebuilt at the start of each fast-mode run
     DevBuf snap;   // [batch][nsfe][nsnapf][N]
     DevBuf out;    // [batch][ncomp][nrec][nt]
     DevBuf adstf;  // [batch][ncomp][nrec][nt]
@@ -120,6 +122,10 @@ struct Ctx {
     PointList f_src_v, f_rec_v, f_rec_t;
     int f_src_first = -2, f_src_count = -1, f_src_tx = -1, f_rec_tx = -1;
 
+    // envelope misfit: FFT twiddles and radix plan for length nt
+    DevBuf env_w, env_fac, env_scratch;
+    int env_n = -1, env_nfac = 0;
+
     // filter
     DevBuf gauss_w, gauss_tmp, gauss_sx, gauss_sz;
     int gauss_sigma = -1, gauss_nx = -1, gauss_nz = -1, gauss_m = 0;
@@ -127,23 +133,29 @@ struct Ctx {
     std::vector<void *> user_allocs;
     Prof prof;
 
+    // an incremental run (inv_*_begin / inv_run_advance): 0 none, 1 forward,
+    // 2 adjoint
+    int run_kind = 0, run_first = 0, run_count = 0, run_flags = 0, run_it = 0;
+
     int slot_offset_traces() const { return ncomp * nrec * g.nt; }
 };
 
 // fdm.hip -------------------------------------------------------------------
 void build_point_list(Ctx &c, PointList &pl, int nsets, const std::vector<std::vector<int>> &cells,
                       const std::vector<std::vector<int>> &rows);
-void fdm_forward(Ctx &c, int first, int count, int flags);
-void fdm_adjoint(Ctx &c, int first, int count);
+// propagate steps [it0, it1) of a run; it0 == 0 starts from rest
+void fdm_forward(Ctx &c, int first, int count, int flags, int it0, int it1);
+void fdm_adjoint(Ctx &c, int first, int count, int it0, int it1);
 void fdm_zero_state(Ctx &c, int count);
+void fdm_prepare_model(Ctx &c);
 void fdm_kslot_reduce(Ctx &c, int count);
 Geo make_geo(const Ctx &c);
 void fill_dev(Ctx &c, void *p, long n, float v = 0.0f);
 // interaction kernels for adjoint step it (it % sfe == 0) with the adjoint
 // velocities vx/vz/vy of the current state set
 void fdm_interact(Ctx &c, int count, int it, const float *vx, const float *vz, const float *vy);
-void fused_forward(Ctx &c, int first, int count, int flags);
-void fused_adjoint(Ctx &c, int first, int count);
+void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1);
+void fused_adjoint(Ctx &c, int first, int count, int it0, int it1);
 void fused_invalidate_lists(Ctx &c);
 // device pointer of the current buffer of a ping-pong field
 inline float *cur_field(Ctx &c, DevBuf &a, DevBuf &b) {
@@ -157,6 +169,8 @@ void prof_release(Ctx &c);
 // misfit.hip ----------------------------------------------------------------
 void misfit_waveform(Ctx &c, int count, int first);
 void misfit_envelope(Ctx &c, int count, int first);
+void envelope_trace(Ctx &c, const float *syn, const float *obs, float *adstf, double *chi_dev);
+double misfit_trace(Ctx &c, int type, const float *syn, const float *obs, float *adstf);
 
 // linalg.hip ----------------------------------------------------------------
 void gauss_filter(Ctx &c, float *field, int nx, int nz, int sigma);

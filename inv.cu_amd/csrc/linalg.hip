@@ -80,6 +80,7 @@ void gauss_filter(Ctx &c, float *field, int nx, int nz, int sigma) {
             for (int n = 0; n <= j; n++) acc += w[j - n];
             sz[j] = acc;
         }
+        INV_HIP(hipStreamSynchronize(c.stream));
         c.gauss_w.ensure(nmax * sizeof(float));
         c.gauss_sx.ensure(nx * sizeof(float));
         c.gauss_sz.ensure(nz * sizeof(float));

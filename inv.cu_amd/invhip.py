@@ -20,6 +20,7 @@ INV_SNAP_F32, INV_SNAP_F16, INV_SNAP_BF16 = 0, 1, 2
 INV_FWD_SNAPSHOTS = 1
 INV_PROF_VELOCITY, INV_PROF_STRESS, INV_PROF_INTERACT, INV_PROF_STEP = 0, 1, 2, 3
 INV_VARIANT_FUSED, INV_VARIANT_SPLIT = 0, 1
+INV_MATH_EXACT, INV_MATH_FAST = 0, 1
 
 STATUS = {0: "INV_OK", -1: "INV_ERR_ARG", -2: "INV_ERR_HIP", -3: "INV_ERR_STATE",
           -4: "INV_ERR_NOMEM", -5: "INV_ERR_UNSUPPORTED"}
@@ -96,6 +97,11 @@ SIGNATURES = {
     "inv_prof_reset": [_vp],
     "inv_prof_read": [_vp, _i, _vp, _vp],
     "inv_set_variant": [_vp, _i, _i],
+    "inv_forward_begin": [_vp, _i, _i, _i],
+    "inv_adjoint_begin": [_vp, _i, _i],
+    "inv_run_advance": [_vp, _i, _vp],
+    "inv_misfit_trace": [_vp, _i, _vp, _vp, _vp, _vp],
+    "inv_set_math": [_vp, _i],
 }
 
 _lib = None
@@ -247,6 +253,24 @@ class Context:
     def adjoint(self, first, count=1):
         self._chk(self.L.inv_adjoint(self.h, first, count))
 
+    def forward_begin(self, first, count=1, snapshots=True):
+        self._chk(self.L.inv_forward_begin(self.h, first, count,
+                                           INV_FWD_SNAPSHOTS if snapshots else 0))
+
+    def adjoint_begin(self, first, count=1):
+        self._chk(self.L.inv_adjoint_begin(self.h, first, count))
+
+    def advance(self, nsteps):
+        pos = C.c_int()
+        self._chk(self.L.inv_run_advance(self.h, nsteps, C.byref(pos)))
+        return pos.value
+
+    def misfit_trace(self, kind, syn_dev, obs_dev, adstf_dev):
+        chi = C.c_double()
+        self._chk(self.L.inv_misfit_trace(self.h, kind, syn_dev, obs_dev, adstf_dev,
+                                          C.byref(chi)))
+        return chi.value
+
     def kernels_zero(self):
         self._chk(self.L.inv_kernels_zero(self.h))
 
@@ -362,6 +386,9 @@ class Context:
 
     def set_step_limit(self, n):
         self._chk(self.L.inv_set_step_limit(self.h, n))
+
+    def set_math(self, mode):
+        self._chk(self.L.inv_set_math(self.h, mode))
 
     def set_variant(self, variant, tile_x=0):
         self._chk(self.L.inv_set_variant(self.h, variant, tile_x))

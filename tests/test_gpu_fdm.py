@@ -1,8 +1,9 @@
 """GPU parity of the propagation path (through the C ABI) against the oracle.
 
-Bars: plain one-shot FP32 path bit-exact with the FP32 oracle (same
-expression order, no fused multiply-add on either side); every FP32 wavefield
-within 1e-5 relative L2 of the FP64 oracle; kernels within 1e-4.
+Bars: in exact math mode the FP32 results equal the FP32 oracle's (same
+expression order, IEEE divisions, no fused multiply-add on either side); in
+fast math mode (coefficient multiplies) every FP32 wavefield is within 1e-5
+relative L2 of the FP32 and of the FP64 oracle.  Kernels within 1e-4.
 """
 import numpy as np
 import pytest
@@ -12,13 +13,29 @@ from conftest import inv, oracle_problem, oracle_traces, rel
 pytestmark = pytest.mark.gpu
 
 CASES = [dict(sh=1, psv=0), dict(sh=0, psv=1), dict(sh=1, psv=1)]
-# (variant, tile_x): fused step kernel with each tile height, and the
-# velocity/stress kernel pair
-VARIANTS = [(0, 8), (0, 16), (0, 32), (1, 0)]
+# (variant, tile_x, math): fused column kernel (0), velocity/stress kernel
+# pair (1), fused linear-index kernel (2), in exact (0) and fast (1) math
+VARIANTS = [(0, 8, 0), (0, 16, 0), (0, 32, 0), (1, 0, 0), (2, 8, 0),
+            (0, 8, 1), (0, 16, 1), (1, 0, 1), (2, 16, 1)]
 
 
 def _vid(v):
-    return "split" if v[0] == 1 else f"fused{v[1]}"
+    kind = {0: f"col{v[1]}", 1: "split", 2: f"lin{v[1]}"}[v[0]]
+    return kind + ("-fast" if v[2] else "-exact")
+
+
+def setup(ctx, v):
+    ctx.set_variant(v[0], v[1])
+    ctx.set_math(v[2])
+
+
+def same(a, ref, fast, tol=1e-5):
+    """Exact mode: equal values.  Fast mode: within tol relative L2."""
+    if fast:
+        e = rel(a, ref)
+        assert e < tol, f"rel L2 {e:.3e} >= {tol}"
+    else:
+        assert np.array_equal(a, ref), f"not equal, rel L2 {rel(a, ref):.3e}"
 
 
 def boundary_problem(**kw):
@@ -42,7 +59,8 @@ def test_forward_plain_matches_oracle(gpu, case, variant):
     P = inv.problems.tiny(**case)
     print(f"forward plain {_ids(case)} {_vid(variant)} {P.nx}x{P.nz} nt={P.nt}", flush=True)
     ctx = P.context(batch=1)
-    ctx.set_variant(*variant)
+    setup(ctx, variant)
+    fast = variant[2]
     O = oracle_problem(P)
     for isrc in range(P.nsrc):
         ctx.forward(isrc, 1, snapshots=True)
@@ -51,9 +69,9 @@ def test_forward_plain_matches_oracle(gpu, case, variant):
         sn = ctx.snapshots_get(0)
         otr, osn, ost = O.forward(isrc, snapshots=True)
         ref = oracle_traces(otr, P)
-        assert np.array_equal(tr, ref), f"traces not bit-exact, rel {rel(tr, ref):.3e}"
-        assert np.array_equal(st, ost), f"state not bit-exact, rel {rel(st, ost):.3e}"
-        assert np.array_equal(sn, osn), f"snapshots not bit-exact, rel {rel(sn, osn):.3e}"
+        same(tr, ref, fast)
+        same(st, ost, fast)
+        same(sn, osn, fast)
         o64, _, s64 = O.forward(isrc, dtype=np.float64)
         assert rel(tr, oracle_traces(o64, P)) < 1e-5
         assert rel(st, s64) < 1e-5
@@ -64,6 +82,7 @@ def test_forward_all_sources_and_velocity_traces(gpu):
     """runForward(-1) fires every source; adj = 0 with trace_type 0 records v."""
     P = inv.problems.tiny(sh=1, psv=1)
     ctx = P.context(batch=1, adj=0)
+    ctx.set_math(0)
     O = oracle_problem(P, adj=0)
     ctx.forward(-1, 1, snapshots=False)
     tr = ctx.traces_get(0)
@@ -75,6 +94,7 @@ def test_forward_all_sources_and_velocity_traces(gpu):
 def test_displacement_traces(gpu):
     P = inv.problems.tiny(sh=0, psv=1).with_(trace_type=1)
     ctx = P.context(batch=1, adj=0)
+    ctx.set_math(0)
     O = oracle_problem(P, adj=0)
     ctx.forward(1, 1, snapshots=False)
     otr, _, _ = O.forward(1)
@@ -89,14 +109,15 @@ def test_tile_edges_match_oracle(gpu, case, variant):
     P = boundary_problem(**case)
     print(f"tile edges {_ids(case)} {_vid(variant)}", flush=True)
     ctx = P.context(batch=2)
-    ctx.set_variant(*variant)
+    setup(ctx, variant)
+    fast = variant[2]
     O = oracle_problem(P)
     ctx.forward(1, 2, snapshots=True)
     for s in range(2):
         otr, osn, ost = O.forward(1 + s, snapshots=True)
-        assert np.array_equal(ctx.traces_get(s), oracle_traces(otr, P))
-        assert np.array_equal(ctx.state_get(s), ost)
-        assert np.array_equal(ctx.snapshots_get(s), osn)
+        same(ctx.traces_get(s), oracle_traces(otr, P), fast)
+        same(ctx.state_get(s), ost, fast)
+        same(ctx.snapshots_get(s), osn, fast)
     ctx.close()
 
 
@@ -107,8 +128,8 @@ def test_batched_forward_equals_plain(gpu, case, variant):
     P = inv.problems.tiny(nsrc=4, **case)
     plain = P.context(batch=1)
     batch = P.context(batch=4)
-    plain.set_variant(*variant)
-    batch.set_variant(*variant)
+    setup(plain, variant)
+    setup(batch, variant)
     batch.forward(0, 4, snapshots=True)
     for s in range(4):
         plain.forward(s, 1, snapshots=True)
@@ -124,7 +145,8 @@ def test_batched_forward_equals_plain(gpu, case, variant):
 def test_adjoint_plain_matches_oracle(gpu, case, variant):
     P = boundary_problem(**case)
     ctx = P.context(batch=1)
-    ctx.set_variant(*variant)
+    setup(ctx, variant)
+    fast = variant[2]
     O = oracle_problem(P)
     ctx.kernels_zero()
     ks = [np.zeros(P.N, np.float32) for _ in range(3)]
@@ -136,13 +158,17 @@ def test_adjoint_plain_matches_oracle(gpu, case, variant):
         ctx.adjoint(isrc, 1)
         _, osn, _ = O.forward(isrc, snapshots=True)
         comps = (["y"] if P.sh else []) + (["x", "z"] if P.psv else [])
+        if fast:
+            # drive the oracle with the GPU's own snapshots: this compares the
+            # adjoint alone, not the forward drift before it
+            osn = ctx.snapshots_get(0)
         O.adjoint({c: adstf[i] for i, c in enumerate(comps)}, osn, kernels=ks)
-        assert np.array_equal(ctx.state_get(0), O.adjoint(
-            {c: adstf[i] for i, c in enumerate(comps)}, osn)[3])
+        same(ctx.state_get(0), O.adjoint({c: adstf[i] for i, c in enumerate(comps)}, osn)[3],
+             fast)
     kl, km, kr = ctx.kernels_get()
     for a, b, name in ((kl, ks[0], "k_lambda"), (km, ks[1], "k_mu"), (kr, ks[2], "k_rho")):
         if not P.psv and name == "k_lambda":
             assert not np.any(a)
             continue
-        assert np.array_equal(a, b), f"{name} not bit-exact, rel {rel(a, b):.3e}"
+        same(a, b, fast, tol=1e-4)
     ctx.close()
