@@ -33,6 +33,12 @@ HBM_PEAK_GBPS = 8000.0
 # the 7 wavefields (vx, vz, ux, uz, sxx, szz, sxz) read and written once and
 # the 3 material fields read once, FP32.
 PSV_STEP_BYTES = 4 * (2 * 7 + 3)
+# HBM bytes per cell of one launch of the default step kernel (fused column
+# kernel, tile 58x8, fast math, P-SV), measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE and scaled by a 256 MiB copy (fetch x2.0, write x1.0); see
+# profiles/k_col_psv_marmousi737x240_16shots.md.  Only reported for that
+# configuration.
+MEASURED_TRAFFIC_PER_CELL = 69.33
 
 
 def parse():
@@ -51,6 +57,7 @@ def parse():
     ap.add_argument("--variant", type=int, default=0, help="0 fused step kernel, 1 kernel pair")
     ap.add_argument("--tile-x", type=int, default=8, help="fused tile rows (8, 16, 32)")
     ap.add_argument("--math", type=int, default=1, help="0 exact (IEEE divisions), 1 fast")
+    ap.add_argument("--graphs", type=int, default=1, help="capture time loops in HIP graphs")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -97,6 +104,7 @@ def main():
     ctx = P.context(batch=B, model="true")
     ctx.set_variant(a.variant, a.tile_x)
     ctx.set_math(a.math)
+    ctx.set_graphs(a.graphs)
     # observed data: forward on the true model (Misfit::init, misfit.h:135-148)
     for first in range(0, P.nsrc, B):
         n = min(B, P.nsrc - first)
@@ -135,9 +143,21 @@ def main():
     nv, tv = ctx.prof_read(inv.invhip.INV_PROF_VELOCITY)
     ns, ts = ctx.prof_read(inv.invhip.INV_PROF_STRESS)
     nf, tf = ctx.prof_read(inv.invhip.INV_PROF_STEP)
+    nfr, tfr = ctx.prof_read(inv.invhip.INV_PROF_FWD_RUN)
+    nar, tar = ctx.prof_read(inv.invhip.INV_PROF_ADJ_RUN)
     roof = None
     cells = B * P.N
-    if nf:
+    step_s = None
+    if nfr:
+        # replayed forward graphs: every launch in them but a handful of
+        # set-up fills is the step kernel, so time / steps is its duration
+        step_s = tfr / nfr / 1e3
+        kernel = "k_col<false,true,TX,FAST> (fused P-SV time step), from forward-graph replays"
+        avg = {"step": round(step_s * 1e6, 2)}
+        samples = {"forward_steps": nfr}
+        if nar:
+            avg["adjoint_step_incl_interaction"] = round(tar / nar * 1e3, 2)
+    elif nf:
         step_s = tf / nf / 1e3
         kernel = "k_col<false,true,TX,FAST> (fused P-SV time step)"
         avg = {"step": round(tf / nf * 1e3, 2)}
@@ -147,10 +167,12 @@ def main():
         kernel = "P-SV stencil step (velocity + stress launch pair)"
         avg = {"velocity": round(tv / nv * 1e3, 2), "stress": round(ts / ns * 1e3, 2)}
         samples = {"velocity": nv, "stress": ns}
-    if nf or (nv and ns):
+    if step_s:
         achieved = PSV_STEP_BYTES * cells / step_s / 1e9
+        profiled = (a.variant, a.tile_x, a.math, B) == (0, 8, 1, 16) and nfr
+        traffic = round(MEASURED_TRAFFIC_PER_CELL * cells) if profiled else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "kernel": kernel, "alg_bytes_per_launch": PSV_STEP_BYTES * cells,
                 "avg_us": avg, "samples": samples}
     ni, ti = ctx.prof_read(inv.invhip.INV_PROF_INTERACT)
@@ -161,6 +183,7 @@ def main():
         pctx = P.context(batch=1, model="true")
         pctx.set_variant(a.variant, a.tile_x)
         pctx.set_math(a.math)
+        pctx.set_graphs(False)  # the plain path: one shot per launch, no graph
         for s in range(P.nsrc):
             pctx.forward(s, 1, snapshots=False)
             pctx.obs_from_traces(s, 1)
@@ -174,7 +197,7 @@ def main():
         pctx.close()
         plain = {"ms_per_gradient": round(pms, 2), "gcell_per_s": round(work / pms / 1e6, 2),
                  "gradients_per_hr": round(3.6e6 / pms, 1),
-                 "note": "one shot per launch, same kernels, no batching"}
+                 "note": "one shot per launch, same kernels, no batching, no graph"}
         print(f"# plain path: {pms:.1f} ms per gradient", file=sys.stderr, flush=True)
 
     cpu = None if a.no_cpu else cpu_baseline(P, a.cpu_shots)

@@ -201,16 +201,18 @@ int inv_memcpy_d2h(inv_ctx *ctx, void *host, const void *dev, size_t bytes);
 int inv_state_get(inv_ctx *ctx, int slot, float *host);
 int inv_snapshots_get(inv_ctx *ctx, int slot, float *host);
 /* ---- measurement -------------------------------------------------------
- * Sampled kernel timing: with every > 0, each propagation kernel launched on
- * a time step it with it % every == 0 is bracketed by HIP events on the
- * context stream; inv_prof_read returns the number of sampled launches and
- * their summed duration.  No counterpart in the reference (which times only
+ * Sampled kernel timing: with every > 0, each propagation kernel launched
+ * directly on a time step it with it % every == 0 is bracketed by HIP events
+ * on the context stream, and so is every replay of a captured run (counted
+ * in time steps); inv_prof_read returns the count and the summed duration.  No counterpart in the reference (which times only
  * the whole run, optimizer/optimizer.h:432,456). */
 enum inv_prof_kernel {
     INV_PROF_VELOCITY = 0,  /* velocity half-step (kernel-pair variant) */
     INV_PROF_STRESS = 1,    /* stress half-step (kernel-pair variant) */
     INV_PROF_INTERACT = 2,  /* adjoint kernel accumulation */
-    INV_PROF_STEP = 3       /* whole time step (fused variant) */
+    INV_PROF_STEP = 3,      /* whole time step (fused variant) */
+    INV_PROF_FWD_RUN = 4,   /* replayed forward graphs (count = time steps) */
+    INV_PROF_ADJ_RUN = 5    /* replayed adjoint graphs (count = time steps) */
 };
 int inv_prof_enable(inv_ctx *ctx, int every);
 int inv_prof_reset(inv_ctx *ctx);
@@ -231,6 +233,10 @@ int inv_set_variant(inv_ctx *ctx, int variant, int tile_x);
  * the instructions. */
 enum inv_math { INV_MATH_EXACT = 0, INV_MATH_FAST = 1 };
 int inv_set_math(inv_ctx *ctx, int mode);
+
+/* HIP graphs (default on): a whole forward or adjoint run is captured once
+ * per (run, shot range, flags) and replayed; setting changes re-capture. */
+int inv_set_graphs(inv_ctx *ctx, int on);
 
 /* Run only the first nsteps steps of the next forward (0 = all). */
 int inv_set_step_limit(inv_ctx *ctx, int nsteps);

@@ -148,6 +148,21 @@ static void check_slots(Ctx &c, int first, int count) {
 
 static void sync(Ctx &c) { INV_HIP(hipStreamSynchronize(c.stream)); }
 
+static void drop_graphs(Ctx &c) {
+    (void)hipStreamSynchronize(c.stream);
+    for (auto &kv : c.graphs) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    c.graphs.clear();
+}
+
+// Everything a captured loop bakes in changes: forget the graphs.
+static void invalidate(Ctx &c) {
+    drop_graphs(c);
+    c.gen++;
+}
+
 }  // namespace inv
 
 using namespace inv;
@@ -259,13 +274,15 @@ int inv_ctx_destroy(inv_ctx *ctx) {
                       &c.k_rho, &c.kslot, &c.vx, &c.vz, &c.ux, &c.uz, &c.sxx, &c.szz, &c.sxz,
                       &c.vy, &c.uy, &c.sxy, &c.szy, &c.snap, &c.out, &c.adstf, &c.obs, &c.chi,
                       &c.taper, &c.red, &c.scal, &c.stf, &c.rec_list.ptr, &c.rec_list.cell,
-                      &c.rec_list.row, &c.src_list.ptr, &c.src_list.cell, &c.src_list.row,
+                      &c.rec_list.row,
                       &c.gauss_w, &c.gauss_tmp, &c.gauss_sx, &c.gauss_sz, &c.vx_b, &c.vz_b,
                       &c.sxx_b, &c.szz_b, &c.sxz_b, &c.vy_b, &c.sxy_b, &c.szy_b,
-                      &c.f_src_v.ptr, &c.f_src_v.cell, &c.f_src_v.row, &c.f_rec_v.ptr,
+                      &c.f_rec_v.ptr,
                       &c.f_rec_v.cell, &c.f_rec_v.row, &c.f_rec_t.ptr, &c.f_rec_t.cell,
                       &c.f_rec_t.row, &c.env_w, &c.env_fac, &c.env_scratch, &c.bdt};
     for (DevBuf *b : bufs) b->release();
+    for (auto &kv : c.src_lists) release_list(kv.second);
+    drop_graphs(c);
     for (void *p : c.user_allocs) (void)hipFree(p);
     prof_release(c);
     (void)hipStreamDestroy(ctx->stream);
@@ -313,8 +330,8 @@ int inv_grid_set(inv_ctx *ctx, const inv_grid_desc *d) {
         build_taper(c);
         alloc_slots(c);
         alloc_traces(c);
-        c.src_list_first = -2;
         c.parity = 0;
+        invalidate(c);
         fused_invalidate_lists(c);
         if (c.have_rec) build_receivers(c);
     });
@@ -447,7 +464,7 @@ int inv_sources_set(inv_ctx *ctx, int nsrc, const int *ix, const int *iz, const 
         INV_HIP(hipMemcpy(c.stf.as<char>() + b, stf_y, b, hipMemcpyHostToDevice));
         INV_HIP(hipMemcpy(c.stf.as<char>() + 2 * b, stf_z, b, hipMemcpyHostToDevice));
         c.have_src = true;
-        c.src_list_first = -2;
+        invalidate(c);
         fused_invalidate_lists(c);
         alloc_traces(c);
     });
@@ -465,13 +482,15 @@ int inv_receivers_set(inv_ctx *ctx, int nrec, const int *ix, const int *iz) {
         c.rec_ix.assign(ix, ix + nrec);
         c.rec_iz.assign(iz, iz + nrec);
         build_receivers(c);
+        invalidate(c);
         fused_invalidate_lists(c);
         c.have_rec = true;
         alloc_traces(c);
     });
 }
 
-static void run_steps(Ctx &c, int kind, int first, int count, int flags, int it0, int it1) {
+static void run_steps_direct(Ctx &c, int kind, int first, int count, int flags, int it0,
+                             int it1) {
     const bool split = c.variant == INV_VARIANT_SPLIT;
     if (kind == 1) {
         if (split) fdm_forward(c, first, count, flags, it0, it1);
@@ -484,6 +503,62 @@ static void run_steps(Ctx &c, int kind, int first, int count, int flags, int it0
 
 static int forward_end(const Ctx &c) {
     return (c.step_limit > 0 && c.step_limit < c.g.nt) ? c.step_limit : c.g.nt;
+}
+
+// A whole forward or adjoint run is captured once into a HIP graph (every
+// launch of the time loop, nt to nt + nt/sfe kernels) and replayed after
+// that; runs with event sampling, partial runs and stepping go direct.
+static void run_steps(Ctx &c, int kind, int first, int count, int flags, int it0, int it1) {
+    const int end = kind == 1 ? forward_end(c) : c.g.nt;
+    if (!c.use_graphs || it0 != 0 || it1 != end) {
+        run_steps_direct(c, kind, first, count, flags, it0, it1);
+        return;
+    }
+    const std::string key = std::to_string(kind) + "/" + std::to_string(first) + "/" +
+                            std::to_string(count) + "/" + std::to_string(flags) + "/" +
+                            std::to_string(end);
+    auto found = c.graphs.find(key);
+    if (found == c.graphs.end() || found->second.gen != c.gen) {
+        if (found != c.graphs.end()) {
+            sync(c);
+            if (found->second.exec) (void)hipGraphExecDestroy(found->second.exec);
+            if (found->second.graph) (void)hipGraphDestroy(found->second.graph);
+            c.graphs.erase(found);
+        }
+        // everything that allocates, uploads or synchronises happens before
+        // the capture; inside it the run only launches kernels (no event
+        // sampling inside a capture)
+        if (c.variant == INV_VARIANT_SPLIT) fdm_prepare(c, kind, first, count);
+        else fused_prepare(c, kind, first, count);
+        sync(c);
+        Ctx::GraphEntry e;
+        const int every = c.prof.every;
+        c.prof.every = 0;
+        INV_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            run_steps_direct(c, kind, first, count, flags, it0, it1);
+        } catch (...) {
+            c.prof.every = every;
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(c.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        c.prof.every = every;
+        INV_HIP(hipStreamEndCapture(c.stream, &e.graph));
+        INV_HIP(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+        e.gen = c.gen;
+        found = c.graphs.emplace(key, e).first;
+    }
+    // with sampling on, each replay is bracketed by events (units = steps)
+    hipEvent_t ev;
+    const bool p = prof_begin_any(c, &ev);
+    INV_HIP(hipGraphLaunch(found->second.exec, c.stream));
+    if (p) prof_end(c, kind == 1 ? PROF_FWD_RUN : PROF_ADJ_RUN, ev, end);
+    // the replay does not run the host loop: set the state-set parity it ends
+    // on (the fused kernels alternate sets each step, the kernel pair works in
+    // place)
+    c.parity = c.variant == INV_VARIANT_SPLIT ? 0 : (end & 1);
 }
 
 int inv_forward(inv_ctx *ctx, int first, int count, int flags) {
@@ -879,6 +954,7 @@ int inv_set_variant(inv_ctx *ctx, int variant, int tile_x) {
         sync(c);
         c.variant = variant;
         if (tile_x) c.tile_x = tile_x;
+        invalidate(c);
         fused_invalidate_lists(c);
     });
 }
@@ -889,6 +965,15 @@ int inv_set_math(inv_ctx *ctx, int mode) {
                     "unknown math mode");
         sync(c);
         c.math = mode;
+        invalidate(c);
+    });
+}
+
+int inv_set_graphs(inv_ctx *ctx, int on) {
+    return guard(ctx, [&](Ctx &c) {
+        sync(c);
+        c.use_graphs = on != 0;
+        invalidate(c);
     });
 }
 

@@ -401,10 +401,17 @@ bool prof_begin(Ctx &c, int it, hipEvent_t *a) {
     return true;
 }
 
-void prof_end(Ctx &c, int kind, hipEvent_t a) {
+void prof_end(Ctx &c, int kind, hipEvent_t a, int units) {
     hipEvent_t b = prof_event(c);
     INV_HIP(hipEventRecord(b, c.stream));
-    c.prof.pending.push_back({kind, a, b});
+    c.prof.pending.push_back({kind, a, b, units});
+}
+
+bool prof_begin_any(Ctx &c, hipEvent_t *a) {
+    if (c.prof.every <= 0 || c.prof.pending.size() >= PROF_MAX_PENDING) return false;
+    *a = prof_event(c);
+    INV_HIP(hipEventRecord(*a, c.stream));
+    return true;
 }
 
 void prof_resolve(Ctx &c) {
@@ -413,7 +420,7 @@ void prof_resolve(Ctx &c) {
     for (auto &r : c.prof.pending) {
         float ms = 0;
         INV_HIP(hipEventElapsedTime(&ms, r.a, r.b));
-        c.prof.count[r.kind] += 1;
+        c.prof.count[r.kind] += r.units;
         c.prof.ms[r.kind] += ms;
         c.prof.pool.push_back(r.a);
         c.prof.pool.push_back(r.b);
@@ -485,23 +492,20 @@ static void make_common(Ctx &c, VelArgs &va, StrArgs &sa) {
     va.ntiles = c.rec_list.ntiles;
 }
 
+void release_list(PointList &pl) {
+    pl.ptr.release();
+    pl.cell.release();
+    pl.row.release();
+}
+
+void fdm_prepare(Ctx &c, int kind, int first, int count) {
+    if (kind == 1) source_list(c, first, count, 0);
+}
+
 void fdm_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
     // point lists of this batch's sources: set b = shot first+b (or every
     // source in set 0 for first == -1, runForward(-1) of fdm.h:171)
-    if (c.src_list_first != first || c.src_list_count != count) {
-        std::vector<std::vector<int>> cells(count), rows(count);
-        for (int b = 0; b < count; b++) {
-            for (int is = 0; is < c.nsrc; is++) {
-                if (first < 0 || is == first + b) {
-                    cells[b].push_back(c.src_ix[is] * c.g.nz + c.src_iz[is]);
-                    rows[b].push_back(is);
-                }
-            }
-        }
-        build_point_list(c, c.src_list, count, cells, rows);
-        c.src_list_first = first;
-        c.src_list_count = count;
-    }
+    PointList &sl = source_list(c, first, count, 0);
     VelArgs va;
     StrArgs sa;
     make_common(c, va, sa);
@@ -510,11 +514,11 @@ void fdm_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
     va.inj_y = c.stf.as<float>() + nsrc * nt;
     va.inj_z = c.stf.as<float>() + 2 * nsrc * nt;
     va.inj_slot_stride = 0;
-    va.inj_ptr = c.src_list.ptr.as<int>();
-    va.inj_cell = c.src_list.cell.as<int>();
-    va.inj_row = c.src_list.row.as<int>();
+    va.inj_ptr = sl.ptr.as<int>();
+    va.inj_cell = sl.cell.as<int>();
+    va.inj_row = sl.row.as<int>();
     va.inj_per_slot = 1;
-    va.ntiles = c.src_list.ntiles;
+    va.ntiles = sl.ntiles;
     const long tr = (long)c.nrec * nt;
     const int cy = 0, cx = c.g.sh ? 1 : 0, cz = cx + 1;
     va.out_y = c.out.as<float>() + cy * tr;

@@ -613,10 +613,12 @@ static void build_tiled(Ctx &c, PointList &pl, int nsets, const std::vector<std:
 }
 
 void fused_invalidate_lists(Ctx &c) {
-    c.f_src_first = -2;
-    c.f_src_tx = -1;
+    INV_HIP(hipStreamSynchronize(c.stream));
+    for (auto &kv : c.src_lists) release_list(kv.second);
+    c.src_lists.clear();
     c.f_rec_tx = -1;
 }
+
 
 static void ensure_rec_lists(Ctx &c) {
     if (c.f_rec_tx == list_key(c)) return;
@@ -630,8 +632,11 @@ static void ensure_rec_lists(Ctx &c) {
     c.f_rec_tx = list_key(c);
 }
 
-static void ensure_src_list(Ctx &c, int first, int count) {
-    if (c.f_src_first == first && c.f_src_count == count && c.f_src_tx == list_key(c)) return;
+PointList &source_list(Ctx &c, int first, int count, int key) {
+    const long long k = ((long long)(first + 2) << 40) | ((long long)count << 20) | key;
+    auto found = c.src_lists.find(k);
+    if (found != c.src_lists.end()) return found->second;
+    PointList &pl = c.src_lists[k];
     std::vector<std::vector<int>> cells(count), rows(count);
     for (int b = 0; b < count; b++)
         for (int is = 0; is < c.nsrc; is++)
@@ -639,10 +644,9 @@ static void ensure_src_list(Ctx &c, int first, int count) {
                 cells[b].push_back(c.src_ix[is] * c.g.nz + c.src_iz[is]);
                 rows[b].push_back(is);
             }
-    build_tiled(c, c.f_src_v, count, cells, rows, c.tile_x, 1, 2);
-    c.f_src_first = first;
-    c.f_src_count = count;
-    c.f_src_tx = list_key(c);
+    if (key == 0) build_point_list(c, pl, count, cells, rows);
+    else build_tiled(c, pl, count, cells, rows, c.tile_x, 1, 2);
+    return pl;
 }
 
 static void ensure_second_set(Ctx &c) {
@@ -654,6 +658,12 @@ static void ensure_second_set(Ctx &c) {
     if (c.g.sh) {
         c.vy_b.ensure(b); c.sxy_b.ensure(b); c.szy_b.ensure(b);
     }
+}
+
+void fused_prepare(Ctx &c, int kind, int first, int count) {
+    ensure_second_set(c);
+    ensure_rec_lists(c);
+    if (kind == 1) source_list(c, first, count, list_key(c));
 }
 
 static void set_state_ptrs(Ctx &c, StepArgs &a, int par) {
@@ -735,9 +745,8 @@ static void common_args(Ctx &c, StepArgs &a) {
 }
 
 void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
-    ensure_second_set(c);
-    ensure_rec_lists(c);
-    ensure_src_list(c, first, count);
+    fused_prepare(c, 1, first, count);
+    PointList &sl = source_list(c, first, count, list_key(c));
     StepArgs a;
     common_args(c, a);
     const long nt = c.g.nt, nsrc = c.nsrc;
@@ -745,9 +754,9 @@ void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
     a.inj_y = c.stf.as<float>() + nsrc * nt;
     a.inj_z = c.stf.as<float>() + 2 * nsrc * nt;
     a.inj_slot_stride = 0;
-    a.inj_ptr = c.f_src_v.ptr.as<int>();
-    a.inj_cell = c.f_src_v.cell.as<int>();
-    a.inj_row = c.f_src_v.row.as<int>();
+    a.inj_ptr = sl.ptr.as<int>();
+    a.inj_cell = sl.cell.as<int>();
+    a.inj_row = sl.row.as<int>();
     a.inj_per_slot = 1;
     const long tr = (long)c.nrec * nt;
     const int cy = 0, cx = c.g.sh ? 1 : 0, cz = cx + 1;
@@ -779,9 +788,7 @@ void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
 }
 
 void fused_adjoint(Ctx &c, int first, int count, int it0, int it1) {
-    (void)first;
-    ensure_second_set(c);
-    ensure_rec_lists(c);
+    fused_prepare(c, 2, first, count);
     StepArgs a;
     common_args(c, a);
     const long nt = c.g.nt;

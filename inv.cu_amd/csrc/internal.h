@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -56,12 +57,16 @@ constexpr int TILE_Z = 64;  // one wavefront along z (the contiguous axis)
 constexpr int TILE_X = 4;   // wavefronts per block along x
 
 // Sampled kernel timing with HIP events on the context stream.
-enum ProfKind { PROF_VELOCITY = 0, PROF_STRESS = 1, PROF_INTERACT = 2, PROF_STEP = 3, PROF_NKIND = 4 };
+enum ProfKind {
+    PROF_VELOCITY = 0, PROF_STRESS = 1, PROF_INTERACT = 2, PROF_STEP = 3,
+    PROF_FWD_RUN = 4, PROF_ADJ_RUN = 5, PROF_NKIND = 6
+};
 struct Prof {
     int every = 0;  // sample every n-th time step (0 = off)
     struct Rec {
         int kind;
         hipEvent_t a, b;
+        int units;  // time steps covered by the interval
     };
     std::vector<Rec> pending;
     std::vector<hipEvent_t> pool;
@@ -115,12 +120,25 @@ struct Ctx {
     std::vector<int> src_ix, src_iz, rec_ix, rec_iz;
     DevBuf stf;  // [3][nsrc][nt]  x, y, z
     PointList rec_list;  // receivers, one set shared by all slots
-    PointList src_list;  // sources of the current forward, one set per slot
-    int src_list_first = -2, src_list_count = -1;
+    // sources of a forward, one set per slot, kept per shot range (and tile
+    // shape) so that a captured graph of one batch never reads another's
+    std::map<long long, PointList> src_lists;
     // the same sets bucketed by fused tile: injection points by the tile's
     // velocity region (tile plus halo), recording points by the tile itself
-    PointList f_src_v, f_rec_v, f_rec_t;
-    int f_src_first = -2, f_src_count = -1, f_src_tx = -1, f_rec_tx = -1;
+    PointList f_rec_v, f_rec_t;
+    int f_rec_tx = -1;
+
+    // captured time loops: one executable graph per (run kind, shots, flags,
+    // kernel configuration); `gen` changes whenever buffers or settings they
+    // bake in change
+    struct GraphEntry {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        long gen = -1;
+    };
+    std::map<std::string, GraphEntry> graphs;
+    long gen = 0;
+    bool use_graphs = true;
 
     // envelope misfit: FFT twiddles and radix plan for length nt
     DevBuf env_w, env_fac, env_scratch;
@@ -157,12 +175,19 @@ void fdm_interact(Ctx &c, int count, int it, const float *vx, const float *vz, c
 void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1);
 void fused_adjoint(Ctx &c, int first, int count, int it0, int it1);
 void fused_invalidate_lists(Ctx &c);
+void fused_prepare(Ctx &c, int kind, int first, int count);
+void fdm_prepare(Ctx &c, int kind, int first, int count);
+// the per-slot source list of shots first..first+count-1 for a tile shape
+// (key 0: the kernel-pair tiles); built on first use
+PointList &source_list(Ctx &c, int first, int count, int key);
+void release_list(PointList &pl);
 // device pointer of the current buffer of a ping-pong field
 inline float *cur_field(Ctx &c, DevBuf &a, DevBuf &b) {
     return (c.parity & 1) ? b.as<float>() : a.as<float>();
 }
 bool prof_begin(Ctx &c, int it, hipEvent_t *a);
-void prof_end(Ctx &c, int kind, hipEvent_t a);
+void prof_end(Ctx &c, int kind, hipEvent_t a, int units = 1);
+bool prof_begin_any(Ctx &c, hipEvent_t *a);
 void prof_resolve(Ctx &c);
 void prof_release(Ctx &c);
 

@@ -19,6 +19,7 @@ INV_LAMBDA, INV_MU, INV_RHO, INV_K_LAMBDA, INV_K_MU, INV_K_RHO, INV_OUT, INV_ADS
 INV_SNAP_F32, INV_SNAP_F16, INV_SNAP_BF16 = 0, 1, 2
 INV_FWD_SNAPSHOTS = 1
 INV_PROF_VELOCITY, INV_PROF_STRESS, INV_PROF_INTERACT, INV_PROF_STEP = 0, 1, 2, 3
+INV_PROF_FWD_RUN, INV_PROF_ADJ_RUN = 4, 5
 INV_VARIANT_FUSED, INV_VARIANT_SPLIT = 0, 1
 INV_MATH_EXACT, INV_MATH_FAST = 0, 1
 
@@ -102,6 +103,7 @@ SIGNATURES = {
     "inv_run_advance": [_vp, _i, _vp],
     "inv_misfit_trace": [_vp, _i, _vp, _vp, _vp, _vp],
     "inv_set_math": [_vp, _i],
+    "inv_set_graphs": [_vp, _i],
 }
 
 _lib = None
@@ -386,6 +388,9 @@ class Context:
 
     def set_step_limit(self, n):
         self._chk(self.L.inv_set_step_limit(self.h, n))
+
+    def set_graphs(self, on):
+        self._chk(self.L.inv_set_graphs(self.h, 1 if on else 0))
 
     def set_math(self, mode):
         self._chk(self.L.inv_set_math(self.h, mode))

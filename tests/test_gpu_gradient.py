@@ -215,3 +215,29 @@ def test_model_swap_after_async_forward(gpu):
                                              dtype=np.float64)
     assert abs(f1 - f64) <= 1e-5 * abs(f64)
     ctx.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1], ids=["fused", "split"])
+def test_graph_replay_equals_direct(gpu, variant):
+    """Captured-and-replayed runs give the same bits as direct launches, over
+    several gradients and alternating shot batches (graph per batch)."""
+    P = inv.problems.tiny(sh=1, psv=1, nsrc=4)
+    obs = _obs_from_true(P)
+    res = []
+    for graphs in (False, True):
+        ctx = P.context(batch=2)
+        ctx.set_variant(variant)
+        ctx.set_graphs(graphs)
+        for s in range(P.nsrc):
+            ctx.obs_set(s, obs[s])
+        out = []
+        for _ in range(3):
+            f = ctx.gradient(0, 2)
+            out.append((f, ctx.kernels_get(), ctx.state_get(1)))
+        res.append(out)
+        ctx.close()
+    for (fa, ka, sa), (fb, kb, sb) in zip(*res):
+        assert fa == fb
+        for x, y in zip(ka, kb):
+            assert np.array_equal(x, y)
+        assert np.array_equal(sa, sb)
