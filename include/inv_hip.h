@@ -196,8 +196,9 @@ int inv_memcpy_d2h(inv_ctx *ctx, void *host, const void *dev, size_t bytes);
 /* ---- test hooks ---------------------------------------------------------
  * Wavefields of one slot after the last forward/adjoint:
  * SH {vy, uy, sxy, szy} then P-SV {vx, vz, ux, uz, sxx, szz, sxz}, each [N].
- * Snapshots of one slot: [nsfe][nf][N], SH {uy, vy} then P-SV {ux, uz, vx, vz},
- * decoded to float. */
+ * Snapshots of one slot: [slots][nf][N], SH {uy, vy} then P-SV {ux, uz, vx, vz},
+ * decoded to float; slots = nt/sfe (or ckpt/sfe with checkpoints, holding the
+ * segment last used). */
 int inv_state_get(inv_ctx *ctx, int slot, float *host);
 int inv_snapshots_get(inv_ctx *ctx, int slot, float *host);
 /* ---- measurement -------------------------------------------------------
@@ -240,6 +241,22 @@ int inv_set_graphs(inv_ctx *ctx, int on);
 
 /* Run only the first nsteps steps of the next forward (0 = all). */
 int inv_set_step_limit(inv_ctx *ctx, int nsteps);
+
+/* Forward-snapshot storage for the adjoint.  No counterpart in the reference,
+ * which keeps FP32 snapshots of every sfe-th step in host memory
+ * (solver/fdm.h:403-425,521-560).
+ *   format   INV_SNAP_F32, or INV_SNAP_F16 / INV_SNAP_BF16: each snapshot
+ *            field is stored as x * 2^15 / max|x| rounded to nearest even and
+ *            decoded before use; all arithmetic stays FP32.
+ *   ckpt     0: every snapshot of the run is kept (nt/sfe slots per shot);
+ *            K > 0 (K % sfe == 0, nt % K == 0): the forward keeps the FP32
+ *            state every K steps plus the snapshots of its last K steps, and
+ *            the adjoint recomputes each earlier K-step segment from its
+ *            checkpoint before using it (K/sfe slots per shot).
+ * Needs the fused kernel (INV_VARIANT_FUSED) for anything but (F32, 0).
+ * Call after inv_grid_set (which resets to F32, 0); buffers are sized at the
+ * next run.  inv_snapshots_get then returns the slots held, decoded. */
+int inv_set_snapshots(inv_ctx *ctx, int format, int ckpt);
 
 #ifdef __cplusplus
 }

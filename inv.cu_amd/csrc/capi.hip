@@ -104,8 +104,15 @@ static void alloc_slots(Ctx &c) {
     INV_HIP(hipMemset(c.k_rho.p, 0, N * f));
     if (c.g.adj && c.g.batch > 1) c.kslot.ensure(B * 3 * N * f);
     else c.kslot.release();
-    if (c.g.adj && c.nsfe > 0) c.snap.ensure(B * (size_t)c.nsfe * c.nsnapf * N * f);
-    else c.snap.release();
+    // snapshot storage is sized before the first run that needs it, for the
+    // format and checkpoint interval in force then (inv_set_snapshots)
+    c.snap.release();
+    c.smax.release();
+    c.ckbuf.release();
+    c.rstate.release();
+    c.snap_format = INV_SNAP_F32;
+    c.ckpt = 0;
+    c.nslots = c.nsfe;
 }
 
 static void alloc_traces(Ctx &c) {
@@ -272,7 +279,7 @@ int inv_ctx_destroy(inv_ctx *ctx) {
     Ctx &c = *ctx;
     DevBuf *bufs[] = {&c.lam, &c.mu, &c.rho, &c.abx, &c.abzb, &c.abzt, &c.k_lambda, &c.k_mu,
                       &c.k_rho, &c.kslot, &c.vx, &c.vz, &c.ux, &c.uz, &c.sxx, &c.szz, &c.sxz,
-                      &c.vy, &c.uy, &c.sxy, &c.szy, &c.snap, &c.out, &c.adstf, &c.obs, &c.chi,
+                      &c.vy, &c.uy, &c.sxy, &c.szy, &c.snap, &c.smax, &c.ckbuf, &c.rstate, &c.out, &c.adstf, &c.obs, &c.chi,
                       &c.taper, &c.red, &c.scal, &c.stf, &c.rec_list.ptr, &c.rec_list.cell,
                       &c.rec_list.row,
                       &c.gauss_w, &c.gauss_tmp, &c.gauss_sx, &c.gauss_sz, &c.vx_b, &c.vz_b,
@@ -510,6 +517,10 @@ static int forward_end(const Ctx &c) {
 // that; runs with event sampling, partial runs and stepping go direct.
 static void run_steps(Ctx &c, int kind, int first, int count, int flags, int it0, int it1) {
     const int end = kind == 1 ? forward_end(c) : c.g.nt;
+    INV_REQUIRE(c.variant == INV_VARIANT_FUSED || (c.snap_format == INV_SNAP_F32 && c.ckpt == 0),
+                INV_ERR_UNSUPPORTED,
+                "16-bit snapshots and checkpoints need the fused column kernel (variant 0)");
+    snap_storage(c);
     if (!c.use_graphs || it0 != 0 || it1 != end) {
         run_steps_direct(c, kind, first, count, flags, it0, it1);
         return;
@@ -574,7 +585,7 @@ int inv_adjoint(inv_ctx *ctx, int first, int count) {
     return guard(ctx, [&](Ctx &c) {
         require_ready(c);
         check_slots(c, first < 0 ? 0 : first, count);
-        INV_REQUIRE(c.g.adj && c.snap.p, INV_ERR_STATE, "adjoint needs adj = 1 (snapshots)");
+        INV_REQUIRE(c.g.adj, INV_ERR_STATE, "adjoint needs adj = 1 (snapshots)");
         run_steps(c, 2, first, count, 0, 0, c.g.nt);
         c.run_kind = 0;
     });
@@ -596,7 +607,7 @@ int inv_adjoint_begin(inv_ctx *ctx, int first, int count) {
     return guard(ctx, [&](Ctx &c) {
         require_ready(c);
         check_slots(c, first < 0 ? 0 : first, count);
-        INV_REQUIRE(c.g.adj && c.snap.p, INV_ERR_STATE, "adjoint needs adj = 1 (snapshots)");
+        INV_REQUIRE(c.g.adj, INV_ERR_STATE, "adjoint needs adj = 1 (snapshots)");
         c.run_kind = 2;
         c.run_first = first;
         c.run_count = count;
@@ -908,12 +919,74 @@ int inv_state_get(inv_ctx *ctx, int slot, float *host) {
     });
 }
 
+// IEEE binary16 to float (subnormals included)
+static float half_to_float(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    float v;
+    if (e == 0) {
+        v = std::ldexp((float)m, -24);
+        return sign ? -v : v;
+    }
+    const uint32_t bits = e == 31 ? (sign | 0x7f800000u | (m << 13)) : (sign | ((e + 112u) << 23) | (m << 13));
+    std::memcpy(&v, &bits, 4);
+    return v;
+}
+
 int inv_snapshots_get(inv_ctx *ctx, int slot, float *host) {
     return guard(ctx, [&](Ctx &c) {
         INV_REQUIRE(c.snap.p && host && slot >= 0 && slot < c.g.batch, INV_ERR_ARG, "bad slot");
         sync(c);
-        const size_t b = (size_t)c.nsfe * c.nsnapf * c.N * sizeof(float);
-        INV_HIP(hipMemcpy(host, c.snap.as<char>() + slot * b, b, hipMemcpyDeviceToHost));
+        const size_t n = (size_t)c.nslots * c.nsnapf * c.N;
+        if (c.snap_format == INV_SNAP_F32) {
+            INV_HIP(hipMemcpy(host, c.snap.as<char>() + slot * n * sizeof(float), n * sizeof(float),
+                              hipMemcpyDeviceToHost));
+            return;
+        }
+        std::vector<uint16_t> q(n);
+        std::vector<uint32_t> mx((size_t)c.nslots * c.g.batch * c.nsnapf);
+        INV_HIP(hipMemcpy(q.data(), c.snap.as<char>() + slot * n * 2, n * 2, hipMemcpyDeviceToHost));
+        INV_HIP(hipMemcpy(mx.data(), c.smax.p, mx.size() * 4, hipMemcpyDeviceToHost));
+        for (int l = 0; l < c.nslots; l++)
+            for (int f = 0; f < c.nsnapf; f++) {
+                float m;
+                std::memcpy(&m, &mx[((size_t)l * c.g.batch + slot) * c.nsnapf + f], 4);
+                const float s = m * (1.0f / 32768.0f);
+                const size_t o = ((size_t)l * c.nsnapf + f) * c.N;
+                for (long k = 0; k < c.N; k++) {
+                    const uint16_t v = q[o + k];
+                    float d;
+                    if (c.snap_format == INV_SNAP_F16) {
+                        d = half_to_float(v);
+                    } else {
+                        const uint32_t bits = (uint32_t)v << 16;
+                        std::memcpy(&d, &bits, 4);
+                    }
+                    host[o + k] = d * s;
+                }
+            }
+    });
+}
+
+int inv_set_snapshots(inv_ctx *ctx, int format, int ckpt_interval) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_grid, INV_ERR_STATE, "grid not set");
+        INV_REQUIRE(format == INV_SNAP_F32 || format == INV_SNAP_F16 || format == INV_SNAP_BF16,
+                    INV_ERR_ARG, "unknown snapshot format");
+        INV_REQUIRE(ckpt_interval >= 0, INV_ERR_ARG, "checkpoint interval must be >= 0");
+        if (ckpt_interval > 0)
+            INV_REQUIRE(c.g.adj && c.g.sfe > 0 && ckpt_interval % c.g.sfe == 0 &&
+                            c.g.nt % ckpt_interval == 0,
+                        INV_ERR_ARG,
+                        "checkpoint interval must be a multiple of sfe that divides nt (adj = 1)");
+        sync(c);
+        c.snap.release();
+        c.smax.release();
+        c.ckbuf.release();
+        c.rstate.release();
+        c.snap_format = format;
+        c.ckpt = ckpt_interval;
+        c.nslots = ckpt_interval > 0 ? ckpt_interval / c.g.sfe : c.nsfe;
+        invalidate(c);
     });
 }
 

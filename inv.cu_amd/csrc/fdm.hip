@@ -47,8 +47,10 @@ struct StrArgs {
 struct IntArgs {
     Geo g;
     const float *vx, *vz, *ux, *uz, *vy, *uy;
-    const float *snap;  // slot-0 base of the snapshot slot being read
-    long snap_slot_stride;
+    const void *snap;  // slot-0 base of the snapshot slot being read
+    long snap_slot_stride;  // elements per batch slot
+    const unsigned *smax;   // 16-bit formats: max-abs [batch][nsnapf] of the slot
+    int nsnapf;
     float *kl, *km, *kr;
     long k_slot_stride;  // 0: every slot accumulates into the same kernels
     float ndt;
@@ -198,7 +200,30 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_stress(const StrArgs a) {
 
 // interaction* kernels of solver/fdm.h:198-219 with the D+ derivatives of the
 // adjoint and the forward displacement (fdm.h:586-605) computed in place.
-template <bool SH, bool PSV, bool FAST>
+// FP32 snapshot field read through the same accessor interface as Snap16
+struct Snap32 {
+    const float *p;
+    __device__ __forceinline__ float operator[](long k) const { return p[k]; }
+};
+
+template <int FMT>
+struct SnapField {
+    using type = Snap16<FMT>;
+    __device__ static type at(const IntArgs &a, int b, int f) {
+        const unsigned short *q =
+            static_cast<const unsigned short *>(a.snap) + (long)b * a.snap_slot_stride + f * a.g.N;
+        return type{q, __uint_as_float(a.smax[b * a.nsnapf + f]) * (1.0f / SNAP_RANGE)};
+    }
+};
+template <>
+struct SnapField<INV_SNAP_F32> {
+    using type = Snap32;
+    __device__ static type at(const IntArgs &a, int b, int f) {
+        return Snap32{static_cast<const float *>(a.snap) + (long)b * a.snap_slot_stride + f * a.g.N};
+    }
+};
+
+template <bool SH, bool PSV, bool FAST, int FMT>
 __global__ __launch_bounds__(TILE_Z *TILE_X) void k_interact(const IntArgs a) {
     const int j = blockIdx.x * TILE_Z + threadIdx.x;
     const int i = blockIdx.y * TILE_X + threadIdx.y;
@@ -210,40 +235,39 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_interact(const IntArgs a) {
     const long so = (long)b * g.N;
     const bool xin = i >= 1 && i < g.nx - 2;
     const bool zin = j >= 1 && j < g.nz - 2;
-    const float *sp = a.snap + (long)b * a.snap_slot_stride;
     float *kl = a.kl + (long)b * a.k_slot_stride;
     float *km = a.km + (long)b * a.k_slot_stride;
     float *kr = a.kr + (long)b * a.k_slot_stride;
     const float ndt = a.ndt;
     if (SH) {
         const float *uy = a.uy + so;
-        const float *uy_fw = sp, *vy_fw = sp + g.N;
+        const auto uy_fw = SnapField<FMT>::at(a, b, 0), vy_fw = SnapField<FMT>::at(a, b, 1);
         const float dx_ = xin ? d_plus<FAST>(uy, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
         const float dz_ = zin ? d_plus<FAST>(uy, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z) : 0.0f;
-        const float dxf = xin ? d_plus<FAST>(uy_fw, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
-        const float dzf = zin ? d_plus<FAST>(uy_fw, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z) : 0.0f;
+        const float dxf = xin ? d_plus_acc<FAST>(uy_fw, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
+        const float dzf = zin ? d_plus_acc<FAST>(uy_fw, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z) : 0.0f;
         if (a.inv_rho) kr[k] -= vy_fw[k] * a.vy[so + k] * ndt;
         if (a.inv_mu) km[k] -= (dx_ * dxf + dz_ * dzf) * ndt;
     }
     if (PSV) {
         const float *ux = a.ux + so, *uz = a.uz + so;
-        const long f0 = SH ? 2 : 0;
-        const float *ux_fw = sp + f0 * g.N, *uz_fw = sp + (f0 + 1) * g.N;
-        const float *vx_fw = sp + (f0 + 2) * g.N, *vz_fw = sp + (f0 + 3) * g.N;
+        const int f0 = SH ? 2 : 0;
+        const auto ux_fw = SnapField<FMT>::at(a, b, f0), uz_fw = SnapField<FMT>::at(a, b, f0 + 1);
+        const auto vx_fw = SnapField<FMT>::at(a, b, f0 + 2), vz_fw = SnapField<FMT>::at(a, b, f0 + 3);
         float xx, xz, zx, zz, xxf, xzf, zxf, zzf;
         if (xin) {
             xx = d_plus<FAST>(ux, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x);
             zx = d_plus<FAST>(uz, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x);
-            xxf = d_plus<FAST>(ux_fw, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x);
-            zxf = d_plus<FAST>(uz_fw, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x);
+            xxf = d_plus_acc<FAST>(ux_fw, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x);
+            zxf = d_plus_acc<FAST>(uz_fw, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x);
         } else {
             xx = zx = xxf = zxf = 0.0f;
         }
         if (zin) {
             xz = d_plus<FAST>(ux, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
             zz = d_plus<FAST>(uz, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
-            xzf = d_plus<FAST>(ux_fw, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
-            zzf = d_plus<FAST>(uz_fw, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
+            xzf = d_plus_acc<FAST>(ux_fw, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
+            zzf = d_plus_acc<FAST>(uz_fw, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
         } else {
             xz = zz = xzf = zzf = 0.0f;
         }
@@ -548,15 +572,25 @@ void fdm_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
     }
 }
 
-template <bool SH, bool PSV>
-static void launch_interact(Ctx &c, const IntArgs &ia, int count) {
+template <bool SH, bool PSV, int FMT>
+static void launch_interact_f(Ctx &c, const IntArgs &ia, int count) {
     if (c.math == INV_MATH_FAST)
-        k_interact<SH, PSV, true><<<tile_grid(c, count), dim3(TILE_Z, TILE_X), 0, c.stream>>>(ia);
+        k_interact<SH, PSV, true, FMT><<<tile_grid(c, count), dim3(TILE_Z, TILE_X), 0, c.stream>>>(ia);
     else
-        k_interact<SH, PSV, false><<<tile_grid(c, count), dim3(TILE_Z, TILE_X), 0, c.stream>>>(ia);
+        k_interact<SH, PSV, false, FMT><<<tile_grid(c, count), dim3(TILE_Z, TILE_X), 0, c.stream>>>(ia);
 }
 
-void fdm_interact(Ctx &c, int count, int it, const float *vx, const float *vz, const float *vy) {
+template <bool SH, bool PSV>
+static void launch_interact(Ctx &c, const IntArgs &ia, int count) {
+    switch (c.snap_format) {
+        case INV_SNAP_F16: launch_interact_f<SH, PSV, INV_SNAP_F16>(c, ia, count); break;
+        case INV_SNAP_BF16: launch_interact_f<SH, PSV, INV_SNAP_BF16>(c, ia, count); break;
+        default: launch_interact_f<SH, PSV, INV_SNAP_F32>(c, ia, count); break;
+    }
+}
+
+void fdm_interact(Ctx &c, int count, int it, int slot, const float *vx, const float *vz,
+                  const float *vy) {
     IntArgs ia{};
     ia.g = make_geo(c);
     ia.vx = vx;
@@ -565,8 +599,13 @@ void fdm_interact(Ctx &c, int count, int it, const float *vx, const float *vz, c
     ia.ux = c.ux.as<float>();
     ia.uz = c.uz.as<float>();
     ia.uy = c.uy.as<float>();
-    ia.snap_slot_stride = (long)c.nsfe * c.nsnapf * c.N;
-    ia.snap = c.snap.as<float>() + (long)(it / c.g.sfe) * c.nsnapf * c.N;
+    INV_REQUIRE(slot >= 0 && slot < c.nslots && c.snap.p && c.smax.p, INV_ERR_STATE,
+                "snapshot slot outside the snapshot buffer");
+    const long esize = c.snap_format == INV_SNAP_F32 ? 4 : 2;
+    ia.snap_slot_stride = (long)c.nslots * c.nsnapf * c.N;
+    ia.snap = c.snap.as<char>() + (long)slot * c.nsnapf * c.N * esize;
+    ia.smax = c.smax.as<unsigned>() + (long)slot * c.g.batch * c.nsnapf;
+    ia.nsnapf = c.nsnapf;
     ia.ndt = (float)c.g.sfe * c.g.dt;  // sfe * dt, fdm.h:585
     ia.inv_lambda = c.g.inv_lambda;
     ia.inv_mu = c.g.inv_mu;
@@ -626,7 +665,8 @@ void fdm_adjoint(Ctx &c, int first, int count, int it0, int it1) {
         va.it = it;
         step(c, va, sa, count, it);
         if (it % c.g.sfe == 0)
-            fdm_interact(c, count, it, c.vx.as<float>(), c.vz.as<float>(), c.vy.as<float>());
+            fdm_interact(c, count, it, it / c.g.sfe, c.vx.as<float>(), c.vz.as<float>(),
+                         c.vy.as<float>());
     }
     if (!direct && it1 == c.g.nt) fdm_kslot_reduce(c, count);
 }

@@ -47,11 +47,26 @@ struct StepArgs {
     long out_slot_stride;
     const int *rec_ptr, *rec_cell, *rec_row;
     int rec_disp;
-    // snapshot slot base (u before, v after the step, fdm.h:521-560) or null
-    float *snap;
-    long snap_slot_stride;
-    long psv_snap_off;  // offset of the P-SV fields in a snapshot slot
+    // Snapshots (fdm.h:521-560: a slot holds u before and v after a step).
+    // snap_u: slot base (batch slot 0) that receives this step's input u, or
+    // null; snap_v: slot base that receives v, this step's result when
+    // v_after is set, else this step's input (the previous step's result),
+    // or null.  16-bit formats scale each field by the max-abs read from
+    // mx_u / mx_v ([batch][nsnapf] tables of the slot); acc_u / acc_v collect
+    // max |u| and max |v| after this step into the table of a slot still to
+    // be written.  The linear kernel supports FP32 with v_after only.
+    void *snap_u, *snap_v;
+    int v_after, snap_fmt, nsnapf, psv_f0;
+    const unsigned *mx_u, *mx_v;
+    unsigned *acc_u, *acc_v;
+    long snap_slot_stride;  // elements per batch slot
 };
+
+__device__ __forceinline__ float wave_max(float x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
+    return x;
+}
 
 template <bool SH, bool PSV, int TX, bool FAST>
 __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
@@ -175,11 +190,12 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
             l_vy[e] = v;
             if (own) {
                 float u = puy[k];
-                if (a.snap) a.snap[(long)b * a.snap_slot_stride + kc] = u;
+                float *sp = static_cast<float *>(a.snap_u);
+                if (sp) sp[(long)b * a.snap_slot_stride + kc] = u;
                 u += v * g.dt;
                 a.vy1[o] = v;
                 a.uy[o] = u;
-                if (a.snap) a.snap[(long)b * a.snap_slot_stride + g.N + kc] = v;
+                if (sp) sp[(long)b * a.snap_slot_stride + g.N + kc] = v;
                 for (int p = r0; p < r1; p++)
                     if (a.rec_cell[p] == kc)
                         a.out_y[(long)b * a.out_slot_stride + (long)a.rec_row[p] * a.nt + it] =
@@ -212,8 +228,9 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
             if (own) {
                 float ux = pux[k], uz = puz[k];
                 const int f0 = SH ? 2 : 0;
-                float *sp = a.snap ? a.snap + (long)b * a.snap_slot_stride + (long)f0 * g.N + kc
-                                   : nullptr;
+                float *sp = a.snap_u ? static_cast<float *>(a.snap_u) + (long)b * a.snap_slot_stride +
+                                           (long)f0 * g.N + kc
+                                     : nullptr;
                 if (sp) {
                     sp[0] = ux;
                     sp[g.N] = uz;
@@ -401,6 +418,25 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
     }
     const long it = a.it;
 
+    // snapshot fields of this wave type: u at fu (and fu+1), v at fv (and fv+1)
+    const int fmt = a.snap_fmt;
+    const int fu = PSV ? a.psv_f0 : 0, fv = PSV ? a.psv_f0 + 2 : 1;
+    const long sb = (long)b * a.snap_slot_stride;
+    float su1 = 1.0f, su2 = 1.0f, sv1 = 1.0f, sv2 = 1.0f;
+    if (fmt != INV_SNAP_F32) {
+        const int mb = b * a.nsnapf;
+        if (a.snap_u) {
+            su1 = snap_enc_scale(a.mx_u[mb + fu]);
+            if (PSV) su2 = snap_enc_scale(a.mx_u[mb + fu + 1]);
+        }
+        if (a.snap_v) {
+            sv1 = snap_enc_scale(a.mx_v[mb + fv]);
+            if (PSV) sv2 = snap_enc_scale(a.mx_v[mb + fv + 1]);
+        }
+    }
+    const bool track = a.acc_u || a.acc_v;
+    float mu1 = 0.0f, mu2 = 0.0f, mv1 = 0.0f, mv2 = 0.0f;  // max |u|, |v| after the step
+
     // 2. velocity of step n+1, rows i0-1 .. i0+TX+1, columns j0-1 .. j0+CZ+1
     {
         const int j = jv;
@@ -438,11 +474,15 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                 l_vy[e] = v;
                 if (own) {
                     float u = qu1[p];
-                    if (a.snap) a.snap[(long)b * a.snap_slot_stride + kc] = u;
+                    if (a.snap_u) snap_put(a.snap_u, fmt, sb + kc, u, su1);
                     u += v * g.dt;
                     a.vy1[o] = v;
                     a.uy[o] = u;
-                    if (a.snap) a.snap[(long)b * a.snap_slot_stride + g.N + kc] = v;
+                    if (a.snap_v) snap_put(a.snap_v, fmt, sb + g.N + kc, a.v_after ? v : qv1[p], sv1);
+                    if (track) {
+                        mu1 = fmaxf(mu1, fabsf(u));
+                        mv1 = fmaxf(mv1, fabsf(v));
+                    }
                     for (int pp = r0; pp < r1; pp++)
                         if (a.rec_cell[pp] == kc)
                             a.out_y[(long)b * a.out_slot_stride + (long)a.rec_row[pp] * a.nt + it] =
@@ -474,11 +514,10 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                 l_vz[e] = vz;
                 if (own) {
                     float ux = qu1[p], uz = qu2[p];
-                    float *sp = a.snap ? a.snap + (long)b * a.snap_slot_stride + a.psv_snap_off + kc
-                                       : nullptr;
-                    if (sp) {
-                        sp[0] = ux;
-                        sp[g.N] = uz;
+                    if (a.snap_u) {
+                        const long su = sb + (long)fu * g.N + kc;
+                        snap_put(a.snap_u, fmt, su, ux, su1);
+                        snap_put(a.snap_u, fmt, su + g.N, uz, su2);
                     }
                     ux += vx * g.dt;
                     uz += vz * g.dt;
@@ -486,9 +525,16 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                     a.vz1[o] = vz;
                     a.ux[o] = ux;
                     a.uz[o] = uz;
-                    if (sp) {
-                        sp[2 * g.N] = vx;
-                        sp[3 * g.N] = vz;
+                    if (a.snap_v) {
+                        const long sv = sb + (long)fv * g.N + kc;
+                        snap_put(a.snap_v, fmt, sv, a.v_after ? vx : qv1[p], sv1);
+                        snap_put(a.snap_v, fmt, sv + g.N, a.v_after ? vz : qv2[p], sv2);
+                    }
+                    if (track) {
+                        mu1 = fmaxf(mu1, fabsf(ux));
+                        mu2 = fmaxf(mu2, fabsf(uz));
+                        mv1 = fmaxf(mv1, fabsf(vx));
+                        mv2 = fmaxf(mv2, fabsf(vz));
                     }
                     for (int pp = r0; pp < r1; pp++)
                         if (a.rec_cell[pp] == kc) {
@@ -498,6 +544,27 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                             a.out_z[kr] = a.rec_disp ? uz : vz;
                         }
                 }
+            }
+        }
+    }
+    // max-abs of this tile's u and v after the step, one atomic per wavefront
+    // (non-negative floats order as their bit patterns)
+    if (track) {
+        mu1 = wave_max(mu1);
+        mv1 = wave_max(mv1);
+        if (PSV) {
+            mu2 = wave_max(mu2);
+            mv2 = wave_max(mv2);
+        }
+        if (lane == 0) {
+            const int mb = b * a.nsnapf;
+            if (a.acc_u) {
+                atomicMax(a.acc_u + mb + fu, __float_as_uint(mu1));
+                if (PSV) atomicMax(a.acc_u + mb + fu + 1, __float_as_uint(mu2));
+            }
+            if (a.acc_v) {
+                atomicMax(a.acc_v + mb + fv, __float_as_uint(mv1));
+                if (PSV) atomicMax(a.acc_v + mb + fv + 1, __float_as_uint(mv2));
             }
         }
     }
@@ -660,22 +727,221 @@ static void ensure_second_set(Ctx &c) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Snapshot storage.  Forward snapshots for the adjoint hold FP32, FP16 or BF16
+// values (16-bit ones scaled per snapshot field by its max-abs).  They cover
+// either every snapshot step of the run (ckpt = 0) or, with checkpoints every
+// ckpt steps, one ckpt-step segment: the forward keeps the FP32 state at each
+// segment start plus the snapshots of its last segment, and the adjoint
+// recomputes the forward steps of each earlier segment from its checkpoint
+// just before it needs that segment's snapshots.
+// ---------------------------------------------------------------------------
+static size_t snap_esize(const Ctx &c) { return c.snap_format == INV_SNAP_F32 ? 4 : 2; }
+static int nstate_fields(const Ctx &c) { return (c.g.sh ? 4 : 0) + (c.g.psv ? 7 : 0); }
+static int nreplay_fields(const Ctx &c) { return (c.g.sh ? 7 : 0) + (c.g.psv ? 12 : 0); }
+
+void snap_storage(Ctx &c) {
+    if (!c.g.adj || c.nsfe <= 0) return;
+    const size_t B = (size_t)c.g.batch, N = (size_t)c.N;
+    c.nslots = c.ckpt > 0 ? c.ckpt / c.g.sfe : c.nsfe;
+    c.snap.ensure(B * (size_t)c.nslots * c.nsnapf * N * snap_esize(c));
+    c.smax.ensure((size_t)c.nslots * B * c.nsnapf * sizeof(unsigned));
+    if (c.ckpt > 0) {
+        const size_t nck = (size_t)(c.g.nt / c.ckpt - 1);
+        if (nck > 0) c.ckbuf.ensure(nck * nstate_fields(c) * B * N * sizeof(float));
+        c.rstate.ensure((size_t)nreplay_fields(c) * B * N * sizeof(float));
+    }
+}
+
+// one wavefield set: v and s double-buffered, u in place
+struct Fields {
+    float *vx[2], *vz[2], *sxx[2], *szz[2], *sxz[2], *vy[2], *sxy[2], *szy[2];
+    float *ux, *uz, *uy;
+};
+
+static Fields main_fields(Ctx &c) {
+    Fields F{};
+    auto two = [](DevBuf &a, DevBuf &b, float **out) {
+        out[0] = a.as<float>();
+        out[1] = b.as<float>();
+    };
+    two(c.vx, c.vx_b, F.vx); two(c.vz, c.vz_b, F.vz);
+    two(c.sxx, c.sxx_b, F.sxx); two(c.szz, c.szz_b, F.szz); two(c.sxz, c.sxz_b, F.sxz);
+    two(c.vy, c.vy_b, F.vy); two(c.sxy, c.sxy_b, F.sxy); two(c.szy, c.szy_b, F.szy);
+    F.ux = c.ux.as<float>();
+    F.uz = c.uz.as<float>();
+    F.uy = c.uy.as<float>();
+    return F;
+}
+
+// the recomputation's own wavefields, carved from one buffer
+static Fields replay_fields(Ctx &c) {
+    Fields F{};
+    float *p = c.rstate.as<float>();
+    const long n = (long)c.g.batch * c.N;
+    auto take = [&]() {
+        float *q = p;
+        p += n;
+        return q;
+    };
+    if (c.g.psv) {
+        F.vx[0] = take(); F.vx[1] = take(); F.vz[0] = take(); F.vz[1] = take();
+        F.sxx[0] = take(); F.sxx[1] = take(); F.szz[0] = take(); F.szz[1] = take();
+        F.sxz[0] = take(); F.sxz[1] = take(); F.ux = take(); F.uz = take();
+    }
+    if (c.g.sh) {
+        F.vy[0] = take(); F.vy[1] = take(); F.sxy[0] = take(); F.sxy[1] = take();
+        F.szy[0] = take(); F.szy[1] = take(); F.uy = take();
+    }
+    return F;
+}
+
+// current fields of a set in checkpoint order: SH {vy, uy, sxy, szy}, then
+// P-SV {vx, vz, ux, uz, sxx, szz, sxz}
+static int state_list(const Ctx &c, const Fields &F, int par, float **out) {
+    const int p = par & 1;
+    int n = 0;
+    if (c.g.sh) {
+        out[n++] = F.vy[p]; out[n++] = F.uy; out[n++] = F.sxy[p]; out[n++] = F.szy[p];
+    }
+    if (c.g.psv) {
+        out[n++] = F.vx[p]; out[n++] = F.vz[p]; out[n++] = F.ux; out[n++] = F.uz;
+        out[n++] = F.sxx[p]; out[n++] = F.szz[p]; out[n++] = F.sxz[p];
+    }
+    return n;
+}
+
+// checkpoint ck >= 1 holds the state before step ck * ckpt
+static float *ckpt_field(Ctx &c, int ck, int f) {
+    return c.ckbuf.as<float>() + ((long)(ck - 1) * nstate_fields(c) + f) * (long)c.g.batch * c.N;
+}
+
+static void set_state_ptrs(StepArgs &a, const Fields &F, int par) {
+    const int p0 = par & 1, p1 = p0 ^ 1;
+    a.vx0 = F.vx[p0]; a.vx1 = F.vx[p1];
+    a.vz0 = F.vz[p0]; a.vz1 = F.vz[p1];
+    a.sxx0 = F.sxx[p0]; a.sxx1 = F.sxx[p1];
+    a.szz0 = F.szz[p0]; a.szz1 = F.szz[p1];
+    a.sxz0 = F.sxz[p0]; a.sxz1 = F.sxz[p1];
+    a.vy0 = F.vy[p0]; a.vy1 = F.vy[p1];
+    a.sxy0 = F.sxy[p0]; a.sxy1 = F.sxy[p1];
+    a.szy0 = F.szy[p0]; a.szy1 = F.szy[p1];
+    a.ux = F.ux;
+    a.uz = F.uz;
+    a.uy = F.uy;
+}
+
+// Where forward step it's snapshot goes: slot isfe = nsfe - (it+1)/sfe of the
+// run (fdm.h:521-560), held at local index isfe - base when that lies in the
+// buffer.
+struct SnapPlan {
+    bool on = false;
+    int fmt = INV_SNAP_F32, sfe = 1, nsfe = 0, nslots = 0, base = 0;
+    char *snap = nullptr;
+    unsigned *smax = nullptr;
+    long slot_bytes = 0;  // one snapshot slot of batch slot 0
+    long smax_slot = 0;   // max-abs entries per slot: [batch][nsnapf]
+    int local(long it) const {
+        if (!on || it < 0 || (it + 1) % sfe != 0) return -1;
+        const long l = nsfe - (it + 1) / sfe - base;
+        return (l >= 0 && l < nslots) ? (int)l : -1;
+    }
+    void *slot(int l) const { return snap + l * slot_bytes; }
+    unsigned *mx(int l) const { return smax + l * smax_slot; }
+};
+
+static SnapPlan snap_plan(Ctx &c, bool on, int base) {
+    SnapPlan s;
+    s.on = on && c.snap.p && c.smax.p && c.g.sfe > 0;
+    s.fmt = c.snap_format;
+    s.sfe = c.g.sfe > 0 ? c.g.sfe : 1;
+    s.nsfe = c.nsfe;
+    s.nslots = c.nslots;
+    s.base = base;
+    s.snap = c.snap.as<char>();
+    s.smax = c.smax.as<unsigned>();
+    s.slot_bytes = (long)c.nsnapf * c.N * (long)snap_esize(c);
+    s.smax_slot = (long)c.g.batch * c.nsnapf;
+    return s;
+}
+
+// Snapshot work of forward step it.  FP32: u before and v after go to the
+// step's slot in the step itself.  16-bit: step it writes u before into its
+// slot (scaled by the max |u| gathered by step it-1), writes the previous
+// step's v into that step's slot (scaled by the max |v| gathered there), and
+// gathers max |u| for the next snapshot step and max |v| for its own.
+static void snap_roles(const SnapPlan &s, StepArgs &a, long it, long first_step) {
+    a.snap_u = a.snap_v = nullptr;
+    a.mx_u = a.mx_v = nullptr;
+    a.acc_u = a.acc_v = nullptr;
+    a.v_after = 1;
+    const int l = s.local(it);
+    if (s.fmt == INV_SNAP_F32) {
+        if (l >= 0) a.snap_u = a.snap_v = s.slot(l);
+        return;
+    }
+    a.v_after = 0;
+    if (l >= 0) {
+        a.snap_u = s.slot(l);
+        a.mx_u = s.mx(l);
+        a.acc_v = s.mx(l);
+    }
+    const int lp = it > first_step ? s.local(it - 1) : -1;
+    if (lp >= 0) {
+        a.snap_v = s.slot(lp);
+        a.mx_v = s.mx(lp);
+    }
+    const int ln = s.local(it + 1);
+    if (ln >= 0) a.acc_u = s.mx(ln);
+}
+
+// v after the last step of a run or segment into its snapshot slot (16-bit
+// formats; no later step will write it).  Idempotent.
+__global__ void k_snap_flush(const float *__restrict__ v1, const float *__restrict__ v2,
+                             unsigned short *__restrict__ dst, long slot_stride, int nsnapf, int fv,
+                             const unsigned *__restrict__ mx, int fmt, long N) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (k >= N) return;
+    unsigned short *d = dst + (long)b * slot_stride + (long)fv * N + k;
+    d[0] = snap_enc16(v1[(long)b * N + k] * snap_enc_scale(mx[b * nsnapf + fv]), fmt);
+    if (v2) d[N] = snap_enc16(v2[(long)b * N + k] * snap_enc_scale(mx[b * nsnapf + fv + 1]), fmt);
+}
+
+// acc[b][f] = max(acc[b][f], max |x_b|) over count slots of x [batch][N]
+__global__ __launch_bounds__(256) void k_absmax(const float *__restrict__ x, long N, unsigned *acc,
+                                                int nsnapf, int f) {
+    const int b = blockIdx.y;
+    float m = 0.0f;
+    for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < N; k += (long)gridDim.x * 256)
+        m = fmaxf(m, fabsf(x[(long)b * N + k]));
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) atomicMax(acc + b * nsnapf + f, __float_as_uint(m));
+}
+
+static void snap_flush(Ctx &c, const SnapPlan &sp, const Fields &F, int par, int count, long last) {
+    if (sp.fmt == INV_SNAP_F32) return;
+    const int l = sp.local(last);
+    if (l < 0) return;
+    const dim3 grid((unsigned)((c.N + 255) / 256), (unsigned)count);
+    const long stride = (long)c.nslots * c.nsnapf * c.N;
+    auto *dst = static_cast<unsigned short *>(sp.slot(l));
+    const int p = par & 1;
+    if (c.g.sh)
+        k_snap_flush<<<grid, 256, 0, c.stream>>>(F.vy[p], nullptr, dst, stride, c.nsnapf, 1,
+                                                 sp.mx(l), sp.fmt, c.N);
+    if (c.g.psv)
+        k_snap_flush<<<grid, 256, 0, c.stream>>>(F.vx[p], F.vz[p], dst, stride, c.nsnapf,
+                                                 (c.g.sh ? 2 : 0) + 2, sp.mx(l), sp.fmt, c.N);
+    INV_HIP(hipGetLastError());
+}
+
 void fused_prepare(Ctx &c, int kind, int first, int count) {
     ensure_second_set(c);
     ensure_rec_lists(c);
-    if (kind == 1) source_list(c, first, count, list_key(c));
-}
-
-static void set_state_ptrs(Ctx &c, StepArgs &a, int par) {
-    auto pick = [&](DevBuf &x, DevBuf &y, int p) { return (p & 1) ? y.as<float>() : x.as<float>(); };
-    a.vx0 = pick(c.vx, c.vx_b, par); a.vx1 = pick(c.vx, c.vx_b, par + 1);
-    a.vz0 = pick(c.vz, c.vz_b, par); a.vz1 = pick(c.vz, c.vz_b, par + 1);
-    a.sxx0 = pick(c.sxx, c.sxx_b, par); a.sxx1 = pick(c.sxx, c.sxx_b, par + 1);
-    a.szz0 = pick(c.szz, c.szz_b, par); a.szz1 = pick(c.szz, c.szz_b, par + 1);
-    a.sxz0 = pick(c.sxz, c.sxz_b, par); a.sxz1 = pick(c.sxz, c.sxz_b, par + 1);
-    a.vy0 = pick(c.vy, c.vy_b, par); a.vy1 = pick(c.vy, c.vy_b, par + 1);
-    a.sxy0 = pick(c.sxy, c.sxy_b, par); a.sxy1 = pick(c.sxy, c.sxy_b, par + 1);
-    a.szy0 = pick(c.szy, c.szy_b, par); a.szy1 = pick(c.szy, c.szy_b, par + 1);
+    snap_storage(c);
+    // the adjoint's recomputation injects the forward's sources
+    if (kind == 1 || (c.ckpt > 0 && c.g.adj)) source_list(c, first, count, list_key(c));
 }
 
 template <bool SH, bool PSV, bool FAST>
@@ -709,8 +975,8 @@ static void launch_m(Ctx &c, const StepArgs &a, unsigned blocks) {
     }
 }
 
-static void fused_step(Ctx &c, StepArgs &a, int count, int it) {
-    set_state_ptrs(c, a, c.parity);
+static void fused_step(Ctx &c, StepArgs &a, const Fields &F, int &par, int count, int it) {
+    set_state_ptrs(a, F, par);
     a.it = it;
     const unsigned blocks = (unsigned)(count * a.tiles_x * a.tiles_z);
     hipEvent_t ev;
@@ -719,7 +985,7 @@ static void fused_step(Ctx &c, StepArgs &a, int count, int it) {
     else launch_m<false>(c, a, blocks);
     INV_HIP(hipGetLastError());
     if (p) prof_end(c, PROF_STEP, ev);
-    c.parity ^= 1;
+    par ^= 1;
 }
 
 static void common_args(Ctx &c, StepArgs &a) {
@@ -733,22 +999,23 @@ static void common_args(Ctx &c, StepArgs &a) {
     a.abx = c.abx.as<float>();
     a.abzb = c.abzb.as<float>();
     a.abzt = c.abzt.as<float>();
-    a.ux = c.ux.as<float>();
-    a.uz = c.uz.as<float>();
-    a.uy = c.uy.as<float>();
     a.nt = c.g.nt;
+    a.snap_fmt = c.snap_format;
+    a.nsnapf = c.nsnapf;
+    a.psv_f0 = c.g.sh ? 2 : 0;
+    a.snap_slot_stride = (long)c.nslots * c.nsnapf * c.N;
+    a.v_after = 1;
     INV_REQUIRE(a.rho && a.lam && a.mu && a.abx && a.abzb && a.abzt, INV_ERR_STATE,
                 "model or absorbing profile not allocated");
-    INV_REQUIRE((!c.g.psv || (a.ux && a.uz && c.vx.p && c.vx_b.p && c.sxx_b.p)) &&
-                    (!c.g.sh || (a.uy && c.vy.p && c.vy_b.p && c.sxy_b.p)),
+    INV_REQUIRE((!c.g.psv || (c.ux.p && c.uz.p && c.vx.p && c.vx_b.p && c.sxx_b.p)) &&
+                    (!c.g.sh || (c.uy.p && c.vy.p && c.vy_b.p && c.sxy_b.p)),
                 INV_ERR_STATE, "wavefields not allocated");
 }
 
-void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
-    fused_prepare(c, 1, first, count);
+// sources of shots first.. as the forward injects them; receivers recorded
+// unless record is false (recomputation)
+static void forward_args(Ctx &c, StepArgs &a, int first, int count, bool record) {
     PointList &sl = source_list(c, first, count, list_key(c));
-    StepArgs a;
-    common_args(c, a);
     const long nt = c.g.nt, nsrc = c.nsrc;
     a.inj_x = c.stf.as<float>();
     a.inj_y = c.stf.as<float>() + nsrc * nt;
@@ -764,27 +1031,78 @@ void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
     a.out_x = c.out.as<float>() + cx * tr;
     a.out_z = c.out.as<float>() + cz * tr;
     a.out_slot_stride = (long)c.ncomp * tr;
-    a.rec_ptr = c.nrec ? c.f_rec_t.ptr.as<int>() : nullptr;
+    a.rec_ptr = (record && c.nrec) ? c.f_rec_t.ptr.as<int>() : nullptr;
     a.rec_cell = c.f_rec_t.cell.as<int>();
     a.rec_row = c.f_rec_t.row.as<int>();
     a.rec_disp = (c.g.adj || c.g.trace_type == 1) ? 1 : 0;
-    a.snap_slot_stride = (long)c.nsfe * c.nsnapf * c.N;
-    a.psv_snap_off = (c.g.sh ? 2L : 0L) * c.N;
-    const bool store = (flags & INV_FWD_SNAPSHOTS) && c.g.adj && c.g.sfe > 0 && c.snap.p;
+}
 
+void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
+    fused_prepare(c, 1, first, count);
+    StepArgs a;
+    common_args(c, a);
+    forward_args(c, a, first, count, true);
+    const bool store = (flags & INV_FWD_SNAPSHOTS) && c.g.adj && c.g.sfe > 0 && c.snap.p;
+    const SnapPlan sp = snap_plan(c, store, 0);
+    const Fields F = main_fields(c);
     if (it0 == 0) {
         c.parity = 0;
         fdm_zero_state(c, count);
         fdm_prepare_model(c);
+        if (sp.on && sp.fmt != INV_SNAP_F32)
+            fill_dev(c, c.smax.p, (long)c.nslots * c.g.batch * c.nsnapf);
     }
     for (int it = it0; it < it1; it++) {
-        a.snap = nullptr;
-        if (store && (it + 1) % c.g.sfe == 0) {
-            const long isfe = c.nsfe - (it + 1) / c.g.sfe;
-            a.snap = c.snap.as<float>() + isfe * c.nsnapf * c.N;
+        if (sp.on && c.ckpt > 0 && it > 0 && it % c.ckpt == 0) {
+            float *cur[11];
+            const int nf = state_list(c, F, c.parity, cur);
+            for (int f = 0; f < nf; f++)
+                vec_copy(c, ckpt_field(c, it / c.ckpt, f), cur[f], (size_t)count * c.N);
         }
-        fused_step(c, a, count, it);
+        snap_roles(sp, a, it, 0);
+        fused_step(c, a, F, c.parity, count, it);
     }
+    if (sp.on && it1 > it0) snap_flush(c, sp, F, c.parity, count, it1 - 1);
+}
+
+// Recompute forward segment fseg (steps fseg*ckpt ..) from its checkpoint,
+// storing its snapshots for adjoint segment aseg.
+static void replay_segment(Ctx &c, int first, int count, int fseg, int aseg) {
+    const int K = c.ckpt;
+    const long f0 = (long)fseg * K;
+    const SnapPlan sp = snap_plan(c, true, aseg * (K / c.g.sfe));
+    if (sp.fmt != INV_SNAP_F32) fill_dev(c, c.smax.p, (long)c.nslots * c.g.batch * c.nsnapf);
+    const Fields R = replay_fields(c);
+    int par = 0;
+    float *dst[11];
+    const int nf = state_list(c, R, 0, dst);
+    const size_t n = (size_t)count * c.N;
+    for (int f = 0; f < nf; f++) {
+        if (fseg == 0) fill_dev(c, dst[f], (long)n);
+        else vec_copy(c, dst[f], ckpt_field(c, fseg, f), n);
+    }
+    if (sp.fmt != INV_SNAP_F32 && c.g.sfe == 1) {
+        // step f0 is a snapshot step whose u before is the restored state
+        const int l = sp.local(f0);
+        if (l >= 0) {
+            const dim3 grid((unsigned)std::min<long>(1024, (c.N + 255) / 256), (unsigned)count);
+            if (c.g.sh) k_absmax<<<grid, 256, 0, c.stream>>>(R.uy, c.N, sp.mx(l), c.nsnapf, 0);
+            if (c.g.psv) {
+                const int fu = c.g.sh ? 2 : 0;
+                k_absmax<<<grid, 256, 0, c.stream>>>(R.ux, c.N, sp.mx(l), c.nsnapf, fu);
+                k_absmax<<<grid, 256, 0, c.stream>>>(R.uz, c.N, sp.mx(l), c.nsnapf, fu + 1);
+            }
+            INV_HIP(hipGetLastError());
+        }
+    }
+    StepArgs a;
+    common_args(c, a);
+    forward_args(c, a, first, count, false);
+    for (long it = f0; it < f0 + K; it++) {
+        snap_roles(sp, a, it, f0);
+        fused_step(c, a, R, par, count, (int)it);
+    }
+    snap_flush(c, sp, R, par, count, f0 + K - 1);
 }
 
 void fused_adjoint(Ctx &c, int first, int count, int it0, int it1) {
@@ -803,7 +1121,7 @@ void fused_adjoint(Ctx &c, int first, int count, int it0, int it1) {
     a.inj_row = c.f_rec_v.row.as<int>();
     a.inj_per_slot = 0;
     a.rec_ptr = nullptr;
-    a.snap = nullptr;
+    const Fields F = main_fields(c);
     const bool direct = c.g.batch == 1;
     if (it0 == 0) {
         if (!direct) fill_dev(c, c.kslot.p, 3L * count * c.N);
@@ -811,11 +1129,15 @@ void fused_adjoint(Ctx &c, int first, int count, int it0, int it1) {
         fdm_zero_state(c, count);
         fdm_prepare_model(c);
     }
+    const int K = c.ckpt, S = K > 0 ? c.g.nt / K : 1, m = K > 0 ? K / c.g.sfe : 0;
     for (int it = it0; it < it1; it++) {
-        fused_step(c, a, count, it);
-        if (it % c.g.sfe == 0)
-            fdm_interact(c, count, it, cur_field(c, c.vx, c.vx_b), cur_field(c, c.vz, c.vz_b),
-                         cur_field(c, c.vy, c.vy_b));
+        if (K > 0 && it > 0 && it % K == 0) replay_segment(c, first, count, S - 1 - it / K, it / K);
+        fused_step(c, a, F, c.parity, count, it);
+        if (it % c.g.sfe == 0) {
+            const int slot = it / c.g.sfe - (K > 0 ? (it / K) * m : 0);
+            fdm_interact(c, count, it, slot, cur_field(c, c.vx, c.vx_b),
+                         cur_field(c, c.vz, c.vz_b), cur_field(c, c.vy, c.vy_b));
+        }
     }
     if (!direct && it1 == c.g.nt) fdm_kslot_reduce(c, count);
 }

@@ -87,7 +87,12 @@ struct Ctx {
     int ncomp = 0;  // trace components: y (sh) then x, z (psv)
     int nsnapf = 0;  // snapshot fields per slot
     int step_limit = 0;
+    // forward snapshots for the adjoint: storage format, checkpoint interval
+    // (0: every snapshot of the run is kept) and the slots held per batch slot
+    // (nsfe, or ckpt / sfe with checkpoints)
     int snap_format = INV_SNAP_F32;
+    int ckpt = 0;
+    int nslots = 0;
 
     // model and absorbing profile (1-D factor tables: ab = ((ax*bz)*tz))
     DevBuf lam, mu, rho, abx, abzb, abzt;
@@ -106,7 +111,10 @@ struct Ctx {
     int tile_x = 8;    // rows per fused tile (8, 16 or 32)
     int math = INV_MATH_FAST;
     DevBuf bdt;        // dt / rho, rebuilt at the start of each fast-mode run
-    DevBuf snap;   // [batch][nsfe][nsnapf][N]
+    DevBuf snap;   // [batch][nslots][nsnapf][N] of FP32, FP16 or BF16
+    DevBuf smax;   // max-abs of each snapshot field [nslots][batch][nsnapf] (FP32 bits)
+    DevBuf ckbuf;  // state checkpoints [nt/ckpt - 1][state fields][batch][N]
+    DevBuf rstate; // wavefields of the checkpoint recomputation
     DevBuf out;    // [batch][ncomp][nrec][nt]
     DevBuf adstf;  // [batch][ncomp][nrec][nt]
     DevBuf obs;    // [nsrc][ncomp][nrec][nt]
@@ -170,8 +178,14 @@ void fdm_kslot_reduce(Ctx &c, int count);
 Geo make_geo(const Ctx &c);
 void fill_dev(Ctx &c, void *p, long n, float v = 0.0f);
 // interaction kernels for adjoint step it (it % sfe == 0) with the adjoint
-// velocities vx/vz/vy of the current state set
-void fdm_interact(Ctx &c, int count, int it, const float *vx, const float *vz, const float *vy);
+// velocities vx/vz/vy of the current state set, reading snapshot buffer slot
+// `slot`
+void fdm_interact(Ctx &c, int count, int it, int slot, const float *vx, const float *vz,
+                  const float *vy);
+// sizes the snapshot, max-abs, checkpoint and recomputation buffers for the
+// current format and checkpoint interval (allocation happens before a run,
+// never inside a captured one)
+void snap_storage(Ctx &c);
 void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1);
 void fused_adjoint(Ctx &c, int first, int count, int it0, int it1);
 void fused_invalidate_lists(Ctx &c);

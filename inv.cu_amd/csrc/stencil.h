@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/inv_hip.h"
+
 namespace inv {
 
 struct Geo {
@@ -32,6 +34,59 @@ __device__ __forceinline__ float d_minus(const float *__restrict__ f, long k, lo
 template <bool FAST>
 __device__ __forceinline__ float d_plus(const float *__restrict__ f, long k, long s, float c8,
                                         float c24, float a1, float a2) {
+    if (FAST) return a1 * (f[k + s] - f[k]) - a2 * (f[k + 2 * s] - f[k - s]);
+    return 9.0f * (f[k + s] - f[k]) / c8 - (f[k + 2 * s] - f[k - s]) / c24;
+}
+
+// ---------------------------------------------------------------------------
+// Reduced-precision snapshot storage.  A snapshot field x with max-abs m is
+// stored as q = x * (SNAP_RANGE / m) in FP16 or BF16 (round to nearest even)
+// and read back as q * (m / SNAP_RANGE).  SNAP_RANGE = 2^15 keeps the largest
+// value below FP16's 65504 and leaves 2^-24 / 2^15 of m as the smallest
+// normal magnitude, so values down to ~1e-9 m keep FP16's full 11-bit
+// significand.  Arithmetic stays FP32.
+// ---------------------------------------------------------------------------
+constexpr float SNAP_RANGE = 32768.0f;
+
+__device__ __forceinline__ unsigned short snap_enc16(float y, int fmt) {
+    if (fmt == INV_SNAP_F16) {
+        const _Float16 h = (_Float16)y;
+        return __builtin_bit_cast(unsigned short, h);
+    }
+    unsigned u = __float_as_uint(y);  // BF16: round to nearest even (y finite)
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float snap_dec16(unsigned short q, int fmt) {
+    if (fmt == INV_SNAP_F16) return (float)__builtin_bit_cast(_Float16, q);
+    return __uint_as_float((unsigned)q << 16);
+}
+
+// encode factor SNAP_RANGE / m for a stored max-abs m (0 for an all-zero field)
+__device__ __forceinline__ float snap_enc_scale(unsigned mbits) {
+    const float m = __uint_as_float(mbits);
+    return m > 0.0f ? SNAP_RANGE / m : 0.0f;
+}
+
+// store one snapshot value: FP32 as is, FP16/BF16 scaled by s
+__device__ __forceinline__ void snap_put(void *base, int fmt, long idx, float x, float s) {
+    if (fmt == INV_SNAP_F32) static_cast<float *>(base)[idx] = x;
+    else static_cast<unsigned short *>(base)[idx] = snap_enc16(x * s, fmt);
+}
+
+// read access to one 16-bit snapshot field, decoded to FP32
+template <int FMT>
+struct Snap16 {
+    const unsigned short *q;
+    float s;  // m / SNAP_RANGE
+    __device__ __forceinline__ float operator[](long k) const { return snap_dec16(q[k], FMT) * s; }
+};
+
+// D+ of a field read through an accessor (the same expression as d_plus)
+template <bool FAST, class A>
+__device__ __forceinline__ float d_plus_acc(const A &f, long k, long s, float c8, float c24,
+                                            float a1, float a2) {
     if (FAST) return a1 * (f[k + s] - f[k]) - a2 * (f[k + 2 * s] - f[k - s]);
     return 9.0f * (f[k + s] - f[k]) / c8 - (f[k + 2 * s] - f[k - s]) / c24;
 }

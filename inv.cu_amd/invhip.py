@@ -20,7 +20,7 @@ INV_SNAP_F32, INV_SNAP_F16, INV_SNAP_BF16 = 0, 1, 2
 INV_FWD_SNAPSHOTS = 1
 INV_PROF_VELOCITY, INV_PROF_STRESS, INV_PROF_INTERACT, INV_PROF_STEP = 0, 1, 2, 3
 INV_PROF_FWD_RUN, INV_PROF_ADJ_RUN = 4, 5
-INV_VARIANT_FUSED, INV_VARIANT_SPLIT = 0, 1
+INV_VARIANT_FUSED, INV_VARIANT_SPLIT, INV_VARIANT_FUSED_LINEAR = 0, 1, 2
 INV_MATH_EXACT, INV_MATH_FAST = 0, 1
 
 STATUS = {0: "INV_OK", -1: "INV_ERR_ARG", -2: "INV_ERR_HIP", -3: "INV_ERR_STATE",
@@ -94,6 +94,7 @@ SIGNATURES = {
     "inv_state_get": [_vp, _i, _vp],
     "inv_snapshots_get": [_vp, _i, _vp],
     "inv_set_step_limit": [_vp, _i],
+    "inv_set_snapshots": [_vp, _i, _i],
     "inv_prof_enable": [_vp, _i],
     "inv_prof_reset": [_vp],
     "inv_prof_read": [_vp, _i, _vp, _vp],
@@ -206,6 +207,7 @@ class Context:
             setattr(d, k, v)
         self._chk(self.L.inv_grid_set(self.h, C.byref(d)))
         self.desc = d
+        self._ckpt = 0  # inv_grid_set resets snapshot storage to FP32, no checkpoints
         return d
 
     @property
@@ -381,13 +383,19 @@ class Context:
         return a
 
     def snapshots_get(self, slot=0):
-        nsfe = self.desc.nt // self.desc.sfe
-        a = np.empty((nsfe, self.nsnap, self.N), np.float32)
+        ckpt = getattr(self, "_ckpt", 0)
+        nslots = (ckpt if ckpt else self.desc.nt) // self.desc.sfe
+        a = np.empty((nslots, self.nsnap, self.N), np.float32)
         self._chk(self.L.inv_snapshots_get(self.h, slot, _p(a)))
         return a
 
     def set_step_limit(self, n):
         self._chk(self.L.inv_set_step_limit(self.h, n))
+
+    def set_snapshots(self, fmt=INV_SNAP_F32, ckpt=0):
+        """Snapshot storage format (INV_SNAP_*) and checkpoint interval (0: none)."""
+        self._chk(self.L.inv_set_snapshots(self.h, fmt, ckpt))
+        self._ckpt = ckpt
 
     def set_graphs(self, on):
         self._chk(self.L.inv_set_graphs(self.h, 1 if on else 0))
