@@ -258,6 +258,112 @@ int inv_set_step_limit(inv_ctx *ctx, int nsteps);
  * next run.  inv_snapshots_get then returns the slots held, decoded. */
 int inv_set_snapshots(inv_ctx *ctx, int format, int ckpt);
 
+/* ---- request server (continuous batching of shot solves) ----------------
+ * No counterpart in the reference, which runs one inversion per process
+ * (inv.cu:33-87).  A server owns resident models and a set of lanes, each a
+ * context with its own HIP stream driven by its own host thread; a scheduler
+ * thread admits queued requests into idle lanes as lanes and device memory
+ * free up, so batches on different lanes overlap on the device.  Requests of
+ * one batching class (same grid, spacing, dt, wave type, mode, sfe,
+ * absorbing strips and receivers) share one launch, one shot per slot, each
+ * slot reading its own request's model; shorter runs are padded to the
+ * batch's longest, with results identical to the request run alone.
+ * Admission keeps resident models plus every lane's buffers (wavefields,
+ * snapshots, traces) within the HBM budget.
+ *
+ * Each request is one shot-solve job: forward of nshots shots on a resident
+ * model (FdmSolver::runForward, solver/fdm.h:515-570) returning receiver
+ * traces, and for adjoint = 1 the waveform misfit against the given
+ * observed traces (misfit/waveform.h:4-50) and the adjoint
+ * (solver/fdm.h:571-612) returning the lambda/mu/rho kernels summed over the
+ * request's shots (Misfit::run(kernel=true) without the filter,
+ * misfit/misfit.h:15-45).
+ *
+ * Host arrays of a request are read while it runs and its outputs written
+ * when it completes: they must stay valid until inv_server_wait returns for
+ * it.  Requests may be submitted from any thread; the server's functions are
+ * thread-safe. */
+typedef struct inv_server inv_server;
+
+typedef struct inv_server_desc {
+    int device;
+    size_t hbm_budget;   /* device bytes for models + lanes (0: 90% of free memory) */
+    int max_lanes;       /* batches in flight at once, one stream each (1..16) */
+    int max_slots;       /* shots per batch */
+    int max_bypass;      /* times the oldest request may be overtaken (0: strict FIFO) */
+    int snap_format;     /* INV_SNAP_* of adjoint requests' forward snapshots */
+    int math;            /* INV_MATH_* */
+    int tile_x;          /* fused tile rows (8, 16, 32; 0 = 8) */
+    int plain;           /* 1: one request at a time, one shot per launch, in
+                            arrival order (the plain baseline loop) */
+} inv_server_desc;
+
+typedef struct inv_request {
+    int nx, nz;
+    float dx, dz, dt;
+    int nt;
+    int sh, psv;         /* exactly one wave type */
+    int adjoint;         /* 0 forward only; 1 forward + waveform misfit + adjoint */
+    int trace_type;      /* forward-only traces: 0 velocity, 1 displacement */
+    int sfe;             /* snapshot interval (adjoint: nt % sfe == 0) */
+    int abs_width;
+    float abs_alpha;
+    int abs_left, abs_right, abs_top, abs_bottom;
+    int model;           /* resident model id (inv_server_model_add) */
+    int nshots;
+    const int *src_ix, *src_iz;          /* [nshots] grid indices */
+    const float *stf_x, *stf_y, *stf_z;  /* [nshots][nt] wavelets */
+    int nrec;
+    const int *rec_ix, *rec_iz;          /* [nrec] grid indices */
+    const float *obs;    /* adjoint: observed traces [nshots][ncomp][nrec][nt] */
+    float *traces;       /* out or NULL: [nshots][ncomp][nrec][nt]; displacement when adjoint */
+    float *k_lambda, *k_mu, *k_rho;      /* out or NULL (adjoint): [nx*nz] */
+} inv_request;
+
+enum inv_request_status {
+    INV_REQ_QUEUED = 0, INV_REQ_RUNNING = 1, INV_REQ_DONE = 2, INV_REQ_FAILED = 3
+};
+
+typedef struct inv_request_info {
+    int status;          /* inv_request_status */
+    int error;           /* inv_status of a failed request */
+    double misfit;       /* adjoint: summed trace misfits (FP64) */
+    double t_submit, t_start, t_done;  /* seconds on the server clock */
+    int lane;            /* lane that ran it */
+    int batch_shots;     /* shots in the launch it shared */
+    int batch_requests;  /* requests in that launch */
+} inv_request_info;
+
+typedef struct inv_server_stats {
+    long long submitted, completed, failed, batches, shots;
+    double cell_updates;  /* completed work: sum over shots of nx*nz*nt*(1 or 2) */
+    size_t budget, models_bytes;
+    size_t bytes_held;    /* models + lane reservations now */
+    size_t bytes_peak;    /* highest bytes_held so far */
+    size_t bytes_actual;  /* device bytes the lanes' contexts hold now */
+    int lanes_busy, queued;
+    double now;           /* server clock, seconds since creation */
+} inv_server_stats;
+
+int inv_server_create(const inv_server_desc *desc, inv_server **out);
+/* Waits for queued and running requests, then releases everything. */
+int inv_server_destroy(inv_server *srv);
+const char *inv_server_last_error(const inv_server *srv);
+/* Uploads a model (lambda, mu, rho host arrays [nx*nz], z fastest) to stay
+ * resident on the device; *id names it in requests. */
+int inv_server_model_add(inv_server *srv, int nx, int nz, const float *lambda, const float *mu,
+                         const float *rho, int *id);
+/* Releases a model no queued or running request uses (INV_ERR_STATE otherwise). */
+int inv_server_model_remove(inv_server *srv, int id);
+int inv_server_submit(inv_server *srv, const inv_request *req, long long *id);
+/* Waits up to timeout_s (< 0: forever) for a request; returns INV_OK once it
+ * is done (its record is then released), its error code if it failed,
+ * INV_ERR_STATE on timeout.  info may be NULL. */
+int inv_server_wait(inv_server *srv, long long id, double timeout_s, inv_request_info *info);
+/* Current state of a request without waiting or releasing it. */
+int inv_server_poll(inv_server *srv, long long id, inv_request_info *info);
+int inv_server_stats_get(inv_server *srv, inv_server_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,6 +5,6 @@ package holds its ctypes binding (invhip) and synthetic setups (problems).
 The directory name is not an importable identifier: load it with
 importlib under the module name `inv_cu_amd`.
 """
-from . import invhip, problems  # noqa: F401
+from . import invhip, problems, serving  # noqa: F401
 
-__all__ = ["invhip", "problems"]
+__all__ = ["invhip", "problems", "serving"]

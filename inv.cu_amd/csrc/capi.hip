@@ -62,11 +62,8 @@ static void build_absbound(Ctx &c) {
 }
 
 // taper of misfit/waveform.h:9-27 in FP32 (with its double literals 0.5)
-static void build_taper(Ctx &c) {
-    const int nt = c.g.nt;
-    const float dt = c.g.dt;
+void waveform_taper(int nt, float dt, float *tw) {
     const float pi = 3.1415927f;
-    std::vector<float> tw(nt);
     float t_end = (nt - 1) * dt;
     float taper_width = t_end / 10;
     float t_min = taper_width;
@@ -77,6 +74,12 @@ static void build_taper(Ctx &c) {
         else if (t >= t_max) tw[it] = (float)(0.5 + 0.5 * cosf(pi * (t_max - t) / (taper_width)));
         else tw[it] = 1;
     }
+}
+
+static void build_taper(Ctx &c) {
+    const int nt = c.g.nt;
+    std::vector<float> tw(nt);
+    waveform_taper(nt, c.g.dt, tw.data());
     c.taper.ensure(nt * sizeof(float));
     INV_HIP(hipMemcpy(c.taper.p, tw.data(), nt * sizeof(float), hipMemcpyHostToDevice));
 }
@@ -103,13 +106,15 @@ static void alloc_slots(Ctx &c) {
     INV_HIP(hipMemset(c.k_mu.p, 0, N * f));
     INV_HIP(hipMemset(c.k_rho.p, 0, N * f));
     if (c.g.adj && c.g.batch > 1) c.kslot.ensure(B * 3 * N * f);
-    else c.kslot.release();
+    else if (!c.retain) c.kslot.release();
     // snapshot storage is sized before the first run that needs it, for the
     // format and checkpoint interval in force then (inv_set_snapshots)
-    c.snap.release();
-    c.smax.release();
-    c.ckbuf.release();
-    c.rstate.release();
+    if (!c.retain) {
+        c.snap.release();
+        c.smax.release();
+        c.ckbuf.release();
+        c.rstate.release();
+    }
     c.snap_format = INV_SNAP_F32;
     c.ckpt = 0;
     c.nslots = c.nsfe;
@@ -170,11 +175,65 @@ static void invalidate(Ctx &c) {
     c.gen++;
 }
 
+std::vector<DevBuf *> ctx_buffers(Ctx &c) {
+    return {&c.lam, &c.mu, &c.rho, &c.abx, &c.abzb, &c.abzt, &c.k_lambda, &c.k_mu, &c.k_rho,
+            &c.kslot, &c.vx, &c.vz, &c.ux, &c.uz, &c.sxx, &c.szz, &c.sxz, &c.vy, &c.uy,
+            &c.sxy, &c.szy, &c.snap, &c.smax, &c.ckbuf, &c.rstate, &c.out, &c.adstf, &c.obs,
+            &c.chi, &c.taper, &c.red, &c.scal, &c.stf, &c.rec_list.ptr, &c.rec_list.cell,
+            &c.rec_list.row, &c.gauss_w, &c.gauss_tmp, &c.gauss_sx, &c.gauss_sz, &c.vx_b,
+            &c.vz_b, &c.sxx_b, &c.szz_b, &c.sxz_b, &c.vy_b, &c.sxy_b, &c.szy_b, &c.f_rec_v.ptr,
+            &c.f_rec_v.cell, &c.f_rec_v.row, &c.f_rec_t.ptr, &c.f_rec_t.cell, &c.f_rec_t.row,
+            &c.env_w, &c.env_fac, &c.env_scratch, &c.bdt, &c.slot_model};
+}
+
+size_t ctx_bytes_held(Ctx &c) {
+    size_t n = 0;
+    for (DevBuf *b : ctx_buffers(c)) n += b->bytes;
+    for (auto &kv : c.src_lists) n += kv.second.ptr.bytes + kv.second.cell.bytes + kv.second.row.bytes;
+    return n;
+}
+
+// Upper bound of what a context holds after a forward (+ adjoint) run with
+// the fused kernel, the waveform misfit and no filter: every buffer of
+// alloc_slots, alloc_traces, the second state set and the snapshot storage,
+// plus allocator rounding of 2 MiB per buffer and 16 MiB for the small ones
+// (point lists, tapers, absorbing tables, reduction scratch).
+size_t ctx_bytes_estimate(const inv_grid_desc &d, int nsrc, int nrec, int snap_format, int ckpt) {
+    const size_t B = (size_t)d.batch, N = (size_t)d.nx * d.nz, f = sizeof(float);
+    const int ncomp = (d.sh ? 1 : 0) + (d.psv ? 2 : 0), nsnapf = (d.sh ? 2 : 0) + (d.psv ? 4 : 0);
+    const int nstate = (d.sh ? 4 : 0) + (d.psv ? 7 : 0), nsecond = (d.sh ? 3 : 0) + (d.psv ? 5 : 0);
+    size_t n = 0;
+    int nbuf = 0;
+    auto add = [&](size_t b) {
+        n += b;
+        nbuf++;
+    };
+    for (int i = 0; i < nstate + nsecond; i++) add(B * N * f);
+    for (int i = 0; i < 7; i++) add(N * f);  // lam, mu, rho, dt/rho, three kernels
+    if (d.adj && B > 1) add(3 * B * N * f);
+    const size_t tr = (size_t)ncomp * std::max(nrec, 1) * d.nt * f;
+    add(B * tr);
+    add(B * tr);
+    add((size_t)std::max(nsrc, 1) * tr);
+    add(3 * (size_t)std::max(nsrc, 1) * d.nt * f);
+    add(B * std::max(nrec, 1) * ncomp * sizeof(double));
+    if (d.adj && d.sfe > 0) {
+        const int nsfe = d.nt / d.sfe;
+        const int nslots = ckpt > 0 ? ckpt / d.sfe : nsfe;
+        const size_t es = snap_format == INV_SNAP_F32 ? 4 : 2;
+        add(B * (size_t)nslots * nsnapf * N * es);
+        add((size_t)nslots * B * nsnapf * sizeof(unsigned));
+        if (ckpt > 0) {
+            add((size_t)(d.nt / ckpt - 1) * nstate * B * N * f);
+            add((size_t)((d.sh ? 7 : 0) + (d.psv ? 12 : 0)) * B * N * f);
+        }
+    }
+    return n + (size_t)nbuf * (2u << 20) + (16u << 20);
+}
+
 }  // namespace inv
 
 using namespace inv;
-
-struct inv_ctx : public inv::Ctx {};
 
 template <typename F>
 static int guard(inv_ctx *ctx, F &&f) {
@@ -277,17 +336,7 @@ int inv_ctx_destroy(inv_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     Ctx &c = *ctx;
-    DevBuf *bufs[] = {&c.lam, &c.mu, &c.rho, &c.abx, &c.abzb, &c.abzt, &c.k_lambda, &c.k_mu,
-                      &c.k_rho, &c.kslot, &c.vx, &c.vz, &c.ux, &c.uz, &c.sxx, &c.szz, &c.sxz,
-                      &c.vy, &c.uy, &c.sxy, &c.szy, &c.snap, &c.smax, &c.ckbuf, &c.rstate, &c.out, &c.adstf, &c.obs, &c.chi,
-                      &c.taper, &c.red, &c.scal, &c.stf, &c.rec_list.ptr, &c.rec_list.cell,
-                      &c.rec_list.row,
-                      &c.gauss_w, &c.gauss_tmp, &c.gauss_sx, &c.gauss_sz, &c.vx_b, &c.vz_b,
-                      &c.sxx_b, &c.szz_b, &c.sxz_b, &c.vy_b, &c.sxy_b, &c.szy_b,
-                      &c.f_rec_v.ptr,
-                      &c.f_rec_v.cell, &c.f_rec_v.row, &c.f_rec_t.ptr, &c.f_rec_t.cell,
-                      &c.f_rec_t.row, &c.env_w, &c.env_fac, &c.env_scratch, &c.bdt};
-    for (DevBuf *b : bufs) b->release();
+    for (DevBuf *b : ctx_buffers(c)) b->release();
     for (auto &kv : c.src_lists) release_list(kv.second);
     drop_graphs(c);
     for (void *p : c.user_allocs) (void)hipFree(p);
@@ -296,7 +345,6 @@ int inv_ctx_destroy(inv_ctx *ctx) {
     delete ctx;
     return INV_OK;
 }
-
 const char *inv_last_error(const inv_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int inv_sync(inv_ctx *ctx) {
@@ -333,6 +381,7 @@ int inv_grid_set(inv_ctx *ctx, const inv_grid_desc *d) {
         c.nsnapf = (d->sh ? 2 : 0) + (d->psv ? 4 : 0);
         c.have_grid = true;
         c.have_model = false;
+        c.slot_models = false;
         build_absbound(c);
         build_taper(c);
         alloc_slots(c);
@@ -356,6 +405,7 @@ int inv_model_upload_lmr(inv_ctx *ctx, const float *lambda, const float *mu, con
         INV_HIP(hipMemcpy(c.mu.p, mu, b, hipMemcpyHostToDevice));
         INV_HIP(hipMemcpy(c.rho.p, rho, b, hipMemcpyHostToDevice));
         c.have_model = true;
+        c.slot_models = false;
     });
 }
 
@@ -433,6 +483,7 @@ int inv_model_from_points(inv_ctx *ctx, int npt, const float *px, const float *p
         INV_HIP(hipMemcpy(c.mu.p, m.data(), b, hipMemcpyHostToDevice));
         INV_HIP(hipMemcpy(c.rho.p, r.data(), b, hipMemcpyHostToDevice));
         c.have_model = true;
+        c.slot_models = false;
     });
 }
 
@@ -520,6 +571,8 @@ static void run_steps(Ctx &c, int kind, int first, int count, int flags, int it0
     INV_REQUIRE(c.variant == INV_VARIANT_FUSED || (c.snap_format == INV_SNAP_F32 && c.ckpt == 0),
                 INV_ERR_UNSUPPORTED,
                 "16-bit snapshots and checkpoints need the fused column kernel (variant 0)");
+    INV_REQUIRE(!c.slot_models || c.variant != INV_VARIANT_SPLIT, INV_ERR_UNSUPPORTED,
+                "per-slot models need a fused step kernel");
     snap_storage(c);
     if (!c.use_graphs || it0 != 0 || it1 != end) {
         run_steps_direct(c, kind, first, count, flags, it0, it1);

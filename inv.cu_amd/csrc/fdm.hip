@@ -380,8 +380,14 @@ void fill_dev(Ctx &c, void *p, long n, float v) {
     INV_HIP(hipGetLastError());
 }
 
+void launch_bdt(float *bdt, const float *rho, float dt, long n, hipStream_t stream) {
+    k_bdt<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(bdt, rho, dt, n);
+    INV_HIP(hipGetLastError());
+}
+
 void fdm_prepare_model(Ctx &c) {
-    if (c.math != INV_MATH_FAST) return;
+    // per-slot models come with their own dt / rho
+    if (c.math != INV_MATH_FAST || c.slot_models) return;
     // the model may have changed through its device pointers since the last
     // run (the optimizer updates it in place): rebuild dt / rho.  The buffer
     // is allocated by inv_grid_set with the model.
@@ -669,6 +675,31 @@ void fdm_adjoint(Ctx &c, int first, int count, int it0, int it1) {
                          c.vy.as<float>());
     }
     if (!direct && it1 == c.g.nt) fdm_kslot_reduce(c, count);
+}
+
+// out[3][N] = 0 + kslot[s0] + kslot[s0+1] + ... in slot order
+__global__ void k_kslot_sum(float *__restrict__ out, const float *__restrict__ ks, long N, int s0,
+                            int n) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    float a = 0.0f, m = 0.0f, r = 0.0f;
+    for (int b = s0; b < s0 + n; b++) {
+        const float *s = ks + (long)b * 3 * N;
+        a += s[k];
+        m += s[N + k];
+        r += s[2 * N + k];
+    }
+    out[k] = a;
+    out[N + k] = m;
+    out[2 * N + k] = r;
+}
+
+void kslot_sum(Ctx &c, float *out, int s0, int n) {
+    INV_REQUIRE(c.kslot.p && s0 >= 0 && n >= 1 && s0 + n <= c.g.batch, INV_ERR_STATE,
+                "per-slot kernel accumulators missing or slot range out of bounds");
+    k_kslot_sum<<<(unsigned)((c.N + 255) / 256), 256, 0, c.stream>>>(out, c.kslot.as<float>(),
+                                                                    c.N, s0, n);
+    INV_HIP(hipGetLastError());
 }
 
 void fdm_kslot_reduce(Ctx &c, int count) {

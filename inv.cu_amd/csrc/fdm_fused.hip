@@ -33,6 +33,10 @@ struct StepArgs {
     Geo g;
     int tiles_x, tiles_z;
     const float *rho, *lam, *mu, *abx, *abzb, *abzt;
+    // Per-slot models ([batch][3] pointers {rho or dt/rho, lam, mu}), or null
+    // when every slot shares rho/lam/mu above.  Batches of shots from
+    // different models (the request server's lanes) set it.
+    const float *const *slot_model;
     const float *vx0, *vz0, *sxx0, *szz0, *sxz0, *vy0, *sxy0, *szy0;  // step n
     float *vx1, *vz1, *sxx1, *szz1, *sxz1, *vy1, *sxy1, *szy1;        // step n+1
     float *ux, *uz, *uy;                                              // in place
@@ -90,6 +94,12 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
     const Geo &g = a.g;
     const int nx = g.nx, nz = g.nz;
     const long so = (long)b * g.N;
+    const float *m_rho = a.rho, *m_lam = a.lam, *m_mu = a.mu;
+    if (a.slot_model) {
+        m_rho = a.slot_model[3 * b];
+        m_lam = a.slot_model[3 * b + 1];
+        m_mu = a.slot_model[3 * b + 2];
+    }
 
     // 0. issue the global loads of phases 2 and 3 first, so that they are in
     //    flight together with the stress loads of phase 1 (the barriers below
@@ -103,7 +113,7 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
         const bool in = e < VR * VW && i >= 0 && i < nx && j >= 0 && j < nz;
         const bool own = in && vr >= 1 && vr <= TX && vc >= 1 && vc <= FZ;
         const long kc = (long)i * nz + j;
-        prho[k] = in ? a.rho[kc] : 1.0f;
+        prho[k] = in ? m_rho[kc] : 1.0f;
         if (PSV) {
             pvx[k] = in ? a.vx0[so + kc] : 0.0f;
             pvz[k] = in ? a.vz0[so + kc] : 0.0f;
@@ -122,8 +132,8 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
         const int i = i0 + e / FZ, j = j0 + (e % FZ);
         const bool in = i < nx && j < nz;
         const long kc = (long)i * nz + j;
-        pmu[k] = in ? a.mu[kc] : 0.0f;
-        if (PSV) plam[k] = in ? a.lam[kc] : 0.0f;
+        pmu[k] = in ? m_mu[kc] : 0.0f;
+        if (PSV) plam[k] = in ? m_lam[kc] : 0.0f;
     }
 
     // 1. stress of step n on the tile plus halo
@@ -334,6 +344,12 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
     const Geo &g = a.g;
     const int nx = g.nx, nz = g.nz;
     const long so = (long)b * g.N;
+    const float *m_rho = a.rho, *m_lam = a.lam, *m_mu = a.mu;
+    if (a.slot_model) {
+        m_rho = a.slot_model[3 * b];
+        m_lam = a.slot_model[3 * b + 1];
+        m_mu = a.slot_model[3 * b + 2];
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
 
@@ -365,7 +381,7 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
     for (int p = 0; p < PV; p++) {
         const int vr = wave + p * NWAVE;
         const long kc = (long)clampi(i0 - 1 + vr, 0, nx - 1) * nz + jvc;
-        qr[p] = a.rho[kc];
+        qr[p] = m_rho[kc];
         if (PSV) {
             qv1[p] = a.vx0[so + kc];
             qv2[p] = a.vz0[so + kc];
@@ -381,8 +397,8 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
     for (int p = 0; p < PT; p++) {
         const int tr = wave + p * NWAVE;
         const long kc = (long)clampi(i0 + tr, 0, nx - 1) * nz + jtc;
-        qmu[p] = a.mu[kc];
-        if (PSV) qlam[p] = a.lam[kc];
+        qmu[p] = m_mu[kc];
+        if (PSV) qlam[p] = m_lam[kc];
     }
     const float abz = a.abzb[jvc], abt = a.abzt[jvc];
 
@@ -996,6 +1012,7 @@ static void common_args(Ctx &c, StepArgs &a) {
     a.rho = c.math == INV_MATH_FAST ? c.bdt.as<float>() : c.rho.as<float>();
     a.lam = c.lam.as<float>();
     a.mu = c.mu.as<float>();
+    a.slot_model = c.slot_models ? c.slot_model.as<const float *const>() : nullptr;
     a.abx = c.abx.as<float>();
     a.abzb = c.abzb.as<float>();
     a.abzt = c.abzt.as<float>();

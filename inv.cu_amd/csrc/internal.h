@@ -111,6 +111,15 @@ struct Ctx {
     int tile_x = 8;    // rows per fused tile (8, 16 or 32)
     int math = INV_MATH_FAST;
     DevBuf bdt;        // dt / rho, rebuilt at the start of each fast-mode run
+    // per-slot models: [batch][3] device pointers {rho (exact) or dt/rho
+    // (fast), lam, mu} owned by the caller (the request server's model
+    // store); when set, lam/mu/rho/bdt above are not read by the fused
+    // kernels.  inv_grid_set clears it.
+    DevBuf slot_model;
+    bool slot_models = false;
+    // keep snapshot and per-slot kernel buffers across inv_grid_set (they only
+    // grow), for contexts that run one batch after another of similar size
+    bool retain = false;
     DevBuf snap;   // [batch][nslots][nsnapf][N] of FP32, FP16 or BF16
     DevBuf smax;   // max-abs of each snapshot field [nslots][batch][nsnapf] (FP32 bits)
     DevBuf ckbuf;  // state checkpoints [nt/ckpt - 1][state fields][batch][N]
@@ -166,6 +175,15 @@ struct Ctx {
     int slot_offset_traces() const { return ncomp * nrec * g.nt; }
 };
 
+// capi.hip ------------------------------------------------------------------
+// every device buffer a context owns (released by inv_ctx_destroy)
+std::vector<DevBuf *> ctx_buffers(Ctx &c);
+// device bytes a context holds now
+size_t ctx_bytes_held(Ctx &c);
+// upper bound of the device bytes a context holds after a forward (+ adjoint)
+// run of desc with nsrc sources, nrec receivers and the given snapshot storage
+size_t ctx_bytes_estimate(const inv_grid_desc &d, int nsrc, int nrec, int snap_format, int ckpt);
+
 // fdm.hip -------------------------------------------------------------------
 void build_point_list(Ctx &c, PointList &pl, int nsets, const std::vector<std::vector<int>> &cells,
                       const std::vector<std::vector<int>> &rows);
@@ -175,6 +193,11 @@ void fdm_adjoint(Ctx &c, int first, int count, int it0, int it1);
 void fdm_zero_state(Ctx &c, int count);
 void fdm_prepare_model(Ctx &c);
 void fdm_kslot_reduce(Ctx &c, int count);
+// dt / rho of the fast math mode into bdt[n] on a stream
+void launch_bdt(float *bdt, const float *rho, float dt, long n, hipStream_t stream);
+// out[3][N] = sum over slots s0..s0+n-1 (in slot order, from 0) of the
+// per-slot kernel accumulators kslot[batch][3][N]
+void kslot_sum(Ctx &c, float *out, int s0, int n);
 Geo make_geo(const Ctx &c);
 void fill_dev(Ctx &c, void *p, long n, float v = 0.0f);
 // interaction kernels for adjoint step it (it % sfe == 0) with the adjoint
@@ -207,6 +230,14 @@ void prof_release(Ctx &c);
 
 // misfit.hip ----------------------------------------------------------------
 void misfit_waveform(Ctx &c, int count, int first);
+// Waveform misfit of slots 0..count-1 whose traces are shorter than the run:
+// slot b holds nt_b = slot_nt[b] samples at the start of each [nt] row of
+// out and obs, its taper is row b of taper_tab [count][nt], and its adjoint
+// source is written time-reversed into the last nt_b samples of each adstf
+// row, the first nt - nt_b samples zeroed.  Device arrays.
+void misfit_waveform_slots(Ctx &c, int count, const int *slot_nt, const float *taper_tab);
+// waveform taper of misfit/waveform.h:9-27 for nt samples of step dt
+void waveform_taper(int nt, float dt, float *tw);
 void misfit_envelope(Ctx &c, int count, int first);
 void envelope_trace(Ctx &c, const float *syn, const float *obs, float *adstf, double *chi_dev);
 double misfit_trace(Ctx &c, int type, const float *syn, const float *obs, float *adstf);
@@ -223,3 +254,6 @@ float vec_amax(Ctx &c, const float *a, size_t n);
 void sum_f64(Ctx &c, const double *x, size_t n, double *dev_out);
 
 }  // namespace inv
+
+// The opaque context of the C ABI is the internal one.
+struct inv_ctx : public inv::Ctx {};

@@ -42,6 +42,45 @@ __global__ __launch_bounds__(MISFIT_BLOCK) void k_waveform(
     if (threadIdx.x == 0) chi[((long)b * nrec + ir) * ncomp + c] = (double)((float)sqrt(ss) * sqrt_dt);
 }
 
+// The same misfit for slots whose traces are shorter than the run (the
+// request server pads a batch to its longest member): slot b uses the first
+// nt_b samples of its rows and its own taper row, and writes its adjoint
+// source into the last nt_b samples of an nt-sample row (zeros before), so
+// that an adjoint of nt steps injects nothing until step nt - nt_b and then
+// exactly what an adjoint of nt_b steps would.  The per-sample arithmetic and
+// the FP64 summation order are those of k_waveform at nt = nt_b.
+__global__ __launch_bounds__(MISFIT_BLOCK) void k_waveform_slots(
+    const float *__restrict__ syn_all, const float *__restrict__ obs_all,
+    float *__restrict__ adstf_all, const float *__restrict__ tw_all,
+    const int *__restrict__ slot_nt, double *__restrict__ chi, int nt, int nrec, int ncomp,
+    float sqrt_dt) {
+    const int c = blockIdx.x / nrec, ir = blockIdx.x % nrec, b = blockIdx.y;
+    const long base = ((long)b * ncomp + c) * nrec * nt + (long)ir * nt;
+    const float *syn = syn_all + base, *obs = obs_all + base, *tw = tw_all + (long)b * nt;
+    const int nb = slot_nt[b], pad = nt - nb;
+    float *ad = adstf_all + base + pad;
+    for (int it = threadIdx.x; it < pad; it += MISFIT_BLOCK) adstf_all[base + it] = 0.0f;
+    double ss = 0;
+    for (int it = threadIdx.x; it < nb; it += MISFIT_BLOCK) {
+        const float w = tw[it];
+        const float r = (syn[it] - obs[it]) * w;
+        ad[nb - 1 - it] = r * w * 2;
+        ss += (double)r * (double)r;
+    }
+    ss = block_sum<MISFIT_BLOCK>(ss);
+    if (threadIdx.x == 0) chi[((long)b * nrec + ir) * ncomp + c] = (double)((float)sqrt(ss) * sqrt_dt);
+}
+
+void misfit_waveform_slots(Ctx &c, int count, const int *slot_nt, const float *taper_tab) {
+    INV_REQUIRE(count >= 1 && count <= c.g.batch, INV_ERR_ARG, "slot count out of range");
+    const float sqrt_dt = sqrtf(c.g.dt);
+    dim3 grid(c.ncomp * c.nrec, count);
+    k_waveform_slots<<<grid, MISFIT_BLOCK, 0, c.stream>>>(
+        c.out.as<float>(), c.obs.as<float>(), c.adstf.as<float>(), taper_tab, slot_nt,
+        c.chi.as<double>(), c.g.nt, c.nrec, c.ncomp, sqrt_dt);
+    INV_HIP(hipGetLastError());
+}
+
 void misfit_waveform(Ctx &c, int count, int first) {
     const long tr = (long)c.ncomp * c.nrec * c.g.nt;
     const float sqrt_dt = sqrtf(c.g.dt);

@@ -105,7 +105,17 @@ SIGNATURES = {
     "inv_misfit_trace": [_vp, _i, _vp, _vp, _vp, _vp],
     "inv_set_math": [_vp, _i],
     "inv_set_graphs": [_vp, _i],
+    "inv_server_create": [_vp, _vp],
+    "inv_server_destroy": [_vp],
+    "inv_server_last_error": [_vp],
+    "inv_server_model_add": [_vp, _i, _i, _vp, _vp, _vp, _vp],
+    "inv_server_model_remove": [_vp, _i],
+    "inv_server_submit": [_vp, _vp, _vp],
+    "inv_server_wait": [_vp, C.c_longlong, C.c_double, _vp],
+    "inv_server_poll": [_vp, C.c_longlong, _vp],
+    "inv_server_stats_get": [_vp, _vp],
 }
+_CHAR_RESULT = {"inv_last_error", "inv_server_last_error"}
 
 _lib = None
 
@@ -121,7 +131,7 @@ def load():
         for name, args in SIGNATURES.items():
             fn = getattr(L, name)
             fn.argtypes = args
-            fn.restype = C.c_char_p if name == "inv_last_error" else C.c_int
+            fn.restype = C.c_char_p if name in _CHAR_RESULT else C.c_int
         _lib = L
     return _lib
 
@@ -418,3 +428,168 @@ class Context:
         ms = C.c_double()
         self._chk(self.L.inv_prof_read(self.h, kernel, C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+
+# ---------------------------------------------------------------------------
+# request server (inv_server_* of include/inv_hip.h)
+# ---------------------------------------------------------------------------
+INV_REQ_QUEUED, INV_REQ_RUNNING, INV_REQ_DONE, INV_REQ_FAILED = 0, 1, 2, 3
+
+
+class ServerDesc(C.Structure):
+    _fields_ = [("device", C.c_int), ("hbm_budget", C.c_size_t), ("max_lanes", C.c_int),
+                ("max_slots", C.c_int), ("max_bypass", C.c_int), ("snap_format", C.c_int),
+                ("math", C.c_int), ("tile_x", C.c_int), ("plain", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("dx", C.c_float), ("dz", C.c_float),
+                ("dt", C.c_float), ("nt", C.c_int), ("sh", C.c_int), ("psv", C.c_int),
+                ("adjoint", C.c_int), ("trace_type", C.c_int), ("sfe", C.c_int),
+                ("abs_width", C.c_int), ("abs_alpha", C.c_float), ("abs_left", C.c_int),
+                ("abs_right", C.c_int), ("abs_top", C.c_int), ("abs_bottom", C.c_int),
+                ("model", C.c_int), ("nshots", C.c_int), ("src_ix", _vp), ("src_iz", _vp),
+                ("stf_x", _vp), ("stf_y", _vp), ("stf_z", _vp), ("nrec", C.c_int),
+                ("rec_ix", _vp), ("rec_iz", _vp), ("obs", _vp), ("traces", _vp),
+                ("k_lambda", _vp), ("k_mu", _vp), ("k_rho", _vp)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("error", C.c_int), ("misfit", C.c_double),
+                ("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double),
+                ("lane", C.c_int), ("batch_shots", C.c_int), ("batch_requests", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong),
+                ("failed", C.c_longlong), ("batches", C.c_longlong), ("shots", C.c_longlong),
+                ("cell_updates", C.c_double), ("budget", C.c_size_t),
+                ("models_bytes", C.c_size_t), ("bytes_held", C.c_size_t),
+                ("bytes_peak", C.c_size_t), ("bytes_actual", C.c_size_t),
+                ("lanes_busy", C.c_int), ("queued", C.c_int), ("now", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Server:
+    """The library's request server (inv_server): resident models, lanes with
+    their own streams, continuous batching of shot-solve requests.
+
+    submit() keeps the request's numpy arrays alive until wait() returns; the
+    result arrays (traces, and kernels for adjoint requests) are allocated
+    here and returned by wait()."""
+
+    def __init__(self, device=0, hbm_budget=0, max_lanes=4, max_slots=16, max_bypass=4,
+                 snap_format=INV_SNAP_F32, math=INV_MATH_FAST, tile_x=8, plain=False):
+        self.L = load()
+        d = ServerDesc(device, hbm_budget, max_lanes, max_slots, max_bypass, snap_format, math,
+                       tile_x, 1 if plain else 0)
+        h = C.c_void_p()
+        rc = self.L.inv_server_create(C.byref(d), C.byref(h))
+        if rc != INV_OK:
+            raise InvError(rc, "inv_server_create failed")
+        self.h = h
+        self._live = {}
+
+    def _chk(self, rc):
+        if rc != INV_OK:
+            raise InvError(rc, self.L.inv_server_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.inv_server_destroy(self.h)
+            self.h = None
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def model_add(self, lam, mu, rho, nx, nz):
+        a = [np.ascontiguousarray(x, np.float32).ravel() for x in (lam, mu, rho)]
+        assert all(x.size == nx * nz for x in a)
+        mid = C.c_int()
+        self._chk(self.L.inv_server_model_add(self.h, nx, nz, *(_p(x) for x in a),
+                                              C.byref(mid)))
+        return mid.value
+
+    def model_remove(self, mid):
+        self._chk(self.L.inv_server_model_remove(self.h, mid))
+
+    def submit(self, *, model, nx, nz, dx, dt, nt, src_ix, src_iz, stf_x, stf_y, stf_z,
+               rec_ix, rec_iz, sh=0, psv=1, adjoint=False, obs=None, trace_type=0, sfe=10,
+               abs_width=20, abs_alpha=0.015, abs_left=1, abs_right=1, abs_top=1,
+               abs_bottom=1, dz=None, kernels=True):
+        """Queue one request; returns its id."""
+        ns = len(src_ix)
+        ncomp = 1 if sh else 2
+        keep = {
+            "src_ix": np.ascontiguousarray(src_ix, np.int32),
+            "src_iz": np.ascontiguousarray(src_iz, np.int32),
+            "stf_x": np.ascontiguousarray(stf_x, np.float32).reshape(ns, nt),
+            "stf_y": np.ascontiguousarray(stf_y, np.float32).reshape(ns, nt),
+            "stf_z": np.ascontiguousarray(stf_z, np.float32).reshape(ns, nt),
+            "rec_ix": np.ascontiguousarray(rec_ix, np.int32),
+            "rec_iz": np.ascontiguousarray(rec_iz, np.int32),
+            "traces": np.zeros((ns, ncomp, len(rec_ix), nt), np.float32),
+        }
+        r = Request()
+        r.nx, r.nz, r.dx, r.dz, r.dt, r.nt = nx, nz, dx, dx if dz is None else dz, dt, nt
+        r.sh, r.psv = (1, 0) if sh else (0, 1)
+        r.adjoint = 1 if adjoint else 0
+        r.trace_type, r.sfe = trace_type, sfe
+        r.abs_width, r.abs_alpha = abs_width, abs_alpha
+        r.abs_left, r.abs_right, r.abs_top, r.abs_bottom = abs_left, abs_right, abs_top, abs_bottom
+        r.model, r.nshots, r.nrec = model, ns, len(rec_ix)
+        for k in ("src_ix", "src_iz", "stf_x", "stf_y", "stf_z", "rec_ix", "rec_iz", "traces"):
+            setattr(r, k, _p(keep[k]).value)
+        if adjoint:
+            keep["obs"] = np.ascontiguousarray(obs, np.float32).reshape(ns, ncomp, len(rec_ix), nt)
+            r.obs = _p(keep["obs"]).value
+            if kernels:
+                for k in ("k_lambda", "k_mu", "k_rho"):
+                    keep[k] = np.zeros(nx * nz, np.float32)
+                    setattr(r, k, _p(keep[k]).value)
+        rid = C.c_longlong()
+        self._chk(self.L.inv_server_submit(self.h, C.byref(r), C.byref(rid)))
+        keep["req"] = r
+        self._live[rid.value] = keep
+        return rid.value
+
+    def wait(self, rid, timeout=-1.0):
+        """Wait for a request: (info dict, traces, kernels or None)."""
+        info = RequestInfo()
+        rc = self.L.inv_server_wait(self.h, rid, float(timeout), C.byref(info))
+        if rc == INV_ERR_STATE_TIMEOUT and info.status in (INV_REQ_QUEUED, INV_REQ_RUNNING):
+            raise TimeoutError(f"request {rid} still {info.status}")
+        keep = self._live.pop(rid, None)
+        self._chk(rc)
+        ks = None
+        if keep is not None and "k_lambda" in keep:
+            ks = (keep["k_lambda"], keep["k_mu"], keep["k_rho"])
+        return info.as_dict(), (keep["traces"] if keep else None), ks
+
+    def poll(self, rid):
+        info = RequestInfo()
+        self._chk(self.L.inv_server_poll(self.h, rid, C.byref(info)))
+        return info.as_dict()
+
+    def stats(self):
+        st = ServerStats()
+        self._chk(self.L.inv_server_stats_get(self.h, C.byref(st)))
+        return st.as_dict()
+
+
+INV_ERR_STATE_TIMEOUT = -3

@@ -266,3 +266,58 @@ def tiny(nx=48, nz=40, dx=10.0, dt=1e-3, nt=300, nsrc=3, nrec=12, sh=1, psv=1, s
     return Problem(name, nx, nz, f32(dx), f32(dx), f32(dt), nt, sh, psv, lam, mu, rho.ravel(),
                    lam0, mu0, rho0.ravel(), src_ix, src_iz, sx, sy, sz, rec_ix, rec_iz, sfe=sfe,
                    abs_width=abs_width, meta=dict(f0=f0, t0=t0, seed=seed))
+
+
+# ---------------------------------------------------------------------------
+# mixed-size shot-solve requests for the request server
+# ---------------------------------------------------------------------------
+SERVE_GRIDS = ((256, 256), (512, 512), (1024, 512), (2048, 1024))
+
+
+def layered_random(nx, nz, seed, vmin=1500.0, vmax=4700.0):
+    """Seeded random layered model: 3-8 flat layers whose vp rises with depth
+    between vmin and vmax (+-150 m/s jitter), vs = vp/1.73, Gardner density
+    310 vp^0.25.  Returns lambda, mu, rho (FP32, z fastest)."""
+    rng = np.random.default_rng(seed)
+    nl = int(rng.integers(3, 9))
+    tops = np.sort(rng.integers(1, nz - 1, nl - 1))
+    vp_l = np.sort(rng.uniform(vmin, vmax, nl)) + rng.uniform(-150.0, 150.0, nl)
+    vp_l = np.clip(vp_l, vmin, vmax)
+    layer = np.searchsorted(tops, np.arange(nz), side="right")
+    vp = np.broadcast_to(vp_l[layer][None, :], (nx, nz)).astype(np.float32)
+    vs = (vp / np.float32(1.73)).astype(np.float32)
+    rho = (310.0 * vp.astype(np.float64) ** 0.25).astype(np.float32)
+    lam, mu = vps2lm(vp.ravel(), vs.ravel(), rho.ravel())
+    return lam, mu, rho.ravel()
+
+
+def serving_request(rid, seed=3, dx=10.0, dt=1e-3, sfe=10, f0=8.0, t0=0.15, amp=1e10,
+                    grids=SERVE_GRIDS, nt_range=(1000, 6000), max_shots=8):
+    """Request `rid` of the mixed-size serving load: grid drawn uniformly from
+    `grids`, nt uniform over the multiples of sfe in nt_range, SH or P-SV and
+    forward-only or forward+adjoint with probability 1/2 each, 1..max_shots
+    shots evenly spaced at 2 cells depth, a random layered model seeded
+    1000 + rid, receivers every 4 cells on row 2 (the same for every request
+    on a grid, so like-sized requests can share a launch).  Observed traces of
+    adjoint requests are zero (the misfit is then the tapered trace energy;
+    the work is the same as for real data)."""
+    rng = np.random.default_rng([seed, rid])
+    nx, nz = grids[int(rng.integers(len(grids)))]
+    nt = int(rng.integers(nt_range[0] // sfe, nt_range[1] // sfe + 1)) * sfe
+    sh = bool(rng.random() < 0.5)
+    adjoint = bool(rng.random() < 0.5)
+    ns = int(rng.integers(1, max_shots + 1))
+    xmax = (nx - 1) * dx
+    src_ix = np.array([invhip.calc_idx((s + 0.5) * xmax / ns, xmax, nx) for s in range(ns)],
+                      np.int32)
+    sx, sy, sz = _wavelets(nt, dt, f0, t0, amp, 0.0, ns)
+    rec_ix = np.arange(2, nx - 2, 4, dtype=np.int32)
+    return dict(rid=rid, model_seed=1000 + rid, nx=nx, nz=nz, dx=float(dx), dt=float(dt), nt=nt,
+                sh=int(sh), psv=int(not sh), adjoint=adjoint, sfe=sfe, src_ix=src_ix,
+                src_iz=np.full(ns, 2, np.int32), stf_x=sx, stf_y=sy, stf_z=sz, rec_ix=rec_ix,
+                rec_iz=np.full(len(rec_ix), 2, np.int32))
+
+
+def request_cell_updates(req):
+    """Cell-updates of a request: shots * nx * nz * nt * (2 with the adjoint)."""
+    return len(req["src_ix"]) * req["nx"] * req["nz"] * req["nt"] * (2 if req["adjoint"] else 1)
