@@ -39,6 +39,7 @@ PSV_STEP_BYTES = 4 * (2 * 7 + 3)
 # profiles/k_col_psv_marmousi737x240_16shots.md.  Only reported for that
 # configuration.
 MEASURED_TRAFFIC_PER_CELL = 69.33
+PROFILED_TILE = (0, 8, 1, 1, 16)  # variant, rows, columns per lane, math, batch
 
 
 def parse():
@@ -55,12 +56,56 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the one-shot GPU path")
     ap.add_argument("--cpu-shots", type=int, default=1)
     ap.add_argument("--variant", type=int, default=0, help="0 fused step kernel, 1 kernel pair")
-    ap.add_argument("--tile-x", type=int, default=8, help="fused tile rows (8, 16, 32)")
+    ap.add_argument("--tile-x", type=int, default=0,
+                    help="fused tile rows (0: inv_autotune picks rows and columns per lane)")
+    ap.add_argument("--cols", type=int, default=1, help="columns per lane with --tile-x")
     ap.add_argument("--math", type=int, default=1, help="0 exact (IEEE divisions), 1 fast")
     ap.add_argument("--graphs", type=int, default=1, help="capture time loops in HIP graphs")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--serve-requests", type=int, default=12,
+                    help="mixed-size serving leg: requests per run (0 = skip)")
+    ap.add_argument("--serve-rate-x", type=float, default=1.0,
+                    help="serving leg arrival rate, in multiples of the plain loop's capacity")
+    ap.add_argument("--serving-sweep", type=int, default=0,
+                    help="run only the serving sweep over this many requests (one JSON line "
+                         "per run) and exit")
     return ap.parse_args()
+
+
+def serving_leg(inv, n, rate_x, snap_format):
+    """p50/p99 latency and throughput of the request server against the plain
+    one-request-at-a-time loop on one Poisson arrival sequence of n mixed-size
+    requests (grids 256x256 .. 2048x1024, nt 1000-6000, SH/P-SV, forward or
+    forward+adjoint, 1-8 shots), at rate_x times the plain loop's capacity."""
+    S = inv.serving
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    reqs = S.make_requests(n)
+    kw = dict(snap_format=snap_format, max_lanes=4, max_slots=16)
+    base, _ = S.run_load(reqs, rate=None, plain=True, log=log, **kw)
+    rate = rate_x * base["requests_per_s"]
+    plain, _ = S.run_load(reqs, rate=rate, plain=True, log=log, **kw)
+    batched, _ = S.run_load(reqs, rate=rate, plain=False, log=log, **kw)
+    keys = ("p50_latency_s", "p99_latency_s", "requests_per_s", "gcell_per_s", "batches",
+            "mean_requests_per_batch", "peak_bytes")
+    return {"requests": n, "rate_per_s": round(rate, 4), "rate_x_plain_capacity": rate_x,
+            "plain_capacity_req_per_s": round(base["requests_per_s"], 4),
+            "snapshots": ["fp32", "fp16", "bf16"][snap_format],
+            "plain": {k: plain[k] for k in keys}, "batched": {k: batched[k] for k in keys},
+            "p99_speedup": round(plain["p99_latency_s"] / batched["p99_latency_s"], 3)}
+
+
+def configure_kernel(ctx, a, count):
+    """The step kernel: autotuned (default) or as given; returns its tile."""
+    if a.variant == 0 and a.tile_x == 0:
+        v, tx, cols = ctx.autotune(0, count, steps=20)
+    elif a.variant == 0:
+        ctx.set_tile(a.tile_x, a.cols)
+        v, tx, cols = 0, a.tile_x, a.cols
+    else:
+        ctx.set_variant(a.variant, a.tile_x)
+        v, tx, cols = a.variant, a.tile_x, 1
+    return {"variant": v, "rows": tx, "cols_per_lane": cols}
 
 
 def cpu_baseline(P, shots):
@@ -97,14 +142,20 @@ def main():
     import __graft_entry__ as graft
 
     inv = graft.load_package()
+    if a.serving_sweep:
+        log = lambda m: print(m, file=sys.stderr, flush=True)
+        for s in inv.serving.sweep(a.serving_sweep, log=log,
+                                   snap_format=inv.invhip.INV_SNAP_F16):
+            print(json.dumps(s), flush=True)
+        return
     P = inv.problems.marmousi(nt=a.nt, sigma=a.sigma)
     B = min(a.batch, P.nsrc)
     print(f"# workload {P.name} {P.nx}x{P.nz} shots={P.nsrc} nt={P.nt} batch={B}",
           file=sys.stderr, flush=True)
     ctx = P.context(batch=B, model="true")
-    ctx.set_variant(a.variant, a.tile_x)
     ctx.set_math(a.math)
     ctx.set_graphs(a.graphs)
+    tile = configure_kernel(ctx, a, B)
     # observed data: forward on the true model (Misfit::init, misfit.h:135-148)
     for first in range(0, P.nsrc, B):
         n = min(B, P.nsrc - first)
@@ -152,14 +203,14 @@ def main():
         # replayed forward graphs: every launch in them but a handful of
         # set-up fills is the step kernel, so time / steps is its duration
         step_s = tfr / nfr / 1e3
-        kernel = "k_col<false,true,TX,FAST> (fused P-SV time step), from forward-graph replays"
+        kernel = "fused P-SV time step (k_col / k_colv, autotuned tile), from forward-graph replays"
         avg = {"step": round(step_s * 1e6, 2)}
         samples = {"forward_steps": nfr}
         if nar:
             avg["adjoint_step_incl_interaction"] = round(tar / nar * 1e3, 2)
     elif nf:
         step_s = tf / nf / 1e3
-        kernel = "k_col<false,true,TX,FAST> (fused P-SV time step)"
+        kernel = "fused P-SV time step (k_col / k_colv)"
         avg = {"step": round(tf / nf * 1e3, 2)}
         samples = {"step": nf}
     elif nv and ns:
@@ -169,7 +220,8 @@ def main():
         samples = {"velocity": nv, "stress": ns}
     if step_s:
         achieved = PSV_STEP_BYTES * cells / step_s / 1e9
-        profiled = (a.variant, a.tile_x, a.math, B) == (0, 8, 1, 16) and nfr
+        profiled = ((tile["variant"], tile["rows"], tile["cols_per_lane"], a.math, B) ==
+                    PROFILED_TILE and nfr)
         traffic = round(MEASURED_TRAFFIC_PER_CELL * cells) if profiled else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
@@ -181,7 +233,7 @@ def main():
     plain = None
     if not a.no_plain:
         pctx = P.context(batch=1, model="true")
-        pctx.set_variant(a.variant, a.tile_x)
+        configure_kernel(pctx, a, 1)
         pctx.set_math(a.math)
         pctx.set_graphs(False)  # the plain path: one shot per launch, no graph
         for s in range(P.nsrc):
@@ -201,6 +253,9 @@ def main():
         print(f"# plain path: {pms:.1f} ms per gradient", file=sys.stderr, flush=True)
 
     cpu = None if a.no_cpu else cpu_baseline(P, a.cpu_shots)
+    serving = None
+    if a.serve_requests > 0:
+        serving = serving_leg(inv, a.serve_requests, a.serve_rate_x, inv.invhip.INV_SNAP_F16)
 
     out = {
         "metric": METRIC,
@@ -219,11 +274,12 @@ def main():
                                "16 shots, 368 receivers, dt 1 ms, nt 4000",
                    "nx": P.nx, "nz": P.nz, "shots": P.nsrc, "receivers": P.nrec, "nt": P.nt,
                    "batch": B, "sfe": P.sfe, "filter_sigma": P.sigma, "misfit": "waveform",
-                   "snapshots": "fp32 in HBM"},
+                   "snapshots": "fp32 in HBM", "step_kernel_tile": tile},
         "gradients_per_hr": round(3.6e6 / ms, 1),
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain": plain,
+        "serving": serving,
     }
     if ni:
         out["interaction_avg_us"] = round(ti / ni * 1e3, 2)

@@ -227,6 +227,23 @@ int inv_prof_read(inv_ctx *ctx, int kernel, long long *count, double *total_ms);
  * are identical between variants. */
 enum inv_variant { INV_VARIANT_FUSED = 0, INV_VARIANT_SPLIT = 1, INV_VARIANT_FUSED_LINEAR = 2 };
 int inv_set_variant(inv_ctx *ctx, int variant, int tile_x);
+/* The fused column kernel with tiles of tile_x rows and cols_per_lane
+ * adjacent columns per lane (float2 / float4 loads and stores): 1 column per
+ * lane with 8, 16 or 32 rows (58 columns per tile), 2 with 4, 8 or 16 rows
+ * (120 columns), 4 with 4 or 8 rows (248 columns); 2 and 4 need nz divisible
+ * by them.  Results are identical for every tile.  Selects variant 0. */
+int inv_set_tile(inv_ctx *ctx, int tile_x, int cols_per_lane);
+/* Autotuner: times `steps` forward steps of shots first..first+count-1 (with
+ * snapshots and as many adjoint steps when the context keeps per-slot kernel
+ * accumulators, batch > 1) with each tile of inv_set_tile that nz allows and
+ * selects the fastest (a tile must beat the ones tried before it, in the
+ * order above, by 2 %).  The
+ * choice is cached per (nx, nz, wave types, math mode) for the process, so
+ * later contexts and batch sizes of the same grid take it without timing.
+ * Overwrites the slots' wavefields, traces and snapshots.  No counterpart in the
+ * reference (fixed 32x32 blocks, lib/device.h:7-65).  Outputs may be NULL. */
+int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, int *tile_rows,
+                 int *cols_per_lane);
 /* Stencil arithmetic: INV_MATH_EXACT evaluates the reference's expressions
  * (solver/fdm.h:76-161) with IEEE FP32 divisions, bitwise equal to them;
  * INV_MATH_FAST (default) multiplies by FP32-rounded coefficients 9/(8h),
@@ -293,7 +310,7 @@ typedef struct inv_server_desc {
     int max_bypass;      /* times the oldest request may be overtaken (0: strict FIFO) */
     int snap_format;     /* INV_SNAP_* of adjoint requests' forward snapshots */
     int math;            /* INV_MATH_* */
-    int tile_x;          /* fused tile rows (8, 16, 32; 0 = 8) */
+    int tile_x;          /* fused tile rows (8, 16, 32); 0: inv_autotune per grid */
     int plain;           /* 1: one request at a time, one shot per launch, in
                             arrival order (the plain baseline loop) */
 } inv_server_desc;

@@ -2,8 +2,11 @@
 // per-context resource management behind it.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <new>
 
 #include "internal.h"
@@ -571,6 +574,8 @@ static void run_steps(Ctx &c, int kind, int first, int count, int flags, int it0
     INV_REQUIRE(c.variant == INV_VARIANT_FUSED || (c.snap_format == INV_SNAP_F32 && c.ckpt == 0),
                 INV_ERR_UNSUPPORTED,
                 "16-bit snapshots and checkpoints need the fused column kernel (variant 0)");
+    INV_REQUIRE(c.variant != INV_VARIANT_FUSED || c.g.nz % c.col_v == 0, INV_ERR_UNSUPPORTED,
+                "the column kernel's columns per lane must divide nz");
     INV_REQUIRE(!c.slot_models || c.variant != INV_VARIANT_SPLIT, INV_ERR_UNSUPPORTED,
                 "per-slot models need a fused step kernel");
     snap_storage(c);
@@ -1080,8 +1085,141 @@ int inv_set_variant(inv_ctx *ctx, int variant, int tile_x) {
         sync(c);
         c.variant = variant;
         if (tile_x) c.tile_x = tile_x;
+        c.col_v = 1;
         invalidate(c);
         fused_invalidate_lists(c);
+    });
+}
+
+// tile heights the column kernel is built for, per columns per lane
+static bool tile_ok(int tile_x, int v) {
+    if (v == 1) return tile_x == 8 || tile_x == 16 || tile_x == 32;
+    if (v == 2) return tile_x == 4 || tile_x == 8 || tile_x == 16;
+    if (v == 4) return tile_x == 4 || tile_x == 8;
+    return false;
+}
+
+int inv_set_tile(inv_ctx *ctx, int tile_x, int cols_per_lane) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(tile_ok(tile_x, cols_per_lane), INV_ERR_ARG,
+                    "tile must be 8/16/32 rows x 1 column per lane, 4/8/16 x 2 or 4/8 x 4");
+        INV_REQUIRE(!c.have_grid || c.g.nz % cols_per_lane == 0, INV_ERR_UNSUPPORTED,
+                    "nz must be divisible by the columns per lane");
+        sync(c);
+        c.variant = INV_VARIANT_FUSED;
+        c.tile_x = tile_x;
+        c.col_v = cols_per_lane;
+        invalidate(c);
+        fused_invalidate_lists(c);
+    });
+}
+
+// ---- autotuner -------------------------------------------------------------
+// Candidate step kernels and tile shapes, timed on the context's own shape;
+// the best is cached per (grid, slots, wave types, math mode) for the process.
+namespace {
+struct TuneChoice {
+    int variant, tile, cols;
+};
+std::mutex tune_mu;
+std::map<std::string, TuneChoice> tune_cache;
+
+void apply_choice(Ctx &c, const TuneChoice &t) {
+    c.variant = t.variant;
+    c.tile_x = t.tile;
+    c.col_v = t.cols;
+    invalidate(c);
+    fused_invalidate_lists(c);
+}
+}  // namespace
+
+int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, int *tile_rows,
+                 int *cols_per_lane) {
+    return guard(ctx, [&](Ctx &c) {
+        require_ready(c);
+        check_slots(c, first, count);
+        INV_REQUIRE(steps >= 1 && steps <= c.g.nt, INV_ERR_ARG, "steps must be in [1, nt]");
+        // shots per launch scale the grid of tiles, not the tile's work: the
+        // choice is kept per grid, wave types and math mode
+        const std::string key = std::to_string(c.g.nx) + "x" + std::to_string(c.g.nz) + "/" +
+                                std::to_string(c.g.sh) + std::to_string(c.g.psv) + "/" +
+                                std::to_string(c.math);
+        TuneChoice best{-1, 0, 0};
+        {
+            std::lock_guard<std::mutex> lk(tune_mu);
+            auto f = tune_cache.find(key);
+            if (f != tune_cache.end()) best = f->second;
+        }
+        if (best.variant < 0) {
+            std::vector<TuneChoice> cand;
+            for (int v : {1, 2, 4})
+                if (c.g.nz % v == 0)
+                    for (int tx : {4, 8, 16, 32})
+                        if (tile_ok(tx, v)) cand.push_back({INV_VARIANT_FUSED, tx, v});
+            const int every = c.prof.every;
+            c.prof.every = 0;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            double best_ms = 1e300;
+            try {
+                INV_HIP(hipEventCreate(&e0));
+                INV_HIP(hipEventCreate(&e1));
+                // what a gradient runs: forward steps storing snapshots, then
+                // adjoint steps with their kernel accumulation (when those go
+                // to per-slot accumulators, which the next adjoint clears;
+                // a one-slot context adds into the kernels themselves)
+                const bool adj = c.g.adj && c.g.sfe > 0 && c.g.batch > 1;
+                const int flags = adj ? INV_FWD_SNAPSHOTS : 0;
+                const bool verbose = getenv("INV_AUTOTUNE_VERBOSE") != nullptr;
+                auto pass = [&]() {
+                    fused_forward(c, first, count, flags, 0, steps);
+                    if (adj) fused_adjoint(c, first, count, 0, steps);
+                };
+                for (size_t k = 0; k < cand.size(); k++) {
+                    const TuneChoice &t = cand[k];
+                    apply_choice(c, t);
+                    fused_prepare(c, 1, first, count);
+                    if (adj) fused_prepare(c, 2, first, count);
+                    // untimed passes (the first candidate's also bring the
+                    // clocks up), then the best of three timed passes
+                    for (int w = 0; w < (k == 0 ? 4 : 1); w++) pass();
+                    float best_t = 1e30f;
+                    for (int rep = 0; rep < 3; rep++) {
+                        INV_HIP(hipEventRecord(e0, c.stream));
+                        pass();
+                        INV_HIP(hipEventRecord(e1, c.stream));
+                        INV_HIP(hipEventSynchronize(e1));
+                        float ms = 0;
+                        INV_HIP(hipEventElapsedTime(&ms, e0, e1));
+                        best_t = std::min(best_t, ms);
+                    }
+                    if (verbose)
+                        fprintf(stderr, "inv_autotune %s: %d rows x %d cols per lane: %.3f ms\n",
+                                key.c_str(), t.tile, t.cols, best_t);
+                    // candidates come in order of preference (one column
+                    // per lane first): a later one must be 2 % faster, so
+                    // that timing noise does not pick between near-ties
+                    if (best_t < 0.98 * best_ms) {
+                        best_ms = best_t;
+                        best = t;
+                    }
+                }
+            } catch (...) {
+                c.prof.every = every;
+                if (e0) (void)hipEventDestroy(e0);
+                if (e1) (void)hipEventDestroy(e1);
+                throw;
+            }
+            c.prof.every = every;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            std::lock_guard<std::mutex> lk(tune_mu);
+            tune_cache[key] = best;
+        }
+        apply_choice(c, best);
+        c.parity = 0;
+        if (variant) *variant = best.variant;
+        if (tile_rows) *tile_rows = best.tile;
+        if (cols_per_lane) *cols_per_lane = best.cols;
     });
 }
 

@@ -23,54 +23,12 @@
 #include <algorithm>
 
 #include "internal.h"
+#include "step.h"
 
 namespace inv {
 
 constexpr int FZ = 64;        // tile width along z: one wavefront
 constexpr int FBLOCK = 256;   // four wavefronts per tile
-
-struct StepArgs {
-    Geo g;
-    int tiles_x, tiles_z;
-    const float *rho, *lam, *mu, *abx, *abzb, *abzt;
-    // Per-slot models ([batch][3] pointers {rho or dt/rho, lam, mu}), or null
-    // when every slot shares rho/lam/mu above.  Batches of shots from
-    // different models (the request server's lanes) set it.
-    const float *const *slot_model;
-    const float *vx0, *vz0, *sxx0, *szz0, *sxz0, *vy0, *sxy0, *szy0;  // step n
-    float *vx1, *vz1, *sxx1, *szz1, *sxz1, *vy1, *sxy1, *szy1;        // step n+1
-    float *ux, *uz, *uy;                                              // in place
-    // point injection (addSTF, fdm.h:163-180), bucketed by velocity region
-    const float *inj_x, *inj_y, *inj_z;
-    long inj_slot_stride;
-    const int *inj_ptr, *inj_cell, *inj_row;
-    int inj_per_slot;
-    int nt, it;
-    // receiver recording (saveRec, fdm.h:181-196), bucketed by tile
-    float *out_x, *out_y, *out_z;
-    long out_slot_stride;
-    const int *rec_ptr, *rec_cell, *rec_row;
-    int rec_disp;
-    // Snapshots (fdm.h:521-560: a slot holds u before and v after a step).
-    // snap_u: slot base (batch slot 0) that receives this step's input u, or
-    // null; snap_v: slot base that receives v, this step's result when
-    // v_after is set, else this step's input (the previous step's result),
-    // or null.  16-bit formats scale each field by the max-abs read from
-    // mx_u / mx_v ([batch][nsnapf] tables of the slot); acc_u / acc_v collect
-    // max |u| and max |v| after this step into the table of a slot still to
-    // be written.  The linear kernel supports FP32 with v_after only.
-    void *snap_u, *snap_v;
-    int v_after, snap_fmt, nsnapf, psv_f0;
-    const unsigned *mx_u, *mx_v;
-    unsigned *acc_u, *acc_v;
-    long snap_slot_stride;  // elements per batch slot
-};
-
-__device__ __forceinline__ float wave_max(float x) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
-    return x;
-}
 
 template <bool SH, bool PSV, int TX, bool FAST>
 __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
@@ -637,10 +595,15 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
 // ---------------------------------------------------------------------------
 // output columns per tile along z: 64 for the linear-index kernel, 58 for the
 // column kernel (whose stress tile, 58 + 6 halo columns, is one wavefront wide)
-static int fz_of(const Ctx &c) { return c.variant == INV_VARIANT_FUSED_LINEAR ? FZ : CZ; }
+// (with 2 or 4 columns per lane: 120 or 248)
+static int fz_of(const Ctx &c) {
+    if (c.variant == INV_VARIANT_FUSED_LINEAR) return FZ;
+    return col_strip(c.col_v);
+}
+static int tile_rows(const Ctx &c) { return c.tile_x; }
 static int tiles_x_of(const Ctx &c, int TX) { return (c.g.nx + TX - 1) / TX; }
 static int tiles_z_of(const Ctx &c) { return (c.g.nz + fz_of(c) - 1) / fz_of(c); }
-static int list_key(const Ctx &c) { return c.tile_x * 1000 + fz_of(c); }
+static int list_key(const Ctx &c) { return tile_rows(c) * 1000 + fz_of(c); }
 
 // CSR of points by tile: a point (i, j) belongs to tile (tx, tz) when
 // tx*TX - lo <= i < (tx+1)*TX + hi and tz*W - lo <= j < (tz+1)*W + hi, W the
@@ -710,8 +673,8 @@ static void ensure_rec_lists(Ctx &c) {
         cells[0].push_back(c.rec_ix[r] * c.g.nz + c.rec_iz[r]);
         rows[0].push_back(r);
     }
-    build_tiled(c, c.f_rec_v, 1, cells, rows, c.tile_x, 1, 2);
-    build_tiled(c, c.f_rec_t, 1, cells, rows, c.tile_x, 0, 0);
+    build_tiled(c, c.f_rec_v, 1, cells, rows, tile_rows(c), 1, 2);
+    build_tiled(c, c.f_rec_t, 1, cells, rows, tile_rows(c), 0, 0);
     c.f_rec_tx = list_key(c);
 }
 
@@ -728,7 +691,7 @@ PointList &source_list(Ctx &c, int first, int count, int key) {
                 rows[b].push_back(is);
             }
     if (key == 0) build_point_list(c, pl, count, cells, rows);
-    else build_tiled(c, pl, count, cells, rows, c.tile_x, 1, 2);
+    else build_tiled(c, pl, count, cells, rows, tile_rows(c), 1, 2);
     return pl;
 }
 
@@ -920,8 +883,8 @@ __global__ void k_snap_flush(const float *__restrict__ v1, const float *__restri
     const int b = blockIdx.y;
     if (k >= N) return;
     unsigned short *d = dst + (long)b * slot_stride + (long)fv * N + k;
-    d[0] = snap_enc16(v1[(long)b * N + k] * snap_enc_scale(mx[b * nsnapf + fv]), fmt);
-    if (v2) d[N] = snap_enc16(v2[(long)b * N + k] * snap_enc_scale(mx[b * nsnapf + fv + 1]), fmt);
+    d[0] = snap_enc_mul(v1[(long)b * N + k], snap_enc_scale(mx[b * nsnapf + fv]), fmt);
+    if (v2) d[N] = snap_enc_mul(v2[(long)b * N + k], snap_enc_scale(mx[b * nsnapf + fv + 1]), fmt);
 }
 
 // acc[b][f] = max(acc[b][f], max |x_b|) over count slots of x [batch][N]
@@ -980,7 +943,10 @@ static void launch_col(Ctx &c, const StepArgs &a, unsigned blocks) {
 
 template <bool FAST>
 static void launch_m(Ctx &c, const StepArgs &a, unsigned blocks) {
-    if (c.variant == INV_VARIANT_FUSED_LINEAR) {
+    if (c.variant == INV_VARIANT_FUSED && c.col_v > 1) {
+        if (c.g.sh) wide_launch<true, false, FAST>(c, a, blocks);
+        if (c.g.psv) wide_launch<false, true, FAST>(c, a, blocks);
+    } else if (c.variant == INV_VARIANT_FUSED_LINEAR) {
         if (c.g.sh && c.g.psv) launch_lin<true, true, FAST>(c, a, blocks);
         else if (c.g.sh) launch_lin<true, false, FAST>(c, a, blocks);
         else launch_lin<false, true, FAST>(c, a, blocks);
@@ -1007,7 +973,7 @@ static void fused_step(Ctx &c, StepArgs &a, const Fields &F, int &par, int count
 static void common_args(Ctx &c, StepArgs &a) {
     a = StepArgs{};
     a.g = make_geo(c);
-    a.tiles_x = tiles_x_of(c, c.tile_x);
+    a.tiles_x = tiles_x_of(c, tile_rows(c));
     a.tiles_z = tiles_z_of(c);
     a.rho = c.math == INV_MATH_FAST ? c.bdt.as<float>() : c.rho.as<float>();
     a.lam = c.lam.as<float>();

@@ -108,7 +108,8 @@ struct Ctx {
     DevBuf vx_b, vz_b, sxx_b, szz_b, sxz_b, vy_b, sxy_b, szy_b;
     int parity = 0;
     int variant = 0;   // 0 fused step kernel, 1 velocity/stress kernel pair
-    int tile_x = 8;    // rows per fused tile (8, 16 or 32)
+    int tile_x = 8;    // rows per fused tile (4, 8, 16 or 32)
+    int col_v = 1;     // columns per lane of the fused column kernel (1, 2 or 4)
     int math = INV_MATH_FAST;
     DevBuf bdt;        // dt / rho, rebuilt at the start of each fast-mode run
     // per-slot models: [batch][3] device pointers {rho (exact) or dt/rho

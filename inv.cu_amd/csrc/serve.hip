@@ -176,7 +176,7 @@ static void configure(Server &S, Lane &L, const ClassInfo &ci, int nt, int batch
     g.nt = nt;
     g.batch = batch;
     check(L.ctx, inv_grid_set(L.ctx, &g));
-    check(L.ctx, inv_set_variant(L.ctx, INV_VARIANT_FUSED, S.d.tile_x));
+    if (S.d.tile_x) check(L.ctx, inv_set_variant(L.ctx, INV_VARIANT_FUSED, S.d.tile_x));
     check(L.ctx, inv_set_math(L.ctx, S.d.math));
     check(L.ctx, inv_set_graphs(L.ctx, 0));
     if (g.adj) check(L.ctx, inv_set_snapshots(L.ctx, S.d.snap_format, 0));
@@ -247,6 +247,9 @@ static void run_batched(Server &S, Lane &L, const std::vector<Request *> &batch)
     check(L.ctx, inv_sources_set(L.ctx, shots, ix.data(), iz.data(), sx.data(), sy.data(),
                                  sz.data()));
     set_slot_models(S, L, batch, shots);
+    // tile of the step kernel: timed once per grid class, cached after that
+    if (!S.d.tile_x)
+        check(L.ctx, inv_autotune(L.ctx, 0, shots, std::min(nt, 10), nullptr, nullptr, nullptr));
     check(L.ctx, inv_forward(L.ctx, 0, shots, adj ? INV_FWD_SNAPSHOTS : 0));
 
     // traces: slot b's rows hold nt samples, the request's the first nt_b
@@ -331,6 +334,8 @@ static void run_plain(Server &S, Lane &L, Request *q) {
     const long rows = (long)c.ncomp * c.nrec;
     check(L.ctx, inv_sources_set(L.ctx, r.nshots, r.src_ix, r.src_iz, r.stf_x, r.stf_y, r.stf_z));
     set_slot_models(S, L, {q}, 1);
+    if (!S.d.tile_x)
+        check(L.ctx, inv_autotune(L.ctx, 0, 1, std::min(r.nt, 10), nullptr, nullptr, nullptr));
     if (adj) check(L.ctx, inv_kernels_zero(L.ctx));
     double total = 0;
     for (int s = 0; s < r.nshots; s++) {
@@ -567,7 +572,6 @@ int inv_server_create(const inv_server_desc *desc, inv_server **out) {
     std::unique_ptr<inv_server> S(new (std::nothrow) inv_server());
     if (!S) return INV_ERR_NOMEM;
     S->d = *desc;
-    if (S->d.tile_x == 0) S->d.tile_x = 8;
     if (hipSetDevice(desc->device) != hipSuccess ||
         hipStreamCreateWithFlags(&S->ctl, hipStreamNonBlocking) != hipSuccess) {
         (void)hipGetLastError();

@@ -25,17 +25,28 @@ struct Geo {
 // place of two ~11-instruction divides, the same FP32 accuracy against FP64
 // but not bitwise equal to the exact mode.  f may point to global memory or
 // to LDS; (c8, c24) and (a1, a2) are the x or z constants of Geo.
+// On values: dm(f[k], f[k-s], f[k+s], f[k-2s]) and dp(f[k+s], f[k], f[k+2s], f[k-s]).
+template <bool FAST>
+__device__ __forceinline__ float dm(float f0, float fm1, float fp1, float fm2, float c8, float c24,
+                                    float a1, float a2) {
+    if (FAST) return a1 * (f0 - fm1) - a2 * (fp1 - fm2);
+    return 9.0f * (f0 - fm1) / c8 - (fp1 - fm2) / c24;
+}
+template <bool FAST>
+__device__ __forceinline__ float dp(float fp1, float f0, float fp2, float fm1, float c8, float c24,
+                                    float a1, float a2) {
+    if (FAST) return a1 * (fp1 - f0) - a2 * (fp2 - fm1);
+    return 9.0f * (fp1 - f0) / c8 - (fp2 - fm1) / c24;
+}
 template <bool FAST>
 __device__ __forceinline__ float d_minus(const float *__restrict__ f, long k, long s, float c8,
                                          float c24, float a1, float a2) {
-    if (FAST) return a1 * (f[k] - f[k - s]) - a2 * (f[k + s] - f[k - 2 * s]);
-    return 9.0f * (f[k] - f[k - s]) / c8 - (f[k + s] - f[k - 2 * s]) / c24;
+    return dm<FAST>(f[k], f[k - s], f[k + s], f[k - 2 * s], c8, c24, a1, a2);
 }
 template <bool FAST>
 __device__ __forceinline__ float d_plus(const float *__restrict__ f, long k, long s, float c8,
                                         float c24, float a1, float a2) {
-    if (FAST) return a1 * (f[k + s] - f[k]) - a2 * (f[k + 2 * s] - f[k - s]);
-    return 9.0f * (f[k + s] - f[k]) / c8 - (f[k + 2 * s] - f[k - s]) / c24;
+    return dp<FAST>(f[k + s], f[k], f[k + 2 * s], f[k - s], c8, c24, a1, a2);
 }
 
 // ---------------------------------------------------------------------------
@@ -69,10 +80,20 @@ __device__ __forceinline__ float snap_enc_scale(unsigned mbits) {
     return m > 0.0f ? SNAP_RANGE / m : 0.0f;
 }
 
-// store one snapshot value: FP32 as is, FP16/BF16 scaled by s
+// store one snapshot value: FP32 as is, FP16/BF16 scaled by s.  The scaled
+// value is rounded to FP32 first and then to 16 bits: the empty asm keeps the
+// compiler from folding the multiply and the conversion into one mixed-
+// precision instruction (a single rounding), which it does in some kernels
+// and not in others, so that every kernel stores the same bits.
+__device__ __forceinline__ unsigned short snap_enc_mul(float x, float s, int fmt) {
+    float y = x * s;
+    __asm__ volatile("" : "+v"(y));
+    return snap_enc16(y, fmt);
+}
+
 __device__ __forceinline__ void snap_put(void *base, int fmt, long idx, float x, float s) {
     if (fmt == INV_SNAP_F32) static_cast<float *>(base)[idx] = x;
-    else static_cast<unsigned short *>(base)[idx] = snap_enc16(x * s, fmt);
+    else static_cast<unsigned short *>(base)[idx] = snap_enc_mul(x, s, fmt);
 }
 
 // read access to one 16-bit snapshot field, decoded to FP32

@@ -1,0 +1,61 @@
+// Arguments of the fused time-step kernels (fdm_fused.hip, fdm_wide.hip).
+#pragma once
+
+#include "internal.h"
+
+namespace inv {
+
+struct StepArgs {
+    Geo g;
+    int tiles_x, tiles_z;
+    const float *rho, *lam, *mu, *abx, *abzb, *abzt;
+    // Per-slot models ([batch][3] pointers {rho or dt/rho, lam, mu}), or null
+    // when every slot shares rho/lam/mu above.  Batches of shots from
+    // different models (the request server's lanes) set it.
+    const float *const *slot_model;
+    const float *vx0, *vz0, *sxx0, *szz0, *sxz0, *vy0, *sxy0, *szy0;  // step n
+    float *vx1, *vz1, *sxx1, *szz1, *sxz1, *vy1, *sxy1, *szy1;        // step n+1
+    float *ux, *uz, *uy;                                              // in place
+    // point injection (addSTF, fdm.h:163-180), bucketed by velocity region
+    const float *inj_x, *inj_y, *inj_z;
+    long inj_slot_stride;
+    const int *inj_ptr, *inj_cell, *inj_row;
+    int inj_per_slot;
+    int nt, it;
+    // receiver recording (saveRec, fdm.h:181-196), bucketed by tile
+    float *out_x, *out_y, *out_z;
+    long out_slot_stride;
+    const int *rec_ptr, *rec_cell, *rec_row;
+    int rec_disp;
+    // Snapshots (fdm.h:521-560: a slot holds u before and v after a step).
+    // snap_u: slot base (batch slot 0) that receives this step's input u, or
+    // null; snap_v: slot base that receives v, this step's result when
+    // v_after is set, else this step's input (the previous step's result),
+    // or null.  16-bit formats scale each field by the max-abs read from
+    // mx_u / mx_v ([batch][nsnapf] tables of the slot); acc_u / acc_v collect
+    // max |u| and max |v| after this step into the table of a slot still to
+    // be written.  The linear kernel supports FP32 with v_after only.
+    void *snap_u, *snap_v;
+    int v_after, snap_fmt, nsnapf, psv_f0;
+    const unsigned *mx_u, *mx_v;
+    unsigned *acc_u, *acc_v;
+    long snap_slot_stride;  // elements per batch slot
+};
+
+__device__ __forceinline__ float wave_max(float x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
+    return x;
+}
+
+// Launch the wide column kernel (fdm_wide.hip): tile_x rows, c.col_v = 2 or
+// 4 columns per lane; `blocks` tiles (shot slots x tile rows x tile columns).
+template <bool SH, bool PSV, bool FAST>
+void wide_launch(Ctx &c, const StepArgs &a, unsigned blocks);
+
+// Columns a column-kernel tile owns with V columns per lane: 64 V minus a
+// halo of 3 columns each side (V = 1) or 4 (V = 2, 4: keeps vectors aligned).
+constexpr int col_halo(int V) { return V == 1 ? 3 : 4; }
+constexpr int col_strip(int V) { return 64 * V - 2 * col_halo(V); }
+
+}  // namespace inv

@@ -99,6 +99,8 @@ SIGNATURES = {
     "inv_prof_reset": [_vp],
     "inv_prof_read": [_vp, _i, _vp, _vp],
     "inv_set_variant": [_vp, _i, _i],
+    "inv_set_tile": [_vp, _i, _i],
+    "inv_autotune": [_vp, _i, _i, _i, _vp, _vp, _vp],
     "inv_forward_begin": [_vp, _i, _i, _i],
     "inv_adjoint_begin": [_vp, _i, _i],
     "inv_run_advance": [_vp, _i, _vp],
@@ -416,6 +418,18 @@ class Context:
     def set_variant(self, variant, tile_x=0):
         self._chk(self.L.inv_set_variant(self.h, variant, tile_x))
 
+    def autotune(self, first=0, count=1, steps=20):
+        """Time the candidate step kernels on this shape and select the
+        fastest: returns (variant, tile_rows, cols_per_lane)."""
+        v, t, w = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self.L.inv_autotune(self.h, first, count, steps, C.byref(v), C.byref(t),
+                                      C.byref(w)))
+        return v.value, t.value, w.value
+
+    def set_tile(self, tile_x=8, cols_per_lane=1):
+        """Fused column kernel with tile_x rows and cols_per_lane columns per lane."""
+        self._chk(self.L.inv_set_tile(self.h, tile_x, cols_per_lane))
+
     # ---- measurement ----
     def prof_enable(self, every):
         self._chk(self.L.inv_prof_enable(self.h, every))
@@ -484,7 +498,7 @@ class Server:
     here and returned by wait()."""
 
     def __init__(self, device=0, hbm_budget=0, max_lanes=4, max_slots=16, max_bypass=4,
-                 snap_format=INV_SNAP_F32, math=INV_MATH_FAST, tile_x=8, plain=False):
+                 snap_format=INV_SNAP_F32, math=INV_MATH_FAST, tile_x=0, plain=False):
         self.L = load()
         d = ServerDesc(device, hbm_budget, max_lanes, max_slots, max_bypass, snap_format, math,
                        tile_x, 1 if plain else 0)

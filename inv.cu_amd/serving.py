@@ -98,3 +98,18 @@ def run_load(reqs, rate=None, seed=3, plain=False, log=None, **server_kw):
             f"p99 {out['p99_latency_s']:.3f} s, {out['requests_per_s']:.2f} req/s, "
             f"{out['gcell_per_s']:.1f} Gcell/s, {out['batches']} batches")
     return out, infos
+
+
+def sweep(n=32, rates_x=(0.25, 0.5, 1.0, 2.0, 4.0), seed=3, log=None, **server_kw):
+    """C3 sweep: the plain loop's capacity from a back-to-back run of the n
+    requests, then Poisson arrivals at rates_x times that capacity through the
+    plain loop and the batched server.  Returns a list of summaries."""
+    reqs = make_requests(n, seed=seed)
+    base, _ = run_load(reqs, rate=None, seed=seed, plain=True, log=log, **server_kw)
+    cap = base["requests_per_s"]
+    out = [dict(base, rate_x=None, capacity_req_per_s=cap)]
+    for x in rates_x:
+        for plain in (True, False):
+            s, _ = run_load(reqs, rate=x * cap, seed=seed, plain=plain, log=log, **server_kw)
+            out.append(dict(s, rate_x=x, capacity_req_per_s=cap))
+    return out

@@ -13,19 +13,31 @@ from conftest import inv, oracle_problem, oracle_traces, rel
 pytestmark = pytest.mark.gpu
 
 CASES = [dict(sh=1, psv=0), dict(sh=0, psv=1), dict(sh=1, psv=1)]
-# (variant, tile_x, math): fused column kernel (0), velocity/stress kernel
-# pair (1), fused linear-index kernel (2), in exact (0) and fast (1) math
+# (variant, tile, math): fused column kernel (0) with tile_x rows, or with a
+# (rows, columns per lane) tile of the wide kernel; velocity/stress kernel
+# pair (1); fused linear-index kernel (2); in exact (0) and fast (1) math
 VARIANTS = [(0, 8, 0), (0, 16, 0), (0, 32, 0), (1, 0, 0), (2, 8, 0),
-            (0, 8, 1), (0, 16, 1), (1, 0, 1), (2, 16, 1)]
+            (0, (4, 2), 0), (0, (16, 2), 0), (0, (4, 4), 0), (0, (8, 4), 0),
+            (0, 8, 1), (0, 16, 1), (1, 0, 1), (2, 16, 1), (0, (8, 2), 1), (0, (4, 4), 1),
+            (0, (8, 4), 1)]
 
 
 def _vid(v):
-    kind = {0: f"col{v[1]}", 1: "split", 2: f"lin{v[1]}"}[v[0]]
+    if isinstance(v[1], tuple):
+        kind = f"col{v[1][0]}x{v[1][1]}"
+    else:
+        kind = {0: f"col{v[1]}", 1: "split", 2: f"lin{v[1]}"}[v[0]]
     return kind + ("-fast" if v[2] else "-exact")
 
 
 def setup(ctx, v):
-    ctx.set_variant(v[0], v[1])
+    if isinstance(v[1], tuple):
+        if ctx.desc.nz % v[1][1]:
+            ctx.close()
+            pytest.skip(f"nz={ctx.desc.nz} is not divisible by {v[1][1]} columns per lane")
+        ctx.set_tile(*v[1])
+    else:
+        ctx.set_variant(v[0], v[1])
     ctx.set_math(v[2])
 
 
@@ -172,3 +184,28 @@ def test_adjoint_plain_matches_oracle(gpu, case, variant):
             continue
         same(a, b, fast, tol=1e-4)
     ctx.close()
+
+
+def test_autotune_selects_a_candidate_and_keeps_results(gpu):
+    """inv_autotune picks one of its candidates, the results after it equal
+    the column kernel's bit for bit, and a second context of the same shape
+    gets the cached choice."""
+    P = inv.problems.tiny(sh=1, psv=1, nsrc=3, nx=70, nz=120, nt=120)
+    ref = P.context(batch=3)
+    ref.forward(0, 3, snapshots=False)
+    want = [ref.traces_get(b) for b in range(3)]
+    ref.close()
+    ctx = P.context(batch=3)
+    choice = ctx.autotune(0, 3, steps=10)
+    print(f"autotune choice {choice}", flush=True)
+    assert choice[0] == 0
+    assert (choice[1], choice[2]) in [(8, 1), (16, 1), (32, 1), (4, 2), (8, 2), (16, 2), (4, 4),
+                                      (8, 4)]
+    assert P.nz % choice[2] == 0
+    ctx.forward(0, 3, snapshots=False)
+    for b in range(3):
+        assert np.array_equal(ctx.traces_get(b), want[b])
+    ctx.close()
+    again = P.context(batch=3)
+    assert again.autotune(0, 3, steps=10) == choice
+    again.close()

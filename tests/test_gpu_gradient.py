@@ -19,7 +19,7 @@ def _obs_from_true(P):
 
 
 @pytest.mark.parametrize("math", [0, 1], ids=["exact", "fast"])
-@pytest.mark.parametrize("variant", [0, 1], ids=["fused", "split"])
+@pytest.mark.parametrize("variant", [0, 1, (8, 2)], ids=["fused", "split", "wide"])
 @pytest.mark.parametrize("batch", [1, 3])
 @pytest.mark.parametrize("sigma", [0, 3])
 def test_gradient_matches_oracle(gpu, batch, sigma, variant, math):
@@ -27,7 +27,10 @@ def test_gradient_matches_oracle(gpu, batch, sigma, variant, math):
     obs = _obs_from_true(P)
     print(f"gradient batch={batch} sigma={sigma} variant={variant} math={math}", flush=True)
     ctx = P.context(batch=batch)
-    ctx.set_variant(variant)
+    if isinstance(variant, tuple):
+        ctx.set_tile(*variant)
+    else:
+        ctx.set_variant(variant)
     ctx.set_math(math)
     for s in range(P.nsrc):
         ctx.obs_set(s, obs[s])
@@ -217,7 +220,7 @@ def test_model_swap_after_async_forward(gpu):
     ctx.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1], ids=["fused", "split"])
+@pytest.mark.parametrize("variant", [0, 1, (4, 4)], ids=["fused", "split", "wide"])
 def test_graph_replay_equals_direct(gpu, variant):
     """Captured-and-replayed runs give the same bits as direct launches, over
     several gradients and alternating shot batches (graph per batch)."""
@@ -226,7 +229,10 @@ def test_graph_replay_equals_direct(gpu, variant):
     res = []
     for graphs in (False, True):
         ctx = P.context(batch=2)
-        ctx.set_variant(variant)
+        if isinstance(variant, tuple):
+            ctx.set_tile(*variant)
+        else:
+            ctx.set_variant(variant)
         ctx.set_graphs(graphs)
         for s in range(P.nsrc):
             ctx.obs_set(s, obs[s])

@@ -132,3 +132,33 @@ def test_unsupported_combinations_rejected(gpu):
     ctx.set_variant(H.INV_VARIANT_FUSED)
     ctx.forward(0, 1, snapshots=True)
     ctx.close()
+
+
+@pytest.mark.parametrize("fmt", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("tile", [(4, 2), (16, 2), (4, 4), (8, 4)])
+def test_wide_kernel_equals_column_kernel(gpu, fmt, tile):
+    """The wide column kernel (2 or 4 columns per lane) gives bit for bit the
+    one-column kernel's traces and gradient, with 16-bit snapshots (max-abs
+    tracking) and checkpoints."""
+    P = inv.problems.tiny(sh=1, psv=1, nsrc=3, nt=300, sfe=10)
+    print(f"wide tile {tile} {fmt}", flush=True)
+    obs = _obs(P)
+    out = []
+    for variant in ("col", "wide"):
+        ctx = P.context(batch=3)
+        if variant == "wide":
+            ctx.set_tile(*tile)
+        ctx.set_snapshots(FMTS[fmt], 100)
+        for s in range(P.nsrc):
+            ctx.obs_set(s, obs[s])
+        f = ctx.gradient(misfit_type=0, sigma=0)
+        ctx.forward(0, 3, snapshots=False)
+        tr = [ctx.traces_get(b) for b in range(3)]
+        out.append((f, ctx.kernels_get(), tr))
+        ctx.close()
+    (fa, ka, ta), (fb, kb, tb) = out
+    assert fa == fb
+    for x, y in zip(ka, kb):
+        assert np.array_equal(x, y)
+    for x, y in zip(ta, tb):
+        assert np.array_equal(x, y)
