@@ -1,17 +1,31 @@
 #!/usr/bin/env python3
 """Benchmark: forward+adjoint gradient throughput on one MI355X.
 
-A step is one full gradient evaluation of the batched P-SV Marmousi-shaped
-workload (737x240 cells at 12.5 m, 16 shots, 368 receivers, dt 1 ms,
-nt 4000): forward of every shot with snapshots, waveform misfit and adjoint
-sources, adjoint with lambda/mu/rho kernel accumulation, one-sided Gaussian
-smoothing of the three kernels.  All 16 shots run as one batch per launch.
-Inputs (model, observed data) are resident in HBM before the timed region.
+A step is one full gradient evaluation (Misfit::run(kernel=true),
+misfit/misfit.h:15-51): forward of every shot with snapshots, misfit and
+adjoint sources, adjoint with kernel accumulation, one-sided Gaussian
+smoothing of the kernels.  Inputs (model, observed data) are resident in HBM
+before the timed region.  Workloads (--workload):
+
+  c1 (default)  batched P-SV, Marmousi-shaped 737x240 at 12.5 m, 16 shots,
+                368 receivers, dt 1 ms, nt 4000, waveform misfit, FP32
+                snapshots; all 16 shots per launch.
+  c2            SH, the same generator at 1474x480 and 6.25 m, 64 shots,
+                737 receivers, dt 0.5 ms, nt 8000, FP16 snapshots every 10
+                steps with 800-step checkpoints (recomputed in the adjoint);
+                plus the error of FP32 / FP16 / BF16 snapshot storage against
+                the FP64 oracle on a 2-shot, 2000-step sample.
+  c4            P-SV, the generator at 4096x2048 and 5 m, dt 0.4 ms,
+                nt 10000, 2048 receivers, envelope misfit, FP16 snapshots with
+                1000-step checkpoints; timed on 16 of the 256 shots (one
+                launch; a full gradient is 16 such batches), plus the FP64
+                check on the 1024x512 / 20 m downsampled model.
 
 Prints one JSON line (rank 0).  Extra keys: roofline (stencil step vs HBM
 peak, timed live with HIP events inside the library), cpu_baseline (the CPU
-oracle on a bounded sample of the same workload), plain (the one-shot-per-
-launch GPU path on the same workload).
+oracle on a bounded sample of the same workload), plain (c1: the one-shot-
+per-launch GPU path on the same workload), serving (c1: p50/p99 latency of
+the request server against the plain loop), accuracy (c2, c4).
 """
 from __future__ import annotations
 
@@ -33,6 +47,8 @@ HBM_PEAK_GBPS = 8000.0
 # the 7 wavefields (vx, vz, ux, uz, sxx, szz, sxz) read and written once and
 # the 3 material fields read once, FP32.
 PSV_STEP_BYTES = 4 * (2 * 7 + 3)
+# SH: the 4 wavefields (vy, uy, sxy, szy) read and written, mu and rho read
+SH_STEP_BYTES = 4 * (2 * 4 + 2)
 # HBM bytes per cell of one launch of the default step kernel (fused column
 # kernel, tile 58x8, fast math, P-SV), measured with rocprofv3 FETCH_SIZE and
 # WRITE_SIZE and scaled by a 256 MiB copy (fetch x2.0, write x1.0); see
@@ -48,13 +64,16 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=16, help="shots per launch")
-    ap.add_argument("--nt", type=int, default=4000)
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c4"])
+    ap.add_argument("--nt", type=int, default=0, help="time steps (0: the workload's)")
     ap.add_argument("--sigma", type=int, default=5, help="Gaussian filter sigma (cells)")
     ap.add_argument("--prof-every", type=int, default=16,
                     help="time every n-th step's stencil launches with HIP events (0 = off)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-shot GPU path")
-    ap.add_argument("--cpu-shots", type=int, default=1)
+    ap.add_argument("--no-accuracy", action="store_true", help="c2/c4: skip the FP64 check")
+    ap.add_argument("--cpu-shots", type=int, default=8,
+                    help="c1: shots of the CPU oracle's timed sample")
     ap.add_argument("--variant", type=int, default=0, help="0 fused step kernel, 1 kernel pair")
     ap.add_argument("--tile-x", type=int, default=0,
                     help="fused tile rows (0: inv_autotune picks rows and columns per lane)")
@@ -108,29 +127,123 @@ def configure_kernel(ctx, a, count):
     return {"variant": v, "rows": tx, "cols_per_lane": cols}
 
 
-def cpu_baseline(P, shots):
-    """The CPU oracle (FP32, OpenMP) on `shots` shots of the same workload."""
+def cpu_baseline(P, shots, nt=None):
+    """The CPU oracle (FP32, OpenMP) on `shots` shots of the same workload
+    (the first nt steps when nt is given)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle  # the checker, used here only as the timed CPU baseline
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     pyoracle.set_threads(threads)
-    Ps = P.with_(src_ix=P.src_ix[:shots], src_iz=P.src_iz[:shots], stf_x=P.stf_x[:shots],
-                 stf_y=P.stf_y[:shots], stf_z=P.stf_z[:shots])
+    nt = nt or P.nt
+    Ps = P.with_(src_ix=P.src_ix[:shots], src_iz=P.src_iz[:shots],
+                 stf_x=np.ascontiguousarray(P.stf_x[:shots, :nt]),
+                 stf_y=np.ascontiguousarray(P.stf_y[:shots, :nt]),
+                 stf_z=np.ascontiguousarray(P.stf_z[:shots, :nt]), nt=nt)
     ab = pyoracle.absbound(P.nx, P.nz, P.abs_width, P.abs_alpha, P.abs_left, P.abs_right,
                            P.abs_bottom, P.abs_top)
-    O = pyoracle.Problem(P.nx, P.nz, P.dx, P.dz, P.dt, P.nt, P.lam0, P.mu0, P.rho0, ab,
+    O = pyoracle.Problem(P.nx, P.nz, P.dx, P.dz, P.dt, nt, P.lam0, P.mu0, P.rho0, ab,
                          Ps.src_ix, Ps.src_iz, Ps.stf_x, Ps.stf_y, Ps.stf_z, P.rec_ix, P.rec_iz,
                          sh=P.sh, psv=P.psv, sfe=P.sfe, adj=1)
-    obs = np.zeros((shots, P.ncomp, P.nrec, P.nt), np.float32)
+    obs = np.zeros((shots, P.ncomp, P.nrec, nt), np.float32)
     t0 = time.perf_counter()
-    O.gradient(obs, 0, P.sigma, dtype=np.float32)
+    O.gradient(obs, P.misfit, P.sigma, dtype=np.float32)
     dt = time.perf_counter() - t0
     work = Ps.cell_updates(shots)
     return {"value": work / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"oracle FP32 gradient (forward+misfit+adjoint+filter) of {shots} of the "
-                      f"16 shots, full nt={P.nt}, {P.nx}x{P.nz}; {dt:.1f} s"}
+                      f"{P.nsrc} shots, nt={nt} of {P.nt}, {P.nx}x{P.nz}; {dt:.1f} s"}
+
+
+def workload(inv, a):
+    """(problem, shots per launch, snapshot format, checkpoint interval,
+    bytes per cell-update of the step kernel, description) of a workload."""
+    pr, H = inv.problems, inv.invhip
+    if a.workload == "c1":
+        P = pr.marmousi(nt=a.nt or 4000, sigma=a.sigma)
+        return (P, 16, H.INV_SNAP_F32, 0, PSV_STEP_BYTES,
+                f"batched P-SV gradient, Marmousi-shaped 737x240 @ 12.5 m, 16 shots, "
+                f"368 receivers, dt 1 ms, nt {P.nt}")
+    if a.workload == "c2":
+        # SH waves do not enter the water layer (mu = 0 there): shots and
+        # receivers sit on the seafloor, 2 rows below the 12 water rows
+        P = pr.marmousi(nx=1474, nz=480, dx=6.25, dt=5e-4, nt=a.nt or 8000, nsrc=64, nrec=737,
+                        f0=8.0, t0=0.15, seed=2, sh=1, psv=0, sigma=a.sigma, src_depth=14,
+                        rec_depth=14, name="marmousi-sh")
+        return (P, 64, H.INV_SNAP_F16, 800, SH_STEP_BYTES,
+                f"batched SH gradient, Marmousi-shaped 1474x480 @ 6.25 m, 64 shots, "
+                f"737 seafloor receivers, dt 0.5 ms, nt {P.nt}, FP16 snapshots every 10 "
+                f"steps, 800-step checkpoints")
+    P = pr.marmousi(nx=4096, nz=2048, dx=5.0, dt=4e-4, nt=a.nt or 10000, nsrc=256, nrec=2048,
+                    f0=12.0, t0=0.1, seed=4, sigma=a.sigma, name="marmousi-large")
+    P = P.with_(misfit=1)
+    pick = np.arange(0, P.nsrc, 16)  # 16 of the 256 shots, spread over the line
+    P = P.with_(src_ix=P.src_ix[pick], src_iz=P.src_iz[pick], stf_x=P.stf_x[pick],
+                stf_y=P.stf_y[pick], stf_z=P.stf_z[pick])
+    return (P, 16, H.INV_SNAP_F16, 1000, PSV_STEP_BYTES,
+            f"batched P-SV gradient, 4096x2048 @ 5 m, 16 of 256 shots in one launch, 2048 "
+            f"receivers, dt 0.4 ms, nt {P.nt}, envelope misfit, FP16 snapshots every 10 steps, "
+            f"1000-step checkpoints")
+
+
+def accuracy(inv, a):
+    """Gradient and misfit error of FP32, FP16-snapshot and BF16-snapshot
+    storage (all arithmetic FP32) against the FP64 oracle, on a sample the
+    CPU runs in seconds: c2 on its own model (2 shots, nt 2000), c4 on the
+    1024x512 / 20 m downsampled model (2 shots, dt 2 ms, nt 2000)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle  # the checker: the FP64 reference of this comparison
+
+    pr, H = inv.problems, inv.invhip
+    if a.workload == "c2":
+        P = pr.marmousi(nx=1474, nz=480, dx=6.25, dt=5e-4, nt=2000, nsrc=2, nrec=737, f0=8.0,
+                        t0=0.15, seed=2, sh=1, psv=0, sigma=0, src_depth=14, rec_depth=14,
+                        name="marmousi-sh")
+        ckpt, sample = 400, "1474x480 SH, 2 shots, nt 2000 of 8000"
+    else:
+        P = pr.marmousi(nx=1024, nz=512, dx=20.0, dt=2e-3, nt=2000, nsrc=2, nrec=512, f0=6.0,
+                        t0=0.2, seed=4, sigma=0, name="marmousi-large-ds").with_(misfit=1)
+        ckpt, sample = 400, "1024x512 @ 20 m P-SV (c4 downsampled), 2 shots, dt 2 ms, nt 2000"
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    pyoracle.set_threads(threads)
+    ctx = P.context(batch=2, model="true")
+    ctx.forward(0, 2, snapshots=False)
+    obs = np.stack([ctx.traces_get(b) for b in range(2)])
+    ctx.close()
+    ab = pyoracle.absbound(P.nx, P.nz, P.abs_width, P.abs_alpha, P.abs_left, P.abs_right,
+                           P.abs_bottom, P.abs_top)
+    O = pyoracle.Problem(P.nx, P.nz, P.dx, P.dz, P.dt, P.nt, P.lam0, P.mu0, P.rho0, ab,
+                         P.src_ix, P.src_iz, P.stf_x, P.stf_y, P.stf_z, P.rec_ix, P.rec_iz,
+                         sh=P.sh, psv=P.psv, sfe=P.sfe, adj=1)
+    t0 = time.perf_counter()
+    f64, *k64 = O.gradient(obs.astype(np.float64), P.misfit, 0, dtype=np.float64)
+    tcpu = time.perf_counter() - t0
+    names = ("k_lambda", "k_mu", "k_rho")
+    out = {"sample": sample, "reference": f"oracle FP64 ({tcpu:.1f} s on {threads} threads)",
+           "misfit_fp64": f64}
+    for label, fmt, ck in (("fp32", H.INV_SNAP_F32, 0), ("fp16", H.INV_SNAP_F16, ckpt),
+                           ("bf16", H.INV_SNAP_BF16, ckpt)):
+        ctx = P.context(batch=2, model="init")
+        ctx.set_snapshots(fmt, ck)
+        for s_ in range(2):
+            ctx.obs_set(s_, obs[s_])
+        f = ctx.gradient(misfit_type=P.misfit, sigma=0)
+        ks = ctx.kernels_get()
+        ctx.close()
+        row = {"misfit_rel_err": abs(f - f64) / abs(f64), "checkpoints": ck}
+        # whole grid, and the solid cells only (mu > 0): in the water layer the
+        # mu kernel is singular next to the sources and FP32 itself (GPU and
+        # CPU alike) departs from FP64 there
+        solid = P.mu0 > 0
+        for n, x, y in zip(names, ks, k64):
+            if np.any(y):
+                row[n + "_rel_l2"] = float(np.linalg.norm(x - y) / np.linalg.norm(y))
+                row[n + "_rel_l2_solid"] = float(np.linalg.norm((x - y)[solid]) /
+                                                 np.linalg.norm(y[solid]))
+        out[label] = row
+        print(f"# accuracy {label}: {row}", file=sys.stderr, flush=True)
+    return out
 
 
 def main():
@@ -148,14 +261,17 @@ def main():
                                    snap_format=inv.invhip.INV_SNAP_F16):
             print(json.dumps(s), flush=True)
         return
-    P = inv.problems.marmousi(nt=a.nt, sigma=a.sigma)
-    B = min(a.batch, P.nsrc)
-    print(f"# workload {P.name} {P.nx}x{P.nz} shots={P.nsrc} nt={P.nt} batch={B}",
+    P, B, snap_fmt, ckpt, step_bytes, desc = workload(inv, a)
+    if a.workload == "c1":
+        B = min(a.batch, P.nsrc)
+    print(f"# workload {a.workload} {P.name} {P.nx}x{P.nz} shots={P.nsrc} nt={P.nt} batch={B}",
           file=sys.stderr, flush=True)
     ctx = P.context(batch=B, model="true")
     ctx.set_math(a.math)
     ctx.set_graphs(a.graphs)
+    ctx.set_snapshots(snap_fmt, ckpt)  # before any run sizes the snapshot storage
     tile = configure_kernel(ctx, a, B)
+    print(f"# step kernel tile {tile}", file=sys.stderr, flush=True)
     # observed data: forward on the true model (Misfit::init, misfit.h:135-148)
     for first in range(0, P.nsrc, B):
         n = min(B, P.nsrc - first)
@@ -175,6 +291,7 @@ def main():
         ctx.free(dst)
     for _ in range(a.warmup):
         ctx.gradient(P.misfit, P.sigma)
+        print("# warm-up gradient done", file=sys.stderr, flush=True)
     ctx.sync()
     ctx.prof_reset()
     ctx.prof_enable(a.prof_every)
@@ -219,19 +336,19 @@ def main():
         avg = {"velocity": round(tv / nv * 1e3, 2), "stress": round(ts / ns * 1e3, 2)}
         samples = {"velocity": nv, "stress": ns}
     if step_s:
-        achieved = PSV_STEP_BYTES * cells / step_s / 1e9
-        profiled = ((tile["variant"], tile["rows"], tile["cols_per_lane"], a.math, B) ==
-                    PROFILED_TILE and nfr)
+        achieved = step_bytes * cells / step_s / 1e9
+        profiled = (a.workload == "c1" and (tile["variant"], tile["rows"], tile["cols_per_lane"],
+                                             a.math, B) == PROFILED_TILE and nfr)
         traffic = round(MEASURED_TRAFFIC_PER_CELL * cells) if profiled else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                "kernel": kernel, "alg_bytes_per_launch": PSV_STEP_BYTES * cells,
+                "kernel": kernel, "alg_bytes_per_launch": step_bytes * cells,
                 "avg_us": avg, "samples": samples}
     ni, ti = ctx.prof_read(inv.invhip.INV_PROF_INTERACT)
     ctx.close()
 
     plain = None
-    if not a.no_plain:
+    if not a.no_plain and a.workload == "c1":
         pctx = P.context(batch=1, model="true")
         configure_kernel(pctx, a, 1)
         pctx.set_math(a.math)
@@ -252,9 +369,14 @@ def main():
                  "note": "one shot per launch, same kernels, no batching, no graph"}
         print(f"# plain path: {pms:.1f} ms per gradient", file=sys.stderr, flush=True)
 
-    cpu = None if a.no_cpu else cpu_baseline(P, a.cpu_shots)
+    cpu = None
+    if not a.no_cpu:
+        # c1 in full (a sample of its shots); c2 / c4: 2 shots x 200 steps
+        cpu = (cpu_baseline(P, a.cpu_shots) if a.workload == "c1" else
+               cpu_baseline(P, 2, nt=200))
+    acc = accuracy(inv, a) if a.workload in ("c2", "c4") and not a.no_accuracy else None
     serving = None
-    if a.serve_requests > 0:
+    if a.serve_requests > 0 and a.workload == "c1":
         serving = serving_leg(inv, a.serve_requests, a.serve_rate_x, inv.invhip.INV_SNAP_F16)
 
     out = {
@@ -270,16 +392,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded Marmousi-shaped model; observed data modelled on the true model)",
-        "config": {"workload": "batched P-SV gradient, Marmousi-shaped 737x240 @ 12.5 m, "
-                               "16 shots, 368 receivers, dt 1 ms, nt 4000",
+        "config": {"workload": desc, "name": a.workload,
                    "nx": P.nx, "nz": P.nz, "shots": P.nsrc, "receivers": P.nrec, "nt": P.nt,
-                   "batch": B, "sfe": P.sfe, "filter_sigma": P.sigma, "misfit": "waveform",
-                   "snapshots": "fp32 in HBM", "step_kernel_tile": tile},
+                   "batch": B, "sfe": P.sfe, "filter_sigma": P.sigma,
+                   "misfit": ["waveform", "envelope"][P.misfit],
+                   "snapshots": ["fp32", "fp16", "bf16"][snap_fmt] + " in HBM",
+                   "checkpoint_interval": ckpt, "step_kernel_tile": tile},
         "gradients_per_hr": round(3.6e6 / ms, 1),
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain": plain,
         "serving": serving,
+        "accuracy": acc,
     }
     if ni:
         out["interaction_avg_us"] = round(ti / ni * 1e3, 2)

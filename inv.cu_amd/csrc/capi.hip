@@ -578,7 +578,7 @@ static void run_steps(Ctx &c, int kind, int first, int count, int flags, int it0
                 "the column kernel's columns per lane must divide nz");
     INV_REQUIRE(!c.slot_models || c.variant != INV_VARIANT_SPLIT, INV_ERR_UNSUPPORTED,
                 "per-slot models need a fused step kernel");
-    snap_storage(c);
+    if (kind == 2 || (flags & INV_FWD_SNAPSHOTS)) snap_storage(c);
     if (!c.use_graphs || it0 != 0 || it1 != end) {
         run_steps_direct(c, kind, first, count, flags, it0, it1);
         return;
@@ -598,7 +598,7 @@ static void run_steps(Ctx &c, int kind, int first, int count, int flags, int it0
         // the capture; inside it the run only launches kernels (no event
         // sampling inside a capture)
         if (c.variant == INV_VARIANT_SPLIT) fdm_prepare(c, kind, first, count);
-        else fused_prepare(c, kind, first, count);
+        else fused_prepare(c, kind, first, count, flags);
         sync(c);
         Ctx::GraphEntry e;
         const int every = c.prof.every;
@@ -1167,7 +1167,17 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                 // adjoint steps with their kernel accumulation (when those go
                 // to per-slot accumulators, which the next adjoint clears;
                 // a one-slot context adds into the kernels themselves)
-                const bool adj = c.g.adj && c.g.sfe > 0 && c.g.batch > 1;
+                bool adj = c.g.adj && c.g.sfe > 0 && c.g.batch > 1;
+                if (adj) {
+                    // snapshot storage for the current format; without the
+                    // memory for it, time forward steps only
+                    try {
+                        snap_storage(c);
+                    } catch (const Error &e) {
+                        if (e.code != INV_ERR_NOMEM) throw;
+                        adj = false;
+                    }
+                }
                 const int flags = adj ? INV_FWD_SNAPSHOTS : 0;
                 const bool verbose = getenv("INV_AUTOTUNE_VERBOSE") != nullptr;
                 auto pass = [&]() {
@@ -1177,8 +1187,7 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                 for (size_t k = 0; k < cand.size(); k++) {
                     const TuneChoice &t = cand[k];
                     apply_choice(c, t);
-                    fused_prepare(c, 1, first, count);
-                    if (adj) fused_prepare(c, 2, first, count);
+                    if (adj) fused_prepare(c, 2, first, count, 0);
                     // untimed passes (the first candidate's also bring the
                     // clocks up), then the best of three timed passes
                     for (int w = 0; w < (k == 0 ? 4 : 1); w++) pass();

@@ -433,6 +433,9 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
             }
             const long kc = (long)i * nz + j;
             const long o = so + kc;
+            // cells of this row the wavefront covers: point lists are scanned
+            // with one wave-uniform test per point before the per-lane one
+            const long rlo = (long)i * nz + (j0 - 1), rhi = rlo + 63;
             const bool xin = i >= 2 && i < nx - 2;
             const bool own = jown && vr >= 1 && vr <= TX;
             const int se = (vr + 2) * W + (lane + 2);
@@ -441,9 +444,12 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
             if (!PSV) {
                 float d = xin ? d_minus<FAST>(l_sxy, se, W, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
                 if (zin) d += d_minus<FAST>(l_szy, se, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
-                for (int pp = p0; pp < p1; pp++)
-                    if (a.inj_cell[pp] == kc)
+                for (int pp = p0; pp < p1; pp++) {
+                    const int cp = a.inj_cell[pp];
+                    if (cp < rlo || cp > rhi) continue;
+                    if (cp == kc)
                         d += a.inj_y[(long)b * a.inj_slot_stride + (long)a.inj_row[pp] * a.nt + it];
+                }
                 const float v = vel_update<FAST>(ab, qv1[p], g.dt, d, rho);
                 l_vy[e] = v;
                 if (own) {
@@ -457,10 +463,13 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                         mu1 = fmaxf(mu1, fabsf(u));
                         mv1 = fmaxf(mv1, fabsf(v));
                     }
-                    for (int pp = r0; pp < r1; pp++)
-                        if (a.rec_cell[pp] == kc)
+                    for (int pp = r0; pp < r1; pp++) {
+                        const int cp = a.rec_cell[pp];
+                        if (cp < rlo || cp > rhi) continue;
+                        if (cp == kc)
                             a.out_y[(long)b * a.out_slot_stride + (long)a.rec_row[pp] * a.nt + it] =
                                 a.rec_disp ? u : v;
+                    }
                 }
             } else {
                 float dx_, dz_;
@@ -475,13 +484,16 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                     dx_ += d_minus<FAST>(l_sxz, se, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
                     dz_ += d_minus<FAST>(l_szz, se, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
                 }
-                for (int pp = p0; pp < p1; pp++)
-                    if (a.inj_cell[pp] == kc) {
+                for (int pp = p0; pp < p1; pp++) {
+                    const int cp = a.inj_cell[pp];
+                    if (cp < rlo || cp > rhi) continue;
+                    if (cp == kc) {
                         const long ks =
                             (long)b * a.inj_slot_stride + (long)a.inj_row[pp] * a.nt + it;
                         dx_ += a.inj_x[ks];
                         dz_ += a.inj_z[ks];
                     }
+                }
                 const float vx = vel_update<FAST>(ab, qv1[p], g.dt, dx_, rho);
                 const float vz = vel_update<FAST>(ab, qv2[p], g.dt, dz_, rho);
                 l_vx[e] = vx;
@@ -510,13 +522,16 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                         mv1 = fmaxf(mv1, fabsf(vx));
                         mv2 = fmaxf(mv2, fabsf(vz));
                     }
-                    for (int pp = r0; pp < r1; pp++)
-                        if (a.rec_cell[pp] == kc) {
+                    for (int pp = r0; pp < r1; pp++) {
+                        const int cp = a.rec_cell[pp];
+                        if (cp < rlo || cp > rhi) continue;
+                        if (cp == kc) {
                             const long kr =
                                 (long)b * a.out_slot_stride + (long)a.rec_row[pp] * a.nt + it;
                             a.out_x[kr] = a.rec_disp ? ux : vx;
                             a.out_z[kr] = a.rec_disp ? uz : vz;
                         }
+                    }
                 }
             }
         }
@@ -915,10 +930,11 @@ static void snap_flush(Ctx &c, const SnapPlan &sp, const Fields &F, int par, int
     INV_HIP(hipGetLastError());
 }
 
-void fused_prepare(Ctx &c, int kind, int first, int count) {
+void fused_prepare(Ctx &c, int kind, int first, int count, int flags) {
     ensure_second_set(c);
     ensure_rec_lists(c);
-    snap_storage(c);
+    // snapshot storage only for runs that store or read snapshots
+    if (kind == 2 || (flags & INV_FWD_SNAPSHOTS)) snap_storage(c);
     // the adjoint's recomputation injects the forward's sources
     if (kind == 1 || (c.ckpt > 0 && c.g.adj)) source_list(c, first, count, list_key(c));
 }
@@ -1021,7 +1037,7 @@ static void forward_args(Ctx &c, StepArgs &a, int first, int count, bool record)
 }
 
 void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
-    fused_prepare(c, 1, first, count);
+    fused_prepare(c, 1, first, count, flags);
     StepArgs a;
     common_args(c, a);
     forward_args(c, a, first, count, true);
@@ -1089,7 +1105,7 @@ static void replay_segment(Ctx &c, int first, int count, int fseg, int aseg) {
 }
 
 void fused_adjoint(Ctx &c, int first, int count, int it0, int it1) {
-    fused_prepare(c, 2, first, count);
+    fused_prepare(c, 2, first, count, 0);
     StepArgs a;
     common_args(c, a);
     const long nt = c.g.nt;

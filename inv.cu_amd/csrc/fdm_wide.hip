@@ -199,6 +199,9 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         const bool rin = jin && i >= 0 && i < nx;
         const bool xin = i >= 2 && i < nx - 2;
         const float abx = a.abx[clampi(i, 0, nx - 1)];
+        // cells of this row the wavefront covers: point lists are scanned with
+        // one wave-uniform test per point before the per-lane one
+        const long rlo = (long)i * nz + (j0 - HL), rhi = rlo + W - 1;
         float n1[V], n2[V];
         if (!PSV) {
             float x0[V], x1[V], x2[V], x3[V], ez[3 * V];
@@ -216,9 +219,12 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
                 if (j >= 2 && j < nz - 2)
                     d += dm<FAST>(ez[V + v], ez[V + v - 1], ez[V + v + 1], ez[V + v - 2], g.c8dz,
                                   g.c24dz, g.a1z, g.a2z);
-                for (int pp = p0; pp < p1; pp++)
-                    if (a.inj_cell[pp] == kc)
+                for (int pp = p0; pp < p1; pp++) {
+                    const int cp = a.inj_cell[pp];
+                    if (cp < rlo || cp > rhi) continue;
+                    if (cp == kc)
                         d += I1[(long)b * a.inj_slot_stride + (long)a.inj_row[pp] * a.nt + it];
+                }
                 const float ab = (abx * abz[v]) * abt[v];
                 n1[v] = rin ? vel_update<FAST>(ab, qv1[p][v], g.dt, d, qr[p][v]) : 0.0f;
             }
@@ -252,13 +258,16 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
                     dz_ += dm<FAST>(e2[V + v], e2[V + v - 1], e2[V + v + 1], e2[V + v - 2],
                                     g.c8dz, g.c24dz, g.a1z, g.a2z);
                 }
-                for (int pp = p0; pp < p1; pp++)
-                    if (a.inj_cell[pp] == kc) {
+                for (int pp = p0; pp < p1; pp++) {
+                    const int cp = a.inj_cell[pp];
+                    if (cp < rlo || cp > rhi) continue;
+                    if (cp == kc) {
                         const long ks =
                             (long)b * a.inj_slot_stride + (long)a.inj_row[pp] * a.nt + it;
                         dx_ += I1[ks];
                         dz_ += I2[ks];
                     }
+                }
                 const float ab = (abx * abz[v]) * abt[v];
                 n1[v] = rin ? vel_update<FAST>(ab, qv1[p][v], g.dt, dx_, qr[p][v]) : 0.0f;
                 n2[v] = rin ? vel_update<FAST>(ab, qv2[p][v], g.dt, dz_, qr[p][v]) : 0.0f;
@@ -298,12 +307,15 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
                         mv2 = fmaxf(mv2, fabsf(n2[v]));
                     }
                 }
-                for (int pp = r0; pp < r1; pp++)
-                    if (a.rec_cell[pp] == kc) {
+                for (int pp = r0; pp < r1; pp++) {
+                    const int cp = a.rec_cell[pp];
+                    if (cp < rlo || cp > rhi) continue;
+                    if (cp == kc) {
                         const long kr = (long)b * a.out_slot_stride + (long)a.rec_row[pp] * a.nt + it;
                         O1[kr] = a.rec_disp ? u : n1[v];
                         if (PSV) O2[kr] = a.rec_disp ? uu : n2[v];
                     }
+                }
             }
             vstore<V>(V1o + o, n1);
             vstore<V>(U1 + o, nu1);

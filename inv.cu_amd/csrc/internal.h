@@ -213,7 +213,9 @@ void snap_storage(Ctx &c);
 void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1);
 void fused_adjoint(Ctx &c, int first, int count, int it0, int it1);
 void fused_invalidate_lists(Ctx &c);
-void fused_prepare(Ctx &c, int kind, int first, int count);
+// buffers, point lists and (for runs with snapshots) snapshot storage of a
+// forward (kind 1, with its flags) or adjoint (kind 2)
+void fused_prepare(Ctx &c, int kind, int first, int count, int flags);
 void fdm_prepare(Ctx &c, int kind, int first, int count);
 // the per-slot source list of shots first..first+count-1 for a tile shape
 // (key 0: the kernel-pair tiles); built on first use
