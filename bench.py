@@ -7,6 +7,11 @@ adjoint sources, adjoint with kernel accumulation, one-sided Gaussian
 smoothing of the kernels.  Inputs (model, observed data) are resident in HBM
 before the timed region.  Workloads (--workload):
 
+  c0            P-SV checkerboard 200x200 at 10 m (+-10 % squares of 20 cells
+                on vp, vs, rho), 4 explosive Ricker sources at 20 cells depth,
+                40 receivers, dt 1 ms, nt 2000, waveform misfit, sigma 5; all 4
+                shots per launch; the CPU oracle runs the whole gradient on the
+                same observed data and the kernels are compared with the GPU's.
   c1 (default)  batched P-SV, Marmousi-shaped 737x240 at 12.5 m, 16 shots,
                 368 receivers, dt 1 ms, nt 4000, waveform misfit, FP32
                 snapshots; all 16 shots per launch.
@@ -23,7 +28,8 @@ before the timed region.  Workloads (--workload):
 
 Prints one JSON line (rank 0).  Extra keys: roofline (stencil step vs HBM
 peak, timed live with HIP events inside the library), cpu_baseline (the CPU
-oracle on a bounded sample of the same workload), plain (c1: the one-shot-
+oracle on a bounded sample of the same workload; c0: all of it), parity (c0:
+GPU gradient against the CPU oracle's), plain (c1: the one-shot-
 per-launch GPU path on the same workload), serving (c1: p50/p99 latency of
 the request server against the plain loop), accuracy (c2, c4).
 """
@@ -64,7 +70,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=16, help="shots per launch")
-    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c4"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c4"])
     ap.add_argument("--nt", type=int, default=0, help="time steps (0: the workload's)")
     ap.add_argument("--sigma", type=int, default=5, help="Gaussian filter sigma (cells)")
     ap.add_argument("--prof-every", type=int, default=16,
@@ -127,9 +133,12 @@ def configure_kernel(ctx, a, count):
     return {"variant": v, "rows": tx, "cols_per_lane": cols}
 
 
-def cpu_baseline(P, shots, nt=None):
+def cpu_baseline(P, shots, nt=None, obs=None):
     """The CPU oracle (FP32, OpenMP) on `shots` shots of the same workload
-    (the first nt steps when nt is given)."""
+    (the first nt steps when nt is given), against observed data `obs`
+    ([shots][ncomp][nrec][nt], zeros when None: the work is the same).
+    Returns the baseline record and the oracle's (misfit, k_lambda, k_mu,
+    k_rho)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle  # the checker, used here only as the timed CPU baseline
 
@@ -139,27 +148,34 @@ def cpu_baseline(P, shots, nt=None):
     Ps = P.with_(src_ix=P.src_ix[:shots], src_iz=P.src_iz[:shots],
                  stf_x=np.ascontiguousarray(P.stf_x[:shots, :nt]),
                  stf_y=np.ascontiguousarray(P.stf_y[:shots, :nt]),
-                 stf_z=np.ascontiguousarray(P.stf_z[:shots, :nt]), nt=nt)
+                 stf_z=np.ascontiguousarray(P.stf_z[:shots, :nt]), nt=nt,
+                 src_type=None if P.src_type is None else P.src_type[:shots])
     ab = pyoracle.absbound(P.nx, P.nz, P.abs_width, P.abs_alpha, P.abs_left, P.abs_right,
                            P.abs_bottom, P.abs_top)
     O = pyoracle.Problem(P.nx, P.nz, P.dx, P.dz, P.dt, nt, P.lam0, P.mu0, P.rho0, ab,
                          Ps.src_ix, Ps.src_iz, Ps.stf_x, Ps.stf_y, Ps.stf_z, P.rec_ix, P.rec_iz,
-                         sh=P.sh, psv=P.psv, sfe=P.sfe, adj=1)
-    obs = np.zeros((shots, P.ncomp, P.nrec, nt), np.float32)
+                         sh=P.sh, psv=P.psv, sfe=P.sfe, adj=1, src_type=Ps.src_type)
+    if obs is None:
+        obs = np.zeros((shots, P.ncomp, P.nrec, nt), np.float32)
     t0 = time.perf_counter()
-    O.gradient(obs, P.misfit, P.sigma, dtype=np.float32)
+    res = O.gradient(obs, P.misfit, P.sigma, dtype=np.float32)
     dt = time.perf_counter() - t0
     work = Ps.cell_updates(shots)
     return {"value": work / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"oracle FP32 gradient (forward+misfit+adjoint+filter) of {shots} of the "
-                      f"{P.nsrc} shots, nt={nt} of {P.nt}, {P.nx}x{P.nz}; {dt:.1f} s"}
+                      f"{P.nsrc} shots, nt={nt} of {P.nt}, {P.nx}x{P.nz}; {dt:.1f} s"}, res
 
 
 def workload(inv, a):
     """(problem, shots per launch, snapshot format, checkpoint interval,
     bytes per cell-update of the step kernel, description) of a workload."""
     pr, H = inv.problems, inv.invhip
+    if a.workload == "c0":
+        P = pr.checker(nt=a.nt or 2000, sigma=a.sigma, explosive=True)
+        return (P, 4, H.INV_SNAP_F32, 0, PSV_STEP_BYTES,
+                f"batched P-SV gradient, checkerboard 200x200 @ 10 m, 4 explosive shots, "
+                f"40 receivers, dt 1 ms, nt {P.nt}, sigma {P.sigma}")
     if a.workload == "c1":
         P = pr.marmousi(nt=a.nt or 4000, sigma=a.sigma)
         return (P, 16, H.INV_SNAP_F32, 0, PSV_STEP_BYTES,
@@ -345,6 +361,10 @@ def main():
                 "kernel": kernel, "alg_bytes_per_launch": step_bytes * cells,
                 "avg_us": avg, "samples": samples}
     ni, ti = ctx.prof_read(inv.invhip.INV_PROF_INTERACT)
+    gpu_grad = None
+    if a.workload == "c0":
+        # the last timed gradient and its observed data, for the CPU comparison
+        gpu_grad = (f, ctx.kernels_get(), np.stack([ctx.obs_get(s) for s in range(P.nsrc)]))
     ctx.close()
 
     plain = None
@@ -369,11 +389,25 @@ def main():
                  "note": "one shot per launch, same kernels, no batching, no graph"}
         print(f"# plain path: {pms:.1f} ms per gradient", file=sys.stderr, flush=True)
 
-    cpu = None
+    cpu = parity = None
     if not a.no_cpu:
-        # c1 in full (a sample of its shots); c2 / c4: 2 shots x 200 steps
-        cpu = (cpu_baseline(P, a.cpu_shots) if a.workload == "c1" else
-               cpu_baseline(P, 2, nt=200))
+        # c0 in full; c1 in full on a sample of its shots; c2 / c4 on 4 shots x
+        # 2000 steps / 2 shots x 150 steps (10-30 s of CPU work each)
+        if a.workload == "c0":
+            cpu, (fo, *ko) = cpu_baseline(P, P.nsrc, obs=gpu_grad[2])
+            fg, kg, _ = gpu_grad
+            parity = {"reference": "oracle FP32 gradient of all shots on the same observed data",
+                      "math": ["exact", "fast"][a.math],
+                      "misfit_rel_err": abs(fg - fo) / abs(fo)}
+            for n, x, y in zip(("k_lambda", "k_mu", "k_rho"), kg, ko):
+                parity[n + "_rel_l2"] = float(np.linalg.norm(x - y) / np.linalg.norm(y))
+            print(f"# parity {parity}", file=sys.stderr, flush=True)
+        elif a.workload == "c1":
+            cpu = cpu_baseline(P, min(a.cpu_shots, P.nsrc))[0]
+        elif a.workload == "c2":
+            cpu = cpu_baseline(P, 4, nt=2000)[0]
+        else:
+            cpu = cpu_baseline(P, 2, nt=150)[0]
     acc = accuracy(inv, a) if a.workload in ("c2", "c4") and not a.no_accuracy else None
     serving = None
     if a.serve_requests > 0 and a.workload == "c1":
@@ -391,7 +425,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded Marmousi-shaped model; observed data modelled on the true model)",
+        "data": ("synthetic (checkerboard model; observed data modelled on the true model)"
+                 if a.workload == "c0" else "synthetic (seeded Marmousi-shaped model; observed "
+                                            "data modelled on the true model)"),
         "config": {"workload": desc, "name": a.workload,
                    "nx": P.nx, "nz": P.nz, "shots": P.nsrc, "receivers": P.nrec, "nt": P.nt,
                    "batch": B, "sfe": P.sfe, "filter_sigma": P.sigma,
@@ -405,6 +441,8 @@ def main():
         "serving": serving,
         "accuracy": acc,
     }
+    if parity:
+        out["parity"] = parity
     if ni:
         out["interaction_avg_us"] = round(ti / ni * 1e3, 2)
     print(json.dumps(out))

@@ -117,6 +117,14 @@ int inv_model_to_vps(inv_ctx *ctx, float *vp, float *vs, float *rho);
  * wavelets stf_{x,y,z} [nsrc][nt] (Source::init, source/source.h:17-37). */
 int inv_sources_set(inv_ctx *ctx, int nsrc, const int *ix, const int *iz, const float *stf_x,
                     const float *stf_y, const float *stf_z);
+/* Source mechanism per source, after inv_sources_set (which resets all to 0):
+ * 0 the reference's point force (addSTF, solver/fdm.h:163-180: stf_x and
+ * stf_z added to the P-SV stress divergence, stf_y to the SH one); 1 an
+ * explosive source with no counterpart in the reference (source id 1 of
+ * sources.dat here): after the stress update of step it, stf_y[it] (= amp
+ * times the wavelet) is added to sxx and to szz at the source cell; P-SV
+ * only, fused column kernel only.  type == NULL resets to 0. */
+int inv_sources_type(inv_ctx *ctx, const int *type);
 int inv_receivers_set(inv_ctx *ctx, int nrec, const int *ix, const int *iz);
 
 /* ---- propagation -------------------------------------------------------- */

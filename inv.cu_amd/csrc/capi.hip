@@ -519,6 +519,7 @@ int inv_sources_set(inv_ctx *ctx, int nsrc, const int *ix, const int *iz, const 
         c.nsrc = nsrc;
         c.src_ix.assign(ix, ix + nsrc);
         c.src_iz.assign(iz, iz + nsrc);
+        c.src_type.assign(nsrc, 0);
         const size_t b = (size_t)nsrc * c.g.nt * sizeof(float);
         c.stf.ensure(3 * b);
         INV_HIP(hipMemcpy(c.stf.as<char>(), stf_x, b, hipMemcpyHostToDevice));
@@ -528,6 +529,23 @@ int inv_sources_set(inv_ctx *ctx, int nsrc, const int *ix, const int *iz, const 
         invalidate(c);
         fused_invalidate_lists(c);
         alloc_traces(c);
+    });
+}
+
+int inv_sources_type(inv_ctx *ctx, const int *type) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_src, INV_ERR_STATE, "sources not set");
+        std::vector<int> t(c.nsrc, 0);
+        if (type)
+            for (int s = 0; s < c.nsrc; s++) {
+                INV_REQUIRE(type[s] == 0 || type[s] == 1, INV_ERR_ARG,
+                            "source type must be 0 (point force) or 1 (explosive)");
+                t[s] = type[s];
+            }
+        sync(c);
+        c.src_type = t;
+        invalidate(c);
+        fused_invalidate_lists(c);
     });
 }
 
@@ -578,6 +596,8 @@ static void run_steps(Ctx &c, int kind, int first, int count, int flags, int it0
                 "the column kernel's columns per lane must divide nz");
     INV_REQUIRE(!c.slot_models || c.variant != INV_VARIANT_SPLIT, INV_ERR_UNSUPPORTED,
                 "per-slot models need a fused step kernel");
+    INV_REQUIRE(!c.any_explosive() || c.variant == INV_VARIANT_FUSED, INV_ERR_UNSUPPORTED,
+                "explosive sources need the fused column kernel (variant 0)");
     if (kind == 2 || (flags & INV_FWD_SNAPSHOTS)) snap_storage(c);
     if (!c.use_graphs || it0 != 0 || it1 != end) {
         run_steps_direct(c, kind, first, count, flags, it0, it1);

@@ -561,6 +561,11 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
 
     // 3. stress of step n+1, rows i0 .. i0+TX-1, columns j0 .. j0+CZ-1
     if (!jtin) return;
+    int q0 = 0, q1 = 0;
+    if (PSV && a.sinj_ptr) {
+        q0 = a.sinj_ptr[b * per + t];
+        q1 = a.sinj_ptr[b * per + t + 1];
+    }
     {
         const int j = jt;
         const bool zin = j >= 1 && j < nz - 2;
@@ -597,8 +602,18 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                     zz = 0.0f;
                 }
                 const float lam = qlam[p];
-                a.sxx1[o] = l_sxx[se] + g.dt * ((lam + 2 * mu) * xx + lam * zz);
-                a.szz1[o] = l_szz[se] + g.dt * ((lam + 2 * mu) * zz + lam * xx);
+                float nxx = l_sxx[se] + g.dt * ((lam + 2 * mu) * xx + lam * zz);
+                float nzz = l_szz[se] + g.dt * ((lam + 2 * mu) * zz + lam * xx);
+                // explosive sources this tile owns: stf_y into sxx and szz
+                for (int pp = q0; pp < q1; pp++)
+                    if (a.sinj_cell[pp] == kc) {
+                        const float f =
+                            a.inj_y[(long)b * a.inj_slot_stride + (long)a.sinj_row[pp] * a.nt + it];
+                        nxx += f;
+                        nzz += f;
+                    }
+                a.sxx1[o] = nxx;
+                a.szz1[o] = nzz;
                 a.sxz1[o] = l_sxz[se] + g.dt * (mu * (xz + zx));
             }
         }
@@ -677,6 +692,8 @@ void fused_invalidate_lists(Ctx &c) {
     INV_HIP(hipStreamSynchronize(c.stream));
     for (auto &kv : c.src_lists) release_list(kv.second);
     c.src_lists.clear();
+    for (auto &kv : c.src_slists) release_list(kv.second);
+    c.src_slists.clear();
     c.f_rec_tx = -1;
 }
 
@@ -701,12 +718,29 @@ PointList &source_list(Ctx &c, int first, int count, int key) {
     std::vector<std::vector<int>> cells(count), rows(count);
     for (int b = 0; b < count; b++)
         for (int is = 0; is < c.nsrc; is++)
-            if (first < 0 || is == first + b) {
+            if ((first < 0 || is == first + b) && !(is < (int)c.src_type.size() && c.src_type[is] == 1)) {
                 cells[b].push_back(c.src_ix[is] * c.g.nz + c.src_iz[is]);
                 rows[b].push_back(is);
             }
     if (key == 0) build_point_list(c, pl, count, cells, rows);
     else build_tiled(c, pl, count, cells, rows, tile_rows(c), 1, 2);
+    return pl;
+}
+
+PointList &stress_source_list(Ctx &c, int first, int count, int key) {
+    const long long k = ((long long)(first + 2) << 40) | ((long long)count << 20) | key;
+    auto found = c.src_slists.find(k);
+    if (found != c.src_slists.end()) return found->second;
+    PointList &pl = c.src_slists[k];
+    std::vector<std::vector<int>> cells(count), rows(count);
+    for (int b = 0; b < count; b++)
+        for (int is = 0; is < c.nsrc; is++)
+            if ((first < 0 || is == first + b) && is < (int)c.src_type.size() &&
+                c.src_type[is] == 1) {
+                cells[b].push_back(c.src_ix[is] * c.g.nz + c.src_iz[is]);
+                rows[b].push_back(is);
+            }
+    build_tiled(c, pl, count, cells, rows, tile_rows(c), 0, 0);
     return pl;
 }
 
@@ -936,7 +970,10 @@ void fused_prepare(Ctx &c, int kind, int first, int count, int flags) {
     // snapshot storage only for runs that store or read snapshots
     if (kind == 2 || (flags & INV_FWD_SNAPSHOTS)) snap_storage(c);
     // the adjoint's recomputation injects the forward's sources
-    if (kind == 1 || (c.ckpt > 0 && c.g.adj)) source_list(c, first, count, list_key(c));
+    if (kind == 1 || (c.ckpt > 0 && c.g.adj)) {
+        source_list(c, first, count, list_key(c));
+        if (c.any_explosive()) stress_source_list(c, first, count, list_key(c));
+    }
 }
 
 template <bool SH, bool PSV, bool FAST>
@@ -1024,6 +1061,13 @@ static void forward_args(Ctx &c, StepArgs &a, int first, int count, bool record)
     a.inj_cell = sl.cell.as<int>();
     a.inj_row = sl.row.as<int>();
     a.inj_per_slot = 1;
+    a.sinj_ptr = a.sinj_cell = a.sinj_row = nullptr;
+    if (c.any_explosive() && c.g.psv) {
+        PointList &xl = stress_source_list(c, first, count, list_key(c));
+        a.sinj_ptr = xl.ptr.as<int>();
+        a.sinj_cell = xl.cell.as<int>();
+        a.sinj_row = xl.row.as<int>();
+    }
     const long tr = (long)c.nrec * nt;
     const int cy = 0, cx = c.g.sh ? 1 : 0, cz = cx + 1;
     a.out_y = c.out.as<float>() + cy * tr;

@@ -349,6 +349,11 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
 
     // 3. stress of step n+1 on the owned cells (velocity LDS rows tr .. tr+3)
     if (!cown) return;
+    int q0 = 0, q1 = 0;
+    if (PSV && a.sinj_ptr) {
+        q0 = a.sinj_ptr[b * per + t];
+        q1 = a.sinj_ptr[b * per + t + 1];
+    }
 #pragma unroll
     for (int p = 0; p < PT; p++) {
         const int tr = wave + p * NW;
@@ -416,6 +421,15 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
                 o1[v] = sa[v] + g.dt * ((lam + 2 * mu) * xx + lam * zz);
                 o2[v] = sc[v] + g.dt * ((lam + 2 * mu) * zz + lam * xx);
                 o3[v] = sxz[v] + g.dt * (mu * (xz + zx));
+                // explosive sources this tile owns: stf_y into sxx and szz
+                const long kc = (long)i * nz + j;
+                for (int pp = q0; pp < q1; pp++)
+                    if (a.sinj_cell[pp] == kc) {
+                        const float f =
+                            a.inj_y[(long)b * a.inj_slot_stride + (long)a.sinj_row[pp] * a.nt + it];
+                        o1[v] += f;
+                        o2[v] += f;
+                    }
             }
             vstore<V>(S1o + o, o1);
             vstore<V>(S2o + o, o2);

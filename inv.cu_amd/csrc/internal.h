@@ -137,10 +137,21 @@ struct Ctx {
     int nsrc = 0, nrec = 0;
     std::vector<int> src_ix, src_iz, rec_ix, rec_iz;
     DevBuf stf;  // [3][nsrc][nt]  x, y, z
+    // source mechanism per source: 0 point force (addSTF), 1 explosive (stf_y
+    // added to sxx and szz after the stress update)
+    std::vector<int> src_type;
+    bool any_explosive() const {
+        for (int t : src_type)
+            if (t == 1) return true;
+        return false;
+    }
     PointList rec_list;  // receivers, one set shared by all slots
     // sources of a forward, one set per slot, kept per shot range (and tile
     // shape) so that a captured graph of one batch never reads another's
     std::map<long long, PointList> src_lists;
+    // explosive sources of a forward, one set per slot, bucketed by the tile
+    // that owns their cell (the fused kernels add them to the stress)
+    std::map<long long, PointList> src_slists;
     // the same sets bucketed by fused tile: injection points by the tile's
     // velocity region (tile plus halo), recording points by the tile itself
     PointList f_rec_v, f_rec_t;
@@ -220,6 +231,8 @@ void fdm_prepare(Ctx &c, int kind, int first, int count);
 // the per-slot source list of shots first..first+count-1 for a tile shape
 // (key 0: the kernel-pair tiles); built on first use
 PointList &source_list(Ctx &c, int first, int count, int key);
+// the explosive sources of the same shots, bucketed by owning fused tile
+PointList &stress_source_list(Ctx &c, int first, int count, int key);
 void release_list(PointList &pl);
 // device pointer of the current buffer of a ping-pong field
 inline float *cur_field(Ctx &c, DevBuf &a, DevBuf &b) {

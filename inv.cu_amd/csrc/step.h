@@ -21,6 +21,10 @@ struct StepArgs {
     long inj_slot_stride;
     const int *inj_ptr, *inj_cell, *inj_row;
     int inj_per_slot;
+    // explosive sources (per slot, bucketed by owning tile): inj_y of the
+    // source's row is added to sxx and szz after the stress update; null when
+    // there are none
+    const int *sinj_ptr, *sinj_cell, *sinj_row;
     int nt, it;
     // receiver recording (saveRec, fdm.h:181-196), bucketed by tile
     float *out_x, *out_y, *out_z;

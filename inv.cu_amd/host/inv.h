@@ -164,6 +164,8 @@ class Source {
         create();
     }
     virtual void create() = 0;
+    // injection mechanism (inv_sources_type): 0 point force, 1 explosive
+    virtual int mechanism() const { return 0; }
     virtual ~Source() {}
 };
 
@@ -174,6 +176,14 @@ class RickerSource : public Source {
                                    stf_z.data());
         if (rc != INV_OK) throw std::runtime_error("invalid Ricker parameters");
     }
+};
+
+// Source id 1, no counterpart in the reference (whose only source is the
+// point force of id 0): the same Ricker wavelet, injected as an explosion
+// (amp times the wavelet added to sxx and szz after each stress update).
+class ExplosiveRickerSource : public RickerSource {
+  public:
+    int mechanism() const override { return 1; }
 };
 
 namespace module {
@@ -304,6 +314,7 @@ class FdmSolver : public Solver {
         // sources: positions (calcIdx, fdm.h:454-455) and wavelets (solver.h:191-200)
         vector<int> sx(nsrc), szi(nsrc);
         vector<float> stfx(nsrc * nt), stfy(nsrc * nt), stfz(nsrc * nt);
+        vector<int> mech(nsrc, 0);
         for (size_t is = 0; is < nsrc; is++) {
             const vector<float> &s = config->src[is];
             src_x.push_back(s[0]);
@@ -313,12 +324,14 @@ class FdmSolver : public Solver {
             std::unique_ptr<Source> src(module::source((size_t)s[2]));
             if (!src) throw std::runtime_error("unknown source type");
             src->init(s.data() + 3, nt, dt);
+            mech[is] = src->mechanism();
             std::copy(src->stf_x.begin(), src->stf_x.end(), stfx.begin() + is * nt);
             std::copy(src->stf_y.begin(), src->stf_y.end(), stfy.begin() + is * nt);
             std::copy(src->stf_z.begin(), src->stf_z.end(), stfz.begin() + is * nt);
         }
         INVH(ctx, inv_sources_set(ctx, (int)nsrc, sx.data(), szi.data(), stfx.data(),
                                   stfy.data(), stfz.data()));
+        if (std::count(mech.begin(), mech.end(), 1)) INVH(ctx, inv_sources_type(ctx, mech.data()));
         vector<int> rx(nrec), rz(nrec);
         for (size_t ir = 0; ir < nrec; ir++) {
             rec_x.push_back(config->rec[ir][0]);
@@ -1227,7 +1240,13 @@ class LBFGSOptimizer : public Optimizer {
 // ---------------------------------------------------------------------------
 namespace module {
 inline Solver *solver(size_t id) { return id == 0 ? new FdmSolver() : nullptr; }
-inline Source *source(size_t id) { return id == 0 ? new RickerSource() : nullptr; }
+inline Source *source(size_t id) {
+    switch (id) {
+        case 0: return new RickerSource();
+        case 1: return new ExplosiveRickerSource();
+        default: return nullptr;
+    }
+}
 inline Filter *filter(size_t id) { return id == 0 ? new GaussianFilter() : nullptr; }
 inline Misfit *misfit(size_t id) {
     switch (id) {

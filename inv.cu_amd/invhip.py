@@ -65,6 +65,7 @@ SIGNATURES = {
     "inv_model_to_vps": [_vp, _vp, _vp, _vp],
     "inv_sources_set": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
     "inv_receivers_set": [_vp, _i, _vp, _vp],
+    "inv_sources_type": [_vp, _vp],
     "inv_forward": [_vp, _i, _i, _i],
     "inv_adjoint": [_vp, _i, _i],
     "inv_kernels_zero": [_vp],
@@ -255,6 +256,12 @@ class Context:
         sx, sy, sz = f(stf_x), f(stf_y), f(stf_z)
         self.nsrc = len(ix)
         self._chk(self.L.inv_sources_set(self.h, len(ix), _p(ix), _p(iz), _p(sx), _p(sy), _p(sz)))
+
+    def sources_type(self, types):
+        """Source mechanism per source: 0 point force, 1 explosive."""
+        t = np.ascontiguousarray(types, np.int32)
+        assert len(t) == self.nsrc
+        self._chk(self.L.inv_sources_type(self.h, _p(t)))
 
     def receivers_set(self, ix, iz):
         ix = np.ascontiguousarray(ix, np.int32)

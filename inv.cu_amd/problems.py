@@ -63,6 +63,7 @@ class Problem:
     trace_type: int = 0
     misfit: int = 0
     sigma: int = 0
+    src_type: np.ndarray = None  # per source: 0 point force (default), 1 explosive
     meta: dict = field(default_factory=dict)
 
     @property
@@ -100,6 +101,8 @@ class Problem:
         ctx.grid_set(**self.grid_kwargs(batch=batch, adj=adj))
         self.load_model(ctx, model)
         ctx.sources_set(self.src_ix, self.src_iz, self.stf_x, self.stf_y, self.stf_z)
+        if self.src_type is not None and np.any(self.src_type):
+            ctx.sources_type(self.src_type)
         ctx.receivers_set(self.rec_ix, self.rec_iz)
         return ctx
 
@@ -137,11 +140,12 @@ def _smooth(a, sigma):
 # ---------------------------------------------------------------------------
 def checker(nx=200, nz=200, dx=10.0, dt=1e-3, nt=2000, cell=20, frac=0.10, vp0=3000.0,
             vs0=1732.0, rho0=2200.0, nsrc=4, nrec=40, f0=10.0, t0=0.15, amp=1e10, sh=0,
-            psv=1, src_depth=20, rec_depth=10, sigma=5, name="checker"):
+            psv=1, src_depth=20, rec_depth=10, sigma=5, explosive=False, name="checker"):
     """P-SV checkerboard gradient setup: background +-frac on vp, vs, rho in
     squares of `cell` cells, sign (-1)^(floor(i/cell)+floor(j/cell)); the
     initial model is the background.  Sources evenly spaced in x at
-    `src_depth` cells, receivers at `rec_depth` cells."""
+    `src_depth` cells (explosive ones with explosive=True), receivers at
+    `rec_depth` cells."""
     i = np.arange(nx)[:, None]
     j = np.arange(nz)[None, :]
     sign = np.where(((i // cell) + (j // cell)) % 2 == 0, 1.0, -1.0)
@@ -162,6 +166,7 @@ def checker(nx=200, nz=200, dx=10.0, dt=1e-3, nt=2000, cell=20, frac=0.10, vp0=3
                    rho.ravel(), lam0, mu0, b * f32(rho0), src_ix,
                    np.full(nsrc, src_depth, np.int32), sx, sy, sz, rec_ix,
                    np.full(nrec, rec_depth, np.int32), sigma=sigma,
+                   src_type=np.full(nsrc, 1 if explosive else 0, np.int32),
                    meta=dict(f0=f0, t0=t0, amp=amp))
 
 

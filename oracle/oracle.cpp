@@ -285,9 +285,9 @@ int forward(const orc_sim *s, int isrc, int nsteps, T *out_x, T *out_y, T *out_z
             }
         }
         p.divS();
-        // addSTF, solver/fdm.h:163-180
+        // addSTF, solver/fdm.h:163-180 (point-force sources)
         for (int is = 0; is < s->nsrc; is++) {
-            if (isrc < 0 || isrc == is) {
+            if ((isrc < 0 || isrc == is) && !(s->src_type && s->src_type[is] == 1)) {
                 long km = (long)s->src_ix[is] * nz + s->src_iz[is];
                 long ks = (long)is * nt + it;
                 if (s->sh) p.dsy[km] += (T)s->stf_y[ks];
@@ -298,6 +298,15 @@ int forward(const orc_sim *s, int isrc, int nsteps, T *out_x, T *out_y, T *out_z
             }
         }
         p.divV();
+        // explosive sources: equal stf into sxx and szz after the stress update
+        if (s->psv && s->src_type)
+            for (int is = 0; is < s->nsrc; is++)
+                if ((isrc < 0 || isrc == is) && s->src_type[is] == 1) {
+                    const long km = (long)s->src_ix[is] * nz + s->src_iz[is];
+                    const T f = (T)s->stf_y[(long)is * nt + it];
+                    p.sxx[km] += f;
+                    p.szz[km] += f;
+                }
         // saveRec, solver/fdm.h:181-196 with the choice of fdm.h:540-551
         const bool disp = s->adj || s->trace_type == 1;
         for (int ir = 0; ir < s->nrec; ir++) {

@@ -43,6 +43,11 @@ typedef struct {
     const float *stf_x, *stf_y, *stf_z;   /* [nsrc][nt] */
     int nrec;
     const int *rec_ix, *rec_iz;
+    /* [nsrc] or NULL (all 0).  0: the reference's point force (addSTF,
+     * solver/fdm.h:163-180).  1: explosive source, no reference counterpart:
+     * after the stress update of step it, stf_y[is][it] is added to sxx and
+     * to szz at the source cell (P-SV only). */
+    const int *src_type;
 } orc_sim;
 
 /* ---- setup (float, as the reference computes them) ---- */

@@ -36,6 +36,7 @@ class Sim(C.Structure):
         ("nsrc", C.c_int), ("src_ix", C.c_void_p), ("src_iz", C.c_void_p),
         ("stf_x", C.c_void_p), ("stf_y", C.c_void_p), ("stf_z", C.c_void_p),
         ("nrec", C.c_int), ("rec_ix", C.c_void_p), ("rec_iz", C.c_void_p),
+        ("src_type", C.c_void_p),
     ]
 
 
@@ -155,7 +156,7 @@ class Problem:
 
     def __init__(self, nx, nz, dx, dz, dt, nt, lam, mu, rho, ab, src_ix, src_iz, stf_x,
                  stf_y, stf_z, rec_ix, rec_iz, sh=1, psv=0, sfe=10, adj=1, trace_type=0,
-                 inv_lambda=1, inv_mu=1, inv_rho=1):
+                 inv_lambda=1, inv_mu=1, inv_rho=1, src_type=None):
         f = lambda a: np.ascontiguousarray(a, np.float32).ravel()
         i = lambda a: np.ascontiguousarray(a, np.int32).ravel()
         self.nx, self.nz, self.nt = int(nx), int(nz), int(nt)
@@ -168,6 +169,8 @@ class Problem:
         self.stf_z = f(stf_z).reshape(-1)
         self.sh, self.psv, self.sfe, self.adj, self.trace_type = sh, psv, sfe, adj, trace_type
         self.inv = (inv_lambda, inv_mu, inv_rho)
+        # source mechanism per source: 0 point force, 1 explosive (see oracle.h)
+        self.src_type = None if src_type is None else i(src_type)
         self.sim = self._make()
 
     def _make(self):
@@ -179,6 +182,7 @@ class Problem:
         s.nsrc, s.src_ix, s.src_iz = self.nsrc, _ptr(self.src_ix), _ptr(self.src_iz)
         s.stf_x, s.stf_y, s.stf_z = _ptr(self.stf_x), _ptr(self.stf_y), _ptr(self.stf_z)
         s.nrec, s.rec_ix, s.rec_iz = self.nrec, _ptr(self.rec_ix), _ptr(self.rec_iz)
+        s.src_type = None if self.src_type is None else _ptr(self.src_type)
         return s
 
     @property

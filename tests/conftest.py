@@ -37,7 +37,8 @@ def oracle_problem(P, model="init", adj=1, trace_type=None):
                             P.src_iz, P.stf_x, P.stf_y, P.stf_z, P.rec_ix, P.rec_iz, sh=P.sh,
                             psv=P.psv, sfe=P.sfe, adj=adj,
                             trace_type=P.trace_type if trace_type is None else trace_type,
-                            inv_lambda=P.inv_lambda, inv_mu=P.inv_mu, inv_rho=P.inv_rho)
+                            inv_lambda=P.inv_lambda, inv_mu=P.inv_mu, inv_rho=P.inv_rho,
+                            src_type=P.src_type)
 
 
 def oracle_traces(tr, P):

@@ -236,3 +236,58 @@ def test_geometry_helpers():
     assert pyoracle.calc_idx(0.0, 100.0, 11) == 0
     assert pyoracle.calc_idx(100.0, 100.0, 11) == 10
     assert pyoracle.calc_idx(54.9, 100.0, 11) == 5
+
+
+def _explosive(nx, nz, nt, src, stf_y, ab, src_type=1, stf_xz=None):
+    lam, mu, rho, _ = homogeneous(nx, nz, nt)
+    one = lambda a: np.stack([np.asarray(a, np.float32)])
+    sx, sz = (np.zeros(nt, np.float32),) * 2 if stf_xz is None else stf_xz
+    return pyoracle.Problem(nx, nz, 10.0, 10.0, 1e-3, nt, lam, mu, rho, ab, [src[0]], [src[1]],
+                            one(sx), one(stf_y), one(sz), [2], [2], sh=0, psv=1, adj=0,
+                            src_type=[src_type])
+
+
+def test_explosive_source_first_step():
+    """Explosive source (source type 1, no reference counterpart): from rest,
+    one step leaves exactly stf_y[0] in sxx and szz at the source cell and
+    nothing anywhere else.  The point-force wavelets stf_x / stf_z are set
+    and must not be injected."""
+    nx, nz, nt = 30, 26, 4
+    src = (12, 17)
+    stf_y = np.array([2.5, 0.0, 0.0, 0.0], np.float32)
+    junk = (np.full(nt, 7.0, np.float32), np.full(nt, -3.0, np.float32))
+    P = _explosive(nx, nz, nt, src, stf_y, np.ones(nx * nz, np.float32), stf_xz=junk)
+    _, _, st = P.forward(0, nsteps=1, dtype=np.float64)
+    vx, vz, ux, uz, sxx, szz, sxz = st
+    k = src[0] * nz + src[1]
+    want = np.zeros(nx * nz)
+    want[k] = 2.5
+    assert np.array_equal(sxx, want) and np.array_equal(szz, want)
+    for f in (vx, vz, ux, uz, sxz):
+        assert not np.any(f)
+    # the same wavelets as a point force move the velocity instead
+    F = _explosive(nx, nz, nt, src, stf_y, np.ones(nx * nz, np.float32), src_type=0,
+                   stf_xz=junk)
+    _, _, st = F.forward(0, nsteps=1, dtype=np.float64)
+    assert st[0][k] != 0 and st[1][k] != 0
+
+
+def test_explosive_source_xz_symmetry():
+    """The scheme is symmetric under exchanging x and z on a square
+    homogeneous grid with all four absorbing strips (the update equations map
+    onto each other, vx <-> vz, sxx <-> szz), so an explosion at the centre
+    gives vx(i, j) = vz(j, i) and sxx(i, j) = szz(j, i); an injection into one
+    component only, a sign error or an off-centre cell breaks it.  A point
+    force (the control) does not have the symmetry."""
+    n, nt = 41, 150
+    ab = pyoracle.absbound(n, n, 10, 0.015)
+    stf = pyoracle.source(nt, np.float32(1e-3), 30.0, 0.04, 0.0, 1e9)
+    P = _explosive(n, n, nt, (20, 20), stf[1], ab)
+    _, _, st = P.forward(0, dtype=np.float64)
+    vx, vz, ux, uz, sxx, szz, sxz = (f.reshape(n, n) for f in st)
+    assert np.abs(vx).max() > 0
+    assert rel(vx, vz.T) < 1e-10 and rel(ux, uz.T) < 1e-10
+    assert rel(sxx, szz.T) < 1e-10 and rel(sxz, sxz.T) < 1e-10
+    F = _explosive(n, n, nt, (20, 20), stf[1], ab, src_type=0, stf_xz=(stf[0], stf[2]))
+    _, _, st = F.forward(0, dtype=np.float64)
+    assert rel(st[0].reshape(n, n), st[1].reshape(n, n).T) > 0.1
