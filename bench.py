@@ -84,6 +84,10 @@ def parse():
     ap.add_argument("--tile-x", type=int, default=0,
                     help="fused tile rows (0: inv_autotune picks rows and columns per lane)")
     ap.add_argument("--cols", type=int, default=1, help="columns per lane with --tile-x")
+    ap.add_argument("--pair-rows", type=int, default=-1,
+                    help="two-step kernel tile rows (-1: the autotuner decides, 0: one step "
+                         "per launch)")
+    ap.add_argument("--pair-cols", type=int, default=2, help="two-step kernel columns per lane")
     ap.add_argument("--math", type=int, default=1, help="0 exact (IEEE divisions), 1 fast")
     ap.add_argument("--graphs", type=int, default=1, help="capture time loops in HIP graphs")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -120,9 +124,11 @@ def serving_leg(inv, n, rate_x, snap_format):
             "p99_speedup": round(plain["p99_latency_s"] / batched["p99_latency_s"], 3)}
 
 
-def configure_kernel(ctx, a, count):
-    """The step kernel: autotuned (default) or as given; returns its tile."""
+def configure_kernel(ctx, a, count, pair=True):
+    """The step kernels: autotuned (default) or as given; returns the tiles.
+    pair=False keeps one step per launch."""
     if a.variant == 0 and a.tile_x == 0:
+        # a one-step tile and, over it, a two-step tile if one is faster
         v, tx, cols = ctx.autotune(0, count, steps=20)
     elif a.variant == 0:
         ctx.set_tile(a.tile_x, a.cols)
@@ -130,7 +136,15 @@ def configure_kernel(ctx, a, count):
     else:
         ctx.set_variant(a.variant, a.tile_x)
         v, tx, cols = a.variant, a.tile_x, 1
-    return {"variant": v, "rows": tx, "cols_per_lane": cols}
+    if not pair or a.variant != 0 or (a.tile_x != 0 and a.pair_rows < 0):
+        ctx.set_pair(0)
+    elif a.pair_rows >= 0:
+        ctx.set_pair(a.pair_rows, a.pair_cols)
+    tile = {"variant": v, "rows": tx, "cols_per_lane": cols}
+    pr, pv = ctx.get_pair()
+    if pr:
+        tile["two_step"] = {"rows": pr, "cols_per_lane": pv}
+    return tile
 
 
 def cpu_baseline(P, shots, nt=None, obs=None):
@@ -278,6 +292,8 @@ def main():
             print(json.dumps(s), flush=True)
         return
     P, B, snap_fmt, ckpt, step_bytes, desc = workload(inv, a)
+    if ckpt and P.nt % ckpt:
+        ckpt = P.nt  # a shortened run (--nt) the interval does not divide: one segment
     if a.workload == "c1":
         B = min(a.batch, P.nsrc)
     print(f"# workload {a.workload} {P.name} {P.nx}x{P.nz} shots={P.nsrc} nt={P.nt} batch={B}",
@@ -324,43 +340,81 @@ def main():
     print(f"# timed {a.steps} gradients: {ms:.1f} ms each, misfit {f:.6e}", file=sys.stderr,
           flush=True)
 
-    nv, tv = ctx.prof_read(inv.invhip.INV_PROF_VELOCITY)
-    ns, ts = ctx.prof_read(inv.invhip.INV_PROF_STRESS)
-    nf, tf = ctx.prof_read(inv.invhip.INV_PROF_STEP)
-    nfr, tfr = ctx.prof_read(inv.invhip.INV_PROF_FWD_RUN)
-    nar, tar = ctx.prof_read(inv.invhip.INV_PROF_ADJ_RUN)
+    H = inv.invhip
+    nv, tv = ctx.prof_read(H.INV_PROF_VELOCITY)
+    ns, ts = ctx.prof_read(H.INV_PROF_STRESS)
+    nf, tf = ctx.prof_read(H.INV_PROF_STEP)
+    npr, tpr = ctx.prof_read(H.INV_PROF_PAIR)
+    nfr, tfr = ctx.prof_read(H.INV_PROF_FWD_RUN)
+    nar, tar = ctx.prof_read(H.INV_PROF_ADJ_RUN)
+    ni, ti = ctx.prof_read(H.INV_PROF_INTERACT)
     roof = None
     cells = B * P.N
-    step_s = None
-    if nfr:
+    two = "two_step" in tile
+    # a forward graph holds one kind of step launch when every pair of steps
+    # is paired (no 16-bit snapshot scales, no checkpoints, nt even) or none
+    uniform = not two or (snap_fmt == H.INV_SNAP_F32 and ckpt == 0 and P.nt % 2 == 0)
+    launch_s, kernel, avg, samples, per_launch_steps = None, None, {}, {}, 1
+    if nfr and uniform:
         # replayed forward graphs: every launch in them but a handful of
-        # set-up fills is the step kernel, so time / steps is its duration
-        step_s = tfr / nfr / 1e3
-        kernel = "fused P-SV time step (k_col / k_colv, autotuned tile), from forward-graph replays"
-        avg = {"step": round(step_s * 1e6, 2)}
+        # set-up fills is the step kernel, so time / launches is its duration
+        per_launch_steps = 2 if two else 1
+        launch_s = tfr / (nfr / per_launch_steps) / 1e3
+        kernel = ("two-step P-SV kernel k_pair, from forward-graph replays" if two else
+                  "fused P-SV time step (k_col / k_colv, autotuned tile), from forward-graph "
+                  "replays")
+        avg["two_step_launch" if two else "step"] = round(launch_s * 1e6, 2)
         samples = {"forward_steps": nfr}
-        if nar:
-            avg["adjoint_step_incl_interaction"] = round(tar / nar * 1e3, 2)
+    elif two:
+        # mixed one- and two-step launches: time each launch of a short direct
+        # forward (graphs off) right after the timed region
+        if not (npr and nf):
+            ctx.set_graphs(False)
+            ctx.set_step_limit(min(P.nt, 400))
+            ctx.prof_reset()
+            ctx.prof_enable(1)
+            ctx.forward(0, B, snapshots=True)
+            ctx.sync()
+            ctx.prof_enable(0)
+            ctx.set_step_limit(0)
+            nf, tf = ctx.prof_read(H.INV_PROF_STEP)
+            npr, tpr = ctx.prof_read(H.INV_PROF_PAIR)
+        if npr:
+            per_launch_steps = 2
+            launch_s = tpr / npr / 1e3
+            kernel = "two-step P-SV kernel k_pair, HIP events on direct launches"
+            avg["two_step_launch"] = round(launch_s * 1e6, 2)
+            if nf:
+                avg["one_step_launch"] = round(tf / nf * 1e3, 2)
+            samples = {"two_step": npr, "one_step": nf}
     elif nf:
-        step_s = tf / nf / 1e3
+        launch_s = tf / nf / 1e3
         kernel = "fused P-SV time step (k_col / k_colv)"
-        avg = {"step": round(tf / nf * 1e3, 2)}
+        avg["step"] = round(launch_s * 1e6, 2)
         samples = {"step": nf}
     elif nv and ns:
-        step_s = (tv / nv + ts / ns) / 1e3
+        launch_s = (tv / nv + ts / ns) / 1e3
         kernel = "P-SV stencil step (velocity + stress launch pair)"
         avg = {"velocity": round(tv / nv * 1e3, 2), "stress": round(ts / ns * 1e3, 2)}
         samples = {"velocity": nv, "stress": ns}
-    if step_s:
-        achieved = step_bytes * cells / step_s / 1e9
-        profiled = (a.workload == "c1" and (tile["variant"], tile["rows"], tile["cols_per_lane"],
-                                             a.math, B) == PROFILED_TILE and nfr)
+    if nfr:
+        avg["forward_per_step"] = round(tfr / nfr * 1e3, 2)
+    if nar:
+        avg["adjoint_per_step_incl_interaction"] = round(tar / nar * 1e3, 2)
+    if launch_s:
+        # each wavefield and model field is read and written once per launch
+        achieved = step_bytes * cells / launch_s / 1e9
+        profiled = (a.workload == "c1" and not two and
+                    (tile["variant"], tile["rows"], tile["cols_per_lane"], a.math, B)
+                    == PROFILED_TILE and nfr)
         traffic = round(MEASURED_TRAFFIC_PER_CELL * cells) if profiled else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "kernel": kernel, "alg_bytes_per_launch": step_bytes * cells,
-                "avg_us": avg, "samples": samples}
-    ni, ti = ctx.prof_read(inv.invhip.INV_PROF_INTERACT)
+                "steps_per_launch": per_launch_steps, "avg_us": avg, "samples": samples}
+        if per_launch_steps == 2:
+            # the same cell-updates at one step per launch would need this
+            roof["one_step_equivalent_gbps"] = round(2 * achieved, 1)
     gpu_grad = None
     if a.workload == "c0":
         # the last timed gradient and its observed data, for the CPU comparison
@@ -370,7 +424,7 @@ def main():
     plain = None
     if not a.no_plain and a.workload == "c1":
         pctx = P.context(batch=1, model="true")
-        configure_kernel(pctx, a, 1)
+        configure_kernel(pctx, a, 1, pair=False)
         pctx.set_math(a.math)
         pctx.set_graphs(False)  # the plain path: one shot per launch, no graph
         for s in range(P.nsrc):

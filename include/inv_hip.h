@@ -221,7 +221,8 @@ enum inv_prof_kernel {
     INV_PROF_INTERACT = 2,  /* adjoint kernel accumulation */
     INV_PROF_STEP = 3,      /* whole time step (fused variant) */
     INV_PROF_FWD_RUN = 4,   /* replayed forward graphs (count = time steps) */
-    INV_PROF_ADJ_RUN = 5    /* replayed adjoint graphs (count = time steps) */
+    INV_PROF_ADJ_RUN = 5,   /* replayed adjoint graphs (count = time steps) */
+    INV_PROF_PAIR = 6       /* two-step launches (count = launches) */
 };
 int inv_prof_enable(inv_ctx *ctx, int every);
 int inv_prof_reset(inv_ctx *ctx);
@@ -241,11 +242,27 @@ int inv_set_variant(inv_ctx *ctx, int variant, int tile_x);
  * (120 columns), 4 with 4 or 8 rows (248 columns); 2 and 4 need nz divisible
  * by them.  Results are identical for every tile.  Selects variant 0. */
 int inv_set_tile(inv_ctx *ctx, int tile_x, int cols_per_lane);
+/* Two time steps per launch (fused variant): the stress and velocity of the
+ * tile stay in LDS between the steps, so each field crosses HBM once per two
+ * steps.  tile_x rows x cols_per_lane columns per lane: 4, 8 or 12 rows x 2
+ * (116 columns per tile) or 4 or 8 x 4 (240 columns); tile_x = 0 turns it off
+ * (default).  Runs use it for every pair of steps that has no checkpoint
+ * between them, no kernel accumulation after the first and no 16-bit
+ * snapshot scale collected by the first and read by the second; the other
+ * steps use the one-step kernel.  Results are identical with and without it.
+ * Needs nz divisible by cols_per_lane.  No counterpart in the reference (one
+ * kernel launch per stencil half-step, solver/fdm.h:534-551). */
+int inv_set_pair(inv_ctx *ctx, int tile_x, int cols_per_lane);
+/* The two-step tile in use (tile_x 0: off), as set by inv_set_pair or chosen
+ * by inv_autotune. */
+int inv_get_pair(inv_ctx *ctx, int *tile_x, int *cols_per_lane);
 /* Autotuner: times `steps` forward steps of shots first..first+count-1 (with
  * snapshots and as many adjoint steps when the context keeps per-slot kernel
  * accumulators, batch > 1) with each tile of inv_set_tile that nz allows and
  * selects the fastest (a tile must beat the ones tried before it, in the
- * order above, by 2 %).  The
+ * order above, by 2 %); then, over that tile, each two-step tile of
+ * inv_set_pair that nz and the device's LDS allow, kept if 2 % faster than
+ * the best so far (inv_get_pair reports it).  The
  * choice is cached per (nx, nz, wave types, math mode) for the process, so
  * later contexts and batch sizes of the same grid take it without timing.
  * Overwrites the slots' wavefields, traces and snapshots.  No counterpart in the

@@ -598,6 +598,8 @@ static void run_steps(Ctx &c, int kind, int first, int count, int flags, int it0
                 "per-slot models need a fused step kernel");
     INV_REQUIRE(!c.any_explosive() || c.variant == INV_VARIANT_FUSED, INV_ERR_UNSUPPORTED,
                 "explosive sources need the fused column kernel (variant 0)");
+    INV_REQUIRE(c.variant != INV_VARIANT_FUSED || c.pair_tx == 0 || c.g.nz % c.pair_v == 0,
+                INV_ERR_UNSUPPORTED, "the two-step kernel's columns per lane must divide nz");
     if (kind == 2 || (flags & INV_FWD_SNAPSHOTS)) snap_storage(c);
     if (!c.use_graphs || it0 != 0 || it1 != end) {
         run_steps_direct(c, kind, first, count, flags, it0, it1);
@@ -634,6 +636,7 @@ static void run_steps(Ctx &c, int kind, int first, int count, int flags, int it0
             throw;
         }
         c.prof.every = every;
+        e.parity = c.parity;  // the host loop ran: the set the replay ends on
         INV_HIP(hipStreamEndCapture(c.stream, &e.graph));
         INV_HIP(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
         e.gen = c.gen;
@@ -645,9 +648,9 @@ static void run_steps(Ctx &c, int kind, int first, int count, int flags, int it0
     INV_HIP(hipGraphLaunch(found->second.exec, c.stream));
     if (p) prof_end(c, kind == 1 ? PROF_FWD_RUN : PROF_ADJ_RUN, ev, end);
     // the replay does not run the host loop: set the state-set parity it ends
-    // on (the fused kernels alternate sets each step, the kernel pair works in
-    // place)
-    c.parity = c.variant == INV_VARIANT_SPLIT ? 0 : (end & 1);
+    // on (the fused kernels alternate sets each launch, one or two steps per
+    // launch, as the captured loop recorded; the kernel pair works in place)
+    c.parity = c.variant == INV_VARIANT_SPLIT ? 0 : found->second.parity;
 }
 
 int inv_forward(inv_ctx *ctx, int first, int count, int flags) {
@@ -1134,12 +1137,40 @@ int inv_set_tile(inv_ctx *ctx, int tile_x, int cols_per_lane) {
     });
 }
 
+static bool pair_tile_ok(int rows, int cols) {
+    return rows == 0 || (cols == 2 && (rows == 4 || rows == 8 || rows == 12)) ||
+           (cols == 4 && (rows == 4 || rows == 8));
+}
+
+int inv_set_pair(inv_ctx *ctx, int tile_x, int cols_per_lane) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(pair_tile_ok(tile_x, cols_per_lane), INV_ERR_ARG,
+                    "two-step tile must be 4/8/12 rows x 2 columns per lane or 4/8 x 4 (0 rows: off)");
+        INV_REQUIRE(!c.have_grid || tile_x == 0 || c.g.nz % cols_per_lane == 0,
+                    INV_ERR_UNSUPPORTED, "nz must be divisible by the columns per lane");
+        sync(c);
+        c.pair_tx = tile_x;
+        c.pair_v = tile_x ? cols_per_lane : 2;
+        invalidate(c);
+        fused_invalidate_lists(c);
+    });
+}
+
+int inv_get_pair(inv_ctx *ctx, int *tile_x, int *cols_per_lane) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(tile_x && cols_per_lane, INV_ERR_ARG, "null output");
+        *tile_x = c.pair_tx;
+        *cols_per_lane = c.pair_v;
+    });
+}
+
 // ---- autotuner -------------------------------------------------------------
 // Candidate step kernels and tile shapes, timed on the context's own shape;
 // the best is cached per (grid, slots, wave types, math mode) for the process.
 namespace {
 struct TuneChoice {
     int variant, tile, cols;
+    int pair_tx = 0, pair_v = 2;  // two-step kernel tile (0 rows: off)
 };
 std::mutex tune_mu;
 std::map<std::string, TuneChoice> tune_cache;
@@ -1148,6 +1179,8 @@ void apply_choice(Ctx &c, const TuneChoice &t) {
     c.variant = t.variant;
     c.tile_x = t.tile;
     c.col_v = t.cols;
+    c.pair_tx = t.pair_tx;
+    c.pair_v = t.pair_v;
     invalidate(c);
     fused_invalidate_lists(c);
 }
@@ -1204,13 +1237,12 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                     fused_forward(c, first, count, flags, 0, steps);
                     if (adj) fused_adjoint(c, first, count, 0, steps);
                 };
-                for (size_t k = 0; k < cand.size(); k++) {
-                    const TuneChoice &t = cand[k];
+                auto timed = [&](const TuneChoice &t, bool first_cand) {
                     apply_choice(c, t);
                     if (adj) fused_prepare(c, 2, first, count, 0);
                     // untimed passes (the first candidate's also bring the
                     // clocks up), then the best of three timed passes
-                    for (int w = 0; w < (k == 0 ? 4 : 1); w++) pass();
+                    for (int w = 0; w < (first_cand ? 4 : 1); w++) pass();
                     float best_t = 1e30f;
                     for (int rep = 0; rep < 3; rep++) {
                         INV_HIP(hipEventRecord(e0, c.stream));
@@ -1222,16 +1254,44 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                         best_t = std::min(best_t, ms);
                     }
                     if (verbose)
-                        fprintf(stderr, "inv_autotune %s: %d rows x %d cols per lane: %.3f ms\n",
-                                key.c_str(), t.tile, t.cols, best_t);
-                    // candidates come in order of preference (one column
-                    // per lane first): a later one must be 2 % faster, so
-                    // that timing noise does not pick between near-ties
-                    if (best_t < 0.98 * best_ms) {
-                        best_ms = best_t;
-                        best = t;
+                        fprintf(stderr,
+                                "inv_autotune %s: %d rows x %d cols per lane, two-step %d x %d: "
+                                "%.3f ms\n",
+                                key.c_str(), t.tile, t.cols, t.pair_tx, t.pair_v, best_t);
+                    return best_t;
+                };
+                // one-step tiles, in order of preference (one column per
+                // lane first): a later one must be 2 % faster, so that timing
+                // noise does not pick between near-ties
+                for (size_t k = 0; k < cand.size(); k++) {
+                    const float ms = timed(cand[k], k == 0);
+                    if (ms < 0.98 * best_ms) {
+                        best_ms = ms;
+                        best = cand[k];
                     }
                 }
+                // then two-step tiles over the chosen one-step tile (which
+                // runs the steps that cannot be paired); a tile needing more
+                // LDS than the device allows is skipped
+                const TuneChoice single = best;
+                for (int v : {2, 4})
+                    for (int tx : {4, 8, 12}) {
+                        if (c.g.nz % v || !pair_tile_ok(tx, v)) continue;
+                        TuneChoice t = single;
+                        t.pair_tx = tx;
+                        t.pair_v = v;
+                        float ms;
+                        try {
+                            ms = timed(t, false);
+                        } catch (const Error &e) {
+                            if (e.code != INV_ERR_UNSUPPORTED) throw;
+                            continue;
+                        }
+                        if (ms < 0.98 * best_ms) {
+                            best_ms = ms;
+                            best = t;
+                        }
+                    }
             } catch (...) {
                 c.prof.every = every;
                 if (e0) (void)hipEventDestroy(e0);

@@ -1,0 +1,332 @@
+// Row updates of the column kernels with V adjacent columns per lane (the
+// wide one-step kernel k_colv, fdm_wide.hip, and the two-step kernel k_pair,
+// fdm_pair.hip).  One function per half step, so both kernels evaluate every
+// cell with the same expressions in the same order (those of k_col and of the
+// reference's divS / updateV / divV / updateS, solver/fdm.h:534-551) and give
+// bitwise the same wavefields.
+#pragma once
+
+#include "step.h"
+
+namespace inv {
+
+template <int V>
+__device__ __forceinline__ void vload(const float *__restrict__ p, float (&o)[V]) {
+    if constexpr (V == 2) {
+        const float2 t = *reinterpret_cast<const float2 *>(p);
+        o[0] = t.x;
+        o[1] = t.y;
+    } else {
+        const float4 t = *reinterpret_cast<const float4 *>(p);
+        o[0] = t.x;
+        o[1] = t.y;
+        o[2] = t.z;
+        o[3] = t.w;
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void vstore(float *__restrict__ p, const float (&o)[V]) {
+    if constexpr (V == 2) *reinterpret_cast<float2 *>(p) = make_float2(o[0], o[1]);
+    else *reinterpret_cast<float4 *>(p) = make_float4(o[0], o[1], o[2], o[3]);
+}
+
+// e[V + k] = row[c0 + k] for k in [-V, 2V): the lane's vector and its two
+// neighbours from an LDS row of W columns (indices clamped into the row; the
+// clamped values only reach columns no owned cell depends on)
+template <int V, int W>
+__device__ __forceinline__ void window(const float *row, int c0, float (&e)[3 * V]) {
+    const int cl = c0 - V < 0 ? 0 : c0 - V;
+    const int cr = c0 + V > W - V ? W - V : c0 + V;
+    float t[V];
+    vload<V>(row + cl, t);
+#pragma unroll
+    for (int v = 0; v < V; v++) e[v] = t[v];
+    vload<V>(row + c0, t);
+#pragma unroll
+    for (int v = 0; v < V; v++) e[V + v] = t[v];
+    vload<V>(row + cr, t);
+#pragma unroll
+    for (int v = 0; v < V; v++) e[2 * V + v] = t[v];
+}
+
+// Velocity of step it+1 (divS, addSTF, updateV) on the lane's V columns
+// jb .. jb+V-1 of grid row i.  ls1/ls2/ls3: stress of step it in LDS rows of
+// W columns (sxx, szz, sxz for P-SV; sxy, szy for SH), grid row i at LDS row
+// r, rows r-2 .. r+1 read.  v1/v2: the lane's velocity of step it, rho: rho
+// or dt/rho (fast math).  Point forces of step it from the list entries
+// p0 .. p1 whose cell lies in grid cells rlo .. rhi (this row's span of the
+// tile) are added in list order.  Cells outside the grid (rin false) give 0.
+template <bool PSV, int V, int W, bool FAST>
+__device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, const float *ls2,
+                                        const float *ls3, int r, int c0, int i, int jb, bool rin,
+                                        float abx, const float (&abz)[V], const float (&abt)[V],
+                                        const float (&v1)[V], const float (&v2)[V],
+                                        const float (&rho)[V], int p0, int p1, long rlo,
+                                        long rhi, int b, long it, float (&n1)[V],
+                                        float (&n2)[V]) {
+    const Geo &g = a.g;
+    const int nx = g.nx, nz = g.nz;
+    const bool xin = i >= 2 && i < nx - 2;
+    const float *I1 = PSV ? a.inj_x : a.inj_y, *I2 = a.inj_z;
+    if (!PSV) {
+        float x0[V], x1[V], x2[V], x3[V], ez[3 * V];
+        vload<V>(&ls1[(r - 2) * W + c0], x0);
+        vload<V>(&ls1[(r - 1) * W + c0], x1);
+        vload<V>(&ls1[r * W + c0], x2);
+        vload<V>(&ls1[(r + 1) * W + c0], x3);
+        window<V, W>(&ls2[r * W], c0, ez);
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const int j = jb + v;
+            const long kc = (long)i * nz + j;
+            float d = xin ? dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, g.a1x, g.a2x)
+                          : 0.0f;
+            if (j >= 2 && j < nz - 2)
+                d += dm<FAST>(ez[V + v], ez[V + v - 1], ez[V + v + 1], ez[V + v - 2], g.c8dz,
+                              g.c24dz, g.a1z, g.a2z);
+            for (int pp = p0; pp < p1; pp++) {
+                const int cp = a.inj_cell[pp];
+                if (cp < rlo || cp > rhi) continue;
+                if (cp == kc) d += I1[(long)b * a.inj_slot_stride + (long)a.inj_row[pp] * a.nt + it];
+            }
+            const float ab = (abx * abz[v]) * abt[v];
+            n1[v] = rin ? vel_update<FAST>(ab, v1[v], g.dt, d, rho[v]) : 0.0f;
+        }
+    } else {
+        float x0[V], x1[V], x2[V], x3[V], z0[V], z1[V], z2[V], z3[V], e3[3 * V], e2[3 * V];
+        vload<V>(&ls1[(r - 2) * W + c0], x0);
+        vload<V>(&ls1[(r - 1) * W + c0], x1);
+        vload<V>(&ls1[r * W + c0], x2);
+        vload<V>(&ls1[(r + 1) * W + c0], x3);
+        vload<V>(&ls3[(r - 2) * W + c0], z0);
+        vload<V>(&ls3[(r - 1) * W + c0], z1);
+        vload<V>(&ls3[r * W + c0], z2);
+        vload<V>(&ls3[(r + 1) * W + c0], z3);
+        window<V, W>(&ls3[r * W], c0, e3);
+        window<V, W>(&ls2[r * W], c0, e2);
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const int j = jb + v;
+            const long kc = (long)i * nz + j;
+            float dx_, dz_;
+            if (xin) {
+                dx_ = dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+                dz_ = dm<FAST>(z2[v], z1[v], z3[v], z0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+            } else {
+                dx_ = 0.0f;
+                dz_ = 0.0f;
+            }
+            if (j >= 2 && j < nz - 2) {
+                dx_ += dm<FAST>(e3[V + v], e3[V + v - 1], e3[V + v + 1], e3[V + v - 2], g.c8dz,
+                                g.c24dz, g.a1z, g.a2z);
+                dz_ += dm<FAST>(e2[V + v], e2[V + v - 1], e2[V + v + 1], e2[V + v - 2], g.c8dz,
+                                g.c24dz, g.a1z, g.a2z);
+            }
+            for (int pp = p0; pp < p1; pp++) {
+                const int cp = a.inj_cell[pp];
+                if (cp < rlo || cp > rhi) continue;
+                if (cp == kc) {
+                    const long ks = (long)b * a.inj_slot_stride + (long)a.inj_row[pp] * a.nt + it;
+                    dx_ += I1[ks];
+                    dz_ += I2[ks];
+                }
+            }
+            const float ab = (abx * abz[v]) * abt[v];
+            n1[v] = rin ? vel_update<FAST>(ab, v1[v], g.dt, dx_, rho[v]) : 0.0f;
+            n2[v] = rin ? vel_update<FAST>(ab, v2[v], g.dt, dz_, rho[v]) : 0.0f;
+        }
+    }
+}
+
+// Stress of step it+1 (divV, updateS) on the lane's V columns jb .. jb+V-1
+// of grid row i, from the velocity of step it+1 in LDS rows of W columns
+// (vx, vz for P-SV; vy for SH), grid row i at LDS row q, rows q-1 .. q+2
+// read, and the stress of step it at the lane's cells (sa, sc, sxz: sxx,
+// szz, sxz or sxy, szy).  Explosive sources are the caller's.
+template <bool PSV, int V, int W, bool FAST>
+__device__ __forceinline__ void stress_row(const Geo &g, const float *lv1, const float *lv2, int q,
+                                           int c0, int i, int jb, const float (&sa)[V],
+                                           const float (&sc)[V], const float (&sxz)[V],
+                                           const float (&mu)[V], const float (&lam)[V],
+                                           float (&o1)[V], float (&o2)[V], float (&o3)[V]) {
+    const int nx = g.nx, nz = g.nz;
+    const bool xin = i >= 1 && i < nx - 2;
+    float y0[V], y1[V], y2[V], y3[V], e1[3 * V];
+    vload<V>(&lv1[(q - 1) * W + c0], y0);
+    vload<V>(&lv1[q * W + c0], y1);
+    vload<V>(&lv1[(q + 1) * W + c0], y2);
+    vload<V>(&lv1[(q + 2) * W + c0], y3);
+    window<V, W>(&lv1[q * W], c0, e1);
+    if (!PSV) {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const int j = jb + v;
+            const float dx_ =
+                xin ? dp<FAST>(y2[v], y1[v], y3[v], y0[v], g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
+            const float dz_ = (j >= 1 && j < nz - 2)
+                                  ? dp<FAST>(e1[V + v + 1], e1[V + v], e1[V + v + 2],
+                                             e1[V + v - 1], g.c8dz, g.c24dz, g.a1z, g.a2z)
+                                  : 0.0f;
+            o1[v] = sa[v] + g.dt * mu[v] * dx_;
+            o2[v] = sc[v] + g.dt * mu[v] * dz_;
+        }
+    } else {
+        float w0[V], w1[V], w2[V], w3[V], e2[3 * V];
+        vload<V>(&lv2[(q - 1) * W + c0], w0);
+        vload<V>(&lv2[q * W + c0], w1);
+        vload<V>(&lv2[(q + 1) * W + c0], w2);
+        vload<V>(&lv2[(q + 2) * W + c0], w3);
+        window<V, W>(&lv2[q * W], c0, e2);
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const int j = jb + v;
+            const bool zin = j >= 1 && j < nz - 2;
+            float xx, zx, xz, zz;
+            if (xin) {
+                xx = dp<FAST>(y2[v], y1[v], y3[v], y0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+                zx = dp<FAST>(w2[v], w1[v], w3[v], w0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+            } else {
+                xx = 0.0f;
+                zx = 0.0f;
+            }
+            if (zin) {
+                xz = dp<FAST>(e1[V + v + 1], e1[V + v], e1[V + v + 2], e1[V + v - 1], g.c8dz,
+                              g.c24dz, g.a1z, g.a2z);
+                zz = dp<FAST>(e2[V + v + 1], e2[V + v], e2[V + v + 2], e2[V + v - 1], g.c8dz,
+                              g.c24dz, g.a1z, g.a2z);
+            } else {
+                xz = 0.0f;
+                zz = 0.0f;
+            }
+            const float m = mu[v], l = lam[v];
+            o1[v] = sa[v] + g.dt * ((l + 2 * m) * xx + l * zz);
+            o2[v] = sc[v] + g.dt * ((l + 2 * m) * zz + l * xx);
+            o3[v] = sxz[v] + g.dt * (m * (xz + zx));
+        }
+    }
+}
+
+// Snapshot scales of one step's roles (16-bit formats): su1, su2 for u and
+// sv1, sv2 for v (1 in FP32 or without the role).
+struct SnapScales {
+    float su1 = 1.0f, su2 = 1.0f, sv1 = 1.0f, sv2 = 1.0f;
+};
+template <bool PSV>
+__device__ __forceinline__ SnapScales snap_scales(const StepRoles &ro, int fmt, int mb, int fu,
+                                                  int fv) {
+    SnapScales s;
+    if (fmt != INV_SNAP_F32) {
+        if (ro.snap_u) {
+            s.su1 = snap_enc_scale(ro.mx_u[mb + fu]);
+            if (PSV) s.su2 = snap_enc_scale(ro.mx_u[mb + fu + 1]);
+        }
+        if (ro.snap_v) {
+            s.sv1 = snap_enc_scale(ro.mx_v[mb + fv]);
+            if (PSV) s.sv2 = snap_enc_scale(ro.mx_v[mb + fv + 1]);
+        }
+    }
+    return s;
+}
+
+// The owned cells of grid row i after the velocity update of step it, in
+// the order of k_colv: u(it) into its snapshot, u(it+1) = u(it) + v(it+1) dt,
+// v into its snapshot (v(it+1) with v_after, else v(it)), max-abs tracking
+// of u(it+1) and v(it+1), receivers of step it.  u1/u2 hold u(it) on entry and
+// u(it+1) on return; w1/w2: v(it); n1/n2: v(it+1).
+template <bool PSV, int V>
+__device__ __forceinline__ void owned_row(const StepArgs &a, const StepRoles &ro,
+                                          const SnapScales &ss, int fu, int fv, long sb, int i,
+                                          int jb, long it, int b, int r0, int r1, long rlo,
+                                          long rhi, float (&u1)[V], float (&u2)[V],
+                                          const float (&w1)[V], const float (&w2)[V],
+                                          const float (&n1)[V], const float (&n2)[V],
+                                          float &mu1, float &mu2, float &mv1, float &mv2) {
+    const Geo &g = a.g;
+    const int fmt = a.snap_fmt;
+    const bool track = ro.acc_u || ro.acc_v;
+    float *O1 = PSV ? a.out_x : a.out_y, *O2 = a.out_z;
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        const long kc = (long)i * g.nz + jb + v;
+        float u = u1[v];
+        float uu = PSV ? u2[v] : 0.0f;
+        if (ro.snap_u) {
+            snap_put(ro.snap_u, fmt, sb + (long)fu * g.N + kc, u, ss.su1);
+            if (PSV) snap_put(ro.snap_u, fmt, sb + (long)(fu + 1) * g.N + kc, uu, ss.su2);
+        }
+        u += n1[v] * g.dt;
+        if (PSV) uu += n2[v] * g.dt;
+        u1[v] = u;
+        if (PSV) u2[v] = uu;
+        if (ro.snap_v) {
+            snap_put(ro.snap_v, fmt, sb + (long)fv * g.N + kc, ro.v_after ? n1[v] : w1[v], ss.sv1);
+            if (PSV)
+                snap_put(ro.snap_v, fmt, sb + (long)(fv + 1) * g.N + kc,
+                         ro.v_after ? n2[v] : w2[v], ss.sv2);
+        }
+        if (track) {
+            mu1 = fmaxf(mu1, fabsf(u));
+            mv1 = fmaxf(mv1, fabsf(n1[v]));
+            if (PSV) {
+                mu2 = fmaxf(mu2, fabsf(uu));
+                mv2 = fmaxf(mv2, fabsf(n2[v]));
+            }
+        }
+        for (int pp = r0; pp < r1; pp++) {
+            const int cp = a.rec_cell[pp];
+            if (cp < rlo || cp > rhi) continue;
+            if (cp == kc) {
+                const long kr = (long)b * a.out_slot_stride + (long)a.rec_row[pp] * a.nt + it;
+                O1[kr] = a.rec_disp ? u : n1[v];
+                if (PSV) O2[kr] = a.rec_disp ? uu : n2[v];
+            }
+        }
+    }
+}
+
+// Max-abs of the wavefront's u and v into one step's tables, one atomic
+// per wavefront and field; every lane of the wavefront calls it.
+template <bool PSV>
+__device__ __forceinline__ void track_flush(const StepRoles &ro, int lane, int mb, int fu, int fv,
+                                            float mu1, float mu2, float mv1, float mv2) {
+    if (!(ro.acc_u || ro.acc_v)) return;
+    mu1 = wave_max(mu1);
+    mv1 = wave_max(mv1);
+    if (PSV) {
+        mu2 = wave_max(mu2);
+        mv2 = wave_max(mv2);
+    }
+    if (lane == 0) {
+        if (ro.acc_u) {
+            atomicMax(ro.acc_u + mb + fu, __float_as_uint(mu1));
+            if (PSV) atomicMax(ro.acc_u + mb + fu + 1, __float_as_uint(mu2));
+        }
+        if (ro.acc_v) {
+            atomicMax(ro.acc_v + mb + fv, __float_as_uint(mv1));
+            if (PSV) atomicMax(ro.acc_v + mb + fv + 1, __float_as_uint(mv2));
+        }
+    }
+}
+
+// Explosive sources of step it (source type 1) among list entries q0 .. q1
+// at the lane's cells of grid row i: their wavelet is added to sxx and szz.
+template <int V>
+__device__ __forceinline__ void explode_row(const StepArgs &a, int q0, int q1, int i, int jb,
+                                            int b, long it, float (&o1)[V], float (&o2)[V]) {
+    for (int pp = q0; pp < q1; pp++) {
+        const long cp = a.sinj_cell[pp];
+        const long d = cp - ((long)i * a.g.nz + jb);
+        if (d < 0 || d >= V) continue;
+#pragma unroll
+        for (int v = 0; v < V; v++)
+            if (d == v) {
+                const float f =
+                    a.inj_y[(long)b * a.inj_slot_stride + (long)a.sinj_row[pp] * a.nt + it];
+                o1[v] += f;
+                o2[v] += f;
+            }
+    }
+}
+
+}  // namespace inv

@@ -638,10 +638,11 @@ static int list_key(const Ctx &c) { return tile_rows(c) * 1000 + fz_of(c); }
 // CSR of points by tile: a point (i, j) belongs to tile (tx, tz) when
 // tx*TX - lo <= i < (tx+1)*TX + hi and tz*W - lo <= j < (tz+1)*W + hi, W the
 // tile width along z.
+// Point lists bucketed by tile (TX rows x FZ columns): a point goes to every
+// tile whose rows and columns extended by lo before and hi after contain it.
 static void build_tiled(Ctx &c, PointList &pl, int nsets, const std::vector<std::vector<int>> &cells,
-                        const std::vector<std::vector<int>> &rows, int TX, int lo, int hi) {
-    const int ntx = tiles_x_of(c, TX), ntz = tiles_z_of(c);
-    const int FZ = fz_of(c);
+                        const std::vector<std::vector<int>> &rows, int TX, int FZ, int lo, int hi) {
+    const int ntx = tiles_x_of(c, TX), ntz = (c.g.nz + FZ - 1) / FZ;
     const int ntiles = ntx * ntz;
     auto tiles_of = [&](int cell, std::vector<int> &out) {
         out.clear();
@@ -695,6 +696,7 @@ void fused_invalidate_lists(Ctx &c) {
     for (auto &kv : c.src_slists) release_list(kv.second);
     c.src_slists.clear();
     c.f_rec_tx = -1;
+    c.p_rec_key = -1;
 }
 
 
@@ -705,43 +707,95 @@ static void ensure_rec_lists(Ctx &c) {
         cells[0].push_back(c.rec_ix[r] * c.g.nz + c.rec_iz[r]);
         rows[0].push_back(r);
     }
-    build_tiled(c, c.f_rec_v, 1, cells, rows, tile_rows(c), 1, 2);
-    build_tiled(c, c.f_rec_t, 1, cells, rows, tile_rows(c), 0, 0);
+    build_tiled(c, c.f_rec_v, 1, cells, rows, tile_rows(c), fz_of(c), 1, 2);
+    build_tiled(c, c.f_rec_t, 1, cells, rows, tile_rows(c), fz_of(c), 0, 0);
     c.f_rec_tx = list_key(c);
 }
 
-PointList &source_list(Ctx &c, int first, int count, int key) {
-    const long long k = ((long long)(first + 2) << 40) | ((long long)count << 20) | key;
-    auto found = c.src_lists.find(k);
-    if (found != c.src_lists.end()) return found->second;
-    PointList &pl = c.src_lists[k];
-    std::vector<std::vector<int>> cells(count), rows(count);
+// Point forces (type 0) and explosions (type 1) of shots first .. first+count-1
+// in batch slots 0 .. count-1 (first < 0: every source in slot 0).
+static void source_points(const Ctx &c, int first, int count, int type,
+                          std::vector<std::vector<int>> &cells, std::vector<std::vector<int>> &rows) {
+    cells.assign(count, {});
+    rows.assign(count, {});
     for (int b = 0; b < count; b++)
-        for (int is = 0; is < c.nsrc; is++)
-            if ((first < 0 || is == first + b) && !(is < (int)c.src_type.size() && c.src_type[is] == 1)) {
+        for (int is = 0; is < c.nsrc; is++) {
+            const int ty = is < (int)c.src_type.size() ? c.src_type[is] : 0;
+            if ((first < 0 || is == first + b) && ty == type) {
                 cells[b].push_back(c.src_ix[is] * c.g.nz + c.src_iz[is]);
                 rows[b].push_back(is);
             }
+        }
+}
+
+static long long list_id(int first, int count, int key) {
+    return ((long long)(first + 2) << 40) | ((long long)count << 20) | key;
+}
+
+PointList &source_list(Ctx &c, int first, int count, int key) {
+    const long long k = list_id(first, count, key);
+    auto found = c.src_lists.find(k);
+    if (found != c.src_lists.end()) return found->second;
+    PointList &pl = c.src_lists[k];
+    std::vector<std::vector<int>> cells, rows;
+    source_points(c, first, count, 0, cells, rows);
     if (key == 0) build_point_list(c, pl, count, cells, rows);
-    else build_tiled(c, pl, count, cells, rows, tile_rows(c), 1, 2);
+    else build_tiled(c, pl, count, cells, rows, tile_rows(c), fz_of(c), 1, 2);
     return pl;
 }
 
 PointList &stress_source_list(Ctx &c, int first, int count, int key) {
-    const long long k = ((long long)(first + 2) << 40) | ((long long)count << 20) | key;
+    const long long k = list_id(first, count, key);
     auto found = c.src_slists.find(k);
     if (found != c.src_slists.end()) return found->second;
     PointList &pl = c.src_slists[k];
-    std::vector<std::vector<int>> cells(count), rows(count);
-    for (int b = 0; b < count; b++)
-        for (int is = 0; is < c.nsrc; is++)
-            if ((first < 0 || is == first + b) && is < (int)c.src_type.size() &&
-                c.src_type[is] == 1) {
-                cells[b].push_back(c.src_ix[is] * c.g.nz + c.src_iz[is]);
-                rows[b].push_back(is);
-            }
-    build_tiled(c, pl, count, cells, rows, tile_rows(c), 0, 0);
+    std::vector<std::vector<int>> cells, rows;
+    source_points(c, first, count, 1, cells, rows);
+    build_tiled(c, pl, count, cells, rows, tile_rows(c), fz_of(c), 0, 0);
     return pl;
+}
+
+// Two-step kernel geometry: pair_tx rows x pair_strip(pair_v) columns.  Its
+// lists cover the rows and columns each launch updates: forces and adjoint
+// sources (velocity of the first step, rows and columns -4 .. +4 inclusive
+// around the tile: lo 4, hi 5 past its end), explosions (stress of the first
+// step, -3 .. +2: lo 3, hi 3), receivers (owned cells: 0, 0).
+static bool pair_on(const Ctx &c) { return c.pair_tx > 0 && c.variant == INV_VARIANT_FUSED; }
+static int pair_fz(const Ctx &c) { return pair_strip(c.pair_v); }
+static int pair_key(const Ctx &c) { return 500000 + c.pair_tx * 1000 + pair_fz(c); }
+
+static PointList &pair_source_list(Ctx &c, int first, int count) {
+    const long long k = list_id(first, count, pair_key(c));
+    auto found = c.src_lists.find(k);
+    if (found != c.src_lists.end()) return found->second;
+    PointList &pl = c.src_lists[k];
+    std::vector<std::vector<int>> cells, rows;
+    source_points(c, first, count, 0, cells, rows);
+    build_tiled(c, pl, count, cells, rows, c.pair_tx, pair_fz(c), 4, 5);
+    return pl;
+}
+
+static PointList &pair_stress_source_list(Ctx &c, int first, int count) {
+    const long long k = list_id(first, count, pair_key(c));
+    auto found = c.src_slists.find(k);
+    if (found != c.src_slists.end()) return found->second;
+    PointList &pl = c.src_slists[k];
+    std::vector<std::vector<int>> cells, rows;
+    source_points(c, first, count, 1, cells, rows);
+    build_tiled(c, pl, count, cells, rows, c.pair_tx, pair_fz(c), 3, 3);
+    return pl;
+}
+
+static void ensure_pair_rec_lists(Ctx &c) {
+    if (c.p_rec_key == pair_key(c)) return;
+    std::vector<std::vector<int>> cells(1), rows(1);
+    for (int r = 0; r < c.nrec; r++) {
+        cells[0].push_back(c.rec_ix[r] * c.g.nz + c.rec_iz[r]);
+        rows[0].push_back(r);
+    }
+    build_tiled(c, c.p_rec_v, 1, cells, rows, c.pair_tx, pair_fz(c), 4, 5);
+    build_tiled(c, c.p_rec_t, 1, cells, rows, c.pair_tx, pair_fz(c), 0, 0);
+    c.p_rec_key = pair_key(c);
 }
 
 static void ensure_second_set(Ctx &c) {
@@ -974,6 +1028,13 @@ void fused_prepare(Ctx &c, int kind, int first, int count, int flags) {
         source_list(c, first, count, list_key(c));
         if (c.any_explosive()) stress_source_list(c, first, count, list_key(c));
     }
+    if (pair_on(c)) {
+        ensure_pair_rec_lists(c);
+        if (kind == 1 || (c.ckpt > 0 && c.g.adj)) {
+            pair_source_list(c, first, count);
+            if (c.any_explosive()) pair_stress_source_list(c, first, count);
+        }
+    }
 }
 
 template <bool SH, bool PSV, bool FAST>
@@ -1021,6 +1082,43 @@ static void fused_step(Ctx &c, StepArgs &a, const Fields &F, int &par, int count
     INV_HIP(hipGetLastError());
     if (p) prof_end(c, PROF_STEP, ev);
     par ^= 1;
+}
+
+template <bool FAST>
+static void pair_m(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned blocks) {
+    // SH and P-SV are independent systems: one launch each
+    if (c.g.sh) pair_launch<true, false, FAST>(c, a, r2, blocks);
+    if (c.g.psv) pair_launch<false, true, FAST>(c, a, r2, blocks);
+}
+
+// Steps it and it+1 in one launch of the two-step kernel (roles of step it
+// in a, of step it+1 in r2).
+static void pair_step(Ctx &c, StepArgs &a, const StepRoles &r2, const Fields &F, int &par,
+                      int count, int it) {
+    set_state_ptrs(a, F, par);
+    a.it = it;
+    const unsigned blocks = (unsigned)(count * a.tiles_x * a.tiles_z);
+    hipEvent_t ev;
+    const bool p = prof_begin(c, it, &ev);
+    if (c.math == INV_MATH_FAST) pair_m<true>(c, a, r2, blocks);
+    else pair_m<false>(c, a, r2, blocks);
+    INV_HIP(hipGetLastError());
+    if (p) prof_end(c, PROF_PAIR, ev);
+    par ^= 1;
+}
+
+static StepRoles roles_of(const StepArgs &a) {
+    return StepRoles{a.snap_u, a.snap_v, a.v_after, a.mx_u, a.mx_v, a.acc_u, a.acc_v};
+}
+
+static void set_roles(StepArgs &a, const StepRoles &r) {
+    a.snap_u = r.snap_u;
+    a.snap_v = r.snap_v;
+    a.v_after = r.v_after;
+    a.mx_u = r.mx_u;
+    a.mx_v = r.mx_v;
+    a.acc_u = r.acc_u;
+    a.acc_v = r.acc_v;
 }
 
 static void common_args(Ctx &c, StepArgs &a) {
@@ -1080,6 +1178,37 @@ static void forward_args(Ctx &c, StepArgs &a, int first, int count, bool record)
     a.rec_disp = (c.g.adj || c.g.trace_type == 1) ? 1 : 0;
 }
 
+// Arguments of the two-step kernel: those of the one-step kernel with the
+// pair geometry's tiles and point lists (built by fused_prepare).
+static StepArgs pair_args(Ctx &c, const StepArgs &a, int first, int count, bool adjoint,
+                          bool record) {
+    StepArgs p = a;
+    p.tiles_x = tiles_x_of(c, c.pair_tx);
+    p.tiles_z = (c.g.nz + pair_fz(c) - 1) / pair_fz(c);
+    ensure_pair_rec_lists(c);
+    if (adjoint) {
+        p.inj_ptr = c.nrec ? c.p_rec_v.ptr.as<int>() : nullptr;
+        p.inj_cell = c.p_rec_v.cell.as<int>();
+        p.inj_row = c.p_rec_v.row.as<int>();
+        p.rec_ptr = nullptr;
+        return p;
+    }
+    PointList &sl = pair_source_list(c, first, count);
+    p.inj_ptr = sl.ptr.as<int>();
+    p.inj_cell = sl.cell.as<int>();
+    p.inj_row = sl.row.as<int>();
+    if (a.sinj_ptr) {
+        PointList &xl = pair_stress_source_list(c, first, count);
+        p.sinj_ptr = xl.ptr.as<int>();
+        p.sinj_cell = xl.cell.as<int>();
+        p.sinj_row = xl.row.as<int>();
+    }
+    p.rec_ptr = (record && c.nrec) ? c.p_rec_t.ptr.as<int>() : nullptr;
+    p.rec_cell = c.p_rec_t.cell.as<int>();
+    p.rec_row = c.p_rec_t.row.as<int>();
+    return p;
+}
+
 void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
     fused_prepare(c, 1, first, count, flags);
     StepArgs a;
@@ -1095,7 +1224,13 @@ void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
         if (sp.on && sp.fmt != INV_SNAP_F32)
             fill_dev(c, c.smax.p, (long)c.nslots * c.g.batch * c.nsnapf);
     }
-    for (int it = it0; it < it1; it++) {
+    // two steps per launch where the steps allow it: no checkpoint before the
+    // second step, no max-abs collected by the first step (the second would
+    // read it)
+    const bool pr = pair_on(c);
+    StepArgs a2;
+    if (pr) a2 = pair_args(c, a, first, count, false, true);
+    for (int it = it0; it < it1;) {
         if (sp.on && c.ckpt > 0 && it > 0 && it % c.ckpt == 0) {
             float *cur[11];
             const int nf = state_list(c, F, c.parity, cur);
@@ -1103,7 +1238,17 @@ void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
                 vec_copy(c, ckpt_field(c, it / c.ckpt, f), cur[f], (size_t)count * c.N);
         }
         snap_roles(sp, a, it, 0);
-        fused_step(c, a, F, c.parity, count, it);
+        if (pr && it + 1 < it1 && !a.acc_u && !a.acc_v &&
+            !(sp.on && c.ckpt > 0 && (it + 1) % c.ckpt == 0)) {
+            StepArgs n = a;
+            snap_roles(sp, n, it + 1, 0);
+            set_roles(a2, roles_of(a));
+            pair_step(c, a2, roles_of(n), F, c.parity, count, it);
+            it += 2;
+        } else {
+            fused_step(c, a, F, c.parity, count, it);
+            it += 1;
+        }
     }
     if (sp.on && it1 > it0) snap_flush(c, sp, F, c.parity, count, it1 - 1);
 }
@@ -1141,9 +1286,21 @@ static void replay_segment(Ctx &c, int first, int count, int fseg, int aseg) {
     StepArgs a;
     common_args(c, a);
     forward_args(c, a, first, count, false);
-    for (long it = f0; it < f0 + K; it++) {
+    const bool pr = pair_on(c);
+    StepArgs a2;
+    if (pr) a2 = pair_args(c, a, first, count, false, false);
+    for (long it = f0; it < f0 + K;) {
         snap_roles(sp, a, it, f0);
-        fused_step(c, a, R, par, count, (int)it);
+        if (pr && it + 1 < f0 + K && !a.acc_u && !a.acc_v) {
+            StepArgs n = a;
+            snap_roles(sp, n, it + 1, f0);
+            set_roles(a2, roles_of(a));
+            pair_step(c, a2, roles_of(n), R, par, count, (int)it);
+            it += 2;
+        } else {
+            fused_step(c, a, R, par, count, (int)it);
+            it += 1;
+        }
     }
     snap_flush(c, sp, R, par, count, f0 + K - 1);
 }
@@ -1173,14 +1330,25 @@ void fused_adjoint(Ctx &c, int first, int count, int it0, int it1) {
         fdm_prepare_model(c);
     }
     const int K = c.ckpt, S = K > 0 ? c.g.nt / K : 1, m = K > 0 ? K / c.g.sfe : 0;
-    for (int it = it0; it < it1; it++) {
-        if (K > 0 && it > 0 && it % K == 0) replay_segment(c, first, count, S - 1 - it / K, it / K);
-        fused_step(c, a, F, c.parity, count, it);
-        if (it % c.g.sfe == 0) {
-            const int slot = it / c.g.sfe - (K > 0 ? (it / K) * m : 0);
-            fdm_interact(c, count, it, slot, cur_field(c, c.vx, c.vx_b),
+    // two steps per launch where no kernel accumulation follows the first;
+    // a segment recomputation due before the second step runs before both
+    // (the interactions that read the old segment are done by then)
+    const bool pr = pair_on(c) && c.g.sfe > 1;
+    StepArgs a2;
+    if (pr) a2 = pair_args(c, a, first, count, true, false);
+    for (int it = it0; it < it1;) {
+        const bool two = pr && it + 1 < it1 && it % c.g.sfe != 0;
+        const int last = two ? it + 1 : it;
+        if (K > 0 && last > 0 && last % K == 0)
+            replay_segment(c, first, count, S - 1 - last / K, last / K);
+        if (two) pair_step(c, a2, StepRoles{}, F, c.parity, count, it);
+        else fused_step(c, a, F, c.parity, count, it);
+        if (last % c.g.sfe == 0) {
+            const int slot = last / c.g.sfe - (K > 0 ? (last / K) * m : 0);
+            fdm_interact(c, count, last, slot, cur_field(c, c.vx, c.vx_b),
                          cur_field(c, c.vz, c.vz_b), cur_field(c, c.vy, c.vy_b));
         }
+        it = last + 1;
     }
     if (!direct && it1 == c.g.nt) fdm_kslot_reduce(c, count);
 }

@@ -1,0 +1,316 @@
+// Two time steps per launch.  The fused step of k_colv (divS -> addSTF ->
+// updateV -> divV -> updateS -> saveRec, solver/fdm.h:534-551) is applied
+// twice to a tile held in LDS, so every wavefield and model field crosses HBM
+// once per two steps: 68 B per two P-SV cell-updates instead of per one (40 B
+// for SH).  The price is recomputing a deeper halo: the stress of step n is
+// read on rows i0-6 .. i0+TX+5 and columns j0-6 .. j0+S+5, the velocity of
+// step n+1 is computed on rows i0-4 .. i0+TX+4, the stress of step n+1 on
+// rows i0-3 .. i0+TX+2, the velocity of step n+2 on rows i0-1 .. i0+TX+1 and
+// the stress of step n+2 on the owned rows (and columns likewise; edge
+// columns of a phase hold values no later owned cell depends on).
+//
+// Row x = i - i0 of the tile is handled by wavefront x mod 4 in every phase,
+// so the registers of a row (velocity, u, rho, lambda, mu) stay with one
+// wavefront from the loads to the last store.  The stress buffer in LDS holds
+// step n and is overwritten in place by step n+1 (each cell's update reads
+// its own old value and only velocity neighbours); the velocity buffer holds
+// step n+1 and is overwritten by step n+2 likewise.  Four barriers.
+//
+// Every cell is evaluated by the row functions of colv.h, shared with
+// k_colv, so two launches of this kernel give bitwise the results of four
+// launches of k_colv (and of k_col and the FP32 oracle in exact math).
+// Point lists are bucketed for this geometry: forces with a halo of 4 rows
+// and columns, explosions 3, receivers 0.  Snapshot roles of the first step
+// come in StepArgs, of the second in StepRoles; a role whose max-abs table is
+// filled by the first step must not be read by the second (the host never
+// pairs such steps).  Needs nz % V == 0 and dynamic LDS of pair_lds_bytes.
+#include "colv.h"
+
+namespace inv {
+
+constexpr int PBLOCK = 256;  // four wavefronts per tile
+
+template <bool SH, bool PSV, int TX, int V, bool FAST>
+__global__ __launch_bounds__(PBLOCK) void k_pair(const StepArgs a, const StepRoles r2) {
+    static_assert(SH != PSV, "one wave type per launch");
+    static_assert(V == 2 || V == 4, "2 or 4 columns per lane");
+    static_assert(TX % 4 == 0, "tile rows a multiple of the wavefronts");
+    constexpr int HL = pair_halo(V), W = 64 * V, S = pair_strip(V), NW = PBLOCK / 64;
+    constexpr int SR = TX + 12;                 // stress rows x = -6 .. TX+5
+    constexpr int VR = TX + 9;                  // velocity rows x = -4 .. TX+4
+    constexpr int PS = (SR + NW - 1) / NW;      // stress rows loaded per wavefront
+    constexpr int P = (VR + NW - 1) / NW;       // rows x = wave - 4 + 4 p per wavefront
+    extern __shared__ __align__(16) float smem[];
+    float *const l_s1 = smem;
+    float *const l_s2 = l_s1 + SR * W;
+    float *const l_s3 = PSV ? l_s2 + SR * W : l_s2;  // unused for SH
+    float *const l_v1 = (PSV ? l_s3 : l_s2) + SR * W;
+    float *const l_v2 = PSV ? l_v1 + VR * W : l_v1;  // unused for SH
+
+    // XCD-aware tile order (as k_col)
+    const int nw = gridDim.x, w = blockIdx.x;
+    const int q = nw >> 3, rmd = nw & 7, xcd = w & 7;
+    const int L = xcd * q + (xcd < rmd ? xcd : rmd) + (w >> 3);
+    const int per = a.tiles_x * a.tiles_z;
+    const int b = L / per;
+    const int t = L - b * per;
+    const int tx = t / a.tiles_z, tz = t - tx * a.tiles_z;
+    const int i0 = tx * TX, j0 = tz * S;
+    const Geo &g = a.g;
+    const int nx = g.nx, nz = g.nz;
+    const long so = (long)b * g.N;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c0 = lane * V;          // the lane's first LDS column
+    const int jb = j0 - HL + c0;      // its first grid column (a multiple of V)
+    const int jl = jb < 0 ? 0 : (jb > nz - V ? nz - V : jb);
+    const bool jin = jb >= 0 && jb < nz;  // all V columns in the grid, or none
+    const bool cown = jin && c0 >= HL && c0 < HL + S;
+    auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
+    const float *m_rho = a.rho, *m_lam = a.lam, *m_mu = a.mu;
+    if (a.slot_model) {
+        m_rho = a.slot_model[3 * b];
+        m_lam = a.slot_model[3 * b + 1];
+        m_mu = a.slot_model[3 * b + 2];
+    }
+    const float *S1 = PSV ? a.sxx0 : a.sxy0, *S2 = PSV ? a.szz0 : a.szy0, *S3 = a.sxz0;
+    const float *V1 = PSV ? a.vx0 : a.vy0, *V2 = a.vz0;
+    float *U1 = PSV ? a.ux : a.uy, *U2 = a.uz;
+    float *V1o = PSV ? a.vx1 : a.vy1, *V2o = a.vz1;
+    float *S1o = PSV ? a.sxx1 : a.sxy1, *S2o = PSV ? a.szz1 : a.szy1, *S3o = a.sxz1;
+
+    // all global loads first, at clamped addresses
+    float s1[PS][V], s2[PS][V], s3[PS][V];
+#pragma unroll
+    for (int p = 0; p < PS; p++) {
+        const int r = wave + p * NW;
+        const long k = so + (long)clampi(i0 - 6 + r, 0, nx - 1) * nz + jl;
+        vload<V>(S1 + k, s1[p]);
+        vload<V>(S2 + k, s2[p]);
+        if (PSV) vload<V>(S3 + k, s3[p]);
+    }
+    float qv1[P][V], qv2[P][V], qr[P][V], qu1[P][V], qu2[P][V], qmu[P][V], qlam[P][V];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int x = wave - 4 + p * NW;
+        const long kc = (long)clampi(i0 + x, 0, nx - 1) * nz + jl;
+        if (x < TX + 5) {
+            vload<V>(m_rho + kc, qr[p]);
+            vload<V>(V1 + so + kc, qv1[p]);
+            if (PSV) vload<V>(V2 + so + kc, qv2[p]);
+        }
+        if (x >= 0 && x < TX) {  // u only on owned rows (wave-uniform)
+            vload<V>(U1 + so + kc, qu1[p]);
+            if (PSV) vload<V>(U2 + so + kc, qu2[p]);
+        }
+        if (x >= -3 && x < TX + 3) {  // rows of the two stress updates
+            vload<V>(m_mu + kc, qmu[p]);
+            if (PSV) vload<V>(m_lam + kc, qlam[p]);
+        }
+    }
+    float abz[V], abt[V];
+    vload<V>(a.abzb + jl, abz);
+    vload<V>(a.abzt + jl, abt);
+
+    // stress of step n on rows x = -6 .. TX+5 (LDS row x + 6), 0 outside the grid
+#pragma unroll
+    for (int p = 0; p < PS; p++) {
+        const int r = wave + p * NW;
+        if (r < SR) {
+            const int i = i0 - 6 + r;
+            if (!(jin && i >= 0 && i < nx)) {
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    s1[p][v] = 0.0f;
+                    s2[p][v] = 0.0f;
+                    s3[p][v] = 0.0f;
+                }
+            }
+            vstore<V>(&l_s1[r * W + c0], s1[p]);
+            vstore<V>(&l_s2[r * W + c0], s2[p]);
+            if (PSV) vstore<V>(&l_s3[r * W + c0], s3[p]);
+        }
+    }
+    __syncthreads();
+
+    int p0 = 0, p1 = 0;
+    if (a.inj_ptr) {
+        const int set = a.inj_per_slot ? b : 0;
+        p0 = a.inj_ptr[set * per + t];
+        p1 = a.inj_ptr[set * per + t + 1];
+    }
+    int r0 = 0, r1 = 0;
+    if (a.rec_ptr) {
+        r0 = a.rec_ptr[t];
+        r1 = a.rec_ptr[t + 1];
+    }
+    int q0 = 0, q1 = 0;
+    if (PSV && a.sinj_ptr) {
+        q0 = a.sinj_ptr[b * per + t];
+        q1 = a.sinj_ptr[b * per + t + 1];
+    }
+    const long it = a.it;
+    const int fu = PSV ? a.psv_f0 : 0, fv = PSV ? a.psv_f0 + 2 : 1;
+    const long sb = (long)b * a.snap_slot_stride;
+    const int mb = b * a.nsnapf;
+    const StepRoles r1s{a.snap_u, a.snap_v, a.v_after, a.mx_u, a.mx_v, a.acc_u, a.acc_v};
+    float mu1 = 0.0f, mu2 = 0.0f, mv1 = 0.0f, mv2 = 0.0f;
+
+    // A. velocity of step n+1 on rows x = -4 .. TX+4 (LDS row x + 4)
+    {
+        const SnapScales ss = snap_scales<PSV>(r1s, a.snap_fmt, mb, fu, fv);
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const int x = wave - 4 + p * NW;
+            if (x >= TX + 5) break;
+            const int i = i0 + x;
+            const bool rin = jin && i >= 0 && i < nx;
+            const float abx = a.abx[clampi(i, 0, nx - 1)];
+            const long rlo = (long)i * nz + (j0 - HL), rhi = rlo + W - 1;
+            float n1[V], n2[V];
+            vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, c0, i, jb, rin, abx, abz, abt,
+                                     qv1[p], qv2[p], qr[p], p0, p1, rlo, rhi, b, it, n1, n2);
+            vstore<V>(&l_v1[(x + 4) * W + c0], n1);
+            if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], n2);
+            if (x >= 0 && x < TX && cown && i < nx)
+                owned_row<PSV, V>(a, r1s, ss, fu, fv, sb, i, jb, it, b, r0, r1, rlo, rhi, qu1[p],
+                                  qu2[p], qv1[p], qv2[p], n1, n2, mu1, mu2, mv1, mv2);
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                qv1[p][v] = n1[v];
+                if (PSV) qv2[p][v] = n2[v];
+            }
+        }
+        track_flush<PSV>(r1s, lane, mb, fu, fv, mu1, mu2, mv1, mv2);
+    }
+    __syncthreads();
+
+    // B. stress of step n+1 on rows x = -3 .. TX+2, in place (0 outside the grid)
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int x = wave - 4 + p * NW;
+        if (x < -3) continue;
+        if (x >= TX + 3) break;
+        const int i = i0 + x;
+        const bool rin = jin && i >= 0 && i < nx;
+        const int r = x + 6;
+        float sa[V], sc[V], sxz[V], o1[V], o2[V], o3[V];
+        vload<V>(&l_s1[r * W + c0], sa);
+        vload<V>(&l_s2[r * W + c0], sc);
+        if (PSV) vload<V>(&l_s3[r * W + c0], sxz);
+        stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, x + 4, c0, i, jb, sa, sc, sxz, qmu[p], qlam[p],
+                                    o1, o2, o3);
+        if (PSV) explode_row<V>(a, q0, q1, i, jb, b, it, o1, o2);
+        if (!rin) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                o1[v] = 0.0f;
+                o2[v] = 0.0f;
+                o3[v] = 0.0f;
+            }
+        }
+        vstore<V>(&l_s1[r * W + c0], o1);
+        vstore<V>(&l_s2[r * W + c0], o2);
+        if (PSV) vstore<V>(&l_s3[r * W + c0], o3);
+    }
+    __syncthreads();
+
+    // C. velocity of step n+2 on rows x = -1 .. TX+1, in place; owned cells
+    // store v and u
+    mu1 = mu2 = mv1 = mv2 = 0.0f;
+    {
+        const SnapScales ss = snap_scales<PSV>(r2, a.snap_fmt, mb, fu, fv);
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const int x = wave - 4 + p * NW;
+            if (x < -1) continue;
+            if (x >= TX + 2) break;
+            const int i = i0 + x;
+            const bool rin = jin && i >= 0 && i < nx;
+            const float abx = a.abx[clampi(i, 0, nx - 1)];
+            const long rlo = (long)i * nz + (j0 - HL), rhi = rlo + W - 1;
+            float n1[V], n2[V];
+            vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, c0, i, jb, rin, abx, abz, abt,
+                                     qv1[p], qv2[p], qr[p], p0, p1, rlo, rhi, b, it + 1, n1, n2);
+            vstore<V>(&l_v1[(x + 4) * W + c0], n1);
+            if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], n2);
+            if (x >= 0 && x < TX && cown && i < nx) {
+                owned_row<PSV, V>(a, r2, ss, fu, fv, sb, i, jb, it + 1, b, r0, r1, rlo, rhi,
+                                  qu1[p], qu2[p], qv1[p], qv2[p], n1, n2, mu1, mu2, mv1, mv2);
+                const long o = so + (long)i * nz + jb;
+                vstore<V>(V1o + o, n1);
+                vstore<V>(U1 + o, qu1[p]);
+                if (PSV) {
+                    vstore<V>(V2o + o, n2);
+                    vstore<V>(U2 + o, qu2[p]);
+                }
+            }
+        }
+        track_flush<PSV>(r2, lane, mb, fu, fv, mu1, mu2, mv1, mv2);
+    }
+    __syncthreads();
+
+    // D. stress of step n+2 on the owned cells
+    if (!cown) return;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int x = wave - 4 + p * NW;
+        if (x < 0) continue;
+        if (x >= TX) break;
+        const int i = i0 + x;
+        if (i >= nx) break;
+        const int r = x + 6;
+        const long o = so + (long)i * nz + jb;
+        float sa[V], sc[V], sxz[V], o1[V], o2[V], o3[V];
+        vload<V>(&l_s1[r * W + c0], sa);
+        vload<V>(&l_s2[r * W + c0], sc);
+        if (PSV) vload<V>(&l_s3[r * W + c0], sxz);
+        stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, x + 4, c0, i, jb, sa, sc, sxz, qmu[p], qlam[p],
+                                    o1, o2, o3);
+        if (PSV) explode_row<V>(a, q0, q1, i, jb, b, it + 1, o1, o2);
+        vstore<V>(S1o + o, o1);
+        vstore<V>(S2o + o, o2);
+        if (PSV) vstore<V>(S3o + o, o3);
+    }
+}
+
+template <class K>
+static void pair_go(Ctx &c, K kern, int lds, const StepArgs &a, const StepRoles &r2,
+                    unsigned blocks) {
+    if (lds > 65536)
+        INV_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    kern<<<blocks, PBLOCK, lds, c.stream>>>(a, r2);
+}
+
+template <bool SH, bool PSV, bool FAST>
+void pair_launch(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned blocks) {
+    const int rows = c.pair_tx, V = c.pair_v;
+    const int lds = pair_lds_bytes(PSV, rows, V);
+    if (lds > 65536) {
+        // a tile larger than the default per-block LDS needs the opt-in limit
+        int dev = 0, optin = 0;
+        INV_HIP(hipGetDevice(&dev));
+        INV_HIP(hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev));
+        INV_REQUIRE(lds <= optin, INV_ERR_UNSUPPORTED,
+                    "two-step tile needs more LDS per block than the device allows");
+    }
+    if (V == 2) {
+        switch (rows) {
+            case 4: pair_go(c, k_pair<SH, PSV, 4, 2, FAST>, lds, a, r2, blocks); break;
+            case 12: pair_go(c, k_pair<SH, PSV, 12, 2, FAST>, lds, a, r2, blocks); break;
+            default: pair_go(c, k_pair<SH, PSV, 8, 2, FAST>, lds, a, r2, blocks); break;
+        }
+    } else {
+        switch (rows) {
+            case 8: pair_go(c, k_pair<SH, PSV, 8, 4, FAST>, lds, a, r2, blocks); break;
+            default: pair_go(c, k_pair<SH, PSV, 4, 4, FAST>, lds, a, r2, blocks); break;
+        }
+    }
+}
+
+template void pair_launch<true, false, true>(Ctx &, const StepArgs &, const StepRoles &, unsigned);
+template void pair_launch<false, true, true>(Ctx &, const StepArgs &, const StepRoles &, unsigned);
+template void pair_launch<true, false, false>(Ctx &, const StepArgs &, const StepRoles &, unsigned);
+template void pair_launch<false, true, false>(Ctx &, const StepArgs &, const StepRoles &, unsigned);
+
+}  // namespace inv

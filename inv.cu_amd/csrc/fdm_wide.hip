@@ -14,52 +14,11 @@
 // update u, snapshots and receivers) and goes to LDS; the stress of step n+1
 // is computed on the owned cells.  Arithmetic is expression for expression
 // that of k_col, so results are bitwise identical to it.  Needs nz % V == 0.
-#include "internal.h"
-#include "step.h"
+#include "colv.h"
 
 namespace inv {
 
 constexpr int WBLOCK = 256;  // four wavefronts per tile
-
-template <int V>
-__device__ __forceinline__ void vload(const float *__restrict__ p, float (&o)[V]) {
-    if constexpr (V == 2) {
-        const float2 t = *reinterpret_cast<const float2 *>(p);
-        o[0] = t.x;
-        o[1] = t.y;
-    } else {
-        const float4 t = *reinterpret_cast<const float4 *>(p);
-        o[0] = t.x;
-        o[1] = t.y;
-        o[2] = t.z;
-        o[3] = t.w;
-    }
-}
-
-template <int V>
-__device__ __forceinline__ void vstore(float *__restrict__ p, const float (&o)[V]) {
-    if constexpr (V == 2) *reinterpret_cast<float2 *>(p) = make_float2(o[0], o[1]);
-    else *reinterpret_cast<float4 *>(p) = make_float4(o[0], o[1], o[2], o[3]);
-}
-
-// e[V + k] = row[c0 + k] for k in [-V, 2V): the lane's vector and its two
-// neighbours from an LDS row of W columns (indices clamped into the row; the
-// clamped values only reach columns no owned cell depends on)
-template <int V, int W>
-__device__ __forceinline__ void window(const float *row, int c0, float (&e)[3 * V]) {
-    const int cl = c0 - V < 0 ? 0 : c0 - V;
-    const int cr = c0 + V > W - V ? W - V : c0 + V;
-    float t[V];
-    vload<V>(row + cl, t);
-#pragma unroll
-    for (int v = 0; v < V; v++) e[v] = t[v];
-    vload<V>(row + c0, t);
-#pragma unroll
-    for (int v = 0; v < V; v++) e[V + v] = t[v];
-    vload<V>(row + cr, t);
-#pragma unroll
-    for (int v = 0; v < V; v++) e[2 * V + v] = t[v];
-}
 
 template <bool SH, bool PSV, int TX, int V, bool FAST>
 __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
@@ -101,7 +60,6 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
     float *U1 = PSV ? a.ux : a.uy, *U2 = a.uz;
     float *V1o = PSV ? a.vx1 : a.vy1, *V2o = a.vz1;
     float *S1o = PSV ? a.sxx1 : a.sxy1, *S2o = PSV ? a.szz1 : a.szy1, *S3o = a.sxz1;
-    const float *I1 = PSV ? a.inj_x : a.inj_y, *I2 = a.inj_z;
     float *O1 = PSV ? a.out_x : a.out_y, *O2 = a.out_z;
 
     // all global loads of the three phases first
@@ -197,82 +155,13 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         if (vr >= VR) break;
         const int i = i0 - 1 + vr;
         const bool rin = jin && i >= 0 && i < nx;
-        const bool xin = i >= 2 && i < nx - 2;
         const float abx = a.abx[clampi(i, 0, nx - 1)];
         // cells of this row the wavefront covers: point lists are scanned with
         // one wave-uniform test per point before the per-lane one
         const long rlo = (long)i * nz + (j0 - HL), rhi = rlo + W - 1;
         float n1[V], n2[V];
-        if (!PSV) {
-            float x0[V], x1[V], x2[V], x3[V], ez[3 * V];
-            vload<V>(&l_s1[vr * W + c0], x0);
-            vload<V>(&l_s1[(vr + 1) * W + c0], x1);
-            vload<V>(&l_s1[(vr + 2) * W + c0], x2);
-            vload<V>(&l_s1[(vr + 3) * W + c0], x3);
-            window<V, W>(&l_s2[(vr + 2) * W], c0, ez);
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                const int j = jb + v;
-                const long kc = (long)i * nz + j;
-                float d = xin ? dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, g.a1x, g.a2x)
-                              : 0.0f;
-                if (j >= 2 && j < nz - 2)
-                    d += dm<FAST>(ez[V + v], ez[V + v - 1], ez[V + v + 1], ez[V + v - 2], g.c8dz,
-                                  g.c24dz, g.a1z, g.a2z);
-                for (int pp = p0; pp < p1; pp++) {
-                    const int cp = a.inj_cell[pp];
-                    if (cp < rlo || cp > rhi) continue;
-                    if (cp == kc)
-                        d += I1[(long)b * a.inj_slot_stride + (long)a.inj_row[pp] * a.nt + it];
-                }
-                const float ab = (abx * abz[v]) * abt[v];
-                n1[v] = rin ? vel_update<FAST>(ab, qv1[p][v], g.dt, d, qr[p][v]) : 0.0f;
-            }
-        } else {
-            float x0[V], x1[V], x2[V], x3[V], z0[V], z1[V], z2[V], z3[V], e3[3 * V], e2[3 * V];
-            vload<V>(&l_s1[vr * W + c0], x0);
-            vload<V>(&l_s1[(vr + 1) * W + c0], x1);
-            vload<V>(&l_s1[(vr + 2) * W + c0], x2);
-            vload<V>(&l_s1[(vr + 3) * W + c0], x3);
-            vload<V>(&l_s3[vr * W + c0], z0);
-            vload<V>(&l_s3[(vr + 1) * W + c0], z1);
-            vload<V>(&l_s3[(vr + 2) * W + c0], z2);
-            vload<V>(&l_s3[(vr + 3) * W + c0], z3);
-            window<V, W>(&l_s3[(vr + 2) * W], c0, e3);
-            window<V, W>(&l_s2[(vr + 2) * W], c0, e2);
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                const int j = jb + v;
-                const long kc = (long)i * nz + j;
-                float dx_, dz_;
-                if (xin) {
-                    dx_ = dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
-                    dz_ = dm<FAST>(z2[v], z1[v], z3[v], z0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
-                } else {
-                    dx_ = 0.0f;
-                    dz_ = 0.0f;
-                }
-                if (j >= 2 && j < nz - 2) {
-                    dx_ += dm<FAST>(e3[V + v], e3[V + v - 1], e3[V + v + 1], e3[V + v - 2],
-                                    g.c8dz, g.c24dz, g.a1z, g.a2z);
-                    dz_ += dm<FAST>(e2[V + v], e2[V + v - 1], e2[V + v + 1], e2[V + v - 2],
-                                    g.c8dz, g.c24dz, g.a1z, g.a2z);
-                }
-                for (int pp = p0; pp < p1; pp++) {
-                    const int cp = a.inj_cell[pp];
-                    if (cp < rlo || cp > rhi) continue;
-                    if (cp == kc) {
-                        const long ks =
-                            (long)b * a.inj_slot_stride + (long)a.inj_row[pp] * a.nt + it;
-                        dx_ += I1[ks];
-                        dz_ += I2[ks];
-                    }
-                }
-                const float ab = (abx * abz[v]) * abt[v];
-                n1[v] = rin ? vel_update<FAST>(ab, qv1[p][v], g.dt, dx_, qr[p][v]) : 0.0f;
-                n2[v] = rin ? vel_update<FAST>(ab, qv2[p][v], g.dt, dz_, qr[p][v]) : 0.0f;
-            }
-        }
+        vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, vr + 2, c0, i, jb, rin, abx, abz, abt, qv1[p],
+                                 qv2[p], qr[p], p0, p1, rlo, rhi, b, it, n1, n2);
         vstore<V>(&l_v1[vr * W + c0], n1);
         if (PSV) vstore<V>(&l_v2[vr * W + c0], n2);
         // owned cells: v, u, snapshots, receivers
@@ -360,81 +249,18 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         if (tr >= TX) break;
         const int i = i0 + tr;
         if (i >= nx) break;
-        const bool xin = i >= 1 && i < nx - 2;
         const long o = so + (long)i * nz + jb;
-        float o1[V], o2[V], o3[V];
-        float y0[V], y1[V], y2[V], y3[V], e1[3 * V];
-        vload<V>(&l_v1[tr * W + c0], y0);
-        vload<V>(&l_v1[(tr + 1) * W + c0], y1);
-        vload<V>(&l_v1[(tr + 2) * W + c0], y2);
-        vload<V>(&l_v1[(tr + 3) * W + c0], y3);
-        window<V, W>(&l_v1[(tr + 1) * W], c0, e1);
-        float sa[V], sc[V];
+        float sa[V], sc[V], sxz[V], o1[V], o2[V], o3[V];
         vload<V>(&l_s1[(tr + 3) * W + c0], sa);
         vload<V>(&l_s2[(tr + 3) * W + c0], sc);
-        if (!PSV) {
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                const int j = jb + v;
-                const float mu = qmu[p][v];
-                const float dx_ =
-                    xin ? dp<FAST>(y2[v], y1[v], y3[v], y0[v], g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
-                const float dz_ = (j >= 1 && j < nz - 2)
-                                      ? dp<FAST>(e1[V + v + 1], e1[V + v], e1[V + v + 2],
-                                                 e1[V + v - 1], g.c8dz, g.c24dz, g.a1z, g.a2z)
-                                      : 0.0f;
-                o1[v] = sa[v] + g.dt * mu * dx_;
-                o2[v] = sc[v] + g.dt * mu * dz_;
-            }
-            vstore<V>(S1o + o, o1);
-            vstore<V>(S2o + o, o2);
-        } else {
-            float w0[V], w1[V], w2[V], w3[V], e2[3 * V], sxz[V];
-            vload<V>(&l_v2[tr * W + c0], w0);
-            vload<V>(&l_v2[(tr + 1) * W + c0], w1);
-            vload<V>(&l_v2[(tr + 2) * W + c0], w2);
-            vload<V>(&l_v2[(tr + 3) * W + c0], w3);
-            window<V, W>(&l_v2[(tr + 1) * W], c0, e2);
-            vload<V>(&l_s3[(tr + 3) * W + c0], sxz);
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                const int j = jb + v;
-                const bool zin = j >= 1 && j < nz - 2;
-                float xx, zx, xz, zz;
-                if (xin) {
-                    xx = dp<FAST>(y2[v], y1[v], y3[v], y0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
-                    zx = dp<FAST>(w2[v], w1[v], w3[v], w0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
-                } else {
-                    xx = 0.0f;
-                    zx = 0.0f;
-                }
-                if (zin) {
-                    xz = dp<FAST>(e1[V + v + 1], e1[V + v], e1[V + v + 2], e1[V + v - 1], g.c8dz,
-                                  g.c24dz, g.a1z, g.a2z);
-                    zz = dp<FAST>(e2[V + v + 1], e2[V + v], e2[V + v + 2], e2[V + v - 1], g.c8dz,
-                                  g.c24dz, g.a1z, g.a2z);
-                } else {
-                    xz = 0.0f;
-                    zz = 0.0f;
-                }
-                const float mu = qmu[p][v], lam = qlam[p][v];
-                o1[v] = sa[v] + g.dt * ((lam + 2 * mu) * xx + lam * zz);
-                o2[v] = sc[v] + g.dt * ((lam + 2 * mu) * zz + lam * xx);
-                o3[v] = sxz[v] + g.dt * (mu * (xz + zx));
-                // explosive sources this tile owns: stf_y into sxx and szz
-                const long kc = (long)i * nz + j;
-                for (int pp = q0; pp < q1; pp++)
-                    if (a.sinj_cell[pp] == kc) {
-                        const float f =
-                            a.inj_y[(long)b * a.inj_slot_stride + (long)a.sinj_row[pp] * a.nt + it];
-                        o1[v] += f;
-                        o2[v] += f;
-                    }
-            }
-            vstore<V>(S1o + o, o1);
-            vstore<V>(S2o + o, o2);
-            vstore<V>(S3o + o, o3);
-        }
+        if (PSV) vload<V>(&l_s3[(tr + 3) * W + c0], sxz);
+        stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, tr + 1, c0, i, jb, sa, sc, sxz, qmu[p],
+                                    qlam[p], o1, o2, o3);
+        // explosive sources this tile owns: stf_y into sxx and szz
+        if (PSV) explode_row<V>(a, q0, q1, i, jb, b, it, o1, o2);
+        vstore<V>(S1o + o, o1);
+        vstore<V>(S2o + o, o2);
+        if (PSV) vstore<V>(S3o + o, o3);
     }
 }
 

@@ -59,7 +59,7 @@ constexpr int TILE_X = 4;   // wavefronts per block along x
 // Sampled kernel timing with HIP events on the context stream.
 enum ProfKind {
     PROF_VELOCITY = 0, PROF_STRESS = 1, PROF_INTERACT = 2, PROF_STEP = 3,
-    PROF_FWD_RUN = 4, PROF_ADJ_RUN = 5, PROF_NKIND = 6
+    PROF_FWD_RUN = 4, PROF_ADJ_RUN = 5, PROF_PAIR = 6, PROF_NKIND = 7
 };
 struct Prof {
     int every = 0;  // sample every n-th time step (0 = off)
@@ -110,6 +110,9 @@ struct Ctx {
     int variant = 0;   // 0 fused step kernel, 1 velocity/stress kernel pair
     int tile_x = 8;    // rows per fused tile (4, 8, 16 or 32)
     int col_v = 1;     // columns per lane of the fused column kernel (1, 2 or 4)
+    // two-step kernel (fused variant): tile rows (0 = off) and columns per lane
+    int pair_tx = 0;
+    int pair_v = 2;
     int math = INV_MATH_FAST;
     DevBuf bdt;        // dt / rho, rebuilt at the start of each fast-mode run
     // per-slot models: [batch][3] device pointers {rho (exact) or dt/rho
@@ -156,6 +159,9 @@ struct Ctx {
     // velocity region (tile plus halo), recording points by the tile itself
     PointList f_rec_v, f_rec_t;
     int f_rec_tx = -1;
+    // the same for the two-step kernel's tile geometry
+    PointList p_rec_v, p_rec_t;
+    int p_rec_key = -1;
 
     // captured time loops: one executable graph per (run kind, shots, flags,
     // kernel configuration); `gen` changes whenever buffers or settings they
@@ -164,6 +170,7 @@ struct Ctx {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         long gen = -1;
+        int parity = 0;  // state-set parity the captured loop ends on
     };
     std::map<std::string, GraphEntry> graphs;
     long gen = 0;

@@ -62,4 +62,28 @@ void wide_launch(Ctx &c, const StepArgs &a, unsigned blocks);
 constexpr int col_halo(int V) { return V == 1 ? 3 : 4; }
 constexpr int col_strip(int V) { return 64 * V - 2 * col_halo(V); }
 
+// The snapshot fields of StepArgs for one time step (see StepArgs).
+struct StepRoles {
+    void *snap_u, *snap_v;
+    int v_after;
+    const unsigned *mx_u, *mx_v;
+    unsigned *acc_u, *acc_v;
+};
+
+// Two-step kernel (fdm_pair.hip): two time steps per launch on tiles of
+// tile rows x pair_strip(V) columns; the stress of step n is read with a
+// halo of 6 rows and 6 columns (8 with V = 4, for alignment) each side.
+constexpr int pair_halo(int V) { return V == 2 ? 6 : 8; }
+constexpr int pair_strip(int V) { return 64 * V - 2 * pair_halo(V); }
+// LDS bytes of a two-step tile: stress of steps n / n+1 on rows+12 rows and
+// velocity of steps n+1 / n+2 on rows+9 rows, 64 V columns each.
+constexpr int pair_lds_bytes(bool psv, int rows, int V) {
+    return 4 * 64 * V * ((psv ? 3 : 2) * (rows + 12) + (psv ? 2 : 1) * (rows + 9));
+}
+
+// Launch the two-step kernel for steps a.it and a.it + 1 (roles of the
+// second step in r2): tile rows c.pair_tx, c.pair_v columns per lane.
+template <bool SH, bool PSV, bool FAST>
+void pair_launch(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned blocks);
+
 }  // namespace inv

@@ -19,7 +19,7 @@ INV_LAMBDA, INV_MU, INV_RHO, INV_K_LAMBDA, INV_K_MU, INV_K_RHO, INV_OUT, INV_ADS
 INV_SNAP_F32, INV_SNAP_F16, INV_SNAP_BF16 = 0, 1, 2
 INV_FWD_SNAPSHOTS = 1
 INV_PROF_VELOCITY, INV_PROF_STRESS, INV_PROF_INTERACT, INV_PROF_STEP = 0, 1, 2, 3
-INV_PROF_FWD_RUN, INV_PROF_ADJ_RUN = 4, 5
+INV_PROF_FWD_RUN, INV_PROF_ADJ_RUN, INV_PROF_PAIR = 4, 5, 6
 INV_VARIANT_FUSED, INV_VARIANT_SPLIT, INV_VARIANT_FUSED_LINEAR = 0, 1, 2
 INV_MATH_EXACT, INV_MATH_FAST = 0, 1
 
@@ -101,6 +101,8 @@ SIGNATURES = {
     "inv_prof_read": [_vp, _i, _vp, _vp],
     "inv_set_variant": [_vp, _i, _i],
     "inv_set_tile": [_vp, _i, _i],
+    "inv_set_pair": [_vp, _i, _i],
+    "inv_get_pair": [_vp, _vp, _vp],
     "inv_autotune": [_vp, _i, _i, _i, _vp, _vp, _vp],
     "inv_forward_begin": [_vp, _i, _i, _i],
     "inv_adjoint_begin": [_vp, _i, _i],
@@ -436,6 +438,17 @@ class Context:
     def set_tile(self, tile_x=8, cols_per_lane=1):
         """Fused column kernel with tile_x rows and cols_per_lane columns per lane."""
         self._chk(self.L.inv_set_tile(self.h, tile_x, cols_per_lane))
+
+    def set_pair(self, tile_x=8, cols_per_lane=2):
+        """Two steps per launch with tile_x rows and cols_per_lane columns per
+        lane where the steps allow it (tile_x = 0: off)."""
+        self._chk(self.L.inv_set_pair(self.h, tile_x, cols_per_lane))
+
+    def get_pair(self):
+        """(rows, columns per lane) of the two-step tile in use (rows 0: off)."""
+        r, v = C.c_int(), C.c_int()
+        self._chk(self.L.inv_get_pair(self.h, C.byref(r), C.byref(v)))
+        return r.value, v.value
 
     # ---- measurement ----
     def prof_enable(self, every):
