@@ -308,7 +308,11 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
         m_lam = a.slot_model[3 * b + 1];
         m_mu = a.slot_model[3 * b + 2];
     }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wavefront index is uniform across the wavefront: reading it through
+    // readfirstlane lets every row quantity derived from it live in scalar
+    // registers (scalar branches and address arithmetic instead of per-lane)
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
 
     // All global loads of the three phases are issued first, at clamped (so

@@ -42,7 +42,11 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
     const Geo &g = a.g;
     const int nx = g.nx, nz = g.nz;
     const long so = (long)b * g.N;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wavefront index is uniform across the wavefront: reading it through
+    // readfirstlane lets every row quantity derived from it live in scalar
+    // registers (scalar branches and address arithmetic instead of per-lane)
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c0 = lane * V;          // the lane's first LDS column
     const int jb = j0 - HL + c0;      // its first grid column (a multiple of V)
     const int jl = jb < 0 ? 0 : (jb > nz - V ? nz - V : jb);
