@@ -285,30 +285,6 @@ __device__ __forceinline__ void owned_row(const StepArgs &a, const StepRoles &ro
     }
 }
 
-// Max-abs of the wavefront's u and v into one step's tables, one atomic
-// per wavefront and field; every lane of the wavefront calls it.
-template <bool PSV>
-__device__ __forceinline__ void track_flush(const StepRoles &ro, int lane, int mb, int fu, int fv,
-                                            float mu1, float mu2, float mv1, float mv2) {
-    if (!(ro.acc_u || ro.acc_v)) return;
-    mu1 = wave_max(mu1);
-    mv1 = wave_max(mv1);
-    if (PSV) {
-        mu2 = wave_max(mu2);
-        mv2 = wave_max(mv2);
-    }
-    if (lane == 0) {
-        if (ro.acc_u) {
-            atomicMax(ro.acc_u + mb + fu, __float_as_uint(mu1));
-            if (PSV) atomicMax(ro.acc_u + mb + fu + 1, __float_as_uint(mu2));
-        }
-        if (ro.acc_v) {
-            atomicMax(ro.acc_v + mb + fv, __float_as_uint(mv1));
-            if (PSV) atomicMax(ro.acc_v + mb + fv + 1, __float_as_uint(mv2));
-        }
-    }
-}
-
 // Explosive sources of step it (source type 1) among list entries q0 .. q1
 // at the lane's cells of grid row i: their wavelet is added to sxx and szz.
 template <int V>

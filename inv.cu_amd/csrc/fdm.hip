@@ -278,6 +278,193 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_interact(const IntArgs a) {
     }
 }
 
+// The same accumulation on column strips (nz even): a wavefront owns 124
+// columns (2 adjacent columns per lane, read as float2, or as one 32-bit word
+// holding two 16-bit snapshot values, with 2 halo columns each side) of RW
+// rows.  The x-derivatives come from the rows i-1 .. i+2 held in registers,
+// the z-derivatives from the neighbouring lanes (shuffles), so each field is
+// read about once; k_interact re-reads every neighbour through the caches.
+// Each cell's expressions and their order are k_interact's (SH before P-SV
+// on the shared accumulators), so the kernels agree bit for bit.
+constexpr int ICOLS = 124;  // columns a wavefront owns
+
+template <int FMT>
+__device__ __forceinline__ void snap_pair(const IntArgs &a, int b, int f, long k, float &x0,
+                                          float &x1) {
+    if constexpr (FMT == INV_SNAP_F32) {
+        const float *q = static_cast<const float *>(a.snap) + (long)b * a.snap_slot_stride +
+                         (long)f * a.g.N;
+        const float2 t = *reinterpret_cast<const float2 *>(q + k);
+        x0 = t.x;
+        x1 = t.y;
+    } else {
+        const unsigned short *q = static_cast<const unsigned short *>(a.snap) +
+                                  (long)b * a.snap_slot_stride + (long)f * a.g.N;
+        const float sc = __uint_as_float(a.smax[b * a.nsnapf + f]) * (1.0f / SNAP_RANGE);
+        const unsigned w = *reinterpret_cast<const unsigned *>(q + k);
+        x0 = snap_dec16((unsigned short)(w & 0xffffu), FMT) * sc;
+        x1 = snap_dec16((unsigned short)(w >> 16), FMT) * sc;
+    }
+}
+
+// D+ along z at the lane's two columns from their values e0, e1 (columns
+// c, c+1) and the neighbouring lanes' (c-1 from the lane before, c+2, c+3
+// from the lane after); every lane of the wavefront must call it
+template <bool FAST>
+__device__ __forceinline__ void dz_pair(float e0, float e1, const Geo &g, float &d0, float &d1) {
+    const float m1 = __shfl_up(e1, 1);
+    const float p2 = __shfl_down(e0, 1);
+    const float p3 = __shfl_down(e1, 1);
+    d0 = dp<FAST>(e1, e0, p2, m1, g.c8dz, g.c24dz, g.a1z, g.a2z);
+    d1 = dp<FAST>(p2, e1, p3, e0, g.c8dz, g.c24dz, g.a1z, g.a2z);
+}
+
+template <bool SH, bool PSV, bool FAST, int FMT, int RW>
+__global__ __launch_bounds__(256) void k_interact_col(const IntArgs a, int tiles_z) {
+    constexpr int WR = RW + 3;  // rows i0-1 .. i0+RW+1 of the derivative fields
+    const Geo &g = a.g;
+    const int nx = g.nx, nz = g.nz;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tz = blockIdx.x % tiles_z, tx = blockIdx.x / tiles_z;
+    const int b = blockIdx.y;
+    const int i0 = (tx * 4 + wave) * RW;
+    if (i0 >= nx) return;  // whole wavefront; no block barrier follows
+    const int jb = tz * ICOLS - 2 + 2 * lane;
+    const int jl = jb < 0 ? 0 : (jb > nz - 2 ? nz - 2 : jb);
+    const bool own = lane >= 1 && lane <= 62 && jb < nz;
+    const long so = (long)b * g.N;
+    auto row = [&](int i) { return (long)(i < 0 ? 0 : (i > nx - 1 ? nx - 1 : i)) * nz + jl; };
+    float *kl = a.kl + (long)b * a.k_slot_stride;
+    float *km = a.km + (long)b * a.k_slot_stride;
+    float *kr = a.kr + (long)b * a.k_slot_stride;
+    const float ndt = a.ndt;
+    const int f0 = SH ? 2 : 0;
+    // the derivative fields on rows i0-1 .. i0+RW+1
+    float ay[WR][2], fy[WR][2], ax[WR][2], az[WR][2], fx[WR][2], fz[WR][2];
+#pragma unroll
+    for (int r = 0; r < WR; r++) {
+        const long k = row(i0 - 1 + r);
+        if (SH) {
+            const float2 t = *reinterpret_cast<const float2 *>(a.uy + so + k);
+            ay[r][0] = t.x;
+            ay[r][1] = t.y;
+            snap_pair<FMT>(a, b, 0, k, fy[r][0], fy[r][1]);
+        }
+        if (PSV) {
+            const float2 t = *reinterpret_cast<const float2 *>(a.ux + so + k);
+            const float2 u = *reinterpret_cast<const float2 *>(a.uz + so + k);
+            ax[r][0] = t.x;
+            ax[r][1] = t.y;
+            az[r][0] = u.x;
+            az[r][1] = u.y;
+            snap_pair<FMT>(a, b, f0, k, fx[r][0], fx[r][1]);
+            snap_pair<FMT>(a, b, f0 + 1, k, fz[r][0], fz[r][1]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < RW; t++) {
+        const int i = i0 + t;
+        if (i >= nx) break;
+        const bool xin = i >= 1 && i < nx - 2;
+        const long k = (long)i * nz + jl;
+        float krv[2], kmv[2], klv[2];
+        const bool z0 = jl >= 1 && jl < nz - 2, z1 = jl + 1 >= 1 && jl + 1 < nz - 2;
+        if (own) {
+            if (a.inv_rho) {
+                const float2 q = *reinterpret_cast<const float2 *>(kr + k);
+                krv[0] = q.x;
+                krv[1] = q.y;
+            }
+            if (a.inv_mu) {
+                const float2 q = *reinterpret_cast<const float2 *>(km + k);
+                kmv[0] = q.x;
+                kmv[1] = q.y;
+            }
+            if (PSV && a.inv_lambda) {
+                const float2 q = *reinterpret_cast<const float2 *>(kl + k);
+                klv[0] = q.x;
+                klv[1] = q.y;
+            }
+        }
+        if (SH) {
+            float dzz[2], dzf[2];
+            dz_pair<FAST>(ay[t + 1][0], ay[t + 1][1], g, dzz[0], dzz[1]);
+            dz_pair<FAST>(fy[t + 1][0], fy[t + 1][1], g, dzf[0], dzf[1]);
+            if (own) {
+                float vf[2];
+                snap_pair<FMT>(a, b, 1, k, vf[0], vf[1]);
+                const float2 vv = *reinterpret_cast<const float2 *>(a.vy + so + k);
+                const float vy[2] = {vv.x, vv.y};
+#pragma unroll
+                for (int v = 0; v < 2; v++) {
+                    const bool zin = v ? z1 : z0;
+                    const float dx_ = xin ? dp<FAST>(ay[t + 2][v], ay[t + 1][v], ay[t + 3][v],
+                                                     ay[t][v], g.c8dx, g.c24dx, g.a1x, g.a2x)
+                                          : 0.0f;
+                    const float dz_ = zin ? dzz[v] : 0.0f;
+                    const float dxf = xin ? dp<FAST>(fy[t + 2][v], fy[t + 1][v], fy[t + 3][v],
+                                                     fy[t][v], g.c8dx, g.c24dx, g.a1x, g.a2x)
+                                          : 0.0f;
+                    const float dzf_ = zin ? dzf[v] : 0.0f;
+                    if (a.inv_rho) krv[v] -= vf[v] * vy[v] * ndt;
+                    if (a.inv_mu) kmv[v] -= (dx_ * dxf + dz_ * dzf_) * ndt;
+                }
+            }
+        }
+        if (PSV) {
+            float dxz[2], dzz[2], dxzf[2], dzzf[2];
+            dz_pair<FAST>(ax[t + 1][0], ax[t + 1][1], g, dxz[0], dxz[1]);
+            dz_pair<FAST>(az[t + 1][0], az[t + 1][1], g, dzz[0], dzz[1]);
+            dz_pair<FAST>(fx[t + 1][0], fx[t + 1][1], g, dxzf[0], dxzf[1]);
+            dz_pair<FAST>(fz[t + 1][0], fz[t + 1][1], g, dzzf[0], dzzf[1]);
+            if (own) {
+                float vfx[2], vfz[2];
+                snap_pair<FMT>(a, b, f0 + 2, k, vfx[0], vfx[1]);
+                snap_pair<FMT>(a, b, f0 + 3, k, vfz[0], vfz[1]);
+                const float2 vxx = *reinterpret_cast<const float2 *>(a.vx + so + k);
+                const float2 vzz = *reinterpret_cast<const float2 *>(a.vz + so + k);
+                const float vx[2] = {vxx.x, vxx.y}, vz[2] = {vzz.x, vzz.y};
+#pragma unroll
+                for (int v = 0; v < 2; v++) {
+                    const bool zin = v ? z1 : z0;
+                    float xx, xz, zx, zz, xxf, xzf, zxf, zzf;
+                    if (xin) {
+                        xx = dp<FAST>(ax[t + 2][v], ax[t + 1][v], ax[t + 3][v], ax[t][v], g.c8dx,
+                                      g.c24dx, g.a1x, g.a2x);
+                        zx = dp<FAST>(az[t + 2][v], az[t + 1][v], az[t + 3][v], az[t][v], g.c8dx,
+                                      g.c24dx, g.a1x, g.a2x);
+                        xxf = dp<FAST>(fx[t + 2][v], fx[t + 1][v], fx[t + 3][v], fx[t][v], g.c8dx,
+                                       g.c24dx, g.a1x, g.a2x);
+                        zxf = dp<FAST>(fz[t + 2][v], fz[t + 1][v], fz[t + 3][v], fz[t][v], g.c8dx,
+                                       g.c24dx, g.a1x, g.a2x);
+                    } else {
+                        xx = zx = xxf = zxf = 0.0f;
+                    }
+                    if (zin) {
+                        xz = dxz[v];
+                        zz = dzz[v];
+                        xzf = dxzf[v];
+                        zzf = dzzf[v];
+                    } else {
+                        xz = zz = xzf = zzf = 0.0f;
+                    }
+                    if (a.inv_rho) krv[v] -= (vfx[v] * vx[v] + vfz[v] * vz[v]) * ndt;
+                    if (a.inv_mu)
+                        kmv[v] -= (2 * xx * xxf + 2 * zz * zzf + (xz + zx) * (zxf + xzf)) * ndt;
+                    if (a.inv_lambda) klv[v] -= ((xx + zz) * (xxf + zzf)) * ndt;
+                }
+            }
+        }
+        if (own) {
+            if (a.inv_rho) *reinterpret_cast<float2 *>(kr + k) = make_float2(krv[0], krv[1]);
+            if (a.inv_mu) *reinterpret_cast<float2 *>(km + k) = make_float2(kmv[0], kmv[1]);
+            if (PSV && a.inv_lambda)
+                *reinterpret_cast<float2 *>(kl + k) = make_float2(klv[0], klv[1]);
+        }
+    }
+}
+
 // k_total[k] += sum over slots (in slot order) of the per-slot accumulators.
 __global__ void k_kslot_reduce(float *__restrict__ kl, float *__restrict__ km,
                                float *__restrict__ kr, const float *__restrict__ ks, long N,
@@ -586,8 +773,35 @@ static void launch_interact_f(Ctx &c, const IntArgs &ia, int count) {
         k_interact<SH, PSV, false, FMT><<<tile_grid(c, count), dim3(TILE_Z, TILE_X), 0, c.stream>>>(ia);
 }
 
+template <bool SH, bool PSV, int FMT, int RW>
+static void launch_interact_col(Ctx &c, const IntArgs &ia, int count) {
+    const int tiles_z = (c.g.nz + ICOLS - 1) / ICOLS;
+    const int tiles_x = (c.g.nx + 4 * RW - 1) / (4 * RW);
+    const dim3 grid((unsigned)(tiles_z * tiles_x), (unsigned)count);
+    if (c.math == INV_MATH_FAST)
+        k_interact_col<SH, PSV, true, FMT, RW><<<grid, 256, 0, c.stream>>>(ia, tiles_z);
+    else
+        k_interact_col<SH, PSV, false, FMT, RW><<<grid, 256, 0, c.stream>>>(ia, tiles_z);
+}
+
+template <bool SH, bool PSV, int FMT>
+static void launch_interact_cf(Ctx &c, const IntArgs &ia, int count) {
+    // 8 rows per wavefront where that still gives a few blocks per CU, else 4
+    const long blocks8 = (long)((c.g.nz + ICOLS - 1) / ICOLS) * ((c.g.nx + 31) / 32) * count;
+    if (blocks8 >= 2048) launch_interact_col<SH, PSV, FMT, 8>(c, ia, count);
+    else launch_interact_col<SH, PSV, FMT, 4>(c, ia, count);
+}
+
 template <bool SH, bool PSV>
 static void launch_interact(Ctx &c, const IntArgs &ia, int count) {
+    if (c.g.nz % 2 == 0 && c.g.nz >= 4) {
+        switch (c.snap_format) {
+            case INV_SNAP_F16: launch_interact_cf<SH, PSV, INV_SNAP_F16>(c, ia, count); break;
+            case INV_SNAP_BF16: launch_interact_cf<SH, PSV, INV_SNAP_BF16>(c, ia, count); break;
+            default: launch_interact_cf<SH, PSV, INV_SNAP_F32>(c, ia, count); break;
+        }
+        return;
+    }
     switch (c.snap_format) {
         case INV_SNAP_F16: launch_interact_f<SH, PSV, INV_SNAP_F16>(c, ia, count); break;
         case INV_SNAP_BF16: launch_interact_f<SH, PSV, INV_SNAP_BF16>(c, ia, count); break;

@@ -290,6 +290,7 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
     __shared__ float l_sxx[PSV ? SR * W : 1], l_szz[PSV ? SR * W : 1], l_sxz[PSV ? SR * W : 1];
     __shared__ float l_sxy[SH ? SR * W : 1], l_szy[SH ? SR * W : 1];
     __shared__ float l_vx[PSV ? VR * W : 1], l_vz[PSV ? VR * W : 1], l_vy[SH ? VR * W : 1];
+    __shared__ float l_track[4 * NWAVE];
 
     const int nw = gridDim.x, w = blockIdx.x;
     const int q = nw >> 3, rmd = nw & 7, xcd = w & 7;
@@ -540,28 +541,11 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
             }
         }
     }
-    // max-abs of this tile's u and v after the step, one atomic per wavefront
-    // (non-negative floats order as their bit patterns)
-    if (track) {
-        mu1 = wave_max(mu1);
-        mv1 = wave_max(mv1);
-        if (PSV) {
-            mu2 = wave_max(mu2);
-            mv2 = wave_max(mv2);
-        }
-        if (lane == 0) {
-            const int mb = b * a.nsnapf;
-            if (a.acc_u) {
-                atomicMax(a.acc_u + mb + fu, __float_as_uint(mu1));
-                if (PSV) atomicMax(a.acc_u + mb + fu + 1, __float_as_uint(mu2));
-            }
-            if (a.acc_v) {
-                atomicMax(a.acc_v + mb + fv, __float_as_uint(mv1));
-                if (PSV) atomicMax(a.acc_v + mb + fv + 1, __float_as_uint(mv2));
-            }
-        }
-    }
+    // max-abs of this tile's u and v after the step
+    if (track) track_stage<PSV>(l_track, wave, lane, mu1, mu2, mv1, mv2);
     __syncthreads();
+    if (track && threadIdx.x == 0)
+        track_commit<PSV>(l_track, NWAVE, a.acc_u, a.acc_v, b * a.nsnapf, fu, fv);
 
     // 3. stress of step n+1, rows i0 .. i0+TX-1, columns j0 .. j0+CZ-1
     if (!jtin) return;

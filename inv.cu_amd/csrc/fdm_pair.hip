@@ -41,6 +41,7 @@ __global__ __launch_bounds__(PBLOCK) void k_pair(const StepArgs a, const StepRol
     constexpr int PS = (SR + NW - 1) / NW;      // stress rows loaded per wavefront
     constexpr int P = (VR + NW - 1) / NW;       // rows x = wave - 4 + 4 p per wavefront
     extern __shared__ __align__(16) float smem[];
+    __shared__ float l_track[8 * NW];  // per-wavefront maxima of the two steps
     float *const l_s1 = smem;
     float *const l_s2 = l_s1 + SR * W;
     float *const l_s3 = PSV ? l_s2 + SR * W : l_s2;  // unused for SH
@@ -184,9 +185,11 @@ __global__ __launch_bounds__(PBLOCK) void k_pair(const StepArgs a, const StepRol
                 if (PSV) qv2[p][v] = n2[v];
             }
         }
-        track_flush<PSV>(r1s, lane, mb, fu, fv, mu1, mu2, mv1, mv2);
+        if (r1s.acc_u || r1s.acc_v) track_stage<PSV>(l_track, wave, lane, mu1, mu2, mv1, mv2);
     }
     __syncthreads();
+    if ((r1s.acc_u || r1s.acc_v) && threadIdx.x == 0)
+        track_commit<PSV>(l_track, NW, r1s.acc_u, r1s.acc_v, mb, fu, fv);
 
     // B. stress of step n+1 on rows x = -3 .. TX+2, in place (0 outside the grid)
 #pragma unroll
@@ -249,9 +252,11 @@ __global__ __launch_bounds__(PBLOCK) void k_pair(const StepArgs a, const StepRol
                 }
             }
         }
-        track_flush<PSV>(r2, lane, mb, fu, fv, mu1, mu2, mv1, mv2);
+        if (r2.acc_u || r2.acc_v) track_stage<PSV>(l_track + 4 * NW, wave, lane, mu1, mu2, mv1, mv2);
     }
     __syncthreads();
+    if ((r2.acc_u || r2.acc_v) && threadIdx.x == 0)
+        track_commit<PSV>(l_track + 4 * NW, NW, r2.acc_u, r2.acc_v, mb, fu, fv);
 
     // D. stress of step n+2 on the owned cells
     if (!cown) return;

@@ -29,6 +29,7 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
     constexpr int PS = (SR + NW - 1) / NW, PV = (VR + NW - 1) / NW, PT = (TX + NW - 1) / NW;
     __shared__ __align__(16) float l_s1[SR * W], l_s2[SR * W], l_s3[PSV ? SR * W : 4];
     __shared__ __align__(16) float l_v1[VR * W], l_v2[PSV ? VR * W : 4];
+    __shared__ float l_track[4 * NW];
 
     // XCD-aware tile order (as k_col)
     const int nw = gridDim.x, w = blockIdx.x;
@@ -218,27 +219,11 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
             }
         }
     }
-    // max-abs of this tile's u and v after the step, one atomic per wavefront
-    if (track) {
-        mu1 = wave_max(mu1);
-        mv1 = wave_max(mv1);
-        if (PSV) {
-            mu2 = wave_max(mu2);
-            mv2 = wave_max(mv2);
-        }
-        if (lane == 0) {
-            const int mb = b * a.nsnapf;
-            if (a.acc_u) {
-                atomicMax(a.acc_u + mb + fu, __float_as_uint(mu1));
-                if (PSV) atomicMax(a.acc_u + mb + fu + 1, __float_as_uint(mu2));
-            }
-            if (a.acc_v) {
-                atomicMax(a.acc_v + mb + fv, __float_as_uint(mv1));
-                if (PSV) atomicMax(a.acc_v + mb + fv + 1, __float_as_uint(mv2));
-            }
-        }
-    }
+    // max-abs of this tile's u and v after the step
+    if (track) track_stage<PSV>(l_track, wave, lane, mu1, mu2, mv1, mv2);
     __syncthreads();
+    if (track && threadIdx.x == 0)
+        track_commit<PSV>(l_track, NW, a.acc_u, a.acc_v, b * a.nsnapf, fu, fv);
 
     // 3. stress of step n+1 on the owned cells (velocity LDS rows tr .. tr+3)
     if (!cown) return;

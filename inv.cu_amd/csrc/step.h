@@ -52,6 +52,53 @@ __device__ __forceinline__ float wave_max(float x) {
     return x;
 }
 
+// Max-abs tracking for the 16-bit snapshot scales.  Each wavefront's maxima
+// of u and v after the step go to LDS (track_stage, every lane calls it);
+// after the block barrier that follows, one thread merges the block's and
+// raises a table only where it holds less (track_commit).  That is at most
+// one atomic per block and field, and none once a table holds a larger
+// value, instead of one per wavefront on the same few addresses (which
+// serialised chip-wide: +80 % on those steps).  Tables only grow, so a stale
+// read can only cause an unneeded atomic, never skip a needed one.
+// Non-negative floats order as their bit patterns.
+__device__ __forceinline__ void raise_max(unsigned *p, float m) {
+    const unsigned bits = __float_as_uint(m);
+    if (bits > __atomic_load_n(p, __ATOMIC_RELAXED)) atomicMax(p, bits);
+}
+
+template <bool PSV>
+__device__ __forceinline__ void track_stage(float *lt, int wave, int lane, float mu1, float mu2,
+                                            float mv1, float mv2) {
+    mu1 = wave_max(mu1);
+    mv1 = wave_max(mv1);
+    if (PSV) {
+        mu2 = wave_max(mu2);
+        mv2 = wave_max(mv2);
+    }
+    if (lane == 0) {
+        lt[wave * 4] = mu1;
+        lt[wave * 4 + 1] = mu2;
+        lt[wave * 4 + 2] = mv1;
+        lt[wave * 4 + 3] = mv2;
+    }
+}
+
+template <bool PSV>
+__device__ __forceinline__ void track_commit(const float *lt, int nwave, unsigned *acc_u,
+                                             unsigned *acc_v, int mb, int fu, int fv) {
+    float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int w = 0; w < nwave; w++)
+        for (int q = 0; q < 4; q++) m[q] = fmaxf(m[q], lt[w * 4 + q]);
+    if (acc_u) {
+        raise_max(acc_u + mb + fu, m[0]);
+        if (PSV) raise_max(acc_u + mb + fu + 1, m[1]);
+    }
+    if (acc_v) {
+        raise_max(acc_v + mb + fv, m[2]);
+        if (PSV) raise_max(acc_v + mb + fv + 1, m[3]);
+    }
+}
+
 // Launch the wide column kernel (fdm_wide.hip): tile_x rows, c.col_v = 2 or
 // 4 columns per lane; `blocks` tiles (shot slots x tile rows x tile columns).
 template <bool SH, bool PSV, bool FAST>
