@@ -50,19 +50,57 @@ __device__ __forceinline__ void window(const float *row, int c0, float (&e)[3 * 
     for (int v = 0; v < V; v++) e[2 * V + v] = t[v];
 }
 
+// A tile's entries p0 .. p1-1 of a point list, the first 64 held one per
+// lane (read once per block, with the other loads; every lane of the
+// wavefront must make the call, before any lane leaves the kernel).  Scanning them per row
+// is then a lane read per entry instead of a scalar load and its round trip
+// (tiles holding many receivers spent most of their time in those).  Entries
+// past the first 64 are read from memory.
+struct LanePts {
+    int p0, p1, cell, row;
+};
+
+__device__ __forceinline__ LanePts lane_pts(const int *cell, const int *row, int p0, int p1,
+                                            int lane) {
+    LanePts t{p0, p1, -1, 0};
+    if (lane < p1 - p0) {
+        t.cell = cell[p0 + lane];
+        t.row = row[p0 + lane];
+    }
+    return t;
+}
+
+// f(cell, row) for the entries whose cell lies in rlo .. rhi, in list order
+template <class F>
+__device__ __forceinline__ void scan_pts(const LanePts &t, const int *cell, const int *row,
+                                         long rlo, long rhi, F &&f) {
+    const int n = t.p1 - t.p0, m = n < 64 ? n : 64;
+    for (int q = 0; q < m; q++) {
+        const int cp = __builtin_amdgcn_readlane(t.cell, q);
+        if (cp < rlo || cp > rhi) continue;
+        f(cp, __builtin_amdgcn_readlane(t.row, q));
+    }
+    for (int pp = t.p0 + 64; pp < t.p1; pp++) {
+        const int cp = cell[pp];
+        if (cp < rlo || cp > rhi) continue;
+        f(cp, row[pp]);
+    }
+}
+
 // Velocity of step it+1 (divS, addSTF, updateV) on the lane's V columns
 // jb .. jb+V-1 of grid row i.  ls1/ls2/ls3: stress of step it in LDS rows of
 // W columns (sxx, szz, sxz for P-SV; sxy, szy for SH), grid row i at LDS row
 // r, rows r-2 .. r+1 read.  v1/v2: the lane's velocity of step it, rho: rho
 // or dt/rho (fast math).  Point forces of step it from the list entries
 // p0 .. p1 whose cell lies in grid cells rlo .. rhi (this row's span of the
-// tile) are added in list order.  Cells outside the grid (rin false) give 0.
+// tile, held per lane in ip) are added in list order after the derivatives.
+// Cells outside the grid (rin false) give 0.
 template <bool PSV, int V, int W, bool FAST>
 __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, const float *ls2,
                                         const float *ls3, int r, int c0, int i, int jb, bool rin,
                                         float abx, const float (&abz)[V], const float (&abt)[V],
                                         const float (&v1)[V], const float (&v2)[V],
-                                        const float (&rho)[V], int p0, int p1, long rlo,
+                                        const float (&rho)[V], const LanePts &ip, long rlo,
                                         long rhi, int b, long it, float (&n1)[V],
                                         float (&n2)[V]) {
     const Geo &g = a.g;
@@ -76,22 +114,29 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
         vload<V>(&ls1[r * W + c0], x2);
         vload<V>(&ls1[(r + 1) * W + c0], x3);
         window<V, W>(&ls2[r * W], c0, ez);
+        float d[V];
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const int j = jb + v;
-            const long kc = (long)i * nz + j;
-            float d = xin ? dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, g.a1x, g.a2x)
-                          : 0.0f;
+            d[v] = xin ? dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, g.a1x, g.a2x)
+                       : 0.0f;
             if (j >= 2 && j < nz - 2)
-                d += dm<FAST>(ez[V + v], ez[V + v - 1], ez[V + v + 1], ez[V + v - 2], g.c8dz,
-                              g.c24dz, g.a1z, g.a2z);
-            for (int pp = p0; pp < p1; pp++) {
-                const int cp = a.inj_cell[pp];
-                if (cp < rlo || cp > rhi) continue;
-                if (cp == kc) d += I1[(long)b * a.inj_slot_stride + (long)a.inj_row[pp] * a.nt + it];
-            }
+                d[v] += dm<FAST>(ez[V + v], ez[V + v - 1], ez[V + v + 1], ez[V + v - 2], g.c8dz,
+                                 g.c24dz, g.a1z, g.a2z);
+        }
+        const long k0 = (long)i * nz + jb;
+        scan_pts(ip, a.inj_cell, a.inj_row, rlo, rhi, [&](int cp, int row) {
+            const long e = cp - k0;
+            if (e < 0 || e >= V) return;
+            const float f = I1[(long)b * a.inj_slot_stride + (long)row * a.nt + it];
+            // a select per column keeps the array in registers
+#pragma unroll
+            for (int v = 0; v < V; v++) d[v] = e == v ? d[v] + f : d[v];
+        });
+#pragma unroll
+        for (int v = 0; v < V; v++) {
             const float ab = (abx * abz[v]) * abt[v];
-            n1[v] = rin ? vel_update<FAST>(ab, v1[v], g.dt, d, rho[v]) : 0.0f;
+            n1[v] = rin ? vel_update<FAST>(ab, v1[v], g.dt, d[v], rho[v]) : 0.0f;
         }
     } else {
         float x0[V], x1[V], x2[V], x3[V], z0[V], z1[V], z2[V], z3[V], e3[3 * V], e2[3 * V];
@@ -105,36 +150,41 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
         vload<V>(&ls3[(r + 1) * W + c0], z3);
         window<V, W>(&ls3[r * W], c0, e3);
         window<V, W>(&ls2[r * W], c0, e2);
+        float dx_[V], dz_[V];
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const int j = jb + v;
-            const long kc = (long)i * nz + j;
-            float dx_, dz_;
             if (xin) {
-                dx_ = dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
-                dz_ = dm<FAST>(z2[v], z1[v], z3[v], z0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+                dx_[v] = dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+                dz_[v] = dm<FAST>(z2[v], z1[v], z3[v], z0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
             } else {
-                dx_ = 0.0f;
-                dz_ = 0.0f;
+                dx_[v] = 0.0f;
+                dz_[v] = 0.0f;
             }
             if (j >= 2 && j < nz - 2) {
-                dx_ += dm<FAST>(e3[V + v], e3[V + v - 1], e3[V + v + 1], e3[V + v - 2], g.c8dz,
-                                g.c24dz, g.a1z, g.a2z);
-                dz_ += dm<FAST>(e2[V + v], e2[V + v - 1], e2[V + v + 1], e2[V + v - 2], g.c8dz,
-                                g.c24dz, g.a1z, g.a2z);
+                dx_[v] += dm<FAST>(e3[V + v], e3[V + v - 1], e3[V + v + 1], e3[V + v - 2],
+                                   g.c8dz, g.c24dz, g.a1z, g.a2z);
+                dz_[v] += dm<FAST>(e2[V + v], e2[V + v - 1], e2[V + v + 1], e2[V + v - 2],
+                                   g.c8dz, g.c24dz, g.a1z, g.a2z);
             }
-            for (int pp = p0; pp < p1; pp++) {
-                const int cp = a.inj_cell[pp];
-                if (cp < rlo || cp > rhi) continue;
-                if (cp == kc) {
-                    const long ks = (long)b * a.inj_slot_stride + (long)a.inj_row[pp] * a.nt + it;
-                    dx_ += I1[ks];
-                    dz_ += I2[ks];
-                }
+        }
+        const long k0 = (long)i * nz + jb;
+        scan_pts(ip, a.inj_cell, a.inj_row, rlo, rhi, [&](int cp, int row) {
+            const long e = cp - k0;
+            if (e < 0 || e >= V) return;
+            const long ks = (long)b * a.inj_slot_stride + (long)row * a.nt + it;
+            const float fx = I1[ks], fz = I2[ks];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                dx_[v] = e == v ? dx_[v] + fx : dx_[v];
+                dz_[v] = e == v ? dz_[v] + fz : dz_[v];
             }
+        });
+#pragma unroll
+        for (int v = 0; v < V; v++) {
             const float ab = (abx * abz[v]) * abt[v];
-            n1[v] = rin ? vel_update<FAST>(ab, v1[v], g.dt, dx_, rho[v]) : 0.0f;
-            n2[v] = rin ? vel_update<FAST>(ab, v2[v], g.dt, dz_, rho[v]) : 0.0f;
+            n1[v] = rin ? vel_update<FAST>(ab, v1[v], g.dt, dx_[v], rho[v]) : 0.0f;
+            n2[v] = rin ? vel_update<FAST>(ab, v2[v], g.dt, dz_[v], rho[v]) : 0.0f;
         }
     }
 }
@@ -232,12 +282,13 @@ __device__ __forceinline__ SnapScales snap_scales(const StepRoles &ro, int fmt, 
 // The owned cells of grid row i after the velocity update of step it, in
 // the order of k_colv: u(it) into its snapshot, u(it+1) = u(it) + v(it+1) dt,
 // v into its snapshot (v(it+1) with v_after, else v(it)), max-abs tracking
-// of u(it+1) and v(it+1), receivers of step it.  u1/u2 hold u(it) on entry and
+// of u(it+1) and v(it+1), receivers of step it (list entries held per lane
+// in rp).  u1/u2 hold u(it) on entry and
 // u(it+1) on return; w1/w2: v(it); n1/n2: v(it+1).
 template <bool PSV, int V>
 __device__ __forceinline__ void owned_row(const StepArgs &a, const StepRoles &ro,
                                           const SnapScales &ss, int fu, int fv, long sb, int i,
-                                          int jb, long it, int b, int r0, int r1, long rlo,
+                                          int jb, long it, int b, const LanePts &rp, long rlo,
                                           long rhi, float (&u1)[V], float (&u2)[V],
                                           const float (&w1)[V], const float (&w2)[V],
                                           const float (&n1)[V], const float (&n2)[V],
@@ -273,36 +324,40 @@ __device__ __forceinline__ void owned_row(const StepArgs &a, const StepRoles &ro
                 mv2 = fmaxf(mv2, fabsf(n2[v]));
             }
         }
-        for (int pp = r0; pp < r1; pp++) {
-            const int cp = a.rec_cell[pp];
-            if (cp < rlo || cp > rhi) continue;
-            if (cp == kc) {
-                const long kr = (long)b * a.out_slot_stride + (long)a.rec_row[pp] * a.nt + it;
-                O1[kr] = a.rec_disp ? u : n1[v];
-                if (PSV) O2[kr] = a.rec_disp ? uu : n2[v];
-            }
-        }
     }
+    const long k0 = (long)i * g.nz + jb;
+    scan_pts(rp, a.rec_cell, a.rec_row, rlo, rhi, [&](int cp, int row) {
+        const long e = cp - k0;
+        if (e < 0 || e >= V) return;
+        float x = 0.0f, y = 0.0f;
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            x = e == v ? (a.rec_disp ? u1[v] : n1[v]) : x;
+            if (PSV) y = e == v ? (a.rec_disp ? u2[v] : n2[v]) : y;
+        }
+        const long kr = (long)b * a.out_slot_stride + (long)row * a.nt + it;
+        O1[kr] = x;
+        if (PSV) O2[kr] = y;
+    });
 }
 
-// Explosive sources of step it (source type 1) among list entries q0 .. q1
-// at the lane's cells of grid row i: their wavelet is added to sxx and szz.
+// Explosive sources of step it (source type 1) among the tile's list
+// entries (held per lane in xp) at the lane's cells of grid row i: their
+// wavelet is added to sxx and szz.
 template <int V>
-__device__ __forceinline__ void explode_row(const StepArgs &a, int q0, int q1, int i, int jb,
+__device__ __forceinline__ void explode_row(const StepArgs &a, const LanePts &xp, int i, int jb,
                                             int b, long it, float (&o1)[V], float (&o2)[V]) {
-    for (int pp = q0; pp < q1; pp++) {
-        const long cp = a.sinj_cell[pp];
-        const long d = cp - ((long)i * a.g.nz + jb);
-        if (d < 0 || d >= V) continue;
+    const long k0 = (long)i * a.g.nz + jb;
+    scan_pts(xp, a.sinj_cell, a.sinj_row, k0 - jb, k0 - jb + a.g.nz - 1, [&](int cp, int row) {
+        const long e = cp - k0;
+        if (e < 0 || e >= V) return;
+        const float f = a.inj_y[(long)b * a.inj_slot_stride + (long)row * a.nt + it];
 #pragma unroll
-        for (int v = 0; v < V; v++)
-            if (d == v) {
-                const float f =
-                    a.inj_y[(long)b * a.inj_slot_stride + (long)a.sinj_row[pp] * a.nt + it];
-                o1[v] += f;
-                o2[v] += f;
-            }
-    }
+        for (int v = 0; v < V; v++) {
+            o1[v] = e == v ? o1[v] + f : o1[v];
+            o2[v] = e == v ? o2[v] + f : o2[v];
+        }
+    });
 }
 
 }  // namespace inv

@@ -153,6 +153,9 @@ __global__ __launch_bounds__(PBLOCK) void k_pair(const StepArgs a, const StepRol
         q0 = a.sinj_ptr[b * per + t];
         q1 = a.sinj_ptr[b * per + t + 1];
     }
+    const LanePts ip = lane_pts(a.inj_cell, a.inj_row, p0, p1, lane);
+    const LanePts rp = lane_pts(a.rec_cell, a.rec_row, r0, r1, lane);
+    const LanePts xp = lane_pts(a.sinj_cell, a.sinj_row, q0, q1, lane);
     const long it = a.it;
     const int fu = PSV ? a.psv_f0 : 0, fv = PSV ? a.psv_f0 + 2 : 1;
     const long sb = (long)b * a.snap_slot_stride;
@@ -173,11 +176,11 @@ __global__ __launch_bounds__(PBLOCK) void k_pair(const StepArgs a, const StepRol
             const long rlo = (long)i * nz + (j0 - HL), rhi = rlo + W - 1;
             float n1[V], n2[V];
             vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, c0, i, jb, rin, abx, abz, abt,
-                                     qv1[p], qv2[p], qr[p], p0, p1, rlo, rhi, b, it, n1, n2);
+                                     qv1[p], qv2[p], qr[p], ip, rlo, rhi, b, it, n1, n2);
             vstore<V>(&l_v1[(x + 4) * W + c0], n1);
             if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], n2);
             if (x >= 0 && x < TX && cown && i < nx)
-                owned_row<PSV, V>(a, r1s, ss, fu, fv, sb, i, jb, it, b, r0, r1, rlo, rhi, qu1[p],
+                owned_row<PSV, V>(a, r1s, ss, fu, fv, sb, i, jb, it, b, rp, rlo, rhi, qu1[p],
                                   qu2[p], qv1[p], qv2[p], n1, n2, mu1, mu2, mv1, mv2);
 #pragma unroll
             for (int v = 0; v < V; v++) {
@@ -206,7 +209,7 @@ __global__ __launch_bounds__(PBLOCK) void k_pair(const StepArgs a, const StepRol
         if (PSV) vload<V>(&l_s3[r * W + c0], sxz);
         stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, x + 4, c0, i, jb, sa, sc, sxz, qmu[p], qlam[p],
                                     o1, o2, o3);
-        if (PSV) explode_row<V>(a, q0, q1, i, jb, b, it, o1, o2);
+        if (PSV) explode_row<V>(a, xp, i, jb, b, it, o1, o2);
         if (!rin) {
 #pragma unroll
             for (int v = 0; v < V; v++) {
@@ -237,11 +240,11 @@ __global__ __launch_bounds__(PBLOCK) void k_pair(const StepArgs a, const StepRol
             const long rlo = (long)i * nz + (j0 - HL), rhi = rlo + W - 1;
             float n1[V], n2[V];
             vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, c0, i, jb, rin, abx, abz, abt,
-                                     qv1[p], qv2[p], qr[p], p0, p1, rlo, rhi, b, it + 1, n1, n2);
+                                     qv1[p], qv2[p], qr[p], ip, rlo, rhi, b, it + 1, n1, n2);
             vstore<V>(&l_v1[(x + 4) * W + c0], n1);
             if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], n2);
             if (x >= 0 && x < TX && cown && i < nx) {
-                owned_row<PSV, V>(a, r2, ss, fu, fv, sb, i, jb, it + 1, b, r0, r1, rlo, rhi,
+                owned_row<PSV, V>(a, r2, ss, fu, fv, sb, i, jb, it + 1, b, rp, rlo, rhi,
                                   qu1[p], qu2[p], qv1[p], qv2[p], n1, n2, mu1, mu2, mv1, mv2);
                 const long o = so + (long)i * nz + jb;
                 vstore<V>(V1o + o, n1);
@@ -275,7 +278,7 @@ __global__ __launch_bounds__(PBLOCK) void k_pair(const StepArgs a, const StepRol
         if (PSV) vload<V>(&l_s3[r * W + c0], sxz);
         stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, x + 4, c0, i, jb, sa, sc, sxz, qmu[p], qlam[p],
                                     o1, o2, o3);
-        if (PSV) explode_row<V>(a, q0, q1, i, jb, b, it + 1, o1, o2);
+        if (PSV) explode_row<V>(a, xp, i, jb, b, it + 1, o1, o2);
         vstore<V>(S1o + o, o1);
         vstore<V>(S2o + o, o2);
         if (PSV) vstore<V>(S3o + o, o3);

@@ -134,6 +134,8 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         r0 = a.rec_ptr[t];
         r1 = a.rec_ptr[t + 1];
     }
+    const LanePts ip = lane_pts(a.inj_cell, a.inj_row, p0, p1, lane);
+    const LanePts rp = lane_pts(a.rec_cell, a.rec_row, r0, r1, lane);
     const long it = a.it;
     const int fmt = a.snap_fmt;
     const int fu = PSV ? a.psv_f0 : 0, fv = PSV ? a.psv_f0 + 2 : 1;
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         const long rlo = (long)i * nz + (j0 - HL), rhi = rlo + W - 1;
         float n1[V], n2[V];
         vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, vr + 2, c0, i, jb, rin, abx, abz, abt, qv1[p],
-                                 qv2[p], qr[p], p0, p1, rlo, rhi, b, it, n1, n2);
+                                 qv2[p], qr[p], ip, rlo, rhi, b, it, n1, n2);
         vstore<V>(&l_v1[vr * W + c0], n1);
         if (PSV) vstore<V>(&l_v2[vr * W + c0], n2);
         // owned cells: v, u, snapshots, receivers
@@ -201,16 +203,21 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
                         mv2 = fmaxf(mv2, fabsf(n2[v]));
                     }
                 }
-                for (int pp = r0; pp < r1; pp++) {
-                    const int cp = a.rec_cell[pp];
-                    if (cp < rlo || cp > rhi) continue;
-                    if (cp == kc) {
-                        const long kr = (long)b * a.out_slot_stride + (long)a.rec_row[pp] * a.nt + it;
-                        O1[kr] = a.rec_disp ? u : n1[v];
-                        if (PSV) O2[kr] = a.rec_disp ? uu : n2[v];
-                    }
-                }
             }
+            const long k0 = (long)i * nz + jb;
+            scan_pts(rp, a.rec_cell, a.rec_row, rlo, rhi, [&](int cp, int row) {
+                const long e = cp - k0;
+                if (e < 0 || e >= V) return;
+                float x = 0.0f, y = 0.0f;
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    x = e == v ? (a.rec_disp ? nu1[v] : n1[v]) : x;
+                    if (PSV) y = e == v ? (a.rec_disp ? nu2[v] : n2[v]) : y;
+                }
+                const long kr = (long)b * a.out_slot_stride + (long)row * a.nt + it;
+                O1[kr] = x;
+                if (PSV) O2[kr] = y;
+            });
             vstore<V>(V1o + o, n1);
             vstore<V>(U1 + o, nu1);
             if (PSV) {
@@ -226,12 +233,14 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         track_commit<PSV>(l_track, NW, a.acc_u, a.acc_v, b * a.nsnapf, fu, fv);
 
     // 3. stress of step n+1 on the owned cells (velocity LDS rows tr .. tr+3)
-    if (!cown) return;
+    // the tile's explosions, one per lane: loaded while every lane is active
     int q0 = 0, q1 = 0;
     if (PSV && a.sinj_ptr) {
         q0 = a.sinj_ptr[b * per + t];
         q1 = a.sinj_ptr[b * per + t + 1];
     }
+    const LanePts xp = lane_pts(a.sinj_cell, a.sinj_row, q0, q1, lane);
+    if (!cown) return;
 #pragma unroll
     for (int p = 0; p < PT; p++) {
         const int tr = wave + p * NW;
@@ -246,7 +255,7 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, tr + 1, c0, i, jb, sa, sc, sxz, qmu[p],
                                     qlam[p], o1, o2, o3);
         // explosive sources this tile owns: stf_y into sxx and szz
-        if (PSV) explode_row<V>(a, q0, q1, i, jb, b, it, o1, o2);
+        if (PSV) explode_row<V>(a, xp, i, jb, b, it, o1, o2);
         vstore<V>(S1o + o, o1);
         vstore<V>(S2o + o, o2);
         if (PSV) vstore<V>(S3o + o, o3);
