@@ -51,11 +51,13 @@ __device__ __forceinline__ void window(const float *row, int c0, float (&e)[3 * 
 }
 
 // A tile's entries p0 .. p1-1 of a point list, the first 64 held one per
-// lane (read once per block, with the other loads; every lane of the
-// wavefront must make the call, before any lane leaves the kernel).  Scanning them per row
+// lane (read once per block, with the other loads).  Scanning them per row
 // is then a lane read per entry instead of a scalar load and its round trip
 // (tiles holding many receivers spent most of their time in those).  Entries
-// past the first 64 are read from memory.
+// past the first 64 are read from memory.  A lane read of an inactive lane's
+// register is undefined, so lane_pts and scan_pts are only called where
+// every lane of the wavefront is active (wave-uniform control flow); per-lane
+// conditions go inside the scan's callback.
 struct LanePts {
     int p0, p1, cell, row;
 };
@@ -285,11 +287,13 @@ __device__ __forceinline__ SnapScales snap_scales(const StepRoles &ro, int fmt, 
 // of u(it+1) and v(it+1), receivers of step it (list entries held per lane
 // in rp).  u1/u2 hold u(it) on entry and
 // u(it+1) on return; w1/w2: v(it); n1/n2: v(it+1).
+// Every lane of the wavefront calls it (wave-uniform row); act: the lane
+// owns its cells of the row.
 template <bool PSV, int V>
 __device__ __forceinline__ void owned_row(const StepArgs &a, const StepRoles &ro,
                                           const SnapScales &ss, int fu, int fv, long sb, int i,
-                                          int jb, long it, int b, const LanePts &rp, long rlo,
-                                          long rhi, float (&u1)[V], float (&u2)[V],
+                                          int jb, bool act, long it, int b, const LanePts &rp,
+                                          long rlo, long rhi, float (&u1)[V], float (&u2)[V],
                                           const float (&w1)[V], const float (&w2)[V],
                                           const float (&n1)[V], const float (&n2)[V],
                                           float &mu1, float &mu2, float &mv1, float &mv2) {
@@ -297,38 +301,40 @@ __device__ __forceinline__ void owned_row(const StepArgs &a, const StepRoles &ro
     const int fmt = a.snap_fmt;
     const bool track = ro.acc_u || ro.acc_v;
     float *O1 = PSV ? a.out_x : a.out_y, *O2 = a.out_z;
+    if (act) {
 #pragma unroll
-    for (int v = 0; v < V; v++) {
-        const long kc = (long)i * g.nz + jb + v;
-        float u = u1[v];
-        float uu = PSV ? u2[v] : 0.0f;
-        if (ro.snap_u) {
-            snap_put(ro.snap_u, fmt, sb + (long)fu * g.N + kc, u, ss.su1);
-            if (PSV) snap_put(ro.snap_u, fmt, sb + (long)(fu + 1) * g.N + kc, uu, ss.su2);
-        }
-        u += n1[v] * g.dt;
-        if (PSV) uu += n2[v] * g.dt;
-        u1[v] = u;
-        if (PSV) u2[v] = uu;
-        if (ro.snap_v) {
-            snap_put(ro.snap_v, fmt, sb + (long)fv * g.N + kc, ro.v_after ? n1[v] : w1[v], ss.sv1);
-            if (PSV)
-                snap_put(ro.snap_v, fmt, sb + (long)(fv + 1) * g.N + kc,
-                         ro.v_after ? n2[v] : w2[v], ss.sv2);
-        }
-        if (track) {
-            mu1 = fmaxf(mu1, fabsf(u));
-            mv1 = fmaxf(mv1, fabsf(n1[v]));
-            if (PSV) {
-                mu2 = fmaxf(mu2, fabsf(uu));
-                mv2 = fmaxf(mv2, fabsf(n2[v]));
+        for (int v = 0; v < V; v++) {
+            const long kc = (long)i * g.nz + jb + v;
+            float u = u1[v];
+            float uu = PSV ? u2[v] : 0.0f;
+            if (ro.snap_u) {
+                snap_put(ro.snap_u, fmt, sb + (long)fu * g.N + kc, u, ss.su1);
+                if (PSV) snap_put(ro.snap_u, fmt, sb + (long)(fu + 1) * g.N + kc, uu, ss.su2);
+            }
+            u += n1[v] * g.dt;
+            if (PSV) uu += n2[v] * g.dt;
+            u1[v] = u;
+            if (PSV) u2[v] = uu;
+            if (ro.snap_v) {
+                snap_put(ro.snap_v, fmt, sb + (long)fv * g.N + kc, ro.v_after ? n1[v] : w1[v], ss.sv1);
+                if (PSV)
+                    snap_put(ro.snap_v, fmt, sb + (long)(fv + 1) * g.N + kc,
+                             ro.v_after ? n2[v] : w2[v], ss.sv2);
+            }
+            if (track) {
+                mu1 = fmaxf(mu1, fabsf(u));
+                mv1 = fmaxf(mv1, fabsf(n1[v]));
+                if (PSV) {
+                    mu2 = fmaxf(mu2, fabsf(uu));
+                    mv2 = fmaxf(mv2, fabsf(n2[v]));
+                }
             }
         }
     }
     const long k0 = (long)i * g.nz + jb;
     scan_pts(rp, a.rec_cell, a.rec_row, rlo, rhi, [&](int cp, int row) {
         const long e = cp - k0;
-        if (e < 0 || e >= V) return;
+        if (!act || e < 0 || e >= V) return;
         float x = 0.0f, y = 0.0f;
 #pragma unroll
         for (int v = 0; v < V; v++) {
@@ -343,7 +349,7 @@ __device__ __forceinline__ void owned_row(const StepArgs &a, const StepRoles &ro
 
 // Explosive sources of step it (source type 1) among the tile's list
 // entries (held per lane in xp) at the lane's cells of grid row i: their
-// wavelet is added to sxx and szz.
+// wavelet is added to sxx and szz.  Every lane of the wavefront calls it.
 template <int V>
 __device__ __forceinline__ void explode_row(const StepArgs &a, const LanePts &xp, int i, int jb,
                                             int b, long it, float (&o1)[V], float (&o2)[V]) {

@@ -23,7 +23,7 @@
 #include <algorithm>
 
 #include "internal.h"
-#include "step.h"
+#include "colv.h"
 
 namespace inv {
 
@@ -395,6 +395,8 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
         r0 = a.rec_ptr[t];
         r1 = a.rec_ptr[t + 1];
     }
+    const LanePts ip = lane_pts(a.inj_cell, a.inj_row, p0, p1, lane);
+    const LanePts rp = lane_pts(a.rec_cell, a.rec_row, r0, r1, lane);
     const long it = a.it;
 
     // snapshot fields of this wave type: u at fu (and fu+1), v at fv (and fv+1)
@@ -427,7 +429,7 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
             if (vr >= VR) break;
             const int i = i0 - 1 + vr;
             const int e = vr * W + lane;
-            if (!(jvin && i >= 0 && i < nx)) {
+            if (!(i >= 0 && i < nx)) {  // wave-uniform
                 if (PSV) {
                     l_vx[e] = 0.0f;
                     l_vz[e] = 0.0f;
@@ -436,29 +438,30 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                 }
                 continue;
             }
+            // every lane computes the row (the point-list scans need every
+            // lane); lanes outside the grid store 0
+            const bool cin = jvin;
             const long kc = (long)i * nz + j;
             const long o = so + kc;
             // cells of this row the wavefront covers: point lists are scanned
             // with one wave-uniform test per point before the per-lane one
             const long rlo = (long)i * nz + (j0 - 1), rhi = rlo + 63;
             const bool xin = i >= 2 && i < nx - 2;
-            const bool own = jown && vr >= 1 && vr <= TX;
+            const bool own = cin && jown && vr >= 1 && vr <= TX;
             const int se = (vr + 2) * W + (lane + 2);
             const float ab = (a.abx[i] * abz) * abt;
             const float rho = qr[p];  // dt / rho in fast mode
             if (!PSV) {
                 float d = xin ? d_minus<FAST>(l_sxy, se, W, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
                 if (zin) d += d_minus<FAST>(l_szy, se, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
-                for (int pp = p0; pp < p1; pp++) {
-                    const int cp = a.inj_cell[pp];
-                    if (cp < rlo || cp > rhi) continue;
-                    if (cp == kc)
-                        d += a.inj_y[(long)b * a.inj_slot_stride + (long)a.inj_row[pp] * a.nt + it];
-                }
+                scan_pts(ip, a.inj_cell, a.inj_row, rlo, rhi, [&](int cp, int row) {
+                    if (cin && cp == kc)
+                        d += a.inj_y[(long)b * a.inj_slot_stride + (long)row * a.nt + it];
+                });
                 const float v = vel_update<FAST>(ab, qv1[p], g.dt, d, rho);
-                l_vy[e] = v;
+                l_vy[e] = cin ? v : 0.0f;
+                float u = qu1[p];
                 if (own) {
-                    float u = qu1[p];
                     if (a.snap_u) snap_put(a.snap_u, fmt, sb + kc, u, su1);
                     u += v * g.dt;
                     a.vy1[o] = v;
@@ -468,14 +471,12 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                         mu1 = fmaxf(mu1, fabsf(u));
                         mv1 = fmaxf(mv1, fabsf(v));
                     }
-                    for (int pp = r0; pp < r1; pp++) {
-                        const int cp = a.rec_cell[pp];
-                        if (cp < rlo || cp > rhi) continue;
-                        if (cp == kc)
-                            a.out_y[(long)b * a.out_slot_stride + (long)a.rec_row[pp] * a.nt + it] =
-                                a.rec_disp ? u : v;
-                    }
                 }
+                scan_pts(rp, a.rec_cell, a.rec_row, rlo, rhi, [&](int cp, int row) {
+                    if (own && cp == kc)
+                        a.out_y[(long)b * a.out_slot_stride + (long)row * a.nt + it] =
+                            a.rec_disp ? u : v;
+                });
             } else {
                 float dx_, dz_;
                 if (xin) {
@@ -489,22 +490,19 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                     dx_ += d_minus<FAST>(l_sxz, se, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
                     dz_ += d_minus<FAST>(l_szz, se, 1, g.c8dz, g.c24dz, g.a1z, g.a2z);
                 }
-                for (int pp = p0; pp < p1; pp++) {
-                    const int cp = a.inj_cell[pp];
-                    if (cp < rlo || cp > rhi) continue;
-                    if (cp == kc) {
-                        const long ks =
-                            (long)b * a.inj_slot_stride + (long)a.inj_row[pp] * a.nt + it;
+                scan_pts(ip, a.inj_cell, a.inj_row, rlo, rhi, [&](int cp, int row) {
+                    if (cin && cp == kc) {
+                        const long ks = (long)b * a.inj_slot_stride + (long)row * a.nt + it;
                         dx_ += a.inj_x[ks];
                         dz_ += a.inj_z[ks];
                     }
-                }
+                });
                 const float vx = vel_update<FAST>(ab, qv1[p], g.dt, dx_, rho);
                 const float vz = vel_update<FAST>(ab, qv2[p], g.dt, dz_, rho);
-                l_vx[e] = vx;
-                l_vz[e] = vz;
+                l_vx[e] = cin ? vx : 0.0f;
+                l_vz[e] = cin ? vz : 0.0f;
+                float ux = qu1[p], uz = qu2[p];
                 if (own) {
-                    float ux = qu1[p], uz = qu2[p];
                     if (a.snap_u) {
                         const long su = sb + (long)fu * g.N + kc;
                         snap_put(a.snap_u, fmt, su, ux, su1);
@@ -527,17 +525,14 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                         mv1 = fmaxf(mv1, fabsf(vx));
                         mv2 = fmaxf(mv2, fabsf(vz));
                     }
-                    for (int pp = r0; pp < r1; pp++) {
-                        const int cp = a.rec_cell[pp];
-                        if (cp < rlo || cp > rhi) continue;
-                        if (cp == kc) {
-                            const long kr =
-                                (long)b * a.out_slot_stride + (long)a.rec_row[pp] * a.nt + it;
-                            a.out_x[kr] = a.rec_disp ? ux : vx;
-                            a.out_z[kr] = a.rec_disp ? uz : vz;
-                        }
-                    }
                 }
+                scan_pts(rp, a.rec_cell, a.rec_row, rlo, rhi, [&](int cp, int row) {
+                    if (own && cp == kc) {
+                        const long kr = (long)b * a.out_slot_stride + (long)row * a.nt + it;
+                        a.out_x[kr] = a.rec_disp ? ux : vx;
+                        a.out_z[kr] = a.rec_disp ? uz : vz;
+                    }
+                });
             }
         }
     }
@@ -547,13 +542,16 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
     if (track && threadIdx.x == 0)
         track_commit<PSV>(l_track, NWAVE, a.acc_u, a.acc_v, b * a.nsnapf, fu, fv);
 
-    // 3. stress of step n+1, rows i0 .. i0+TX-1, columns j0 .. j0+CZ-1
-    if (!jtin) return;
+    // 3. stress of step n+1, rows i0 .. i0+TX-1, columns j0 .. j0+CZ-1 (the
+    // tile's explosions, one per lane, loaded while every lane is active)
     int q0 = 0, q1 = 0;
     if (PSV && a.sinj_ptr) {
         q0 = a.sinj_ptr[b * per + t];
         q1 = a.sinj_ptr[b * per + t + 1];
     }
+    const LanePts xp = lane_pts(a.sinj_cell, a.sinj_row, q0, q1, lane);
+    // every lane runs the rows (the explosion scan needs every lane); lanes
+    // with a column of this tile in the grid store
     {
         const int j = jt;
         const bool zin = j >= 1 && j < nz - 2;
@@ -571,8 +569,10 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
             if (!PSV) {
                 const float dx_ = xin ? d_plus<FAST>(l_vy, ve, W, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
                 const float dz_ = zin ? d_plus<FAST>(l_vy, ve, 1, g.c8dz, g.c24dz, g.a1z, g.a2z) : 0.0f;
-                a.sxy1[o] = l_sxy[se] + g.dt * mu * dx_;
-                a.szy1[o] = l_szy[se] + g.dt * mu * dz_;
+                if (jtin) {
+                    a.sxy1[o] = l_sxy[se] + g.dt * mu * dx_;
+                    a.szy1[o] = l_szy[se] + g.dt * mu * dz_;
+                }
             } else {
                 float xx, zx, xz, zz;
                 if (xin) {
@@ -593,16 +593,19 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                 float nxx = l_sxx[se] + g.dt * ((lam + 2 * mu) * xx + lam * zz);
                 float nzz = l_szz[se] + g.dt * ((lam + 2 * mu) * zz + lam * xx);
                 // explosive sources this tile owns: stf_y into sxx and szz
-                for (int pp = q0; pp < q1; pp++)
-                    if (a.sinj_cell[pp] == kc) {
-                        const float f =
-                            a.inj_y[(long)b * a.inj_slot_stride + (long)a.sinj_row[pp] * a.nt + it];
+                const long xlo = (long)i * nz + j0;
+                scan_pts(xp, a.sinj_cell, a.sinj_row, xlo, xlo + CZ - 1, [&](int cp, int row) {
+                    if (jtin && cp == kc) {
+                        const float f = a.inj_y[(long)b * a.inj_slot_stride + (long)row * a.nt + it];
                         nxx += f;
                         nzz += f;
                     }
-                a.sxx1[o] = nxx;
-                a.szz1[o] = nzz;
-                a.sxz1[o] = l_sxz[se] + g.dt * (mu * (xz + zx));
+                });
+                if (jtin) {
+                    a.sxx1[o] = nxx;
+                    a.szz1[o] = nzz;
+                    a.sxz1[o] = l_sxz[se] + g.dt * (mu * (xz + zx));
+                }
             }
         }
     }

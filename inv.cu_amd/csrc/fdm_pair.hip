@@ -179,9 +179,9 @@ __global__ __launch_bounds__(PBLOCK) void k_pair(const StepArgs a, const StepRol
                                      qv1[p], qv2[p], qr[p], ip, rlo, rhi, b, it, n1, n2);
             vstore<V>(&l_v1[(x + 4) * W + c0], n1);
             if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], n2);
-            if (x >= 0 && x < TX && cown && i < nx)
-                owned_row<PSV, V>(a, r1s, ss, fu, fv, sb, i, jb, it, b, rp, rlo, rhi, qu1[p],
-                                  qu2[p], qv1[p], qv2[p], n1, n2, mu1, mu2, mv1, mv2);
+            if (x >= 0 && x < TX && i < nx)  // wave-uniform
+                owned_row<PSV, V>(a, r1s, ss, fu, fv, sb, i, jb, cown, it, b, rp, rlo, rhi,
+                                  qu1[p], qu2[p], qv1[p], qv2[p], n1, n2, mu1, mu2, mv1, mv2);
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 qv1[p][v] = n1[v];
@@ -243,15 +243,17 @@ __global__ __launch_bounds__(PBLOCK) void k_pair(const StepArgs a, const StepRol
                                      qv1[p], qv2[p], qr[p], ip, rlo, rhi, b, it + 1, n1, n2);
             vstore<V>(&l_v1[(x + 4) * W + c0], n1);
             if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], n2);
-            if (x >= 0 && x < TX && cown && i < nx) {
-                owned_row<PSV, V>(a, r2, ss, fu, fv, sb, i, jb, it + 1, b, rp, rlo, rhi,
+            if (x >= 0 && x < TX && i < nx) {  // wave-uniform
+                owned_row<PSV, V>(a, r2, ss, fu, fv, sb, i, jb, cown, it + 1, b, rp, rlo, rhi,
                                   qu1[p], qu2[p], qv1[p], qv2[p], n1, n2, mu1, mu2, mv1, mv2);
-                const long o = so + (long)i * nz + jb;
-                vstore<V>(V1o + o, n1);
-                vstore<V>(U1 + o, qu1[p]);
-                if (PSV) {
-                    vstore<V>(V2o + o, n2);
-                    vstore<V>(U2 + o, qu2[p]);
+                if (cown) {
+                    const long o = so + (long)i * nz + jb;
+                    vstore<V>(V1o + o, n1);
+                    vstore<V>(U1 + o, qu1[p]);
+                    if (PSV) {
+                        vstore<V>(V2o + o, n2);
+                        vstore<V>(U2 + o, qu2[p]);
+                    }
                 }
             }
         }
@@ -261,8 +263,8 @@ __global__ __launch_bounds__(PBLOCK) void k_pair(const StepArgs a, const StepRol
     if ((r2.acc_u || r2.acc_v) && threadIdx.x == 0)
         track_commit<PSV>(l_track + 4 * NW, NW, r2.acc_u, r2.acc_v, mb, fu, fv);
 
-    // D. stress of step n+2 on the owned cells
-    if (!cown) return;
+    // D. stress of step n+2 on the owned cells (every lane runs the rows, for
+    // the explosion scan; owned lanes store)
 #pragma unroll
     for (int p = 0; p < P; p++) {
         const int x = wave - 4 + p * NW;
@@ -279,9 +281,11 @@ __global__ __launch_bounds__(PBLOCK) void k_pair(const StepArgs a, const StepRol
         stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, x + 4, c0, i, jb, sa, sc, sxz, qmu[p], qlam[p],
                                     o1, o2, o3);
         if (PSV) explode_row<V>(a, xp, i, jb, b, it + 1, o1, o2);
-        vstore<V>(S1o + o, o1);
-        vstore<V>(S2o + o, o2);
-        if (PSV) vstore<V>(S3o + o, o3);
+        if (cown) {
+            vstore<V>(S1o + o, o1);
+            vstore<V>(S2o + o, o2);
+            if (PSV) vstore<V>(S3o + o, o3);
+        }
     }
 }
 

@@ -65,7 +65,6 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
     float *U1 = PSV ? a.ux : a.uy, *U2 = a.uz;
     float *V1o = PSV ? a.vx1 : a.vy1, *V2o = a.vz1;
     float *S1o = PSV ? a.sxx1 : a.sxy1, *S2o = PSV ? a.szz1 : a.szy1, *S3o = a.sxz1;
-    float *O1 = PSV ? a.out_x : a.out_y, *O2 = a.out_z;
 
     // all global loads of the three phases first
     float s1[PS][V], s2[PS][V], s3[PS][V];
@@ -137,21 +136,10 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
     const LanePts ip = lane_pts(a.inj_cell, a.inj_row, p0, p1, lane);
     const LanePts rp = lane_pts(a.rec_cell, a.rec_row, r0, r1, lane);
     const long it = a.it;
-    const int fmt = a.snap_fmt;
     const int fu = PSV ? a.psv_f0 : 0, fv = PSV ? a.psv_f0 + 2 : 1;
     const long sb = (long)b * a.snap_slot_stride;
-    float su1 = 1.0f, su2 = 1.0f, sv1 = 1.0f, sv2 = 1.0f;
-    if (fmt != INV_SNAP_F32) {
-        const int mb = b * a.nsnapf;
-        if (a.snap_u) {
-            su1 = snap_enc_scale(a.mx_u[mb + fu]);
-            if (PSV) su2 = snap_enc_scale(a.mx_u[mb + fu + 1]);
-        }
-        if (a.snap_v) {
-            sv1 = snap_enc_scale(a.mx_v[mb + fv]);
-            if (PSV) sv2 = snap_enc_scale(a.mx_v[mb + fv + 1]);
-        }
-    }
+    const StepRoles r1s{a.snap_u, a.snap_v, a.v_after, a.mx_u, a.mx_v, a.acc_u, a.acc_v};
+    const SnapScales ss = snap_scales<PSV>(r1s, a.snap_fmt, b * a.nsnapf, fu, fv);
     const bool track = a.acc_u || a.acc_v;
     float mu1 = 0.0f, mu2 = 0.0f, mv1 = 0.0f, mv2 = 0.0f;
 
@@ -171,58 +159,19 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
                                  qv2[p], qr[p], ip, rlo, rhi, b, it, n1, n2);
         vstore<V>(&l_v1[vr * W + c0], n1);
         if (PSV) vstore<V>(&l_v2[vr * W + c0], n2);
-        // owned cells: v, u, snapshots, receivers
-        if (vr >= 1 && vr <= TX && cown && i < nx) {
-            const long o = so + (long)i * nz + jb;
-            float nu1[V], nu2[V];
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                const long kc = (long)i * nz + jb + v;
-                float u = qu1[p][v];
-                float uu = PSV ? qu2[p][v] : 0.0f;
-                if (a.snap_u) {
-                    snap_put(a.snap_u, fmt, sb + (long)fu * g.N + kc, u, su1);
-                    if (PSV) snap_put(a.snap_u, fmt, sb + (long)(fu + 1) * g.N + kc, uu, su2);
+        // owned cells: v, u, snapshots, receivers (the row test is
+        // wave-uniform; owned_row takes the lane's ownership)
+        if (vr >= 1 && vr <= TX && i < nx) {
+            owned_row<PSV, V>(a, r1s, ss, fu, fv, sb, i, jb, cown, it, b, rp, rlo, rhi, qu1[p],
+                              qu2[p], qv1[p], qv2[p], n1, n2, mu1, mu2, mv1, mv2);
+            if (cown) {
+                const long o = so + (long)i * nz + jb;
+                vstore<V>(V1o + o, n1);
+                vstore<V>(U1 + o, qu1[p]);
+                if (PSV) {
+                    vstore<V>(V2o + o, n2);
+                    vstore<V>(U2 + o, qu2[p]);
                 }
-                u += n1[v] * g.dt;
-                if (PSV) uu += n2[v] * g.dt;
-                nu1[v] = u;
-                nu2[v] = uu;
-                if (a.snap_v) {
-                    snap_put(a.snap_v, fmt, sb + (long)fv * g.N + kc,
-                             a.v_after ? n1[v] : qv1[p][v], sv1);
-                    if (PSV)
-                        snap_put(a.snap_v, fmt, sb + (long)(fv + 1) * g.N + kc,
-                                 a.v_after ? n2[v] : qv2[p][v], sv2);
-                }
-                if (track) {
-                    mu1 = fmaxf(mu1, fabsf(u));
-                    mv1 = fmaxf(mv1, fabsf(n1[v]));
-                    if (PSV) {
-                        mu2 = fmaxf(mu2, fabsf(uu));
-                        mv2 = fmaxf(mv2, fabsf(n2[v]));
-                    }
-                }
-            }
-            const long k0 = (long)i * nz + jb;
-            scan_pts(rp, a.rec_cell, a.rec_row, rlo, rhi, [&](int cp, int row) {
-                const long e = cp - k0;
-                if (e < 0 || e >= V) return;
-                float x = 0.0f, y = 0.0f;
-#pragma unroll
-                for (int v = 0; v < V; v++) {
-                    x = e == v ? (a.rec_disp ? nu1[v] : n1[v]) : x;
-                    if (PSV) y = e == v ? (a.rec_disp ? nu2[v] : n2[v]) : y;
-                }
-                const long kr = (long)b * a.out_slot_stride + (long)row * a.nt + it;
-                O1[kr] = x;
-                if (PSV) O2[kr] = y;
-            });
-            vstore<V>(V1o + o, n1);
-            vstore<V>(U1 + o, nu1);
-            if (PSV) {
-                vstore<V>(V2o + o, n2);
-                vstore<V>(U2 + o, nu2);
             }
         }
     }
@@ -233,14 +182,13 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         track_commit<PSV>(l_track, NW, a.acc_u, a.acc_v, b * a.nsnapf, fu, fv);
 
     // 3. stress of step n+1 on the owned cells (velocity LDS rows tr .. tr+3)
-    // the tile's explosions, one per lane: loaded while every lane is active
+    // the tile's explosions, one per lane
     int q0 = 0, q1 = 0;
     if (PSV && a.sinj_ptr) {
         q0 = a.sinj_ptr[b * per + t];
         q1 = a.sinj_ptr[b * per + t + 1];
     }
     const LanePts xp = lane_pts(a.sinj_cell, a.sinj_row, q0, q1, lane);
-    if (!cown) return;
 #pragma unroll
     for (int p = 0; p < PT; p++) {
         const int tr = wave + p * NW;
@@ -254,11 +202,14 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         if (PSV) vload<V>(&l_s3[(tr + 3) * W + c0], sxz);
         stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, tr + 1, c0, i, jb, sa, sc, sxz, qmu[p],
                                     qlam[p], o1, o2, o3);
-        // explosive sources this tile owns: stf_y into sxx and szz
+        // explosive sources this tile owns: stf_y into sxx and szz (every
+        // lane scans; owned lanes store)
         if (PSV) explode_row<V>(a, xp, i, jb, b, it, o1, o2);
-        vstore<V>(S1o + o, o1);
-        vstore<V>(S2o + o, o2);
-        if (PSV) vstore<V>(S3o + o, o3);
+        if (cown) {
+            vstore<V>(S1o + o, o1);
+            vstore<V>(S2o + o, o2);
+            if (PSV) vstore<V>(S3o + o, o3);
+        }
     }
 }
 

@@ -200,3 +200,31 @@ def test_pair_rejects_bad_tiles(gpu):
         ctx.set_pair(8, 4)
     ctx.set_pair(0, 2)  # off
     ctx.close()
+
+
+@pytest.mark.parametrize("math", [0, 1], ids=["exact", "fast"])
+@pytest.mark.parametrize("explosive", [False, True], ids=["force", "explosive"])
+def test_every_kernel_records_the_same(gpu, math, explosive):
+    """Traces, wavefields and snapshots of the one-column, wide and two-step
+    kernels agree bit for bit in both math modes.  Every tile here holds a
+    receiver or source as the first entry of its point list, which the
+    kernels keep per lane and read with lane reads: a scan from code where
+    that lane is inactive lost such entries."""
+    P = edge_problem(sh=1, psv=1, nt=300, explosive=explosive)
+    out = {}
+    for name, tile, pair in (("col8", (8, 1), None), ("col16", (16, 1), None),
+                             ("wide4x2", (4, 2), None), ("wide8x4", (8, 4), None),
+                             ("pair8x2", (8, 1), (8, 2)), ("pair4x4", (8, 1), (4, 4))):
+        ctx = P.context(batch=2)
+        ctx.set_math(math)
+        ctx.set_tile(*tile)
+        if pair:
+            ctx.set_pair(*pair)
+        ctx.forward(0, 2, snapshots=True)
+        out[name] = [ctx.traces_get(b) for b in range(2)] + \
+                    [ctx.state_get(b) for b in range(2)] + [ctx.snapshots_get(b) for b in range(2)]
+        ctx.close()
+    ref = out.pop("col8")
+    for name, got in out.items():
+        for k, (a, b) in enumerate(zip(got, ref)):
+            assert np.array_equal(a, b), f"{name} differs from col8 in array {k}"
