@@ -55,13 +55,15 @@ HBM_PEAK_GBPS = 8000.0
 PSV_STEP_BYTES = 4 * (2 * 7 + 3)
 # SH: the 4 wavefields (vy, uy, sxy, szy) read and written, mu and rho read
 SH_STEP_BYTES = 4 * (2 * 4 + 2)
-# HBM bytes per cell of one launch of the default step kernel (fused column
-# kernel, tile 58x8, fast math, P-SV), measured with rocprofv3 FETCH_SIZE and
-# WRITE_SIZE and scaled by a 256 MiB copy (fetch x2.0, write x1.0); see
-# profiles/k_col_psv_marmousi737x240_16shots.md.  Only reported for that
-# configuration.
-MEASURED_TRAFFIC_PER_CELL = 69.33
-PROFILED_TILE = (0, 8, 1, 1, 16)  # variant, rows, columns per lane, math, batch
+# HBM bytes per cell of one launch, measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE (per dispatch, each scaled by what the counter reads for a plain
+# device copy of known size: fetch x2.0, write x1.0), for the tiles the
+# autotuner picks on these workloads (workload, steps per launch, rows,
+# columns per lane, 16 shots per launch, fast math):
+#   c1 one-step 4 x 2: 69.14 B (algorithmic 68), profiles/k_colv_psv_marmousi737x240_16shots.md
+#   c4 two-step 8 x 2: 68.47 B per launch of two steps (algorithmic 68),
+#                      profiles/k_pair_psv_4096x2048_16shots.md
+MEASURED_TRAFFIC_PER_CELL = {("c1", 1, 4, 2): 69.14, ("c4", 2, 8, 2): 68.47}
 
 
 def parse():
@@ -404,10 +406,11 @@ def main():
     if launch_s:
         # each wavefield and model field is read and written once per launch
         achieved = step_bytes * cells / launch_s / 1e9
-        profiled = (a.workload == "c1" and not two and
-                    (tile["variant"], tile["rows"], tile["cols_per_lane"], a.math, B)
-                    == PROFILED_TILE and nfr)
-        traffic = round(MEASURED_TRAFFIC_PER_CELL * cells) if profiled else None
+        t = tile.get("two_step") or tile
+        key = (a.workload, per_launch_steps, t["rows"], t["cols_per_lane"])
+        measured = (MEASURED_TRAFFIC_PER_CELL.get(key)
+                    if tile["variant"] == 0 and a.math == 1 and B == 16 else None)
+        traffic = round(measured * cells) if measured else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "kernel": kernel, "alg_bytes_per_launch": step_bytes * cells,
@@ -445,8 +448,8 @@ def main():
 
     cpu = parity = None
     if not a.no_cpu:
-        # c0 in full; c1 in full on a sample of its shots; c2 / c4 on 4 shots x
-        # 2000 steps / 2 shots x 150 steps (10-30 s of CPU work each)
+        # c0 in full; c1 in full on a sample of its shots; c2 / c4 on 8 shots x
+        # 2000 steps / 2 shots x 100 steps (10-30 s of CPU work each)
         if a.workload == "c0":
             cpu, (fo, *ko) = cpu_baseline(P, P.nsrc, obs=gpu_grad[2])
             fg, kg, _ = gpu_grad
@@ -459,9 +462,9 @@ def main():
         elif a.workload == "c1":
             cpu = cpu_baseline(P, min(a.cpu_shots, P.nsrc))[0]
         elif a.workload == "c2":
-            cpu = cpu_baseline(P, 4, nt=2000)[0]
+            cpu = cpu_baseline(P, 8, nt=2000)[0]
         else:
-            cpu = cpu_baseline(P, 2, nt=150)[0]
+            cpu = cpu_baseline(P, 2, nt=100)[0]
     acc = accuracy(inv, a) if a.workload in ("c2", "c4") and not a.no_accuracy else None
     serving = None
     if a.serve_requests > 0 and a.workload == "c1":
