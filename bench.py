@@ -17,7 +17,8 @@ before the timed region.  Workloads (--workload):
                 snapshots; all 16 shots per launch.
   c2            SH, the same generator at 1474x480 and 6.25 m, 64 shots,
                 737 receivers, dt 0.5 ms, nt 8000, FP16 snapshots every 10
-                steps with 800-step checkpoints (recomputed in the adjoint);
+                steps with 4000-step checkpoints (the adjoint recomputes the first
+                half of the forward; one half's FP16 snapshots take 72 GB);
                 plus the error of FP32 / FP16 / BF16 snapshot storage against
                 the FP64 oracle on a 2-shot, 2000-step sample.
   c4            P-SV, the generator at 4096x2048 and 5 m, dt 0.4 ms,
@@ -203,10 +204,10 @@ def workload(inv, a):
         P = pr.marmousi(nx=1474, nz=480, dx=6.25, dt=5e-4, nt=a.nt or 8000, nsrc=64, nrec=737,
                         f0=8.0, t0=0.15, seed=2, sh=1, psv=0, sigma=a.sigma, src_depth=14,
                         rec_depth=14, name="marmousi-sh")
-        return (P, 64, H.INV_SNAP_F16, 800, SH_STEP_BYTES,
+        return (P, 64, H.INV_SNAP_F16, 4000, SH_STEP_BYTES,
                 f"batched SH gradient, Marmousi-shaped 1474x480 @ 6.25 m, 64 shots, "
                 f"737 seafloor receivers, dt 0.5 ms, nt {P.nt}, FP16 snapshots every 10 "
-                f"steps, 800-step checkpoints")
+                f"steps, 4000-step checkpoints")
     P = pr.marmousi(nx=4096, nz=2048, dx=5.0, dt=4e-4, nt=a.nt or 10000, nsrc=256, nrec=2048,
                     f0=12.0, t0=0.1, seed=4, sigma=a.sigma, name="marmousi-large")
     P = P.with_(misfit=1)
