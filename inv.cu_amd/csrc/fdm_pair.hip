@@ -9,9 +9,10 @@
 // the stress of step n+2 on the owned rows (and columns likewise; edge
 // columns of a phase hold values no later owned cell depends on).
 //
-// Row x = i - i0 of the tile is handled by wavefront x mod 4 in every phase,
-// so the registers of a row (velocity, u, rho, lambda, mu) stay with one
-// wavefront from the loads to the last store.  The stress buffer in LDS holds
+// Row x = i - i0 of the tile is handled by wavefront (x + 4) mod NW in every
+// phase (NW = 4 wavefronts per tile, 8 for the 16-row tile), so the
+// registers of a row (velocity, u, rho, lambda, mu) stay with one wavefront
+// from the loads to the last store.  The stress buffer in LDS holds
 // step n and is overwritten in place by step n+1 (each cell's update reads
 // its own old value and only velocity neighbours); the velocity buffer holds
 // step n+1 and is overwritten by step n+2 likewise.  Four barriers.
@@ -28,18 +29,16 @@
 
 namespace inv {
 
-constexpr int PBLOCK = 256;  // four wavefronts per tile
-
-template <bool SH, bool PSV, int TX, int V, bool FAST>
-__global__ __launch_bounds__(PBLOCK) void k_pair(const StepArgs a, const StepRoles r2) {
+template <bool SH, bool PSV, int TX, int V, int NW, bool FAST>
+__global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRoles r2) {
     static_assert(SH != PSV, "one wave type per launch");
     static_assert(V == 2 || V == 4, "2 or 4 columns per lane");
     static_assert(TX % 4 == 0, "tile rows a multiple of the wavefronts");
-    constexpr int HL = pair_halo(V), W = 64 * V, S = pair_strip(V), NW = PBLOCK / 64;
+    constexpr int HL = pair_halo(V), W = 64 * V, S = pair_strip(V);
     constexpr int SR = TX + 12;                 // stress rows x = -6 .. TX+5
     constexpr int VR = TX + 9;                  // velocity rows x = -4 .. TX+4
     constexpr int PS = (SR + NW - 1) / NW;      // stress rows loaded per wavefront
-    constexpr int P = (VR + NW - 1) / NW;       // rows x = wave - 4 + 4 p per wavefront
+    constexpr int P = (VR + NW - 1) / NW;       // rows x = wave - 4 + NW p per wavefront
     extern __shared__ __align__(16) float smem[];
     __shared__ float l_track[8 * NW];  // per-wavefront maxima of the two steps
     float *const l_s1 = smem;
@@ -290,12 +289,12 @@ __global__ __launch_bounds__(PBLOCK) void k_pair(const StepArgs a, const StepRol
 }
 
 template <class K>
-static void pair_go(Ctx &c, K kern, int lds, const StepArgs &a, const StepRoles &r2,
+static void pair_go(Ctx &c, K kern, int nw, int lds, const StepArgs &a, const StepRoles &r2,
                     unsigned blocks) {
     if (lds > 65536)
         INV_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-    kern<<<blocks, PBLOCK, lds, c.stream>>>(a, r2);
+    kern<<<blocks, 64 * nw, lds, c.stream>>>(a, r2);
 }
 
 template <bool SH, bool PSV, bool FAST>
@@ -312,14 +311,16 @@ void pair_launch(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned blocks
     }
     if (V == 2) {
         switch (rows) {
-            case 4: pair_go(c, k_pair<SH, PSV, 4, 2, FAST>, lds, a, r2, blocks); break;
-            case 12: pair_go(c, k_pair<SH, PSV, 12, 2, FAST>, lds, a, r2, blocks); break;
-            default: pair_go(c, k_pair<SH, PSV, 8, 2, FAST>, lds, a, r2, blocks); break;
+            case 4: pair_go(c, k_pair<SH, PSV, 4, 2, 4, FAST>, 4, lds, a, r2, blocks); break;
+            case 12: pair_go(c, k_pair<SH, PSV, 12, 2, 4, FAST>, 4, lds, a, r2, blocks); break;
+            // 16 rows: eight wavefronts, the same rows per wavefront as 8 x 2
+            case 16: pair_go(c, k_pair<SH, PSV, 16, 2, 8, FAST>, 8, lds, a, r2, blocks); break;
+            default: pair_go(c, k_pair<SH, PSV, 8, 2, 4, FAST>, 4, lds, a, r2, blocks); break;
         }
     } else {
         switch (rows) {
-            case 8: pair_go(c, k_pair<SH, PSV, 8, 4, FAST>, lds, a, r2, blocks); break;
-            default: pair_go(c, k_pair<SH, PSV, 4, 4, FAST>, lds, a, r2, blocks); break;
+            case 8: pair_go(c, k_pair<SH, PSV, 8, 4, 4, FAST>, 4, lds, a, r2, blocks); break;
+            default: pair_go(c, k_pair<SH, PSV, 4, 4, 4, FAST>, 4, lds, a, r2, blocks); break;
         }
     }
 }
