@@ -60,11 +60,14 @@ SH_STEP_BYTES = 4 * (2 * 4 + 2)
 # WRITE_SIZE (per dispatch, each scaled by what the counter reads for a plain
 # device copy of known size: fetch x2.0, write x1.0), for the tiles the
 # autotuner picks on these workloads (workload, steps per launch, rows,
-# columns per lane, 16 shots per launch, fast math):
-#   c1 one-step 4 x 2: 69.14 B (algorithmic 68), profiles/k_colv_psv_marmousi737x240_16shots.md
-#   c4 two-step 8 x 2: 68.47 B per launch of two steps (algorithmic 68),
-#                      profiles/k_pair_psv_4096x2048_16shots.md
-MEASURED_TRAFFIC_PER_CELL = {("c1", 1, 4, 2): 69.14, ("c4", 2, 8, 2): 68.47}
+# columns per lane, every shot of the workload in one launch, fast math):
+#   c1 one-step 4 x 2:   69.14 B (algorithmic 68), profiles/k_colv_psv_marmousi737x240_16shots.md
+#   c4 two-step 16 x 2:  69.63 B per launch of two steps (algorithmic 68),
+#                        profiles/k_pair_psv_4096x2048_16shots.md
+#   c2 two-step 16 x 2:  40.46 B per launch of two steps (algorithmic 40),
+#                        profiles/k_pair_sh_marmousi1474x480_64shots.md
+MEASURED_TRAFFIC_PER_CELL = {("c1", 1, 4, 2): 69.14, ("c4", 2, 16, 2): 69.63,
+                             ("c2", 2, 16, 2): 40.46}
 
 
 def parse():
@@ -410,7 +413,7 @@ def main():
         t = tile.get("two_step") or tile
         key = (a.workload, per_launch_steps, t["rows"], t["cols_per_lane"])
         measured = (MEASURED_TRAFFIC_PER_CELL.get(key)
-                    if tile["variant"] == 0 and a.math == 1 and B == 16 else None)
+                    if tile["variant"] == 0 and a.math == 1 and B == P.nsrc else None)
         traffic = round(measured * cells) if measured else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
