@@ -59,15 +59,16 @@ SH_STEP_BYTES = 4 * (2 * 4 + 2)
 # HBM bytes per cell of one launch, measured with rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (per dispatch, each scaled by what the counter reads for a plain
 # device copy of known size: fetch x2.0, write x1.0), for the tiles the
-# autotuner picks on these workloads (workload, steps per launch, rows,
-# columns per lane, every shot of the workload in one launch, fast math):
-#   c1 one-step 4 x 2:   69.14 B (algorithmic 68), profiles/k_colv_psv_marmousi737x240_16shots.md
-#   c4 two-step 16 x 2:  69.63 B per launch of two steps (algorithmic 68),
-#                        profiles/k_pair_psv_4096x2048_16shots.md
-#   c2 two-step 16 x 2:  40.46 B per launch of two steps (algorithmic 40),
-#                        profiles/k_pair_sh_marmousi1474x480_64shots.md
-MEASURED_TRAFFIC_PER_CELL = {("c1", 1, 4, 2): 69.14, ("c4", 2, 16, 2): 69.63,
-                             ("c2", 2, 16, 2): 40.46}
+# autotuner picks on these workloads, fast math; keyed by (workload, steps
+# per launch, rows, columns per lane, shots per launch):
+#   c1 one-step 4 x 2, 16 shots:  69.14 B (algorithmic 68),
+#                                 profiles/k_colv_psv_marmousi737x240_16shots.md
+#   c4 two-step 16 x 2, 4 shots:  71.40 B per launch of two steps (algorithmic 68),
+#                                 profiles/k_pair_psv_4096x2048_4shots.md
+#   c2 two-step 16 x 2, 64 shots: 40.46 B per launch of two steps (algorithmic 40),
+#                                 profiles/k_pair_sh_marmousi1474x480_64shots.md
+MEASURED_TRAFFIC_PER_CELL = {("c1", 1, 4, 2, 16): 69.14, ("c4", 2, 16, 2, 4): 71.40,
+                             ("c2", 2, 16, 2, 64): 40.46}
 
 
 def parse():
@@ -75,7 +76,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=16, help="shots per launch")
+    ap.add_argument("--batch", type=int, default=0, help="shots per launch (0: the workload's)")
+    ap.add_argument("--ckpt", type=int, default=-1,
+                    help="checkpoint interval in steps for 16-bit snapshots (-1: the workload's, "
+                         "0: none, every snapshot of the run resident)")
     ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c4"])
     ap.add_argument("--nt", type=int, default=0, help="time steps (0: the workload's)")
     ap.add_argument("--sigma", type=int, default=5, help="Gaussian filter sigma (cells)")
@@ -207,20 +211,24 @@ def workload(inv, a):
         P = pr.marmousi(nx=1474, nz=480, dx=6.25, dt=5e-4, nt=a.nt or 8000, nsrc=64, nrec=737,
                         f0=8.0, t0=0.15, seed=2, sh=1, psv=0, sigma=a.sigma, src_depth=14,
                         rec_depth=14, name="marmousi-sh")
-        return (P, 64, H.INV_SNAP_F16, 4000, SH_STEP_BYTES,
+        # every FP16 snapshot of the 64 shots stays in HBM (145 GB): no recomputation
+        return (P, 64, H.INV_SNAP_F16, 0, SH_STEP_BYTES,
                 f"batched SH gradient, Marmousi-shaped 1474x480 @ 6.25 m, 64 shots, "
-                f"737 seafloor receivers, dt 0.5 ms, nt {P.nt}, FP16 snapshots every 10 "
-                f"steps, 4000-step checkpoints")
+                f"737 seafloor receivers, {{batch}} per launch, dt 0.5 ms, nt {P.nt}, FP16 "
+                f"snapshots every 10 steps, {{ckpt}}")
     P = pr.marmousi(nx=4096, nz=2048, dx=5.0, dt=4e-4, nt=a.nt or 10000, nsrc=256, nrec=2048,
                     f0=12.0, t0=0.1, seed=4, sigma=a.sigma, name="marmousi-large")
     P = P.with_(misfit=1)
     pick = np.arange(0, P.nsrc, 16)  # 16 of the 256 shots, spread over the line
     P = P.with_(src_ix=P.src_ix[pick], src_iz=P.src_iz[pick], stf_x=P.stf_x[pick],
                 stf_y=P.stf_y[pick], stf_z=P.stf_z[pick])
-    return (P, 16, H.INV_SNAP_F16, 1000, PSV_STEP_BYTES,
-            f"batched P-SV gradient, 4096x2048 @ 5 m, 16 of 256 shots in one launch, 2048 "
-            f"receivers, dt 0.4 ms, nt {P.nt}, envelope misfit, FP16 snapshots every 10 steps, "
-            f"1000-step checkpoints")
+    # four shots per launch keep all of their FP16 snapshots in HBM (268 GB):
+    # no recomputation.  Sixteen per launch would need 1000-step checkpoints
+    # and recompute nine tenths of the forward (39.8 s against 29.0 s)
+    return (P, 4, H.INV_SNAP_F16, 0, PSV_STEP_BYTES,
+            f"batched P-SV gradient, 4096x2048 @ 5 m, 16 of 256 shots, {{batch}} per launch, "
+            f"2048 receivers, dt 0.4 ms, nt {P.nt}, envelope misfit, FP16 snapshots every 10 "
+            f"steps, {{ckpt}}")
 
 
 def accuracy(inv, a):
@@ -298,10 +306,13 @@ def main():
             print(json.dumps(s), flush=True)
         return
     P, B, snap_fmt, ckpt, step_bytes, desc = workload(inv, a)
+    if a.batch > 0:
+        B = min(a.batch, P.nsrc)
+    if a.ckpt >= 0:
+        ckpt = a.ckpt
     if ckpt and P.nt % ckpt:
         ckpt = P.nt  # a shortened run (--nt) the interval does not divide: one segment
-    if a.workload == "c1":
-        B = min(a.batch, P.nsrc)
+    desc = desc.format(batch=B, ckpt=f"{ckpt}-step checkpoints" if ckpt else "no checkpoints")
     print(f"# workload {a.workload} {P.name} {P.nx}x{P.nz} shots={P.nsrc} nt={P.nt} batch={B}",
           file=sys.stderr, flush=True)
     ctx = P.context(batch=B, model="true")
@@ -357,6 +368,7 @@ def main():
     roof = None
     cells = B * P.N
     two = "two_step" in tile
+    waves = "SH" if P.sh and not P.psv else "P-SV" if P.psv and not P.sh else "SH+P-SV"
     # a forward graph holds one kind of step launch when every pair of steps
     # is paired (no 16-bit snapshot scales, no checkpoints, nt even) or none
     uniform = not two or (snap_fmt == H.INV_SNAP_F32 and ckpt == 0 and P.nt % 2 == 0)
@@ -366,8 +378,8 @@ def main():
         # set-up fills is the step kernel, so time / launches is its duration
         per_launch_steps = 2 if two else 1
         launch_s = tfr / (nfr / per_launch_steps) / 1e3
-        kernel = ("two-step P-SV kernel k_pair, from forward-graph replays" if two else
-                  "fused P-SV time step (k_col / k_colv, autotuned tile), from forward-graph "
+        kernel = (f"two-step {waves} kernel k_pair, from forward-graph replays" if two else
+                  f"fused {waves} time step (k_col / k_colv, autotuned tile), from forward-graph "
                   "replays")
         avg["two_step_launch" if two else "step"] = round(launch_s * 1e6, 2)
         samples = {"forward_steps": nfr}
@@ -388,19 +400,19 @@ def main():
         if npr:
             per_launch_steps = 2
             launch_s = tpr / npr / 1e3
-            kernel = "two-step P-SV kernel k_pair, HIP events on direct launches"
+            kernel = f"two-step {waves} kernel k_pair, HIP events on direct launches"
             avg["two_step_launch"] = round(launch_s * 1e6, 2)
             if nf:
                 avg["one_step_launch"] = round(tf / nf * 1e3, 2)
             samples = {"two_step": npr, "one_step": nf}
     elif nf:
         launch_s = tf / nf / 1e3
-        kernel = "fused P-SV time step (k_col / k_colv)"
+        kernel = f"fused {waves} time step (k_col / k_colv)"
         avg["step"] = round(launch_s * 1e6, 2)
         samples = {"step": nf}
     elif nv and ns:
         launch_s = (tv / nv + ts / ns) / 1e3
-        kernel = "P-SV stencil step (velocity + stress launch pair)"
+        kernel = f"{waves} stencil step (velocity + stress launch pair)"
         avg = {"velocity": round(tv / nv * 1e3, 2), "stress": round(ts / ns * 1e3, 2)}
         samples = {"velocity": nv, "stress": ns}
     if nfr:
@@ -411,9 +423,9 @@ def main():
         # each wavefield and model field is read and written once per launch
         achieved = step_bytes * cells / launch_s / 1e9
         t = tile.get("two_step") or tile
-        key = (a.workload, per_launch_steps, t["rows"], t["cols_per_lane"])
+        key = (a.workload, per_launch_steps, t["rows"], t["cols_per_lane"], B)
         measured = (MEASURED_TRAFFIC_PER_CELL.get(key)
-                    if tile["variant"] == 0 and a.math == 1 and B == P.nsrc else None)
+                    if tile["variant"] == 0 and a.math == 1 else None)
         traffic = round(measured * cells) if measured else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
