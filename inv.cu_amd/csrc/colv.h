@@ -116,15 +116,19 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
         vload<V>(&ls1[r * W + c0], x2);
         vload<V>(&ls1[(r + 1) * W + c0], x3);
         window<V, W>(&ls2[r * W], c0, ez);
+        // every difference is evaluated and the out-of-range ones dropped
+        // by selects: branches here make the compiler sink each column's
+        // LDS reads into its own branch, splitting the vector reads into
+        // single words (twice the LDS instructions, two-way bank conflicts)
         float d[V];
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const int j = jb + v;
-            d[v] = xin ? dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, g.a1x, g.a2x)
-                       : 0.0f;
-            if (j >= 2 && j < nz - 2)
-                d[v] += dm<FAST>(ez[V + v], ez[V + v - 1], ez[V + v + 1], ez[V + v - 2], g.c8dz,
-                                 g.c24dz, g.a1z, g.a2z);
+            const float tx = dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+            const float tz = dm<FAST>(ez[V + v], ez[V + v - 1], ez[V + v + 1], ez[V + v - 2],
+                                      g.c8dz, g.c24dz, g.a1z, g.a2z);
+            d[v] = xin ? tx : 0.0f;
+            d[v] = (j >= 2 && j < nz - 2) ? d[v] + tz : d[v];
         }
         const long k0 = (long)i * nz + jb;
         scan_pts(ip, a.inj_cell, a.inj_row, rlo, rhi, [&](int cp, int row) {
@@ -156,19 +160,17 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const int j = jb + v;
-            if (xin) {
-                dx_[v] = dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
-                dz_[v] = dm<FAST>(z2[v], z1[v], z3[v], z0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
-            } else {
-                dx_[v] = 0.0f;
-                dz_[v] = 0.0f;
-            }
-            if (j >= 2 && j < nz - 2) {
-                dx_[v] += dm<FAST>(e3[V + v], e3[V + v - 1], e3[V + v + 1], e3[V + v - 2],
-                                   g.c8dz, g.c24dz, g.a1z, g.a2z);
-                dz_[v] += dm<FAST>(e2[V + v], e2[V + v - 1], e2[V + v + 1], e2[V + v - 2],
-                                   g.c8dz, g.c24dz, g.a1z, g.a2z);
-            }
+            const bool zin = j >= 2 && j < nz - 2;
+            const float xx = dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+            const float zx = dm<FAST>(z2[v], z1[v], z3[v], z0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+            const float xz = dm<FAST>(e3[V + v], e3[V + v - 1], e3[V + v + 1], e3[V + v - 2],
+                                      g.c8dz, g.c24dz, g.a1z, g.a2z);
+            const float zz = dm<FAST>(e2[V + v], e2[V + v - 1], e2[V + v + 1], e2[V + v - 2],
+                                      g.c8dz, g.c24dz, g.a1z, g.a2z);
+            dx_[v] = xin ? xx : 0.0f;
+            dz_[v] = xin ? zx : 0.0f;
+            dx_[v] = zin ? dx_[v] + xz : dx_[v];
+            dz_[v] = zin ? dz_[v] + zz : dz_[v];
         }
         const long k0 = (long)i * nz + jb;
         scan_pts(ip, a.inj_cell, a.inj_row, rlo, rhi, [&](int cp, int row) {
@@ -214,12 +216,12 @@ __device__ __forceinline__ void stress_row(const Geo &g, const float *lv1, const
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const int j = jb + v;
-            const float dx_ =
-                xin ? dp<FAST>(y2[v], y1[v], y3[v], y0[v], g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
-            const float dz_ = (j >= 1 && j < nz - 2)
-                                  ? dp<FAST>(e1[V + v + 1], e1[V + v], e1[V + v + 2],
-                                             e1[V + v - 1], g.c8dz, g.c24dz, g.a1z, g.a2z)
-                                  : 0.0f;
+            // differences evaluated unconditionally, as in vel_row
+            const float tx = dp<FAST>(y2[v], y1[v], y3[v], y0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+            const float tz = dp<FAST>(e1[V + v + 1], e1[V + v], e1[V + v + 2], e1[V + v - 1],
+                                      g.c8dz, g.c24dz, g.a1z, g.a2z);
+            const float dx_ = xin ? tx : 0.0f;
+            const float dz_ = (j >= 1 && j < nz - 2) ? tz : 0.0f;
             o1[v] = sa[v] + g.dt * mu[v] * dx_;
             o2[v] = sc[v] + g.dt * mu[v] * dz_;
         }
@@ -234,23 +236,14 @@ __device__ __forceinline__ void stress_row(const Geo &g, const float *lv1, const
         for (int v = 0; v < V; v++) {
             const int j = jb + v;
             const bool zin = j >= 1 && j < nz - 2;
-            float xx, zx, xz, zz;
-            if (xin) {
-                xx = dp<FAST>(y2[v], y1[v], y3[v], y0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
-                zx = dp<FAST>(w2[v], w1[v], w3[v], w0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
-            } else {
-                xx = 0.0f;
-                zx = 0.0f;
-            }
-            if (zin) {
-                xz = dp<FAST>(e1[V + v + 1], e1[V + v], e1[V + v + 2], e1[V + v - 1], g.c8dz,
-                              g.c24dz, g.a1z, g.a2z);
-                zz = dp<FAST>(e2[V + v + 1], e2[V + v], e2[V + v + 2], e2[V + v - 1], g.c8dz,
-                              g.c24dz, g.a1z, g.a2z);
-            } else {
-                xz = 0.0f;
-                zz = 0.0f;
-            }
+            const float txx = dp<FAST>(y2[v], y1[v], y3[v], y0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+            const float tzx = dp<FAST>(w2[v], w1[v], w3[v], w0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+            const float txz = dp<FAST>(e1[V + v + 1], e1[V + v], e1[V + v + 2], e1[V + v - 1],
+                                       g.c8dz, g.c24dz, g.a1z, g.a2z);
+            const float tzz = dp<FAST>(e2[V + v + 1], e2[V + v], e2[V + v + 2], e2[V + v - 1],
+                                       g.c8dz, g.c24dz, g.a1z, g.a2z);
+            const float xx = xin ? txx : 0.0f, zx = xin ? tzx : 0.0f;
+            const float xz = zin ? txz : 0.0f, zz = zin ? tzz : 0.0f;
             const float m = mu[v], l = lam[v];
             o1[v] = sa[v] + g.dt * ((l + 2 * m) * xx + l * zz);
             o2[v] = sc[v] + g.dt * ((l + 2 * m) * zz + l * xx);
