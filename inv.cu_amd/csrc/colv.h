@@ -72,10 +72,13 @@ __device__ __forceinline__ LanePts lane_pts(const int *cell, const int *row, int
     return t;
 }
 
-// f(cell, row) for the entries whose cell lies in rlo .. rhi, in list order
+// f(cell, row) for the entries whose cell lies in rlo .. rhi, in list order.
+// Cell indices are those of one shot's grid (int); callers compute what
+// only a matching entry needs inside f, so a tile without entries (most of
+// them) pays for nothing but the empty loop.
 template <class F>
 __device__ __forceinline__ void scan_pts(const LanePts &t, const int *cell, const int *row,
-                                         long rlo, long rhi, F &&f) {
+                                         int rlo, int rhi, F &&f) {
     const int n = t.p1 - t.p0, m = n < 64 ? n : 64;
     for (int q = 0; q < m; q++) {
         const int cp = __builtin_amdgcn_readlane(t.cell, q);
@@ -102,8 +105,8 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
                                         const float *ls3, int r, int c0, int i, int jb, bool rin,
                                         float abx, const float (&abz)[V], const float (&abt)[V],
                                         const float (&v1)[V], const float (&v2)[V],
-                                        const float (&rho)[V], const LanePts &ip, long rlo,
-                                        long rhi, int b, long it, float (&n1)[V],
+                                        const float (&rho)[V], const LanePts &ip, int rlo,
+                                        int rhi, int b, long it, float (&n1)[V],
                                         float (&n2)[V]) {
     const Geo &g = a.g;
     const int nx = g.nx, nz = g.nz;
@@ -130,9 +133,8 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
             d[v] = xin ? tx : 0.0f;
             d[v] = (j >= 2 && j < nz - 2) ? d[v] + tz : d[v];
         }
-        const long k0 = (long)i * nz + jb;
         scan_pts(ip, a.inj_cell, a.inj_row, rlo, rhi, [&](int cp, int row) {
-            const long e = cp - k0;
+            const int e = cp - (i * nz + jb);
             if (e < 0 || e >= V) return;
             const float f = I1[(long)b * a.inj_slot_stride + (long)row * a.nt + it];
             // a select per column keeps the array in registers
@@ -172,9 +174,8 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
             dx_[v] = zin ? dx_[v] + xz : dx_[v];
             dz_[v] = zin ? dz_[v] + zz : dz_[v];
         }
-        const long k0 = (long)i * nz + jb;
         scan_pts(ip, a.inj_cell, a.inj_row, rlo, rhi, [&](int cp, int row) {
-            const long e = cp - k0;
+            const int e = cp - (i * nz + jb);
             if (e < 0 || e >= V) return;
             const long ks = (long)b * a.inj_slot_stride + (long)row * a.nt + it;
             const float fx = I1[ks], fz = I2[ks];
@@ -286,7 +287,7 @@ template <bool PSV, int V>
 __device__ __forceinline__ void owned_row(const StepArgs &a, const StepRoles &ro,
                                           const SnapScales &ss, int fu, int fv, long sb, int i,
                                           int jb, bool act, long it, int b, const LanePts &rp,
-                                          long rlo, long rhi, float (&u1)[V], float (&u2)[V],
+                                          int rlo, int rhi, float (&u1)[V], float (&u2)[V],
                                           const float (&w1)[V], const float (&w2)[V],
                                           const float (&n1)[V], const float (&n2)[V],
                                           float &mu1, float &mu2, float &mv1, float &mv2) {
@@ -324,9 +325,8 @@ __device__ __forceinline__ void owned_row(const StepArgs &a, const StepRoles &ro
             }
         }
     }
-    const long k0 = (long)i * g.nz + jb;
     scan_pts(rp, a.rec_cell, a.rec_row, rlo, rhi, [&](int cp, int row) {
-        const long e = cp - k0;
+        const int e = cp - (i * g.nz + jb);
         if (!act || e < 0 || e >= V) return;
         float x = 0.0f, y = 0.0f;
 #pragma unroll
@@ -346,9 +346,9 @@ __device__ __forceinline__ void owned_row(const StepArgs &a, const StepRoles &ro
 template <int V>
 __device__ __forceinline__ void explode_row(const StepArgs &a, const LanePts &xp, int i, int jb,
                                             int b, long it, float (&o1)[V], float (&o2)[V]) {
-    const long k0 = (long)i * a.g.nz + jb;
-    scan_pts(xp, a.sinj_cell, a.sinj_row, k0 - jb, k0 - jb + a.g.nz - 1, [&](int cp, int row) {
-        const long e = cp - k0;
+    const int r0 = i * a.g.nz;
+    scan_pts(xp, a.sinj_cell, a.sinj_row, r0, r0 + a.g.nz - 1, [&](int cp, int row) {
+        const int e = cp - (r0 + jb);
         if (e < 0 || e >= V) return;
         const float f = a.inj_y[(long)b * a.inj_slot_stride + (long)row * a.nt + it];
 #pragma unroll

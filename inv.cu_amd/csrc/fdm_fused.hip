@@ -445,7 +445,7 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
             const long o = so + kc;
             // cells of this row the wavefront covers: point lists are scanned
             // with one wave-uniform test per point before the per-lane one
-            const long rlo = (long)i * nz + (j0 - 1), rhi = rlo + 63;
+            const int rlo = i * nz + (j0 - 1), rhi = rlo + 63;
             const bool xin = i >= 2 && i < nx - 2;
             const bool own = cin && jown && vr >= 1 && vr <= TX;
             const int se = (vr + 2) * W + (lane + 2);
@@ -593,7 +593,7 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                 float nxx = l_sxx[se] + g.dt * ((lam + 2 * mu) * xx + lam * zz);
                 float nzz = l_szz[se] + g.dt * ((lam + 2 * mu) * zz + lam * xx);
                 // explosive sources this tile owns: stf_y into sxx and szz
-                const long xlo = (long)i * nz + j0;
+                const int xlo = i * nz + j0;
                 scan_pts(xp, a.sinj_cell, a.sinj_row, xlo, xlo + CZ - 1, [&](int cp, int row) {
                     if (jtin && cp == kc) {
                         const float f = a.inj_y[(long)b * a.inj_slot_stride + (long)row * a.nt + it];

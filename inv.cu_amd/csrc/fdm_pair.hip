@@ -172,7 +172,7 @@ __global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRo
             const int i = i0 + x;
             const bool rin = jin && i >= 0 && i < nx;
             const float abx = a.abx[clampi(i, 0, nx - 1)];
-            const long rlo = (long)i * nz + (j0 - HL), rhi = rlo + W - 1;
+            const int rlo = i * nz + (j0 - HL), rhi = rlo + W - 1;
             float n1[V], n2[V];
             vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, c0, i, jb, rin, abx, abz, abt,
                                      qv1[p], qv2[p], qr[p], ip, rlo, rhi, b, it, n1, n2);
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRo
             const int i = i0 + x;
             const bool rin = jin && i >= 0 && i < nx;
             const float abx = a.abx[clampi(i, 0, nx - 1)];
-            const long rlo = (long)i * nz + (j0 - HL), rhi = rlo + W - 1;
+            const int rlo = i * nz + (j0 - HL), rhi = rlo + W - 1;
             float n1[V], n2[V];
             vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, c0, i, jb, rin, abx, abz, abt,
                                      qv1[p], qv2[p], qr[p], ip, rlo, rhi, b, it + 1, n1, n2);

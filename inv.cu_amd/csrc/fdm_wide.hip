@@ -153,7 +153,7 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         const float abx = a.abx[clampi(i, 0, nx - 1)];
         // cells of this row the wavefront covers: point lists are scanned with
         // one wave-uniform test per point before the per-lane one
-        const long rlo = (long)i * nz + (j0 - HL), rhi = rlo + W - 1;
+        const int rlo = i * nz + (j0 - HL), rhi = rlo + W - 1;
         float n1[V], n2[V];
         vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, vr + 2, c0, i, jb, rin, abx, abz, abt, qv1[p],
                                  qv2[p], qr[p], ip, rlo, rhi, b, it, n1, n2);
