@@ -24,13 +24,17 @@
 // and columns, explosions 3, receivers 0.  Snapshot roles of the first step
 // come in StepArgs, of the second in StepRoles; a role whose max-abs table is
 // filled by the first step must not be read by the second (the host never
-// pairs such steps).  Needs nz % V == 0 and dynamic LDS of pair_lds_bytes.
+// pairs such steps).  SNAP = false is the variant for launches with no
+// snapshot role in either step (most of them: every adjoint launch and three
+// of four forward ones with sfe = 10): the roles are compile-time null there,
+// so the snapshot code and the scalar registers of its pointers are gone.
+// Needs nz % V == 0 and dynamic LDS of pair_lds_bytes.
 #include "colv.h"
 
 namespace inv {
 
-template <bool SH, bool PSV, int TX, int V, int NW, bool FAST>
-__global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRoles r2) {
+template <bool SH, bool PSV, int TX, int V, int NW, bool FAST, bool SNAP>
+__global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRoles r2in) {
     static_assert(SH != PSV, "one wave type per launch");
     static_assert(V == 2 || V == 4, "2 or 4 columns per lane");
     static_assert(TX % 4 == 0, "tile rows a multiple of the wavefronts");
@@ -159,7 +163,10 @@ __global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRo
     const int fu = PSV ? a.psv_f0 : 0, fv = PSV ? a.psv_f0 + 2 : 1;
     const long sb = (long)b * a.snap_slot_stride;
     const int mb = b * a.nsnapf;
-    const StepRoles r1s{a.snap_u, a.snap_v, a.v_after, a.mx_u, a.mx_v, a.acc_u, a.acc_v};
+    const StepRoles none{nullptr, nullptr, 1, nullptr, nullptr, nullptr, nullptr};
+    const StepRoles r1s =
+        SNAP ? StepRoles{a.snap_u, a.snap_v, a.v_after, a.mx_u, a.mx_v, a.acc_u, a.acc_v} : none;
+    const StepRoles r2 = SNAP ? r2in : none;
     float mu1 = 0.0f, mu2 = 0.0f, mv1 = 0.0f, mv2 = 0.0f;
 
     // A. velocity of step n+1 on rows x = -4 .. TX+4 (LDS row x + 4)
@@ -299,6 +306,8 @@ static void pair_go(Ctx &c, K kern, int nw, int lds, const StepArgs &a, const St
 
 template <bool SH, bool PSV, bool FAST>
 void pair_launch(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned blocks) {
+    const bool snap = a.snap_u || a.snap_v || a.acc_u || a.acc_v || r2.snap_u || r2.snap_v ||
+                      r2.acc_u || r2.acc_v;
     const int rows = c.pair_tx, V = c.pair_v;
     const int lds = pair_lds_bytes(PSV, rows, V);
     if (lds > 65536) {
@@ -309,20 +318,24 @@ void pair_launch(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned blocks
         INV_REQUIRE(lds <= optin, INV_ERR_UNSUPPORTED,
                     "two-step tile needs more LDS per block than the device allows");
     }
+#define INV_PAIR(R, VV, NWW)                                                                     \
+    (snap ? pair_go(c, k_pair<SH, PSV, R, VV, NWW, FAST, true>, NWW, lds, a, r2, blocks)          \
+          : pair_go(c, k_pair<SH, PSV, R, VV, NWW, FAST, false>, NWW, lds, a, r2, blocks))
     if (V == 2) {
         switch (rows) {
-            case 4: pair_go(c, k_pair<SH, PSV, 4, 2, 4, FAST>, 4, lds, a, r2, blocks); break;
-            case 12: pair_go(c, k_pair<SH, PSV, 12, 2, 4, FAST>, 4, lds, a, r2, blocks); break;
+            case 4: INV_PAIR(4, 2, 4); break;
+            case 12: INV_PAIR(12, 2, 4); break;
             // 16 rows: eight wavefronts, the same rows per wavefront as 8 x 2
-            case 16: pair_go(c, k_pair<SH, PSV, 16, 2, 8, FAST>, 8, lds, a, r2, blocks); break;
-            default: pair_go(c, k_pair<SH, PSV, 8, 2, 4, FAST>, 4, lds, a, r2, blocks); break;
+            case 16: INV_PAIR(16, 2, 8); break;
+            default: INV_PAIR(8, 2, 4); break;
         }
     } else {
         switch (rows) {
-            case 8: pair_go(c, k_pair<SH, PSV, 8, 4, 4, FAST>, 4, lds, a, r2, blocks); break;
-            default: pair_go(c, k_pair<SH, PSV, 4, 4, 4, FAST>, 4, lds, a, r2, blocks); break;
+            case 8: INV_PAIR(8, 4, 4); break;
+            default: INV_PAIR(4, 4, 4); break;
         }
     }
+#undef INV_PAIR
 }
 
 template void pair_launch<true, false, true>(Ctx &, const StepArgs &, const StepRoles &, unsigned);
