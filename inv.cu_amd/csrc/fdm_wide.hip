@@ -13,14 +13,18 @@
 // of step n+1 is computed on the tile plus halo from LDS (owned cells also
 // update u, snapshots and receivers) and goes to LDS; the stress of step n+1
 // is computed on the owned cells.  Arithmetic is expression for expression
-// that of k_col, so results are bitwise identical to it.  Needs nz % V == 0.
+// that of k_col, so results are bitwise identical to it.  SNAP = false is the
+// variant for steps without snapshot roles (with FP32 snapshots every sfe,
+// nine forward steps in ten and every adjoint step): the roles are
+// compile-time null there and their code and registers are gone.  Needs
+// nz % V == 0.
 #include "colv.h"
 
 namespace inv {
 
 constexpr int WBLOCK = 256;  // four wavefronts per tile
 
-template <bool SH, bool PSV, int TX, int V, bool FAST>
+template <bool SH, bool PSV, int TX, int V, bool FAST, bool SNAP>
 __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
     static_assert(SH != PSV, "one wave type per launch");
     static_assert(V == 2 || V == 4, "2 or 4 columns per lane");
@@ -138,9 +142,11 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
     const long it = a.it;
     const int fu = PSV ? a.psv_f0 : 0, fv = PSV ? a.psv_f0 + 2 : 1;
     const long sb = (long)b * a.snap_slot_stride;
-    const StepRoles r1s{a.snap_u, a.snap_v, a.v_after, a.mx_u, a.mx_v, a.acc_u, a.acc_v};
+    const StepRoles r1s =
+        SNAP ? StepRoles{a.snap_u, a.snap_v, a.v_after, a.mx_u, a.mx_v, a.acc_u, a.acc_v}
+             : StepRoles{nullptr, nullptr, 1, nullptr, nullptr, nullptr, nullptr};
     const SnapScales ss = snap_scales<PSV>(r1s, a.snap_fmt, b * a.nsnapf, fu, fv);
-    const bool track = a.acc_u || a.acc_v;
+    const bool track = r1s.acc_u || r1s.acc_v;
     float mu1 = 0.0f, mu2 = 0.0f, mv1 = 0.0f, mv2 = 0.0f;
 
     // 2. velocity of step n+1 on rows i0-1 .. i0+TX+1 (stress LDS rows vr .. vr+3)
@@ -179,7 +185,7 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
     if (track) track_stage<PSV>(l_track, wave, lane, mu1, mu2, mv1, mv2);
     __syncthreads();
     if (track && threadIdx.x == 0)
-        track_commit<PSV>(l_track, NW, a.acc_u, a.acc_v, b * a.nsnapf, fu, fv);
+        track_commit<PSV>(l_track, NW, r1s.acc_u, r1s.acc_v, b * a.nsnapf, fu, fv);
 
     // 3. stress of step n+1 on the owned cells (velocity LDS rows tr .. tr+3)
     // the tile's explosions, one per lane
@@ -215,18 +221,23 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
 
 template <bool SH, bool PSV, bool FAST>
 void wide_launch(Ctx &c, const StepArgs &a, unsigned blocks) {
+    const bool snap = a.snap_u || a.snap_v || a.acc_u || a.acc_v;
+#define INV_COLV(R, VV)                                                                          \
+    (snap ? k_colv<SH, PSV, R, VV, FAST, true><<<blocks, WBLOCK, 0, c.stream>>>(a)                \
+          : k_colv<SH, PSV, R, VV, FAST, false><<<blocks, WBLOCK, 0, c.stream>>>(a))
     if (c.col_v == 2) {
         switch (c.tile_x) {
-            case 4: k_colv<SH, PSV, 4, 2, FAST><<<blocks, WBLOCK, 0, c.stream>>>(a); break;
-            case 8: k_colv<SH, PSV, 8, 2, FAST><<<blocks, WBLOCK, 0, c.stream>>>(a); break;
-            default: k_colv<SH, PSV, 16, 2, FAST><<<blocks, WBLOCK, 0, c.stream>>>(a); break;
+            case 4: INV_COLV(4, 2); break;
+            case 8: INV_COLV(8, 2); break;
+            default: INV_COLV(16, 2); break;
         }
     } else {
         switch (c.tile_x) {
-            case 8: k_colv<SH, PSV, 8, 4, FAST><<<blocks, WBLOCK, 0, c.stream>>>(a); break;
-            default: k_colv<SH, PSV, 4, 4, FAST><<<blocks, WBLOCK, 0, c.stream>>>(a); break;
+            case 8: INV_COLV(8, 4); break;
+            default: INV_COLV(4, 4); break;
         }
     }
+#undef INV_COLV
 }
 
 template void wide_launch<true, false, true>(Ctx &, const StepArgs &, unsigned);
