@@ -28,12 +28,13 @@
 // snapshot role in either step (most of them: every adjoint launch and three
 // of four forward ones with sfe = 10): the roles are compile-time null there,
 // so the snapshot code and the scalar registers of its pointers are gone.
+// REC = false likewise drops receiver recording (adjoint and recomputation).
 // Needs nz % V == 0 and dynamic LDS of pair_lds_bytes.
 #include "colv.h"
 
 namespace inv {
 
-template <bool SH, bool PSV, int TX, int V, int NW, bool FAST, bool SNAP>
+template <bool SH, bool PSV, int TX, int V, int NW, bool FAST, bool SNAP, bool REC>
 __global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRoles r2in) {
     static_assert(SH != PSV, "one wave type per launch");
     static_assert(V == 2 || V == 4, "2 or 4 columns per lane");
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRo
         p1 = a.inj_ptr[set * per + t + 1];
     }
     int r0 = 0, r1 = 0;
-    if (a.rec_ptr) {
+    if (REC && a.rec_ptr) {
         r0 = a.rec_ptr[t];
         r1 = a.rec_ptr[t + 1];
     }
@@ -308,6 +309,7 @@ template <bool SH, bool PSV, bool FAST>
 void pair_launch(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned blocks) {
     const bool snap = a.snap_u || a.snap_v || a.acc_u || a.acc_v || r2.snap_u || r2.snap_v ||
                       r2.acc_u || r2.acc_v;
+    const int var = (snap ? 2 : 0) + (a.rec_ptr ? 1 : 0);
     const int rows = c.pair_tx, V = c.pair_v;
     const int lds = pair_lds_bytes(PSV, rows, V);
     if (lds > 65536) {
@@ -319,8 +321,10 @@ void pair_launch(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned blocks
                     "two-step tile needs more LDS per block than the device allows");
     }
 #define INV_PAIR(R, VV, NWW)                                                                     \
-    (snap ? pair_go(c, k_pair<SH, PSV, R, VV, NWW, FAST, true>, NWW, lds, a, r2, blocks)          \
-          : pair_go(c, k_pair<SH, PSV, R, VV, NWW, FAST, false>, NWW, lds, a, r2, blocks))
+    (var == 3   ? pair_go(c, k_pair<SH, PSV, R, VV, NWW, FAST, true, true>, NWW, lds, a, r2, blocks)  \
+     : var == 2 ? pair_go(c, k_pair<SH, PSV, R, VV, NWW, FAST, true, false>, NWW, lds, a, r2, blocks) \
+     : var == 1 ? pair_go(c, k_pair<SH, PSV, R, VV, NWW, FAST, false, true>, NWW, lds, a, r2, blocks) \
+                : pair_go(c, k_pair<SH, PSV, R, VV, NWW, FAST, false, false>, NWW, lds, a, r2, blocks))
     if (V == 2) {
         switch (rows) {
             case 4: INV_PAIR(4, 2, 4); break;
