@@ -16,7 +16,8 @@
 // that of k_col, so results are bitwise identical to it.  SNAP = false is the
 // variant for steps without snapshot roles (with FP32 snapshots every sfe,
 // nine forward steps in ten and every adjoint step): the roles are
-// compile-time null there and their code and registers are gone.  Needs
+// compile-time null there and their code and registers are gone; REC =
+// false likewise drops receiver recording (adjoint, recomputation).  Needs
 // nz % V == 0.
 #include "colv.h"
 
@@ -24,7 +25,7 @@ namespace inv {
 
 constexpr int WBLOCK = 256;  // four wavefronts per tile
 
-template <bool SH, bool PSV, int TX, int V, bool FAST, bool SNAP>
+template <bool SH, bool PSV, int TX, int V, bool FAST, bool SNAP, bool REC>
 __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
     static_assert(SH != PSV, "one wave type per launch");
     static_assert(V == 2 || V == 4, "2 or 4 columns per lane");
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         p1 = a.inj_ptr[set * per + t + 1];
     }
     int r0 = 0, r1 = 0;
-    if (a.rec_ptr) {
+    if (REC && a.rec_ptr) {
         r0 = a.rec_ptr[t];
         r1 = a.rec_ptr[t + 1];
     }
@@ -222,9 +223,12 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
 template <bool SH, bool PSV, bool FAST>
 void wide_launch(Ctx &c, const StepArgs &a, unsigned blocks) {
     const bool snap = a.snap_u || a.snap_v || a.acc_u || a.acc_v;
+    const int var = (snap ? 2 : 0) + (a.rec_ptr ? 1 : 0);
 #define INV_COLV(R, VV)                                                                          \
-    (snap ? k_colv<SH, PSV, R, VV, FAST, true><<<blocks, WBLOCK, 0, c.stream>>>(a)                \
-          : k_colv<SH, PSV, R, VV, FAST, false><<<blocks, WBLOCK, 0, c.stream>>>(a))
+    (var == 3   ? k_colv<SH, PSV, R, VV, FAST, true, true><<<blocks, WBLOCK, 0, c.stream>>>(a)    \
+     : var == 2 ? k_colv<SH, PSV, R, VV, FAST, true, false><<<blocks, WBLOCK, 0, c.stream>>>(a)   \
+     : var == 1 ? k_colv<SH, PSV, R, VV, FAST, false, true><<<blocks, WBLOCK, 0, c.stream>>>(a)   \
+                : k_colv<SH, PSV, R, VV, FAST, false, false><<<blocks, WBLOCK, 0, c.stream>>>(a))
     if (c.col_v == 2) {
         switch (c.tile_x) {
             case 4: INV_COLV(4, 2); break;
