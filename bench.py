@@ -59,17 +59,18 @@ SH_STEP_BYTES = 4 * (2 * 4 + 2)
 # HBM bytes per cell of one launch, measured with rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (per dispatch, each scaled by what the counter reads for a plain
 # device copy of known size: fetch x2.0, write x1.0), for the tiles the
-# autotuner picks on these workloads, fast math, the variants for launches
-# without snapshot roles (most launches); keyed by (workload, steps per
-# launch, rows, columns per lane, shots per launch):
-#   c1 one-step 4 x 2, 16 shots:  68.61 B (algorithmic 68),
+# autotuner picks on these workloads, fast math, the forward variants
+# without snapshot roles (the launches the live timing samples most);
+# keyed by (workload, steps per launch, rows, columns per lane, shots per
+# launch):
+#   c1 one-step 4 x 2, 16 shots:  68.30 B (algorithmic 68),
 #                                 profiles/k_colv_psv_marmousi737x240_16shots.md
-#   c4 two-step 16 x 2, 4 shots:  70.91 B per launch of two steps (algorithmic 68),
+#   c4 two-step 16 x 2, 4 shots:  70.71 B per launch of two steps (algorithmic 68),
 #                                 profiles/k_pair_psv_4096x2048_4shots.md
-#   c2 two-step 16 x 2, 64 shots: 40.33 B per launch of two steps (algorithmic 40),
+#   c2 two-step 16 x 2, 64 shots: 40.17 B per launch of two steps (algorithmic 40),
 #                                 profiles/k_pair_sh_marmousi1474x480_64shots.md
-MEASURED_TRAFFIC_PER_CELL = {("c1", 1, 4, 2, 16): 68.61, ("c4", 2, 16, 2, 4): 70.91,
-                             ("c2", 2, 16, 2, 64): 40.33}
+MEASURED_TRAFFIC_PER_CELL = {("c1", 1, 4, 2, 16): 68.30, ("c4", 2, 16, 2, 4): 70.71,
+                             ("c2", 2, 16, 2, 64): 40.17}
 
 
 def parse():
