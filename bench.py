@@ -73,7 +73,7 @@ MEASURED_TRAFFIC_PER_CELL = {("c1", 1, 4, 2, 16): 68.30, ("c4", 2, 16, 2, 4): 70
                              ("c2", 2, 16, 2, 64): 40.17}
 
 
-def parse():
+def parse(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
@@ -111,7 +111,7 @@ def parse():
     ap.add_argument("--serving-sweep", type=int, default=0,
                     help="run only the serving sweep over this many requests (one JSON line "
                          "per run) and exit")
-    return ap.parse_args()
+    return ap.parse_args(argv)
 
 
 def serving_leg(inv, n, rate_x, snap_format):
@@ -292,6 +292,19 @@ def accuracy(inv, a):
     return out
 
 
+def launch_plan(a, P, B, ckpt, desc):
+    """Shots per launch, checkpoint interval and the workload description
+    after the --batch / --ckpt overrides."""
+    if a.batch > 0:
+        B = min(a.batch, P.nsrc)
+    if a.ckpt >= 0:
+        ckpt = a.ckpt
+    if ckpt and P.nt % ckpt:
+        ckpt = P.nt  # a shortened run (--nt) the interval does not divide: one segment
+    desc = desc.format(batch=B, ckpt=f"{ckpt}-step checkpoints" if ckpt else "no checkpoints")
+    return B, ckpt, desc
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -308,13 +321,7 @@ def main():
             print(json.dumps(s), flush=True)
         return
     P, B, snap_fmt, ckpt, step_bytes, desc = workload(inv, a)
-    if a.batch > 0:
-        B = min(a.batch, P.nsrc)
-    if a.ckpt >= 0:
-        ckpt = a.ckpt
-    if ckpt and P.nt % ckpt:
-        ckpt = P.nt  # a shortened run (--nt) the interval does not divide: one segment
-    desc = desc.format(batch=B, ckpt=f"{ckpt}-step checkpoints" if ckpt else "no checkpoints")
+    B, ckpt, desc = launch_plan(a, P, B, ckpt, desc)
     print(f"# workload {a.workload} {P.name} {P.nx}x{P.nz} shots={P.nsrc} nt={P.nt} batch={B}",
           file=sys.stderr, flush=True)
     ctx = P.context(batch=B, model="true")
