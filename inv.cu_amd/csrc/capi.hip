@@ -369,7 +369,9 @@ int inv_grid_set(inv_ctx *ctx, const inv_grid_desc *d) {
         INV_REQUIRE(d->nt >= 1 && d->dt > 0 && d->dx > 0 && d->dz > 0, INV_ERR_ARG,
                     "bad nt/dt/dx/dz");
         INV_REQUIRE(d->batch >= 1, INV_ERR_ARG, "batch must be >= 1");
-        INV_REQUIRE((long)d->nx * d->nz < (1L << 31), INV_ERR_ARG, "grid too large");
+        // cell indices (point lists, kernels' row spans, which reach 20 rows
+        // past the grid) are 32-bit
+        INV_REQUIRE((long)(d->nx + 32) * d->nz < (1L << 31), INV_ERR_ARG, "grid too large");
         if (d->adj) {
             // nt % sfe != 0 makes the reference's adjoint read past the last
             // snapshot slot (solver/fdm.h:583-585); reject it here.
