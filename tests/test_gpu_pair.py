@@ -229,3 +229,44 @@ def test_every_kernel_records_the_same(gpu, math, explosive):
     for name, got in out.items():
         for k, (a, b) in enumerate(zip(got, ref)):
             assert np.array_equal(a, b), f"{name} differs from col8 in array {k}"
+
+
+def _run(P, tile, pair, fmt, ckpt, obs=None, snapshots=True):
+    ctx = P.context(batch=2)
+    ctx.set_snapshots(fmt, ckpt)
+    ctx.set_tile(*tile)
+    if pair:
+        ctx.set_pair(*pair)
+    if obs is None:
+        ctx.forward(0, 2, snapshots=snapshots)
+        out = [ctx.traces_get(b) for b in range(2)] + [ctx.state_get(b) for b in range(2)]
+    else:
+        for s in range(P.nsrc):
+            ctx.obs_set(s, obs[s])
+        out = [ctx.gradient(misfit_type=0, sigma=0)] + list(ctx.kernels_get())
+    ctx.close()
+    return out
+
+
+@pytest.mark.parametrize("tile,pair", [((8, 2), None), ((8, 1), (16, 2))],
+                         ids=["wide8x2", "pair16x2"])
+def test_kernel_variants_agree(gpu, tile, pair):
+    """The step kernels are compiled with and without snapshot roles and
+    receiver recording, picked per launch.  A forward that stores snapshots
+    gives the same traces and wavefields as one that does not (FP32 and
+    FP16), and a gradient whose adjoint recomputes checkpoint segments
+    (launches without recording) equals the one that keeps every snapshot."""
+    P = edge_problem(sh=1, psv=1, nt=300, explosive=True)
+    print(f"variants {tile} {pair}", flush=True)
+    for fmt in (H.INV_SNAP_F32, H.INV_SNAP_F16):
+        a = _run(P, tile, pair, fmt, 0, snapshots=True)
+        b = _run(P, tile, pair, fmt, 0, snapshots=False)
+        for k, (x, y) in enumerate(zip(a, b)):
+            assert np.array_equal(x, y), f"fmt {fmt}: array {k} depends on snapshot storage"
+    obs = _obs(P)
+    whole = _run(P, tile, pair, H.INV_SNAP_F16, 0, obs=obs)
+    ckpt = _run(P, tile, pair, H.INV_SNAP_F16, 60, obs=obs)
+    assert whole[0] == ckpt[0]
+    for x, y in zip(whole[1:], ckpt[1:]):
+        assert np.abs(x).max() > 0
+        assert np.array_equal(x, y)
