@@ -95,14 +95,16 @@ __device__ __forceinline__ void scan_pts(const LanePts &t, const int *cell, cons
 // Velocity of step it+1 (divS, addSTF, updateV) on the lane's V columns
 // jb .. jb+V-1 of grid row i.  ls1/ls2/ls3: stress of step it in LDS rows of
 // W columns (sxx, szz, sxz for P-SV; sxy, szy for SH), grid row i at LDS row
-// r, rows r-2 .. r+1 read.  v1/v2: the lane's velocity of step it, rho: rho
+// r of ls1 and ls3 (rows r-2 .. r+1 read) and at row r2 of ls2 (only that row
+// is read: szz and szy enter through z-differences alone).  v1/v2: the lane's velocity of step it, rho: rho
 // or dt/rho (fast math).  Point forces of step it from the list entries
 // p0 .. p1 whose cell lies in grid cells rlo .. rhi (this row's span of the
 // tile, held per lane in ip) are added in list order after the derivatives.
 // Cells outside the grid (rin false) give 0.
 template <bool PSV, int V, int W, bool FAST>
 __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, const float *ls2,
-                                        const float *ls3, int r, int c0, int i, int jb, bool rin,
+                                        const float *ls3, int r, int r2, int c0, int i, int jb,
+                                        bool rin,
                                         float abx, const float (&abz)[V], const float (&abt)[V],
                                         const float (&v1)[V], const float (&v2)[V],
                                         const float (&rho)[V], const LanePts &ip, int rlo,
@@ -118,7 +120,7 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
         vload<V>(&ls1[(r - 1) * W + c0], x1);
         vload<V>(&ls1[r * W + c0], x2);
         vload<V>(&ls1[(r + 1) * W + c0], x3);
-        window<V, W>(&ls2[r * W], c0, ez);
+        window<V, W>(&ls2[r2 * W], c0, ez);
         // every difference is evaluated and the out-of-range ones dropped
         // by selects: branches here make the compiler sink each column's
         // LDS reads into its own branch, splitting the vector reads into
@@ -157,7 +159,7 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
         vload<V>(&ls3[r * W + c0], z2);
         vload<V>(&ls3[(r + 1) * W + c0], z3);
         window<V, W>(&ls3[r * W], c0, e3);
-        window<V, W>(&ls2[r * W], c0, e2);
+        window<V, W>(&ls2[r2 * W], c0, e2);
         float dx_[V], dz_[V];
 #pragma unroll
         for (int v = 0; v < V; v++) {

@@ -42,14 +42,16 @@ __global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRo
     constexpr int HL = pair_halo(V), W = 64 * V, S = pair_strip(V);
     constexpr int SR = TX + 12;                 // stress rows x = -6 .. TX+5
     constexpr int VR = TX + 9;                  // velocity rows x = -4 .. TX+4
+    // szz / szy enter the velocity only through z-differences within a row,
+    // so their buffer holds the velocity rows x = -4 .. TX+4 (LDS row x + 4)
     constexpr int PS = (SR + NW - 1) / NW;      // stress rows loaded per wavefront
     constexpr int P = (VR + NW - 1) / NW;       // rows x = wave - 4 + NW p per wavefront
     extern __shared__ __align__(16) float smem[];
     __shared__ float l_track[8 * NW];  // per-wavefront maxima of the two steps
     float *const l_s1 = smem;
     float *const l_s2 = l_s1 + SR * W;
-    float *const l_s3 = PSV ? l_s2 + SR * W : l_s2;  // unused for SH
-    float *const l_v1 = (PSV ? l_s3 : l_s2) + SR * W;
+    float *const l_s3 = PSV ? l_s2 + VR * W : l_s2;  // unused for SH
+    float *const l_v1 = PSV ? l_s3 + SR * W : l_s2 + VR * W;
     float *const l_v2 = PSV ? l_v1 + VR * W : l_v1;  // unused for SH
 
     // XCD-aware tile order (as k_col)
@@ -94,7 +96,7 @@ __global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRo
         const int r = wave + p * NW;
         const long k = so + (long)clampi(i0 - 6 + r, 0, nx - 1) * nz + jl;
         vload<V>(S1 + k, s1[p]);
-        vload<V>(S2 + k, s2[p]);
+        if (r >= 2 && r < SR - 1) vload<V>(S2 + k, s2[p]);  // wave-uniform
         if (PSV) vload<V>(S3 + k, s3[p]);
     }
     float qv1[P][V], qv2[P][V], qr[P][V], qu1[P][V], qu2[P][V], qmu[P][V], qlam[P][V];
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRo
                 }
             }
             vstore<V>(&l_s1[r * W + c0], s1[p]);
-            vstore<V>(&l_s2[r * W + c0], s2[p]);
+            if (r >= 2 && r < SR - 1) vstore<V>(&l_s2[(r - 2) * W + c0], s2[p]);
             if (PSV) vstore<V>(&l_s3[r * W + c0], s3[p]);
         }
     }
@@ -182,7 +184,7 @@ __global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRo
             const float abx = a.abx[clampi(i, 0, nx - 1)];
             const int rlo = i * nz + (j0 - HL), rhi = rlo + W - 1;
             float n1[V], n2[V];
-            vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, c0, i, jb, rin, abx, abz, abt,
+            vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, x + 4, c0, i, jb, rin, abx, abz, abt,
                                      qv1[p], qv2[p], qr[p], ip, rlo, rhi, b, it, n1, n2);
             vstore<V>(&l_v1[(x + 4) * W + c0], n1);
             if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], n2);
@@ -212,7 +214,7 @@ __global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRo
         const int r = x + 6;
         float sa[V], sc[V], sxz[V], o1[V], o2[V], o3[V];
         vload<V>(&l_s1[r * W + c0], sa);
-        vload<V>(&l_s2[r * W + c0], sc);
+        vload<V>(&l_s2[(x + 4) * W + c0], sc);
         if (PSV) vload<V>(&l_s3[r * W + c0], sxz);
         stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, x + 4, c0, i, jb, sa, sc, sxz, qmu[p], qlam[p],
                                     o1, o2, o3);
@@ -226,7 +228,7 @@ __global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRo
             }
         }
         vstore<V>(&l_s1[r * W + c0], o1);
-        vstore<V>(&l_s2[r * W + c0], o2);
+        vstore<V>(&l_s2[(x + 4) * W + c0], o2);
         if (PSV) vstore<V>(&l_s3[r * W + c0], o3);
     }
     __syncthreads();
@@ -246,7 +248,7 @@ __global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRo
             const float abx = a.abx[clampi(i, 0, nx - 1)];
             const int rlo = i * nz + (j0 - HL), rhi = rlo + W - 1;
             float n1[V], n2[V];
-            vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, c0, i, jb, rin, abx, abz, abt,
+            vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, x + 4, c0, i, jb, rin, abx, abz, abt,
                                      qv1[p], qv2[p], qr[p], ip, rlo, rhi, b, it + 1, n1, n2);
             vstore<V>(&l_v1[(x + 4) * W + c0], n1);
             if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], n2);
@@ -283,7 +285,7 @@ __global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRo
         const long o = so + (long)i * nz + jb;
         float sa[V], sc[V], sxz[V], o1[V], o2[V], o3[V];
         vload<V>(&l_s1[r * W + c0], sa);
-        vload<V>(&l_s2[r * W + c0], sc);
+        vload<V>(&l_s2[(x + 4) * W + c0], sc);
         if (PSV) vload<V>(&l_s3[r * W + c0], sxz);
         stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, x + 4, c0, i, jb, sa, sc, sxz, qmu[p], qlam[p],
                                     o1, o2, o3);

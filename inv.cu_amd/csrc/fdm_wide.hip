@@ -162,7 +162,8 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         // one wave-uniform test per point before the per-lane one
         const int rlo = i * nz + (j0 - HL), rhi = rlo + W - 1;
         float n1[V], n2[V];
-        vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, vr + 2, c0, i, jb, rin, abx, abz, abt, qv1[p],
+        vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, vr + 2, vr + 2, c0, i, jb, rin, abx, abz, abt,
+                                 qv1[p],
                                  qv2[p], qr[p], ip, rlo, rhi, b, it, n1, n2);
         vstore<V>(&l_v1[vr * W + c0], n1);
         if (PSV) vstore<V>(&l_v2[vr * W + c0], n2);

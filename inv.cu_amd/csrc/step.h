@@ -122,10 +122,11 @@ struct StepRoles {
 // halo of 6 rows and 6 columns (8 with V = 4, for alignment) each side.
 constexpr int pair_halo(int V) { return V == 2 ? 6 : 8; }
 constexpr int pair_strip(int V) { return 64 * V - 2 * pair_halo(V); }
-// LDS bytes of a two-step tile: stress of steps n / n+1 on rows+12 rows and
+// LDS bytes of a two-step tile: stress of steps n / n+1 on rows+12 rows
+// (szz or szy on rows+9: they enter only through z-differences) and
 // velocity of steps n+1 / n+2 on rows+9 rows, 64 V columns each.
 constexpr int pair_lds_bytes(bool psv, int rows, int V) {
-    return 4 * 64 * V * ((psv ? 3 : 2) * (rows + 12) + (psv ? 2 : 1) * (rows + 9));
+    return 4 * 64 * V * ((psv ? 2 : 1) * (rows + 12) + (rows + 9) + (psv ? 2 : 1) * (rows + 9));
 }
 
 // Launch the two-step kernel for steps a.it and a.it + 1 (roles of the
