@@ -34,8 +34,11 @@
 
 namespace inv {
 
+// The SH tile of eight wavefronts without snapshot roles asks for eight
+// wavefronts per SIMD: its registers then fit 78 scalar registers (106
+// otherwise, which allowed seven) and four of its 40 KB tiles share a CU.
 template <bool SH, bool PSV, int TX, int V, int NW, bool FAST, bool SNAP, bool REC>
-__global__ __launch_bounds__(64 * NW) void k_pair(const StepArgs a, const StepRoles r2in) {
+__global__ __launch_bounds__(64 * NW, SH && NW == 8 && !SNAP ? 8 : 1) void k_pair(const StepArgs a, const StepRoles r2in) {
     static_assert(SH != PSV, "one wave type per launch");
     static_assert(V == 2 || V == 4, "2 or 4 columns per lane");
     static_assert(TX % 4 == 0, "tile rows a multiple of the wavefronts");
