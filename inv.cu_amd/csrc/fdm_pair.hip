@@ -3,7 +3,8 @@
 // twice to a tile held in LDS, so every wavefield and model field crosses HBM
 // once per two steps: 68 B per two P-SV cell-updates instead of per one (40 B
 // for SH).  The price is recomputing a deeper halo: the stress of step n is
-// read on rows i0-6 .. i0+TX+5 and columns j0-6 .. j0+S+5, the velocity of
+// read on rows i0-6 .. i0+TX+5 (szz / szy, which enter only through
+// z-differences, on i0-4 .. i0+TX+4) and columns j0-6 .. j0+S+5, the velocity of
 // step n+1 is computed on rows i0-4 .. i0+TX+4, the stress of step n+1 on
 // rows i0-3 .. i0+TX+2, the velocity of step n+2 on rows i0-1 .. i0+TX+1 and
 // the stress of step n+2 on the owned rows (and columns likewise; edge
@@ -38,7 +39,8 @@ namespace inv {
 // wavefronts per SIMD: its registers then fit 78 scalar registers (106
 // otherwise, which allowed seven) and four of its 40 KB tiles share a CU.
 template <bool SH, bool PSV, int TX, int V, int NW, bool FAST, bool SNAP, bool REC>
-__global__ __launch_bounds__(64 * NW, SH && NW == 8 && !SNAP ? 8 : 1) void k_pair(const StepArgs a, const StepRoles r2in) {
+__global__ __launch_bounds__(64 * NW, SH && NW == 8 && !SNAP ? 8 : 1) void k_pair(
+    const StepArgs a, const StepRoles r2in) {
     static_assert(SH != PSV, "one wave type per launch");
     static_assert(V == 2 || V == 4, "2 or 4 columns per lane");
     static_assert(TX % 4 == 0, "tile rows a multiple of the wavefronts");
