@@ -244,8 +244,8 @@ int inv_set_variant(inv_ctx *ctx, int variant, int tile_x);
 int inv_set_tile(inv_ctx *ctx, int tile_x, int cols_per_lane);
 /* Two time steps per launch (fused variant): the stress and velocity of the
  * tile stay in LDS between the steps, so each field crosses HBM once per two
- * steps.  tile_x rows x cols_per_lane columns per lane: 4, 8, 12 or 16 rows
- * x 2 (116 columns per tile) or 4 or 8 x 4 (240 columns); tile_x = 0 turns it off
+ * steps.  tile_x rows x cols_per_lane columns per lane: 4, 8, 12, 16 or 20
+ * rows x 2 (116 columns per tile) or 4 or 8 x 4 (240 columns); tile_x = 0 turns it off
  * (default).  Runs use it for every pair of steps that has no checkpoint
  * between them, no kernel accumulation after the first and no 16-bit
  * snapshot scale collected by the first and read by the second; the other

@@ -1140,14 +1140,15 @@ int inv_set_tile(inv_ctx *ctx, int tile_x, int cols_per_lane) {
 }
 
 static bool pair_tile_ok(int rows, int cols) {
-    return rows == 0 || (cols == 2 && (rows == 4 || rows == 8 || rows == 12 || rows == 16)) ||
+    return rows == 0 ||
+           (cols == 2 && (rows == 4 || rows == 8 || rows == 12 || rows == 16 || rows == 20)) ||
            (cols == 4 && (rows == 4 || rows == 8));
 }
 
 int inv_set_pair(inv_ctx *ctx, int tile_x, int cols_per_lane) {
     return guard(ctx, [&](Ctx &c) {
         INV_REQUIRE(pair_tile_ok(tile_x, cols_per_lane), INV_ERR_ARG,
-                    "two-step tile must be 4/8/12/16 rows x 2 columns per lane or 4/8 x 4 (0 rows: off)");
+                    "two-step tile must be 4/8/12/16/20 rows x 2 columns per lane or 4/8 x 4 (0 rows: off)");
         INV_REQUIRE(!c.have_grid || tile_x == 0 || c.g.nz % cols_per_lane == 0,
                     INV_ERR_UNSUPPORTED, "nz must be divisible by the columns per lane");
         sync(c);
@@ -1277,7 +1278,7 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                 // LDS than the device allows is skipped
                 const TuneChoice single = best;
                 for (int v : {2, 4})
-                    for (int tx : {4, 8, 12, 16}) {
+                    for (int tx : {4, 8, 12, 16, 20}) {
                         if (c.g.nz % v || !pair_tile_ok(tx, v)) continue;
                         TuneChoice t = single;
                         t.pair_tx = tx;

@@ -11,7 +11,7 @@
 // columns of a phase hold values no later owned cell depends on).
 //
 // Row x = i - i0 of the tile is handled by wavefront (x + 4) mod NW in every
-// phase (NW = 4 wavefronts per tile, 8 for the 16-row tile), so the
+// phase (NW = 4 wavefronts per tile, 8 for the 16- and 20-row tiles), so the
 // registers of a row (velocity, u, rho, lambda, mu) stay with one wavefront
 // from the loads to the last store.  The stress buffer in LDS holds
 // step n and is overwritten in place by step n+1 (each cell's update reads
@@ -35,11 +35,11 @@
 
 namespace inv {
 
-// The SH tile of eight wavefronts without snapshot roles asks for eight
+// The SH 16-row tile without snapshot roles asks for eight
 // wavefronts per SIMD: its registers then fit 78 scalar registers (106
 // otherwise, which allowed seven) and four of its 40 KB tiles share a CU.
 template <bool SH, bool PSV, int TX, int V, int NW, bool FAST, bool SNAP, bool REC>
-__global__ __launch_bounds__(64 * NW, SH && NW == 8 && !SNAP ? 8 : 1) void k_pair(
+__global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pair(
     const StepArgs a, const StepRoles r2in) {
     static_assert(SH != PSV, "one wave type per launch");
     static_assert(V == 2 || V == 4, "2 or 4 columns per lane");
@@ -336,8 +336,10 @@ void pair_launch(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned blocks
         switch (rows) {
             case 4: INV_PAIR(4, 2, 4); break;
             case 12: INV_PAIR(12, 2, 4); break;
-            // 16 rows: eight wavefronts, the same rows per wavefront as 8 x 2
+            // 16 and 20 rows: eight wavefronts (20 rows keep the register arrays
+            // of 16: four rows per wavefront in every phase)
             case 16: INV_PAIR(16, 2, 8); break;
+            case 20: INV_PAIR(20, 2, 8); break;
             default: INV_PAIR(8, 2, 4); break;
         }
     } else {

@@ -19,7 +19,7 @@ from conftest import inv, oracle_problem, oracle_traces, rel
 pytestmark = pytest.mark.gpu
 
 H = inv.invhip
-PAIRS = [(4, 2), (8, 2), (12, 2), (16, 2), (4, 4), (8, 4)]
+PAIRS = [(4, 2), (8, 2), (12, 2), (16, 2), (20, 2), (4, 4), (8, 4)]
 
 
 def _pid(t):
@@ -104,7 +104,7 @@ def _obs(P):
     return np.stack([oracle_traces(O.forward(s)[0], P) for s in range(P.nsrc)]).astype(np.float32)
 
 
-@pytest.mark.parametrize("tile", [(8, 2), (4, 4), (12, 2), (16, 2)], ids=_pid)
+@pytest.mark.parametrize("tile", [(8, 2), (4, 4), (12, 2), (16, 2), (20, 2)], ids=_pid)
 @pytest.mark.parametrize("fmt,ckpt,graphs", [(H.INV_SNAP_F32, 0, True), (H.INV_SNAP_F16, 100, True),
                                              (H.INV_SNAP_BF16, 60, False),
                                              (H.INV_SNAP_F16, 0, False)],
@@ -193,7 +193,7 @@ def test_pair_profile_counts_launches(gpu):
 def test_pair_rejects_bad_tiles(gpu):
     P = inv.problems.tiny(nx=40, nz=62, nt=20, sh=0, psv=1)  # nz not divisible by 4
     ctx = P.context(batch=1)
-    for t in ((20, 2), (6, 2), (12, 4), (8, 1), (8, 3)):
+    for t in ((24, 2), (6, 2), (12, 4), (8, 1), (8, 3)):
         with pytest.raises(H.InvError):
             ctx.set_pair(*t)
     with pytest.raises(H.InvError):
@@ -215,6 +215,7 @@ def test_every_kernel_records_the_same(gpu, math, explosive):
     for name, tile, pair in (("col8", (8, 1), None), ("col16", (16, 1), None),
                              ("wide4x2", (4, 2), None), ("wide8x4", (8, 4), None),
                              ("pair8x2", (8, 1), (8, 2)), ("pair16x2", (8, 1), (16, 2)),
+                             ("pair20x2", (8, 1), (20, 2)),
                              ("pair4x4", (8, 1), (4, 4))):
         ctx = P.context(batch=2)
         ctx.set_math(math)
@@ -248,8 +249,8 @@ def _run(P, tile, pair, fmt, ckpt, obs=None, snapshots=True):
     return out
 
 
-@pytest.mark.parametrize("tile,pair", [((8, 2), None), ((8, 1), (16, 2))],
-                         ids=["wide8x2", "pair16x2"])
+@pytest.mark.parametrize("tile,pair", [((8, 2), None), ((8, 1), (16, 2)), ((8, 1), (20, 2))],
+                         ids=["wide8x2", "pair16x2", "pair20x2"])
 def test_kernel_variants_agree(gpu, tile, pair):
     """The step kernels are compiled with and without snapshot roles and
     receiver recording, picked per launch.  A forward that stores snapshots
