@@ -308,8 +308,8 @@ def launch_plan(a, P, B, ckpt, desc):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
-              "shot sharding is described in DESIGN.md but not built.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): sharding the "
+              "shots over several GPUs is designed but not built.", file=sys.stderr)
         sys.exit(2)
     import __graft_entry__ as graft
 
