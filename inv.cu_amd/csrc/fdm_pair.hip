@@ -23,9 +23,10 @@
 // launches of k_colv (and of k_col and the FP32 oracle in exact math).
 // Point lists are bucketed for this geometry: forces with a halo of 4 rows
 // and columns, explosions 3, receivers 0.  Snapshot roles of the first step
-// come in StepArgs, of the second in StepRoles; a role whose max-abs table is
-// filled by the first step must not be read by the second (the host never
-// pairs such steps).  SNAP = false is the variant for launches with no
+// come in StepArgs, of the second in StepRoles.  The first step gathers no
+// max-abs (the launcher refuses that role, and the kernel drops it), and a
+// table the second step fills is read by no step of the same launch (the
+// host never pairs such steps).  SNAP = false is the variant for launches with no
 // snapshot role in either step (most of them: every adjoint launch and three
 // of four forward ones with sfe = 10): the roles are compile-time null there,
 // so the snapshot code and the scalar registers of its pointers are gone.
@@ -52,7 +53,7 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
     constexpr int PS = (SR + NW - 1) / NW;      // stress rows loaded per wavefront
     constexpr int P = (VR + NW - 1) / NW;       // rows x = wave - 4 + NW p per wavefront
     extern __shared__ __align__(16) float smem[];
-    __shared__ float l_track[8 * NW];  // per-wavefront maxima of the two steps
+    __shared__ float l_track[4 * NW];  // per-wavefront maxima of the second step
     float *const l_s1 = smem;
     float *const l_s2 = l_s1 + SR * W;
     float *const l_s3 = PSV ? l_s2 + VR * W : l_s2;  // unused for SH
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
     const int mb = b * a.nsnapf;
     const StepRoles none{nullptr, nullptr, 1, nullptr, nullptr, nullptr, nullptr};
     const StepRoles r1s =
-        SNAP ? StepRoles{a.snap_u, a.snap_v, a.v_after, a.mx_u, a.mx_v, a.acc_u, a.acc_v} : none;
+        SNAP ? StepRoles{a.snap_u, a.snap_v, a.v_after, a.mx_u, a.mx_v, nullptr, nullptr} : none;
     const StepRoles r2 = SNAP ? r2in : none;
     float mu1 = 0.0f, mu2 = 0.0f, mv1 = 0.0f, mv2 = 0.0f;
 
@@ -202,11 +203,8 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
                 if (PSV) qv2[p][v] = n2[v];
             }
         }
-        if (r1s.acc_u || r1s.acc_v) track_stage<PSV>(l_track, wave, lane, mu1, mu2, mv1, mv2);
     }
     __syncthreads();
-    if ((r1s.acc_u || r1s.acc_v) && threadIdx.x == 0)
-        track_commit<PSV>(l_track, NW, r1s.acc_u, r1s.acc_v, mb, fu, fv);
 
     // B. stress of step n+1 on rows x = -3 .. TX+2, in place (0 outside the grid)
 #pragma unroll
@@ -271,11 +269,11 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
                 }
             }
         }
-        if (r2.acc_u || r2.acc_v) track_stage<PSV>(l_track + 4 * NW, wave, lane, mu1, mu2, mv1, mv2);
+        if (r2.acc_u || r2.acc_v) track_stage<PSV>(l_track, wave, lane, mu1, mu2, mv1, mv2);
     }
     __syncthreads();
     if ((r2.acc_u || r2.acc_v) && threadIdx.x == 0)
-        track_commit<PSV>(l_track + 4 * NW, NW, r2.acc_u, r2.acc_v, mb, fu, fv);
+        track_commit<PSV>(l_track, NW, r2.acc_u, r2.acc_v, mb, fu, fv);
 
     // D. stress of step n+2 on the owned cells (every lane runs the rows, for
     // the explosion scan; owned lanes store)
@@ -314,8 +312,11 @@ static void pair_go(Ctx &c, K kern, int nw, int lds, const StepArgs &a, const St
 
 template <bool SH, bool PSV, bool FAST>
 void pair_launch(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned blocks) {
-    const bool snap = a.snap_u || a.snap_v || a.acc_u || a.acc_v || r2.snap_u || r2.snap_v ||
-                      r2.acc_u || r2.acc_v;
+    // The first step of a pair never gathers a maximum (the kernel drops that
+    // role): its snapshot would need the maximum before the launch ends.
+    INV_REQUIRE(!a.acc_u && !a.acc_v, INV_ERR_STATE,
+                "two-step launch asked to gather a maximum in its first step");
+    const bool snap = a.snap_u || a.snap_v || r2.snap_u || r2.snap_v || r2.acc_u || r2.acc_v;
     const int var = (snap ? 2 : 0) + (a.rec_ptr ? 1 : 0);
     const int rows = c.pair_tx, V = c.pair_v;
     const int lds = pair_lds_bytes(PSV, rows, V);
