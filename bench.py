@@ -2,43 +2,43 @@
 """Benchmark: forward+adjoint gradient throughput on one MI355X.
 
 A step is one full gradient evaluation (Misfit::run(kernel=true),
-misfit/misfit.h:15-51): forward of every shot with snapshots, misfit and
-adjoint sources, adjoint with kernel accumulation, one-sided Gaussian
+misfit/misfit.h:15-51) of one wave of shots: forward with snapshots, misfit
+and adjoint sources, adjoint with kernel accumulation, one-sided Gaussian
 smoothing of the kernels.  Inputs (model, observed data) are resident in HBM
-before the timed region.  Workloads (--workload):
+before the timed region.  The workloads are defined in
+inv.cu_amd/problems.py (`workload`):
 
-  c0            P-SV checkerboard 200x200 at 10 m (+-10 % squares of 20 cells
-                on vp, vs, rho), 4 explosive Ricker sources at 20 cells depth,
-                40 receivers, dt 1 ms, nt 2000, waveform misfit, sigma 5; all 4
-                shots per launch; the CPU oracle runs the whole gradient on the
-                same observed data and the kernels are compared with the GPU's.
-  c1 (default)  batched P-SV, Marmousi-shaped 737x240 at 12.5 m, 16 shots,
-                368 receivers, dt 1 ms, nt 4000, waveform misfit, FP32
-                snapshots; all 16 shots per launch.
-  c2            SH, the same generator at 1474x480 and 6.25 m, 64 shots,
-                737 receivers, dt 0.5 ms, nt 8000, FP16 snapshots every 10
-                steps with 4000-step checkpoints (the adjoint recomputes the first
-                half of the forward; one half's FP16 snapshots take 72 GB);
-                plus the error of FP32 / FP16 / BF16 snapshot storage against
-                the FP64 oracle on a 2-shot, 2000-step sample.
-  c4            P-SV, the generator at 4096x2048 and 5 m, dt 0.4 ms,
-                nt 10000, 2048 receivers, envelope misfit, FP16 snapshots with
-                1000-step checkpoints; timed on 16 of the 256 shots (one
-                launch; a full gradient is 16 such batches), plus the FP64
-                check on the 1024x512 / 20 m downsampled model.
+  c4 (default)  P-SV, Marmousi-shaped 4096x2048 at 5 m, 256 shots, 2048
+                receivers, dt 0.4 ms, nt 10000, envelope misfit, FP16
+                snapshots; a step is the gradient of one wave of 4 shots (one
+                launch keeps all of their snapshots in HBM, 268 GB), sigma 5.
+                value = Gcell-updates/s of the step; the 256-shot gradient is
+                64 such waves (gradients_per_hr_256_shots).
+  c0            P-SV checkerboard 200x200, 4 explosive shots, nt 2000: the
+                whole gradient is a step; the CPU oracle runs the same
+                gradient and the kernels are compared.
+  c1            P-SV 737x240, 16 shots in one launch, nt 4000, FP32
+                snapshots; also times the one-shot-per-launch plain path.
+  c2            SH 1474x480, 64 shots in one launch, nt 8000, FP16
+                snapshots all resident (145 GB).
 
-Prints one JSON line (rank 0).  Extra keys: roofline (stencil step vs HBM
-peak, timed live with HIP events inside the library), cpu_baseline (the CPU
-oracle on a bounded sample of the same workload; c0: all of it), parity (c0:
-GPU gradient against the CPU oracle's), plain (c1: the one-shot-
-per-launch GPU path on the same workload), serving (c1: p50/p99 latency of
-the request server against the plain loop), accuracy (c2, c4).
+With the default workload, c1 and c2 are also timed (2 gradients each) and
+reported under "extra_workloads" (--extras '' skips them).
+
+Prints one JSON line.  Extra keys: roofline (the dominant stencil kernel
+against the HBM peak, timed live with HIP events on the library's stream),
+cpu_baseline (the CPU oracle on a bounded sample of the same workload, on the
+threads OMP_NUM_THREADS allows, plus a one-thread sample), accuracy (c2, c4:
+gradient error of each snapshot format against the FP64 oracle), serving
+(p50/p99 latency of the request server against the plain loop on mixed-size
+requests), plain (c1).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 from pathlib import Path
@@ -50,19 +50,18 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Gcell-updates/s fwd+adjoint; gradients/hr; p50/p99 shot latency vs plain"
 HBM_PEAK_GBPS = 8000.0
-# Algorithmic HBM bytes per cell-update of one P-SV stencil time step: each of
-# the 7 wavefields (vx, vz, ux, uz, sxx, szz, sxz) read and written once and
-# the 3 material fields read once, FP32.
+# Algorithmic HBM bytes per cell of one stencil launch: each wavefield read and
+# written once, each material field read once, FP32.  P-SV: vx, vz, ux, uz,
+# sxx, szz, sxz and lambda, mu, rho; SH: vy, uy, sxy, szy and mu, rho.  A
+# two-step launch moves them once for two time steps.
 PSV_STEP_BYTES = 4 * (2 * 7 + 3)
-# SH: the 4 wavefields (vy, uy, sxy, szy) read and written, mu and rho read
 SH_STEP_BYTES = 4 * (2 * 4 + 2)
-# HBM bytes per cell of one launch, measured with rocprofv3 FETCH_SIZE and
-# WRITE_SIZE (per dispatch, each scaled by what the counter reads for a plain
-# device copy of known size: fetch x2.0, write x1.0), for the tiles the
-# autotuner picks on these workloads, fast math, the forward variants
-# without snapshot roles (the launches the live timing samples most);
-# keyed by (workload, steps per launch, rows, columns per lane, shots per
-# launch):
+# HBM bytes per cell of one launch from rocprofv3 FETCH_SIZE and WRITE_SIZE
+# (per dispatch; fetch scaled by 2.0 and write by 1.0, the factors the same
+# counters give for a plain device copy of known size), for the tiles the
+# autotuner picks on these workloads, fast math, the forward variant without
+# snapshot roles; keyed by (workload, steps per launch, rows, columns per lane,
+# shots per launch):
 #   c1 one-step 4 x 2, 16 shots:  68.30 B (algorithmic 68),
 #                                 profiles/k_colv_psv_marmousi737x240_16shots.md
 #   c4 two-step 20 x 2, 4 shots:  71.51 B per launch of two steps (algorithmic 68),
@@ -73,16 +72,27 @@ MEASURED_TRAFFIC_PER_CELL = {("c1", 1, 4, 2, 16): 68.30, ("c4", 2, 20, 2, 4): 71
                              ("c2", 2, 16, 2, 64): 40.36}
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def parse(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c4", choices=["c0", "c1", "c2", "c4"])
+    ap.add_argument("--extras", default=None,
+                    help="comma-separated workloads also timed (2 gradients each); default "
+                         "'c1,c2' with --workload c4, none otherwise")
     ap.add_argument("--batch", type=int, default=0, help="shots per launch (0: the workload's)")
+    ap.add_argument("--wave", type=int, default=0,
+                    help="shots per step (0: c4 one launch, the others all shots)")
     ap.add_argument("--ckpt", type=int, default=-1,
                     help="checkpoint interval in steps for 16-bit snapshots (-1: the workload's, "
                          "0: none, every snapshot of the run resident)")
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c4"])
+    ap.add_argument("--snap", default="", choices=["", "fp32", "fp16", "bf16"],
+                    help="snapshot storage (default: the workload's)")
     ap.add_argument("--nt", type=int, default=0, help="time steps (0: the workload's)")
     ap.add_argument("--sigma", type=int, default=5, help="Gaussian filter sigma (cells)")
     ap.add_argument("--prof-every", type=int, default=16,
@@ -104,9 +114,9 @@ def parse(argv=None):
     ap.add_argument("--graphs", type=int, default=1, help="capture time loops in HIP graphs")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (counter calibration)")
-    ap.add_argument("--serve-requests", type=int, default=12,
+    ap.add_argument("--serve-requests", type=int, default=24,
                     help="mixed-size serving leg: requests per run (0 = skip)")
-    ap.add_argument("--serve-rate-x", type=float, default=1.0,
+    ap.add_argument("--serve-rate-x", type=float, default=2.0,
                     help="serving leg arrival rate, in multiples of the plain loop's capacity")
     ap.add_argument("--serving-sweep", type=int, default=0,
                     help="run only the serving sweep over this many requests (one JSON line "
@@ -120,7 +130,6 @@ def serving_leg(inv, n, rate_x, snap_format):
     requests (grids 256x256 .. 2048x1024, nt 1000-6000, SH/P-SV, forward or
     forward+adjoint, 1-8 shots), at rate_x times the plain loop's capacity."""
     S = inv.serving
-    log = lambda m: print(m, file=sys.stderr, flush=True)
     reqs = S.make_requests(n)
     kw = dict(snap_format=snap_format, max_lanes=4, max_slots=16)
     base, _ = S.run_load(reqs, rate=None, plain=True, log=log, **kw)
@@ -133,6 +142,7 @@ def serving_leg(inv, n, rate_x, snap_format):
             "plain_capacity_req_per_s": round(base["requests_per_s"], 4),
             "snapshots": ["fp32", "fp16", "bf16"][snap_format],
             "plain": {k: plain[k] for k in keys}, "batched": {k: batched[k] for k in keys},
+            "p50_speedup": round(plain["p50_latency_s"] / batched["p50_latency_s"], 3),
             "p99_speedup": round(plain["p99_latency_s"] / batched["p99_latency_s"], 3)}
 
 
@@ -159,7 +169,20 @@ def configure_kernel(ctx, a, count, pair=True):
     return tile
 
 
-def cpu_baseline(P, shots, nt=None, obs=None):
+def cpu_info():
+    model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model or platform.processor(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+def cpu_baseline(P, shots, nt=None, obs=None, sigma=None, threads=None):
     """The CPU oracle (FP32, OpenMP) on `shots` shots of the same workload
     (the first nt steps when nt is given), against observed data `obs`
     ([shots][ncomp][nrec][nt], zeros when None: the work is the same).
@@ -168,9 +191,11 @@ def cpu_baseline(P, shots, nt=None, obs=None):
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle  # the checker, used here only as the timed CPU baseline
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    if threads is None:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     pyoracle.set_threads(threads)
     nt = nt or P.nt
+    sigma = P.sigma if sigma is None else sigma
     Ps = P.with_(src_ix=P.src_ix[:shots], src_iz=P.src_iz[:shots],
                  stf_x=np.ascontiguousarray(P.stf_x[:shots, :nt]),
                  stf_y=np.ascontiguousarray(P.stf_y[:shots, :nt]),
@@ -184,74 +209,35 @@ def cpu_baseline(P, shots, nt=None, obs=None):
     if obs is None:
         obs = np.zeros((shots, P.ncomp, P.nrec, nt), np.float32)
     t0 = time.perf_counter()
-    res = O.gradient(obs, P.misfit, P.sigma, dtype=np.float32)
+    res = O.gradient(obs, P.misfit, sigma, dtype=np.float32)
     dt = time.perf_counter() - t0
     work = Ps.cell_updates(shots)
+    what = "forward+misfit+adjoint" + (f"+filter (sigma {sigma})" if sigma else "")
     return {"value": work / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads,
             "kind": "port",
-            "sample": f"oracle FP32 gradient (forward+misfit+adjoint+filter) of {shots} of the "
-                      f"{P.nsrc} shots, nt={nt} of {P.nt}, {P.nx}x{P.nz}; {dt:.1f} s"}, res
+            "sample": f"oracle FP32 {what} of {shots} of the {P.nsrc} shots, nt={nt} of "
+                      f"{P.nt}, {P.nx}x{P.nz}, {threads} thread(s); {dt:.1f} s"}, res
 
 
-def workload(inv, a):
-    """(problem, shots per launch, snapshot format, checkpoint interval,
-    bytes per cell-update of the step kernel, description) of a workload."""
-    pr, H = inv.problems, inv.invhip
-    if a.workload == "c0":
-        P = pr.checker(nt=a.nt or 2000, sigma=a.sigma, explosive=True)
-        return (P, 4, H.INV_SNAP_F32, 0, PSV_STEP_BYTES,
-                f"batched P-SV gradient, checkerboard 200x200 @ 10 m, 4 explosive shots, "
-                f"40 receivers, dt 1 ms, nt {P.nt}, sigma {P.sigma}")
-    if a.workload == "c1":
-        P = pr.marmousi(nt=a.nt or 4000, sigma=a.sigma)
-        return (P, 16, H.INV_SNAP_F32, 0, PSV_STEP_BYTES,
-                f"batched P-SV gradient, Marmousi-shaped 737x240 @ 12.5 m, 16 shots, "
-                f"368 receivers, dt 1 ms, nt {P.nt}")
-    if a.workload == "c2":
-        # SH waves do not enter the water layer (mu = 0 there): shots and
-        # receivers sit on the seafloor, 2 rows below the 12 water rows
-        P = pr.marmousi(nx=1474, nz=480, dx=6.25, dt=5e-4, nt=a.nt or 8000, nsrc=64, nrec=737,
-                        f0=8.0, t0=0.15, seed=2, sh=1, psv=0, sigma=a.sigma, src_depth=14,
-                        rec_depth=14, name="marmousi-sh")
-        # every FP16 snapshot of the 64 shots stays in HBM (145 GB): no recomputation
-        return (P, 64, H.INV_SNAP_F16, 0, SH_STEP_BYTES,
-                f"batched SH gradient, Marmousi-shaped 1474x480 @ 6.25 m, 64 shots, "
-                f"737 seafloor receivers, {{batch}} per launch, dt 0.5 ms, nt {P.nt}, FP16 "
-                f"snapshots every 10 steps, {{ckpt}}")
-    P = pr.marmousi(nx=4096, nz=2048, dx=5.0, dt=4e-4, nt=a.nt or 10000, nsrc=256, nrec=2048,
-                    f0=12.0, t0=0.1, seed=4, sigma=a.sigma, name="marmousi-large")
-    P = P.with_(misfit=1)
-    pick = np.arange(0, P.nsrc, 16)  # 16 of the 256 shots, spread over the line
-    P = P.with_(src_ix=P.src_ix[pick], src_iz=P.src_iz[pick], stf_x=P.stf_x[pick],
-                stf_y=P.stf_y[pick], stf_z=P.stf_z[pick])
-    # four shots per launch keep all of their FP16 snapshots in HBM (268 GB):
-    # no recomputation.  Sixteen per launch would need 1000-step checkpoints
-    # and recompute nine tenths of the forward (39.8 s against 29.0 s)
-    return (P, 4, H.INV_SNAP_F16, 0, PSV_STEP_BYTES,
-            f"batched P-SV gradient, 4096x2048 @ 5 m, 16 of 256 shots, {{batch}} per launch, "
-            f"2048 receivers, dt 0.4 ms, nt {P.nt}, envelope misfit, FP16 snapshots every 10 "
-            f"steps, {{ckpt}}")
-
-
-def accuracy(inv, a):
+def accuracy(inv, name):
     """Gradient and misfit error of FP32, FP16-snapshot and BF16-snapshot
-    storage (all arithmetic FP32) against the FP64 oracle, on a sample the
-    CPU runs in seconds: c2 on its own model (2 shots, nt 2000), c4 on the
-    1024x512 / 20 m downsampled model (2 shots, dt 2 ms, nt 2000)."""
+    storage (all arithmetic FP32) against the FP64 oracle: c2 on its own model
+    and nt (2 shots), c4 on the 1024x512 / 20 m downsampled model (2 shots, dt
+    2 ms, nt 2000).  The kernel error is the relative L2 over the solid cells
+    (mu > 0): in the water layer the mu kernel is singular next to the sources
+    and FP32 itself (GPU and CPU alike) departs from FP64 there."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle  # the checker: the FP64 reference of this comparison
 
     pr, H = inv.problems, inv.invhip
-    if a.workload == "c2":
-        P = pr.marmousi(nx=1474, nz=480, dx=6.25, dt=5e-4, nt=2000, nsrc=2, nrec=737, f0=8.0,
-                        t0=0.15, seed=2, sh=1, psv=0, sigma=0, src_depth=14, rec_depth=14,
-                        name="marmousi-sh")
-        ckpt, sample = 400, "1474x480 SH, 2 shots, nt 2000 of 8000"
+    if name == "c2":
+        P = pr.shots(pr.workload("c2", sigma=0)[0], [20, 44])
+        sample = "1474x480 SH, 2 shots, nt 8000 (full)"
     else:
         P = pr.marmousi(nx=1024, nz=512, dx=20.0, dt=2e-3, nt=2000, nsrc=2, nrec=512, f0=6.0,
                         t0=0.2, seed=4, sigma=0, name="marmousi-large-ds").with_(misfit=1)
-        ckpt, sample = 400, "1024x512 @ 20 m P-SV (c4 downsampled), 2 shots, dt 2 ms, nt 2000"
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        sample = "1024x512 @ 20 m P-SV (c4 downsampled), 2 shots, dt 2 ms, nt 2000"
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     pyoracle.set_threads(threads)
     ctx = P.context(batch=2, model="true")
     ctx.forward(0, 2, snapshots=False)
@@ -267,105 +253,57 @@ def accuracy(inv, a):
     tcpu = time.perf_counter() - t0
     names = ("k_lambda", "k_mu", "k_rho")
     out = {"sample": sample, "reference": f"oracle FP64 ({tcpu:.1f} s on {threads} threads)",
-           "misfit_fp64": f64}
-    for label, fmt, ck in (("fp32", H.INV_SNAP_F32, 0), ("fp16", H.INV_SNAP_F16, ckpt),
-                           ("bf16", H.INV_SNAP_BF16, ckpt)):
+           "measure": "relative L2 over the solid cells (mu > 0)", "misfit_fp64": f64}
+    solid = P.mu0 > 0
+    for label, fmt in (("fp32", H.INV_SNAP_F32), ("fp16", H.INV_SNAP_F16),
+                       ("bf16", H.INV_SNAP_BF16)):
         ctx = P.context(batch=2, model="init")
-        ctx.set_snapshots(fmt, ck)
+        ctx.set_snapshots(fmt, 0)
         for s_ in range(2):
             ctx.obs_set(s_, obs[s_])
         f = ctx.gradient(misfit_type=P.misfit, sigma=0)
         ks = ctx.kernels_get()
         ctx.close()
-        row = {"misfit_rel_err": abs(f - f64) / abs(f64), "checkpoints": ck}
-        # whole grid, and the solid cells only (mu > 0): in the water layer the
-        # mu kernel is singular next to the sources and FP32 itself (GPU and
-        # CPU alike) departs from FP64 there
-        solid = P.mu0 > 0
+        row = {"misfit_rel_err": abs(f - f64) / abs(f64)}
         for n, x, y in zip(names, ks, k64):
             if np.any(y):
-                row[n + "_rel_l2"] = float(np.linalg.norm(x - y) / np.linalg.norm(y))
-                row[n + "_rel_l2_solid"] = float(np.linalg.norm((x - y)[solid]) /
-                                                 np.linalg.norm(y[solid]))
+                row[n] = float(np.linalg.norm((x - y)[solid]) / np.linalg.norm(y[solid]))
         out[label] = row
-        print(f"# accuracy {label}: {row}", file=sys.stderr, flush=True)
+        log(f"# accuracy {name} {label}: {row}")
     return out
 
 
-def launch_plan(a, P, B, ckpt, desc):
-    """Shots per launch, checkpoint interval and the workload description
-    after the --batch / --ckpt overrides."""
+def launch_plan(a, name, P, B, fmt, ckpt, wave):
+    """Shots per launch, shots per step, snapshot format and checkpoint
+    interval after the command-line overrides."""
+    H = sys.modules["inv_cu_amd"].invhip
     if a.batch > 0:
         B = min(a.batch, P.nsrc)
+    if a.wave > 0:
+        wave = a.wave
+    wave = min(max(wave, B), P.nsrc)
     if a.ckpt >= 0:
         ckpt = a.ckpt
+    if a.snap:
+        fmt = {"fp32": H.INV_SNAP_F32, "fp16": H.INV_SNAP_F16, "bf16": H.INV_SNAP_BF16}[a.snap]
     if ckpt and P.nt % ckpt:
         ckpt = P.nt  # a shortened run (--nt) the interval does not divide: one segment
-    desc = desc.format(batch=B, ckpt=f"{ckpt}-step checkpoints" if ckpt else "no checkpoints")
-    return B, ckpt, desc
+    return B, wave, fmt, ckpt
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): sharding the "
-              "shots over several GPUs is designed but not built.", file=sys.stderr)
-        sys.exit(2)
-    import __graft_entry__ as graft
+def describe(name, P, B, wave, fmt, ckpt):
+    snap = ["FP32", "FP16", "BF16"][fmt]
+    ck = f"{ckpt}-step checkpoints" if ckpt else "every snapshot resident"
+    waves = "SH" if P.sh and not P.psv else "P-SV"
+    return (f"{name}: batched {waves} gradient, {P.name} {P.nx}x{P.nz} @ {float(P.dx):g} m, "
+            f"{wave} of {P.nsrc} shots per step, {B} per launch, {P.nrec} receivers, "
+            f"dt {float(P.dt) * 1e3:g} ms, nt {P.nt}, {['waveform', 'envelope'][P.misfit]} "
+            f"misfit, sigma {P.sigma}, {snap} snapshots every {P.sfe} steps, {ck}")
 
-    inv = graft.load_package()
-    if a.serving_sweep:
-        log = lambda m: print(m, file=sys.stderr, flush=True)
-        for s in inv.serving.sweep(a.serving_sweep, log=log,
-                                   snap_format=inv.invhip.INV_SNAP_F16):
-            print(json.dumps(s), flush=True)
-        return
-    P, B, snap_fmt, ckpt, step_bytes, desc = workload(inv, a)
-    B, ckpt, desc = launch_plan(a, P, B, ckpt, desc)
-    print(f"# workload {a.workload} {P.name} {P.nx}x{P.nz} shots={P.nsrc} nt={P.nt} batch={B}",
-          file=sys.stderr, flush=True)
-    ctx = P.context(batch=B, model="true")
-    ctx.set_math(a.math)
-    ctx.set_graphs(a.graphs)
-    ctx.set_snapshots(snap_fmt, ckpt)  # before any run sizes the snapshot storage
-    tile = configure_kernel(ctx, a, B)
-    print(f"# step kernel tile {tile}", file=sys.stderr, flush=True)
-    # observed data: forward on the true model (Misfit::init, misfit.h:135-148)
-    for first in range(0, P.nsrc, B):
-        n = min(B, P.nsrc - first)
-        ctx.forward(first, n, snapshots=False)
-        ctx.obs_from_traces(first, n)
-    P.load_model(ctx, "init")
-    ctx.sync()
-    print("# observed data ready", file=sys.stderr, flush=True)
 
-    if a.copy_calib_mb:
-        nb = a.copy_calib_mb << 20
-        src, dst = ctx.alloc(nb), ctx.alloc(nb)
-        ctx.vec_fill(src, 1.0, nb // 4)
-        ctx.vec_copy(dst, src, nb // 4)
-        ctx.sync()
-        ctx.free(src)
-        ctx.free(dst)
-    for _ in range(a.warmup):
-        ctx.gradient(P.misfit, P.sigma)
-        print("# warm-up gradient done", file=sys.stderr, flush=True)
-    ctx.sync()
-    ctx.prof_reset()
-    ctx.prof_enable(a.prof_every)
-    t0 = time.perf_counter()
-    f = 0.0
-    for _ in range(a.steps):
-        f = ctx.gradient(P.misfit, P.sigma)
-    ctx.sync()
-    t1 = time.perf_counter()
-    ctx.prof_enable(0)
-    ms = (t1 - t0) / a.steps * 1e3
-    work = P.cell_updates()
-    value = work / (ms / 1e3) / 1e9
-    print(f"# timed {a.steps} gradients: {ms:.1f} ms each, misfit {f:.6e}", file=sys.stderr,
-          flush=True)
-
+def roofline(ctx, inv, a, name, P, B, tile, fmt, ckpt, step_bytes):
+    """The dominant stencil kernel's achieved bandwidth: average launch time
+    from HIP events on the library's stream, algorithmic bytes per launch."""
     H = inv.invhip
     nv, tv = ctx.prof_read(H.INV_PROF_VELOCITY)
     ns, ts = ctx.prof_read(H.INV_PROF_STRESS)
@@ -373,18 +311,14 @@ def main():
     npr, tpr = ctx.prof_read(H.INV_PROF_PAIR)
     nfr, tfr = ctx.prof_read(H.INV_PROF_FWD_RUN)
     nar, tar = ctx.prof_read(H.INV_PROF_ADJ_RUN)
-    ni, ti = ctx.prof_read(H.INV_PROF_INTERACT)
-    roof = None
     cells = B * P.N
     two = "two_step" in tile
     waves = "SH" if P.sh and not P.psv else "P-SV" if P.psv and not P.sh else "SH+P-SV"
     # a forward graph holds one kind of step launch when every pair of steps
     # is paired (no 16-bit snapshot scales, no checkpoints, nt even) or none
-    uniform = not two or (snap_fmt == H.INV_SNAP_F32 and ckpt == 0 and P.nt % 2 == 0)
+    uniform = not two or (fmt == H.INV_SNAP_F32 and ckpt == 0 and P.nt % 2 == 0)
     launch_s, kernel, avg, samples, per_launch_steps = None, None, {}, {}, 1
     if nfr and uniform:
-        # replayed forward graphs: every launch in them but a handful of
-        # set-up fills is the step kernel, so time / launches is its duration
         per_launch_steps = 2 if two else 1
         launch_s = tfr / (nfr / per_launch_steps) / 1e3
         kernel = (f"two-step {waves} kernel k_pair, from forward-graph replays" if two else
@@ -395,17 +329,16 @@ def main():
     elif two:
         # mixed one- and two-step launches: time each launch of a short direct
         # forward (graphs off) right after the timed region
-        if not (npr and nf):
-            ctx.set_graphs(False)
-            ctx.set_step_limit(min(P.nt, 400))
-            ctx.prof_reset()
-            ctx.prof_enable(1)
-            ctx.forward(0, B, snapshots=True)
-            ctx.sync()
-            ctx.prof_enable(0)
-            ctx.set_step_limit(0)
-            nf, tf = ctx.prof_read(H.INV_PROF_STEP)
-            npr, tpr = ctx.prof_read(H.INV_PROF_PAIR)
+        ctx.set_graphs(False)
+        ctx.set_step_limit(min(P.nt, 400))
+        ctx.prof_reset()
+        ctx.prof_enable(1)
+        ctx.forward(0, B, snapshots=True)
+        ctx.sync()
+        ctx.prof_enable(0)
+        ctx.set_step_limit(0)
+        nf, tf = ctx.prof_read(H.INV_PROF_STEP)
+        npr, tpr = ctx.prof_read(H.INV_PROF_PAIR)
         if npr:
             per_launch_steps = 2
             launch_s = tpr / npr / 1e3
@@ -428,53 +361,165 @@ def main():
         avg["forward_per_step"] = round(tfr / nfr * 1e3, 2)
     if nar:
         avg["adjoint_per_step_incl_interaction"] = round(tar / nar * 1e3, 2)
-    if launch_s:
-        # each wavefield and model field is read and written once per launch
-        achieved = step_bytes * cells / launch_s / 1e9
-        t = tile.get("two_step") or tile
-        key = (a.workload, per_launch_steps, t["rows"], t["cols_per_lane"], B)
-        measured = (MEASURED_TRAFFIC_PER_CELL.get(key)
-                    if tile["variant"] == 0 and a.math == 1 else None)
-        traffic = round(measured * cells) if measured else None
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                "kernel": kernel, "alg_bytes_per_launch": step_bytes * cells,
-                "steps_per_launch": per_launch_steps, "avg_us": avg, "samples": samples}
-        if per_launch_steps == 2:
-            # the same cell-updates at one step per launch would need this
-            roof["one_step_equivalent_gbps"] = round(2 * achieved, 1)
+    if not launch_s:
+        return None
+    achieved = step_bytes * cells / launch_s / 1e9
+    t = tile.get("two_step") or tile
+    key = (name, per_launch_steps, t["rows"], t["cols_per_lane"], B)
+    measured = (MEASURED_TRAFFIC_PER_CELL.get(key)
+                if tile["variant"] == 0 and a.math == 1 else None)
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": round(measured * cells) if measured else None,
+            "kernel": kernel, "alg_bytes_per_launch": step_bytes * cells,
+            "steps_per_launch": per_launch_steps, "avg_us": avg, "samples": samples}
+    if per_launch_steps == 2:
+        # the same cell-updates at one step per launch would need this
+        roof["one_step_equivalent_gbps"] = round(2 * achieved, 1)
+    ni, ti = ctx.prof_read(H.INV_PROF_INTERACT)
+    if ni:
+        roof["interaction_avg_us"] = round(ti / ni * 1e3, 2)
+    return roof
+
+
+def run_workload(inv, a, name, steps, warmup, headline):
+    """Times `steps` gradients of one shot wave of workload `name`; returns
+    its record (and, for c0, the GPU gradient for the CPU comparison)."""
+    pr, H = inv.problems, inv.invhip
+    nt = a.nt if headline else 0
+    P, B, fmt, ckpt = pr.workload(name, nt=nt, sigma=a.sigma)
+    wave = B if name == "c4" else P.nsrc
+    if headline:
+        B, wave, fmt, ckpt = launch_plan(a, name, P, B, fmt, ckpt, wave)
+    if wave < P.nsrc:
+        # a fixed wave of shots spread over the line
+        P = pr.shots(P, np.linspace(0, P.nsrc - 1, wave).round().astype(int))
+    full_shots = pr.workload(name, nt=nt, sigma=a.sigma)[0].nsrc
+    desc = describe(name, P, B, wave, fmt, ckpt)
+    log(f"# workload {desc}")
+    ctx = P.context(batch=B, model="true")
+    ctx.set_math(a.math)
+    ctx.set_graphs(a.graphs)
+    ctx.set_snapshots(fmt, ckpt)  # before any run sizes the snapshot storage
+    tile = configure_kernel(ctx, a, B) if headline else configure_kernel(
+        ctx, parse(["--math", str(a.math)]), B)
+    log(f"# step kernel tile {tile}")
+    # observed data: forward on the true model (Misfit::init, misfit.h:135-148)
+    for first in range(0, P.nsrc, B):
+        n = min(B, P.nsrc - first)
+        ctx.forward(first, n, snapshots=False)
+        ctx.obs_from_traces(first, n)
+    P.load_model(ctx, "init")
+    ctx.sync()
+    log("# observed data ready")
+    if headline and a.copy_calib_mb:
+        nb = a.copy_calib_mb << 20
+        src, dst = ctx.alloc(nb), ctx.alloc(nb)
+        ctx.vec_fill(src, 1.0, nb // 4)
+        ctx.vec_copy(dst, src, nb // 4)
+        ctx.sync()
+        ctx.free(src)
+        ctx.free(dst)
+    for _ in range(warmup):
+        ctx.gradient(P.misfit, P.sigma)
+        log("# warm-up gradient done")
+    ctx.sync()
+    ctx.prof_reset()
+    ctx.prof_enable(a.prof_every)
+    t0 = time.perf_counter()
+    f = 0.0
+    for _ in range(steps):
+        f = ctx.gradient(P.misfit, P.sigma)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.prof_enable(0)
+    ms = (t1 - t0) / steps * 1e3
+    work = P.cell_updates()
+    value = work / (ms / 1e3) / 1e9
+    log(f"# {name}: timed {steps} gradients of {P.nsrc} shots: {ms:.1f} ms each, "
+        f"{value:.1f} Gcell/s, misfit {f:.6e}")
+    step_bytes = SH_STEP_BYTES if P.sh and not P.psv else PSV_STEP_BYTES
+    roof = roofline(ctx, inv, a, name, P, B, tile, fmt, ckpt, step_bytes)
     gpu_grad = None
-    if a.workload == "c0":
-        # the last timed gradient and its observed data, for the CPU comparison
+    if name == "c0":
         gpu_grad = (f, ctx.kernels_get(), np.stack([ctx.obs_get(s) for s in range(P.nsrc)]))
     ctx.close()
+    waves_per_gradient = full_shots / P.nsrc
+    rec = {"value": round(value, 3), "ms_per_step": round(ms, 3), "steps": steps,
+           "warmup": warmup, "config": {
+               "workload": desc, "name": name, "nx": P.nx, "nz": P.nz,
+               "shots_per_step": P.nsrc, "shots": full_shots, "receivers": P.nrec, "nt": P.nt,
+               "batch": B, "sfe": P.sfe, "filter_sigma": P.sigma,
+               "misfit": ["waveform", "envelope"][P.misfit],
+               "snapshots": ["fp32", "fp16", "bf16"][fmt] + " in HBM",
+               "checkpoint_interval": ckpt, "step_kernel_tile": tile},
+           "roofline": roof}
+    if waves_per_gradient > 1:
+        rec[f"gradients_per_hr_{full_shots}_shots"] = round(
+            3.6e6 / (ms * waves_per_gradient), 2)
+        rec["note"] = (f"a step is the gradient of {P.nsrc} of the {full_shots} shots; the "
+                       f"{full_shots}-shot gradient is {waves_per_gradient:g} such steps "
+                       "(per-shot work is the same), one filter per step")
+    else:
+        rec["gradients_per_hr"] = round(3.6e6 / ms, 1)
+    return rec, P, gpu_grad
 
+
+def plain_c1(inv, a):
+    """The one-shot-per-launch GPU path (no batching, no graph, one step per
+    launch) on c1: the plain comparison point of the batched path."""
+    P = inv.problems.workload("c1", sigma=a.sigma)[0]
+    pctx = P.context(batch=1, model="true")
+    configure_kernel(pctx, parse(["--math", str(a.math)]), 1, pair=False)
+    pctx.set_math(a.math)
+    pctx.set_graphs(False)
+    for s in range(P.nsrc):
+        pctx.forward(s, 1, snapshots=False)
+        pctx.obs_from_traces(s, 1)
+    P.load_model(pctx, "init")
+    pctx.gradient(P.misfit, P.sigma)
+    pctx.sync()
+    t0 = time.perf_counter()
+    pctx.gradient(P.misfit, P.sigma)
+    pctx.sync()
+    pms = (time.perf_counter() - t0) * 1e3
+    pctx.close()
+    log(f"# plain path: {pms:.1f} ms per gradient")
+    return {"ms_per_gradient": round(pms, 2),
+            "gcell_per_s": round(P.cell_updates() / pms / 1e6, 2),
+            "gradients_per_hr": round(3.6e6 / pms, 1),
+            "note": "c1, one shot per launch, one step per launch, no graph"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): sharding the "
+              "shots over several GPUs is designed but not built.", file=sys.stderr)
+        sys.exit(2)
+    import __graft_entry__ as graft
+
+    inv = graft.load_package()
+    if a.serving_sweep:
+        for s in inv.serving.sweep(a.serving_sweep, log=log,
+                                   snap_format=inv.invhip.INV_SNAP_F16):
+            print(json.dumps(s), flush=True)
+        return
+    head, P, gpu_grad = run_workload(inv, a, a.workload, a.steps, a.warmup, headline=True)
+    extras_names = a.extras if a.extras is not None else ("c1,c2" if a.workload == "c4" else "")
+    extras = {}
+    for w in [x for x in extras_names.split(",") if x and x != a.workload]:
+        rec, _, _ = run_workload(inv, a, w, 2, 1, headline=False)
+        extras[w] = rec
     plain = None
-    if not a.no_plain and a.workload == "c1":
-        pctx = P.context(batch=1, model="true")
-        configure_kernel(pctx, a, 1, pair=False)
-        pctx.set_math(a.math)
-        pctx.set_graphs(False)  # the plain path: one shot per launch, no graph
-        for s in range(P.nsrc):
-            pctx.forward(s, 1, snapshots=False)
-            pctx.obs_from_traces(s, 1)
-        P.load_model(pctx, "init")
-        pctx.gradient(P.misfit, P.sigma)
-        pctx.sync()
-        t0 = time.perf_counter()
-        pctx.gradient(P.misfit, P.sigma)
-        pctx.sync()
-        pms = (time.perf_counter() - t0) * 1e3
-        pctx.close()
-        plain = {"ms_per_gradient": round(pms, 2), "gcell_per_s": round(work / pms / 1e6, 2),
-                 "gradients_per_hr": round(3.6e6 / pms, 1),
-                 "note": "one shot per launch, same kernels, no batching, no graph"}
-        print(f"# plain path: {pms:.1f} ms per gradient", file=sys.stderr, flush=True)
+    if not a.no_plain and (a.workload == "c1" or "c1" in extras):
+        plain = plain_c1(inv, a)
 
     cpu = parity = None
     if not a.no_cpu:
-        # c0 in full; c1 in full on a sample of its shots; c2 / c4 on 8 shots x
-        # 2000 steps / 2 shots x 100 steps (10-30 s of CPU work each)
+        info = cpu_info()
+        # c0 in full; c1 on a sample of its shots; c2 / c4: short samples
+        # (10-30 s of CPU work), plus one thread on a smaller one
         if a.workload == "c0":
             cpu, (fo, *ko) = cpu_baseline(P, P.nsrc, obs=gpu_grad[2])
             fg, kg, _ = gpu_grad
@@ -483,26 +528,38 @@ def main():
                       "misfit_rel_err": abs(fg - fo) / abs(fo)}
             for n, x, y in zip(("k_lambda", "k_mu", "k_rho"), kg, ko):
                 parity[n + "_rel_l2"] = float(np.linalg.norm(x - y) / np.linalg.norm(y))
-            print(f"# parity {parity}", file=sys.stderr, flush=True)
+            log(f"# parity {parity}")
+            one = cpu_baseline(P, 1, nt=200, sigma=0, threads=1)[0]
         elif a.workload == "c1":
             cpu = cpu_baseline(P, min(a.cpu_shots, P.nsrc))[0]
+            one = cpu_baseline(P, 1, nt=400, threads=1)[0]
         elif a.workload == "c2":
             cpu = cpu_baseline(P, 8, nt=2000)[0]
+            one = cpu_baseline(P, 1, nt=200, threads=1)[0]
         else:
-            cpu = cpu_baseline(P, 2, nt=100)[0]
-    acc = accuracy(inv, a) if a.workload in ("c2", "c4") and not a.no_accuracy else None
+            # the filter runs once per step of 4 x 10000 steps: left out of the
+            # short samples, where it would dominate
+            cpu = cpu_baseline(P, 1, nt=100, sigma=0)[0]
+            one = cpu_baseline(P, 1, nt=10, sigma=0, threads=1)[0]
+        cpu.update(info)
+        cpu["one_thread"] = {k: one[k] for k in ("value", "cores", "sample")}
+        log(f"# cpu baseline {cpu}")
+    acc = None
+    if not a.no_accuracy:
+        acc = {w: accuracy(inv, w) for w in ("c4", "c2")
+               if w == a.workload or (w in extras and a.workload == "c4")}
     serving = None
-    if a.serve_requests > 0 and a.workload == "c1":
+    if a.serve_requests > 0 and a.workload in ("c1", "c4"):
         serving = serving_leg(inv, a.serve_requests, a.serve_rate_x, inv.invhip.INV_SNAP_F16)
 
     out = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": head["value"],
         "unit": "Gcell-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms, 3),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -510,23 +567,16 @@ def main():
         "data": ("synthetic (checkerboard model; observed data modelled on the true model)"
                  if a.workload == "c0" else "synthetic (seeded Marmousi-shaped model; observed "
                                             "data modelled on the true model)"),
-        "config": {"workload": desc, "name": a.workload,
-                   "nx": P.nx, "nz": P.nz, "shots": P.nsrc, "receivers": P.nrec, "nt": P.nt,
-                   "batch": B, "sfe": P.sfe, "filter_sigma": P.sigma,
-                   "misfit": ["waveform", "envelope"][P.misfit],
-                   "snapshots": ["fp32", "fp16", "bf16"][snap_fmt] + " in HBM",
-                   "checkpoint_interval": ckpt, "step_kernel_tile": tile},
-        "gradients_per_hr": round(3.6e6 / ms, 1),
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "plain": plain,
-        "serving": serving,
-        "accuracy": acc,
+        "config": head["config"],
     }
+    for k in ("gradients_per_hr", "gradients_per_hr_256_shots", "gradients_per_hr_64_shots",
+              "gradients_per_hr_16_shots", "note"):
+        if k in head:
+            out[k] = head[k]
+    out.update({"roofline": head["roofline"], "cpu_baseline": cpu, "plain": plain,
+                "serving": serving, "accuracy": acc, "extra_workloads": extras or None})
     if parity:
         out["parity"] = parity
-    if ni:
-        out["interaction_avg_us"] = round(ti / ni * 1e3, 2)
     print(json.dumps(out))
 
 

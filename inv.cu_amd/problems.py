@@ -318,3 +318,53 @@ def serving_request(rid, seed=3, dx=10.0, dt=1e-3, sfe=10, f0=8.0, t0=0.15, amp=
 def request_cell_updates(req):
     """Cell-updates of a request: shots * nx * nz * nt * (2 with the adjoint)."""
     return len(req["src_ix"]) * req["nx"] * req["nz"] * req["nt"] * (2 if req["adjoint"] else 1)
+
+
+# ---------------------------------------------------------------------------
+# the benchmark workloads (one definition for the bench and the tests)
+# ---------------------------------------------------------------------------
+WORKLOADS = ("c0", "c1", "c2", "c3", "c4")
+
+
+def workload(name, nt=0, sigma=5):
+    """(problem, shots per launch, snapshot format, checkpoint interval) of a
+    benchmark workload; nt > 0 shortens the run.
+
+    c0  P-SV checkerboard 200x200 at 10 m, 4 explosive sources at 20 cells
+        depth, 40 receivers at 10 cells, dt 1 ms, nt 2000, waveform misfit,
+        sigma 5, FP32 snapshots; all 4 shots per launch.
+    c1  P-SV on the Marmousi-shaped model, 737x240 at 12.5 m, 16 shots at
+        f0 5 Hz, 368 surface receivers, dt 1 ms, nt 4000, FP32 snapshots; all
+        16 shots per launch.
+    c2  SH on the same generator at 1474x480 and 6.25 m (seed 2), 64 shots at
+        f0 8 Hz and 737 receivers on the seafloor (row 14: SH waves do not
+        enter the water layer, where mu = 0), dt 0.5 ms, nt 8000, FP16
+        snapshots every 10 steps, all 64 shots' snapshots resident (145 GB).
+    c4  P-SV on the generator at 4096x2048 and 5 m (seed 4), 256 shots at
+        f0 12 Hz, 2048 receivers, dt 0.4 ms, nt 10000, envelope misfit, FP16
+        snapshots; four shots per launch keep all of their snapshots resident
+        (268 GB), so the adjoint recomputes nothing.
+    (c3, the mixed-size request load, is built by `serving_request`.)"""
+    if name == "c0":
+        P = checker(nt=nt or 2000, sigma=sigma, explosive=True)
+        return P, 4, invhip.INV_SNAP_F32, 0
+    if name == "c1":
+        return marmousi(nt=nt or 4000, sigma=sigma), 16, invhip.INV_SNAP_F32, 0
+    if name == "c2":
+        P = marmousi(nx=1474, nz=480, dx=6.25, dt=5e-4, nt=nt or 8000, nsrc=64, nrec=737,
+                     f0=8.0, t0=0.15, seed=2, sh=1, psv=0, sigma=sigma, src_depth=14,
+                     rec_depth=14, name="marmousi-sh")
+        return P, 64, invhip.INV_SNAP_F16, 0
+    if name == "c4":
+        P = marmousi(nx=4096, nz=2048, dx=5.0, dt=4e-4, nt=nt or 10000, nsrc=256, nrec=2048,
+                     f0=12.0, t0=0.1, seed=4, sigma=sigma, name="marmousi-large")
+        return P.with_(misfit=1), 4, invhip.INV_SNAP_F16, 0
+    raise ValueError(f"unknown workload {name!r}")
+
+
+def shots(P, pick):
+    """The problem restricted to the shots `pick` (indices into its sources)."""
+    pick = np.asarray(pick)
+    return P.with_(src_ix=P.src_ix[pick], src_iz=P.src_iz[pick], stf_x=P.stf_x[pick],
+                   stf_y=P.stf_y[pick], stf_z=P.stf_z[pick],
+                   src_type=None if P.src_type is None else P.src_type[pick])

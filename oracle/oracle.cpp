@@ -11,6 +11,7 @@
 // reference's algorithm on host cores.
 #include "oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -567,21 +568,29 @@ void gauss(int nx, int nz, int sigma, T *data) {
         for (int n = 0; n <= j; n++) acc += w[j - n];
         sz[j] = acc;
     }
+    // Terms whose weight underflowed to exactly 0 (m > mcut) add +0 to the
+    // partial sum and are skipped; every sum keeps the reference's order
+    // (n ascending).
+    int mcut = 0;
+    for (int m = 0; m < nmax; m++)
+        if (w[m] != (T)0) mcut = m;
     const long N = (long)nx * nz;
     std::vector<T> tmp(N);
 #pragma omp parallel for num_threads(threads()) schedule(static)
-    for (int i = 0; i < nx; i++) {  // filterKernelX
-        for (int j = 0; j < nz; j++) {
-            T acc = 0;
-            for (int n = 0; n <= i; n++) acc += w[i - n] * data[(long)n * nz + j];
-            tmp[(long)i * nz + j] = acc;
+    for (int i = 0; i < nx; i++) {  // filterKernelX, the j loop innermost
+        T *out = tmp.data() + (long)i * nz;
+        for (int j = 0; j < nz; j++) out[j] = 0;
+        for (int n = std::max(0, i - mcut); n <= i; n++) {
+            const T wn = w[i - n];
+            const T *row = data + (long)n * nz;
+            for (int j = 0; j < nz; j++) out[j] += wn * row[j];
         }
     }
 #pragma omp parallel for num_threads(threads()) schedule(static)
     for (int i = 0; i < nx; i++) {  // filterKernelZ
         for (int j = 0; j < nz; j++) {
             T acc = 0;
-            for (int n = 0; n <= j; n++) acc += w[j - n] * tmp[(long)i * nz + n];
+            for (int n = std::max(0, j - mcut); n <= j; n++) acc += w[j - n] * tmp[(long)i * nz + n];
             T gs = sx[i] * sz[j];  // initialiseGaussian
             data[(long)i * nz + j] = acc / gs;
         }

@@ -1,0 +1,245 @@
+"""GPU parity at the sizes of the benchmark workloads (inv.cu_amd/problems.py
+`workload`), through the C ABI, against the oracle.
+
+* c1 (737x240 P-SV, nt 4000): exact math equals the FP32 oracle for two
+  shots; the benchmark's batched launch (16 shots, autotuned tiles, graphs,
+  fast math) stays within 1e-5 of it and each of its slots equals the
+  one-shot-per-launch path; the fast-math gradient of two shots within 1e-4.
+* c2 (1474x480 SH, nt 8000): exact math equals the FP32 oracle for one shot;
+  fast-math gradients with FP32 snapshots within 1e-4 of the FP32 oracle;
+  the error of FP16 / BF16 snapshot storage against the FP64 oracle at the
+  full nt.
+* c3 (mixed-size requests): one request at each of the four grid sizes,
+  alone and in a mix sharing launches, equal bit for bit.
+* c4 (4096x2048 P-SV): exact math equals the FP32 oracle at nt 100; the
+  two-step kernel equals one step per launch bit for bit for forward runs
+  and FP16-snapshot gradients at nt 200.
+Each test prints its case before the first device call.
+"""
+import numpy as np
+import pytest
+
+from conftest import inv, oracle_problem, oracle_traces, rel
+
+pytestmark = pytest.mark.gpu
+
+PR = inv.problems
+H = inv.invhip
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _obs(P, batch):
+    """Observed traces of every shot, modelled on the true model."""
+    ctx = P.context(batch=batch, model="true")
+    out = []
+    for first in range(0, P.nsrc, batch):
+        n = min(batch, P.nsrc - first)
+        ctx.forward(first, n, snapshots=False)
+        out += [ctx.traces_get(b) for b in range(n)]
+    ctx.close()
+    return np.stack(out)
+
+
+def test_c1_exact_matches_oracle(gpu):
+    P = PR.shots(PR.workload("c1")[0], [3, 11])
+    _say(f"c1 exact: {P.nx}x{P.nz} nt={P.nt}, shots 3 and 11, two per launch")
+    ctx = P.context(batch=2)
+    ctx.set_math(H.INV_MATH_EXACT)
+    tile = ctx.autotune(0, 2, steps=10)
+    _say(f"  tile {tile} pair {ctx.get_pair()}")
+    ctx.forward(0, 2, snapshots=False)
+    O = oracle_problem(P)
+    for s in range(2):
+        otr, _, ost = O.forward(s)
+        assert np.array_equal(ctx.traces_get(s), oracle_traces(otr, P)), f"shot {s}: traces"
+        assert np.array_equal(ctx.state_get(s), ost), f"shot {s}: wavefields"
+    ctx.close()
+
+
+def test_c1_batched_fast_matches_oracle_and_plain(gpu):
+    P, B, fmt, ckpt = PR.workload("c1")
+    _say(f"c1 batched: {P.nx}x{P.nz} nt={P.nt}, {P.nsrc} shots per launch, fast math")
+    ctx = P.context(batch=B)
+    ctx.set_snapshots(fmt, ckpt)
+    tile = ctx.autotune(0, B, steps=10)
+    pair = ctx.get_pair()
+    _say(f"  tile {tile} pair {pair}")
+    ctx.forward(0, B, snapshots=True)
+    tr = [ctx.traces_get(b) for b in range(B)]
+    st = [ctx.state_get(b) for b in range(B)]
+    ctx.close()
+    O = oracle_problem(P)
+    for s in (0, 9):
+        otr, _, ost = O.forward(s)
+        e_tr, e_st = rel(tr[s], oracle_traces(otr, P)), rel(st[s], ost)
+        _say(f"  shot {s}: traces {e_tr:.2e}, wavefields {e_st:.2e} vs the FP32 oracle")
+        assert e_tr < 1e-5 and e_st < 1e-5
+    # the plain path: one shot per launch, no graph, same tiles
+    plain = P.context(batch=1)
+    plain.set_snapshots(fmt, ckpt)
+    plain.set_tile(tile[1], tile[2])
+    if pair[0]:
+        plain.set_pair(*pair)
+    else:
+        plain.set_pair(0)
+    plain.set_graphs(False)
+    for s in range(P.nsrc):
+        plain.forward(s, 1, snapshots=True)
+        assert np.array_equal(plain.traces_get(0), tr[s]), f"shot {s}: traces"
+        assert np.array_equal(plain.state_get(0), st[s]), f"shot {s}: wavefields"
+    plain.close()
+
+
+def test_c1_gradient_matches_oracle(gpu):
+    P = PR.shots(PR.workload("c1", sigma=5)[0], [2, 13])
+    _say(f"c1 gradient: {P.nx}x{P.nz} nt={P.nt}, 2 shots, fast math, sigma {P.sigma}")
+    obs = _obs(P, 2)
+    ctx = P.context(batch=2)
+    ctx.autotune(0, 2, steps=10)
+    for s in range(2):
+        ctx.obs_set(s, obs[s])
+    f = ctx.gradient(P.misfit, P.sigma)
+    ks = ctx.kernels_get()
+    ctx.close()
+    fo, *ko = oracle_problem(P).gradient(obs, P.misfit, P.sigma)
+    _say(f"  misfit {f:.6e} oracle {fo:.6e}")
+    assert abs(f - fo) <= 1e-5 * abs(fo)
+    for name, a, b in zip(("k_lambda", "k_mu", "k_rho"), ks, ko):
+        e = rel(a, b)
+        _say(f"  {name}: {e:.2e}")
+        assert e < 1e-4
+
+
+def test_c2_exact_matches_oracle(gpu):
+    P = PR.shots(PR.workload("c2")[0], [17])
+    _say(f"c2 exact: {P.nx}x{P.nz} SH nt={P.nt}, shot 17")
+    ctx = P.context(batch=1)
+    ctx.set_math(H.INV_MATH_EXACT)
+    ctx.autotune(0, 1, steps=10)
+    _say(f"  pair {ctx.get_pair()}")
+    ctx.forward(0, 1, snapshots=False)
+    otr, _, ost = oracle_problem(P).forward(0)
+    assert np.array_equal(ctx.traces_get(0), oracle_traces(otr, P))
+    assert np.array_equal(ctx.state_get(0), ost)
+    ctx.close()
+
+
+def _c2_pair():
+    P = PR.shots(PR.workload("c2", sigma=0)[0], [20, 44])
+    return P, _obs(P, 2)
+
+
+def test_c2_gradient_matches_oracle(gpu):
+    P, obs = _c2_pair()
+    _say(f"c2 gradient: {P.nx}x{P.nz} SH nt={P.nt}, 2 shots, FP32 snapshots, fast math")
+    ctx = P.context(batch=2)
+    ctx.autotune(0, 2, steps=10)
+    for s in range(2):
+        ctx.obs_set(s, obs[s])
+    f = ctx.gradient(P.misfit, 0)
+    ks = ctx.kernels_get()
+    ctx.close()
+    fo, *ko = oracle_problem(P).gradient(obs, P.misfit, 0)
+    _say(f"  misfit {f:.6e} oracle {fo:.6e}")
+    assert abs(f - fo) <= 1e-5 * abs(fo)
+    for name, a, b in zip(("k_mu", "k_rho"), ks[1:], ko[1:]):
+        e = rel(a, b)
+        _say(f"  {name}: {e:.2e}")
+        assert e < 1e-4
+    assert not np.any(ks[0])
+
+
+def test_c2_snapshot_precision_vs_fp64(gpu):
+    """Error of the gradient against the FP64 oracle at the full nt of c2,
+    for FP32, FP16 and BF16 snapshot storage (all arithmetic FP32); the
+    bounds are the measured errors rounded up."""
+    P, obs = _c2_pair()
+    _say(f"c2 precision: {P.nx}x{P.nz} SH nt={P.nt}, 2 shots, FP64 oracle")
+    fo, *ko = oracle_problem(P).gradient(obs.astype(np.float64), P.misfit, 0, dtype=np.float64)
+    bounds = {H.INV_SNAP_F32: 1e-5, H.INV_SNAP_F16: 1e-3, H.INV_SNAP_BF16: 1e-2}
+    solid = P.mu0 > 0
+    for fmt, bound in bounds.items():
+        ctx = P.context(batch=2)
+        ctx.set_snapshots(fmt, 0)
+        for s in range(2):
+            ctx.obs_set(s, obs[s])
+        f = ctx.gradient(P.misfit, 0)
+        ks = ctx.kernels_get()
+        ctx.close()
+        for name, a, b in zip(("k_mu", "k_rho"), ks[1:], ko[1:]):
+            e = rel(a[solid], b[solid])
+            _say(f"  format {fmt} {name} (solid cells): {e:.2e}; misfit {abs(f - fo) / fo:.1e}")
+            assert e < bound
+
+
+def test_c3_grid_sizes_alone_equal_in_the_mix(gpu):
+    from test_gpu_serving import make_req, run_all
+
+    reqs = []
+    for k, (nx, nz) in enumerate(PR.SERVE_GRIDS):
+        reqs.append(make_req(60 + 2 * k, nx, nz, 200, sh=bool(k % 2), adjoint=True, nshots=2))
+        reqs.append(make_req(61 + 2 * k, nx, nz, 160, sh=bool(k % 2), adjoint=k < 2,
+                             nshots=1))
+    _say(f"c3: {len(reqs)} requests on grids {PR.SERVE_GRIDS}")
+    with H.Server(max_lanes=3, max_slots=4, max_bypass=4) as srv:
+        mixed = run_all(srv, reqs, together=True)
+    with H.Server(max_lanes=1, max_slots=4) as srv:
+        alone = run_all(srv, reqs, together=False)
+    shared = 0
+    for q in reqs:
+        (ia, ta, ka), (ib, tb, kb) = mixed[q["rid"]], alone[q["rid"]]
+        assert ia["status"] == ib["status"] == H.INV_REQ_DONE
+        shared += ia["batch_requests"] > 1
+        assert np.array_equal(ta, tb), f"request {q['rid']} ({q['nx']}x{q['nz']}): traces"
+        if q["adjoint"]:
+            assert ia["misfit"] == ib["misfit"]
+            for x, y in zip(ka, kb):
+                assert np.array_equal(x, y), f"request {q['rid']}: kernels"
+    assert shared >= 2
+
+
+def _c4(nt, pick):
+    return PR.shots(PR.workload("c4", nt=nt, sigma=0)[0], pick)
+
+
+def test_c4_exact_matches_oracle(gpu):
+    P = _c4(100, [100])
+    _say(f"c4 exact: {P.nx}x{P.nz} P-SV nt={P.nt}, shot 100")
+    ctx = P.context(batch=1)
+    ctx.set_math(H.INV_MATH_EXACT)
+    ctx.set_tile(16, 2)
+    ctx.set_pair(20, 2)
+    ctx.forward(0, 1, snapshots=False)
+    otr, _, ost = oracle_problem(P).forward(0)
+    assert np.array_equal(ctx.traces_get(0), oracle_traces(otr, P))
+    assert np.array_equal(ctx.state_get(0), ost)
+    ctx.close()
+
+
+def test_c4_two_step_equals_one_step(gpu):
+    P = _c4(200, [40, 200])
+    _say(f"c4 two-step vs one-step: {P.nx}x{P.nz} nt={P.nt}, 2 shots, FP16 snapshots")
+    obs = _obs(P, 2)
+    res = []
+    for pair in ((20, 2), (0, 2)):
+        ctx = P.context(batch=2)
+        ctx.set_snapshots(H.INV_SNAP_F16, 0)
+        ctx.set_tile(16, 2)
+        ctx.set_pair(*pair)
+        ctx.forward(0, 2, snapshots=True)
+        fw = [(ctx.traces_get(b), ctx.state_get(b)) for b in range(2)]
+        for s in range(2):
+            ctx.obs_set(s, obs[s])
+        f = ctx.gradient(P.misfit, 0)
+        res.append((fw, f, ctx.kernels_get()))
+        ctx.close()
+    (fa, ga, ka), (fb, gb, kb) = res
+    for (ta, sa), (tb, sb) in zip(fa, fb):
+        assert np.array_equal(ta, tb) and np.array_equal(sa, sb)
+    assert ga == gb
+    for x, y in zip(ka, kb):
+        assert np.array_equal(x, y)
+    assert all(np.any(k) for k in ka)

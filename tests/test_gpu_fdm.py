@@ -52,12 +52,17 @@ def same(a, ref, fast, tol=1e-5):
 
 def boundary_problem(**kw):
     """A grid with several tiles in both directions and sources/receivers on
-    tile edges (rows 15/16/31/32, columns 63/64/65/127/128)."""
-    P = inv.problems.tiny(nx=70, nz=150, nt=160, nsrc=4, nrec=10, **kw)
+    tile edges: rows 15/16/31/32 (tiles of 4, 8 and 16 rows); columns 57/58
+    and 115/116 (58-column tiles, one column per lane), 119/120 and 239/240
+    (120 columns, two per lane), 247/248/249 (248 columns, four per lane),
+    63/64/127/128 (64-column tiles).  nz = 256 is divisible by 4, so every
+    tile runs and the four-column tiles have a z-tile edge."""
+    P = inv.problems.tiny(nx=70, nz=256, nt=160, nsrc=4, nrec=14, **kw)
     src_ix = np.array([15, 16, 31, 32], np.int32)
-    src_iz = np.array([63, 64, 65, 128], np.int32)
-    rec_ix = np.array([8, 15, 16, 17, 31, 32, 33, 47, 48, 60], np.int32)
-    rec_iz = np.array([63, 64, 65, 127, 128, 3, 64, 63, 64, 146], np.int32)
+    src_iz = np.array([64, 116, 240, 248], np.int32)
+    rec_ix = np.array([8, 15, 16, 17, 31, 32, 33, 47, 48, 60, 16, 31, 40, 63], np.int32)
+    rec_iz = np.array([57, 58, 63, 64, 119, 120, 127, 247, 248, 249, 115, 239, 3, 252],
+                      np.int32)
     return P.with_(src_ix=src_ix, src_iz=src_iz, rec_ix=rec_ix, rec_iz=rec_iz)
 
 
