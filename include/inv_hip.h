@@ -209,6 +209,23 @@ int inv_memcpy_d2h(inv_ctx *ctx, void *host, const void *dev, size_t bytes);
  * segment last used). */
 int inv_state_get(inv_ctx *ctx, int slot, float *host);
 int inv_snapshots_get(inv_ctx *ctx, int slot, float *host);
+/* Writes one slot's wavefields (host layout of inv_state_get) into the state
+ * that the next half-step or inv_state_get sees. */
+int inv_state_set(inv_ctx *ctx, int slot, const float *host);
+/* The reference's separable phases of a time step (divS and divV,
+ * solver/fdm.h:273-294), on the current wavefields of slots 0..count-1, in
+ * place:
+ *   inv_step_velocity: stress divergence (divSY / divSXZ, fdm.h:76-102), the
+ *     point forces of time step `it` of shots first.. (addSTF, fdm.h:163-180;
+ *     first = -1 with count 1 fires every source) and the velocity and
+ *     displacement update (updateVY / updateVXZ, fdm.h:150-161);
+ *   inv_step_stress: velocity gradient and stress update (divVY / divVXZ,
+ *     updateSY / updateSXZ, fdm.h:103-149).
+ * One time step of runForward is inv_step_velocity(it) then inv_step_stress.
+ * No receiver recording or snapshots; explosive sources are refused
+ * (INV_ERR_UNSUPPORTED).  Test hooks: the runs above are faster. */
+int inv_step_velocity(inv_ctx *ctx, int first, int count, int it);
+int inv_step_stress(inv_ctx *ctx, int count);
 /* ---- measurement -------------------------------------------------------
  * Sampled kernel timing: with every > 0, each propagation kernel launched
  * directly on a time step it with it % every == 0 is bracketed by HIP events

@@ -1002,6 +1002,55 @@ int inv_state_get(inv_ctx *ctx, int slot, float *host) {
     });
 }
 
+int inv_state_set(inv_ctx *ctx, int slot, const float *host) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(c.have_grid && host && slot >= 0 && slot < c.g.batch, INV_ERR_ARG, "bad slot");
+        sync(c);
+        const size_t b = c.N * sizeof(float);
+        int f = 0;
+        auto put = [&](float *d) {
+            INV_HIP(hipMemcpy(reinterpret_cast<char *>(d) + (size_t)slot * b,
+                              reinterpret_cast<const char *>(host) + (size_t)f++ * b, b,
+                              hipMemcpyHostToDevice));
+        };
+        auto cur = [&](DevBuf &x, DevBuf &y) { return cur_field(c, x, y); };
+        if (c.g.sh) {
+            put(cur(c.vy, c.vy_b)); put(c.uy.as<float>());
+            put(cur(c.sxy, c.sxy_b)); put(cur(c.szy, c.szy_b));
+        }
+        if (c.g.psv) {
+            put(cur(c.vx, c.vx_b)); put(cur(c.vz, c.vz_b)); put(c.ux.as<float>());
+            put(c.uz.as<float>()); put(cur(c.sxx, c.sxx_b)); put(cur(c.szz, c.szz_b));
+            put(cur(c.sxz, c.sxz_b));
+        }
+    });
+}
+
+static void check_half_step(Ctx &c, int count) {
+    require_ready(c);
+    INV_REQUIRE(count >= 1 && count <= c.g.batch, INV_ERR_ARG, "count must be in [1, batch]");
+    INV_REQUIRE(!c.slot_models, INV_ERR_UNSUPPORTED, "half-steps need the context's own model");
+    INV_REQUIRE(c.run_kind == 0, INV_ERR_STATE, "an incremental run is in progress");
+}
+
+int inv_step_velocity(inv_ctx *ctx, int first, int count, int it) {
+    return guard(ctx, [&](Ctx &c) {
+        check_half_step(c, count);
+        check_slots(c, first, count);
+        INV_REQUIRE(it >= 0 && it < c.g.nt, INV_ERR_ARG, "time step out of range");
+        INV_REQUIRE(!c.any_explosive(), INV_ERR_UNSUPPORTED,
+                    "half-steps inject point forces only (no explosive sources)");
+        fdm_half_step(c, 0, first, count, it);
+    });
+}
+
+int inv_step_stress(inv_ctx *ctx, int count) {
+    return guard(ctx, [&](Ctx &c) {
+        check_half_step(c, count);
+        fdm_half_step(c, 1, 0, count, 0);
+    });
+}
+
 // IEEE binary16 to float (subnormals included)
 static float half_to_float(uint16_t h) {
     const uint32_t sign = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;

@@ -765,6 +765,68 @@ void fdm_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
     }
 }
 
+// One half-step of the split kernels on the current state set of slots
+// 0..count-1, in place (the velocity half-step reads only the stress and
+// writes v and u; the stress half-step reads only v and writes the stress).
+// which 0: divS + addSTF of time step `it` for shots first.. + updateV;
+// which 1: divV + updateS (solver/fdm.h:273-294).  No receiver recording,
+// snapshots or explosive sources.
+template <bool SH, bool PSV, bool FAST>
+static void launch_half(Ctx &c, int which, const VelArgs &va, const StrArgs &sa, int count) {
+    const dim3 grid = tile_grid(c, count), block(TILE_Z, TILE_X);
+    if (which == 0) k_velocity<SH, PSV, FAST><<<grid, block, 0, c.stream>>>(va);
+    else k_stress<SH, PSV, FAST><<<grid, block, 0, c.stream>>>(sa);
+}
+
+template <bool FAST>
+static void launch_half_m(Ctx &c, int which, const VelArgs &va, const StrArgs &sa, int count) {
+    if (c.g.sh && c.g.psv) launch_half<true, true, FAST>(c, which, va, sa, count);
+    else if (c.g.sh) launch_half<true, false, FAST>(c, which, va, sa, count);
+    else launch_half<false, true, FAST>(c, which, va, sa, count);
+}
+
+void fdm_half_step(Ctx &c, int which, int first, int count, int it) {
+    VelArgs va;
+    StrArgs sa;
+    make_common(c, va, sa);
+    auto cur = [&](DevBuf &x, DevBuf &y) { return cur_field(c, x, y); };
+    if (c.g.psv) {
+        va.vx = cur(c.vx, c.vx_b);
+        va.vz = cur(c.vz, c.vz_b);
+        va.sxx = cur(c.sxx, c.sxx_b);
+        va.szz = cur(c.szz, c.szz_b);
+        va.sxz = cur(c.sxz, c.sxz_b);
+    }
+    if (c.g.sh) {
+        va.vy = cur(c.vy, c.vy_b);
+        va.sxy = cur(c.sxy, c.sxy_b);
+        va.szy = cur(c.szy, c.szy_b);
+    }
+    sa.vx = va.vx; sa.vz = va.vz; sa.vy = va.vy;
+    sa.sxx = const_cast<float *>(va.sxx);
+    sa.szz = const_cast<float *>(va.szz);
+    sa.sxz = const_cast<float *>(va.sxz);
+    sa.sxy = const_cast<float *>(va.sxy);
+    sa.szy = const_cast<float *>(va.szy);
+    if (which == 0) {
+        fdm_prepare_model(c);
+        PointList &sl = source_list(c, first, count, 0);
+        const long nt = c.g.nt, nsrc = c.nsrc;
+        va.inj_x = c.stf.as<float>();
+        va.inj_y = c.stf.as<float>() + nsrc * nt;
+        va.inj_z = c.stf.as<float>() + 2 * nsrc * nt;
+        va.inj_ptr = sl.ptr.as<int>();
+        va.inj_cell = sl.cell.as<int>();
+        va.inj_row = sl.row.as<int>();
+        va.inj_per_slot = 1;
+        va.ntiles = sl.ntiles;
+        va.it = it;
+    }
+    if (c.math == INV_MATH_FAST) launch_half_m<true>(c, which, va, sa, count);
+    else launch_half_m<false>(c, which, va, sa, count);
+    INV_HIP(hipGetLastError());
+}
+
 template <bool SH, bool PSV, int FMT>
 static void launch_interact_f(Ctx &c, const IntArgs &ia, int count) {
     if (c.math == INV_MATH_FAST)

@@ -210,6 +210,9 @@ void build_point_list(Ctx &c, PointList &pl, int nsets, const std::vector<std::v
 void fdm_forward(Ctx &c, int first, int count, int flags, int it0, int it1);
 void fdm_adjoint(Ctx &c, int first, int count, int it0, int it1);
 void fdm_zero_state(Ctx &c, int count);
+// one split-kernel half-step on the current state set (inv_step_velocity:
+// which 0, time step it of shots first..; inv_step_stress: which 1)
+void fdm_half_step(Ctx &c, int which, int first, int count, int it);
 void fdm_prepare_model(Ctx &c);
 void fdm_kslot_reduce(Ctx &c, int count);
 // dt / rho of the fast math mode into bdt[n] on a stream

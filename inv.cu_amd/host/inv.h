@@ -150,30 +150,48 @@ class Source {
   public:
     size_t nt = 0;
     float dt = 0;
-    vector<float> stf_x, stf_y, stf_z;
-    virtual void init(const float *cfg, size_t nt0, float dt0) {
+    float *stf_x = nullptr, *stf_y = nullptr, *stf_z = nullptr;  // host [nt]
+    // the normalised wavelet of the source type into stf[nt]
+    virtual void create(float *stf, float f0, float t0) = 0;
+    virtual void init(float *cfg, size_t nt0, float dt0) {
         f0 = cfg[0];
         t0 = cfg[1];
         angle = cfg[2];
         amp = cfg[3];
         nt = nt0;
         dt = dt0;
-        stf_x.assign(nt, 0);
-        stf_y.assign(nt, 0);
-        stf_z.assign(nt, 0);
-        create();
+        release();
+        stf_x = new float[nt]();
+        stf_y = new float[nt]();
+        stf_z = new float[nt]();
+        vector<float> stf(nt, 0.0f);
+        create(stf.data(), f0, t0);
+        // the components in the FP32 expression order of inv_source_ricker
+        for (size_t it = 0; it < nt; it++) {
+            stf_x[it] = amp * stf[it] * cosf(angle);
+            stf_y[it] = amp * stf[it];
+            stf_z[it] = amp * stf[it] * sinf(angle);
+        }
     }
-    virtual void create() = 0;
     // injection mechanism (inv_sources_type): 0 point force, 1 explosive
     virtual int mechanism() const { return 0; }
-    virtual ~Source() {}
+    virtual ~Source() { release(); }
+
+  private:
+    void release() {
+        delete[] stf_x;
+        delete[] stf_y;
+        delete[] stf_z;
+        stf_x = stf_y = stf_z = nullptr;
+    }
 };
 
 class RickerSource : public Source {
   public:
-    void create() override {
-        int rc = inv_source_ricker((int)nt, dt, f0, t0, angle, amp, stf_x.data(), stf_y.data(),
-                                   stf_z.data());
+    // the library's Ricker wavelet (amp 1, angle 0: stf_y is the wavelet)
+    void create(float *stf, float f0_, float t0_) override {
+        vector<float> sx(nt), sz(nt);
+        int rc = inv_source_ricker((int)nt, dt, f0_, t0_, 0.0f, 1.0f, sx.data(), stf, sz.data());
         if (rc != INV_OK) throw std::runtime_error("invalid Ricker parameters");
     }
 };
@@ -323,11 +341,12 @@ class FdmSolver : public Solver {
             szi[is] = inv_calc_idx(s[1], zmax, nz);
             std::unique_ptr<Source> src(module::source((size_t)s[2]));
             if (!src) throw std::runtime_error("unknown source type");
-            src->init(s.data() + 3, nt, dt);
+            float cfg[4] = {s[3], s[4], s[5], s[6]};
+            src->init(cfg, nt, dt);
             mech[is] = src->mechanism();
-            std::copy(src->stf_x.begin(), src->stf_x.end(), stfx.begin() + is * nt);
-            std::copy(src->stf_y.begin(), src->stf_y.end(), stfy.begin() + is * nt);
-            std::copy(src->stf_z.begin(), src->stf_z.end(), stfz.begin() + is * nt);
+            std::copy(src->stf_x, src->stf_x + nt, stfx.begin() + is * nt);
+            std::copy(src->stf_y, src->stf_y + nt, stfy.begin() + is * nt);
+            std::copy(src->stf_z, src->stf_z + nt, stfz.begin() + is * nt);
         }
         INVH(ctx, inv_sources_set(ctx, (int)nsrc, sx.data(), szi.data(), stfx.data(),
                                   stfy.data(), stfz.data()));
@@ -863,9 +882,10 @@ class Optimizer {
         return false;
     }
     // goodEnough (optimizer.h:235-256); x0 is set from the parabola when it
-    // opens upward
-    bool goodEnough(const double *x, const double *f, size_t n, double &x0) {
+    // opens upward (convex is then true)
+    bool goodEnough(const double *x, const double *f, size_t n, double &x0, bool &convex) {
         const double thresh = std::log10(ls_thresh);
+        convex = false;
         if (!checkBracket(x, f, n)) return false;
         double p[3];
         const size_t idx = argmin(f, n) - 1;
@@ -874,6 +894,7 @@ class Optimizer {
         if (p[0] <= 0) {
             std::cout << "  line search error" << std::endl;
         } else {
+            convex = true;
             x0 = -p[1] / (2 * p[0]);
             for (size_t i = 1; i < n; i++)
                 if (std::fabs(std::log10(x[i] / x0)) < thresh) return true;
@@ -900,12 +921,17 @@ class Optimizer {
             }
             status = 0;
         } else if (checkBracket(x.data(), f.data(), step_count + 1)) {
-            // if the fitted parabola does not open upward the reference keeps an
-            // unset step; the best trial so far is used instead
-            alpha = x[argmin(f.data(), step_count + 1)];
-            double x0 = alpha;
-            if (goodEnough(x.data(), f.data(), step_count + 1, x0)) {
-                alpha = x[argmin(f.data(), step_count + 1)];
+            // A bracket: accept the best trial when a trial lies near the
+            // parabola's vertex, else try the vertex.  Two deviations, so the
+            // loop always ends: when the fitted parabola does not open upward
+            // (the reference then leaves the step unset) the best trial is
+            // accepted, and so it is once the search has made ls_step trials.
+            const double best = x[argmin(f.data(), step_count + 1)];
+            double x0 = best;
+            bool convex = false;
+            if (goodEnough(x.data(), f.data(), step_count + 1, x0, convex) || !convex ||
+                step_count >= ls_step) {
+                alpha = best;
                 status = 1;
             } else {
                 alpha = x0;
@@ -972,10 +998,12 @@ class Optimizer {
         ls_count++;
     }
 
-    virtual double calcStep(size_t step_count, double step_max, int &status) = 0;
+    // the step rule (optimizer.h:359, the reference's float signature; the
+    // rules below compute in FP64 and round the step once)
+    virtual float calcStep(size_t step_count, float step_max, int &status) = 0;
 
     // lineSearch (optimizer.h:360-419)
-    virtual int lineSearch(double f) {
+    virtual int lineSearch(float f) {
         int status = 0;
         double alpha = 0;
         const double norm_m = p_amax(m_new), norm_p = p_amax(p_new);
@@ -1102,8 +1130,8 @@ class NLCGOptimizer : public Optimizer {
     }
     double fletcher() { return p_dot(g_new, g_new) / p_dot(g_old, g_old); }
 
-    double calcStep(size_t step_count, double step_max, int &status) override {
-        return bracket(step_count, step_max, status);
+    float calcStep(size_t step_count, float step_max, int &status) override {
+        return (float)bracket(step_count, step_max, status);
     }
     // computeDirection (nlcg.h:24-52)
     int computeDirection() override {
@@ -1149,8 +1177,8 @@ class LBFGSOptimizer : public Optimizer {
     vector<float *> S, Y;  // mem buffers of 3 fields each, device resident
     vector<int> si;        // slot -> buffer index
 
-    double calcStep(size_t step_count, double step_max, int &status) override {
-        return backtrack(step_count, step_max, status);
+    float calcStep(size_t step_count, float step_max, int &status) override {
+        return (float)backtrack(step_count, step_max, status);
     }
     void restartSearch() override {
         Optimizer::restartSearch();

@@ -43,7 +43,7 @@ class GridDesc(C.Structure):
 
 
 # Every exported function and its argument types (checked against the header
-# by tests/test_capi_exports.py).
+# by tests/test_capi.py).
 _vp = C.c_void_p
 _i = C.c_int
 _f = C.c_float
@@ -93,6 +93,9 @@ SIGNATURES = {
     "inv_memcpy_h2d": [_vp, _vp, _vp, _sz],
     "inv_memcpy_d2h": [_vp, _vp, _vp, _sz],
     "inv_state_get": [_vp, _i, _vp],
+    "inv_state_set": [_vp, _i, _vp],
+    "inv_step_velocity": [_vp, _i, _i, _i],
+    "inv_step_stress": [_vp, _i],
     "inv_snapshots_get": [_vp, _i, _vp],
     "inv_set_step_limit": [_vp, _i],
     "inv_set_snapshots": [_vp, _i, _i],
@@ -402,6 +405,18 @@ class Context:
         a = np.empty((self.nstate, self.N), np.float32)
         self._chk(self.L.inv_state_get(self.h, slot, _p(a)))
         return a
+
+    def state_set(self, slot, a):
+        a = np.ascontiguousarray(a, np.float32).reshape(self.nstate, self.N)
+        self._chk(self.L.inv_state_set(self.h, slot, _p(a)))
+
+    def step_velocity(self, first, count, it):
+        """Velocity half-step of time step it (divS + addSTF + updateV)."""
+        self._chk(self.L.inv_step_velocity(self.h, first, count, it))
+
+    def step_stress(self, count=1):
+        """Stress half-step (divV + updateS)."""
+        self._chk(self.L.inv_step_stress(self.h, count))
 
     def snapshots_get(self, slot=0):
         ckpt = getattr(self, "_ckpt", 0)

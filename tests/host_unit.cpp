@@ -32,8 +32,8 @@ struct TestOpt : Optimizer {
     using Optimizer::bracket;
     using Optimizer::polyfit;
     int rule = 0;  // 0 bracket (NLCG), 1 backtrack (L-BFGS)
-    double calcStep(size_t sc, double sm, int &st) override {
-        return rule ? backtrack(sc, sm, st) : bracket(sc, sm, st);
+    float calcStep(size_t sc, float sm, int &st) override {
+        return (float)(rule ? backtrack(sc, sm, st) : bracket(sc, sm, st));
     }
     int computeDirection() override { return 0; }
     void history(vector<double> lens, vector<double> vals, vector<double> gtg,
@@ -133,6 +133,16 @@ static void test_steps() {
     o.history({0, 0.25, 0.30}, {10, 8, 9}, {4}, {-2}, 10, 1.2);
     a = o.bracket(2, 100, st);
     CHECK(near(a, 0.16785714285714282, 1e-9) && st == 0);
+    // bracketed but the least-squares parabola through the four points from
+    // the one before the best opens downward (numpy.polyfit: -0.725 x^2 ...):
+    // the best trial is accepted rather than evaluated again
+    o.history({0, 1, 2, 3, 4}, {10, 0.1, 0, 3, 0}, {4}, {-2}, 10, 1.2);
+    a = o.bracket(4, 100, st);
+    CHECK(near(a, 2) && st == 1);
+    // bracketed, vertex far from every trial, but ls_step trials made: accept
+    o.history({0, 0.25, 0.30}, {10, 8, 9}, {4}, {-2}, 2, 1.2);
+    a = o.bracket(2, 100, st);
+    CHECK(near(a, 0.25) && st == 1);
     // misfit went up: quadratic backtrack, clamped to [0.1, 0.5] x1
     o.history({0, 0.25}, {10, 12}, {4}, {-2}, 10, 1.2);
     a = o.bracket(1, 100, st);
@@ -165,6 +175,55 @@ static void test_steps() {
     CHECK(near(a, 0.25) && st == 0);
 }
 
+// Plugins written against the reference's interfaces (source/source.h:3-43,
+// optimizer/optimizer.h:359-360): the same virtual signatures override here.
+class BoxSource : public Source {
+  public:
+    void create(float *stf, float f0_, float t0_) override {
+        for (size_t it = 0; it < nt; it++) {
+            const float t = it * dt - t0_;
+            stf[it] = (t >= 0 && t < 1 / f0_) ? 1.0f : 0.0f;
+        }
+    }
+};
+
+class FixedStepOptimizer : public Optimizer {
+  public:
+    int directions = 0;
+    int computeDirection() override {
+        directions++;
+        return 0;
+    }
+    float calcStep(size_t step_count, float step_max, int &status) override {
+        status = step_count > 0 ? 1 : 0;
+        return 0.5f * step_max;
+    }
+};
+
+static void test_reference_style_plugins() {
+    BoxSource src;
+    float cfg[4] = {8.0f, 0.0f, 0.5f, 3.0f};  // f0, t0, angle, amp
+    src.init(cfg, 40, 0.01f);
+    CHECK(src.nt == 40 && src.stf_x && src.stf_y && src.stf_z);
+    CHECK(src.stf_y[0] == 3.0f && src.stf_y[12] == 3.0f && src.stf_y[13] == 0.0f);
+    CHECK(src.stf_x[5] == 3.0f * 1.0f * cosf(0.5f) && src.stf_z[5] == 3.0f * 1.0f * sinf(0.5f));
+    FixedStepOptimizer opt;  // instantiable: every pure virtual is overridden
+    int st = -9;
+    CHECK(opt.calcStep(0, 2.0f, st) == 1.0f && st == 0);
+    CHECK(opt.calcStep(3, 2.0f, st) == 1.0f && st == 1);
+    // the library's own Ricker source through the same interface equals the
+    // C ABI's wavelet bit for bit
+    RickerSource rk;
+    float rcfg[4] = {12.0f, 0.08f, 0.7f, 1e9f};
+    rk.init(rcfg, 300, 1e-3f);
+    vector<float> x(300), y(300), z(300);
+    inv_source_ricker(300, 1e-3f, 12.0f, 0.08f, 0.7f, 1e9f, x.data(), y.data(), z.data());
+    bool same = true;
+    for (int i = 0; i < 300; i++)
+        same = same && rk.stf_x[i] == x[i] && rk.stf_y[i] == y[i] && rk.stf_z[i] == z[i];
+    CHECK(same);
+}
+
 static void test_rotation() {
     // mem = 3, buffers A=0 B=1 C=2: the reference's rotation yields
     // [B,A,B], [A,B,A], [B,A,B] ... (slots 0 and 2 alias, C is never used)
@@ -191,6 +250,7 @@ int main(int argc, char **argv) {
     test_polyfit();
     test_steps();
     test_rotation();
+    test_reference_style_plugins();
     if (failures) {
         std::cout << failures << " FAILED\n";
         return 1;
