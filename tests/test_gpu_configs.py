@@ -94,25 +94,49 @@ def test_c1_batched_fast_matches_oracle_and_plain(gpu):
 
 
 def test_c1_gradient_matches_oracle(gpu):
+    """The gradient (forward, misfit, adjoint, sigma-5 filter) of two c1
+    shots.  Exact math, one shot per launch: the kernels equal the FP32
+    oracle's bit for bit (the misfit sums in FP64, so to 1e-6).  Fast math,
+    both shots in one launch, unfiltered: as close to the FP64 oracle as the
+    FP32 oracle is (the residuals driving the adjoint are small differences
+    of large traces, so any FP32 evaluation carries 1e-4-level kernel error
+    at this size), over the solid cells (the sources sit in the water layer,
+    where the mu kernel is a near-cancelling sum next to them)."""
     P = PR.shots(PR.workload("c1", sigma=5)[0], [2, 13])
-    _say(f"c1 gradient: {P.nx}x{P.nz} nt={P.nt}, 2 shots, fast math, sigma {P.sigma}")
+    _say(f"c1 gradient: {P.nx}x{P.nz} nt={P.nt}, 2 shots, exact math, sigma {P.sigma}")
     obs = _obs(P, 2)
-    ctx = P.context(batch=2)
-    ctx.autotune(0, 2, steps=10)
-    for s in range(2):
-        ctx.obs_set(s, obs[s])
+    ctx = P.context(batch=1)
+    ctx.set_math(H.INV_MATH_EXACT)
+    ctx.autotune(0, 1, steps=10)
+    for s_ in range(2):
+        ctx.obs_set(s_, obs[s_])
     f = ctx.gradient(P.misfit, P.sigma)
     ks = ctx.kernels_get()
     ctx.close()
-    fo, *ko = oracle_problem(P).gradient(obs, P.misfit, P.sigma)
-    _say(f"  misfit {f:.6e} oracle {fo:.6e}")
+    O = oracle_problem(P)
+    fo, *ko = O.gradient(obs, P.misfit, P.sigma)
+    _say(f"  misfit {f:.9e} oracle {fo:.9e}")
+    assert abs(f - fo) <= 1e-6 * abs(fo)
     for name, a, b in zip(("k_lambda", "k_mu", "k_rho"), ks, ko):
-        e = rel(a, b)
-        _say(f"  {name}: {e:.2e}")
-        assert e < 1e-4
-    # the misfit is a norm of small residuals: FP32 trace differences of 1e-6
-    # relative become 1e-5 of it
-    assert abs(f - fo) <= 1e-4 * abs(fo)
+        assert np.array_equal(a, b), f"{name}: rel L2 {rel(a, b):.2e}"
+
+    P0 = P.with_(sigma=0)
+    _say("  fast math, two shots per launch, unfiltered, against the FP64 oracle")
+    ctx = P0.context(batch=2)
+    ctx.autotune(0, 2, steps=10)
+    for s_ in range(2):
+        ctx.obs_set(s_, obs[s_])
+    ctx.gradient(P0.misfit, 0)
+    kg = ctx.kernels_get()
+    ctx.close()
+    O0 = oracle_problem(P0)
+    _, *k32 = O0.gradient(obs, P0.misfit, 0)
+    _, *k64 = O0.gradient(obs.astype(np.float64), P0.misfit, 0, dtype=np.float64)
+    solid = P.mu0 > 0
+    for name, a, b, c in zip(("k_lambda", "k_mu", "k_rho"), kg, k32, k64):
+        eg, eo = rel(a[solid], c[solid]), rel(b[solid], c[solid])
+        _say(f"  {name} vs FP64 (solid cells): GPU {eg:.2e}, FP32 oracle {eo:.2e}")
+        assert eg <= 2 * eo + 1e-6
 
 
 def test_c2_exact_matches_oracle(gpu):
