@@ -172,6 +172,19 @@ int inv_misfit(inv_ctx *ctx, int first, int count, int type, double *chi, double
 int inv_misfit_trace(inv_ctx *ctx, int type, const float *syn, const float *obs, float *adstf,
                      double *chi);
 
+/* The waveform misfit of ntr traces [ntr][nt] on device pointers (nt of the
+ * grid), with no device allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Writes adstf [ntr][nt] (the
+ * time-reversed adjoint sources of misfit/waveform.h:4-50, as
+ * inv_misfit_trace) and, if chi != NULL, the per-trace misfits chi[ntr]
+ * (host); *total (host) = their FP64 sum.  ntr <= 65536: the per-trace
+ * misfits live in a 512 KiB device array of the library (one per GPU, held
+ * until the process ends), shared by every call on that GPU, so calls that
+ * overlap on one GPU race.  Needs inv_grid_set (the taper). */
+int inv_misfit_traces_noalloc(inv_ctx *ctx, const float *syn, const float *obs, float *adstf,
+                              int ntr, double *chi, double *total);
+
 /* ---- whole gradient: Misfit::run(kernel) (misfit/misfit.h:15-51) ---------
  * Forward every shot in batches of `batch`, misfit against resident obs,
  * adjoint (if kernel), then the Gaussian filter (sigma > 0) on the three

@@ -330,6 +330,13 @@ int inv_ctx_create(int device, inv_ctx **out) {
         delete c;
         return INV_ERR_HIP;
     }
+    try {
+        noalloc_init(device);  // the no-allocation misfit's scratch on this device
+    } catch (const inv::Error &e) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return e.code;
+    }
     *out = c;
     return INV_OK;
 }

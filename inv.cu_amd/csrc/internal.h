@@ -264,9 +264,18 @@ void misfit_waveform(Ctx &c, int count, int first);
 void misfit_waveform_slots(Ctx &c, int count, const int *slot_nt, const float *taper_tab);
 // waveform taper of misfit/waveform.h:9-27 for nt samples of step dt
 void waveform_taper(int nt, float dt, float *tw);
+// k_waveform over ntr traces [ntr][nt] on device pointers: adjoint sources
+// into adstf, the misfit of trace r into chi[r] (device)
+void launch_waveform_traces(const float *syn, const float *obs, float *adstf, const float *tw,
+                            double *chi, int nt, int ntr, float dt, hipStream_t stream);
 void misfit_envelope(Ctx &c, int count, int first);
 void envelope_trace(Ctx &c, const float *syn, const float *obs, float *adstf, double *chi_dev);
 double misfit_trace(Ctx &c, int type, const float *syn, const float *obs, float *adstf);
+
+// noalloc.hip ---------------------------------------------------------------
+// resolves, once per device (with that device current), the device address of
+// the no-allocation entry point's scratch array
+void noalloc_init(int device);
 
 // linalg.hip ----------------------------------------------------------------
 void gauss_filter(Ctx &c, float *field, int nx, int nz, int sigma);

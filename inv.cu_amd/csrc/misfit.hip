@@ -91,6 +91,15 @@ void misfit_waveform(Ctx &c, int count, int first) {
     INV_HIP(hipGetLastError());
 }
 
+void launch_waveform_traces(const float *syn, const float *obs, float *adstf, const float *tw,
+                            double *chi, int nt, int ntr, float dt, hipStream_t stream) {
+    // ntr traces as one component of ntr receivers of one slot: trace r is
+    // row r of [ntr][nt], its misfit chi[r]
+    k_waveform<<<dim3(ntr, 1), MISFIT_BLOCK, 0, stream>>>(syn, obs, adstf, tw, chi, nt, ntr, 1,
+                                                         sqrtf(dt));
+    INV_HIP(hipGetLastError());
+}
+
 double misfit_trace(Ctx &c, int type, const float *syn, const float *obs, float *adstf) {
     c.scal.ensure(8 * sizeof(double));
     if (type == 0) {

@@ -111,6 +111,7 @@ SIGNATURES = {
     "inv_adjoint_begin": [_vp, _i, _i],
     "inv_run_advance": [_vp, _i, _vp],
     "inv_misfit_trace": [_vp, _i, _vp, _vp, _vp, _vp],
+    "inv_misfit_traces_noalloc": [_vp, _vp, _vp, _vp, _i, _vp, _vp],
     "inv_set_math": [_vp, _i],
     "inv_set_graphs": [_vp, _i],
     "inv_server_create": [_vp, _vp],
@@ -298,6 +299,16 @@ class Context:
         self._chk(self.L.inv_misfit_trace(self.h, kind, syn_dev, obs_dev, adstf_dev,
                                           C.byref(chi)))
         return chi.value
+
+    def misfit_traces_noalloc(self, syn_dev, obs_dev, adstf_dev, ntr):
+        """Waveform misfit of ntr device traces with no device allocation;
+        returns (per-trace misfits, total)."""
+        chi = np.zeros(ntr, np.float64)
+        tot = C.c_double()
+        self._chk(self.L.inv_misfit_traces_noalloc(self.h, C.c_void_p(syn_dev),
+                                                   C.c_void_p(obs_dev), C.c_void_p(adstf_dev),
+                                                   ntr, _p(chi), C.byref(tot)))
+        return chi, tot.value
 
     def kernels_zero(self):
         self._chk(self.L.inv_kernels_zero(self.h))
