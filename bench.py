@@ -91,7 +91,7 @@ def parse(argv=None):
     ap.add_argument("--ckpt", type=int, default=-1,
                     help="checkpoint interval in steps for 16-bit snapshots (-1: the workload's, "
                          "0: none, every snapshot of the run resident)")
-    ap.add_argument("--snap", default="", choices=["", "fp32", "fp16", "bf16"],
+    ap.add_argument("--snap", default="", choices=["", "fp32", "fp16", "bf16", "i16"],
                     help="snapshot storage (default: the workload's)")
     ap.add_argument("--nt", type=int, default=0, help="time steps (0: the workload's)")
     ap.add_argument("--sigma", type=int, default=5, help="Gaussian filter sigma (cells)")
@@ -140,7 +140,7 @@ def serving_leg(inv, n, rate_x, snap_format):
             "mean_requests_per_batch", "peak_bytes")
     return {"requests": n, "rate_per_s": round(rate, 4), "rate_x_plain_capacity": rate_x,
             "plain_capacity_req_per_s": round(base["requests_per_s"], 4),
-            "snapshots": ["fp32", "fp16", "bf16"][snap_format],
+            "snapshots": ["fp32", "fp16", "bf16", "i16"][snap_format],
             "plain": {k: plain[k] for k in keys}, "batched": {k: batched[k] for k in keys},
             "p50_speedup": round(plain["p50_latency_s"] / batched["p50_latency_s"], 3),
             "p99_speedup": round(plain["p99_latency_s"] / batched["p99_latency_s"], 3)}
@@ -256,7 +256,7 @@ def accuracy(inv, name):
            "measure": "relative L2 over the solid cells (mu > 0)", "misfit_fp64": f64}
     solid = P.mu0 > 0
     for label, fmt in (("fp32", H.INV_SNAP_F32), ("fp16", H.INV_SNAP_F16),
-                       ("bf16", H.INV_SNAP_BF16)):
+                       ("bf16", H.INV_SNAP_BF16), ("i16", H.INV_SNAP_I16)):
         ctx = P.context(batch=2, model="init")
         ctx.set_snapshots(fmt, 0)
         for s_ in range(2):
@@ -285,14 +285,15 @@ def launch_plan(a, name, P, B, fmt, ckpt, wave):
     if a.ckpt >= 0:
         ckpt = a.ckpt
     if a.snap:
-        fmt = {"fp32": H.INV_SNAP_F32, "fp16": H.INV_SNAP_F16, "bf16": H.INV_SNAP_BF16}[a.snap]
+        fmt = {"fp32": H.INV_SNAP_F32, "fp16": H.INV_SNAP_F16, "bf16": H.INV_SNAP_BF16,
+               "i16": H.INV_SNAP_I16}[a.snap]
     if ckpt and P.nt % ckpt:
         ckpt = P.nt  # a shortened run (--nt) the interval does not divide: one segment
     return B, wave, fmt, ckpt
 
 
 def describe(name, P, B, wave, fmt, ckpt):
-    snap = ["FP32", "FP16", "BF16"][fmt]
+    snap = ["FP32", "FP16", "BF16", "I16"][fmt]
     ck = f"{ckpt}-step checkpoints" if ckpt else "every snapshot resident"
     waves = "SH" if P.sh and not P.psv else "P-SV"
     return (f"{name}: batched {waves} gradient, {P.name} {P.nx}x{P.nz} @ {float(P.dx):g} m, "
@@ -451,7 +452,7 @@ def run_workload(inv, a, name, steps, warmup, headline):
                "shots_per_step": P.nsrc, "shots": full_shots, "receivers": P.nrec, "nt": P.nt,
                "batch": B, "sfe": P.sfe, "filter_sigma": P.sigma,
                "misfit": ["waveform", "envelope"][P.misfit],
-               "snapshots": ["fp32", "fp16", "bf16"][fmt] + " in HBM",
+               "snapshots": ["fp32", "fp16", "bf16", "i16"][fmt] + " in HBM",
                "checkpoint_interval": ckpt, "step_kernel_tile": tile},
            "roofline": roof}
     if waves_per_gradient > 1:
@@ -502,7 +503,7 @@ def main():
     inv = graft.load_package()
     if a.serving_sweep:
         for s in inv.serving.sweep(a.serving_sweep, log=log,
-                                   snap_format=inv.invhip.INV_SNAP_F16):
+                                   snap_format=inv.invhip.INV_SNAP_I16):
             print(json.dumps(s), flush=True)
         return
     head, P, gpu_grad = run_workload(inv, a, a.workload, a.steps, a.warmup, headline=True)
@@ -550,7 +551,7 @@ def main():
                if w == a.workload or (w in extras and a.workload == "c4")}
     serving = None
     if a.serve_requests > 0 and a.workload in ("c1", "c4"):
-        serving = serving_leg(inv, a.serve_requests, a.serve_rate_x, inv.invhip.INV_SNAP_F16)
+        serving = serving_leg(inv, a.serve_requests, a.serve_rate_x, inv.invhip.INV_SNAP_I16)
 
     out = {
         "metric": METRIC,

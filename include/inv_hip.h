@@ -52,7 +52,12 @@ enum inv_field {
 };
 
 /* Snapshot storage formats (arithmetic is always FP32). */
-enum inv_snap_format { INV_SNAP_F32 = 0, INV_SNAP_F16 = 1, INV_SNAP_BF16 = 2 };
+enum inv_snap_format {
+    INV_SNAP_F32 = 0,
+    INV_SNAP_F16 = 1,   /* FP16, power-of-two scale per row segment */
+    INV_SNAP_BF16 = 2,  /* BF16, the same scaling */
+    INV_SNAP_I16 = 3    /* 16-bit fixed point relative to the row segment's max-abs */
+};
 
 /* Forward flags. */
 enum inv_forward_flags {

@@ -114,7 +114,7 @@ static void alloc_slots(Ctx &c) {
     // format and checkpoint interval in force then (inv_set_snapshots)
     if (!c.retain) {
         c.snap.release();
-        c.smax.release();
+        c.sexp.release();
         c.ckbuf.release();
         c.rstate.release();
     }
@@ -181,7 +181,7 @@ static void invalidate(Ctx &c) {
 std::vector<DevBuf *> ctx_buffers(Ctx &c) {
     return {&c.lam, &c.mu, &c.rho, &c.abx, &c.abzb, &c.abzt, &c.k_lambda, &c.k_mu, &c.k_rho,
             &c.kslot, &c.vx, &c.vz, &c.ux, &c.uz, &c.sxx, &c.szz, &c.sxz, &c.vy, &c.uy,
-            &c.sxy, &c.szy, &c.snap, &c.smax, &c.ckbuf, &c.rstate, &c.out, &c.adstf, &c.obs,
+            &c.sxy, &c.szy, &c.snap, &c.sexp, &c.ckbuf, &c.rstate, &c.out, &c.adstf, &c.obs,
             &c.chi, &c.taper, &c.red, &c.scal, &c.stf, &c.rec_list.ptr, &c.rec_list.cell,
             &c.rec_list.row, &c.gauss_w, &c.gauss_tmp, &c.gauss_sx, &c.gauss_sz, &c.vx_b,
             &c.vz_b, &c.sxx_b, &c.szz_b, &c.sxz_b, &c.vy_b, &c.sxy_b, &c.szy_b, &c.f_rec_v.ptr,
@@ -225,7 +225,8 @@ size_t ctx_bytes_estimate(const inv_grid_desc &d, int nsrc, int nrec, int snap_f
         const int nslots = ckpt > 0 ? ckpt / d.sfe : nsfe;
         const size_t es = snap_format == INV_SNAP_F32 ? 4 : 2;
         add(B * (size_t)nslots * nsnapf * N * es);
-        add((size_t)nslots * B * nsnapf * sizeof(unsigned));
+        if (snap_format != INV_SNAP_F32)
+            add((size_t)nslots * snap_tab_bytes(d.batch, nsnapf, d.nx, d.nz));
         if (ckpt > 0) {
             add((size_t)(d.nt / ckpt - 1) * nstate * B * N * f);
             add((size_t)((d.sh ? 7 : 0) + (d.psv ? 12 : 0)) * B * N * f);
@@ -1082,35 +1083,44 @@ int inv_snapshots_get(inv_ctx *ctx, int slot, float *host) {
             return;
         }
         std::vector<uint16_t> q(n);
-        std::vector<uint32_t> mx((size_t)c.nslots * c.g.batch * c.nsnapf);
+        const size_t tb = snap_tab_bytes(c.g.batch, c.nsnapf, c.g.nx, c.g.nz);
+        std::vector<signed char> tab((size_t)c.nslots * tb);
         INV_HIP(hipMemcpy(q.data(), c.snap.as<char>() + slot * n * 2, n * 2, hipMemcpyDeviceToHost));
-        INV_HIP(hipMemcpy(mx.data(), c.smax.p, mx.size() * 4, hipMemcpyDeviceToHost));
-        for (int l = 0; l < c.nslots; l++)
+        INV_HIP(hipMemcpy(tab.data(), c.sexp.p, tab.size(), hipMemcpyDeviceToHost));
+        const int nseg = snap_nseg(c.g.nz);
+        for (int l = 0; l < c.nslots; l++) {
+            const signed char *t = tab.data() + (size_t)l * tb;
+            int segw;
+            std::memcpy(&segw, t, 4);
+            segw = segw > 0 ? segw : 1;
             for (int f = 0; f < c.nsnapf; f++) {
-                float m;
-                std::memcpy(&m, &mx[((size_t)l * c.g.batch + slot) * c.nsnapf + f], 4);
-                const float s = m * (1.0f / 32768.0f);
                 const size_t o = ((size_t)l * c.nsnapf + f) * c.N;
+                const signed char *rows = t + SNAPTAB_HDR + ((size_t)slot * c.nsnapf + f) * c.g.nx * nseg;
                 for (long k = 0; k < c.N; k++) {
+                    const int i = (int)(k / c.g.nz), j = (int)(k % c.g.nz);
+                    const int e = rows[(size_t)i * nseg + std::min(j / segw, nseg - 1)];
                     const uint16_t v = q[o + k];
                     float d;
                     if (c.snap_format == INV_SNAP_F16) {
                         d = half_to_float(v);
+                    } else if (c.snap_format == INV_SNAP_I16) {
+                        d = (float)(int16_t)v;
                     } else {
                         const uint32_t bits = (uint32_t)v << 16;
                         std::memcpy(&d, &bits, 4);
                     }
-                    host[o + k] = d * s;
+                    host[o + k] = d * pow2f(-e);
                 }
             }
+        }
     });
 }
 
 int inv_set_snapshots(inv_ctx *ctx, int format, int ckpt_interval) {
     return guard(ctx, [&](Ctx &c) {
         INV_REQUIRE(c.have_grid, INV_ERR_STATE, "grid not set");
-        INV_REQUIRE(format == INV_SNAP_F32 || format == INV_SNAP_F16 || format == INV_SNAP_BF16,
-                    INV_ERR_ARG, "unknown snapshot format");
+        INV_REQUIRE(format >= INV_SNAP_F32 && format <= INV_SNAP_I16, INV_ERR_ARG,
+                    "unknown snapshot format");
         INV_REQUIRE(ckpt_interval >= 0, INV_ERR_ARG, "checkpoint interval must be >= 0");
         if (ckpt_interval > 0)
             INV_REQUIRE(c.g.adj && c.g.sfe > 0 && ckpt_interval % c.g.sfe == 0 &&
@@ -1119,7 +1129,7 @@ int inv_set_snapshots(inv_ctx *ctx, int format, int ckpt_interval) {
                         "checkpoint interval must be a multiple of sfe that divides nt (adj = 1)");
         sync(c);
         c.snap.release();
-        c.smax.release();
+        c.sexp.release();
         c.ckbuf.release();
         c.rstate.release();
         c.snap_format = format;

@@ -255,86 +255,98 @@ __device__ __forceinline__ void stress_row(const Geo &g, const float *lv1, const
     }
 }
 
-// Snapshot scales of one step's roles (16-bit formats): su1, su2 for u and
-// sv1, sv2 for v (1 in FP32 or without the role).
-struct SnapScales {
-    float su1 = 1.0f, su2 = 1.0f, sv1 = 1.0f, sv2 = 1.0f;
-};
-template <bool PSV>
-__device__ __forceinline__ SnapScales snap_scales(const StepRoles &ro, int fmt, int mb, int fu,
-                                                  int fv) {
-    SnapScales s;
-    if (fmt != INV_SNAP_F32) {
-        if (ro.snap_u) {
-            s.su1 = snap_enc_scale(ro.mx_u[mb + fu]);
-            if (PSV) s.su2 = snap_enc_scale(ro.mx_u[mb + fu + 1]);
-        }
-        if (ro.snap_v) {
-            s.sv1 = snap_enc_scale(ro.mx_v[mb + fv]);
-            if (PSV) s.sv2 = snap_enc_scale(ro.mx_v[mb + fv + 1]);
+// Segment exponents of one row for the 16-bit formats: the max-abs of the
+// fields over the lanes that own cells of the row (wave reductions; every
+// lane of the wavefront calls it), their exponents (snap_exp), stored by lane
+// 0 into the slot's table at segment seg, with the segment width segw in the
+// table header.  e[0], e[1]: u (two fields for P-SV); e[2], e[3]: v.
+template <bool PSV, int V>
+__device__ __forceinline__ void row_exponents(const StepArgs &a, const StepRoles &ro, int fu,
+                                              int fv, int i, int b, int seg, int segw, int lane,
+                                              bool act, const float (&u1)[V], const float (&u2)[V],
+                                              const float (&n1)[V], const float (&n2)[V],
+                                              int (&e)[4]) {
+    float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (act) {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            m[0] = fmaxf(m[0], fabsf(u1[v]));
+            m[2] = fmaxf(m[2], fabsf(n1[v]));
+            if (PSV) {
+                m[1] = fmaxf(m[1], fabsf(u2[v]));
+                m[3] = fmaxf(m[3], fabsf(n2[v]));
+            }
         }
     }
-    return s;
+#pragma unroll
+    for (int q = 0; q < 4; q++) e[q] = (PSV || q % 2 == 0) ? snap_exp(wave_max(m[q])) : 0;
+    if (lane == 0) {
+        signed char *t = ro.sexp;
+        const int nx = a.g.nx;
+        t[snaptab_row(b, a.nsnapf, fu, nx, i, a.nseg) + seg] = (signed char)e[0];
+        t[snaptab_row(b, a.nsnapf, fv, nx, i, a.nseg) + seg] = (signed char)e[2];
+        if (PSV) {
+            t[snaptab_row(b, a.nsnapf, fu + 1, nx, i, a.nseg) + seg] = (signed char)e[1];
+            t[snaptab_row(b, a.nsnapf, fv + 1, nx, i, a.nseg) + seg] = (signed char)e[3];
+        }
+        *reinterpret_cast<int *>(t) = segw;
+    }
+}
+
+// one snapshot value: FP32 as is, 16-bit formats scaled by 2^e
+__device__ __forceinline__ void snap_put(void *base, int fmt, long idx, float x, int e) {
+    if (fmt == INV_SNAP_F32) static_cast<float *>(base)[idx] = x;
+    else static_cast<unsigned short *>(base)[idx] = snap_enc16(x, e, fmt);
 }
 
 // The owned cells of grid row i after the velocity update of step it, in
 // the order of k_colv: u(it) into its snapshot, u(it+1) = u(it) + v(it+1) dt,
-// v into its snapshot (v(it+1) with v_after, else v(it)), max-abs tracking
-// of u(it+1) and v(it+1), receivers of step it (list entries held per lane
-// in rp).  u1/u2 hold u(it) on entry and
-// u(it+1) on return; w1/w2: v(it); n1/n2: v(it+1).
+// v(it+1) into its snapshot, receivers of step it (list entries held per lane
+// in rp).  u1/u2 hold u(it) on entry and u(it+1) on return; n1/n2: v(it+1).
+// 16-bit snapshots are scaled per segment: this wavefront's owned cells of
+// the row, segment `seg` of width segw of the slot's exponent table.
 // Every lane of the wavefront calls it (wave-uniform row); act: the lane
 // owns its cells of the row.
 template <bool PSV, int V>
-__device__ __forceinline__ void owned_row(const StepArgs &a, const StepRoles &ro,
-                                          const SnapScales &ss, int fu, int fv, long sb, int i,
-                                          int jb, bool act, long it, int b, const LanePts &rp,
+__device__ __forceinline__ void owned_row(const StepArgs &a, const StepRoles &ro, int fu, int fv,
+                                          long sb, int i, int jb, bool act, long it, int b,
+                                          int seg, int segw, int lane, const LanePts &rp,
                                           int rlo, int rhi, float (&u1)[V], float (&u2)[V],
-                                          const float (&w1)[V], const float (&w2)[V],
-                                          const float (&n1)[V], const float (&n2)[V],
-                                          float &mu1, float &mu2, float &mv1, float &mv2) {
+                                          const float (&n1)[V], const float (&n2)[V]) {
     const Geo &g = a.g;
     const int fmt = a.snap_fmt;
-    const bool track = ro.acc_u || ro.acc_v;
     float *O1 = PSV ? a.out_x : a.out_y, *O2 = a.out_z;
+    int e[4] = {0, 0, 0, 0};
+    if (ro.snap && fmt != INV_SNAP_F32)
+        row_exponents<PSV, V>(a, ro, fu, fv, i, b, seg, segw, lane, act, u1, u2, n1, n2, e);
     if (act) {
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const long kc = (long)i * g.nz + jb + v;
             float u = u1[v];
             float uu = PSV ? u2[v] : 0.0f;
-            if (ro.snap_u) {
-                snap_put(ro.snap_u, fmt, sb + (long)fu * g.N + kc, u, ss.su1);
-                if (PSV) snap_put(ro.snap_u, fmt, sb + (long)(fu + 1) * g.N + kc, uu, ss.su2);
+            if (ro.snap) {
+                snap_put(ro.snap, fmt, sb + (long)fu * g.N + kc, u, e[0]);
+                if (PSV) snap_put(ro.snap, fmt, sb + (long)(fu + 1) * g.N + kc, uu, e[1]);
             }
             u += n1[v] * g.dt;
             if (PSV) uu += n2[v] * g.dt;
             u1[v] = u;
             if (PSV) u2[v] = uu;
-            if (ro.snap_v) {
-                snap_put(ro.snap_v, fmt, sb + (long)fv * g.N + kc, ro.v_after ? n1[v] : w1[v], ss.sv1);
-                if (PSV)
-                    snap_put(ro.snap_v, fmt, sb + (long)(fv + 1) * g.N + kc,
-                             ro.v_after ? n2[v] : w2[v], ss.sv2);
-            }
-            if (track) {
-                mu1 = fmaxf(mu1, fabsf(u));
-                mv1 = fmaxf(mv1, fabsf(n1[v]));
-                if (PSV) {
-                    mu2 = fmaxf(mu2, fabsf(uu));
-                    mv2 = fmaxf(mv2, fabsf(n2[v]));
-                }
+            if (ro.snap) {
+                snap_put(ro.snap, fmt, sb + (long)fv * g.N + kc, n1[v], e[2]);
+                if (PSV) snap_put(ro.snap, fmt, sb + (long)(fv + 1) * g.N + kc, n2[v], e[3]);
             }
         }
     }
     scan_pts(rp, a.rec_cell, a.rec_row, rlo, rhi, [&](int cp, int row) {
-        const int e = cp - (i * g.nz + jb);
-        if (!act || e < 0 || e >= V) return;
+        const int e_ = cp - (i * g.nz + jb);
+        if (!act || e_ < 0 || e_ >= V) return;
         float x = 0.0f, y = 0.0f;
 #pragma unroll
         for (int v = 0; v < V; v++) {
-            x = e == v ? (a.rec_disp ? u1[v] : n1[v]) : x;
-            if (PSV) y = e == v ? (a.rec_disp ? u2[v] : n2[v]) : y;
+            x = e_ == v ? (a.rec_disp ? u1[v] : n1[v]) : x;
+            if (PSV) y = e_ == v ? (a.rec_disp ? u2[v] : n2[v]) : y;
         }
         const long kr = (long)b * a.out_slot_stride + (long)row * a.nt + it;
         O1[kr] = x;

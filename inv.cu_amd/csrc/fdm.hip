@@ -49,8 +49,10 @@ struct IntArgs {
     const float *vx, *vz, *ux, *uz, *vy, *uy;
     const void *snap;  // slot-0 base of the snapshot slot being read
     long snap_slot_stride;  // elements per batch slot
-    const unsigned *smax;   // 16-bit formats: max-abs [batch][nsnapf] of the slot
-    int nsnapf;
+    // 16-bit formats: the slot's exponent table (header: segment width, then
+    // [batch][nsnapf][nx][nseg] exponents)
+    const signed char *sexp;
+    int nsnapf, nseg;
     float *kl, *km, *kr;
     long k_slot_stride;  // 0: every slot accumulates into the same kernels
     float ndt;
@@ -206,13 +208,21 @@ struct Snap32 {
     __device__ __forceinline__ float operator[](long k) const { return p[k]; }
 };
 
+// the segment width of the slot's writer (at least 1; segments are clamped
+// to the table in any case)
+__device__ __forceinline__ int snap_segw(const IntArgs &a) {
+    const int w = *reinterpret_cast<const int *>(a.sexp);
+    return w > 0 ? w : 1;
+}
+
 template <int FMT>
 struct SnapField {
     using type = Snap16<FMT>;
     __device__ static type at(const IntArgs &a, int b, int f) {
         const unsigned short *q =
             static_cast<const unsigned short *>(a.snap) + (long)b * a.snap_slot_stride + f * a.g.N;
-        return type{q, __uint_as_float(a.smax[b * a.nsnapf + f]) * (1.0f / SNAP_RANGE)};
+        return type{q, a.sexp + snaptab_row(b, a.nsnapf, f, a.g.nx, 0, a.nseg), a.g.nz, a.nseg,
+                    snap_segw(a)};
     }
 };
 template <>
@@ -288,9 +298,10 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_interact(const IntArgs a) {
 // on the shared accumulators), so the kernels agree bit for bit.
 constexpr int ICOLS = 124;  // columns a wavefront owns
 
+// the two snapshot values of field f at cells k, k+1 (row i, segment seg)
 template <int FMT>
-__device__ __forceinline__ void snap_pair(const IntArgs &a, int b, int f, long k, float &x0,
-                                          float &x1) {
+__device__ __forceinline__ void snap_pair(const IntArgs &a, int b, int f, long k, int i, int seg,
+                                          float &x0, float &x1) {
     if constexpr (FMT == INV_SNAP_F32) {
         const float *q = static_cast<const float *>(a.snap) + (long)b * a.snap_slot_stride +
                          (long)f * a.g.N;
@@ -300,10 +311,10 @@ __device__ __forceinline__ void snap_pair(const IntArgs &a, int b, int f, long k
     } else {
         const unsigned short *q = static_cast<const unsigned short *>(a.snap) +
                                   (long)b * a.snap_slot_stride + (long)f * a.g.N;
-        const float sc = __uint_as_float(a.smax[b * a.nsnapf + f]) * (1.0f / SNAP_RANGE);
+        const int e = a.sexp[snaptab_row(b, a.nsnapf, f, a.g.nx, i, a.nseg) + seg];
         const unsigned w = *reinterpret_cast<const unsigned *>(q + k);
-        x0 = snap_dec16((unsigned short)(w & 0xffffu), FMT) * sc;
-        x1 = snap_dec16((unsigned short)(w >> 16), FMT) * sc;
+        x0 = snap_dec16((unsigned short)(w & 0xffffu), e, FMT);
+        x1 = snap_dec16((unsigned short)(w >> 16), e, FMT);
     }
 }
 
@@ -334,7 +345,14 @@ __global__ __launch_bounds__(256) void k_interact_col(const IntArgs a, int tiles
     const int jl = jb < 0 ? 0 : (jb > nz - 2 ? nz - 2 : jb);
     const bool own = lane >= 1 && lane <= 62 && jb < nz;
     const long so = (long)b * g.N;
-    auto row = [&](int i) { return (long)(i < 0 ? 0 : (i > nx - 1 ? nx - 1 : i)) * nz + jl; };
+    auto rowc = [&](int i) { return i < 0 ? 0 : (i > nx - 1 ? nx - 1 : i); };
+    auto row = [&](int i) { return (long)rowc(i) * nz + jl; };
+    // the lane's snapshot segment (columns jl, jl+1 share it: widths are even)
+    int seg = 0;
+    if constexpr (FMT != INV_SNAP_F32) {
+        seg = jl / snap_segw(a);
+        seg = seg < a.nseg - 1 ? seg : a.nseg - 1;
+    }
     float *kl = a.kl + (long)b * a.k_slot_stride;
     float *km = a.km + (long)b * a.k_slot_stride;
     float *kr = a.kr + (long)b * a.k_slot_stride;
@@ -345,11 +363,12 @@ __global__ __launch_bounds__(256) void k_interact_col(const IntArgs a, int tiles
 #pragma unroll
     for (int r = 0; r < WR; r++) {
         const long k = row(i0 - 1 + r);
+        const int ic = rowc(i0 - 1 + r);
         if (SH) {
             const float2 t = *reinterpret_cast<const float2 *>(a.uy + so + k);
             ay[r][0] = t.x;
             ay[r][1] = t.y;
-            snap_pair<FMT>(a, b, 0, k, fy[r][0], fy[r][1]);
+            snap_pair<FMT>(a, b, 0, k, ic, seg, fy[r][0], fy[r][1]);
         }
         if (PSV) {
             const float2 t = *reinterpret_cast<const float2 *>(a.ux + so + k);
@@ -358,8 +377,8 @@ __global__ __launch_bounds__(256) void k_interact_col(const IntArgs a, int tiles
             ax[r][1] = t.y;
             az[r][0] = u.x;
             az[r][1] = u.y;
-            snap_pair<FMT>(a, b, f0, k, fx[r][0], fx[r][1]);
-            snap_pair<FMT>(a, b, f0 + 1, k, fz[r][0], fz[r][1]);
+            snap_pair<FMT>(a, b, f0, k, ic, seg, fx[r][0], fx[r][1]);
+            snap_pair<FMT>(a, b, f0 + 1, k, ic, seg, fz[r][0], fz[r][1]);
         }
     }
 #pragma unroll
@@ -393,7 +412,7 @@ __global__ __launch_bounds__(256) void k_interact_col(const IntArgs a, int tiles
             dz_pair<FAST>(fy[t + 1][0], fy[t + 1][1], g, dzf[0], dzf[1]);
             if (own) {
                 float vf[2];
-                snap_pair<FMT>(a, b, 1, k, vf[0], vf[1]);
+                snap_pair<FMT>(a, b, 1, k, i, seg, vf[0], vf[1]);
                 const float2 vv = *reinterpret_cast<const float2 *>(a.vy + so + k);
                 const float vy[2] = {vv.x, vv.y};
 #pragma unroll
@@ -420,8 +439,8 @@ __global__ __launch_bounds__(256) void k_interact_col(const IntArgs a, int tiles
             dz_pair<FAST>(fz[t + 1][0], fz[t + 1][1], g, dzzf[0], dzzf[1]);
             if (own) {
                 float vfx[2], vfz[2];
-                snap_pair<FMT>(a, b, f0 + 2, k, vfx[0], vfx[1]);
-                snap_pair<FMT>(a, b, f0 + 3, k, vfz[0], vfz[1]);
+                snap_pair<FMT>(a, b, f0 + 2, k, i, seg, vfx[0], vfx[1]);
+                snap_pair<FMT>(a, b, f0 + 3, k, i, seg, vfz[0], vfz[1]);
                 const float2 vxx = *reinterpret_cast<const float2 *>(a.vx + so + k);
                 const float2 vzz = *reinterpret_cast<const float2 *>(a.vz + so + k);
                 const float vx[2] = {vxx.x, vxx.y}, vz[2] = {vzz.x, vzz.y};
@@ -860,6 +879,7 @@ static void launch_interact(Ctx &c, const IntArgs &ia, int count) {
         switch (c.snap_format) {
             case INV_SNAP_F16: launch_interact_cf<SH, PSV, INV_SNAP_F16>(c, ia, count); break;
             case INV_SNAP_BF16: launch_interact_cf<SH, PSV, INV_SNAP_BF16>(c, ia, count); break;
+            case INV_SNAP_I16: launch_interact_cf<SH, PSV, INV_SNAP_I16>(c, ia, count); break;
             default: launch_interact_cf<SH, PSV, INV_SNAP_F32>(c, ia, count); break;
         }
         return;
@@ -867,6 +887,7 @@ static void launch_interact(Ctx &c, const IntArgs &ia, int count) {
     switch (c.snap_format) {
         case INV_SNAP_F16: launch_interact_f<SH, PSV, INV_SNAP_F16>(c, ia, count); break;
         case INV_SNAP_BF16: launch_interact_f<SH, PSV, INV_SNAP_BF16>(c, ia, count); break;
+        case INV_SNAP_I16: launch_interact_f<SH, PSV, INV_SNAP_I16>(c, ia, count); break;
         default: launch_interact_f<SH, PSV, INV_SNAP_F32>(c, ia, count); break;
     }
 }
@@ -881,13 +902,17 @@ void fdm_interact(Ctx &c, int count, int it, int slot, const float *vx, const fl
     ia.ux = c.ux.as<float>();
     ia.uz = c.uz.as<float>();
     ia.uy = c.uy.as<float>();
-    INV_REQUIRE(slot >= 0 && slot < c.nslots && c.snap.p && c.smax.p, INV_ERR_STATE,
+    const bool s16 = c.snap_format != INV_SNAP_F32;
+    INV_REQUIRE(slot >= 0 && slot < c.nslots && c.snap.p && (!s16 || c.sexp.p), INV_ERR_STATE,
                 "snapshot slot outside the snapshot buffer");
-    const long esize = c.snap_format == INV_SNAP_F32 ? 4 : 2;
+    const long esize = s16 ? 2 : 4;
     ia.snap_slot_stride = (long)c.nslots * c.nsnapf * c.N;
     ia.snap = c.snap.as<char>() + (long)slot * c.nsnapf * c.N * esize;
-    ia.smax = c.smax.as<unsigned>() + (long)slot * c.g.batch * c.nsnapf;
+    ia.sexp = s16 ? c.sexp.as<signed char>() +
+                        (long)slot * snap_tab_bytes(c.g.batch, c.nsnapf, c.g.nx, c.g.nz)
+                  : nullptr;
     ia.nsnapf = c.nsnapf;
+    ia.nseg = snap_nseg(c.g.nz);
     ia.ndt = (float)c.g.sfe * c.g.dt;  // sfe * dt, fdm.h:585
     ia.inv_lambda = c.g.inv_lambda;
     ia.inv_mu = c.g.inv_mu;

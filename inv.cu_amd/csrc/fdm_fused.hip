@@ -158,7 +158,7 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
             l_vy[e] = v;
             if (own) {
                 float u = puy[k];
-                float *sp = static_cast<float *>(a.snap_u);
+                float *sp = static_cast<float *>(a.snap);
                 if (sp) sp[(long)b * a.snap_slot_stride + kc] = u;
                 u += v * g.dt;
                 a.vy1[o] = v;
@@ -196,9 +196,9 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
             if (own) {
                 float ux = pux[k], uz = puz[k];
                 const int f0 = SH ? 2 : 0;
-                float *sp = a.snap_u ? static_cast<float *>(a.snap_u) + (long)b * a.snap_slot_stride +
-                                           (long)f0 * g.N + kc
-                                     : nullptr;
+                float *sp = a.snap ? static_cast<float *>(a.snap) + (long)b * a.snap_slot_stride +
+                                         (long)f0 * g.N + kc
+                                   : nullptr;
                 if (sp) {
                     sp[0] = ux;
                     sp[g.N] = uz;
@@ -290,7 +290,6 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
     __shared__ float l_sxx[PSV ? SR * W : 1], l_szz[PSV ? SR * W : 1], l_sxz[PSV ? SR * W : 1];
     __shared__ float l_sxy[SH ? SR * W : 1], l_szy[SH ? SR * W : 1];
     __shared__ float l_vx[PSV ? VR * W : 1], l_vz[PSV ? VR * W : 1], l_vy[SH ? VR * W : 1];
-    __shared__ float l_track[4 * NWAVE];
 
     const int nw = gridDim.x, w = blockIdx.x;
     const int q = nw >> 3, rmd = nw & 7, xcd = w & 7;
@@ -403,20 +402,12 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
     const int fmt = a.snap_fmt;
     const int fu = PSV ? a.psv_f0 : 0, fv = PSV ? a.psv_f0 + 2 : 1;
     const long sb = (long)b * a.snap_slot_stride;
-    float su1 = 1.0f, su2 = 1.0f, sv1 = 1.0f, sv2 = 1.0f;
-    if (fmt != INV_SNAP_F32) {
-        const int mb = b * a.nsnapf;
-        if (a.snap_u) {
-            su1 = snap_enc_scale(a.mx_u[mb + fu]);
-            if (PSV) su2 = snap_enc_scale(a.mx_u[mb + fu + 1]);
-        }
-        if (a.snap_v) {
-            sv1 = snap_enc_scale(a.mx_v[mb + fv]);
-            if (PSV) sv2 = snap_enc_scale(a.mx_v[mb + fv + 1]);
-        }
-    }
-    const bool track = a.acc_u || a.acc_v;
-    float mu1 = 0.0f, mu2 = 0.0f, mv1 = 0.0f, mv2 = 0.0f;  // max |u|, |v| after the step
+    // 16-bit snapshots: one exponent per field and row over the tile's CZ
+    // owned columns (segment tz of the slot's table)
+    const bool seg16 = a.snap && fmt != INV_SNAP_F32;
+    auto put_exp = [&](int f, int i, int e) {
+        a.sexp[snaptab_row(b, a.nsnapf, f, nx, i, a.nseg) + tz] = (signed char)e;
+    };
 
     // 2. velocity of step n+1, rows i0-1 .. i0+TX+1, columns j0-1 .. j0+CZ+1
     {
@@ -461,16 +452,22 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                 const float v = vel_update<FAST>(ab, qv1[p], g.dt, d, rho);
                 l_vy[e] = cin ? v : 0.0f;
                 float u = qu1[p];
+                int eu = 0, ev = 0;
+                if (seg16 && vr >= 1 && vr <= TX) {  // wave-uniform: every lane reduces
+                    eu = snap_exp(wave_max(own ? fabsf(u) : 0.0f));
+                    ev = snap_exp(wave_max(own ? fabsf(v) : 0.0f));
+                    if (lane == 0) {
+                        put_exp(0, i, eu);
+                        put_exp(1, i, ev);
+                        *reinterpret_cast<int *>(a.sexp) = CZ;
+                    }
+                }
                 if (own) {
-                    if (a.snap_u) snap_put(a.snap_u, fmt, sb + kc, u, su1);
+                    if (a.snap) snap_put(a.snap, fmt, sb + kc, u, eu);
                     u += v * g.dt;
                     a.vy1[o] = v;
                     a.uy[o] = u;
-                    if (a.snap_v) snap_put(a.snap_v, fmt, sb + g.N + kc, a.v_after ? v : qv1[p], sv1);
-                    if (track) {
-                        mu1 = fmaxf(mu1, fabsf(u));
-                        mv1 = fmaxf(mv1, fabsf(v));
-                    }
+                    if (a.snap) snap_put(a.snap, fmt, sb + g.N + kc, v, ev);
                 }
                 scan_pts(rp, a.rec_cell, a.rec_row, rlo, rhi, [&](int cp, int row) {
                     if (own && cp == kc)
@@ -502,11 +499,25 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                 l_vx[e] = cin ? vx : 0.0f;
                 l_vz[e] = cin ? vz : 0.0f;
                 float ux = qu1[p], uz = qu2[p];
+                int ex = 0, ez = 0, evx = 0, evz = 0;
+                if (seg16 && vr >= 1 && vr <= TX) {  // wave-uniform: every lane reduces
+                    ex = snap_exp(wave_max(own ? fabsf(ux) : 0.0f));
+                    ez = snap_exp(wave_max(own ? fabsf(uz) : 0.0f));
+                    evx = snap_exp(wave_max(own ? fabsf(vx) : 0.0f));
+                    evz = snap_exp(wave_max(own ? fabsf(vz) : 0.0f));
+                    if (lane == 0) {
+                        put_exp(fu, i, ex);
+                        put_exp(fu + 1, i, ez);
+                        put_exp(fv, i, evx);
+                        put_exp(fv + 1, i, evz);
+                        *reinterpret_cast<int *>(a.sexp) = CZ;
+                    }
+                }
                 if (own) {
-                    if (a.snap_u) {
+                    if (a.snap) {
                         const long su = sb + (long)fu * g.N + kc;
-                        snap_put(a.snap_u, fmt, su, ux, su1);
-                        snap_put(a.snap_u, fmt, su + g.N, uz, su2);
+                        snap_put(a.snap, fmt, su, ux, ex);
+                        snap_put(a.snap, fmt, su + g.N, uz, ez);
                     }
                     ux += vx * g.dt;
                     uz += vz * g.dt;
@@ -514,16 +525,10 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                     a.vz1[o] = vz;
                     a.ux[o] = ux;
                     a.uz[o] = uz;
-                    if (a.snap_v) {
+                    if (a.snap) {
                         const long sv = sb + (long)fv * g.N + kc;
-                        snap_put(a.snap_v, fmt, sv, a.v_after ? vx : qv1[p], sv1);
-                        snap_put(a.snap_v, fmt, sv + g.N, a.v_after ? vz : qv2[p], sv2);
-                    }
-                    if (track) {
-                        mu1 = fmaxf(mu1, fabsf(ux));
-                        mu2 = fmaxf(mu2, fabsf(uz));
-                        mv1 = fmaxf(mv1, fabsf(vx));
-                        mv2 = fmaxf(mv2, fabsf(vz));
+                        snap_put(a.snap, fmt, sv, vx, evx);
+                        snap_put(a.snap, fmt, sv + g.N, vz, evz);
                     }
                 }
                 scan_pts(rp, a.rec_cell, a.rec_row, rlo, rhi, [&](int cp, int row) {
@@ -536,11 +541,7 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
             }
         }
     }
-    // max-abs of this tile's u and v after the step
-    if (track) track_stage<PSV>(l_track, wave, lane, mu1, mu2, mv1, mv2);
     __syncthreads();
-    if (track && threadIdx.x == 0)
-        track_commit<PSV>(l_track, NWAVE, a.acc_u, a.acc_v, b * a.nsnapf, fu, fv);
 
     // 3. stress of step n+1, rows i0 .. i0+TX-1, columns j0 .. j0+CZ-1 (the
     // tile's explosions, one per lane, loaded while every lane is active)
@@ -801,8 +802,9 @@ static void ensure_second_set(Ctx &c) {
 }
 
 // ---------------------------------------------------------------------------
-// Snapshot storage.  Forward snapshots for the adjoint hold FP32, FP16 or BF16
-// values (16-bit ones scaled per snapshot field by its max-abs).  They cover
+// Snapshot storage.  Forward snapshots for the adjoint hold FP32, FP16, BF16
+// or I16 values (16-bit ones scaled per row segment by a power of two, with
+// an exponent table per slot).  They cover
 // either every snapshot step of the run (ckpt = 0) or, with checkpoints every
 // ckpt steps, one ckpt-step segment: the forward keeps the FP32 state at each
 // segment start plus the snapshots of its last segment, and the adjoint
@@ -818,7 +820,8 @@ void snap_storage(Ctx &c) {
     const size_t B = (size_t)c.g.batch, N = (size_t)c.N;
     c.nslots = c.ckpt > 0 ? c.ckpt / c.g.sfe : c.nsfe;
     c.snap.ensure(B * (size_t)c.nslots * c.nsnapf * N * snap_esize(c));
-    c.smax.ensure((size_t)c.nslots * B * c.nsnapf * sizeof(unsigned));
+    if (c.snap_format != INV_SNAP_F32)
+        c.sexp.ensure((size_t)c.nslots * snap_tab_bytes(c.g.batch, c.nsnapf, c.g.nx, c.g.nz));
     if (c.ckpt > 0) {
         const size_t nck = (size_t)(c.g.nt / c.ckpt - 1);
         if (nck > 0) c.ckbuf.ensure(nck * nstate_fields(c) * B * N * sizeof(float));
@@ -911,102 +914,44 @@ struct SnapPlan {
     bool on = false;
     int fmt = INV_SNAP_F32, sfe = 1, nsfe = 0, nslots = 0, base = 0;
     char *snap = nullptr;
-    unsigned *smax = nullptr;
+    signed char *sexp = nullptr;
     long slot_bytes = 0;  // one snapshot slot of batch slot 0
-    long smax_slot = 0;   // max-abs entries per slot: [batch][nsnapf]
+    long tab_bytes = 0;   // one slot's exponent table
     int local(long it) const {
         if (!on || it < 0 || (it + 1) % sfe != 0) return -1;
         const long l = nsfe - (it + 1) / sfe - base;
         return (l >= 0 && l < nslots) ? (int)l : -1;
     }
     void *slot(int l) const { return snap + l * slot_bytes; }
-    unsigned *mx(int l) const { return smax + l * smax_slot; }
+    signed char *tab(int l) const { return sexp ? sexp + l * tab_bytes : nullptr; }
 };
 
 static SnapPlan snap_plan(Ctx &c, bool on, int base) {
     SnapPlan s;
-    s.on = on && c.snap.p && c.smax.p && c.g.sfe > 0;
     s.fmt = c.snap_format;
+    s.on = on && c.snap.p && (s.fmt == INV_SNAP_F32 || c.sexp.p) && c.g.sfe > 0;
     s.sfe = c.g.sfe > 0 ? c.g.sfe : 1;
     s.nsfe = c.nsfe;
     s.nslots = c.nslots;
     s.base = base;
     s.snap = c.snap.as<char>();
-    s.smax = c.smax.as<unsigned>();
+    s.sexp = s.fmt == INV_SNAP_F32 ? nullptr : c.sexp.as<signed char>();
     s.slot_bytes = (long)c.nsnapf * c.N * (long)snap_esize(c);
-    s.smax_slot = (long)c.g.batch * c.nsnapf;
+    s.tab_bytes = (long)snap_tab_bytes(c.g.batch, c.nsnapf, c.g.nx, c.g.nz);
     return s;
 }
 
-// Snapshot work of forward step it.  FP32: u before and v after go to the
-// step's slot in the step itself.  16-bit: step it writes u before into its
-// slot (scaled by the max |u| gathered by step it-1), writes the previous
-// step's v into that step's slot (scaled by the max |v| gathered there), and
-// gathers max |u| for the next snapshot step and max |v| for its own.
-static void snap_roles(const SnapPlan &s, StepArgs &a, long it, long first_step) {
-    a.snap_u = a.snap_v = nullptr;
-    a.mx_u = a.mx_v = nullptr;
-    a.acc_u = a.acc_v = nullptr;
-    a.v_after = 1;
+// Snapshot role of forward step it: u before and v after the step go to the
+// step's slot (and, for 16-bit formats, the exponents of their row segments
+// to the slot's table) in the step itself.
+static void snap_roles(const SnapPlan &s, StepArgs &a, long it) {
+    a.snap = nullptr;
+    a.sexp = nullptr;
     const int l = s.local(it);
-    if (s.fmt == INV_SNAP_F32) {
-        if (l >= 0) a.snap_u = a.snap_v = s.slot(l);
-        return;
-    }
-    a.v_after = 0;
     if (l >= 0) {
-        a.snap_u = s.slot(l);
-        a.mx_u = s.mx(l);
-        a.acc_v = s.mx(l);
+        a.snap = s.slot(l);
+        a.sexp = s.tab(l);
     }
-    const int lp = it > first_step ? s.local(it - 1) : -1;
-    if (lp >= 0) {
-        a.snap_v = s.slot(lp);
-        a.mx_v = s.mx(lp);
-    }
-    const int ln = s.local(it + 1);
-    if (ln >= 0) a.acc_u = s.mx(ln);
-}
-
-// v after the last step of a run or segment into its snapshot slot (16-bit
-// formats; no later step will write it).  Idempotent.
-__global__ void k_snap_flush(const float *__restrict__ v1, const float *__restrict__ v2,
-                             unsigned short *__restrict__ dst, long slot_stride, int nsnapf, int fv,
-                             const unsigned *__restrict__ mx, int fmt, long N) {
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y;
-    if (k >= N) return;
-    unsigned short *d = dst + (long)b * slot_stride + (long)fv * N + k;
-    d[0] = snap_enc_mul(v1[(long)b * N + k], snap_enc_scale(mx[b * nsnapf + fv]), fmt);
-    if (v2) d[N] = snap_enc_mul(v2[(long)b * N + k], snap_enc_scale(mx[b * nsnapf + fv + 1]), fmt);
-}
-
-// acc[b][f] = max(acc[b][f], max |x_b|) over count slots of x [batch][N]
-__global__ __launch_bounds__(256) void k_absmax(const float *__restrict__ x, long N, unsigned *acc,
-                                                int nsnapf, int f) {
-    const int b = blockIdx.y;
-    float m = 0.0f;
-    for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < N; k += (long)gridDim.x * 256)
-        m = fmaxf(m, fabsf(x[(long)b * N + k]));
-    m = wave_max(m);
-    if ((threadIdx.x & 63) == 0) atomicMax(acc + b * nsnapf + f, __float_as_uint(m));
-}
-
-static void snap_flush(Ctx &c, const SnapPlan &sp, const Fields &F, int par, int count, long last) {
-    if (sp.fmt == INV_SNAP_F32) return;
-    const int l = sp.local(last);
-    if (l < 0) return;
-    const dim3 grid((unsigned)((c.N + 255) / 256), (unsigned)count);
-    const long stride = (long)c.nslots * c.nsnapf * c.N;
-    auto *dst = static_cast<unsigned short *>(sp.slot(l));
-    const int p = par & 1;
-    if (c.g.sh)
-        k_snap_flush<<<grid, 256, 0, c.stream>>>(F.vy[p], nullptr, dst, stride, c.nsnapf, 1,
-                                                 sp.mx(l), sp.fmt, c.N);
-    if (c.g.psv)
-        k_snap_flush<<<grid, 256, 0, c.stream>>>(F.vx[p], F.vz[p], dst, stride, c.nsnapf,
-                                                 (c.g.sh ? 2 : 0) + 2, sp.mx(l), sp.fmt, c.N);
-    INV_HIP(hipGetLastError());
 }
 
 void fused_prepare(Ctx &c, int kind, int first, int count, int flags) {
@@ -1098,18 +1043,11 @@ static void pair_step(Ctx &c, StepArgs &a, const StepRoles &r2, const Fields &F,
     par ^= 1;
 }
 
-static StepRoles roles_of(const StepArgs &a) {
-    return StepRoles{a.snap_u, a.snap_v, a.v_after, a.mx_u, a.mx_v, a.acc_u, a.acc_v};
-}
+static StepRoles roles_of(const StepArgs &a) { return StepRoles{a.snap, a.sexp}; }
 
 static void set_roles(StepArgs &a, const StepRoles &r) {
-    a.snap_u = r.snap_u;
-    a.snap_v = r.snap_v;
-    a.v_after = r.v_after;
-    a.mx_u = r.mx_u;
-    a.mx_v = r.mx_v;
-    a.acc_u = r.acc_u;
-    a.acc_v = r.acc_v;
+    a.snap = r.snap;
+    a.sexp = r.sexp;
 }
 
 static void common_args(Ctx &c, StepArgs &a) {
@@ -1129,7 +1067,7 @@ static void common_args(Ctx &c, StepArgs &a) {
     a.nsnapf = c.nsnapf;
     a.psv_f0 = c.g.sh ? 2 : 0;
     a.snap_slot_stride = (long)c.nslots * c.nsnapf * c.N;
-    a.v_after = 1;
+    a.nseg = snap_nseg(c.g.nz);
     INV_REQUIRE(a.rho && a.lam && a.mu && a.abx && a.abzb && a.abzt, INV_ERR_STATE,
                 "model or absorbing profile not allocated");
     INV_REQUIRE((!c.g.psv || (c.ux.p && c.uz.p && c.vx.p && c.vx_b.p && c.sxx_b.p)) &&
@@ -1212,12 +1150,9 @@ void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
         c.parity = 0;
         fdm_zero_state(c, count);
         fdm_prepare_model(c);
-        if (sp.on && sp.fmt != INV_SNAP_F32)
-            fill_dev(c, c.smax.p, (long)c.nslots * c.g.batch * c.nsnapf);
     }
     // two steps per launch where the steps allow it: no checkpoint before the
-    // second step, no max-abs collected by the first step (the second would
-    // read it)
+    // second step
     const bool pr = pair_on(c);
     StepArgs a2;
     if (pr) a2 = pair_args(c, a, first, count, false, true);
@@ -1228,11 +1163,10 @@ void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
             for (int f = 0; f < nf; f++)
                 vec_copy(c, ckpt_field(c, it / c.ckpt, f), cur[f], (size_t)count * c.N);
         }
-        snap_roles(sp, a, it, 0);
-        if (pr && it + 1 < it1 && !a.acc_u && !a.acc_v &&
-            !(sp.on && c.ckpt > 0 && (it + 1) % c.ckpt == 0)) {
+        snap_roles(sp, a, it);
+        if (pr && it + 1 < it1 && !(sp.on && c.ckpt > 0 && (it + 1) % c.ckpt == 0)) {
             StepArgs n = a;
-            snap_roles(sp, n, it + 1, 0);
+            snap_roles(sp, n, it + 1);
             set_roles(a2, roles_of(a));
             pair_step(c, a2, roles_of(n), F, c.parity, count, it);
             it += 2;
@@ -1241,7 +1175,6 @@ void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
             it += 1;
         }
     }
-    if (sp.on && it1 > it0) snap_flush(c, sp, F, c.parity, count, it1 - 1);
 }
 
 // Recompute forward segment fseg (steps fseg*ckpt ..) from its checkpoint,
@@ -1250,7 +1183,6 @@ static void replay_segment(Ctx &c, int first, int count, int fseg, int aseg) {
     const int K = c.ckpt;
     const long f0 = (long)fseg * K;
     const SnapPlan sp = snap_plan(c, true, aseg * (K / c.g.sfe));
-    if (sp.fmt != INV_SNAP_F32) fill_dev(c, c.smax.p, (long)c.nslots * c.g.batch * c.nsnapf);
     const Fields R = replay_fields(c);
     int par = 0;
     float *dst[11];
@@ -1260,20 +1192,6 @@ static void replay_segment(Ctx &c, int first, int count, int fseg, int aseg) {
         if (fseg == 0) fill_dev(c, dst[f], (long)n);
         else vec_copy(c, dst[f], ckpt_field(c, fseg, f), n);
     }
-    if (sp.fmt != INV_SNAP_F32 && c.g.sfe == 1) {
-        // step f0 is a snapshot step whose u before is the restored state
-        const int l = sp.local(f0);
-        if (l >= 0) {
-            const dim3 grid((unsigned)std::min<long>(1024, (c.N + 255) / 256), (unsigned)count);
-            if (c.g.sh) k_absmax<<<grid, 256, 0, c.stream>>>(R.uy, c.N, sp.mx(l), c.nsnapf, 0);
-            if (c.g.psv) {
-                const int fu = c.g.sh ? 2 : 0;
-                k_absmax<<<grid, 256, 0, c.stream>>>(R.ux, c.N, sp.mx(l), c.nsnapf, fu);
-                k_absmax<<<grid, 256, 0, c.stream>>>(R.uz, c.N, sp.mx(l), c.nsnapf, fu + 1);
-            }
-            INV_HIP(hipGetLastError());
-        }
-    }
     StepArgs a;
     common_args(c, a);
     forward_args(c, a, first, count, false);
@@ -1281,10 +1199,10 @@ static void replay_segment(Ctx &c, int first, int count, int fseg, int aseg) {
     StepArgs a2;
     if (pr) a2 = pair_args(c, a, first, count, false, false);
     for (long it = f0; it < f0 + K;) {
-        snap_roles(sp, a, it, f0);
-        if (pr && it + 1 < f0 + K && !a.acc_u && !a.acc_v) {
+        snap_roles(sp, a, it);
+        if (pr && it + 1 < f0 + K) {
             StepArgs n = a;
-            snap_roles(sp, n, it + 1, f0);
+            snap_roles(sp, n, it + 1);
             set_roles(a2, roles_of(a));
             pair_step(c, a2, roles_of(n), R, par, count, (int)it);
             it += 2;
@@ -1293,7 +1211,6 @@ static void replay_segment(Ctx &c, int first, int count, int fseg, int aseg) {
             it += 1;
         }
     }
-    snap_flush(c, sp, R, par, count, f0 + K - 1);
 }
 
 void fused_adjoint(Ctx &c, int first, int count, int it0, int it1) {

@@ -23,14 +23,13 @@
 // launches of k_colv (and of k_col and the FP32 oracle in exact math).
 // Point lists are bucketed for this geometry: forces with a halo of 4 rows
 // and columns, explosions 3, receivers 0.  Snapshot roles of the first step
-// come in StepArgs, of the second in StepRoles.  The first step gathers no
-// max-abs (the launcher refuses that role, and the kernel drops it), and a
-// table the second step fills is read by no step of the same launch (the
-// host never pairs such steps).  SNAP = false is the variant for launches with no
-// snapshot role in either step (most of them: every adjoint launch and three
-// of four forward ones with sfe = 10): the roles are compile-time null there,
-// so the snapshot code and the scalar registers of its pointers are gone.
-// REC = false likewise drops receiver recording (adjoint and recomputation).
+// come in StepArgs, of the second in StepRoles; 16-bit snapshots are scaled
+// per row segment of S columns (the tile's owned columns), segment tz.
+// SNAP = false is the variant for launches with no snapshot role in either
+// step (most of them: every adjoint launch and four of five forward ones with
+// sfe = 10): the roles are compile-time null there, so the snapshot code and
+// the scalar registers of its pointers are gone.  REC = false likewise drops
+// receiver recording (adjoint and recomputation).
 // Needs nz % V == 0 and dynamic LDS of pair_lds_bytes.
 #include "colv.h"
 
@@ -53,7 +52,6 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
     constexpr int PS = (SR + NW - 1) / NW;      // stress rows loaded per wavefront
     constexpr int P = (VR + NW - 1) / NW;       // rows x = wave - 4 + NW p per wavefront
     extern __shared__ __align__(16) float smem[];
-    __shared__ float l_track[4 * NW];  // per-wavefront maxima of the second step
     float *const l_s1 = smem;
     float *const l_s2 = l_s1 + SR * W;
     float *const l_s3 = PSV ? l_s2 + VR * W : l_s2;  // unused for SH
@@ -171,16 +169,12 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
     const long it = a.it;
     const int fu = PSV ? a.psv_f0 : 0, fv = PSV ? a.psv_f0 + 2 : 1;
     const long sb = (long)b * a.snap_slot_stride;
-    const int mb = b * a.nsnapf;
-    const StepRoles none{nullptr, nullptr, 1, nullptr, nullptr, nullptr, nullptr};
-    const StepRoles r1s =
-        SNAP ? StepRoles{a.snap_u, a.snap_v, a.v_after, a.mx_u, a.mx_v, nullptr, nullptr} : none;
+    const StepRoles none{nullptr, nullptr};
+    const StepRoles r1s = SNAP ? StepRoles{a.snap, a.sexp} : none;
     const StepRoles r2 = SNAP ? r2in : none;
-    float mu1 = 0.0f, mu2 = 0.0f, mv1 = 0.0f, mv2 = 0.0f;
 
     // A. velocity of step n+1 on rows x = -4 .. TX+4 (LDS row x + 4)
     {
-        const SnapScales ss = snap_scales<PSV>(r1s, a.snap_fmt, mb, fu, fv);
 #pragma unroll
         for (int p = 0; p < P; p++) {
             const int x = wave - 4 + p * NW;
@@ -195,8 +189,8 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
             vstore<V>(&l_v1[(x + 4) * W + c0], n1);
             if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], n2);
             if (x >= 0 && x < TX && i < nx)  // wave-uniform
-                owned_row<PSV, V>(a, r1s, ss, fu, fv, sb, i, jb, cown, it, b, rp, rlo, rhi,
-                                  qu1[p], qu2[p], qv1[p], qv2[p], n1, n2, mu1, mu2, mv1, mv2);
+                owned_row<PSV, V>(a, r1s, fu, fv, sb, i, jb, cown, it, b, tz, S, lane, rp, rlo,
+                                  rhi, qu1[p], qu2[p], n1, n2);
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 qv1[p][v] = n1[v];
@@ -238,9 +232,7 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
 
     // C. velocity of step n+2 on rows x = -1 .. TX+1, in place; owned cells
     // store v and u
-    mu1 = mu2 = mv1 = mv2 = 0.0f;
     {
-        const SnapScales ss = snap_scales<PSV>(r2, a.snap_fmt, mb, fu, fv);
 #pragma unroll
         for (int p = 0; p < P; p++) {
             const int x = wave - 4 + p * NW;
@@ -256,8 +248,8 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
             vstore<V>(&l_v1[(x + 4) * W + c0], n1);
             if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], n2);
             if (x >= 0 && x < TX && i < nx) {  // wave-uniform
-                owned_row<PSV, V>(a, r2, ss, fu, fv, sb, i, jb, cown, it + 1, b, rp, rlo, rhi,
-                                  qu1[p], qu2[p], qv1[p], qv2[p], n1, n2, mu1, mu2, mv1, mv2);
+                owned_row<PSV, V>(a, r2, fu, fv, sb, i, jb, cown, it + 1, b, tz, S, lane, rp,
+                                  rlo, rhi, qu1[p], qu2[p], n1, n2);
                 if (cown) {
                     const long o = so + (long)i * nz + jb;
                     vstore<V>(V1o + o, n1);
@@ -269,11 +261,8 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
                 }
             }
         }
-        if (r2.acc_u || r2.acc_v) track_stage<PSV>(l_track, wave, lane, mu1, mu2, mv1, mv2);
     }
     __syncthreads();
-    if ((r2.acc_u || r2.acc_v) && threadIdx.x == 0)
-        track_commit<PSV>(l_track, NW, r2.acc_u, r2.acc_v, mb, fu, fv);
 
     // D. stress of step n+2 on the owned cells (every lane runs the rows, for
     // the explosion scan; owned lanes store)
@@ -312,11 +301,7 @@ static void pair_go(Ctx &c, K kern, int nw, int lds, const StepArgs &a, const St
 
 template <bool SH, bool PSV, bool FAST>
 void pair_launch(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned blocks) {
-    // The first step of a pair never gathers a maximum (the kernel drops that
-    // role): its snapshot would need the maximum before the launch ends.
-    INV_REQUIRE(!a.acc_u && !a.acc_v, INV_ERR_STATE,
-                "two-step launch asked to gather a maximum in its first step");
-    const bool snap = a.snap_u || a.snap_v || r2.snap_u || r2.snap_v || r2.acc_u || r2.acc_v;
+    const bool snap = a.snap || r2.snap;
     const int var = (snap ? 2 : 0) + (a.rec_ptr ? 1 : 0);
     const int rows = c.pair_tx, V = c.pair_v;
     const int lds = pair_lds_bytes(PSV, rows, V);

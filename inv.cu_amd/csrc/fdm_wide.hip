@@ -13,7 +13,9 @@
 // of step n+1 is computed on the tile plus halo from LDS (owned cells also
 // update u, snapshots and receivers) and goes to LDS; the stress of step n+1
 // is computed on the owned cells.  Arithmetic is expression for expression
-// that of k_col, so results are bitwise identical to it.  SNAP = false is the
+// that of k_col, so results are bitwise identical to it (16-bit snapshots
+// are scaled per row segment of the owned columns, so their scales follow the
+// tile width).  SNAP = false is the
 // variant for steps without snapshot roles (with FP32 snapshots every sfe,
 // nine forward steps in ten and every adjoint step): the roles are
 // compile-time null there and their code and registers are gone; REC =
@@ -34,7 +36,6 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
     constexpr int PS = (SR + NW - 1) / NW, PV = (VR + NW - 1) / NW, PT = (TX + NW - 1) / NW;
     __shared__ __align__(16) float l_s1[SR * W], l_s2[SR * W], l_s3[PSV ? SR * W : 4];
     __shared__ __align__(16) float l_v1[VR * W], l_v2[PSV ? VR * W : 4];
-    __shared__ float l_track[4 * NW];
 
     // XCD-aware tile order (as k_col)
     const int nw = gridDim.x, w = blockIdx.x;
@@ -143,12 +144,7 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
     const long it = a.it;
     const int fu = PSV ? a.psv_f0 : 0, fv = PSV ? a.psv_f0 + 2 : 1;
     const long sb = (long)b * a.snap_slot_stride;
-    const StepRoles r1s =
-        SNAP ? StepRoles{a.snap_u, a.snap_v, a.v_after, a.mx_u, a.mx_v, a.acc_u, a.acc_v}
-             : StepRoles{nullptr, nullptr, 1, nullptr, nullptr, nullptr, nullptr};
-    const SnapScales ss = snap_scales<PSV>(r1s, a.snap_fmt, b * a.nsnapf, fu, fv);
-    const bool track = r1s.acc_u || r1s.acc_v;
-    float mu1 = 0.0f, mu2 = 0.0f, mv1 = 0.0f, mv2 = 0.0f;
+    const StepRoles r1s = SNAP ? StepRoles{a.snap, a.sexp} : StepRoles{nullptr, nullptr};
 
     // 2. velocity of step n+1 on rows i0-1 .. i0+TX+1 (stress LDS rows vr .. vr+3)
 #pragma unroll
@@ -170,8 +166,8 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         // owned cells: v, u, snapshots, receivers (the row test is
         // wave-uniform; owned_row takes the lane's ownership)
         if (vr >= 1 && vr <= TX && i < nx) {
-            owned_row<PSV, V>(a, r1s, ss, fu, fv, sb, i, jb, cown, it, b, rp, rlo, rhi, qu1[p],
-                              qu2[p], qv1[p], qv2[p], n1, n2, mu1, mu2, mv1, mv2);
+            owned_row<PSV, V>(a, r1s, fu, fv, sb, i, jb, cown, it, b, tz, S, lane, rp, rlo, rhi,
+                              qu1[p], qu2[p], n1, n2);
             if (cown) {
                 const long o = so + (long)i * nz + jb;
                 vstore<V>(V1o + o, n1);
@@ -183,11 +179,7 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
             }
         }
     }
-    // max-abs of this tile's u and v after the step
-    if (track) track_stage<PSV>(l_track, wave, lane, mu1, mu2, mv1, mv2);
     __syncthreads();
-    if (track && threadIdx.x == 0)
-        track_commit<PSV>(l_track, NW, r1s.acc_u, r1s.acc_v, b * a.nsnapf, fu, fv);
 
     // 3. stress of step n+1 on the owned cells (velocity LDS rows tr .. tr+3)
     // the tile's explosions, one per lane
@@ -223,7 +215,7 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
 
 template <bool SH, bool PSV, bool FAST>
 void wide_launch(Ctx &c, const StepArgs &a, unsigned blocks) {
-    const bool snap = a.snap_u || a.snap_v || a.acc_u || a.acc_v;
+    const bool snap = a.snap != nullptr;
     const int var = (snap ? 2 : 0) + (a.rec_ptr ? 1 : 0);
 #define INV_COLV(R, VV)                                                                          \
     (var == 3   ? k_colv<SH, PSV, R, VV, FAST, true, true><<<blocks, WBLOCK, 0, c.stream>>>(a)    \

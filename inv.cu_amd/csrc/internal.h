@@ -124,8 +124,8 @@ struct Ctx {
     // keep snapshot and per-slot kernel buffers across inv_grid_set (they only
     // grow), for contexts that run one batch after another of similar size
     bool retain = false;
-    DevBuf snap;   // [batch][nslots][nsnapf][N] of FP32, FP16 or BF16
-    DevBuf smax;   // max-abs of each snapshot field [nslots][batch][nsnapf] (FP32 bits)
+    DevBuf snap;   // [batch][nslots][nsnapf][N] of FP32, FP16, BF16 or I16
+    DevBuf sexp;   // 16-bit formats: exponent tables [nslots][snap_tab_bytes]
     DevBuf ckbuf;  // state checkpoints [nt/ckpt - 1][state fields][batch][N]
     DevBuf rstate; // wavefields of the checkpoint recomputation
     DevBuf out;    // [batch][ncomp][nrec][nt]
@@ -287,6 +287,14 @@ void vec_fill(Ctx &c, float *out, float v, size_t n);
 double vec_dot(Ctx &c, const float *a, const float *b, size_t n);
 float vec_amax(Ctx &c, const float *a, size_t n);
 void sum_f64(Ctx &c, const double *x, size_t n, double *dev_out);
+
+// Exponent tables of 16-bit snapshots (stencil.h): segments per row (the
+// narrowest tile that writes snapshots owns 58 columns) and bytes per slot.
+inline int snap_nseg(int nz) { return (nz + 57) / 58 + 1; }
+inline size_t snap_tab_bytes(int batch, int nsnapf, int nx, int nz) {
+    const size_t b = SNAPTAB_HDR + (size_t)batch * nsnapf * nx * snap_nseg(nz);
+    return (b + 255) / 256 * 256;
+}
 
 }  // namespace inv
 

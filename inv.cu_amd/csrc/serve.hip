@@ -565,7 +565,7 @@ int inv_server_create(const inv_server_desc *desc, inv_server **out) {
     }
     if (desc->max_lanes < 1 || desc->max_lanes > 16 || desc->max_slots < 1 ||
         desc->max_bypass < 0 || desc->snap_format < INV_SNAP_F32 ||
-        desc->snap_format > INV_SNAP_BF16 ||
+        desc->snap_format > INV_SNAP_I16 ||
         (desc->math != INV_MATH_EXACT && desc->math != INV_MATH_FAST) ||
         (desc->tile_x != 0 && desc->tile_x != 8 && desc->tile_x != 16 && desc->tile_x != 32))
         return INV_ERR_ARG;

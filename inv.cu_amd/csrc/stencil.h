@@ -50,58 +50,89 @@ __device__ __forceinline__ float d_plus(const float *__restrict__ f, long k, lon
 }
 
 // ---------------------------------------------------------------------------
-// Reduced-precision snapshot storage.  A snapshot field x with max-abs m is
-// stored as q = x * (SNAP_RANGE / m) in FP16 or BF16 (round to nearest even)
-// and read back as q * (m / SNAP_RANGE).  SNAP_RANGE = 2^15 keeps the largest
-// value below FP16's 65504 and leaves 2^-24 / 2^15 of m as the smallest
-// normal magnitude, so values down to ~1e-9 m keep FP16's full 11-bit
-// significand.  Arithmetic stays FP32.
+// Reduced-precision snapshot storage.  A snapshot field is scaled per row
+// segment (the owned columns of one tile row of the kernel that writes it) by
+// a power of two 2^e chosen from the segment's max-abs m so that m 2^e lies
+// in [2^14, 2^15) (snap_exp); the product x 2^e is exact in FP32.  The
+// formats store it as
+//   FP16 / BF16  the nearest FP16 / BF16 value (round to nearest even),
+//   I16          the nearest integer, |q| <= 32767 (16-bit fixed point
+//                relative to the segment: 15 bits below its max-abs),
+// and decode by multiplying with 2^-e.  The exponents of a slot live in its
+// exponent table: a SNAPTAB_HDR-byte header holding the segment width (int)
+// of the writing kernel, then one signed byte per [batch][field][row][seg].
+// All arithmetic stays FP32.
 // ---------------------------------------------------------------------------
-constexpr float SNAP_RANGE = 32768.0f;
+constexpr int SNAP_EMAX = 120;
+constexpr int SNAPTAB_HDR = 64;
 
-__device__ __forceinline__ unsigned short snap_enc16(float y, int fmt) {
+// 2^e as a float, e in [-126, 127]
+__device__ __host__ __forceinline__ float pow2f(int e) {
+    const unsigned bits = (unsigned)(127 + e) << 23;
+    float f;
+    __builtin_memcpy(&f, &bits, 4);
+    return f;
+}
+
+// e with m 2^e in [2^14, 32767.5) for normal m > 0 (one less when m 2^e
+// would round to 2^15, so that |x 2^e| rounds to at most 32767 for every
+// |x| <= m), 0 for m = 0, clamped to [-SNAP_EMAX, SNAP_EMAX]
+__device__ __host__ __forceinline__ int snap_exp(float m) {
+    if (!(m > 0.0f)) return 0;
+    unsigned bits;
+    __builtin_memcpy(&bits, &m, 4);
+    const int k = (int)((bits >> 23) & 0xffu) - 126;  // m in [2^(k-1), 2^k)
+    int e = 15 - k;
+    e = e > SNAP_EMAX ? SNAP_EMAX : (e < -SNAP_EMAX ? -SNAP_EMAX : e);
+    if (m * pow2f(e) >= 32767.5f) e -= 1;
+    return e;
+}
+
+// one value of a 16-bit format: x 2^e encoded (e: the segment's exponent)
+__device__ __forceinline__ unsigned short snap_enc16(float x, int e, int fmt) {
+    float y = x * pow2f(e);
+    // keep the product a separate FP32 value (exact here) before the
+    // conversion, in every kernel alike
+    __asm__ volatile("" : "+v"(y));
     if (fmt == INV_SNAP_F16) {
         const _Float16 h = (_Float16)y;
         return __builtin_bit_cast(unsigned short, h);
+    }
+    if (fmt == INV_SNAP_I16) {
+        float r = rintf(y);
+        r = r > 32767.0f ? 32767.0f : (r < -32767.0f ? -32767.0f : r);
+        return (unsigned short)(short)(int)r;
     }
     unsigned u = __float_as_uint(y);  // BF16: round to nearest even (y finite)
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (unsigned short)(u >> 16);
 }
 
-__device__ __forceinline__ float snap_dec16(unsigned short q, int fmt) {
-    if (fmt == INV_SNAP_F16) return (float)__builtin_bit_cast(_Float16, q);
-    return __uint_as_float((unsigned)q << 16);
+__device__ __forceinline__ float snap_dec16(unsigned short q, int e, int fmt) {
+    float y;
+    if (fmt == INV_SNAP_F16) y = (float)__builtin_bit_cast(_Float16, q);
+    else if (fmt == INV_SNAP_I16) y = (float)(short)q;
+    else y = __uint_as_float((unsigned)q << 16);
+    return y * pow2f(-e);
 }
 
-// encode factor SNAP_RANGE / m for a stored max-abs m (0 for an all-zero field)
-__device__ __forceinline__ float snap_enc_scale(unsigned mbits) {
-    const float m = __uint_as_float(mbits);
-    return m > 0.0f ? SNAP_RANGE / m : 0.0f;
+// the exponents of row i of field f of batch slot b in a slot's table
+__device__ __forceinline__ long snaptab_row(int b, int nsnapf, int f, int nx, int i, int nseg) {
+    return SNAPTAB_HDR + (((long)b * nsnapf + f) * nx + i) * nseg;
 }
 
-// store one snapshot value: FP32 as is, FP16/BF16 scaled by s.  The scaled
-// value is rounded to FP32 first and then to 16 bits: the empty asm keeps the
-// compiler from folding the multiply and the conversion into one mixed-
-// precision instruction (a single rounding), which it does in some kernels
-// and not in others, so that every kernel stores the same bits.
-__device__ __forceinline__ unsigned short snap_enc_mul(float x, float s, int fmt) {
-    float y = x * s;
-    __asm__ volatile("" : "+v"(y));
-    return snap_enc16(y, fmt);
-}
-
-__device__ __forceinline__ void snap_put(void *base, int fmt, long idx, float x, float s) {
-    if (fmt == INV_SNAP_F32) static_cast<float *>(base)[idx] = x;
-    else static_cast<unsigned short *>(base)[idx] = snap_enc_mul(x, s, fmt);
-}
-
-// read access to one 16-bit snapshot field, decoded to FP32
+// read access to one 16-bit snapshot field, decoded to FP32 (k a cell index;
+// its exponent is that of its row segment)
 template <int FMT>
 struct Snap16 {
     const unsigned short *q;
-    float s;  // m / SNAP_RANGE
-    __device__ __forceinline__ float operator[](long k) const { return snap_dec16(q[k], FMT) * s; }
+    const signed char *tab;  // the field's rows: tab[i * nseg + seg]
+    int nz, nseg, segw;
+    __device__ __forceinline__ float operator[](long k) const {
+        const int i = (int)(k / nz), j = (int)(k - (long)i * nz);
+        const int sg = j / segw;
+        return snap_dec16(q[k], tab[(long)i * nseg + (sg < nseg - 1 ? sg : nseg - 1)], FMT);
+    }
 };
 
 // D+ of a field read through an accessor (the same expression as d_plus)

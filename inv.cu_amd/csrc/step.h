@@ -32,17 +32,15 @@ struct StepArgs {
     const int *rec_ptr, *rec_cell, *rec_row;
     int rec_disp;
     // Snapshots (fdm.h:521-560: a slot holds u before and v after a step).
-    // snap_u: slot base (batch slot 0) that receives this step's input u, or
-    // null; snap_v: slot base that receives v, this step's result when
-    // v_after is set, else this step's input (the previous step's result),
-    // or null.  16-bit formats scale each field by the max-abs read from
-    // mx_u / mx_v ([batch][nsnapf] tables of the slot); acc_u / acc_v collect
-    // max |u| and max |v| after this step into the table of a slot still to
-    // be written.  The linear kernel supports FP32 with v_after only.
-    void *snap_u, *snap_v;
-    int v_after, snap_fmt, nsnapf, psv_f0;
-    const unsigned *mx_u, *mx_v;
-    unsigned *acc_u, *acc_v;
+    // snap: slot base (batch slot 0) that receives this step's input u and
+    // its result v, or null.  16-bit formats scale each row segment of a
+    // field (the owned columns of one tile row) by a power of two chosen
+    // from the segment's max-abs (snap_exp) and store its exponent in sexp,
+    // the slot's exponent table (SnapTab layout); nseg: segments per row in
+    // the table.  The linear kernel supports FP32 only.
+    void *snap;
+    signed char *sexp;
+    int snap_fmt, nsnapf, psv_f0, nseg;
     long snap_slot_stride;  // elements per batch slot
 };
 
@@ -50,53 +48,6 @@ __device__ __forceinline__ float wave_max(float x) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
     return x;
-}
-
-// Max-abs tracking for the 16-bit snapshot scales.  Each wavefront's maxima
-// of u and v after the step go to LDS (track_stage, every lane calls it);
-// after the block barrier that follows, one thread merges the block's and
-// raises a table only where it holds less (track_commit).  That is at most
-// one atomic per block and field, and none once a table holds a larger
-// value, instead of one per wavefront on the same few addresses (which
-// serialised chip-wide: +80 % on those steps).  Tables only grow, so a stale
-// read can only cause an unneeded atomic, never skip a needed one.
-// Non-negative floats order as their bit patterns.
-__device__ __forceinline__ void raise_max(unsigned *p, float m) {
-    const unsigned bits = __float_as_uint(m);
-    if (bits > __atomic_load_n(p, __ATOMIC_RELAXED)) atomicMax(p, bits);
-}
-
-template <bool PSV>
-__device__ __forceinline__ void track_stage(float *lt, int wave, int lane, float mu1, float mu2,
-                                            float mv1, float mv2) {
-    mu1 = wave_max(mu1);
-    mv1 = wave_max(mv1);
-    if (PSV) {
-        mu2 = wave_max(mu2);
-        mv2 = wave_max(mv2);
-    }
-    if (lane == 0) {
-        lt[wave * 4] = mu1;
-        lt[wave * 4 + 1] = mu2;
-        lt[wave * 4 + 2] = mv1;
-        lt[wave * 4 + 3] = mv2;
-    }
-}
-
-template <bool PSV>
-__device__ __forceinline__ void track_commit(const float *lt, int nwave, unsigned *acc_u,
-                                             unsigned *acc_v, int mb, int fu, int fv) {
-    float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int w = 0; w < nwave; w++)
-        for (int q = 0; q < 4; q++) m[q] = fmaxf(m[q], lt[w * 4 + q]);
-    if (acc_u) {
-        raise_max(acc_u + mb + fu, m[0]);
-        if (PSV) raise_max(acc_u + mb + fu + 1, m[1]);
-    }
-    if (acc_v) {
-        raise_max(acc_v + mb + fv, m[2]);
-        if (PSV) raise_max(acc_v + mb + fv + 1, m[3]);
-    }
 }
 
 // Launch the wide column kernel (fdm_wide.hip): tile_x rows, c.col_v = 2 or
@@ -111,10 +62,8 @@ constexpr int col_strip(int V) { return 64 * V - 2 * col_halo(V); }
 
 // The snapshot fields of StepArgs for one time step (see StepArgs).
 struct StepRoles {
-    void *snap_u, *snap_v;
-    int v_after;
-    const unsigned *mx_u, *mx_v;
-    unsigned *acc_u, *acc_v;
+    void *snap;
+    signed char *sexp;
 };
 
 // Two-step kernel (fdm_pair.hip): two time steps per launch on tiles of

@@ -16,7 +16,7 @@ LIB_PATH = HERE / "libinv_hip.so"
 
 INV_OK = 0
 INV_LAMBDA, INV_MU, INV_RHO, INV_K_LAMBDA, INV_K_MU, INV_K_RHO, INV_OUT, INV_ADSTF, INV_OBS = range(9)
-INV_SNAP_F32, INV_SNAP_F16, INV_SNAP_BF16 = 0, 1, 2
+INV_SNAP_F32, INV_SNAP_F16, INV_SNAP_BF16, INV_SNAP_I16 = 0, 1, 2, 3
 INV_FWD_SNAPSHOTS = 1
 INV_PROF_VELOCITY, INV_PROF_STRESS, INV_PROF_INTERACT, INV_PROF_STEP = 0, 1, 2, 3
 INV_PROF_FWD_RUN, INV_PROF_ADJ_RUN, INV_PROF_PAIR = 4, 5, 6

@@ -34,7 +34,7 @@ def test_default_workload_is_the_large_psv_one():
 def test_sh_workload_keeps_every_snapshot():
     P, B, wave, fmt, ckpt, desc = _plan(["--workload", "c2", "--nt", "200"])
     assert (P.nx, P.nz, P.nsrc, P.sh, P.psv) == (1474, 480, 64, 1, 0)
-    assert fmt == inv.invhip.INV_SNAP_F16
+    assert fmt == inv.invhip.INV_SNAP_I16
     assert B == 64 and wave == 64 and ckpt == 0
     assert "64 per launch" in desc and "every snapshot resident" in desc
 
@@ -42,7 +42,7 @@ def test_sh_workload_keeps_every_snapshot():
 def test_large_psv_workload_steps_one_wave_of_four_shots():
     P, B, wave, fmt, ckpt, desc = _plan(["--workload", "c4", "--nt", "100"])
     assert (P.nx, P.nz, P.nsrc, P.nrec, P.misfit) == (4096, 2048, 256, 2048, 1)
-    assert B == 4 and wave == 4 and ckpt == 0 and fmt == inv.invhip.INV_SNAP_F16
+    assert B == 4 and wave == 4 and ckpt == 0 and fmt == inv.invhip.INV_SNAP_I16
     # FP16 u and v of one shot at every 10th step must fit four times in HBM
     snap_bytes = 4 * 2 * P.nx * P.nz * (10000 // 10)
     assert 4 * snap_bytes < 280e9 < 5 * snap_bytes

@@ -250,26 +250,35 @@ def test_c4_exact_matches_oracle(gpu):
 
 
 def test_c4_two_step_equals_one_step(gpu):
+    """Forward runs and FP32-snapshot gradients of the two-step kernel equal
+    one step per launch bit for bit; with I16 snapshots (segments of 116
+    columns against 120) the gradients agree to 1e-4."""
     P = _c4(200, [40, 200])
-    _say(f"c4 two-step vs one-step: {P.nx}x{P.nz} nt={P.nt}, 2 shots, FP16 snapshots")
+    _say(f"c4 two-step vs one-step: {P.nx}x{P.nz} nt={P.nt}, 2 shots, FP32 and I16 snapshots")
     obs = _obs(P, 2)
-    res = []
-    for pair in ((20, 2), (0, 2)):
-        ctx = P.context(batch=2)
-        ctx.set_snapshots(H.INV_SNAP_F16, 0)
-        ctx.set_tile(16, 2)
-        ctx.set_pair(*pair)
-        ctx.forward(0, 2, snapshots=True)
-        fw = [(ctx.traces_get(b), ctx.state_get(b)) for b in range(2)]
-        for s in range(2):
-            ctx.obs_set(s, obs[s])
-        f = ctx.gradient(P.misfit, 0)
-        res.append((fw, f, ctx.kernels_get()))
-        ctx.close()
-    (fa, ga, ka), (fb, gb, kb) = res
-    for (ta, sa), (tb, sb) in zip(fa, fb):
-        assert np.array_equal(ta, tb) and np.array_equal(sa, sb)
-    assert ga == gb
-    for x, y in zip(ka, kb):
-        assert np.array_equal(x, y)
-    assert all(np.any(k) for k in ka)
+    res = {}
+    for fmt in (H.INV_SNAP_F32, H.INV_SNAP_I16):
+        for pair in ((20, 2), (0, 2)):
+            ctx = P.context(batch=2)
+            ctx.set_snapshots(fmt, 0)
+            ctx.set_tile(16, 2)
+            ctx.set_pair(*pair)
+            ctx.forward(0, 2, snapshots=True)
+            fw = [(ctx.traces_get(b), ctx.state_get(b)) for b in range(2)]
+            for s in range(2):
+                ctx.obs_set(s, obs[s])
+            f = ctx.gradient(P.misfit, 0)
+            res[fmt, pair[0]] = (fw, f, ctx.kernels_get())
+            ctx.close()
+    for fmt in (H.INV_SNAP_F32, H.INV_SNAP_I16):
+        (fa, ga, ka), (fb, gb, kb) = res[fmt, 20], res[fmt, 0]
+        for (ta, sa), (tb, sb) in zip(fa, fb):
+            assert np.array_equal(ta, tb) and np.array_equal(sa, sb)
+        assert ga == gb
+        for x, y in zip(ka, kb):
+            assert np.any(x)
+            if fmt == H.INV_SNAP_F32:
+                assert np.array_equal(x, y)
+            else:
+                _say(f"  I16 kernel difference between tiles: {rel(x, y):.2e}")
+                assert rel(x, y) < 1e-4

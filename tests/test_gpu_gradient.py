@@ -249,7 +249,7 @@ def test_graph_replay_equals_direct(gpu, variant):
         assert np.array_equal(sa, sb)
 
 
-@pytest.mark.parametrize("fmt", [0, 1, 2], ids=["f32", "f16", "bf16"])
+@pytest.mark.parametrize("fmt", [0, 1, 2, 3], ids=["f32", "f16", "bf16", "i16"])
 @pytest.mark.parametrize("nz", [40, 41], ids=["even-nz", "odd-nz"])
 def test_kernel_accumulation_matches_oracle(gpu, nz, fmt):
     """Adjoint kernels against the oracle's adjoint driven with the GPU's own
