@@ -9,8 +9,8 @@ before the timed region.  The workloads are defined in
 inv.cu_amd/problems.py (`workload`):
 
   c4 (default)  P-SV, Marmousi-shaped 4096x2048 at 5 m, 256 shots, 2048
-                receivers, dt 0.4 ms, nt 10000, envelope misfit, FP16
-                snapshots; a step is the gradient of one wave of 4 shots (one
+                receivers, dt 0.4 ms, nt 10000, envelope misfit, I16
+                snapshots (16-bit fixed point per row segment); a step is the gradient of one wave of 4 shots (one
                 launch keeps all of their snapshots in HBM, 268 GB), sigma 5.
                 value = Gcell-updates/s of the step; the 256-shot gradient is
                 64 such waves (gradients_per_hr_256_shots).
@@ -18,8 +18,9 @@ inv.cu_amd/problems.py (`workload`):
                 whole gradient is a step; the CPU oracle runs the same
                 gradient and the kernels are compared.
   c1            P-SV 737x240, 16 shots in one launch, nt 4000, FP32
-                snapshots; also times the one-shot-per-launch plain path.
-  c2            SH 1474x480, 64 shots in one launch, nt 8000, FP16
+                snapshots; also times the one-shot-per-launch plain path and
+                10 NLCG iterations through the command-line driver.
+  c2            SH 1474x480, 64 shots in one launch, nt 8000, I16
                 snapshots all resident (145 GB).
 
 With the default workload, c1 and c2 are also timed (2 gradients each) and
@@ -31,7 +32,7 @@ cpu_baseline (the CPU oracle on a bounded sample of the same workload, on the
 threads OMP_NUM_THREADS allows, plus a one-thread sample), accuracy (c2, c4:
 gradient error of each snapshot format against the FP64 oracle), serving
 (p50/p99 latency of the request server against the plain loop on mixed-size
-requests), plain (c1).
+requests), plain (c1), optimizer_c1 (c1's NLCG run).
 """
 from __future__ import annotations
 
@@ -220,7 +221,7 @@ def cpu_baseline(P, shots, nt=None, obs=None, sigma=None, threads=None):
 
 
 def accuracy(inv, name):
-    """Gradient and misfit error of FP32, FP16-snapshot and BF16-snapshot
+    """Gradient and misfit error of FP32, FP16-, BF16- and I16-snapshot
     storage (all arithmetic FP32) against the FP64 oracle: c2 on its own model
     and nt (2 shots), c4 on the 1024x512 / 20 m downsampled model (2 shots, dt
     2 ms, nt 2000).  The kernel error is the relative L2 over the solid cells
@@ -492,6 +493,51 @@ def plain_c1(inv, a):
             "note": "c1, one shot per launch, one step per launch, no graph"}
 
 
+def optimizer_c1(inv, a, iterations=10):
+    """c1's inversion: `iterations` NLCG iterations (Optimizer::run,
+    optimizer/optimizer.h:428-467: gradient of the 16 shots, direction, line
+    search of forward-only trials, model and kernel export) through the
+    command-line driver on a configuration directory written by
+    problems.write_config, timed by the driver's own clock, which excludes
+    setup and observed-data generation.  It inverts lambda only: the model's
+    water layer has mu = 0, and the reference updates mu without a bound, so
+    the first trial of a mu inversion makes mu negative there and the
+    wavefield diverges (as it would in the reference)."""
+    import re
+    import subprocess
+    import tempfile
+
+    P = inv.problems.workload("c1", sigma=a.sigma)[0]
+    exe = ROOT / "inv.cu_amd" / "inv.out"
+    num = r"(-?(?:nan|inf|[0-9.eE+-]+))"
+    with tempfile.TemporaryDirectory(prefix="inv_c1_") as tmp:
+        root = inv.problems.write_config(P.with_(inv_lambda=1, inv_mu=0, inv_rho=0),
+                                         Path(tmp) / "c1", mode=0, optimizer=0,
+                                         inv_iteration=iterations, batch=16)
+        t0 = time.perf_counter()
+        p = subprocess.run([str(exe), str(root)], capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+    if p.returncode != 0:
+        log(f"# c1 NLCG run failed: {p.stderr[-500:]}")
+        return None
+    m = re.search(r"Elapsed seconds: ([0-9.eE+-]+)\s+evaluations: (\d+)", p.stdout)
+    final = re.search(r"Final misfit: " + num, p.stdout)
+    restarts = p.stdout.count("restarting NLCG")
+    secs = float(m.group(1)) if m else None
+    rec = {"iterations": iterations,
+           "seconds": round(secs, 3) if m else None,
+           "seconds_per_iteration": round(secs / iterations, 3) if m else None,
+           "forward_evaluations": int(m.group(2)) if m else None,
+           "misfit_after": float(final.group(1)) if final else None,
+           "restarts": restarts,
+           "wall_s_including_setup": round(wall, 2),
+           "note": "inv.out mode 0, NLCG (Polak-Ribiere), lambda inverted, 16 shots per "
+                   f"launch, sigma {P.sigma}; misfit normalised to 1 at the start; an "
+                   "evaluation is a forward of every shot (a gradient counts 2)"}
+    log(f"# c1 NLCG run: {rec}")
+    return rec
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -512,9 +558,10 @@ def main():
     for w in [x for x in extras_names.split(",") if x and x != a.workload]:
         rec, _, _ = run_workload(inv, a, w, 2, 1, headline=False)
         extras[w] = rec
-    plain = None
+    plain = opt = None
     if not a.no_plain and (a.workload == "c1" or "c1" in extras):
         plain = plain_c1(inv, a)
+        opt = optimizer_c1(inv, a)
 
     cpu = parity = None
     if not a.no_cpu:
@@ -575,7 +622,8 @@ def main():
         if k in head:
             out[k] = head[k]
     out.update({"roofline": head["roofline"], "cpu_baseline": cpu, "plain": plain,
-                "serving": serving, "accuracy": acc, "extra_workloads": extras or None})
+                "serving": serving, "accuracy": acc, "extra_workloads": extras or None,
+                "optimizer_c1": opt})
     if parity:
         out["parity"] = parity
     print(json.dumps(out))

@@ -634,8 +634,10 @@ class Misfit {
     // launch; when adjoint wavefield exports are due (snapshot and wae > 0)
     // they run one at a time, as in the reference, so that the exports are
     // per shot.  The per-trace misfits of a shot are one launch
-    // (inv_misfit) rather than one call per trace.
-    double run(bool kernel = false, bool snapshot = false) {
+    // (inv_misfit) rather than one call per trace.  Virtual (the
+    // reference's is not) so that a misfit of another kind can drive the
+    // optimizers; code written against the reference is unaffected.
+    virtual double run(bool kernel = false, bool snapshot = false) {
         inv_ctx *ctx = solver->ctx;
         if (kernel) solver->initKernels();
         double misfit = 0;
@@ -1084,6 +1086,10 @@ class Optimizer {
         } else {
             std::cout << std::round(elapsed) << std::endl;
         }
+        // not in the reference: the same time in seconds, and the forward
+        // evaluations of the iterations (gradients count two)
+        std::cout << "Elapsed seconds: " << elapsed << "  evaluations: " << eval_count
+                  << std::endl;
     }
 
     // Optimizer::init (optimizer.h:468-510)

@@ -3,9 +3,12 @@
 Each generator returns a `Problem`: true and initial models on the grid
 (lambda, mu, rho, z fastest), point sources with Ricker wavelets from the
 library's own RickerSource (inv_source_ricker), receivers, and the solver
-flags.  The checkerboard follows the semantics of the reference's
-util/generate_checker.cu:4-39 (square cells of +-dv around a background); the
-layered/faulted generator is a Marmousi-shaped model of our own.
+flags.  `checker` is the adjacent-squares checkerboard of the 200x200
+gradient workload; `checker_gapped` follows the reference's
+util/generate_checker.cu:4-70 (squares separated by gaps, optional one-sided
+smoothing) and reproduces its example model; the layered/faulted generator
+is a Marmousi-shaped model of our own.  `write_config` writes any of them as a
+configuration directory for the command-line driver.
 """
 from __future__ import annotations
 
@@ -371,3 +374,139 @@ def shots(P, pick):
     return P.with_(src_ix=P.src_ix[pick], src_iz=P.src_iz[pick], stf_x=P.stf_x[pick],
                    stf_y=P.stf_y[pick], stf_z=P.stf_z[pick],
                    src_type=None if P.src_type is None else P.src_type[pick])
+
+
+# ---------------------------------------------------------------------------
+# the reference's file inputs: model points, config directories
+# ---------------------------------------------------------------------------
+def gauss_weights(n, sigma):
+    """The one-sided Gaussian weights of filter/gaussian.h:4-8 in FP32:
+    g(m) = exp(-m^2 / (2 sigma^2)) / (sqrt(2 pi) sigma) for m = 0 .. n-1 (the
+    reference's int - size_t argument wraps for negative m, whose weight is
+    then 0)."""
+    m = np.arange(n, dtype=np.float32)
+    s = np.float32(sigma)
+    pi = np.float32(3.1415927)
+    c = np.float32(1) / (np.sqrt(np.float32(2) * pi) * s)
+    return (c * np.exp(-m * m / (np.float32(2) * s * s))).astype(np.float32)
+
+
+def one_sided_gauss(field, nx, nz, sigma):
+    """GaussianFilter::apply (filter/gaussian.h:9-56) on a [nx][nz] field:
+    out(i, j) = sum over n <= i, m <= j of g(i-n) g(j-m) f(n, m), divided by
+    the product of the one-sided weight sums; FP64 sums of FP32 weights."""
+    f = np.asarray(field, np.float64).reshape(nx, nz)
+    w = gauss_weights(max(nx, nz), sigma).astype(np.float64)
+    cut = int(np.nonzero(w)[0].max()) + 1
+    tmp = np.zeros_like(f)
+    for m in range(min(cut, nx)):
+        tmp[m:, :] += w[m] * f[:nx - m, :]
+    out = np.zeros_like(f)
+    for m in range(min(cut, nz)):
+        out[:, m:] += w[m] * tmp[:, :nz - m]
+    sx = np.cumsum(w[:nx].astype(np.float32)).astype(np.float64)
+    sz = np.cumsum(w[:nz].astype(np.float32)).astype(np.float64)
+    return (out / (sx[:, None] * sz[None, :])).astype(np.float32).ravel()
+
+
+def checker_gapped(cx, cz, nx, nz, dx, dz, vp0, vs0, rho0, dvp, dvs, drho, sigma):
+    """The checkerboard of util/generate_checker.cu:4-39 (its model points and
+    optional smoothing, generate_checker.cu:44-70): squares of width
+    wx = dx nx 2/3 / (cx+1) (and wz likewise) separated by gaps of wx/2,
+    the first starting at wx, alternating +-(dvp, dvs, drho) around the
+    background; fields with a perturbation above 0.1 are smoothed with the
+    one-sided Gaussian of sigma cells.  Returns the point arrays x, z, vp,
+    vs, rho in the order of the reference's example files (x fastest)."""
+    f32 = np.float32
+    i = np.arange(nx)[:, None]
+    j = np.arange(nz)[None, :]
+    x = (i * f32(dx)).astype(f32) + np.zeros((1, nz), f32)
+    z = (j * f32(dz)).astype(f32) + np.zeros((nx, 1), f32)
+    # float arithmetic in the kernel's order; the division by 1.5 (a double
+    # literal) and the size_t conversion (0 for the negative values left of
+    # the first square)
+    wx = f32(f32(f32(f32(dx) * f32(nx)) * f32(2)) / f32(3)) / f32(cx + 1)
+    wz = f32(f32(f32(f32(dz) * f32(nz)) * f32(2)) / f32(3)) / f32(cz + 1)
+    tx = ((x - wx) / wx).astype(f32).astype(np.float64) / 1.5
+    tz = ((z - wz) / wz).astype(f32).astype(np.float64) / 1.5
+    idx = np.where(tx > 0, np.floor(tx), 0).astype(np.int64)
+    idz = np.where(tz > 0, np.floor(tz), 0).astype(np.int64)
+    rx = (x - wx).astype(f32).astype(np.float64) - (idx * wx).astype(f32).astype(np.float64) * 1.5
+    rz = (z - wz).astype(f32).astype(np.float64) - (idz * wz).astype(f32).astype(np.float64) * 1.5
+    inside = (rx > 0) & (rx < wx) & (rz > 0) & (rz < wz)
+    same = (idx % 2) == (idz % 2)
+
+    def field(v0, dv):
+        a = np.where(inside, np.where(same, f32(v0 + dv), f32(v0 - dv)), f32(v0)).astype(f32)
+        if sigma and dv > 0.1:
+            a = one_sided_gauss(a, nx, nz, sigma).reshape(nx, nz)
+        return a
+
+    out = [x, z, field(vp0, dvp), field(vs0, dvs), field(rho0, drho)]
+    # x fastest: flat index j * nx + i
+    return tuple(np.ascontiguousarray(a.T, f32).ravel() for a in out)
+
+
+def write_bin(path, a):
+    """`.bin` file: int32 value count, then float32 values (solver/solver.h:109-120)."""
+    a = np.ascontiguousarray(a, np.float32).ravel()
+    with open(path, "wb") as f:
+        f.write(np.int32(a.size).tobytes())
+        f.write(a.tobytes())
+
+
+CONFIG_DEFAULTS = dict(
+    mode=0, clean=1, nthread=0, solver=0, model_type=1, trace_type=0, wfe=0, wae=0,
+    filter=0, trace_file=0, optimizer=0, inv_iteration=1, inv_cycle=0, inv_sharpen=0.0,
+    nlcg_type=1, nlcg_thresh=0.0, lbfgs_mem=5, lbfgs_thresh=0.0, ls_step=10, ls_step_max=0.5,
+    ls_step_init=0.05, ls_thresh=1.2)
+
+
+def write_config(P, path, **over):
+    """A configuration directory the command-line driver (inv.out, inv.cu's
+    modes) runs: config.ini (the keys of lib/config.h, from the problem plus
+    CONFIG_DEFAULTS and `over`), sources.dat (x z type f0 t0 angle amp) and
+    stations.dat (x z) in metres (lib/config.h:45-84), and model_true /
+    model_init with proc000000_{x,z,lambda,mu,rho}.bin (model_type 1) on the
+    grid nodes, z fastest.  The driver derives the same nx, nz from the point
+    count and the extent (solver/fdm.h:356-389)."""
+    from pathlib import Path
+
+    root = Path(path)
+    root.mkdir(parents=True, exist_ok=True)
+    gx, gz, _, _ = invhip.grid_dims(P.nx * P.nz, (P.nx - 1) * float(P.dx),
+                                    (P.nz - 1) * float(P.dz))
+    if (gx, gz) != (P.nx, P.nz):
+        raise ValueError(f"the driver would derive a {gx}x{gz} grid, not {P.nx}x{P.nz}")
+    cfg = dict(CONFIG_DEFAULTS)
+    cfg.update(sh=P.sh, psv=P.psv, nt=P.nt, dt=float(P.dt), sfe=P.sfe,
+               abs_bottom=P.abs_bottom, abs_right=P.abs_right, abs_top=P.abs_top,
+               abs_left=P.abs_left, abs_width=P.abs_width, abs_alpha=P.abs_alpha,
+               misfit=P.misfit, filter=0 if P.sigma > 0 else 1, filter_sigma=P.sigma,
+               inv_lambda=P.inv_lambda, inv_mu=P.inv_mu, inv_rho=P.inv_rho)
+    cfg.update(over)
+    lines = ["[global]"] + [f"{k} = {float(v)!r}" if isinstance(v, (float, np.floating))
+                            else f"{k} = {int(v)}" for k, v in cfg.items()]
+    (root / "config.ini").write_text("\n".join(lines) + "\n")
+    meta = P.meta
+    types = P.src_type if P.src_type is not None else np.zeros(P.nsrc, np.int32)
+    rows = ["# x | z | type | f0 | t0 | angle | amp"]
+    num = lambda v: repr(float(v))
+    for s in range(P.nsrc):
+        rows.append(f"{num(P.src_ix[s] * float(P.dx))} {num(P.src_iz[s] * float(P.dz))} "
+                    f"{int(types[s])} {num(meta['f0'])} {num(meta['t0'])} 0.0 {num(meta['amp'])}")
+    (root / "sources.dat").write_text("\n".join(rows) + "\n")
+    rows = ["# x | z"] + [f"{num(P.rec_ix[r] * float(P.dx))} {num(P.rec_iz[r] * float(P.dz))}"
+                          for r in range(P.nrec)]
+    (root / "stations.dat").write_text("\n".join(rows) + "\n")
+    i = np.arange(P.nx)[:, None]
+    j = np.arange(P.nz)[None, :]
+    x = (i * np.float32(P.dx) + np.zeros((1, P.nz), np.float32)).astype(np.float32).ravel()
+    z = (j * np.float32(P.dz) + np.zeros((P.nx, 1), np.float32)).astype(np.float32).ravel()
+    for name, (lam, mu, rho) in (("model_true", (P.lam, P.mu, P.rho)),
+                                 ("model_init", (P.lam0, P.mu0, P.rho0))):
+        d = root / name
+        d.mkdir(exist_ok=True)
+        for c, a in (("x", x), ("z", z), ("lambda", lam), ("mu", mu), ("rho", rho)):
+            write_bin(d / f"proc000000_{c}.bin", a)
+    return root

@@ -29,7 +29,7 @@ import numpy as np
 import pytest
 
 import pyoracle
-from conftest import ROOT, rel
+from conftest import ROOT, inv, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -229,3 +229,34 @@ def test_mode0_inversion(tmp_path, gpu, optimizer):
     cos = -(dm @ g) / (np.linalg.norm(dm) * np.linalg.norm(g))
     print(f"optimizer {optimizer}: final misfit {final:.4f}, cos(dm, -g) {cos:.6f}")
     assert cos > 0.999
+
+
+def test_mode4_device_query(tmp_path, gpu):
+    """Mode 4 lists the devices (lib/query.h:23-65) and runs nothing else."""
+    (tmp_path / "q").mkdir()
+    (tmp_path / "q" / "config.ini").write_text("mode = 4\n")
+    out = run_cli(tmp_path / "q")
+    assert "HIP device(s) detected" in out and "gfx950" in out
+    assert "Global memory" in out or "global memory" in out
+
+
+def test_written_config_runs_the_library_gradient(tmp_path, gpu):
+    """A configuration directory written by problems.write_config for the
+    checkerboard workload (4 explosive shots, nt 400, sigma 5): mode 2
+    exports the kernels the library's own gradient gives for the same
+    problem, bit for bit (same batch, tile, math and graph settings)."""
+    P = inv.problems.workload("c0", nt=400)[0]
+    root = inv.problems.write_config(P, tmp_path / "c0", mode=2, batch=4)
+    print(f"c0 written config, {P.nx}x{P.nz} nt={P.nt}", flush=True)
+    run_cli(root)
+    got = [read_bin(root / "output" / f"proc000000_k_{c}.bin") for c in ("lambda", "mu", "rho")]
+    ctx = P.context(batch=4, model="true")
+    ctx.forward(0, 4, snapshots=False)
+    ctx.obs_from_traces(0, 4)
+    P.load_model(ctx, "init")
+    ctx.gradient(P.misfit, P.sigma)
+    want = ctx.kernels_get()
+    ctx.close()
+    for c, a, b in zip(("lambda", "mu", "rho"), got, want):
+        assert np.abs(b).max() > 0
+        assert np.array_equal(a, b), f"k_{c}: rel L2 {rel(a, b):.3e}"
