@@ -1087,7 +1087,8 @@ int inv_snapshots_get(inv_ctx *ctx, int slot, float *host) {
         std::vector<signed char> tab((size_t)c.nslots * tb);
         INV_HIP(hipMemcpy(q.data(), c.snap.as<char>() + slot * n * 2, n * 2, hipMemcpyDeviceToHost));
         INV_HIP(hipMemcpy(tab.data(), c.sexp.p, tab.size(), hipMemcpyDeviceToHost));
-        const int nseg = snap_nseg(c.g.nz);
+        const int nseg = snap_nseg(c.g.nz), nbb = snap_nbb(c.g.nz);
+        const long sblk = snap_blk_off(c.g.batch, c.nsnapf, c.g.nx, c.g.nz);
         for (int l = 0; l < c.nslots; l++) {
             const signed char *t = tab.data() + (size_t)l * tb;
             int segw;
@@ -1095,10 +1096,14 @@ int inv_snapshots_get(inv_ctx *ctx, int slot, float *host) {
             segw = segw > 0 ? segw : 1;
             for (int f = 0; f < c.nsnapf; f++) {
                 const size_t o = ((size_t)l * c.nsnapf + f) * c.N;
-                const signed char *rows = t + SNAPTAB_HDR + ((size_t)slot * c.nsnapf + f) * c.g.nx * nseg;
+                const size_t r0 = ((size_t)slot * c.nsnapf + f) * c.g.nx;
+                const signed char *rows = t + SNAPTAB_HDR + r0 * nseg;
+                const unsigned char *blks = reinterpret_cast<const unsigned char *>(t) + sblk +
+                                            r0 * nbb;
                 for (long k = 0; k < c.N; k++) {
                     const int i = (int)(k / c.g.nz), j = (int)(k % c.g.nz);
-                    const int e = rows[(size_t)i * nseg + std::min(j / segw, nseg - 1)];
+                    const int e = snap_cell_exp(rows + (size_t)i * nseg, blks + (size_t)i * nbb,
+                                                nseg, nbb, segw, j);
                     const uint16_t v = q[o + k];
                     float d;
                     if (c.snap_format == INV_SNAP_F16) {

@@ -255,17 +255,20 @@ __device__ __forceinline__ void stress_row(const Geo &g, const float *lv1, const
     }
 }
 
-// Segment exponents of one row for the 16-bit formats: the max-abs of the
-// fields over the lanes that own cells of the row (wave reductions; every
-// lane of the wavefront calls it), their exponents (snap_exp), stored by lane
-// 0 into the slot's table at segment seg, with the segment width segw in the
-// table header.  e[0], e[1]: u (two fields for P-SV); e[2], e[3]: v.
+// Exponents of one row for the 16-bit formats: the max-abs of the fields
+// over the lanes that own cells of the row (wave reductions; every lane of
+// the wavefront calls it) gives the segment exponents (snap_exp), stored by
+// lane 0 into the slot's table at segment seg, with the segment width segw in
+// the table header; the block offsets of the lane's block (snap_blk_put, oc:
+// the lane's first column in the segment) are stored alongside.  e returns
+// the lane's exponents, segment plus block offset: e[0], e[1] for u (two
+// fields for P-SV), e[2], e[3] for v.
 template <bool PSV, int V>
 __device__ __forceinline__ void row_exponents(const StepArgs &a, const StepRoles &ro, int fu,
                                               int fv, int i, int b, int seg, int segw, int lane,
-                                              bool act, const float (&u1)[V], const float (&u2)[V],
-                                              const float (&n1)[V], const float (&n2)[V],
-                                              int (&e)[4]) {
+                                              int oc, bool act, const float (&u1)[V],
+                                              const float (&u2)[V], const float (&n1)[V],
+                                              const float (&n2)[V], int (&e)[4]) {
     float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (act) {
 #pragma unroll
@@ -278,17 +281,23 @@ __device__ __forceinline__ void row_exponents(const StepArgs &a, const StepRoles
             }
         }
     }
+    const int nx = a.g.nx;
+    const int fld[4] = {fu, fu + 1, fv, fv + 1};
+    int es[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int q = 0; q < 4; q++) e[q] = (PSV || q % 2 == 0) ? snap_exp(wave_max(m[q])) : 0;
+    for (int q = 0; q < 4; q++) {
+        if (!PSV && q % 2) continue;
+        es[q] = snap_exp(wave_max(m[q]));
+        unsigned char *blk = reinterpret_cast<unsigned char *>(ro.sexp) +
+                             snaptab_blk(a.sblk, b, a.nsnapf, fld[q], nx, i, a.nbb) +
+                             seg * snap_seg_bytes(segw);
+        e[q] = es[q] + snap_blk_put<V>(m[q], es[q], lane, oc, act, segw, blk);
+    }
     if (lane == 0) {
         signed char *t = ro.sexp;
-        const int nx = a.g.nx;
-        t[snaptab_row(b, a.nsnapf, fu, nx, i, a.nseg) + seg] = (signed char)e[0];
-        t[snaptab_row(b, a.nsnapf, fv, nx, i, a.nseg) + seg] = (signed char)e[2];
-        if (PSV) {
-            t[snaptab_row(b, a.nsnapf, fu + 1, nx, i, a.nseg) + seg] = (signed char)e[1];
-            t[snaptab_row(b, a.nsnapf, fv + 1, nx, i, a.nseg) + seg] = (signed char)e[3];
-        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (PSV || q % 2 == 0) t[snaptab_row(b, a.nsnapf, fld[q], nx, i, a.nseg) + seg] = (signed char)es[q];
         *reinterpret_cast<int *>(t) = segw;
     }
 }
@@ -317,8 +326,9 @@ __device__ __forceinline__ void owned_row(const StepArgs &a, const StepRoles &ro
     const int fmt = a.snap_fmt;
     float *O1 = PSV ? a.out_x : a.out_y, *O2 = a.out_z;
     int e[4] = {0, 0, 0, 0};
-    if (ro.snap && fmt != INV_SNAP_F32)
-        row_exponents<PSV, V>(a, ro, fu, fv, i, b, seg, segw, lane, act, u1, u2, n1, n2, e);
+    if (ro.snap && fmt != INV_SNAP_F32)  // oc: the lane's first column in the segment
+        row_exponents<PSV, V>(a, ro, fu, fv, i, b, seg, segw, lane, jb - seg * segw, act, u1, u2,
+                              n1, n2, e);
     if (act) {
 #pragma unroll
         for (int v = 0; v < V; v++) {

@@ -50,9 +50,11 @@ struct IntArgs {
     const void *snap;  // slot-0 base of the snapshot slot being read
     long snap_slot_stride;  // elements per batch slot
     // 16-bit formats: the slot's exponent table (header: segment width, then
-    // [batch][nsnapf][nx][nseg] exponents)
+    // [batch][nsnapf][nx][nseg] exponents, then nbb block-offset bytes per
+    // [batch][nsnapf][nx] from byte sblk; layout in stencil.h)
     const signed char *sexp;
-    int nsnapf, nseg;
+    int nsnapf, nseg, nbb;
+    long sblk;
     float *kl, *km, *kr;
     long k_slot_stride;  // 0: every slot accumulates into the same kernels
     float ndt;
@@ -221,8 +223,10 @@ struct SnapField {
     __device__ static type at(const IntArgs &a, int b, int f) {
         const unsigned short *q =
             static_cast<const unsigned short *>(a.snap) + (long)b * a.snap_slot_stride + f * a.g.N;
-        return type{q, a.sexp + snaptab_row(b, a.nsnapf, f, a.g.nx, 0, a.nseg), a.g.nz, a.nseg,
-                    snap_segw(a)};
+        return type{q, a.sexp + snaptab_row(b, a.nsnapf, f, a.g.nx, 0, a.nseg),
+                    reinterpret_cast<const unsigned char *>(a.sexp) +
+                        snaptab_blk(a.sblk, b, a.nsnapf, f, a.g.nx, 0, a.nbb),
+                    a.g.nz, a.nseg, a.nbb, snap_segw(a)};
     }
 };
 template <>
@@ -298,10 +302,26 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_interact(const IntArgs a) {
 // on the shared accumulators), so the kernels agree bit for bit.
 constexpr int ICOLS = 124;  // columns a wavefront owns
 
-// the two snapshot values of field f at cells k, k+1 (row i, segment seg)
+// where a column's exponent lives in a row of a slot's table: segment seg,
+// block-offset byte by, nibble shift sh (as snap_cell_exp)
+struct SnapLoc {
+    int seg, by, sh;
+};
+__device__ __forceinline__ SnapLoc snap_loc(const IntArgs &a, int j) {
+    const int segw = snap_segw(a);
+    int sg = j / segw;
+    sg = sg < a.nseg - 1 ? sg : a.nseg - 1;
+    const int lb = (j - sg * segw) >> 2;
+    int by = sg * snap_seg_bytes(segw) + (lb >> 1);
+    by = by < a.nbb - 1 ? by : a.nbb - 1;
+    return SnapLoc{sg, by, (lb & 1) << 2};
+}
+
+// the two snapshot values of field f at cells k, k+1 (row i; their column's
+// place in the table: sl)
 template <int FMT>
-__device__ __forceinline__ void snap_pair(const IntArgs &a, int b, int f, long k, int i, int seg,
-                                          float &x0, float &x1) {
+__device__ __forceinline__ void snap_pair(const IntArgs &a, int b, int f, long k, int i,
+                                          const SnapLoc &sl, float &x0, float &x1) {
     if constexpr (FMT == INV_SNAP_F32) {
         const float *q = static_cast<const float *>(a.snap) + (long)b * a.snap_slot_stride +
                          (long)f * a.g.N;
@@ -311,7 +331,10 @@ __device__ __forceinline__ void snap_pair(const IntArgs &a, int b, int f, long k
     } else {
         const unsigned short *q = static_cast<const unsigned short *>(a.snap) +
                                   (long)b * a.snap_slot_stride + (long)f * a.g.N;
-        const int e = a.sexp[snaptab_row(b, a.nsnapf, f, a.g.nx, i, a.nseg) + seg];
+        const unsigned char *bt = reinterpret_cast<const unsigned char *>(a.sexp);
+        const int e = a.sexp[snaptab_row(b, a.nsnapf, f, a.g.nx, i, a.nseg) + sl.seg] +
+                      ((bt[snaptab_blk(a.sblk, b, a.nsnapf, f, a.g.nx, i, a.nbb) + sl.by] >> sl.sh) &
+                       15);
         const unsigned w = *reinterpret_cast<const unsigned *>(q + k);
         x0 = snap_dec16((unsigned short)(w & 0xffffu), e, FMT);
         x1 = snap_dec16((unsigned short)(w >> 16), e, FMT);
@@ -347,12 +370,10 @@ __global__ __launch_bounds__(256) void k_interact_col(const IntArgs a, int tiles
     const long so = (long)b * g.N;
     auto rowc = [&](int i) { return i < 0 ? 0 : (i > nx - 1 ? nx - 1 : i); };
     auto row = [&](int i) { return (long)rowc(i) * nz + jl; };
-    // the lane's snapshot segment (columns jl, jl+1 share it: widths are even)
-    int seg = 0;
-    if constexpr (FMT != INV_SNAP_F32) {
-        seg = jl / snap_segw(a);
-        seg = seg < a.nseg - 1 ? seg : a.nseg - 1;
-    }
+    // the lane's snapshot segment and block (columns jl, jl+1 share both:
+    // segment widths are even and blocks start on even columns)
+    SnapLoc sl{0, 0, 0};
+    if constexpr (FMT != INV_SNAP_F32) sl = snap_loc(a, jl);
     float *kl = a.kl + (long)b * a.k_slot_stride;
     float *km = a.km + (long)b * a.k_slot_stride;
     float *kr = a.kr + (long)b * a.k_slot_stride;
@@ -368,7 +389,7 @@ __global__ __launch_bounds__(256) void k_interact_col(const IntArgs a, int tiles
             const float2 t = *reinterpret_cast<const float2 *>(a.uy + so + k);
             ay[r][0] = t.x;
             ay[r][1] = t.y;
-            snap_pair<FMT>(a, b, 0, k, ic, seg, fy[r][0], fy[r][1]);
+            snap_pair<FMT>(a, b, 0, k, ic, sl, fy[r][0], fy[r][1]);
         }
         if (PSV) {
             const float2 t = *reinterpret_cast<const float2 *>(a.ux + so + k);
@@ -377,8 +398,8 @@ __global__ __launch_bounds__(256) void k_interact_col(const IntArgs a, int tiles
             ax[r][1] = t.y;
             az[r][0] = u.x;
             az[r][1] = u.y;
-            snap_pair<FMT>(a, b, f0, k, ic, seg, fx[r][0], fx[r][1]);
-            snap_pair<FMT>(a, b, f0 + 1, k, ic, seg, fz[r][0], fz[r][1]);
+            snap_pair<FMT>(a, b, f0, k, ic, sl, fx[r][0], fx[r][1]);
+            snap_pair<FMT>(a, b, f0 + 1, k, ic, sl, fz[r][0], fz[r][1]);
         }
     }
 #pragma unroll
@@ -412,7 +433,7 @@ __global__ __launch_bounds__(256) void k_interact_col(const IntArgs a, int tiles
             dz_pair<FAST>(fy[t + 1][0], fy[t + 1][1], g, dzf[0], dzf[1]);
             if (own) {
                 float vf[2];
-                snap_pair<FMT>(a, b, 1, k, i, seg, vf[0], vf[1]);
+                snap_pair<FMT>(a, b, 1, k, i, sl, vf[0], vf[1]);
                 const float2 vv = *reinterpret_cast<const float2 *>(a.vy + so + k);
                 const float vy[2] = {vv.x, vv.y};
 #pragma unroll
@@ -439,8 +460,8 @@ __global__ __launch_bounds__(256) void k_interact_col(const IntArgs a, int tiles
             dz_pair<FAST>(fz[t + 1][0], fz[t + 1][1], g, dzzf[0], dzzf[1]);
             if (own) {
                 float vfx[2], vfz[2];
-                snap_pair<FMT>(a, b, f0 + 2, k, i, seg, vfx[0], vfx[1]);
-                snap_pair<FMT>(a, b, f0 + 3, k, i, seg, vfz[0], vfz[1]);
+                snap_pair<FMT>(a, b, f0 + 2, k, i, sl, vfx[0], vfx[1]);
+                snap_pair<FMT>(a, b, f0 + 3, k, i, sl, vfz[0], vfz[1]);
                 const float2 vxx = *reinterpret_cast<const float2 *>(a.vx + so + k);
                 const float2 vzz = *reinterpret_cast<const float2 *>(a.vz + so + k);
                 const float vx[2] = {vxx.x, vxx.y}, vz[2] = {vzz.x, vzz.y};
@@ -913,6 +934,8 @@ void fdm_interact(Ctx &c, int count, int it, int slot, const float *vx, const fl
                   : nullptr;
     ia.nsnapf = c.nsnapf;
     ia.nseg = snap_nseg(c.g.nz);
+    ia.nbb = snap_nbb(c.g.nz);
+    ia.sblk = snap_blk_off(c.g.batch, c.nsnapf, c.g.nx, c.g.nz);
     ia.ndt = (float)c.g.sfe * c.g.dt;  // sfe * dt, fdm.h:585
     ia.inv_lambda = c.g.inv_lambda;
     ia.inv_mu = c.g.inv_mu;

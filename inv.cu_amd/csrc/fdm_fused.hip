@@ -277,6 +277,7 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
 // row wrap-around inside a wavefront.
 // ---------------------------------------------------------------------------
 constexpr int CZ = 58;
+static_assert(CZ == SNAP_SEG_WIDTHS[0], "snapshot tables are sized for this segment width");
 
 template <bool SH, bool PSV, int TX, bool FAST>
 __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
@@ -403,10 +404,20 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
     const int fu = PSV ? a.psv_f0 : 0, fv = PSV ? a.psv_f0 + 2 : 1;
     const long sb = (long)b * a.snap_slot_stride;
     // 16-bit snapshots: one exponent per field and row over the tile's CZ
-    // owned columns (segment tz of the slot's table)
+    // owned columns (segment tz of the slot's table, stored by lane 0), plus
+    // the offset of the lane's block of 4 columns (lane l owns column l - 1
+    // of the segment).  mag: |x| at an owned cell, else 0.  Every lane calls it.
     const bool seg16 = a.snap && fmt != INV_SNAP_F32;
-    auto put_exp = [&](int f, int i, int e) {
-        a.sexp[snaptab_row(b, a.nsnapf, f, nx, i, a.nseg) + tz] = (signed char)e;
+    auto cell_exp = [&](int f, int i, float mag, bool own) {
+        const int es = snap_exp(wave_max(mag));
+        if (lane == 0) {
+            a.sexp[snaptab_row(b, a.nsnapf, f, nx, i, a.nseg) + tz] = (signed char)es;
+            *reinterpret_cast<int *>(a.sexp) = CZ;
+        }
+        unsigned char *blk = reinterpret_cast<unsigned char *>(a.sexp) +
+                             snaptab_blk(a.sblk, b, a.nsnapf, f, nx, i, a.nbb) +
+                             tz * snap_seg_bytes(CZ);
+        return es + snap_blk_put<1>(mag, es, lane, lane - 1, own, CZ, blk);
     };
 
     // 2. velocity of step n+1, rows i0-1 .. i0+TX+1, columns j0-1 .. j0+CZ+1
@@ -454,13 +465,8 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                 float u = qu1[p];
                 int eu = 0, ev = 0;
                 if (seg16 && vr >= 1 && vr <= TX) {  // wave-uniform: every lane reduces
-                    eu = snap_exp(wave_max(own ? fabsf(u) : 0.0f));
-                    ev = snap_exp(wave_max(own ? fabsf(v) : 0.0f));
-                    if (lane == 0) {
-                        put_exp(0, i, eu);
-                        put_exp(1, i, ev);
-                        *reinterpret_cast<int *>(a.sexp) = CZ;
-                    }
+                    eu = cell_exp(0, i, own ? fabsf(u) : 0.0f, own);
+                    ev = cell_exp(1, i, own ? fabsf(v) : 0.0f, own);
                 }
                 if (own) {
                     if (a.snap) snap_put(a.snap, fmt, sb + kc, u, eu);
@@ -501,17 +507,10 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                 float ux = qu1[p], uz = qu2[p];
                 int ex = 0, ez = 0, evx = 0, evz = 0;
                 if (seg16 && vr >= 1 && vr <= TX) {  // wave-uniform: every lane reduces
-                    ex = snap_exp(wave_max(own ? fabsf(ux) : 0.0f));
-                    ez = snap_exp(wave_max(own ? fabsf(uz) : 0.0f));
-                    evx = snap_exp(wave_max(own ? fabsf(vx) : 0.0f));
-                    evz = snap_exp(wave_max(own ? fabsf(vz) : 0.0f));
-                    if (lane == 0) {
-                        put_exp(fu, i, ex);
-                        put_exp(fu + 1, i, ez);
-                        put_exp(fv, i, evx);
-                        put_exp(fv + 1, i, evz);
-                        *reinterpret_cast<int *>(a.sexp) = CZ;
-                    }
+                    ex = cell_exp(fu, i, own ? fabsf(ux) : 0.0f, own);
+                    ez = cell_exp(fu + 1, i, own ? fabsf(uz) : 0.0f, own);
+                    evx = cell_exp(fv, i, own ? fabsf(vx) : 0.0f, own);
+                    evz = cell_exp(fv + 1, i, own ? fabsf(vz) : 0.0f, own);
                 }
                 if (own) {
                     if (a.snap) {
@@ -1068,6 +1067,8 @@ static void common_args(Ctx &c, StepArgs &a) {
     a.psv_f0 = c.g.sh ? 2 : 0;
     a.snap_slot_stride = (long)c.nslots * c.nsnapf * c.N;
     a.nseg = snap_nseg(c.g.nz);
+    a.nbb = snap_nbb(c.g.nz);
+    a.sblk = snap_blk_off(c.g.batch, c.nsnapf, c.g.nx, c.g.nz);
     INV_REQUIRE(a.rho && a.lam && a.mu && a.abx && a.abzb && a.abzt, INV_ERR_STATE,
                 "model or absorbing profile not allocated");
     INV_REQUIRE((!c.g.psv || (c.ux.p && c.uz.p && c.vx.p && c.vx_b.p && c.sxx_b.p)) &&

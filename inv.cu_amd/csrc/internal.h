@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <map>
@@ -288,11 +289,25 @@ double vec_dot(Ctx &c, const float *a, const float *b, size_t n);
 float vec_amax(Ctx &c, const float *a, size_t n);
 void sum_f64(Ctx &c, const double *x, size_t n, double *dev_out);
 
-// Exponent tables of 16-bit snapshots (stencil.h): segments per row (the
-// narrowest tile that writes snapshots owns 58 columns) and bytes per slot.
+// Exponent tables of 16-bit snapshots (stencil.h).  A row segment is the
+// owned columns of one tile of the kernel that writes the slot: 58 (k_col),
+// 120 / 248 (k_colv, 2 / 4 columns per lane), 116 / 240 (k_pair, 2 / 4).
+constexpr int SNAP_SEG_WIDTHS[5] = {58, 116, 120, 240, 248};
+// segments per row (the narrowest segment sets the count)
 inline int snap_nseg(int nz) { return (nz + 57) / 58 + 1; }
+// block-offset bytes per row: enough for every segment width
+inline int snap_nbb(int nz) {
+    int m = 1;
+    for (int w : SNAP_SEG_WIDTHS) m = std::max(m, (nz + w - 1) / w * snap_seg_bytes(w));
+    return m;
+}
+// byte offset of the block-offset section in a slot's table
+inline long snap_blk_off(int batch, int nsnapf, int nx, int nz) {
+    return SNAPTAB_HDR + (long)batch * nsnapf * nx * snap_nseg(nz);
+}
 inline size_t snap_tab_bytes(int batch, int nsnapf, int nx, int nz) {
-    const size_t b = SNAPTAB_HDR + (size_t)batch * nsnapf * nx * snap_nseg(nz);
+    const size_t b = (size_t)snap_blk_off(batch, nsnapf, nx, nz) +
+                     (size_t)batch * nsnapf * nx * snap_nbb(nz);
     return (b + 255) / 256 * 256;
 }
 

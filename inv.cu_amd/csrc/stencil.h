@@ -58,13 +58,20 @@ __device__ __forceinline__ float d_plus(const float *__restrict__ f, long k, lon
 //   FP16 / BF16  the nearest FP16 / BF16 value (round to nearest even),
 //   I16          the nearest integer, |q| <= 32767 (16-bit fixed point
 //                relative to the segment: 15 bits below its max-abs),
-// and decode by multiplying with 2^-e.  The exponents of a slot live in its
-// exponent table: a SNAPTAB_HDR-byte header holding the segment width (int)
-// of the writing kernel, then one signed byte per [batch][field][row][seg].
-// All arithmetic stays FP32.
+// and decode by multiplying with 2^-e.  Within a segment, each block of 4
+// cells (counted from the segment's first column) refines the exponent by a
+// 4-bit offset d in [0, 15], the one that brings the block's own max-abs
+// toward [2^14, 2^15) (snap_blk_d): its cells are scaled by 2^(e + d).  The
+// exponents of a slot live in its exponent table: a SNAPTAB_HDR-byte header
+// holding the segment width (int) of the writing kernel, then one signed
+// byte per [batch][field][row][seg], then the block offsets, nbb bytes per
+// [batch][field][row]: two per byte, low nibble first, each segment starting
+// on a byte (segment sg's block lb is nibble lb & 1 of byte
+// sg * snap_seg_bytes(segw) + lb / 2 of its row).  All arithmetic stays FP32.
 // ---------------------------------------------------------------------------
 constexpr int SNAP_EMAX = 120;
 constexpr int SNAPTAB_HDR = 64;
+constexpr int SNAP_DMAX = 15;
 
 // 2^e as a float, e in [-126, 127]
 __device__ __host__ __forceinline__ float pow2f(int e) {
@@ -88,7 +95,33 @@ __device__ __host__ __forceinline__ int snap_exp(float m) {
     return e;
 }
 
-// one value of a 16-bit format: x 2^e encoded (e: the segment's exponent)
+// block-offset bytes of one row segment of width segw
+__device__ __host__ __forceinline__ int snap_seg_bytes(int segw) { return (segw + 7) >> 3; }
+
+// offset d of a block of max-abs bm in a segment of exponent e: the block's
+// own exponent minus e, in [0, SNAP_DMAX] (0 for an all-zero block)
+__device__ __host__ __forceinline__ int snap_blk_d(float bm, int e) {
+    if (!(bm > 0.0f)) return 0;
+    const int d = snap_exp(bm) - e;
+    return d < 0 ? 0 : (d > SNAP_DMAX ? SNAP_DMAX : d);
+}
+
+// the exponent of column j of a row: erow, the row's segment exponents
+// (nseg of them); brow, its nbb block-offset bytes; segw, the writer's
+// segment width.  Indexes are clamped to the row, so a table whose header
+// holds no valid width cannot be read out of bounds.
+__device__ __host__ __forceinline__ int snap_cell_exp(const signed char *erow,
+                                                      const unsigned char *brow, int nseg,
+                                                      int nbb, int segw, int j) {
+    int sg = j / segw;
+    sg = sg < nseg - 1 ? sg : nseg - 1;
+    const int lb = (j - sg * segw) >> 2;
+    int by = sg * snap_seg_bytes(segw) + (lb >> 1);
+    by = by < nbb - 1 ? by : nbb - 1;
+    return erow[sg] + ((brow[by] >> ((lb & 1) << 2)) & 15);
+}
+
+// one value of a 16-bit format: x 2^e encoded (e: its block's exponent)
 __device__ __forceinline__ unsigned short snap_enc16(float x, int e, int fmt) {
     float y = x * pow2f(e);
     // keep the product a separate FP32 value (exact here) before the
@@ -121,17 +154,27 @@ __device__ __forceinline__ long snaptab_row(int b, int nsnapf, int f, int nx, in
     return SNAPTAB_HDR + (((long)b * nsnapf + f) * nx + i) * nseg;
 }
 
+// the block offsets of the same row (sblk: byte offset of the block-offset
+// section in the slot's table, nbb: bytes per row)
+__device__ __forceinline__ long snaptab_blk(long sblk, int b, int nsnapf, int f, int nx, int i,
+                                            int nbb) {
+    return sblk + (((long)b * nsnapf + f) * nx + i) * nbb;
+}
+
 // read access to one 16-bit snapshot field, decoded to FP32 (k a cell index;
-// its exponent is that of its row segment)
+// its exponent is that of its row segment plus its block's offset)
 template <int FMT>
 struct Snap16 {
     const unsigned short *q;
-    const signed char *tab;  // the field's rows: tab[i * nseg + seg]
-    int nz, nseg, segw;
+    const signed char *tab;    // the field's rows: tab[i * nseg + seg]
+    const unsigned char *blk;  // the field's block offsets: blk[i * nbb + byte]
+    int nz, nseg, nbb, segw;
     __device__ __forceinline__ float operator[](long k) const {
         const int i = (int)(k / nz), j = (int)(k - (long)i * nz);
-        const int sg = j / segw;
-        return snap_dec16(q[k], tab[(long)i * nseg + (sg < nseg - 1 ? sg : nseg - 1)], FMT);
+        return snap_dec16(q[k],
+                          snap_cell_exp(tab + (long)i * nseg, blk + (long)i * nbb, nseg, nbb,
+                                        segw, j),
+                          FMT);
     }
 };
 

@@ -35,12 +35,15 @@ struct StepArgs {
     // snap: slot base (batch slot 0) that receives this step's input u and
     // its result v, or null.  16-bit formats scale each row segment of a
     // field (the owned columns of one tile row) by a power of two chosen
-    // from the segment's max-abs (snap_exp) and store its exponent in sexp,
-    // the slot's exponent table (SnapTab layout); nseg: segments per row in
-    // the table.  The linear kernel supports FP32 only.
+    // from the segment's max-abs (snap_exp), refined per block of 4 cells,
+    // and store the exponents in sexp, the slot's exponent table (layout in
+    // stencil.h); nseg: segments per row, sblk: byte offset of the block
+    // offsets in the table, nbb: their bytes per row.  The linear kernel
+    // supports FP32 only.
     void *snap;
     signed char *sexp;
-    int snap_fmt, nsnapf, psv_f0, nseg;
+    int snap_fmt, nsnapf, psv_f0, nseg, nbb;
+    long sblk;
     long snap_slot_stride;  // elements per batch slot
 };
 
@@ -48,6 +51,34 @@ __device__ __forceinline__ float wave_max(float x) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
     return x;
+}
+
+// Block offsets of one field of a row segment (stencil.h): the lane holds V
+// consecutive columns from owned column oc of the segment (oc a multiple of
+// V; a block of 4 columns spans 4 / V lanes when V < 4); m is the max-abs
+// of its cells (0 for a lane that owns none), e the segment's exponent.
+// Returns the offset d of the lane's block; the first lane of each
+// even-numbered block stores the byte of that block and the next one into
+// seg_blk, the segment's block-offset bytes (own: the lane owns its cells).
+// Every lane of the wavefront calls it.
+template <int V>
+__device__ __forceinline__ int snap_blk_put(float m, int e, int lane, int oc, bool own, int segw,
+                                            unsigned char *seg_blk) {
+    constexpr int LPB = V >= 4 ? 1 : 4 / V;  // lanes per block
+    float bm = m;
+    if (LPB > 1) {
+        const int lead = lane - ((oc & 3) / V);
+#pragma unroll
+        for (int k = 0; k < LPB; k++) bm = fmaxf(bm, __shfl(m, lead + k));
+    }
+    const int d = snap_blk_d(bm, e);
+    const int dn = __shfl(d, lane + LPB);  // the next block's offset, from its first lane
+    if (own && (oc & 7) == 0) {
+        const int lb = oc >> 2;
+        const int hi = (lb + 1) * 4 < segw ? dn : 0;
+        seg_blk[lb >> 1] = (unsigned char)(d | (hi << 4));
+    }
+    return d;
 }
 
 // Launch the wide column kernel (fdm_wide.hip): tile_x rows, c.col_v = 2 or
@@ -71,6 +102,9 @@ struct StepRoles {
 // halo of 6 rows and 6 columns (8 with V = 4, for alignment) each side.
 constexpr int pair_halo(int V) { return V == 2 ? 6 : 8; }
 constexpr int pair_strip(int V) { return 64 * V - 2 * pair_halo(V); }
+static_assert(col_strip(2) == SNAP_SEG_WIDTHS[2] && col_strip(4) == SNAP_SEG_WIDTHS[4] &&
+                  pair_strip(2) == SNAP_SEG_WIDTHS[1] && pair_strip(4) == SNAP_SEG_WIDTHS[3],
+              "snapshot tables are sized for these segment widths");
 // LDS bytes of a two-step tile: stress of steps n / n+1 on rows+12 rows
 // (szz or szy on rows+9: they enter only through z-differences) and
 // velocity of steps n+1 / n+2 on rows+9 rows, 64 V columns each.

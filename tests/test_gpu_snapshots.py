@@ -4,8 +4,9 @@ snapshots of every sfe-th step (solver/fdm.h:521-560); the checks are:
 
   * 16-bit snapshots decode to the FP32 snapshots within the rounding bound
     of the format (FP16: 2^-11 |x|, BF16: 2^-8 |x|, plus FP16's subnormal floor
-    2^-38 of the field's max-abs; I16: 2^-15 of the row's max-abs, a bound on
-    the max-abs of the row segment each value is scaled by), slot by slot and
+    2^-38 of the field's max-abs; I16: 2^-15 of the max-abs of the value's
+    block of 4 cells, bounded by that of the 7 cells around it, or 2^-30 of
+    its row's max-abs where the block offset is clamped), slot by slot and
     field by field;
   * gradients from 16-bit snapshots stay within a stated error of the FP64
     oracle (reported per format);
@@ -65,11 +66,17 @@ def test_snapshot_decode_bound(gpu, fmt, sfe):
         x, y = ref[b].astype(np.float64), got[b].astype(np.float64)
         m = np.abs(x).max(axis=2, keepdims=True)
         if fmt == "i16":
-            # [slots][fields][nx][nz]: the max-abs of each row
-            rows = np.abs(x).reshape(x.shape[0], x.shape[1], P.nx, P.nz).max(axis=3)
+            # [slots][fields][nx][nz]: a value's step is set by its block of
+            # 4 cells (inside the 7 cells around it) unless the block's offset
+            # is clamped at 15 below its row segment (inside its row)
+            a4 = np.abs(x).reshape(x.shape[0], x.shape[1], P.nx, P.nz)
+            pad = np.pad(a4, ((0, 0), (0, 0), (0, 0), (3, 3)))
+            win = np.max(np.stack([pad[..., s:s + P.nz] for s in range(7)]), axis=0)
+            rows = a4.max(axis=3, keepdims=True)
             # plus half a step of the smallest scale (exponents are clamped
-            # to 120: rows whose max-abs is below 2^-105)
-            bound = np.repeat(rows, P.nz, axis=2) * 2.0 ** -15 * (1 + 1e-4) + 2.0 ** -121
+            # to 120: blocks whose max-abs is below 2^-105)
+            bound = ((win + rows * 2.0 ** -15) * 2.0 ** -15 * (1 + 1e-4)).reshape(x.shape) \
+                + 2.0 ** -121
         else:
             eps = 2.0 ** -11 if fmt == "f16" else 2.0 ** -8
             bound = eps * np.abs(x) * (1 + 1e-6) + m * 2.0 ** -38
