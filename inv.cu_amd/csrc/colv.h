@@ -25,6 +25,30 @@ __device__ __forceinline__ void vload(const float *__restrict__ p, float (&o)[V]
     }
 }
 
+// Loads through a pointer the compiler cannot place (one read from memory,
+// such as a per-slot model pointer) are flat loads; flat loads count on the
+// same counter as LDS, so every wait for an LDS read would wait for them too.
+// Global memory is what every such pointer addresses.
+// (Native vector types: a copy of HIP's float2 class would pass through a
+// generic reference and lose the address space.)
+using gfloat = __attribute__((address_space(1))) const float;
+template <int V>
+__device__ __forceinline__ void vload_g(const float *p, float (&o)[V]) {
+    typedef float vec __attribute__((ext_vector_type(V)));
+    const vec t = *(const __attribute__((address_space(1))) vec *)p;
+#pragma unroll
+    for (int v = 0; v < V; v++) o[v] = t[v];
+}
+
+// A wave-uniform value read with a vector load, into a vector register: a
+// scalar load would be waited for together with the LDS reads that follow
+// (both count on one counter) in every row that uses it.
+__device__ __forceinline__ float vload_uniform(const float *p, int k) {
+    int kv;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(kv) : "s"(k));
+    return *(const gfloat *)(p + kv);
+}
+
 template <int V>
 __device__ __forceinline__ void vstore(float *__restrict__ p, const float (&o)[V]) {
     if constexpr (V == 2) *reinterpret_cast<float2 *>(p) = make_float2(o[0], o[1]);
@@ -92,6 +116,32 @@ __device__ __forceinline__ void scan_pts(const LanePts &t, const int *cell, cons
     }
 }
 
+// Fast mode drops a difference outside its range through its coefficients
+// instead of a select (dm / dp give 0 there; every operand is finite):
+// those of x per row (scalars), those of z per column of the lane, set once
+// per kernel here.  Cells outside the grid are then left as computed, not
+// zeroed: no cell inside depends on them except through a dropped
+// difference.  Exact mode keeps the reference's conditionals.
+template <int V>
+struct ZCoef {
+    float v1[V], v2[V];  // velocity (D-, 2 <= j < nz - 2)
+    float s1[V], s2[V];  // stress (D+, 1 <= j < nz - 2)
+};
+template <int V>
+__device__ __forceinline__ ZCoef<V> zcoef(const Geo &g, int jb) {
+    ZCoef<V> z;
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        const int j = jb + v;
+        const bool zv = j >= 2 && j < g.nz - 2, zs = j >= 1 && j < g.nz - 2;
+        z.v1[v] = zv ? g.a1z : 0.0f;
+        z.v2[v] = zv ? g.a2z : 0.0f;
+        z.s1[v] = zs ? g.a1z : 0.0f;
+        z.s2[v] = zs ? g.a2z : 0.0f;
+    }
+    return z;
+}
+
 // Velocity of step it+1 (divS, addSTF, updateV) on the lane's V columns
 // jb .. jb+V-1 of grid row i.  ls1/ls2/ls3: stress of step it in LDS rows of
 // W columns (sxx, szz, sxz for P-SV; sxy, szy for SH), grid row i at LDS row
@@ -100,11 +150,12 @@ __device__ __forceinline__ void scan_pts(const LanePts &t, const int *cell, cons
 // or dt/rho (fast math).  Point forces of step it from the list entries
 // p0 .. p1 whose cell lies in grid cells rlo .. rhi (this row's span of the
 // tile, held per lane in ip) are added in list order after the derivatives.
-// Cells outside the grid (rin false) give 0.
+// Cells outside the grid (rin false) give 0 in exact mode; zc: fast mode's z
+// coefficients.
 template <bool PSV, int V, int W, bool FAST>
 __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, const float *ls2,
                                         const float *ls3, int r, int r2, int c0, int i, int jb,
-                                        bool rin,
+                                        bool rin, const ZCoef<V> &zc,
                                         float abx, const float (&abz)[V], const float (&abt)[V],
                                         const float (&v1)[V], const float (&v2)[V],
                                         const float (&rho)[V], const LanePts &ip, int rlo,
@@ -113,6 +164,7 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
     const Geo &g = a.g;
     const int nx = g.nx, nz = g.nz;
     const bool xin = i >= 2 && i < nx - 2;
+    const float cx1 = !FAST || xin ? g.a1x : 0.0f, cx2 = !FAST || xin ? g.a2x : 0.0f;
     const float *I1 = PSV ? a.inj_x : a.inj_y, *I2 = a.inj_z;
     if (!PSV) {
         float x0[V], x1[V], x2[V], x3[V], ez[3 * V];
@@ -129,11 +181,16 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const int j = jb + v;
-            const float tx = dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+            const float tx = dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, cx1, cx2);
             const float tz = dm<FAST>(ez[V + v], ez[V + v - 1], ez[V + v + 1], ez[V + v - 2],
-                                      g.c8dz, g.c24dz, g.a1z, g.a2z);
-            d[v] = xin ? tx : 0.0f;
-            d[v] = (j >= 2 && j < nz - 2) ? d[v] + tz : d[v];
+                                      g.c8dz, g.c24dz, FAST ? zc.v1[v] : g.a1z,
+                                      FAST ? zc.v2[v] : g.a2z);
+            if (FAST) {
+                d[v] = tx + tz;
+            } else {
+                d[v] = xin ? tx : 0.0f;
+                d[v] = (j >= 2 && j < nz - 2) ? d[v] + tz : d[v];
+            }
         }
         scan_pts(ip, a.inj_cell, a.inj_row, rlo, rhi, [&](int cp, int row) {
             const int e = cp - (i * nz + jb);
@@ -146,7 +203,8 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const float ab = (abx * abz[v]) * abt[v];
-            n1[v] = rin ? vel_update<FAST>(ab, v1[v], g.dt, d[v], rho[v]) : 0.0f;
+            const float n = vel_update<FAST>(ab, v1[v], g.dt, d[v], rho[v]);
+            n1[v] = FAST || rin ? n : 0.0f;
         }
     } else {
         float x0[V], x1[V], x2[V], x3[V], z0[V], z1[V], z2[V], z3[V], e3[3 * V], e2[3 * V];
@@ -165,16 +223,22 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
         for (int v = 0; v < V; v++) {
             const int j = jb + v;
             const bool zin = j >= 2 && j < nz - 2;
-            const float xx = dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
-            const float zx = dm<FAST>(z2[v], z1[v], z3[v], z0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+            const float cz1 = FAST ? zc.v1[v] : g.a1z, cz2 = FAST ? zc.v2[v] : g.a2z;
+            const float xx = dm<FAST>(x2[v], x1[v], x3[v], x0[v], g.c8dx, g.c24dx, cx1, cx2);
+            const float zx = dm<FAST>(z2[v], z1[v], z3[v], z0[v], g.c8dx, g.c24dx, cx1, cx2);
             const float xz = dm<FAST>(e3[V + v], e3[V + v - 1], e3[V + v + 1], e3[V + v - 2],
-                                      g.c8dz, g.c24dz, g.a1z, g.a2z);
+                                      g.c8dz, g.c24dz, cz1, cz2);
             const float zz = dm<FAST>(e2[V + v], e2[V + v - 1], e2[V + v + 1], e2[V + v - 2],
-                                      g.c8dz, g.c24dz, g.a1z, g.a2z);
-            dx_[v] = xin ? xx : 0.0f;
-            dz_[v] = xin ? zx : 0.0f;
-            dx_[v] = zin ? dx_[v] + xz : dx_[v];
-            dz_[v] = zin ? dz_[v] + zz : dz_[v];
+                                      g.c8dz, g.c24dz, cz1, cz2);
+            if (FAST) {
+                dx_[v] = xx + xz;
+                dz_[v] = zx + zz;
+            } else {
+                dx_[v] = xin ? xx : 0.0f;
+                dz_[v] = xin ? zx : 0.0f;
+                dx_[v] = zin ? dx_[v] + xz : dx_[v];
+                dz_[v] = zin ? dz_[v] + zz : dz_[v];
+            }
         }
         scan_pts(ip, a.inj_cell, a.inj_row, rlo, rhi, [&](int cp, int row) {
             const int e = cp - (i * nz + jb);
@@ -190,8 +254,10 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const float ab = (abx * abz[v]) * abt[v];
-            n1[v] = rin ? vel_update<FAST>(ab, v1[v], g.dt, dx_[v], rho[v]) : 0.0f;
-            n2[v] = rin ? vel_update<FAST>(ab, v2[v], g.dt, dz_[v], rho[v]) : 0.0f;
+            const float m1 = vel_update<FAST>(ab, v1[v], g.dt, dx_[v], rho[v]);
+            const float m2 = vel_update<FAST>(ab, v2[v], g.dt, dz_[v], rho[v]);
+            n1[v] = FAST || rin ? m1 : 0.0f;
+            n2[v] = FAST || rin ? m2 : 0.0f;
         }
     }
 }
@@ -200,15 +266,18 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
 // of grid row i, from the velocity of step it+1 in LDS rows of W columns
 // (vx, vz for P-SV; vy for SH), grid row i at LDS row q, rows q-1 .. q+2
 // read, and the stress of step it at the lane's cells (sa, sc, sxz: sxx,
-// szz, sxz or sxy, szy).  Explosive sources are the caller's.
+// szz, sxz or sxy, szy).  Explosive sources, and zeroing cells outside the
+// grid in exact mode, are the caller's; zc: fast mode's z coefficients.
 template <bool PSV, int V, int W, bool FAST>
 __device__ __forceinline__ void stress_row(const Geo &g, const float *lv1, const float *lv2, int q,
-                                           int c0, int i, int jb, const float (&sa)[V],
+                                           int c0, int i, int jb, const ZCoef<V> &zc,
+                                           const float (&sa)[V],
                                            const float (&sc)[V], const float (&sxz)[V],
                                            const float (&mu)[V], const float (&lam)[V],
                                            float (&o1)[V], float (&o2)[V], float (&o3)[V]) {
     const int nx = g.nx, nz = g.nz;
     const bool xin = i >= 1 && i < nx - 2;
+    const float cx1 = !FAST || xin ? g.a1x : 0.0f, cx2 = !FAST || xin ? g.a2x : 0.0f;
     float y0[V], y1[V], y2[V], y3[V], e1[3 * V];
     vload<V>(&lv1[(q - 1) * W + c0], y0);
     vload<V>(&lv1[q * W + c0], y1);
@@ -220,13 +289,14 @@ __device__ __forceinline__ void stress_row(const Geo &g, const float *lv1, const
         for (int v = 0; v < V; v++) {
             const int j = jb + v;
             // differences evaluated unconditionally, as in vel_row
-            const float tx = dp<FAST>(y2[v], y1[v], y3[v], y0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+            const float tx = dp<FAST>(y2[v], y1[v], y3[v], y0[v], g.c8dx, g.c24dx, cx1, cx2);
             const float tz = dp<FAST>(e1[V + v + 1], e1[V + v], e1[V + v + 2], e1[V + v - 1],
-                                      g.c8dz, g.c24dz, g.a1z, g.a2z);
-            const float dx_ = xin ? tx : 0.0f;
-            const float dz_ = (j >= 1 && j < nz - 2) ? tz : 0.0f;
-            o1[v] = sa[v] + g.dt * mu[v] * dx_;
-            o2[v] = sc[v] + g.dt * mu[v] * dz_;
+                                      g.c8dz, g.c24dz, FAST ? zc.s1[v] : g.a1z,
+                                      FAST ? zc.s2[v] : g.a2z);
+            const float dx_ = FAST || xin ? tx : 0.0f;
+            const float dz_ = FAST || (j >= 1 && j < nz - 2) ? tz : 0.0f;
+            o1[v] = s_sh<FAST>(sa[v], g.dt, mu[v], dx_);
+            o2[v] = s_sh<FAST>(sc[v], g.dt, mu[v], dz_);
         }
     } else {
         float w0[V], w1[V], w2[V], w3[V], e2[3 * V];
@@ -239,18 +309,18 @@ __device__ __forceinline__ void stress_row(const Geo &g, const float *lv1, const
         for (int v = 0; v < V; v++) {
             const int j = jb + v;
             const bool zin = j >= 1 && j < nz - 2;
-            const float txx = dp<FAST>(y2[v], y1[v], y3[v], y0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
-            const float tzx = dp<FAST>(w2[v], w1[v], w3[v], w0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+            const float cz1 = FAST ? zc.s1[v] : g.a1z, cz2 = FAST ? zc.s2[v] : g.a2z;
+            const float txx = dp<FAST>(y2[v], y1[v], y3[v], y0[v], g.c8dx, g.c24dx, cx1, cx2);
+            const float tzx = dp<FAST>(w2[v], w1[v], w3[v], w0[v], g.c8dx, g.c24dx, cx1, cx2);
             const float txz = dp<FAST>(e1[V + v + 1], e1[V + v], e1[V + v + 2], e1[V + v - 1],
-                                       g.c8dz, g.c24dz, g.a1z, g.a2z);
+                                       g.c8dz, g.c24dz, cz1, cz2);
             const float tzz = dp<FAST>(e2[V + v + 1], e2[V + v], e2[V + v + 2], e2[V + v - 1],
-                                       g.c8dz, g.c24dz, g.a1z, g.a2z);
-            const float xx = xin ? txx : 0.0f, zx = xin ? tzx : 0.0f;
-            const float xz = zin ? txz : 0.0f, zz = zin ? tzz : 0.0f;
+                                       g.c8dz, g.c24dz, cz1, cz2);
+            const float xx = FAST || xin ? txx : 0.0f, zx = FAST || xin ? tzx : 0.0f;
+            const float xz = FAST || zin ? txz : 0.0f, zz = FAST || zin ? tzz : 0.0f;
             const float m = mu[v], l = lam[v];
-            o1[v] = sa[v] + g.dt * ((l + 2 * m) * xx + l * zz);
-            o2[v] = sc[v] + g.dt * ((l + 2 * m) * zz + l * xx);
-            o3[v] = sxz[v] + g.dt * (m * (xz + zx));
+            s_normal<FAST>(sa[v], sc[v], g.dt, l, m, xx, zz, o1[v], o2[v]);
+            o3[v] = s_shear<FAST>(sxz[v], g.dt, m, xz, zx);
         }
     }
 }
@@ -302,6 +372,26 @@ __device__ __forceinline__ void row_exponents(const StepArgs &a, const StepRoles
     }
 }
 
+// the lane's V values of one snapshot field from cell k (a multiple of V):
+// FP32 as a vector, 16-bit formats scaled by 2^e and packed two to a 32-bit
+// word, low half first
+template <int V>
+__device__ __forceinline__ void snap_store(void *base, int fmt, long k, const float (&x)[V],
+                                           int e) {
+    if (fmt == INV_SNAP_F32) {
+        vstore<V>(static_cast<float *>(base) + k, x);
+        return;
+    }
+    unsigned w[V / 2];
+#pragma unroll
+    for (int h = 0; h < V / 2; h++)
+        w[h] = (unsigned)snap_enc16(x[2 * h], e, fmt) |
+               ((unsigned)snap_enc16(x[2 * h + 1], e, fmt) << 16);
+    unsigned short *q = static_cast<unsigned short *>(base) + k;
+    if constexpr (V == 2) *reinterpret_cast<unsigned *>(q) = w[0];
+    else *reinterpret_cast<uint2 *>(q) = make_uint2(w[0], w[1]);
+}
+
 // one snapshot value: FP32 as is, 16-bit formats scaled by 2^e
 __device__ __forceinline__ void snap_put(void *base, int fmt, long idx, float x, int e) {
     if (fmt == INV_SNAP_F32) static_cast<float *>(base)[idx] = x;
@@ -316,7 +406,7 @@ __device__ __forceinline__ void snap_put(void *base, int fmt, long idx, float x,
 // the row, segment `seg` of width segw of the slot's exponent table.
 // Every lane of the wavefront calls it (wave-uniform row); act: the lane
 // owns its cells of the row.
-template <bool PSV, int V>
+template <bool PSV, int V, bool FAST>
 __device__ __forceinline__ void owned_row(const StepArgs &a, const StepRoles &ro, int fu, int fv,
                                           long sb, int i, int jb, bool act, long it, int b,
                                           int seg, int segw, int lane, const LanePts &rp,
@@ -330,23 +420,19 @@ __device__ __forceinline__ void owned_row(const StepArgs &a, const StepRoles &ro
         row_exponents<PSV, V>(a, ro, fu, fv, i, b, seg, segw, lane, jb - seg * segw, act, u1, u2,
                               n1, n2, e);
     if (act) {
+        const long kc = (long)i * g.nz + jb;
+        if (ro.snap) {
+            snap_store<V>(ro.snap, fmt, sb + (long)fu * g.N + kc, u1, e[0]);
+            if (PSV) snap_store<V>(ro.snap, fmt, sb + (long)(fu + 1) * g.N + kc, u2, e[1]);
+        }
 #pragma unroll
         for (int v = 0; v < V; v++) {
-            const long kc = (long)i * g.nz + jb + v;
-            float u = u1[v];
-            float uu = PSV ? u2[v] : 0.0f;
-            if (ro.snap) {
-                snap_put(ro.snap, fmt, sb + (long)fu * g.N + kc, u, e[0]);
-                if (PSV) snap_put(ro.snap, fmt, sb + (long)(fu + 1) * g.N + kc, uu, e[1]);
-            }
-            u += n1[v] * g.dt;
-            if (PSV) uu += n2[v] * g.dt;
-            u1[v] = u;
-            if (PSV) u2[v] = uu;
-            if (ro.snap) {
-                snap_put(ro.snap, fmt, sb + (long)fv * g.N + kc, n1[v], e[2]);
-                if (PSV) snap_put(ro.snap, fmt, sb + (long)(fv + 1) * g.N + kc, n2[v], e[3]);
-            }
+            u1[v] = u_step<FAST>(u1[v], n1[v], g.dt);
+            if (PSV) u2[v] = u_step<FAST>(u2[v], n2[v], g.dt);
+        }
+        if (ro.snap) {
+            snap_store<V>(ro.snap, fmt, sb + (long)fv * g.N + kc, n1, e[2]);
+            if (PSV) snap_store<V>(ro.snap, fmt, sb + (long)(fv + 1) * g.N + kc, n2, e[3]);
         }
     }
     scan_pts(rp, a.rec_cell, a.rec_row, rlo, rhi, [&](int cp, int row) {

@@ -99,7 +99,7 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_velocity(const VelArgs a) {
         float v = a.vy[o], u = a.uy[o];
         if (a.snap) a.snap[(long)b * a.snap_slot_stride + k] = u;
         v = vel_update<FAST>(ab, v, g.dt, d, rho);
-        u += v * g.dt;
+        u = u_step<FAST>(u, v, g.dt);
         a.vy[o] = v;
         a.uy[o] = u;
         if (a.snap) a.snap[(long)b * a.snap_slot_stride + g.N + k] = v;
@@ -138,8 +138,8 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_velocity(const VelArgs a) {
         }
         vx = vel_update<FAST>(ab, vx, g.dt, dx_, rho);
         vz = vel_update<FAST>(ab, vz, g.dt, dz_, rho);
-        ux += vx * g.dt;
-        uz += vz * g.dt;
+        ux = u_step<FAST>(ux, vx, g.dt);
+        uz = u_step<FAST>(uz, vz, g.dt);
         a.vx[o] = vx;
         a.vz[o] = vz;
         a.ux[o] = ux;
@@ -175,8 +175,8 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_stress(const StrArgs a) {
         const float *vy = a.vy + so;
         const float dx_ = xin ? d_plus<FAST>(vy, k, nz, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
         const float dz_ = zin ? d_plus<FAST>(vy, k, 1, g.c8dz, g.c24dz, g.a1z, g.a2z) : 0.0f;
-        a.sxy[o] += g.dt * mu * dx_;
-        a.szy[o] += g.dt * mu * dz_;
+        a.sxy[o] = s_sh<FAST>(a.sxy[o], g.dt, mu, dx_);
+        a.szy[o] = s_sh<FAST>(a.szy[o], g.dt, mu, dz_);
     }
     if (PSV) {
         const float *vx = a.vx + so, *vz = a.vz + so;
@@ -196,9 +196,11 @@ __global__ __launch_bounds__(TILE_Z *TILE_X) void k_stress(const StrArgs a) {
             zz = 0.0f;
         }
         const float lam = a.lam[k];
-        a.sxx[o] += g.dt * ((lam + 2 * mu) * xx + lam * zz);
-        a.szz[o] += g.dt * ((lam + 2 * mu) * zz + lam * xx);
-        a.sxz[o] += g.dt * (mu * (xz + zx));
+        float nxx, nzz;
+        s_normal<FAST>(a.sxx[o], a.szz[o], g.dt, lam, mu, xx, zz, nxx, nzz);
+        a.sxx[o] = nxx;
+        a.szz[o] = nzz;
+        a.sxz[o] = s_shear<FAST>(a.sxz[o], g.dt, mu, xz, zx);
     }
 }
 

@@ -160,7 +160,7 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
                 float u = puy[k];
                 float *sp = static_cast<float *>(a.snap);
                 if (sp) sp[(long)b * a.snap_slot_stride + kc] = u;
-                u += v * g.dt;
+                u = u_step<FAST>(u, v, g.dt);
                 a.vy1[o] = v;
                 a.uy[o] = u;
                 if (sp) sp[(long)b * a.snap_slot_stride + g.N + kc] = v;
@@ -203,8 +203,8 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
                     sp[0] = ux;
                     sp[g.N] = uz;
                 }
-                ux += vx * g.dt;
-                uz += vz * g.dt;
+                ux = u_step<FAST>(ux, vx, g.dt);
+                uz = u_step<FAST>(uz, vz, g.dt);
                 a.vx1[o] = vx;
                 a.vz1[o] = vz;
                 a.ux[o] = ux;
@@ -242,8 +242,8 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
         if (SH) {
             const float dx_ = xin ? d_plus<FAST>(l_vy, ve, VW, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
             const float dz_ = zin ? d_plus<FAST>(l_vy, ve, 1, g.c8dz, g.c24dz, g.a1z, g.a2z) : 0.0f;
-            a.sxy1[o] = l_sxy[se] + g.dt * mu * dx_;
-            a.szy1[o] = l_szy[se] + g.dt * mu * dz_;
+            a.sxy1[o] = s_sh<FAST>(l_sxy[se], g.dt, mu, dx_);
+            a.szy1[o] = s_sh<FAST>(l_szy[se], g.dt, mu, dz_);
         }
         if (PSV) {
             float xx, zx, xz, zz;
@@ -262,9 +262,11 @@ __global__ __launch_bounds__(FBLOCK) void k_step(const StepArgs a) {
                 zz = 0.0f;
             }
             const float lam = plam[k];
-            a.sxx1[o] = l_sxx[se] + g.dt * ((lam + 2 * mu) * xx + lam * zz);
-            a.szz1[o] = l_szz[se] + g.dt * ((lam + 2 * mu) * zz + lam * xx);
-            a.sxz1[o] = l_sxz[se] + g.dt * (mu * (xz + zx));
+            float nxx, nzz;
+            s_normal<FAST>(l_sxx[se], l_szz[se], g.dt, lam, mu, xx, zz, nxx, nzz);
+            a.sxx1[o] = nxx;
+            a.szz1[o] = nzz;
+            a.sxz1[o] = s_shear<FAST>(l_sxz[se], g.dt, mu, xz, zx);
         }
     }
 }
@@ -470,7 +472,7 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                 }
                 if (own) {
                     if (a.snap) snap_put(a.snap, fmt, sb + kc, u, eu);
-                    u += v * g.dt;
+                    u = u_step<FAST>(u, v, g.dt);
                     a.vy1[o] = v;
                     a.uy[o] = u;
                     if (a.snap) snap_put(a.snap, fmt, sb + g.N + kc, v, ev);
@@ -518,8 +520,8 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                         snap_put(a.snap, fmt, su, ux, ex);
                         snap_put(a.snap, fmt, su + g.N, uz, ez);
                     }
-                    ux += vx * g.dt;
-                    uz += vz * g.dt;
+                    ux = u_step<FAST>(ux, vx, g.dt);
+                    uz = u_step<FAST>(uz, vz, g.dt);
                     a.vx1[o] = vx;
                     a.vz1[o] = vz;
                     a.ux[o] = ux;
@@ -570,8 +572,8 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                 const float dx_ = xin ? d_plus<FAST>(l_vy, ve, W, g.c8dx, g.c24dx, g.a1x, g.a2x) : 0.0f;
                 const float dz_ = zin ? d_plus<FAST>(l_vy, ve, 1, g.c8dz, g.c24dz, g.a1z, g.a2z) : 0.0f;
                 if (jtin) {
-                    a.sxy1[o] = l_sxy[se] + g.dt * mu * dx_;
-                    a.szy1[o] = l_szy[se] + g.dt * mu * dz_;
+                    a.sxy1[o] = s_sh<FAST>(l_sxy[se], g.dt, mu, dx_);
+                    a.szy1[o] = s_sh<FAST>(l_szy[se], g.dt, mu, dz_);
                 }
             } else {
                 float xx, zx, xz, zz;
@@ -590,8 +592,8 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                     zz = 0.0f;
                 }
                 const float lam = qlam[p];
-                float nxx = l_sxx[se] + g.dt * ((lam + 2 * mu) * xx + lam * zz);
-                float nzz = l_szz[se] + g.dt * ((lam + 2 * mu) * zz + lam * xx);
+                float nxx, nzz;
+                s_normal<FAST>(l_sxx[se], l_szz[se], g.dt, lam, mu, xx, zz, nxx, nzz);
                 // explosive sources this tile owns: stf_y into sxx and szz
                 const int xlo = i * nz + j0;
                 scan_pts(xp, a.sinj_cell, a.sinj_row, xlo, xlo + CZ - 1, [&](int cp, int row) {
@@ -604,7 +606,7 @@ __global__ __launch_bounds__(FBLOCK) void k_col(const StepArgs a) {
                 if (jtin) {
                     a.sxx1[o] = nxx;
                     a.szz1[o] = nzz;
-                    a.sxz1[o] = l_sxz[se] + g.dt * (mu * (xz + zx));
+                    a.sxz1[o] = s_shear<FAST>(l_sxz[se], g.dt, mu, xz, zx);
                 }
             }
         }

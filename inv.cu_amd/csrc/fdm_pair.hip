@@ -80,6 +80,7 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
     const int jl = jb < 0 ? 0 : (jb > nz - V ? nz - V : jb);
     const bool jin = jb >= 0 && jb < nz;  // all V columns in the grid, or none
     const bool cown = jin && c0 >= HL && c0 < HL + S;
+    const ZCoef<V> zc = zcoef<V>(g, jb);  // fast mode's z coefficients
     auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
     const float *m_rho = a.rho, *m_lam = a.lam, *m_mu = a.mu;
     if (a.slot_model) {
@@ -103,23 +104,25 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
         if (r >= 2 && r < SR - 1) vload<V>(S2 + k, s2[p]);  // wave-uniform
         if (PSV) vload<V>(S3 + k, s3[p]);
     }
-    float qv1[P][V], qv2[P][V], qr[P][V], qu1[P][V], qu2[P][V], qmu[P][V], qlam[P][V];
+    float qv1[P][V], qv2[P][V], qr[P][V], qu1[P][V], qu2[P][V], qmu[P][V], qlam[P][V], qab[P];
 #pragma unroll
     for (int p = 0; p < P; p++) {
         const int x = wave - 4 + p * NW;
-        const long kc = (long)clampi(i0 + x, 0, nx - 1) * nz + jl;
+        const int ic = clampi(i0 + x, 0, nx - 1);
+        const long kc = (long)ic * nz + jl;
         if (x < TX + 5) {
-            vload<V>(m_rho + kc, qr[p]);
+            vload_g<V>(m_rho + kc, qr[p]);
             vload<V>(V1 + so + kc, qv1[p]);
             if (PSV) vload<V>(V2 + so + kc, qv2[p]);
+            qab[p] = vload_uniform(a.abx, ic);
         }
         if (x >= 0 && x < TX) {  // u only on owned rows (wave-uniform)
             vload<V>(U1 + so + kc, qu1[p]);
             if (PSV) vload<V>(U2 + so + kc, qu2[p]);
         }
         if (x >= -3 && x < TX + 3) {  // rows of the two stress updates
-            vload<V>(m_mu + kc, qmu[p]);
-            if (PSV) vload<V>(m_lam + kc, qlam[p]);
+            vload_g<V>(m_mu + kc, qmu[p]);
+            if (PSV) vload_g<V>(m_lam + kc, qlam[p]);
         }
     }
     float abz[V], abt[V];
@@ -181,16 +184,16 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
             if (x >= TX + 5) break;
             const int i = i0 + x;
             const bool rin = jin && i >= 0 && i < nx;
-            const float abx = a.abx[clampi(i, 0, nx - 1)];
             const int rlo = i * nz + (j0 - HL), rhi = rlo + W - 1;
             float n1[V], n2[V];
-            vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, x + 4, c0, i, jb, rin, abx, abz, abt,
-                                     qv1[p], qv2[p], qr[p], ip, rlo, rhi, b, it, n1, n2);
+            vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, x + 4, c0, i, jb, rin, zc,
+                                     qab[p], abz, abt, qv1[p], qv2[p], qr[p], ip, rlo, rhi, b, it,
+                                     n1, n2);
             vstore<V>(&l_v1[(x + 4) * W + c0], n1);
             if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], n2);
             if (x >= 0 && x < TX && i < nx)  // wave-uniform
-                owned_row<PSV, V>(a, r1s, fu, fv, sb, i, jb, cown, it, b, tz, S, lane, rp, rlo,
-                                  rhi, qu1[p], qu2[p], n1, n2);
+                owned_row<PSV, V, FAST>(a, r1s, fu, fv, sb, i, jb, cown, it, b, tz, S, lane, rp,
+                                        rlo, rhi, qu1[p], qu2[p], n1, n2);
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 qv1[p][v] = n1[v];
@@ -213,10 +216,10 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
         vload<V>(&l_s1[r * W + c0], sa);
         vload<V>(&l_s2[(x + 4) * W + c0], sc);
         if (PSV) vload<V>(&l_s3[r * W + c0], sxz);
-        stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, x + 4, c0, i, jb, sa, sc, sxz, qmu[p], qlam[p],
-                                    o1, o2, o3);
+        stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, x + 4, c0, i, jb, zc, sa, sc, sxz, qmu[p],
+                                    qlam[p], o1, o2, o3);
         if (PSV) explode_row<V>(a, xp, i, jb, b, it, o1, o2);
-        if (!rin) {
+        if (!FAST && !rin) {
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 o1[v] = 0.0f;
@@ -240,16 +243,16 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
             if (x >= TX + 2) break;
             const int i = i0 + x;
             const bool rin = jin && i >= 0 && i < nx;
-            const float abx = a.abx[clampi(i, 0, nx - 1)];
             const int rlo = i * nz + (j0 - HL), rhi = rlo + W - 1;
             float n1[V], n2[V];
-            vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, x + 4, c0, i, jb, rin, abx, abz, abt,
-                                     qv1[p], qv2[p], qr[p], ip, rlo, rhi, b, it + 1, n1, n2);
+            vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, x + 4, c0, i, jb, rin, zc,
+                                     qab[p], abz, abt, qv1[p], qv2[p], qr[p], ip, rlo, rhi, b,
+                                     it + 1, n1, n2);
             vstore<V>(&l_v1[(x + 4) * W + c0], n1);
             if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], n2);
             if (x >= 0 && x < TX && i < nx) {  // wave-uniform
-                owned_row<PSV, V>(a, r2, fu, fv, sb, i, jb, cown, it + 1, b, tz, S, lane, rp,
-                                  rlo, rhi, qu1[p], qu2[p], n1, n2);
+                owned_row<PSV, V, FAST>(a, r2, fu, fv, sb, i, jb, cown, it + 1, b, tz, S, lane,
+                                        rp, rlo, rhi, qu1[p], qu2[p], n1, n2);
                 if (cown) {
                     const long o = so + (long)i * nz + jb;
                     vstore<V>(V1o + o, n1);
@@ -279,8 +282,8 @@ __global__ __launch_bounds__(64 * NW, SH && TX == 16 && !SNAP ? 8 : 1) void k_pa
         vload<V>(&l_s1[r * W + c0], sa);
         vload<V>(&l_s2[(x + 4) * W + c0], sc);
         if (PSV) vload<V>(&l_s3[r * W + c0], sxz);
-        stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, x + 4, c0, i, jb, sa, sc, sxz, qmu[p], qlam[p],
-                                    o1, o2, o3);
+        stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, x + 4, c0, i, jb, zc, sa, sc, sxz, qmu[p],
+                                    qlam[p], o1, o2, o3);
         if (PSV) explode_row<V>(a, xp, i, jb, b, it + 1, o1, o2);
         if (cown) {
             vstore<V>(S1o + o, o1);

@@ -59,6 +59,7 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
     const int jl = jb < 0 ? 0 : (jb > nz - V ? nz - V : jb);
     const bool jin = jb >= 0 && jb < nz;  // all V columns in the grid, or none
     const bool cown = jin && c0 >= HL && c0 < HL + S;
+    const ZCoef<V> zc = zcoef<V>(g, jb);  // fast mode's z coefficients
     auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
     const float *m_rho = a.rho, *m_lam = a.lam, *m_mu = a.mu;
     if (a.slot_model) {
@@ -82,12 +83,14 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         vload<V>(S2 + k, s2[p]);
         if (PSV) vload<V>(S3 + k, s3[p]);
     }
-    float qv1[PV][V], qv2[PV][V], qr[PV][V], qu1[PV][V], qu2[PV][V];
+    float qv1[PV][V], qv2[PV][V], qr[PV][V], qu1[PV][V], qu2[PV][V], qab[PV];
 #pragma unroll
     for (int p = 0; p < PV; p++) {
         const int vr = wave + p * NW;
-        const long kc = (long)clampi(i0 - 1 + vr, 0, nx - 1) * nz + jl;
-        vload<V>(m_rho + kc, qr[p]);
+        const int ic = clampi(i0 - 1 + vr, 0, nx - 1);
+        const long kc = (long)ic * nz + jl;
+        vload_g<V>(m_rho + kc, qr[p]);
+        qab[p] = vload_uniform(a.abx, ic);
         vload<V>(V1 + so + kc, qv1[p]);
         if (PSV) vload<V>(V2 + so + kc, qv2[p]);
         if (vr >= 1 && vr <= TX) {  // u only on owned rows (wave-uniform)
@@ -100,8 +103,8 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
     for (int p = 0; p < PT; p++) {
         const int tr = wave + p * NW;
         const long kc = (long)clampi(i0 + tr, 0, nx - 1) * nz + jl;
-        vload<V>(m_mu + kc, qmu[p]);
-        if (PSV) vload<V>(m_lam + kc, qlam[p]);
+        vload_g<V>(m_mu + kc, qmu[p]);
+        if (PSV) vload_g<V>(m_lam + kc, qlam[p]);
     }
     float abz[V], abt[V];
     vload<V>(a.abzb + jl, abz);
@@ -153,21 +156,19 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         if (vr >= VR) break;
         const int i = i0 - 1 + vr;
         const bool rin = jin && i >= 0 && i < nx;
-        const float abx = a.abx[clampi(i, 0, nx - 1)];
         // cells of this row the wavefront covers: point lists are scanned with
         // one wave-uniform test per point before the per-lane one
         const int rlo = i * nz + (j0 - HL), rhi = rlo + W - 1;
         float n1[V], n2[V];
-        vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, vr + 2, vr + 2, c0, i, jb, rin, abx, abz, abt,
-                                 qv1[p],
-                                 qv2[p], qr[p], ip, rlo, rhi, b, it, n1, n2);
+        vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, vr + 2, vr + 2, c0, i, jb, rin, zc, qab[p],
+                                 abz, abt, qv1[p], qv2[p], qr[p], ip, rlo, rhi, b, it, n1, n2);
         vstore<V>(&l_v1[vr * W + c0], n1);
         if (PSV) vstore<V>(&l_v2[vr * W + c0], n2);
         // owned cells: v, u, snapshots, receivers (the row test is
         // wave-uniform; owned_row takes the lane's ownership)
         if (vr >= 1 && vr <= TX && i < nx) {
-            owned_row<PSV, V>(a, r1s, fu, fv, sb, i, jb, cown, it, b, tz, S, lane, rp, rlo, rhi,
-                              qu1[p], qu2[p], n1, n2);
+            owned_row<PSV, V, FAST>(a, r1s, fu, fv, sb, i, jb, cown, it, b, tz, S, lane, rp, rlo,
+                                    rhi, qu1[p], qu2[p], n1, n2);
             if (cown) {
                 const long o = so + (long)i * nz + jb;
                 vstore<V>(V1o + o, n1);
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(WBLOCK) void k_colv(const StepArgs a) {
         vload<V>(&l_s1[(tr + 3) * W + c0], sa);
         vload<V>(&l_s2[(tr + 3) * W + c0], sc);
         if (PSV) vload<V>(&l_s3[(tr + 3) * W + c0], sxz);
-        stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, tr + 1, c0, i, jb, sa, sc, sxz, qmu[p],
+        stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, tr + 1, c0, i, jb, zc, sa, sc, sxz, qmu[p],
                                     qlam[p], o1, o2, o3);
         // explosive sources this tile owns: stf_y into sxx and szz (every
         // lane scans; owned lanes store)

@@ -24,18 +24,25 @@ struct Geo {
 // FP32-rounded coefficients 9/(8h) and 1/(24h) instead: two multiplies in
 // place of two ~11-instruction divides, the same FP32 accuracy against FP64
 // but not bitwise equal to the exact mode.  f may point to global memory or
-// to LDS; (c8, c24) and (a1, a2) are the x or z constants of Geo.
+// to LDS; (c8, c24) and (a1, a2) are the x or z constants of Geo.  Fast
+// mode fuses multiply-adds throughout (dm, dp and the updates below), the
+// same expressions in every kernel, so the fast kernels agree bit for bit
+// with each other as the exact ones do.
 // On values: dm(f[k], f[k-s], f[k+s], f[k-2s]) and dp(f[k+s], f[k], f[k+2s], f[k-s]).
+// Fast mode evaluates a1 (f0 - fm1) - a2 (fp1 - fm2) as one fused
+// multiply-add on the product a2 (fp1 - fm2); with a1 = a2 = 0 it gives a
+// zero, which the fast row kernels use to drop a difference outside its
+// range (finite operands).
 template <bool FAST>
 __device__ __forceinline__ float dm(float f0, float fm1, float fp1, float fm2, float c8, float c24,
                                     float a1, float a2) {
-    if (FAST) return a1 * (f0 - fm1) - a2 * (fp1 - fm2);
+    if (FAST) return __builtin_fmaf(a1, f0 - fm1, -(a2 * (fp1 - fm2)));
     return 9.0f * (f0 - fm1) / c8 - (fp1 - fm2) / c24;
 }
 template <bool FAST>
 __device__ __forceinline__ float dp(float fp1, float f0, float fp2, float fm1, float c8, float c24,
                                     float a1, float a2) {
-    if (FAST) return a1 * (fp1 - f0) - a2 * (fp2 - fm1);
+    if (FAST) return __builtin_fmaf(a1, fp1 - f0, -(a2 * (fp2 - fm1)));
     return 9.0f * (fp1 - f0) / c8 - (fp2 - fm1) / c24;
 }
 template <bool FAST>
@@ -190,8 +197,48 @@ __device__ __forceinline__ float d_plus_acc(const A &f, long k, long s, float c8
 // mode reads the per-run field bdt = dt / rho in place of rho.
 template <bool FAST>
 __device__ __forceinline__ float vel_update(float ab, float v, float dt, float d, float rho_or_bdt) {
-    if (FAST) return ab * (v + d * rho_or_bdt);
+    if (FAST) return ab * __builtin_fmaf(d, rho_or_bdt, v);
     return ab * (v + dt * d / rho_or_bdt);
+}
+
+// stress updates (updateSY / updateSXZ, solver/fdm.h:138-149): the SH
+// stresses s + dt mu d, the normal stresses s + dt ((lam + 2 mu) a + lam b)
+// and the shear stress s + dt (mu (xz + zx)); and the displacement u + v dt
+// (updateVY / updateVXZ, fdm.h:153,159-160), separately rounded in both
+// modes: it is the recurrence by which the reference integrates velocity
+// trace files into displacement (misfit/misfit.h:113-118), so displacement
+// traces recorded here equal integrated velocity traces bit for bit
+template <bool FAST>
+__device__ __forceinline__ float s_sh(float s, float dt, float mu, float d) {
+    if (FAST) return __builtin_fmaf(dt * mu, d, s);
+    return s + dt * mu * d;
+}
+// The normal stresses as a pair: sxx from (xx, zz), szz from (zz, xx).  Fast
+// mode writes them as s + dt (2 mu a + lam (xx + zz)) with the shared term
+// formed once: with mu = 0 (a fluid) both increments are then the same
+// number, as in exact arithmetic, where a fused lam a + lam b would round the
+// two differently and leave a spurious deviatoric stress that the fluid
+// cannot relax (traces in a water layer drift by 1e-4 over 4000 steps).
+template <bool FAST>
+__device__ __forceinline__ void s_normal(float sxx, float szz, float dt, float lam, float mu,
+                                         float xx, float zz, float &oxx, float &ozz) {
+    if (FAST) {
+        const float c = lam * (xx + zz), m2 = 2.0f * mu;
+        oxx = __builtin_fmaf(dt, __builtin_fmaf(m2, xx, c), sxx);
+        ozz = __builtin_fmaf(dt, __builtin_fmaf(m2, zz, c), szz);
+        return;
+    }
+    oxx = sxx + dt * ((lam + 2 * mu) * xx + lam * zz);
+    ozz = szz + dt * ((lam + 2 * mu) * zz + lam * xx);
+}
+template <bool FAST>
+__device__ __forceinline__ float s_shear(float s, float dt, float mu, float xz, float zx) {
+    if (FAST) return __builtin_fmaf(dt * mu, xz + zx, s);
+    return s + dt * (mu * (xz + zx));
+}
+template <bool FAST>
+__device__ __forceinline__ float u_step(float u, float v, float dt) {
+    return u + v * dt;
 }
 
 }  // namespace inv
