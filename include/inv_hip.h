@@ -15,7 +15,7 @@
  *     allocates.  Functions are thread-compatible per context, not
  *     thread-safe; different contexts may run concurrently.
  *   - fields are float32 [nx][nz], z fastest (k = i*nz + j), the reference's
- *     Dim layout (lib/device.h:303-308).  Traces are [nrec][nt] per
+ *     Dim layout (lib/device.h:7-65).  Traces are [nrec][nt] per
  *     component; components are ordered y (if sh) then x, z (if psv).
  *   - "host" pointers are caller-owned host memory copied in or out;
  *     "dev" pointers are device addresses (from inv_field_ptr / inv_alloc).

@@ -285,7 +285,7 @@ int inv_calc_idx(float coord, float length, int n) {
 int inv_source_ricker(int nt, float dt, float f0, float t0, float angle, float amp,
                       float *stf_x, float *stf_y, float *stf_z) {
     if (nt < 1 || !stf_x || !stf_y || !stf_z) return INV_ERR_ARG;
-    const float pi = 3.1415927f;  // host::pi, lib/device.h:336
+    const float pi = 3.1415927f;  // host::pi, lib/device.h:68
     std::vector<float> stf(nt);
     float amax = 0;
     for (int it = 0; it < nt; it++) {
@@ -1346,10 +1346,13 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                 }
                 // then two-step tiles over the chosen one-step tile (which
                 // runs the steps that cannot be paired); a tile needing more
-                // LDS than the device allows is skipped
+                // LDS than the device allows is skipped.  Tallest first: a
+                // shorter tile recomputes more halo per owned row and must
+                // be 2 % faster to replace it (20 and 16 rows are within a
+                // few per cent on large P-SV grids, where the order decided)
                 const TuneChoice single = best;
                 for (int v : {2, 4})
-                    for (int tx : {4, 8, 12, 16, 20}) {
+                    for (int tx : {20, 16, 12, 8, 4}) {
                         if (c.g.nz % v || !pair_tile_ok(tx, v)) continue;
                         TuneChoice t = single;
                         t.pair_tx = tx;

@@ -55,23 +55,20 @@ __device__ __forceinline__ void vstore(float *__restrict__ p, const float (&o)[V
     else *reinterpret_cast<float4 *>(p) = make_float4(o[0], o[1], o[2], o[3]);
 }
 
-// e[V + k] = row[c0 + k] for k in [-V, 2V): the lane's vector and its two
-// neighbours from an LDS row of W columns (indices clamped into the row; the
-// clamped values only reach columns no owned cell depends on)
-template <int V, int W>
-__device__ __forceinline__ void window(const float *row, int c0, float (&e)[3 * V]) {
-    const int cl = c0 - V < 0 ? 0 : c0 - V;
-    const int cr = c0 + V > W - V ? W - V : c0 + V;
-    float t[V];
-    vload<V>(row + cl, t);
+// e[V + k] = row[c0 + k] for k in [-V, 2V): the lane's vector of an LDS row
+// and its two neighbours, from the lane's own vector c = row[c0 .. c0+V-1]
+// and the neighbouring lanes' (DPP): no LDS reads beyond c (reading the
+// neighbours from LDS took narrowed single-word reads at a two-word stride,
+// two-way bank conflicts).  The edge lanes get 0 for the missing neighbour; they are halo
+// lanes, whose cells no owned cell depends on.  Every lane must be active.
+template <int V>
+__device__ __forceinline__ void window_reg(const float (&c)[V], float (&e)[3 * V]) {
 #pragma unroll
-    for (int v = 0; v < V; v++) e[v] = t[v];
-    vload<V>(row + c0, t);
-#pragma unroll
-    for (int v = 0; v < V; v++) e[V + v] = t[v];
-    vload<V>(row + cr, t);
-#pragma unroll
-    for (int v = 0; v < V; v++) e[2 * V + v] = t[v];
+    for (int v = 0; v < V; v++) {
+        e[v] = lane_prev(c[v]);
+        e[V + v] = c[v];
+        e[2 * V + v] = lane_next(c[v]);
+    }
 }
 
 // A tile's entries p0 .. p1-1 of a point list, the first 64 held one per
@@ -172,7 +169,9 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
         vload<V>(&ls1[(r - 1) * W + c0], x1);
         vload<V>(&ls1[r * W + c0], x2);
         vload<V>(&ls1[(r + 1) * W + c0], x3);
-        window<V, W>(&ls2[r2 * W], c0, ez);
+        float zc0[V];
+        vload<V>(&ls2[r2 * W + c0], zc0);
+        window_reg<V>(zc0, ez);
         // every difference is evaluated and the out-of-range ones dropped
         // by selects: branches here make the compiler sink each column's
         // LDS reads into its own branch, splitting the vector reads into
@@ -216,8 +215,10 @@ __device__ __forceinline__ void vel_row(const StepArgs &a, const float *ls1, con
         vload<V>(&ls3[(r - 1) * W + c0], z1);
         vload<V>(&ls3[r * W + c0], z2);
         vload<V>(&ls3[(r + 1) * W + c0], z3);
-        window<V, W>(&ls3[r * W], c0, e3);
-        window<V, W>(&ls2[r2 * W], c0, e2);
+        float zc0[V];
+        vload<V>(&ls2[r2 * W + c0], zc0);
+        window_reg<V>(z2, e3);
+        window_reg<V>(zc0, e2);
         float dx_[V], dz_[V];
 #pragma unroll
         for (int v = 0; v < V; v++) {
@@ -283,7 +284,7 @@ __device__ __forceinline__ void stress_row(const Geo &g, const float *lv1, const
     vload<V>(&lv1[q * W + c0], y1);
     vload<V>(&lv1[(q + 1) * W + c0], y2);
     vload<V>(&lv1[(q + 2) * W + c0], y3);
-    window<V, W>(&lv1[q * W], c0, e1);
+    window_reg<V>(y1, e1);
     if (!PSV) {
 #pragma unroll
         for (int v = 0; v < V; v++) {
@@ -304,7 +305,7 @@ __device__ __forceinline__ void stress_row(const Geo &g, const float *lv1, const
         vload<V>(&lv2[q * W + c0], w1);
         vload<V>(&lv2[(q + 1) * W + c0], w2);
         vload<V>(&lv2[(q + 2) * W + c0], w3);
-        window<V, W>(&lv2[q * W], c0, e2);
+        window_reg<V>(w1, e2);
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const int j = jb + v;

@@ -345,12 +345,13 @@ __device__ __forceinline__ void snap_pair(const IntArgs &a, int b, int f, long k
 
 // D+ along z at the lane's two columns from their values e0, e1 (columns
 // c, c+1) and the neighbouring lanes' (c-1 from the lane before, c+2, c+3
-// from the lane after); every lane of the wavefront must call it
+// from the lane after, by DPP; the edge lanes, which own no cell, get 0);
+// every lane of the wavefront must call it
 template <bool FAST>
 __device__ __forceinline__ void dz_pair(float e0, float e1, const Geo &g, float &d0, float &d1) {
-    const float m1 = __shfl_up(e1, 1);
-    const float p2 = __shfl_down(e0, 1);
-    const float p3 = __shfl_down(e1, 1);
+    const float m1 = lane_prev(e1);
+    const float p2 = lane_next(e0);
+    const float p3 = lane_next(e1);
     d0 = dp<FAST>(e1, e0, p2, m1, g.c8dz, g.c24dz, g.a1z, g.a2z);
     d1 = dp<FAST>(p2, e1, p3, e0, g.c8dz, g.c24dz, g.a1z, g.a2z);
 }

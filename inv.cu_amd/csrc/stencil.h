@@ -45,6 +45,16 @@ __device__ __forceinline__ float dp(float fp1, float f0, float fp2, float fm1, f
     if (FAST) return __builtin_fmaf(a1, fp1 - f0, -(a2 * (fp2 - fm1)));
     return 9.0f * (fp1 - f0) / c8 - (fp2 - fm1) / c24;
 }
+// The value of x in the previous / next lane of the wavefront (DPP
+// wave_shr:1 / wave_shl:1; 0 in lane 0 / lane 63).  Every lane must be
+// active.
+__device__ __forceinline__ float lane_prev(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_next(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, true));
+}
+
 template <bool FAST>
 __device__ __forceinline__ float d_minus(const float *__restrict__ f, long k, long s, float c8,
                                          float c24, float a1, float a2) {

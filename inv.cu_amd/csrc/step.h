@@ -47,10 +47,30 @@ struct StepArgs {
     long snap_slot_stride;  // elements per batch slot
 };
 
+// The maximum of x >= 0 over the wavefront, wave-uniform: DPP within each
+// row of 16 lanes (quad swaps, half-row and row mirrors), then the four
+// rows' results read into scalars and compared as integers (the order of
+// non-negative floats).  Every lane must be active.
+__device__ __forceinline__ float dpp_max(float x, int ctrl) {
+    const int y = ctrl == 0xB1 ? __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xf, 0xf, false)
+                : ctrl == 0x4E ? __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xf, 0xf, false)
+                : ctrl == 0x141 ? __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x141, 0xf, 0xf, false)
+                                : __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x140, 0xf, 0xf, false);
+    return fmaxf(x, __int_as_float(y));
+}
 __device__ __forceinline__ float wave_max(float x) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
-    return x;
+    x = dpp_max(x, 0xB1);   // quad_perm [1, 0, 3, 2]
+    x = dpp_max(x, 0x4E);   // quad_perm [2, 3, 0, 1]
+    x = dpp_max(x, 0x141);  // row_half_mirror
+    x = dpp_max(x, 0x140);  // row_mirror
+    const unsigned b = (unsigned)__float_as_int(x);
+    unsigned m = __builtin_amdgcn_readlane(b, 0);
+    const unsigned m1 = __builtin_amdgcn_readlane(b, 16), m2 = __builtin_amdgcn_readlane(b, 32),
+                   m3 = __builtin_amdgcn_readlane(b, 48);
+    m = m > m1 ? m : m1;
+    m = m > m2 ? m : m2;
+    m = m > m3 ? m : m3;
+    return __int_as_float((int)m);
 }
 
 // Block offsets of one field of a row segment (stencil.h): the lane holds V
@@ -66,13 +86,27 @@ __device__ __forceinline__ int snap_blk_put(float m, int e, int lane, int oc, bo
                                             unsigned char *seg_blk) {
     constexpr int LPB = V >= 4 ? 1 : 4 / V;  // lanes per block
     float bm = m;
-    if (LPB > 1) {
+    int d, dn;
+    if constexpr (V == 4) {
+        d = snap_blk_d(bm, e);
+        dn = __builtin_amdgcn_update_dpp(0, d, 0x130, 0xf, 0xf, true);  // lane + 1
+    } else if constexpr (V == 2) {
+        // the block's other lane is the next one for its first lane, else the
+        // previous one (neighbour exchange by DPP wavefront shifts)
+        const int mb = __float_as_int(m);
+        const float nx = __int_as_float(__builtin_amdgcn_update_dpp(0, mb, 0x130, 0xf, 0xf, true));
+        const float pv = __int_as_float(__builtin_amdgcn_update_dpp(0, mb, 0x138, 0xf, 0xf, true));
+        bm = fmaxf(m, (oc & 3) == 0 ? nx : pv);
+        d = snap_blk_d(bm, e);
+        const int d1 = __builtin_amdgcn_update_dpp(0, d, 0x130, 0xf, 0xf, true);
+        dn = __builtin_amdgcn_update_dpp(0, d1, 0x130, 0xf, 0xf, true);  // lane + 2
+    } else {
         const int lead = lane - ((oc & 3) / V);
 #pragma unroll
         for (int k = 0; k < LPB; k++) bm = fmaxf(bm, __shfl(m, lead + k));
+        d = snap_blk_d(bm, e);
+        dn = __shfl(d, lane + LPB);  // the next block's offset, from its first lane
     }
-    const int d = snap_blk_d(bm, e);
-    const int dn = __shfl(d, lane + LPB);  // the next block's offset, from its first lane
     if (own && (oc & 7) == 0) {
         const int lb = oc >> 2;
         const int hi = (lb + 1) * 4 < segw ? dn : 0;

@@ -33,7 +33,7 @@ inline int threads() {
 #endif
 }
 
-const float kPiF = 3.1415927f;  // host::pi and device::pi, lib/device.h:336,364
+const float kPiF = 3.1415927f;  // host::pi and device::pi, lib/device.h:68,96
 
 // ---------------------------------------------------------------------------
 // stencils, solver/fdm.h:76-136

@@ -12,11 +12,14 @@
  *
  * Parity pins (how each function is checked, see tests/test_oracle_*.py):
  *   - SH forward: reference golden traces example/checker/traces/vy_*.su
- *   - P-SV forward: energy non-growth without absorption, source-receiver
- *     reciprocity, FP32 vs FP64 agreement, brute-force tiny grids
- *   - adjoint kernels: finite-difference gradient check on a tiny model,
- *     snapshot pairing
- *   - waveform misfit: closed form; envelope: numpy FFT Hilbert mask
+ *   - P-SV forward: two 1-D reductions (x-uniform z-forces equal the SH path
+ *     with mu' = lambda + 2 mu, x-forces the SH path with mu, to FP64
+ *     round-off), source-receiver reciprocity, energy without absorption
+ *   - adjoint kernels: finite-difference gradient checks (SH within 1 %,
+ *     the P-SV reductions within 1e-4, a 2-D P-SV model within 5e-3),
+ *     snapshot pairing (sfe 10 against sfe 1)
+ *   - waveform misfit: closed form; envelope: closed forms for cosine
+ *     traces (both branches of the floor) and the numpy FFT Hilbert mask
  *   - Gaussian filter: brute-force one-sided weights
  */
 #ifndef INV_ORACLE_H
@@ -27,7 +30,7 @@ extern "C" {
 #endif
 
 /* Simulation description.  Model fields are float [nx*nz], z fastest
- * (k = i*nz + j), as Dim lays them out (lib/device.h:303-308). */
+ * (k = i*nz + j), as Dim lays them out (lib/device.h:7-65). */
 typedef struct {
     int nx, nz;
     float dx, dz, dt;
