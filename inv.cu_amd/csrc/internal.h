@@ -177,9 +177,11 @@ struct Ctx {
     long gen = 0;
     bool use_graphs = true;
 
-    // envelope misfit: FFT twiddles and radix plan for length nt
-    DevBuf env_w, env_fac, env_scratch;
-    int env_n = -1, env_nfac = 0;
+    // envelope misfit: FFT twiddles and radix plan for length nt, or (env_m >
+    // 0) for the power-of-two length env_m of a Bluestein transform, whose
+    // chirp and filter spectra live in env_blue
+    DevBuf env_w, env_fac, env_scratch, env_blue;
+    int env_n = -1, env_nfac = 0, env_m = 0;
 
     // filter
     DevBuf gauss_w, gauss_tmp, gauss_sx, gauss_sz;

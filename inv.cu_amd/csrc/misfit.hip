@@ -1,6 +1,8 @@
 // Trace misfits and adjoint sources on gfx950, batched over every trace of
 // every shot slot in one launch (the reference runs one <<<nt,1>>> kernel and
 // one cuBLAS norm with a host sync per trace, misfit/misfit.h:31-40).
+#include <complex>
+
 #include "internal.h"
 
 namespace inv {
@@ -122,7 +124,9 @@ double misfit_trace(Ctx &c, int type, const float *syn, const float *obs, float 
 // trace keeps the trace in LDS and runs the three analytic-signal transforms
 // A = IFFT(mask * FFT(x)) (unnormalised) with a mixed-radix Stockham FFT
 // (radix 4, 2, 3, 5, 7; twiddles from a host FP64 table rounded to FP32).
-// Lengths with another prime factor are rejected.
+// A length with another prime factor (cuFFT takes any length) goes through
+// Bluestein's algorithm: the DFT of length n as a convolution with a chirp,
+// done by power-of-two transforms of length M >= 2n - 1 (bluestein_lds).
 // ---------------------------------------------------------------------------
 constexpr int EB = 1024;               // threads per trace
 constexpr int ENV_MAXN = 16 * EB;      // longest trace (elements per thread <= 16)
@@ -194,11 +198,55 @@ __device__ void fft_lds(float2 *buf, int n, const int *__restrict__ fac, int nfa
     }
 }
 
+// The transform plan of one trace length n: a mixed-radix FFT of length n
+// (m = 0), or Bluestein with power-of-two transforms of length m: with the
+// chirp w[k] = exp(-i pi k^2 / n),
+//   X[j] = w[j] (a * b)[j],  a[k] = x[k] w[k],  b[k] = conj(w[|k|]),
+// the cyclic convolution of length m taken as IFFT(FFT(a) B) with
+// B = FFT(b) / m precomputed in FP64 (bf, and bi = FFT(conj b) / m for the
+// inverse transform, whose chirp is conj(w)).  fac / W: factors and twiddles
+// of the FFT length (n or m).
+struct EnvPlan {
+    const float2 *W;
+    const int *fac;
+    int nfac, m;
+    const float2 *chirp, *bf, *bi;
+};
+
+// the DFT of buf[0 .. n-1] in place (buf holds m elements for Bluestein)
+__device__ void dft_lds(float2 *buf, int n, const EnvPlan &p, bool inv) {
+    if (p.m == 0) {
+        fft_lds(buf, n, p.fac, p.nfac, p.W, inv);
+        return;
+    }
+    for (int i = threadIdx.x; i < p.m; i += EB) {
+        float2 a = make_float2(0.0f, 0.0f);
+        if (i < n) {
+            float2 w = p.chirp[i];
+            if (inv) w.y = -w.y;
+            a = cmul(buf[i], w);
+        }
+        buf[i] = a;
+    }
+    __syncthreads();
+    fft_lds(buf, p.m, p.fac, p.nfac, p.W, false);
+    const float2 *B = inv ? p.bi : p.bf;
+    for (int i = threadIdx.x; i < p.m; i += EB) buf[i] = cmul(buf[i], B[i]);
+    __syncthreads();
+    fft_lds(buf, p.m, p.fac, p.nfac, p.W, true);
+    for (int i = threadIdx.x; i < n; i += EB) {
+        float2 w = p.chirp[i];
+        if (inv) w.y = -w.y;
+        buf[i] = cmul(buf[i], w);
+    }
+    __syncthreads();
+}
+
 // analytic signal in place (EnvelopeMisfit::hilbert + _hilbert mask,
 // misfit/envelope.h:4-28,87-93): bin 0 kept, positive bins doubled, the rest
 // zeroed, except that bin n/2+1 of an even length is left untouched.
-__device__ void analytic_lds(float2 *buf, int n, const int *fac, int nfac, const float2 *W) {
-    fft_lds(buf, n, fac, nfac, W, false);
+__device__ void analytic_lds(float2 *buf, int n, const EnvPlan &p) {
+    dft_lds(buf, n, p, false);
     for (int i = threadIdx.x; i < n; i += EB) {
         if (i == 0) continue;
         float2 h = buf[i];
@@ -212,7 +260,7 @@ __device__ void analytic_lds(float2 *buf, int n, const int *fac, int nfac, const
         buf[i] = h;
     }
     __syncthreads();
-    fft_lds(buf, n, fac, nfac, W, true);
+    dft_lds(buf, n, p, true);
 }
 
 template <typename T, typename Op>
@@ -231,8 +279,7 @@ __device__ T env_reduce(T v, Op op) {
 __global__ __launch_bounds__(EB) void k_envelope(const float *__restrict__ syn_all,
                                                  const float *__restrict__ obs_all,
                                                  float *__restrict__ adstf_all,
-                                                 const float2 *__restrict__ W,
-                                                 const int *__restrict__ fac, int nfac,
+                                                 const EnvPlan plan,
                                                  double *__restrict__ chi,
                                                  float *__restrict__ scratch, int nt, int nrec,
                                                  int ncomp, float sqrt_dt) {
@@ -248,7 +295,7 @@ __global__ __launch_bounds__(EB) void k_envelope(const float *__restrict__ syn_a
 
     for (int i = threadIdx.x; i < n; i += EB) ebuf[i] = make_float2(syn[i], 0.0f);  // copyR2C
     __syncthreads();
-    analytic_lds(ebuf, n, fac, nfac, W);
+    analytic_lds(ebuf, n, plan);
     float amax = 0.0f;
     for (int i = threadIdx.x; i < n; i += EB) {
         const float2 a = ebuf[i];
@@ -260,7 +307,7 @@ __global__ __launch_bounds__(EB) void k_envelope(const float *__restrict__ syn_a
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += EB) ebuf[i] = make_float2(obs[i], 0.0f);
     __syncthreads();
-    analytic_lds(ebuf, n, fac, nfac, W);
+    analytic_lds(ebuf, n, plan);
     amax = env_reduce<float>(amax, [](float x, float y) { return x > y ? x : y; });
     // max = max(0.05 * amax(E_syn), 0.05) with double literals (envelope.h:70-73)
     float mx = (float)((double)amax * 0.05);
@@ -279,7 +326,7 @@ __global__ __launch_bounds__(EB) void k_envelope(const float *__restrict__ syn_a
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += EB) ebuf[i] = make_float2(hsyn[i], 0.0f);
     __syncthreads();
-    analytic_lds(ebuf, n, fac, nfac, W);
+    analytic_lds(ebuf, n, plan);
     for (int i = threadIdx.x; i < n; i += EB) {
         const float ersd = ebuf[i].y / fn;
         ad[n - 1 - i] = etmp[i] * syn[i] - ersd;  // prepareEnvelopeSTF
@@ -288,22 +335,58 @@ __global__ __launch_bounds__(EB) void k_envelope(const float *__restrict__ syn_a
     if (threadIdx.x == 0) chi[((long)b * nrec + ir) * ncomp + c] = (double)((float)sqrt(ss) * sqrt_dt);
 }
 
+// radix-2 FFT in FP64 on the host (sign -1 forward), for the Bluestein
+// filter spectra
+static void fft_host(std::vector<std::complex<double>> &a) {
+    const size_t n = a.size();
+    for (size_t i = 1, j = 0; i < n; i++) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const double ang = -2.0 * M_PI / (double)len;
+        for (size_t i = 0; i < n; i += len)
+            for (size_t k = 0; k < len / 2; k++) {
+                const std::complex<double> w(std::cos(ang * (double)k), std::sin(ang * (double)k));
+                const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+                a[i + k] = u + v;
+                a[i + k + len / 2] = u - v;
+            }
+    }
+}
+
+// LDS elements one trace's transform needs
+static int env_lds_elems(const Ctx &c) { return c.env_m ? c.env_m : c.env_n; }
+
 static void envelope_setup(Ctx &c) {
     const int n = c.g.nt;
     if (c.env_n == n) return;
     INV_REQUIRE(n >= 2 && n <= ENV_MAXN, INV_ERR_UNSUPPORTED,
                 "envelope misfit supports 2 <= nt <= " + std::to_string(ENV_MAXN));
-    std::vector<int> fac;
-    int m = n;
-    for (int p : {4, 2, 3, 5, 7})
-        while (m % p == 0) {
-            fac.push_back(p);
-            m /= p;
-        }
-    INV_REQUIRE(m == 1, INV_ERR_UNSUPPORTED,
-                "envelope misfit needs nt with prime factors 2, 3, 5, 7 only (nt = " +
-                    std::to_string(n) + ")");
-    const size_t lds = (size_t)n * sizeof(float2);
+    auto factor = [](int len) {
+        std::vector<int> f;
+        for (int p : {4, 2, 3, 5, 7})
+            while (len % p == 0) {
+                f.push_back(p);
+                len /= p;
+            }
+        return std::make_pair(f, len);
+    };
+    auto [fac, rest] = factor(n);
+    int m = 0;  // Bluestein transform length, 0 for a direct one
+    if (rest != 1) {
+        m = 1;
+        while (m < 2 * n - 1) m <<= 1;
+        INV_REQUIRE(m <= ENV_MAXN, INV_ERR_UNSUPPORTED,
+                    "envelope misfit: nt = " + std::to_string(n) +
+                        " has a prime factor above 7 and is longer than " +
+                        std::to_string(ENV_MAXN / 2));
+        fac = factor(m).first;
+    }
+    const int len = m ? m : n;
+    const size_t lds = (size_t)len * sizeof(float2);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_envelope),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -313,26 +396,70 @@ static void envelope_setup(Ctx &c) {
                                                  " bytes of LDS to one workgroup (nt too long)"};
         }
     }
-    std::vector<float2> W(n);
-    for (int k = 0; k < n; k++) {
-        const double ang = -2.0 * M_PI * (double)k / (double)n;
+    std::vector<float2> W(len);
+    for (int k = 0; k < len; k++) {
+        const double ang = -2.0 * M_PI * (double)k / (double)len;
         W[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
     }
+    std::vector<float2> blue;  // chirp [n], bf [m], bi [m]
+    if (m) {
+        std::vector<std::complex<double>> w(n), bf(m), bi(m);
+        for (int k = 0; k < n; k++) {
+            // k^2 mod 2n keeps the angle small: exp(-i pi k^2 / n) has period 2n in k^2
+            const long q = (long)k * k % (2L * n);
+            const double ang = -M_PI * (double)q / (double)n;
+            w[k] = std::complex<double>(std::cos(ang), std::sin(ang));
+        }
+        for (int k = 0; k < n; k++) {
+            bf[k] = std::conj(w[k]);
+            bi[k] = w[k];
+            if (k) {
+                bf[m - k] = std::conj(w[k]);
+                bi[m - k] = w[k];
+            }
+        }
+        fft_host(bf);
+        fft_host(bi);
+        blue.resize((size_t)n + 2 * m);
+        for (int k = 0; k < n; k++) blue[k] = make_float2((float)w[k].real(), (float)w[k].imag());
+        for (int k = 0; k < m; k++) {
+            blue[n + k] = make_float2((float)(bf[k].real() / m), (float)(bf[k].imag() / m));
+            blue[n + m + k] = make_float2((float)(bi[k].real() / m), (float)(bi[k].imag() / m));
+        }
+    }
     INV_HIP(hipStreamSynchronize(c.stream));
-    c.env_w.ensure(n * sizeof(float2));
+    c.env_w.ensure(len * sizeof(float2));
     c.env_fac.ensure(fac.size() * sizeof(int) + 4);
-    INV_HIP(hipMemcpy(c.env_w.p, W.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    INV_HIP(hipMemcpy(c.env_w.p, W.data(), len * sizeof(float2), hipMemcpyHostToDevice));
     INV_HIP(hipMemcpy(c.env_fac.p, fac.data(), fac.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (m) {
+        c.env_blue.ensure(blue.size() * sizeof(float2));
+        INV_HIP(hipMemcpy(c.env_blue.p, blue.data(), blue.size() * sizeof(float2),
+                          hipMemcpyHostToDevice));
+    }
     c.env_nfac = (int)fac.size();
+    c.env_m = m;
     c.env_n = n;
+}
+
+static EnvPlan env_plan(Ctx &c) {
+    EnvPlan p{c.env_w.as<float2>(), c.env_fac.as<int>(), c.env_nfac, c.env_m, nullptr, nullptr,
+              nullptr};
+    if (c.env_m) {
+        const float2 *b = c.env_blue.as<float2>();
+        p.chirp = b;
+        p.bf = b + c.env_n;
+        p.bi = b + c.env_n + c.env_m;
+    }
+    return p;
 }
 
 void envelope_trace(Ctx &c, const float *syn, const float *obs, float *adstf, double *chi_dev) {
     envelope_setup(c);
     c.env_scratch.ensure(3 * (size_t)c.g.nt * sizeof(float));
-    k_envelope<<<dim3(1, 1), EB, (size_t)c.g.nt * sizeof(float2), c.stream>>>(
-        syn, obs, adstf, c.env_w.as<float2>(), c.env_fac.as<int>(), c.env_nfac, chi_dev,
-        c.env_scratch.as<float>(), c.g.nt, 1, 1, sqrtf(c.g.dt));
+    k_envelope<<<dim3(1, 1), EB, (size_t)env_lds_elems(c) * sizeof(float2), c.stream>>>(
+        syn, obs, adstf, env_plan(c), chi_dev, c.env_scratch.as<float>(), c.g.nt, 1, 1,
+        sqrtf(c.g.dt));
     INV_HIP(hipGetLastError());
 }
 
@@ -341,10 +468,10 @@ void misfit_envelope(Ctx &c, int count, int first) {
     const long tr = (long)c.ncomp * c.nrec * c.g.nt;
     c.env_scratch.ensure(3 * (size_t)count * tr * sizeof(float));
     dim3 grid(c.ncomp * c.nrec, count);
-    k_envelope<<<grid, EB, (size_t)c.g.nt * sizeof(float2), c.stream>>>(
+    k_envelope<<<grid, EB, (size_t)env_lds_elems(c) * sizeof(float2), c.stream>>>(
         c.out.as<float>(), c.obs.as<float>() + (long)first * tr, c.adstf.as<float>(),
-        c.env_w.as<float2>(), c.env_fac.as<int>(), c.env_nfac, c.chi.as<double>(),
-        c.env_scratch.as<float>(), c.g.nt, c.nrec, c.ncomp, sqrtf(c.g.dt));
+        env_plan(c), c.chi.as<double>(), c.env_scratch.as<float>(), c.g.nt, c.nrec, c.ncomp,
+        sqrtf(c.g.dt));
     INV_HIP(hipGetLastError());
 }
 

@@ -113,12 +113,14 @@ def test_vector_ops(gpu):
     ctx.close()
 
 
-@pytest.mark.parametrize("nt", [300, 320, 375, 2000])
+@pytest.mark.parametrize("nt", [300, 320, 375, 2000, 311, 1130])
 def test_misfit_envelope_matches_oracle(gpu, nt):
     """Envelope adjoint sources and misfits against the oracle (FP64 Hilbert
-    transform): the GPU transform is FP32, so agreement is to tolerance."""
+    transform): the GPU transform is FP32, so agreement is to tolerance.
+    311 (prime) and 1130 (2 5 113) go through Bluestein transforms of length
+    1024 and 4096."""
     import pyoracle
-    P = inv.problems.tiny(sh=1, psv=1, nsrc=2, nt=nt, sfe=5)
+    P = inv.problems.tiny(sh=1, psv=1, nsrc=2, nt=nt, sfe=5 if nt % 5 == 0 else 1)
     obs = _obs_from_true(P)
     ctx = P.context(batch=2)
     for s in range(P.nsrc):
@@ -144,12 +146,37 @@ def test_misfit_envelope_matches_oracle(gpu, nt):
 
 
 def test_envelope_rejects_unsupported_length(gpu):
-    P = inv.problems.tiny(sh=1, psv=0, nsrc=1, nt=311, sfe=1)  # 311 is prime
-    ctx = P.context(batch=1)
+    """Bluestein transforms need 2 nt - 1 <= 16384 points of LDS: a prime nt
+    above 8192 is refused (a 7-smooth one up to 16384 is not)."""
+    P = inv.problems.tiny(sh=1, psv=0, nsrc=1, nx=24, nz=20, nrec=2, nt=8209, sfe=1)
+    ctx = P.context(batch=1, adj=0)
     ctx.forward(0, 1, snapshots=False)
     with pytest.raises(inv.invhip.InvError) as e:
         ctx.misfit(0, 1, kind=1)
     assert e.value.code == -5
+    ctx.close()
+
+
+@pytest.mark.parametrize("nt", [8190, 4099])
+def test_envelope_long_bluestein_trace(gpu, nt):
+    """The longest Bluestein transform (8190 = 2 3^2 5 7 13: length 16384)
+    and a prime length on one device trace against the FP64 oracle."""
+    import pyoracle
+    P = inv.problems.tiny(sh=1, psv=0, nsrc=1, nx=24, nz=20, nrec=2, nt=nt, sfe=1)
+    ctx = P.context(batch=1, adj=0)
+    rng = np.random.default_rng(nt)
+    t = np.arange(nt) * P.dt
+    syn = (np.sin(40 * t) * np.exp(-3 * t) + 0.1 * rng.standard_normal(nt)).astype(np.float32)
+    obs = (np.sin(41 * t) * np.exp(-3 * t) + 0.1 * rng.standard_normal(nt)).astype(np.float32)
+    d = [ctx.alloc(4 * nt) for _ in range(3)]
+    ctx.h2d(d[0], syn)
+    ctx.h2d(d[1], obs)
+    chi = ctx.misfit_trace(1, d[0], d[1], d[2])
+    x, a = pyoracle.envelope(syn.astype(np.float64), obs.astype(np.float64), P.dt, np.float64)
+    e = rel(ctx.d2h(d[2], nt), a)
+    print(f"nt={nt}: envelope adjoint source rel err vs FP64 {e:.2e}, misfit {abs(chi - x) / x:.2e}")
+    assert abs(chi - x) <= 1e-4 * x
+    assert e < 1e-4
     ctx.close()
 
 
