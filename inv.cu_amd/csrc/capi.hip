@@ -464,24 +464,14 @@ int inv_model_from_points(inv_ctx *ctx, int npt, const float *px, const float *p
                 r[k] = prho[hit[k]];
             }
         } else {
-            // general case: nearest point by brute force, as mesh2grid
-            const float dmax = xmax * xmax + zmax * zmax;
-            for (int i = 0; i < nx; i++)
-                for (int j = 0; j < nz; j++) {
-                    const long k = (long)i * nz + j;
-                    const float x = i * dx, z = j * dz;
-                    float dmin = dmax;
-                    for (int p = 0; p < npt; p++) {
-                        const float ex = x - px[p], ez = z - pz[p];
-                        const float d = ex * ex + ez * ez;
-                        if (d < dmin) {
-                            dmin = d;
-                            l[k] = p1[p];
-                            m[k] = p2[p];
-                            r[k] = prho[p];
-                        }
-                    }
-                }
+            // general case: the nearest point of each node, on the device
+            // (the same choice as mesh2grid's scan, model.hip)
+            sync(c);
+            model_nearest(c, npt, px, pz, p1, p2, prho, xmax * xmax + zmax * zmax);
+            const size_t b = c.N * sizeof(float);
+            INV_HIP(hipMemcpy(l.data(), c.lam.p, b, hipMemcpyDeviceToHost));
+            INV_HIP(hipMemcpy(m.data(), c.mu.p, b, hipMemcpyDeviceToHost));
+            INV_HIP(hipMemcpy(r.data(), c.rho.p, b, hipMemcpyDeviceToHost));
         }
         sync(c);
         if (model_type == 0) {

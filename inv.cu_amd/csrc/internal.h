@@ -291,6 +291,11 @@ double vec_dot(Ctx &c, const float *a, const float *b, size_t n);
 float vec_amax(Ctx &c, const float *a, size_t n);
 void sum_f64(Ctx &c, const double *x, size_t n, double *dev_out);
 
+// mesh2grid on the device (model.hip): the nearest of npt host points for
+// every node into c.lam / c.mu / c.rho (values as given, before vps2lm)
+void model_nearest(Ctx &c, int npt, const float *px, const float *pz, const float *p1,
+                   const float *p2, const float *prho, float dmax);
+
 // Exponent tables of 16-bit snapshots (stencil.h).  A row segment is the
 // owned columns of one tile of the kernel that writes the slot: 58 (k_col),
 // 120 / 248 (k_colv, 2 / 4 columns per lane), 116 / 240 (k_pair, 2 / 4).

@@ -255,6 +255,15 @@ class Context:
         self._chk(self.L.inv_model_download_lmr(self.h, *(_p(a) for a in out)))
         return out
 
+    def model_from_points(self, px, pz, p1, p2, prho, xmax, zmax, model_type=0):
+        """mesh2grid (+ vps2lm for model_type 0) of npt model points
+        (inv_model_from_points)."""
+        f = lambda a: np.ascontiguousarray(a, np.float32).ravel()
+        px, pz, p1, p2, prho = f(px), f(pz), f(p1), f(p2), f(prho)
+        self._chk(self.L.inv_model_from_points(self.h, len(px), _p(px), _p(pz), _p(p1), _p(p2),
+                                               _p(prho), float(xmax), float(zmax),
+                                               int(model_type)))
+
     def sources_set(self, ix, iz, stf_x, stf_y, stf_z):
         ix = np.ascontiguousarray(ix, np.int32)
         iz = np.ascontiguousarray(iz, np.int32)
