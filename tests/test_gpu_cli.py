@@ -104,7 +104,7 @@ def full(tmp_path_factory, gpu):
     return make_example(tmp_path_factory.mktemp("checker"))
 
 
-@pytest.mark.parametrize("math,tol", [(1, 1e-4), (0, 1e-5)])
+@pytest.mark.parametrize("math,tol", [(1, 1e-5), (0, 1e-5)])
 def test_mode3_traces_match_reference_golden(full, math, tol):
     run_cli(full, ("mode", 3), ("math", math))
     d = np.load(FIX)
