@@ -65,11 +65,11 @@ SH_STEP_BYTES = 4 * (2 * 4 + 2)
 # shots per launch):
 #   c1 one-step 4 x 2, 16 shots:  68.30 B (algorithmic 68),
 #                                 profiles/k_colv_psv_marmousi737x240_16shots.md
-#   c4 two-step 20 x 2, 4 shots:  71.51 B per launch of two steps (algorithmic 68),
+#   c4 two-step 20 x 2, 4 shots:  70.75 B per launch of two steps (algorithmic 68),
 #                                 profiles/k_pair_psv_4096x2048_4shots.md
 #   c2 two-step 16 x 2, 64 shots: 40.36 B per launch of two steps (algorithmic 40),
 #                                 profiles/k_pair_sh_marmousi1474x480_64shots.md
-MEASURED_TRAFFIC_PER_CELL = {("c1", 1, 4, 2, 16): 68.30, ("c4", 2, 20, 2, 4): 71.51,
+MEASURED_TRAFFIC_PER_CELL = {("c1", 1, 4, 2, 16): 68.30, ("c4", 2, 20, 2, 4): 70.75,
                              ("c2", 2, 16, 2, 64): 40.36}
 
 
