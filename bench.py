@@ -111,6 +111,9 @@ def parse(argv=None):
                     help="two-step kernel tile rows (-1: the autotuner decides, 0: one step "
                          "per launch)")
     ap.add_argument("--pair-cols", type=int, default=2, help="two-step kernel columns per lane")
+    ap.add_argument("--persistent", type=int, default=-1,
+                    help="persistent two-step kernel: 1 on (two-step tile 8 x 4), 0 off, -1 as "
+                         "the autotuner chose")
     ap.add_argument("--math", type=int, default=1, help="0 exact (IEEE divisions), 1 fast")
     ap.add_argument("--graphs", type=int, default=1, help="capture time loops in HIP graphs")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -163,10 +166,16 @@ def configure_kernel(ctx, a, count, pair=True):
         ctx.set_pair(0)
     elif a.pair_rows >= 0:
         ctx.set_pair(a.pair_rows, a.pair_cols)
+    if a.persistent == 1:
+        ctx.set_pair(8, 4)
+    if a.persistent >= 0:
+        ctx.set_persistent(a.persistent)
     tile = {"variant": v, "rows": tx, "cols_per_lane": cols}
     pr, pv = ctx.get_pair()
     if pr:
         tile["two_step"] = {"rows": pr, "cols_per_lane": pv}
+        if ctx.get_persistent():
+            tile["two_step"]["persistent"] = True
     return tile
 
 
@@ -323,7 +332,10 @@ def roofline(ctx, inv, a, name, P, B, tile, fmt, ckpt, step_bytes):
     if nfr and uniform:
         per_launch_steps = 2 if two else 1
         launch_s = tfr / (nfr / per_launch_steps) / 1e3
-        kernel = (f"two-step {waves} kernel k_pair, from forward-graph replays" if two else
+        res = two and tile["two_step"].get("persistent")
+        kernel = (f"persistent two-step {waves} kernel k_res (one launch per forward), from "
+                  "forward-graph replays" if res else
+                  f"two-step {waves} kernel k_pair, from forward-graph replays" if two else
                   f"fused {waves} time step (k_col / k_colv, autotuned tile), from forward-graph "
                   "replays")
         avg["two_step_launch" if two else "step"] = round(launch_s * 1e6, 2)

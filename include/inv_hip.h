@@ -257,7 +257,8 @@ enum inv_prof_kernel {
     INV_PROF_STEP = 3,      /* whole time step (fused variant) */
     INV_PROF_FWD_RUN = 4,   /* replayed forward graphs (count = time steps) */
     INV_PROF_ADJ_RUN = 5,   /* replayed adjoint graphs (count = time steps) */
-    INV_PROF_PAIR = 6       /* two-step launches (count = launches) */
+    INV_PROF_PAIR = 6,      /* two-step launches (count = launches) */
+    INV_PROF_RESIDENT = 7   /* persistent launches (count = time steps) */
 };
 int inv_prof_enable(inv_ctx *ctx, int every);
 int inv_prof_reset(inv_ctx *ctx);
@@ -291,6 +292,22 @@ int inv_set_pair(inv_ctx *ctx, int tile_x, int cols_per_lane);
 /* The two-step tile in use (tile_x 0: off), as set by inv_set_pair or chosen
  * by inv_autotune. */
 int inv_get_pair(inv_ctx *ctx, int *tile_x, int *cols_per_lane);
+/* Persistent two-step kernel (mode 1; 0 off, the default): one launch runs
+ * every pair of steps of a forward, or of an adjoint between two kernel
+ * accumulations, on tiles of 4 or 8 rows spanning all nz columns, one
+ * workgroup per tile.  A tile's wavefields stay in LDS and registers between
+ * steps; only halo rows cross memory, exchanged with the tiles the halo
+ * reaches through tagged words (no grid-wide barrier).  For launch-bound
+ * grids: needs the two-step tile 8 x 4 or 4 x 4 (inv_set_pair(ctx, 8, 4)),
+ * nz <= 240 and divisible by 4, no checkpoints, and every tile of a launch
+ * resident on the device at once (shots x ceil(nx/rows) workgroups;
+ * INV_ERR_UNSUPPORTED at run time otherwise).  Results are identical to those of inv_set_pair alone.  A wait
+ * for a neighbour gives up after a bounded time instead of hanging; the run
+ * then ends with INV_ERR_HIP at the next call that synchronises.  Mode 2 (a
+ * test hook) gives up every wait at once.  No counterpart in the reference
+ * (one kernel launch per stencil half-step, solver/fdm.h:534-551). */
+int inv_set_persistent(inv_ctx *ctx, int mode);
+int inv_get_persistent(inv_ctx *ctx, int *mode);
 /* Autotuner: times `steps` forward steps of shots first..first+count-1 (with
  * snapshots and as many adjoint steps when the context keeps per-slot kernel
  * accumulators, batch > 1) with each tile of inv_set_tile that nz allows and

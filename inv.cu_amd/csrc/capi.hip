@@ -10,6 +10,7 @@
 #include <new>
 
 #include "internal.h"
+#include "step.h"
 
 namespace inv {
 
@@ -161,7 +162,23 @@ static void check_slots(Ctx &c, int first, int count) {
     }
 }
 
-static void sync(Ctx &c) { INV_HIP(hipStreamSynchronize(c.stream)); }
+// Waits for the context's stream; a persistent-kernel launch that gave up
+// waiting for a neighbouring tile (its tiles were not all resident) left its
+// error word set: reported here, and cleared.
+static void sync(Ctx &c) {
+    INV_HIP(hipStreamSynchronize(c.stream));
+    if (c.res_ctl.p) {
+        unsigned w[2] = {0, 0};
+        INV_HIP(hipMemcpy(w, c.res_ctl.p, sizeof w, hipMemcpyDeviceToHost));
+        if (w[1]) {
+            w[1] = 0;
+            INV_HIP(hipMemcpy(c.res_ctl.p, w, sizeof w, hipMemcpyHostToDevice));
+            throw Error{INV_ERR_HIP,
+                        "persistent kernel: a tile gave up waiting for a neighbouring tile's "
+                        "halo rows; the run's wavefields are invalid"};
+        }
+    }
+}
 
 static void drop_graphs(Ctx &c) {
     (void)hipStreamSynchronize(c.stream);
@@ -187,7 +204,7 @@ std::vector<DevBuf *> ctx_buffers(Ctx &c) {
             &c.vz_b, &c.sxx_b, &c.szz_b, &c.sxz_b, &c.vy_b, &c.sxy_b, &c.szy_b, &c.f_rec_v.ptr,
             &c.f_rec_v.cell, &c.f_rec_v.row, &c.f_rec_t.ptr, &c.f_rec_t.cell, &c.f_rec_t.row,
             &c.env_w, &c.env_fac, &c.env_scratch, &c.env_blue, &c.bdt,
-            &c.slot_model};
+            &c.slot_model, &c.xbuf, &c.res_ctl};
 }
 
 size_t ctx_bytes_held(Ctx &c) {
@@ -1221,6 +1238,22 @@ int inv_set_pair(inv_ctx *ctx, int tile_x, int cols_per_lane) {
     });
 }
 
+int inv_set_persistent(inv_ctx *ctx, int mode) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(mode >= 0 && mode <= 2, INV_ERR_ARG, "persistent mode must be 0, 1 or 2");
+        sync(c);
+        c.persist = mode;
+        invalidate(c);
+    });
+}
+
+int inv_get_persistent(inv_ctx *ctx, int *mode) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(mode, INV_ERR_ARG, "null output");
+        *mode = c.persist;
+    });
+}
+
 int inv_get_pair(inv_ctx *ctx, int *tile_x, int *cols_per_lane) {
     return guard(ctx, [&](Ctx &c) {
         INV_REQUIRE(tile_x && cols_per_lane, INV_ERR_ARG, "null output");
@@ -1236,6 +1269,7 @@ namespace {
 struct TuneChoice {
     int variant, tile, cols;
     int pair_tx = 0, pair_v = 2;  // two-step kernel tile (0 rows: off)
+    int persist = 0;              // persistent two-step kernel
 };
 std::mutex tune_mu;
 std::map<std::string, TuneChoice> tune_cache;
@@ -1246,6 +1280,7 @@ void apply_choice(Ctx &c, const TuneChoice &t) {
     c.col_v = t.cols;
     c.pair_tx = t.pair_tx;
     c.pair_v = t.pair_v;
+    c.persist = t.persist;
     invalidate(c);
     fused_invalidate_lists(c);
 }
@@ -1320,9 +1355,10 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                     }
                     if (verbose)
                         fprintf(stderr,
-                                "inv_autotune %s: %d rows x %d cols per lane, two-step %d x %d: "
+                                "inv_autotune %s: %d rows x %d cols per lane, two-step %d x %d%s: "
                                 "%.3f ms\n",
-                                key.c_str(), t.tile, t.cols, t.pair_tx, t.pair_v, best_t);
+                                key.c_str(), t.tile, t.cols, t.pair_tx, t.pair_v,
+                                t.persist ? " persistent" : "", best_t);
                     return best_t;
                 };
                 // one-step tiles, in order of preference (one column per
@@ -1360,6 +1396,28 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                             best = t;
                         }
                     }
+                // then the persistent kernel (grids one tile wide, whose
+                // tiles all fit the device at once), over the same one-step
+                // tile
+                for (int tx : {8, 4}) {
+                    if (c.g.nz % RES_V || c.g.nz > pair_strip(RES_V) || c.ckpt) break;
+                    TuneChoice t = single;
+                    t.pair_tx = tx;
+                    t.pair_v = RES_V;
+                    t.persist = 1;
+                    try {
+                        const float ms = timed(t, false);
+                        sync(c);  // a wait that gave up is reported here
+                        if (ms < 0.98 * best_ms) {
+                            best_ms = ms;
+                            best = t;
+                        }
+                    } catch (const Error &e) {
+                        // not resident at once on this device, or a wait gave
+                        // up: not a candidate
+                        if (e.code != INV_ERR_UNSUPPORTED && e.code != INV_ERR_HIP) throw;
+                    }
+                }
             } catch (...) {
                 c.prof.every = every;
                 if (e0) (void)hipEventDestroy(e0);

@@ -955,8 +955,12 @@ static void snap_roles(const SnapPlan &s, StepArgs &a, long it) {
     }
 }
 
+static bool res_on(const Ctx &c);
+static void res_prepare(Ctx &c, int count);
+
 void fused_prepare(Ctx &c, int kind, int first, int count, int flags) {
     ensure_second_set(c);
+    if (res_on(c)) res_prepare(c, count);
     ensure_rec_lists(c);
     // snapshot storage only for runs that store or read snapshots
     if (kind == 2 || (flags & INV_FWD_SNAPSHOTS)) snap_storage(c);
@@ -1141,6 +1145,76 @@ static StepArgs pair_args(Ctx &c, const StepArgs &a, int first, int count, bool 
     return p;
 }
 
+// Persistent runs (fdm_resident.hip): every pair of steps of a forward, or
+// of an adjoint between kernel accumulations, in one launch.
+static bool res_on(const Ctx &c) { return c.persist != 0 && pair_on(c); }
+
+static void res_require(const Ctx &c) {
+    INV_REQUIRE((c.pair_tx == 4 || c.pair_tx == 8) && c.pair_v == RES_V, INV_ERR_UNSUPPORTED,
+                "the persistent kernel runs the two-step tiles of 4 or 8 rows x 4 columns per "
+                "lane");
+    INV_REQUIRE(c.g.nz <= pair_strip(RES_V) && c.g.nz % RES_V == 0, INV_ERR_UNSUPPORTED,
+                "the persistent kernel holds one tile across z: nz <= 240 and divisible by 4");
+    INV_REQUIRE(c.ckpt == 0, INV_ERR_UNSUPPORTED,
+                "the persistent kernel does not run checkpointed snapshots");
+}
+
+static void res_prepare(Ctx &c, int count) {
+    res_require(c);
+    if (!c.res_ctl.p) {
+        c.res_ctl.ensure(2 * sizeof(unsigned));
+        const unsigned z[2] = {0, 0};
+        INV_HIP(hipMemcpy(c.res_ctl.p, z, sizeof z, hipMemcpyHostToDevice));
+    }
+    const size_t need = res_xbuf_bytes(count, c.g.nx, c.pair_tx);
+    if (c.xbuf.bytes < need) {
+        c.xbuf.ensure(need);
+        // tag 0 is never expected: a fresh buffer holds no valid word
+        fill_dev(c, c.xbuf.p, (long)(c.xbuf.bytes / sizeof(float)));
+    }
+}
+
+template <bool FAST>
+static void res_m(Ctx &c, const StepArgs &a, const ResArgs &ra, unsigned blocks) {
+    // SH and P-SV are independent systems: one launch each
+    if (c.g.sh) res_launch<true, false, FAST>(c, a, ra, blocks);
+    if (c.g.psv) res_launch<false, true, FAST>(c, a, ra, blocks);
+}
+
+// iters iterations (steps it .. it + 2 iters - 1) from the current state set
+// into the other one; snapshot roles from sp when it is on
+static void res_run(Ctx &c, StepArgs &a2, const SnapPlan &sp, const Fields &F, int count,
+                    int it, int iters) {
+    set_state_ptrs(a2, F, c.parity);
+    a2.it = it;
+    set_roles(a2, StepRoles{nullptr, nullptr});
+    ResArgs ra{};
+    ra.xbuf = c.xbuf.as<unsigned long long>();
+    ra.ctl = c.res_ctl.as<unsigned>();
+    ra.iters = iters;
+    ra.spin_limit = c.persist == 2 ? -1 : RES_SPIN;
+    if (sp.on) {
+        ra.snap = sp.snap;
+        ra.sexp = sp.sexp;
+        ra.slot_bytes = sp.slot_bytes;
+        ra.tab_bytes = sp.tab_bytes;
+        ra.sfe = sp.sfe;
+        ra.nsfe = sp.nsfe;
+        ra.nslots = sp.nslots;
+        ra.sbase = sp.base;
+    }
+    INV_REQUIRE(c.xbuf.bytes >= res_xbuf_bytes(count, c.g.nx, c.pair_tx) && c.res_ctl.p, INV_ERR_STATE,
+                "persistent kernel buffers not prepared");
+    const unsigned blocks = (unsigned)(count * a2.tiles_x);
+    hipEvent_t ev;
+    const bool p = prof_begin_any(c, &ev);
+    if (c.math == INV_MATH_FAST) res_m<true>(c, a2, ra, blocks);
+    else res_m<false>(c, a2, ra, blocks);
+    INV_HIP(hipGetLastError());
+    if (p) prof_end(c, PROF_RES, ev, 2 * iters);
+    c.parity ^= 1;
+}
+
 void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
     fused_prepare(c, 1, first, count, flags);
     StepArgs a;
@@ -1159,6 +1233,20 @@ void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
     const bool pr = pair_on(c);
     StepArgs a2;
     if (pr) a2 = pair_args(c, a, first, count, false, true);
+    if (res_on(c)) {
+        // every pair of steps in persistent launches, an odd last step alone
+        int it = it0;
+        while (it + 1 < it1) {
+            const int K = std::min((it1 - it) / 2, RES_KMAX);
+            res_run(c, a2, sp, F, count, it, K);
+            it += 2 * K;
+        }
+        if (it < it1) {
+            snap_roles(sp, a, it);
+            fused_step(c, a, F, c.parity, count, it);
+        }
+        return;
+    }
     for (int it = it0; it < it1;) {
         if (sp.on && c.ckpt > 0 && it > 0 && it % c.ckpt == 0) {
             float *cur[11];
@@ -1247,8 +1335,25 @@ void fused_adjoint(Ctx &c, int first, int count, int it0, int it1) {
     const bool pr = pair_on(c) && c.g.sfe > 1;
     StepArgs a2;
     if (pr) a2 = pair_args(c, a, first, count, true, false);
+    const bool res = res_on(c) && pr;
     for (int it = it0; it < it1;) {
         const bool two = pr && it + 1 < it1 && it % c.g.sfe != 0;
+        if (two && res) {
+            // the pairs up to the next kernel accumulation in one launch
+            int iters = 0, nxt = it, last = it;
+            while (nxt + 1 < it1 && nxt % c.g.sfe != 0 && iters < RES_KMAX) {
+                iters++;
+                last = nxt + 1;
+                nxt += 2;
+                if (last % c.g.sfe == 0) break;
+            }
+            res_run(c, a2, SnapPlan{}, F, count, it, iters);
+            if (last % c.g.sfe == 0)
+                fdm_interact(c, count, last, last / c.g.sfe, cur_field(c, c.vx, c.vx_b),
+                             cur_field(c, c.vz, c.vz_b), cur_field(c, c.vy, c.vy_b));
+            it = last + 1;
+            continue;
+        }
         const int last = two ? it + 1 : it;
         if (K > 0 && last > 0 && last % K == 0)
             replay_segment(c, first, count, S - 1 - last / K, last / K);

@@ -1,0 +1,501 @@
+// Persistent two-step kernel for grids whose shots fit the chip at one tile
+// per workgroup.  One launch runs `iters` iterations of the two-step kernel's
+// phases (k_pair, fdm_pair.hip: divS -> addSTF -> updateV -> divV -> updateS
+// -> saveRec twice, solver/fdm.h:534-551) on tiles of TX rows spanning every
+// column of the grid.  Between iterations a tile's wavefields stay on the
+// chip: the stress in LDS, the velocity, displacement and model in
+// registers.  Only the rows a neighbouring tile reads as its halo cross
+// memory, through an exchange buffer of tagged 64-bit words (the value's bits
+// and the iteration's tag, stored and loaded as relaxed device-scope
+// atomics): a tile publishes its owned rows of stress and velocity at the end
+// of an iteration and, at the start of the next, reads its halo rows (six
+// each side) from the tiles that own them (the neighbour on each side for
+// 8-row tiles, two on each side for 4-row tiles), waiting until every word
+// carries the tag it expects.  A word is read only when its tag is right, so
+// no fence orders the value against a separate flag; a tile waits only for
+// the tiles its halo reaches, never for the whole grid.
+//
+// Two buffers alternate by iteration parity.  The tiles a halo reaches are
+// the tiles that read it (the relation is symmetric), and a tile cannot
+// finish iteration k+1 before each of them has published iteration k, which
+// they do after reading its iteration k-1 rows: the buffer it overwrites is
+// never still being read.  Tags are epoch * 8192 + k + 1 with the epoch
+// counted up on the device before every launch (graph replays included), so
+// a word left over from another launch never matches.
+//
+// Every tile waits only for tiles that are running: the host launches the
+// kernel only when the whole grid fits the device at once (occupancy), and
+// every wait gives up after a bounded number of polls, setting an error word
+// the host turns into an error, so that a grid which is not co-resident ends
+// instead of hanging.
+//
+// Results are bitwise those of the same number of k_pair launches: every cell
+// is evaluated by the row functions of colv.h, and a halo row holds what
+// k_pair would load from memory: the owning tile's value inside the grid,
+// 0 for stress outside it, the value at the clamped row and column for
+// velocity outside it.  The last iteration stores the state as k_pair does.
+#include "colv.h"
+
+namespace inv {
+
+// exchanged fields of a tile row: s1, s2, s3 (stress) and v1, v2 (velocity)
+constexpr int RES_NF = 5;
+
+__device__ __forceinline__ void xput(unsigned long long *q, float v, unsigned tag) {
+    const unsigned long long w =
+        ((unsigned long long)tag << 32) | (unsigned long long)__float_as_uint(v);
+    __hip_atomic_store(q, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long xload(unsigned long long *q) {
+    return __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// a loaded word's value into v; true when it carries the expected tag
+__device__ __forceinline__ bool xtake(unsigned long long w, unsigned tag, float &v) {
+    v = __uint_as_float((unsigned)(w & 0xffffffffull));
+    return (unsigned)(w >> 32) == tag;
+}
+
+// snapshot role of forward step s (SnapPlan::local in fdm_fused.hip)
+__device__ __forceinline__ StepRoles res_role(const ResArgs &ra, long s) {
+    if (!ra.snap || (s + 1) % ra.sfe != 0) return StepRoles{nullptr, nullptr};
+    const long l = ra.nsfe - (s + 1) / ra.sfe - ra.sbase;
+    if (l < 0 || l >= ra.nslots) return StepRoles{nullptr, nullptr};
+    return StepRoles{ra.snap + l * ra.slot_bytes, ra.sexp ? ra.sexp + l * ra.tab_bytes : nullptr};
+}
+
+__global__ void k_res_epoch(unsigned *ctl) { ctl[0] += 1u; }
+
+template <bool SH, bool PSV, int TX, int V, int NW, bool FAST, bool SNAP, bool REC>
+__global__ __launch_bounds__(64 * NW, 1) void k_res(const StepArgs a, const ResArgs ra) {
+    static_assert(SH != PSV, "one wave type per launch");
+    static_assert(TX % 4 == 0, "tile rows a multiple of four");
+    constexpr int HL = pair_halo(V), W = 64 * V, S = pair_strip(V);
+    constexpr int SR = TX + 12;  // stress rows x = -6 .. TX+5
+    constexpr int VR = TX + 9;   // velocity rows x = -4 .. TX+4
+    constexpr int PS = (SR + NW - 1) / NW;
+    constexpr int P = (VR + NW - 1) / NW;
+    extern __shared__ __align__(16) float smem[];
+    __shared__ int l_abort;
+    float *const l_s1 = smem;
+    float *const l_s2 = l_s1 + SR * W;
+    float *const l_s3 = PSV ? l_s2 + VR * W : l_s2;
+    float *const l_v1 = PSV ? l_s3 + SR * W : l_s2 + VR * W;
+    float *const l_v2 = PSV ? l_v1 + VR * W : l_v1;
+
+    // one tile across z: tile tx of shot slot b (XCD-aware order, as k_pair)
+    const int nw = gridDim.x, w = blockIdx.x;
+    const int q = nw >> 3, rmd = nw & 7, xcd = w & 7;
+    const int L = xcd * q + (xcd < rmd ? xcd : rmd) + (w >> 3);
+    const int per = a.tiles_x;
+    const int b = L / per;
+    const int tx = L - b * per;
+    const int i0 = tx * TX;
+    const Geo &g = a.g;
+    const int nx = g.nx, nz = g.nz;
+    const long so = (long)b * g.N;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c0 = lane * V;
+    const int jb = -HL + c0;
+    const int jl = jb < 0 ? 0 : (jb > nz - V ? nz - V : jb);
+    const bool jin = jb >= 0 && jb < nz;
+    const bool cown = jin && c0 >= HL && c0 < HL + S;
+    const int csrc = jl + HL;  // LDS column of grid column jl (the lane's own when jin)
+    const ZCoef<V> zc = zcoef<V>(g, jb);
+    auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
+    const float *m_rho = a.rho, *m_lam = a.lam, *m_mu = a.mu;
+    if (a.slot_model) {
+        m_rho = a.slot_model[3 * b];
+        m_lam = a.slot_model[3 * b + 1];
+        m_mu = a.slot_model[3 * b + 2];
+    }
+    const float *S1 = PSV ? a.sxx0 : a.sxy0, *S2 = PSV ? a.szz0 : a.szy0, *S3 = a.sxz0;
+    const float *V1 = PSV ? a.vx0 : a.vy0, *V2 = a.vz0;
+    float *U1 = PSV ? a.ux : a.uy, *U2 = a.uz;
+    float *V1o = PSV ? a.vx1 : a.vy1, *V2o = a.vz1;
+    float *S1o = PSV ? a.sxx1 : a.sxy1, *S2o = PSV ? a.szz1 : a.szy1, *S3o = a.sxz1;
+
+    // the state at the launch's first step, as k_pair loads it
+    float s1[PS][V], s2[PS][V], s3[PS][V];
+#pragma unroll
+    for (int p = 0; p < PS; p++) {
+        const int r = wave + p * NW;
+        const long k = so + (long)clampi(i0 - 6 + r, 0, nx - 1) * nz + jl;
+        vload<V>(S1 + k, s1[p]);
+        if (r >= 2 && r < SR - 1) vload<V>(S2 + k, s2[p]);
+        if (PSV) vload<V>(S3 + k, s3[p]);
+    }
+    float qv1[P][V], qv2[P][V], qr[P][V], qu1[P][V], qu2[P][V], qmu[P][V], qlam[P][V], qab[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int x = wave - 4 + p * NW;
+        const int ic = clampi(i0 + x, 0, nx - 1);
+        const long kc = (long)ic * nz + jl;
+        if (x < TX + 5) {
+            vload_g<V>(m_rho + kc, qr[p]);
+            vload<V>(V1 + so + kc, qv1[p]);
+            if (PSV) vload<V>(V2 + so + kc, qv2[p]);
+            qab[p] = vload_uniform(a.abx, ic);
+        }
+        if (x >= 0 && x < TX) {
+            vload<V>(U1 + so + kc, qu1[p]);
+            if (PSV) vload<V>(U2 + so + kc, qu2[p]);
+        }
+        if (x >= -3 && x < TX + 3) {
+            vload_g<V>(m_mu + kc, qmu[p]);
+            if (PSV) vload_g<V>(m_lam + kc, qlam[p]);
+        }
+    }
+    float abz[V], abt[V];
+    vload<V>(a.abzb + jl, abz);
+    vload<V>(a.abzt + jl, abt);
+#pragma unroll
+    for (int p = 0; p < PS; p++) {
+        const int r = wave + p * NW;
+        if (r < SR) {
+            const int i = i0 - 6 + r;
+            if (!(jin && i >= 0 && i < nx)) {
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    s1[p][v] = 0.0f;
+                    s2[p][v] = 0.0f;
+                    s3[p][v] = 0.0f;
+                }
+            }
+            vstore<V>(&l_s1[r * W + c0], s1[p]);
+            if (r >= 2 && r < SR - 1) vstore<V>(&l_s2[(r - 2) * W + c0], s2[p]);
+            if (PSV) vstore<V>(&l_s3[r * W + c0], s3[p]);
+        }
+    }
+    if (threadIdx.x == 0) l_abort = 0;
+    __syncthreads();
+
+    int p0 = 0, p1 = 0;
+    if (a.inj_ptr) {
+        const int set = a.inj_per_slot ? b : 0;
+        p0 = a.inj_ptr[set * per + tx];
+        p1 = a.inj_ptr[set * per + tx + 1];
+    }
+    int r0 = 0, r1 = 0;
+    if (REC && a.rec_ptr) {
+        r0 = a.rec_ptr[tx];
+        r1 = a.rec_ptr[tx + 1];
+    }
+    int q0 = 0, q1 = 0;
+    if (PSV && a.sinj_ptr) {
+        q0 = a.sinj_ptr[b * per + tx];
+        q1 = a.sinj_ptr[b * per + tx + 1];
+    }
+    const LanePts ip = lane_pts(a.inj_cell, a.inj_row, p0, p1, lane);
+    const LanePts rp = lane_pts(a.rec_cell, a.rec_row, r0, r1, lane);
+    const LanePts xp = lane_pts(a.sinj_cell, a.sinj_row, q0, q1, lane);
+    const int fu = PSV ? a.psv_f0 : 0, fv = PSV ? a.psv_f0 + 2 : 1;
+    const long sb = (long)b * a.snap_slot_stride;
+    const StepRoles none{nullptr, nullptr};
+    const unsigned ep = __builtin_amdgcn_readfirstlane(ra.ctl[0]);
+    // exchange rows of this slot: [tile][parity][field][row][64 V columns]
+    unsigned long long *const xb = ra.xbuf + (long)b * per * (2L * RES_NF * TX * W);
+    auto xrow = [&](int tile, int par, int f, int r) {
+        return xb + ((((long)tile * 2 + par) * RES_NF + f) * TX + r) * W;
+    };
+
+    for (int k = 0; k < ra.iters; k++) {
+        const long it = a.it + 2L * k;
+        const int par = k & 1;
+        const unsigned tag = ep * 8192u + (unsigned)k + 1u;
+        const bool last = k == ra.iters - 1;
+        const StepRoles r1s = SNAP ? res_role(ra, it) : none;
+        const StepRoles r2 = SNAP ? res_role(ra, it + 1) : none;
+
+        if (k > 0) {
+            // halo rows of the state: the rows of the previous iteration
+            // (parity par ^ 1) published by the tiles that own them; stress
+            // 0 outside the grid, velocity at the clamped row and column.
+            // Each attempt issues every load of the wavefront's halo rows
+            // before it reads any (one memory round trip), then checks the
+            // tags; stress rows and velocity rows take one attempt loop each.
+            const unsigned want = tag - 1u;
+            const int pp = par ^ 1;
+            bool timed_out = ra.spin_limit < 0;
+            for (int spins = 0; !timed_out; spins++) {
+                unsigned long long ws[PS][3][V];
+#pragma unroll
+                for (int p = 0; p < PS; p++) {
+                    const int r = wave + p * NW, x = r - 6, i = i0 + x;
+                    if (r >= SR || (x >= 0 && x < TX && i < nx) || i < 0 || i >= nx)
+                        continue;  // wave-uniform: no stress row to read
+                    const int tt = i / TX;
+                    unsigned long long *const q = xrow(tt, pp, 0, i - tt * TX) + c0;
+#pragma unroll
+                    for (int v = 0; v < V; v++) {
+                        ws[p][0][v] = xload(q + v);
+                        if (r >= 2 && r < SR - 1) ws[p][1][v] = xload(q + TX * W + v);
+                        if (PSV) ws[p][2][v] = xload(q + 2 * TX * W + v);
+                    }
+                }
+                bool good = true;
+#pragma unroll
+                for (int p = 0; p < PS; p++) {
+                    const int r = wave + p * NW, x = r - 6, i = i0 + x;
+                    if (r >= SR || (x >= 0 && x < TX && i < nx)) continue;  // wave-uniform
+                    const bool has2 = r >= 2 && r < SR - 1;
+                    const bool rd = jin && i >= 0 && i < nx;
+                    float t1[V], t2[V], t3[V];
+#pragma unroll
+                    for (int v = 0; v < V; v++) {
+                        t1[v] = t2[v] = t3[v] = 0.0f;
+                        if (rd) {
+                            good &= xtake(ws[p][0][v], want, t1[v]);
+                            if (has2) good &= xtake(ws[p][1][v], want, t2[v]);
+                            if (PSV) good &= xtake(ws[p][2][v], want, t3[v]);
+                        }
+                    }
+                    vstore<V>(&l_s1[r * W + c0], t1);
+                    if (has2) vstore<V>(&l_s2[(r - 2) * W + c0], t2);
+                    if (PSV) vstore<V>(&l_s3[r * W + c0], t3);
+                }
+                if (__all(good)) break;
+                if (spins >= ra.spin_limit) timed_out = true;
+                else __builtin_amdgcn_s_sleep(2);
+            }
+            for (int spins = 0; !timed_out; spins++) {
+                unsigned long long wv[P][2][V];
+#pragma unroll
+                for (int p = 0; p < P; p++) {
+                    const int x = wave - 4 + p * NW, i = i0 + x;
+                    if (x >= TX + 5 || (x >= 0 && x < TX && i < nx)) continue;  // wave-uniform
+                    const int ic = clampi(i, 0, nx - 1);
+                    const int tt = ic / TX;
+                    unsigned long long *const q = xrow(tt, pp, 3, ic - tt * TX) + csrc;
+#pragma unroll
+                    for (int v = 0; v < V; v++) {
+                        wv[p][0][v] = xload(q + v);
+                        if (PSV) wv[p][1][v] = xload(q + TX * W + v);
+                    }
+                }
+                bool good = true;
+#pragma unroll
+                for (int p = 0; p < P; p++) {
+                    const int x = wave - 4 + p * NW, i = i0 + x;
+                    if (x >= TX + 5 || (x >= 0 && x < TX && i < nx)) continue;  // wave-uniform
+#pragma unroll
+                    for (int v = 0; v < V; v++) {
+                        good &= xtake(wv[p][0][v], want, qv1[p][v]);
+                        if (PSV) good &= xtake(wv[p][1][v], want, qv2[p][v]);
+                    }
+                }
+                if (__all(good)) break;
+                if (spins >= ra.spin_limit) timed_out = true;
+                else __builtin_amdgcn_s_sleep(2);
+            }
+            // an owned row keeps its velocity; lanes outside the grid take
+            // the clamped column's, from phase C's LDS rows
+            if (!jin) {
+#pragma unroll
+                for (int p = 0; p < P; p++) {
+                    const int x = wave - 4 + p * NW, i = i0 + x;
+                    if (x >= 0 && x < TX && i < nx) {  // wave-uniform
+                        vload<V>(&l_v1[(x + 4) * W + csrc], qv1[p]);
+                        if (PSV) vload<V>(&l_v2[(x + 4) * W + csrc], qv2[p]);
+                    }
+                }
+            }
+            if (timed_out && lane == 0) {
+                l_abort = 1;
+                __hip_atomic_store(ra.ctl + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            if (l_abort) return;  // the whole workgroup, after the barrier
+        }
+
+        // A. velocity of step n+1 on rows x = -4 .. TX+4 (LDS row x + 4)
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const int x = wave - 4 + p * NW;
+            if (x >= TX + 5) break;
+            const int i = i0 + x;
+            const bool rin = jin && i >= 0 && i < nx;
+            const int rlo = i * nz - HL, rhi = rlo + W - 1;
+            float n1[V], n2[V];
+            vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, x + 4, c0, i, jb, rin, zc,
+                                     qab[p], abz, abt, qv1[p], qv2[p], qr[p], ip, rlo, rhi, b, it,
+                                     n1, n2);
+            vstore<V>(&l_v1[(x + 4) * W + c0], n1);
+            if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], n2);
+            if (x >= 0 && x < TX && i < nx)  // wave-uniform
+                owned_row<PSV, V, FAST>(a, r1s, fu, fv, sb, i, jb, cown, it, b, 0, S, lane, rp,
+                                        rlo, rhi, qu1[p], qu2[p], n1, n2);
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                qv1[p][v] = n1[v];
+                if (PSV) qv2[p][v] = n2[v];
+            }
+        }
+        __syncthreads();
+
+        // B. stress of step n+1 on rows x = -3 .. TX+2, in place
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const int x = wave - 4 + p * NW;
+            if (x < -3) continue;
+            if (x >= TX + 3) break;
+            const int i = i0 + x;
+            const bool rin = jin && i >= 0 && i < nx;
+            const int r = x + 6;
+            float sa[V], sc[V], sxz[V], o1[V], o2[V], o3[V];
+            vload<V>(&l_s1[r * W + c0], sa);
+            vload<V>(&l_s2[(x + 4) * W + c0], sc);
+            if (PSV) vload<V>(&l_s3[r * W + c0], sxz);
+            stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, x + 4, c0, i, jb, zc, sa, sc, sxz, qmu[p],
+                                        qlam[p], o1, o2, o3);
+            if (PSV) explode_row<V>(a, xp, i, jb, b, it, o1, o2);
+            if (!FAST && !rin) {
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    o1[v] = 0.0f;
+                    o2[v] = 0.0f;
+                    o3[v] = 0.0f;
+                }
+            }
+            vstore<V>(&l_s1[r * W + c0], o1);
+            vstore<V>(&l_s2[(x + 4) * W + c0], o2);
+            if (PSV) vstore<V>(&l_s3[r * W + c0], o3);
+        }
+        __syncthreads();
+
+        // C. velocity of step n+2 on rows x = -1 .. TX+1, in place; owned rows
+        // keep it (and publish it, or store it with u after the last iteration)
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const int x = wave - 4 + p * NW;
+            if (x < -1) continue;
+            if (x >= TX + 2) break;
+            const int i = i0 + x;
+            const bool rin = jin && i >= 0 && i < nx;
+            const int rlo = i * nz - HL, rhi = rlo + W - 1;
+            float n1[V], n2[V];
+            vel_row<PSV, V, W, FAST>(a, l_s1, l_s2, l_s3, x + 6, x + 4, c0, i, jb, rin, zc,
+                                     qab[p], abz, abt, qv1[p], qv2[p], qr[p], ip, rlo, rhi, b,
+                                     it + 1, n1, n2);
+            vstore<V>(&l_v1[(x + 4) * W + c0], n1);
+            if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], n2);
+            if (x >= 0 && x < TX && i < nx) {  // wave-uniform
+                owned_row<PSV, V, FAST>(a, r2, fu, fv, sb, i, jb, cown, it + 1, b, 0, S, lane, rp,
+                                        rlo, rhi, qu1[p], qu2[p], n1, n2);
+                if (cown) {
+                    if (last) {
+                        const long o = so + (long)i * nz + jb;
+                        vstore<V>(V1o + o, n1);
+                        vstore<V>(U1 + o, qu1[p]);
+                        if (PSV) {
+                            vstore<V>(V2o + o, n2);
+                            vstore<V>(U2 + o, qu2[p]);
+                        }
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < V; v++) {
+                            xput(xrow(tx, par, 3, x) + c0 + v, n1[v], tag);
+                            if (PSV) xput(xrow(tx, par, 4, x) + c0 + v, n2[v], tag);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    qv1[p][v] = n1[v];
+                    if (PSV) qv2[p][v] = n2[v];
+                }
+            }
+        }
+        __syncthreads();
+
+        // D. stress of step n+2 on the owned rows: kept in LDS in place (0 in
+        // lanes outside the grid, as k_pair loads them) and published, or
+        // stored after the last iteration
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const int x = wave - 4 + p * NW;
+            if (x < 0) continue;
+            if (x >= TX) break;
+            const int i = i0 + x;
+            if (i >= nx) break;
+            const int r = x + 6;
+            float sa[V], sc[V], sxz[V], o1[V], o2[V], o3[V];
+            vload<V>(&l_s1[r * W + c0], sa);
+            vload<V>(&l_s2[(x + 4) * W + c0], sc);
+            if (PSV) vload<V>(&l_s3[r * W + c0], sxz);
+            stress_row<PSV, V, W, FAST>(g, l_v1, l_v2, x + 4, c0, i, jb, zc, sa, sc, sxz, qmu[p],
+                                        qlam[p], o1, o2, o3);
+            if (PSV) explode_row<V>(a, xp, i, jb, b, it + 1, o1, o2);
+            if (last) {
+                if (cown) {
+                    const long o = so + (long)i * nz + jb;
+                    vstore<V>(S1o + o, o1);
+                    vstore<V>(S2o + o, o2);
+                    if (PSV) vstore<V>(S3o + o, o3);
+                }
+                continue;
+            }
+            if (cown) {
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    xput(xrow(tx, par, 0, x) + c0 + v, o1[v], tag);
+                    xput(xrow(tx, par, 1, x) + c0 + v, o2[v], tag);
+                    if (PSV) xput(xrow(tx, par, 2, x) + c0 + v, o3[v], tag);
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; v++) o1[v] = o2[v] = o3[v] = 0.0f;
+            }
+            vstore<V>(&l_s1[r * W + c0], o1);
+            vstore<V>(&l_s2[(x + 4) * W + c0], o2);
+            if (PSV) vstore<V>(&l_s3[r * W + c0], o3);
+        }
+    }
+}
+
+template <class K>
+static void res_go(Ctx &c, K kern, int nw, int lds, const StepArgs &a, const ResArgs &ra,
+                   unsigned blocks) {
+    // every tile waits for its neighbours: the whole grid must be resident
+    static int fit = -1;  // blocks per CU of this instantiation
+    if (fit < 0) {
+        if (lds > 65536)
+            INV_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        int nb = 0;
+        INV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(kern), 64 * nw, lds));
+        fit = nb;
+    }
+    int dev = 0, cus = 0;
+    INV_HIP(hipGetDevice(&dev));
+    INV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    INV_REQUIRE((long)fit * cus >= (long)blocks, INV_ERR_UNSUPPORTED,
+                "persistent kernel: the tiles of this launch do not fit the device at once "
+                "(fewer shots per launch or a smaller grid)");
+    k_res_epoch<<<1, 1, 0, c.stream>>>(ra.ctl);
+    kern<<<blocks, 64 * nw, lds, c.stream>>>(a, ra);
+}
+
+template <bool SH, bool PSV, int TX, bool FAST>
+static void res_launch_tx(Ctx &c, const StepArgs &a, const ResArgs &ra, unsigned blocks) {
+    constexpr int V = RES_V, NW = RES_NW;
+    const int lds = pair_lds_bytes(PSV, TX, V);
+    if (ra.snap) res_go(c, k_res<SH, PSV, TX, V, NW, FAST, true, true>, NW, lds, a, ra, blocks);
+    else if (a.rec_ptr)
+        res_go(c, k_res<SH, PSV, TX, V, NW, FAST, false, true>, NW, lds, a, ra, blocks);
+    else res_go(c, k_res<SH, PSV, TX, V, NW, FAST, false, false>, NW, lds, a, ra, blocks);
+}
+
+template <bool SH, bool PSV, bool FAST>
+void res_launch(Ctx &c, const StepArgs &a, const ResArgs &ra, unsigned blocks) {
+    if (c.pair_tx == 4) res_launch_tx<SH, PSV, 4, FAST>(c, a, ra, blocks);
+    else res_launch_tx<SH, PSV, 8, FAST>(c, a, ra, blocks);
+}
+
+template void res_launch<true, false, true>(Ctx &, const StepArgs &, const ResArgs &, unsigned);
+template void res_launch<false, true, true>(Ctx &, const StepArgs &, const ResArgs &, unsigned);
+template void res_launch<true, false, false>(Ctx &, const StepArgs &, const ResArgs &, unsigned);
+template void res_launch<false, true, false>(Ctx &, const StepArgs &, const ResArgs &, unsigned);
+
+}  // namespace inv

@@ -60,7 +60,7 @@ constexpr int TILE_X = 4;   // wavefronts per block along x
 // Sampled kernel timing with HIP events on the context stream.
 enum ProfKind {
     PROF_VELOCITY = 0, PROF_STRESS = 1, PROF_INTERACT = 2, PROF_STEP = 3,
-    PROF_FWD_RUN = 4, PROF_ADJ_RUN = 5, PROF_PAIR = 6, PROF_NKIND = 7
+    PROF_FWD_RUN = 4, PROF_ADJ_RUN = 5, PROF_PAIR = 6, PROF_RES = 7, PROF_NKIND = 8
 };
 struct Prof {
     int every = 0;  // sample every n-th time step (0 = off)
@@ -114,6 +114,11 @@ struct Ctx {
     // two-step kernel (fused variant): tile rows (0 = off) and columns per lane
     int pair_tx = 0;
     int pair_v = 2;
+    // persistent two-step kernel (fdm_resident.hip): 0 off, 1 on, 2 on with
+    // every wait given up at once (tests the error path); its halo exchange
+    // buffer and control words {launch epoch, error}
+    int persist = 0;
+    DevBuf xbuf, res_ctl;
     int math = INV_MATH_FAST;
     DevBuf bdt;        // dt / rho, rebuilt at the start of each fast-mode run
     // per-slot models: [batch][3] device pointers {rho (exact) or dt/rho

@@ -151,4 +151,34 @@ constexpr int pair_lds_bytes(bool psv, int rows, int V) {
 template <bool SH, bool PSV, bool FAST>
 void pair_launch(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned blocks);
 
+// Persistent two-step kernel (fdm_resident.hip): `iters` iterations of two
+// steps per launch on tiles of c.pair_tx = 4 or 8 rows x every column (nz <=
+// 240 with RES_V = 4 columns per lane), one workgroup of RES_NW wavefronts
+// per tile, halo rows exchanged through xbuf ([batch][tiles_x][2][5][rows]
+// [64 RES_V] tagged words).  ctl: [0] the launch epoch (counted up before each launch),
+// [1] the error word (1: a tile gave up waiting for a neighbour).  Snapshot
+// roles of each step follow from the plan below (snap null: none).  The
+// state is read from the `0` set of the StepArgs and written to the `1` set.
+constexpr int RES_V = 4, RES_NW = 8;
+constexpr int RES_KMAX = 8191;     // iterations per launch (tag space)
+constexpr int RES_SPIN = 1 << 20;  // polls of one wait before it gives up
+struct ResArgs {
+    unsigned long long *xbuf;
+    unsigned *ctl;
+    int iters;
+    int spin_limit;  // < 0: give up at the first wait (tests the error path)
+    char *snap;
+    signed char *sexp;
+    long slot_bytes, tab_bytes;
+    int sfe, nsfe, nslots, sbase;
+};
+template <bool SH, bool PSV, bool FAST>
+void res_launch(Ctx &c, const StepArgs &a, const ResArgs &ra, unsigned blocks);
+// bytes of the exchange buffer for count shot slots of a grid of nx rows in
+// tiles of tx rows
+inline size_t res_xbuf_bytes(int count, int nx, int tx) {
+    const long tiles = (nx + tx - 1) / tx;
+    return (size_t)count * tiles * 2 * 5 * tx * 64 * RES_V * sizeof(unsigned long long);
+}
+
 }  // namespace inv

@@ -19,7 +19,7 @@ INV_LAMBDA, INV_MU, INV_RHO, INV_K_LAMBDA, INV_K_MU, INV_K_RHO, INV_OUT, INV_ADS
 INV_SNAP_F32, INV_SNAP_F16, INV_SNAP_BF16, INV_SNAP_I16 = 0, 1, 2, 3
 INV_FWD_SNAPSHOTS = 1
 INV_PROF_VELOCITY, INV_PROF_STRESS, INV_PROF_INTERACT, INV_PROF_STEP = 0, 1, 2, 3
-INV_PROF_FWD_RUN, INV_PROF_ADJ_RUN, INV_PROF_PAIR = 4, 5, 6
+INV_PROF_FWD_RUN, INV_PROF_ADJ_RUN, INV_PROF_PAIR, INV_PROF_RESIDENT = 4, 5, 6, 7
 INV_VARIANT_FUSED, INV_VARIANT_SPLIT, INV_VARIANT_FUSED_LINEAR = 0, 1, 2
 INV_MATH_EXACT, INV_MATH_FAST = 0, 1
 
@@ -106,6 +106,8 @@ SIGNATURES = {
     "inv_set_tile": [_vp, _i, _i],
     "inv_set_pair": [_vp, _i, _i],
     "inv_get_pair": [_vp, _vp, _vp],
+    "inv_set_persistent": [_vp, _i],
+    "inv_get_persistent": [_vp, _vp],
     "inv_autotune": [_vp, _i, _i, _i, _vp, _vp, _vp],
     "inv_forward_begin": [_vp, _i, _i, _i],
     "inv_adjoint_begin": [_vp, _i, _i],
@@ -484,6 +486,16 @@ class Context:
         r, v = C.c_int(), C.c_int()
         self._chk(self.L.inv_get_pair(self.h, C.byref(r), C.byref(v)))
         return r.value, v.value
+
+    def set_persistent(self, mode=1):
+        """Persistent two-step kernel (1), off (0), or on with every wait
+        given up at once (2, a test hook)."""
+        self._chk(self.L.inv_set_persistent(self.h, mode))
+
+    def get_persistent(self):
+        m = C.c_int()
+        self._chk(self.L.inv_get_persistent(self.h, C.byref(m)))
+        return m.value
 
     # ---- measurement ----
     def prof_enable(self, every):
