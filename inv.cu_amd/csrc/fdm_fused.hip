@@ -1169,7 +1169,7 @@ static void res_prepare(Ctx &c, int count) {
     const size_t need = res_xbuf_bytes(count, c.g.nx, c.pair_tx);
     if (c.xbuf.bytes < need) {
         c.xbuf.ensure(need);
-        // tag 0 is never expected: a fresh buffer holds no valid word
+        // tag 0 is never expected: a fresh buffer holds no published flag
         fill_dev(c, c.xbuf.p, (long)(c.xbuf.bytes / sizeof(float)));
     }
 }
@@ -1189,7 +1189,8 @@ static void res_run(Ctx &c, StepArgs &a2, const SnapPlan &sp, const Fields &F, i
     a2.it = it;
     set_roles(a2, StepRoles{nullptr, nullptr});
     ResArgs ra{};
-    ra.xbuf = c.xbuf.as<unsigned long long>();
+    ra.xbuf = c.xbuf.as<float>();
+    ra.xflag = reinterpret_cast<unsigned *>(ra.xbuf + res_xbuf_floats(count, c.g.nx, c.pair_tx));
     ra.ctl = c.res_ctl.as<unsigned>();
     ra.iters = iters;
     ra.spin_limit = c.persist == 2 ? -1 : RES_SPIN;

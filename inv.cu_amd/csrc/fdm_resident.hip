@@ -5,15 +5,16 @@
 // column of the grid.  Between iterations a tile's wavefields stay on the
 // chip: the stress in LDS, the velocity, displacement and model in
 // registers.  Only the rows a neighbouring tile reads as its halo cross
-// memory, through an exchange buffer of tagged 64-bit words (the value's bits
-// and the iteration's tag, stored and loaded as relaxed device-scope
-// atomics): a tile publishes its owned rows of stress and velocity at the end
-// of an iteration and, at the start of the next, reads its halo rows (six
-// each side) from the tiles that own them (the neighbour on each side for
-// 8-row tiles, two on each side for 4-row tiles), waiting until every word
-// carries the tag it expects.  A word is read only when its tag is right, so
-// no fence orders the value against a separate flag; a tile waits only for
-// the tiles its halo reaches, never for the whole grid.
+// memory: a tile writes its owned rows of stress and velocity to an exchange
+// buffer during an iteration; at its end every wavefront waits for its
+// stores, the workgroup meets at a barrier and one thread publishes the
+// iteration's tag in the tile's flag with a device-scope release.  At the
+// start of the next iteration one wavefront polls the flags of the tiles
+// its halo rows (six each side) come from (the neighbour on each side for
+// 8-row tiles, two on each side for 4-row tiles, and the tile itself, whose
+// rows the clamped edge rows read), then acquires, and after a barrier the
+// workgroup reads the rows.  A tile waits only for the tiles its halo
+// reaches, never for the whole grid.
 //
 // Two buffers alternate by iteration parity.  The tiles a halo reaches are
 // the tiles that read it (the relation is symmetric), and a tile cannot
@@ -21,7 +22,7 @@
 // they do after reading its iteration k-1 rows: the buffer it overwrites is
 // never still being read.  Tags are epoch * 8192 + k + 1 with the epoch
 // counted up on the device before every launch (graph replays included), so
-// a word left over from another launch never matches.
+// a flag left over from another launch never matches.
 //
 // Every tile waits only for tiles that are running: the host launches the
 // kernel only when the whole grid fits the device at once (occupancy), and
@@ -40,20 +41,6 @@ namespace inv {
 
 // exchanged fields of a tile row: s1, s2, s3 (stress) and v1, v2 (velocity)
 constexpr int RES_NF = 5;
-
-__device__ __forceinline__ void xput(unsigned long long *q, float v, unsigned tag) {
-    const unsigned long long w =
-        ((unsigned long long)tag << 32) | (unsigned long long)__float_as_uint(v);
-    __hip_atomic_store(q, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned long long xload(unsigned long long *q) {
-    return __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// a loaded word's value into v; true when it carries the expected tag
-__device__ __forceinline__ bool xtake(unsigned long long w, unsigned tag, float &v) {
-    v = __uint_as_float((unsigned)(w & 0xffffffffull));
-    return (unsigned)(w >> 32) == tag;
-}
 
 // snapshot role of forward step s (SnapPlan::local in fdm_fused.hip)
 __device__ __forceinline__ StepRoles res_role(const ResArgs &ra, long s) {
@@ -193,11 +180,15 @@ __global__ __launch_bounds__(64 * NW, 1) void k_res(const StepArgs a, const ResA
     const long sb = (long)b * a.snap_slot_stride;
     const StepRoles none{nullptr, nullptr};
     const unsigned ep = __builtin_amdgcn_readfirstlane(ra.ctl[0]);
-    // exchange rows of this slot: [tile][parity][field][row][64 V columns]
-    unsigned long long *const xb = ra.xbuf + (long)b * per * (2L * RES_NF * TX * W);
+    // exchange rows of this slot: [tile][parity][field][row][64 V columns],
+    // and a flag per tile and parity holding the tag of the iteration whose
+    // rows it last published
+    float *const xb = ra.xbuf + (long)b * per * (2L * RES_NF * TX * W);
     auto xrow = [&](int tile, int par, int f, int r) {
         return xb + ((((long)tile * 2 + par) * RES_NF + f) * TX + r) * W;
     };
+    unsigned *const xf = ra.xflag + (long)b * per * 2;
+    constexpr int REACH = (6 + TX - 1) / TX;  // tiles a halo reaches on each side
 
     for (int k = 0; k < ra.iters; k++) {
         const long it = a.it + 2L * k;
@@ -208,104 +199,75 @@ __global__ __launch_bounds__(64 * NW, 1) void k_res(const StepArgs a, const ResA
         const StepRoles r2 = SNAP ? res_role(ra, it + 1) : none;
 
         if (k > 0) {
-            // halo rows of the state: the rows of the previous iteration
-            // (parity par ^ 1) published by the tiles that own them; stress
-            // 0 outside the grid, velocity at the clamped row and column.
-            // Each attempt issues every load of the wavefront's halo rows
-            // before it reads any (one memory round trip), then checks the
-            // tags; stress rows and velocity rows take one attempt loop each.
+            // wait until the tiles this tile's halo reaches (and the tile
+            // itself, whose rows the clamped edge rows read) have published
+            // the previous iteration (parity par ^ 1): one lane polls each
+            // flag, then an acquire makes their rows visible to the
+            // workgroup (after the barrier)
             const unsigned want = tag - 1u;
             const int pp = par ^ 1;
-            bool timed_out = ra.spin_limit < 0;
-            for (int spins = 0; !timed_out; spins++) {
-                unsigned long long ws[PS][3][V];
-#pragma unroll
-                for (int p = 0; p < PS; p++) {
-                    const int r = wave + p * NW, x = r - 6, i = i0 + x;
-                    if (r >= SR || (x >= 0 && x < TX && i < nx) || i < 0 || i >= nx)
-                        continue;  // wave-uniform: no stress row to read
-                    const int tt = i / TX;
-                    unsigned long long *const q = xrow(tt, pp, 0, i - tt * TX) + c0;
-#pragma unroll
-                    for (int v = 0; v < V; v++) {
-                        ws[p][0][v] = xload(q + v);
-                        if (r >= 2 && r < SR - 1) ws[p][1][v] = xload(q + TX * W + v);
-                        if (PSV) ws[p][2][v] = xload(q + 2 * TX * W + v);
-                    }
+            if (wave == 0) {
+                const int t = tx - REACH + lane;
+                const bool need = lane <= 2 * REACH && t >= 0 && t < per;
+                unsigned *const fl = xf + (need ? t : tx) * 2 + pp;
+                bool timed_out = ra.spin_limit < 0;
+                for (int spins = 0; !timed_out; spins++) {
+                    const bool ok =
+                        !need || __hip_atomic_load(fl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == want;
+                    if (__all(ok)) break;
+                    if (spins >= ra.spin_limit) timed_out = true;
+                    else __builtin_amdgcn_s_sleep(1);
                 }
-                bool good = true;
-#pragma unroll
-                for (int p = 0; p < PS; p++) {
-                    const int r = wave + p * NW, x = r - 6, i = i0 + x;
-                    if (r >= SR || (x >= 0 && x < TX && i < nx)) continue;  // wave-uniform
-                    const bool has2 = r >= 2 && r < SR - 1;
-                    const bool rd = jin && i >= 0 && i < nx;
-                    float t1[V], t2[V], t3[V];
-#pragma unroll
-                    for (int v = 0; v < V; v++) {
-                        t1[v] = t2[v] = t3[v] = 0.0f;
-                        if (rd) {
-                            good &= xtake(ws[p][0][v], want, t1[v]);
-                            if (has2) good &= xtake(ws[p][1][v], want, t2[v]);
-                            if (PSV) good &= xtake(ws[p][2][v], want, t3[v]);
-                        }
-                    }
-                    vstore<V>(&l_s1[r * W + c0], t1);
-                    if (has2) vstore<V>(&l_s2[(r - 2) * W + c0], t2);
-                    if (PSV) vstore<V>(&l_s3[r * W + c0], t3);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                if (timed_out && lane == 0) {
+                    l_abort = 1;
+                    __hip_atomic_store(ra.ctl + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
-                if (__all(good)) break;
-                if (spins >= ra.spin_limit) timed_out = true;
-                else __builtin_amdgcn_s_sleep(2);
-            }
-            for (int spins = 0; !timed_out; spins++) {
-                unsigned long long wv[P][2][V];
-#pragma unroll
-                for (int p = 0; p < P; p++) {
-                    const int x = wave - 4 + p * NW, i = i0 + x;
-                    if (x >= TX + 5 || (x >= 0 && x < TX && i < nx)) continue;  // wave-uniform
-                    const int ic = clampi(i, 0, nx - 1);
-                    const int tt = ic / TX;
-                    unsigned long long *const q = xrow(tt, pp, 3, ic - tt * TX) + csrc;
-#pragma unroll
-                    for (int v = 0; v < V; v++) {
-                        wv[p][0][v] = xload(q + v);
-                        if (PSV) wv[p][1][v] = xload(q + TX * W + v);
-                    }
-                }
-                bool good = true;
-#pragma unroll
-                for (int p = 0; p < P; p++) {
-                    const int x = wave - 4 + p * NW, i = i0 + x;
-                    if (x >= TX + 5 || (x >= 0 && x < TX && i < nx)) continue;  // wave-uniform
-#pragma unroll
-                    for (int v = 0; v < V; v++) {
-                        good &= xtake(wv[p][0][v], want, qv1[p][v]);
-                        if (PSV) good &= xtake(wv[p][1][v], want, qv2[p][v]);
-                    }
-                }
-                if (__all(good)) break;
-                if (spins >= ra.spin_limit) timed_out = true;
-                else __builtin_amdgcn_s_sleep(2);
-            }
-            // an owned row keeps its velocity; lanes outside the grid take
-            // the clamped column's, from phase C's LDS rows
-            if (!jin) {
-#pragma unroll
-                for (int p = 0; p < P; p++) {
-                    const int x = wave - 4 + p * NW, i = i0 + x;
-                    if (x >= 0 && x < TX && i < nx) {  // wave-uniform
-                        vload<V>(&l_v1[(x + 4) * W + csrc], qv1[p]);
-                        if (PSV) vload<V>(&l_v2[(x + 4) * W + csrc], qv2[p]);
-                    }
-                }
-            }
-            if (timed_out && lane == 0) {
-                l_abort = 1;
-                __hip_atomic_store(ra.ctl + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             __syncthreads();
             if (l_abort) return;  // the whole workgroup, after the barrier
+            // halo rows: stress 0 outside the grid, velocity at the clamped
+            // row and column; an owned row keeps its velocity, lanes outside
+            // the grid taking the clamped column's from phase C's LDS rows
+#pragma unroll
+            for (int p = 0; p < PS; p++) {
+                const int r = wave + p * NW, x = r - 6, i = i0 + x;
+                if (r >= SR || (x >= 0 && x < TX && i < nx)) continue;  // wave-uniform
+                const bool has2 = r >= 2 && r < SR - 1;
+                float t1[V], t2[V], t3[V];
+#pragma unroll
+                for (int v = 0; v < V; v++) t1[v] = t2[v] = t3[v] = 0.0f;
+                if (i >= 0 && i < nx) {  // wave-uniform
+                    const int tt = i / TX;
+                    const float *const q = xrow(tt, pp, 0, i - tt * TX) + c0;
+                    if (jin) {
+                        vload<V>(q, t1);
+                        if (has2) vload<V>(q + TX * W, t2);
+                        if (PSV) vload<V>(q + 2 * TX * W, t3);
+                    }
+                }
+                vstore<V>(&l_s1[r * W + c0], t1);
+                if (has2) vstore<V>(&l_s2[(r - 2) * W + c0], t2);
+                if (PSV) vstore<V>(&l_s3[r * W + c0], t3);
+            }
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                const int x = wave - 4 + p * NW, i = i0 + x;
+                if (x >= TX + 5) break;
+                if (x >= 0 && x < TX && i < nx) {  // wave-uniform
+                    if (!jin) {
+                        vload<V>(&l_v1[(x + 4) * W + csrc], qv1[p]);
+                        if (PSV) vload<V>(&l_v2[(x + 4) * W + csrc], qv2[p]);
+                    }
+                    continue;
+                }
+                const int ic = clampi(i, 0, nx - 1);
+                const int tt = ic / TX;
+                const float *const q = xrow(tt, pp, 3, ic - tt * TX) + csrc;
+                vload<V>(q, qv1[p]);
+                if (PSV) vload<V>(q + TX * W, qv2[p]);
+            }
+            __syncthreads();
         }
 
         // A. velocity of step n+1 on rows x = -4 .. TX+4 (LDS row x + 4)
@@ -392,11 +354,8 @@ __global__ __launch_bounds__(64 * NW, 1) void k_res(const StepArgs a, const ResA
                             vstore<V>(U2 + o, qu2[p]);
                         }
                     } else {
-#pragma unroll
-                        for (int v = 0; v < V; v++) {
-                            xput(xrow(tx, par, 3, x) + c0 + v, n1[v], tag);
-                            if (PSV) xput(xrow(tx, par, 4, x) + c0 + v, n2[v], tag);
-                        }
+                        vstore<V>(xrow(tx, par, 3, x) + c0, n1);
+                        if (PSV) vstore<V>(xrow(tx, par, 4, x) + c0, n2);
                     }
                 }
 #pragma unroll
@@ -436,12 +395,9 @@ __global__ __launch_bounds__(64 * NW, 1) void k_res(const StepArgs a, const ResA
                 continue;
             }
             if (cown) {
-#pragma unroll
-                for (int v = 0; v < V; v++) {
-                    xput(xrow(tx, par, 0, x) + c0 + v, o1[v], tag);
-                    xput(xrow(tx, par, 1, x) + c0 + v, o2[v], tag);
-                    if (PSV) xput(xrow(tx, par, 2, x) + c0 + v, o3[v], tag);
-                }
+                vstore<V>(xrow(tx, par, 0, x) + c0, o1);
+                vstore<V>(xrow(tx, par, 1, x) + c0, o2);
+                if (PSV) vstore<V>(xrow(tx, par, 2, x) + c0, o3);
             } else {
 #pragma unroll
                 for (int v = 0; v < V; v++) o1[v] = o2[v] = o3[v] = 0.0f;
@@ -449,6 +405,16 @@ __global__ __launch_bounds__(64 * NW, 1) void k_res(const StepArgs a, const ResA
             vstore<V>(&l_s1[r * W + c0], o1);
             vstore<V>(&l_s2[(x + 4) * W + c0], o2);
             if (PSV) vstore<V>(&l_s3[r * W + c0], o3);
+        }
+        if (!last) {
+            // publish: every wavefront's stores of the iteration have
+            // completed before the barrier, then one release (write-back to
+            // device scope) and the flag
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (threadIdx.x == 0)
+                __hip_atomic_store(xf + tx * 2 + par, tag, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }

@@ -155,15 +155,18 @@ void pair_launch(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned blocks
 // steps per launch on tiles of c.pair_tx = 4 or 8 rows x every column (nz <=
 // 240 with RES_V = 4 columns per lane), one workgroup of RES_NW wavefronts
 // per tile, halo rows exchanged through xbuf ([batch][tiles_x][2][5][rows]
-// [64 RES_V] tagged words).  ctl: [0] the launch epoch (counted up before each launch),
-// [1] the error word (1: a tile gave up waiting for a neighbour).  Snapshot
+// [64 RES_V] floats) with a flag per tile and parity in xflag
+// ([batch][tiles_x][2]: the tag of the iteration last published).  ctl: [0]
+// the launch epoch (counted up before each launch), [1] the error word (1: a
+// tile gave up waiting for a neighbour).  Snapshot
 // roles of each step follow from the plan below (snap null: none).  The
 // state is read from the `0` set of the StepArgs and written to the `1` set.
 constexpr int RES_V = 4, RES_NW = 8;
 constexpr int RES_KMAX = 8191;     // iterations per launch (tag space)
 constexpr int RES_SPIN = 1 << 20;  // polls of one wait before it gives up
 struct ResArgs {
-    unsigned long long *xbuf;
+    float *xbuf;
+    unsigned *xflag;
     unsigned *ctl;
     int iters;
     int spin_limit;  // < 0: give up at the first wait (tests the error path)
@@ -174,11 +177,15 @@ struct ResArgs {
 };
 template <bool SH, bool PSV, bool FAST>
 void res_launch(Ctx &c, const StepArgs &a, const ResArgs &ra, unsigned blocks);
-// bytes of the exchange buffer for count shot slots of a grid of nx rows in
-// tiles of tx rows
+// floats of the exchange rows for count shot slots of a grid of nx rows in
+// tiles of tx rows, and bytes of the rows and flags together
+inline size_t res_xbuf_floats(int count, int nx, int tx) {
+    const long tiles = (nx + tx - 1) / tx;
+    return (size_t)count * tiles * 2 * 5 * tx * 64 * RES_V;
+}
 inline size_t res_xbuf_bytes(int count, int nx, int tx) {
     const long tiles = (nx + tx - 1) / tx;
-    return (size_t)count * tiles * 2 * 5 * tx * 64 * RES_V * sizeof(unsigned long long);
+    return res_xbuf_floats(count, nx, tx) * sizeof(float) + (size_t)count * tiles * 2 * sizeof(unsigned);
 }
 
 }  // namespace inv
