@@ -1182,9 +1182,11 @@ static void res_m(Ctx &c, const StepArgs &a, const ResArgs &ra, unsigned blocks)
 }
 
 // iters iterations (steps it .. it + 2 iters - 1) from the current state set
-// into the other one; snapshot roles from sp when it is on
+// into the other one; snapshot roles from sp when it is on, or (adjoint, inl)
+// the kernel accumulations of the steps that are multiples of sfe in the
+// launch, reading the snapshots of plan sp
 static void res_run(Ctx &c, StepArgs &a2, const SnapPlan &sp, const Fields &F, int count,
-                    int it, int iters) {
+                    int it, int iters, bool inl = false) {
     set_state_ptrs(a2, F, c.parity);
     a2.it = it;
     set_roles(a2, StepRoles{nullptr, nullptr});
@@ -1194,7 +1196,29 @@ static void res_run(Ctx &c, StepArgs &a2, const SnapPlan &sp, const Fields &F, i
     ra.ctl = c.res_ctl.as<unsigned>();
     ra.iters = iters;
     ra.spin_limit = c.persist == 2 ? -1 : RES_SPIN;
-    if (sp.on) {
+    if (inl) {
+        // the accumulators of fdm_interact (fdm.hip): per-slot ones for a
+        // batched adjoint, the kernels themselves for one slot
+        ra.int_on = 1;
+        ra.ndt = (float)c.g.sfe * c.g.dt;
+        ra.inv_lambda = c.g.inv_lambda;
+        ra.inv_mu = c.g.inv_mu;
+        ra.inv_rho = c.g.inv_rho;
+        if (c.g.batch == 1) {
+            ra.kl = c.k_lambda.as<float>();
+            ra.km = c.k_mu.as<float>();
+            ra.kr = c.k_rho.as<float>();
+            ra.k_slot_stride = 0;
+        } else {
+            ra.kl = c.kslot.as<float>();
+            ra.km = c.kslot.as<float>() + c.N;
+            ra.kr = c.kslot.as<float>() + 2 * c.N;
+            ra.k_slot_stride = 3 * c.N;
+        }
+        INV_REQUIRE(sp.snap && (sp.fmt == INV_SNAP_F32 || sp.sexp) && c.nslots * c.g.sfe >= c.g.nt,
+                    INV_ERR_STATE, "snapshots of the run not in the buffer");
+    }
+    if (sp.on || inl) {
         ra.snap = sp.snap;
         ra.sexp = sp.sexp;
         ra.slot_bytes = sp.slot_bytes;
@@ -1340,16 +1364,22 @@ void fused_adjoint(Ctx &c, int first, int count, int it0, int it1) {
     for (int it = it0; it < it1;) {
         const bool two = pr && it + 1 < it1 && it % c.g.sfe != 0;
         if (two && res) {
-            // the pairs up to the next kernel accumulation in one launch
+            // every pair up to the next step that cannot pair (one whose
+            // kernel accumulation would fall between its two steps) in one
+            // launch, with the accumulations of the pairs' second steps in it
+            // (one wave type: the accumulation order of a cell is then that
+            // of per-pair launches); with SH and P-SV together, up to the
+            // next accumulation, which fdm_interact runs after the launch
+            const bool inl = !(c.g.sh && c.g.psv);
             int iters = 0, nxt = it, last = it;
             while (nxt + 1 < it1 && nxt % c.g.sfe != 0 && iters < RES_KMAX) {
                 iters++;
                 last = nxt + 1;
                 nxt += 2;
-                if (last % c.g.sfe == 0) break;
+                if (!inl && last % c.g.sfe == 0) break;
             }
-            res_run(c, a2, SnapPlan{}, F, count, it, iters);
-            if (last % c.g.sfe == 0)
+            res_run(c, a2, inl ? snap_plan(c, true, 0) : SnapPlan{}, F, count, it, iters, inl);
+            if (!inl && last % c.g.sfe == 0)
                 fdm_interact(c, count, last, last / c.g.sfe, cur_field(c, c.vx, c.vx_b),
                              cur_field(c, c.vz, c.vz_b), cur_field(c, c.vy, c.vy_b));
             it = last + 1;

@@ -52,7 +52,128 @@ __device__ __forceinline__ StepRoles res_role(const ResArgs &ra, long s) {
 
 __global__ void k_res_epoch(unsigned *ctl) { ctl[0] += 1u; }
 
-template <bool SH, bool PSV, int TX, int V, int NW, bool FAST, bool SNAP, bool REC>
+// the lane's V values of snapshot field f of shot slot b, row i, columns
+// jl .. jl+V-1 of snapshot slot `slot`, decoded to FP32 (the reads of
+// k_interact_col, fdm.hip)
+template <int V>
+__device__ __forceinline__ void res_snap_row(const StepArgs &a, const ResArgs &ra, long slot,
+                                             int b, int f, int i, int jl, float (&o)[V]) {
+    const long k = (long)b * a.snap_slot_stride + (long)f * a.g.N + (long)i * a.g.nz + jl;
+    const char *base = ra.snap + slot * ra.slot_bytes;
+    if (a.snap_fmt == INV_SNAP_F32) {
+        vload<V>(reinterpret_cast<const float *>(base) + k, o);
+        return;
+    }
+    const unsigned short *q = reinterpret_cast<const unsigned short *>(base) + k;
+    const signed char *tab = ra.sexp + slot * ra.tab_bytes;
+    const int w = *reinterpret_cast<const int *>(tab);
+    const int segw = w > 0 ? w : 1;
+    const signed char *erow = tab + snaptab_row(b, a.nsnapf, f, a.g.nx, i, a.nseg);
+    const unsigned char *brow = reinterpret_cast<const unsigned char *>(tab) +
+                                snaptab_blk(a.sblk, b, a.nsnapf, f, a.g.nx, i, a.nbb);
+#pragma unroll
+    for (int v = 0; v < V; v++)
+        o[v] = snap_dec16(q[v], snap_cell_exp(erow, brow, a.nseg, a.nbb, segw, jl + v),
+                          a.snap_fmt);
+}
+
+// Kernel accumulation (interaction*, solver/fdm.h:198-219, with the D+
+// derivatives of fdm.h:586-605) on the owned cells of grid row i after step
+// `last`: the expressions and their order of k_interact_col, so the kernels
+// agree bit for bit.  lu1 / lu2: the adjoint displacement in LDS rows of W
+// columns, row i at LDS row q (rows q-1 .. q+2 read); v1 / v2: the adjoint
+// velocity at the lane's cells.  Every lane of the wavefront calls it.
+template <bool PSV, int V, int W, bool FAST>
+__device__ __forceinline__ void res_interact_row(const StepArgs &a, const ResArgs &ra, int b,
+                                                 int i, int q, int c0, int jb, int jl, bool own,
+                                                 long last, const float *lu1, const float *lu2,
+                                                 const float (&v1)[V], const float (&v2)[V]) {
+    const Geo &g = a.g;
+    const int nx = g.nx, nz = g.nz;
+    const long slot = last / ra.sfe;
+    const int fu = PSV ? a.psv_f0 : 0, fv = PSV ? a.psv_f0 + 2 : 1;
+    const bool xin = i >= 1 && i < nx - 2;
+    auto rowc = [&](int r) { return r < 0 ? 0 : (r > nx - 1 ? nx - 1 : r); };
+    // D+ along x (rows i-1 .. i+2) and z (columns j-1 .. j+2, neighbouring
+    // lanes) of one field at the lane's cells, one field at a time (every
+    // lane of the wavefront takes part in the z window)
+    auto derivs = [&](const float (&r0)[V], const float (&r1)[V], const float (&r2)[V],
+                      const float (&r3)[V], float (&dx)[V], float (&dz)[V]) {
+        float e[3 * V];
+        window_reg<V>(r1, e);
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            dx[v] = dp<FAST>(r2[v], r1[v], r3[v], r0[v], g.c8dx, g.c24dx, g.a1x, g.a2x);
+            dz[v] = dp<FAST>(e[V + v + 1], e[V + v], e[V + v + 2], e[V + v - 1], g.c8dz, g.c24dz,
+                             g.a1z, g.a2z);
+        }
+    };
+    auto lds_derivs = [&](const float *lu, float (&dx)[V], float (&dz)[V]) {
+        float r0[V], r1[V], r2[V], r3[V];
+        vload<V>(&lu[(q - 1) * W + c0], r0);
+        vload<V>(&lu[q * W + c0], r1);
+        vload<V>(&lu[(q + 1) * W + c0], r2);
+        vload<V>(&lu[(q + 2) * W + c0], r3);
+        derivs(r0, r1, r2, r3, dx, dz);
+    };
+    auto snap_derivs = [&](int f, float (&dx)[V], float (&dz)[V]) {
+        float r0[V], r1[V], r2[V], r3[V];
+        res_snap_row<V>(a, ra, slot, b, f, rowc(i - 1), jl, r0);
+        res_snap_row<V>(a, ra, slot, b, f, i, jl, r1);
+        res_snap_row<V>(a, ra, slot, b, f, rowc(i + 1), jl, r2);
+        res_snap_row<V>(a, ra, slot, b, f, rowc(i + 2), jl, r3);
+        derivs(r0, r1, r2, r3, dx, dz);
+    };
+    // adjoint and forward displacement derivatives: x and z of u1 (ux or
+    // uy), of u2 (uz), and of their snapshots
+    float d1x[V], d1z[V], f1x[V], f1z[V], d2x[V], d2z[V], f2x[V], f2z[V];
+    lds_derivs(lu1, d1x, d1z);
+    snap_derivs(fu, f1x, f1z);
+    if (PSV) {
+        lds_derivs(lu2, d2x, d2z);
+        snap_derivs(fu + 1, f2x, f2z);
+    }
+    if (!own) return;
+    float vf1[V], vf2[V];
+    res_snap_row<V>(a, ra, slot, b, fv, i, jl, vf1);
+    if (PSV) res_snap_row<V>(a, ra, slot, b, fv + 1, i, jl, vf2);
+    const long k = (long)b * ra.k_slot_stride + (long)i * nz + jb;
+    float kr[V], km[V], kl[V];
+    if (ra.inv_rho) vload<V>(ra.kr + k, kr);
+    if (ra.inv_mu) vload<V>(ra.km + k, km);
+    if (PSV && ra.inv_lambda) vload<V>(ra.kl + k, kl);
+    const float ndt = ra.ndt;
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        const int j = jb + v;
+        const bool zin = j >= 1 && j < nz - 2;
+        if (!PSV) {
+            const float dx_ = xin ? d1x[v] : 0.0f;
+            const float dz_ = zin ? d1z[v] : 0.0f;
+            const float dxf = xin ? f1x[v] : 0.0f;
+            const float dzf_ = zin ? f1z[v] : 0.0f;
+            if (ra.inv_rho) kr[v] -= vf1[v] * v1[v] * ndt;
+            if (ra.inv_mu) km[v] -= (dx_ * dxf + dz_ * dzf_) * ndt;
+        } else {
+            const float xx = xin ? d1x[v] : 0.0f, zx = xin ? d2x[v] : 0.0f;
+            const float xxf = xin ? f1x[v] : 0.0f, zxf = xin ? f2x[v] : 0.0f;
+            const float xz = zin ? d1z[v] : 0.0f, zz = zin ? d2z[v] : 0.0f;
+            const float xzf = zin ? f1z[v] : 0.0f, zzf = zin ? f2z[v] : 0.0f;
+            if (ra.inv_rho) kr[v] -= (vf1[v] * v1[v] + vf2[v] * v2[v]) * ndt;
+            if (ra.inv_mu) km[v] -= (2 * xx * xxf + 2 * zz * zzf + (xz + zx) * (zxf + xzf)) * ndt;
+            if (ra.inv_lambda) kl[v] -= ((xx + zz) * (xxf + zzf)) * ndt;
+        }
+    }
+    if (ra.inv_rho) vstore<V>(ra.kr + k, kr);
+    if (ra.inv_mu) vstore<V>(ra.km + k, km);
+    if (PSV && ra.inv_lambda) vstore<V>(ra.kl + k, kl);
+}
+
+// INT: the adjoint variant that accumulates the kernels in the launch (one
+// wave type per context); it tracks the displacement on the halo rows -1, TX
+// and TX+1 (u += v dt with the velocity it computes there anyway, the
+// owner's values bit for bit), which the D+ x-derivatives read.
+template <bool SH, bool PSV, int TX, int V, int NW, bool FAST, bool SNAP, bool REC, bool INT>
 __global__ __launch_bounds__(64 * NW, 1) void k_res(const StepArgs a, const ResArgs ra) {
     static_assert(SH != PSV, "one wave type per launch");
     static_assert(TX % 4 == 0, "tile rows a multiple of four");
@@ -124,7 +245,7 @@ __global__ __launch_bounds__(64 * NW, 1) void k_res(const StepArgs a, const ResA
             if (PSV) vload<V>(V2 + so + kc, qv2[p]);
             qab[p] = vload_uniform(a.abx, ic);
         }
-        if (x >= 0 && x < TX) {
+        if (x >= (INT ? -1 : 0) && x < TX + (INT ? 2 : 0)) {
             vload<V>(U1 + so + kc, qu1[p]);
             if (PSV) vload<V>(U2 + so + kc, qu2[p]);
         }
@@ -227,8 +348,7 @@ __global__ __launch_bounds__(64 * NW, 1) void k_res(const StepArgs a, const ResA
             __syncthreads();
             if (l_abort) return;  // the whole workgroup, after the barrier
             // halo rows: stress 0 outside the grid, velocity at the clamped
-            // row and column; an owned row keeps its velocity, lanes outside
-            // the grid taking the clamped column's from phase C's LDS rows
+            // row and column
 #pragma unroll
             for (int p = 0; p < PS; p++) {
                 const int r = wave + p * NW, x = r - 6, i = i0 + x;
@@ -254,13 +374,7 @@ __global__ __launch_bounds__(64 * NW, 1) void k_res(const StepArgs a, const ResA
             for (int p = 0; p < P; p++) {
                 const int x = wave - 4 + p * NW, i = i0 + x;
                 if (x >= TX + 5) break;
-                if (x >= 0 && x < TX && i < nx) {  // wave-uniform
-                    if (!jin) {
-                        vload<V>(&l_v1[(x + 4) * W + csrc], qv1[p]);
-                        if (PSV) vload<V>(&l_v2[(x + 4) * W + csrc], qv2[p]);
-                    }
-                    continue;
-                }
+                if (x >= 0 && x < TX && i < nx) continue;  // owned (phase D): wave-uniform
                 const int ic = clampi(i, 0, nx - 1);
                 const int tt = ic / TX;
                 const float *const q = xrow(tt, pp, 3, ic - tt * TX) + csrc;
@@ -287,6 +401,13 @@ __global__ __launch_bounds__(64 * NW, 1) void k_res(const StepArgs a, const ResA
             if (x >= 0 && x < TX && i < nx)  // wave-uniform
                 owned_row<PSV, V, FAST>(a, r1s, fu, fv, sb, i, jb, cown, it, b, 0, S, lane, rp,
                                         rlo, rhi, qu1[p], qu2[p], n1, n2);
+            if (INT && (x == -1 || x == TX || x == TX + 1)) {
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    qu1[p][v] = u_step<FAST>(qu1[p][v], n1[v], g.dt);
+                    if (PSV) qu2[p][v] = u_step<FAST>(qu2[p][v], n2[v], g.dt);
+                }
+            }
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 qv1[p][v] = n1[v];
@@ -341,6 +462,13 @@ __global__ __launch_bounds__(64 * NW, 1) void k_res(const StepArgs a, const ResA
                                      it + 1, n1, n2);
             vstore<V>(&l_v1[(x + 4) * W + c0], n1);
             if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], n2);
+            if (INT && (x == -1 || x == TX || x == TX + 1)) {
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    qu1[p][v] = u_step<FAST>(qu1[p][v], n1[v], g.dt);
+                    if (PSV) qu2[p][v] = u_step<FAST>(qu2[p][v], n2[v], g.dt);
+                }
+            }
             if (x >= 0 && x < TX && i < nx) {  // wave-uniform
                 owned_row<PSV, V, FAST>(a, r2, fu, fv, sb, i, jb, cown, it + 1, b, 0, S, lane, rp,
                                         rlo, rhi, qu1[p], qu2[p], n1, n2);
@@ -394,6 +522,12 @@ __global__ __launch_bounds__(64 * NW, 1) void k_res(const StepArgs a, const ResA
                 }
                 continue;
             }
+            // the velocity of lanes outside the grid becomes the clamped
+            // column's (phase C's LDS row), as k_pair would load it
+            if (!jin) {
+                vload<V>(&l_v1[(x + 4) * W + csrc], qv1[p]);
+                if (PSV) vload<V>(&l_v2[(x + 4) * W + csrc], qv2[p]);
+            }
             if (cown) {
                 vstore<V>(xrow(tx, par, 0, x) + c0, o1);
                 vstore<V>(xrow(tx, par, 1, x) + c0, o2);
@@ -415,6 +549,34 @@ __global__ __launch_bounds__(64 * NW, 1) void k_res(const StepArgs a, const ResA
             if (threadIdx.x == 0)
                 __hip_atomic_store(xf + tx * 2 + par, tag, __ATOMIC_RELEASE,
                                    __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (INT && (it + 1) % ra.sfe == 0) {
+            // kernel accumulation after step it + 1: the displacement of rows
+            // -1 .. TX+1 staged in the velocity rows of LDS (free until the
+            // next phase A, behind the next iteration's barriers)
+            __syncthreads();
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                const int x = wave - 4 + p * NW;
+                if (x < -1) continue;
+                if (x > TX + 1) break;
+                vstore<V>(&l_v1[(x + 4) * W + c0], qu1[p]);
+                if (PSV) vstore<V>(&l_v2[(x + 4) * W + c0], qu2[p]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                const int x = wave - 4 + p * NW;
+                if (x < 0) continue;
+                if (x >= TX) break;
+                const int i = i0 + x;
+                if (i >= nx) break;
+                res_interact_row<PSV, V, W, FAST>(a, ra, b, i, x + 4, c0, jb, jl, cown, it + 1,
+                                                  l_v1, l_v2, qv1[p], qv2[p]);
+                // one row at a time: interleaving the rows' loads would need
+                // more registers than the kernel has left
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
     }
 }
@@ -447,10 +609,13 @@ template <bool SH, bool PSV, int TX, bool FAST>
 static void res_launch_tx(Ctx &c, const StepArgs &a, const ResArgs &ra, unsigned blocks) {
     constexpr int V = RES_V, NW = RES_NW;
     const int lds = pair_lds_bytes(PSV, TX, V);
-    if (ra.snap) res_go(c, k_res<SH, PSV, TX, V, NW, FAST, true, true>, NW, lds, a, ra, blocks);
+    if (ra.int_on)
+        res_go(c, k_res<SH, PSV, TX, V, NW, FAST, false, false, true>, NW, lds, a, ra, blocks);
+    else if (ra.snap)
+        res_go(c, k_res<SH, PSV, TX, V, NW, FAST, true, true, false>, NW, lds, a, ra, blocks);
     else if (a.rec_ptr)
-        res_go(c, k_res<SH, PSV, TX, V, NW, FAST, false, true>, NW, lds, a, ra, blocks);
-    else res_go(c, k_res<SH, PSV, TX, V, NW, FAST, false, false>, NW, lds, a, ra, blocks);
+        res_go(c, k_res<SH, PSV, TX, V, NW, FAST, false, true, false>, NW, lds, a, ra, blocks);
+    else res_go(c, k_res<SH, PSV, TX, V, NW, FAST, false, false, false>, NW, lds, a, ra, blocks);
 }
 
 template <bool SH, bool PSV, bool FAST>

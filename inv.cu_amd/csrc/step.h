@@ -174,6 +174,16 @@ struct ResArgs {
     signed char *sexp;
     long slot_bytes, tab_bytes;
     int sfe, nsfe, nslots, sbase;
+    // adjoint (int_on): the kernel accumulation after each iteration whose
+    // second step `last` is a multiple of sfe, in the launch, reading
+    // snapshot slot last / sfe of the buffer above (snap, slot_bytes apart;
+    // exponent tables from sexp, tab_bytes apart) and adding into kl, km, kr
+    // (+ shot slot x k_slot_stride) times ndt
+    int int_on;
+    float ndt;
+    float *kl, *km, *kr;
+    long k_slot_stride;
+    int inv_lambda, inv_mu, inv_rho;
 };
 template <bool SH, bool PSV, bool FAST>
 void res_launch(Ctx &c, const StepArgs &a, const ResArgs &ra, unsigned blocks);

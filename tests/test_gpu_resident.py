@@ -79,12 +79,12 @@ def _obs(P):
     return np.stack([oracle_traces(O.forward(s)[0], P) for s in range(P.nsrc)]).astype(np.float32)
 
 
-def _gradient(P, obs, batch, persist, fmt, graphs=True, math=H.INV_MATH_FAST, misfit=0):
+def _gradient(P, obs, batch, persist, fmt, graphs=True, math=H.INV_MATH_FAST, misfit=0, rows=8):
     ctx = P.context(batch=batch)
     ctx.set_math(math)
     ctx.set_graphs(graphs)
     ctx.set_snapshots(fmt, 0)
-    ctx.set_pair(8, 4)
+    ctx.set_pair(rows, 4)
     ctx.set_persistent(persist)
     ctx.prof_enable(1 if persist else 0)
     for s in range(P.nsrc):
@@ -100,6 +100,8 @@ def _gradient(P, obs, batch, persist, fmt, graphs=True, math=H.INV_MATH_FAST, mi
     return f, k, tr, st, n
 
 
+@pytest.mark.parametrize("waves", ["psv", "sh", "sh+psv"])
+@pytest.mark.parametrize("rows", [8, 4])
 @pytest.mark.parametrize("batch", [1, 3])
 @pytest.mark.parametrize("fmt,graphs,math", [(H.INV_SNAP_F32, True, H.INV_MATH_FAST),
                                              (H.INV_SNAP_F32, False, H.INV_MATH_EXACT),
@@ -107,20 +109,24 @@ def _gradient(P, obs, batch, persist, fmt, graphs=True, math=H.INV_MATH_FAST, mi
                                              (H.INV_SNAP_F16, False, H.INV_MATH_FAST)],
                          ids=["f32-graphs-fast", "f32-direct-exact", "i16-graphs-fast",
                               "f16-direct-fast"])
-def test_resident_gradient_equals_pair(gpu, batch, fmt, graphs, math):
+def test_resident_gradient_equals_pair(gpu, waves, rows, batch, fmt, graphs, math):
     """Gradient (forward with snapshots, misfit, adjoint with kernel
-    accumulations every sfe steps) and a following forward: the per-pair
-    launches' results bit for bit (the same 240-column snapshot segments)."""
-    P = res_problem(sh=1, psv=1, nt=300, explosive=True)
+    accumulations every sfe steps: in the launch for one wave type, after
+    launches of the pairs up to each accumulation for both) and a following
+    forward: the per-pair launches' results bit for bit (the same
+    240-column snapshot segments)."""
+    sh, psv = {"psv": (0, 1), "sh": (1, 0), "sh+psv": (1, 1)}[waves]
+    P = res_problem(sh=sh, psv=psv, nt=300, explosive=bool(psv))
     P = inv.problems.shots(P, np.arange(batch))
     obs = _obs(P)
-    print(f"resident gradient batch={batch} fmt={fmt} graphs={graphs} math={math}", flush=True)
-    f1, k1, t1, s1, n1 = _gradient(P, obs, batch, 1, fmt, graphs, math)
-    f0, k0, t0, s0, _ = _gradient(P, obs, batch, 0, fmt, graphs, math)
+    print(f"resident gradient {waves} rows={rows} batch={batch} fmt={fmt} graphs={graphs} "
+          f"math={math}", flush=True)
+    f1, k1, t1, s1, n1 = _gradient(P, obs, batch, 1, fmt, graphs, math, rows=rows)
+    f0, k0, t0, s0, _ = _gradient(P, obs, batch, 0, fmt, graphs, math, rows=rows)
     assert n1 > 0 or graphs, "persistent launches ran"
     assert f1 == f0
-    for a, b in zip(k1, k0):
-        assert np.abs(b).max() > 0
+    assert np.abs(k0[1]).max() > 0 and np.abs(k0[2]).max() > 0  # k_mu, k_rho
+    for a, b in zip(k1, k0):  # k_lambda is zero for SH alone
         assert np.array_equal(a, b), f"kernels differ, rel L2 {rel(a, b):.3e}"
     for a, b in zip(t1 + s1, t0 + s0):
         assert np.array_equal(a, b)
