@@ -27,7 +27,8 @@ With the default workload, c1 and c2 are also timed (2 gradients each) and
 reported under "extra_workloads" (--extras '' skips them).
 
 Prints one JSON line.  Extra keys: roofline (the dominant stencil kernel
-against the HBM peak, timed live with HIP events on the library's stream),
+against the HBM peak, timed live with HIP events on the library's stream, and
+against the bandwidth a plain device copy of 1 GiB reaches on the same GPU),
 cpu_baseline (the CPU oracle on a bounded sample of the same workload, on the
 threads OMP_NUM_THREADS allows, plus a one-thread sample), accuracy (c2, c4:
 gradient error of each snapshot format against the FP64 oracle), serving
@@ -479,6 +480,31 @@ def run_workload(inv, a, name, steps, warmup, headline):
     return rec, P, gpu_grad
 
 
+def copy_bandwidth(inv, mib=1024, reps=10):
+    """The device's practical read+write bandwidth: reps back-to-back device
+    copies of `mib` MiB (the library's vec_copy, 16-byte vectors) between two
+    synchronisations, best of three such runs; bytes read plus written per
+    second."""
+    P = inv.problems.tiny(nt=10, nsrc=1, nrec=1)
+    ctx = P.context(batch=1, adj=0)
+    nb = mib << 20
+    src, dst = ctx.alloc(nb), ctx.alloc(nb)
+    ctx.vec_fill(src, 1.0, nb // 4)
+    ctx.vec_copy(dst, src, nb // 4)
+    ctx.sync()
+    best = 1e30
+    for _ in range(3):
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.vec_copy(dst, src, nb // 4)
+        ctx.sync()
+        best = min(best, (time.perf_counter() - t0) / reps)
+    ctx.free(src)
+    ctx.free(dst)
+    ctx.close()
+    return 2 * nb / best / 1e9
+
+
 def plain_c1(inv, a):
     """The one-shot-per-launch GPU path (no batching, no graph, one step per
     launch) on c1: the plain comparison point of the batched path."""
@@ -565,6 +591,12 @@ def main():
             print(json.dumps(s), flush=True)
         return
     head, P, gpu_grad = run_workload(inv, a, a.workload, a.steps, a.warmup, headline=True)
+    if head["roofline"]:
+        # the achieved bandwidth against what a plain device copy reaches
+        copy = copy_bandwidth(inv)
+        head["roofline"]["copy_gbps"] = round(copy, 1)
+        head["roofline"]["frac_of_copy"] = round(head["roofline"]["achieved"] / copy, 4)
+        log(f"# device copy {copy:.1f} GB/s")
     extras_names = a.extras if a.extras is not None else ("c1,c2" if a.workload == "c4" else "")
     extras = {}
     for w in [x for x in extras_names.split(",") if x and x != a.workload]:

@@ -119,9 +119,25 @@ __global__ void k_mul(float *__restrict__ c, const float *__restrict__ a,
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
         c[k] = a[k] * b[k];
 }
+// c = a: 16-byte vectors over the 16-byte-aligned part (device buffers are
+// 256-byte aligned; an offset pointer copies its unaligned head and tail one
+// float at a time), four vectors in flight per thread
 __global__ void k_copy(float *__restrict__ c, const float *__restrict__ a, long n) {
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
-        c[k] = a[k];
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, nt = (long)gridDim.x * blockDim.x;
+    const bool al = ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(c)) & 15) == 0;
+    const long nv = al ? n / 4 : 0;
+    const float4 *a4 = reinterpret_cast<const float4 *>(a);
+    float4 *c4 = reinterpret_cast<float4 *>(c);
+    long k = t;
+    for (; k + 3 * nt < nv; k += 4 * nt) {
+        const float4 x0 = a4[k], x1 = a4[k + nt], x2 = a4[k + 2 * nt], x3 = a4[k + 3 * nt];
+        c4[k] = x0;
+        c4[k + nt] = x1;
+        c4[k + 2 * nt] = x2;
+        c4[k + 3 * nt] = x3;
+    }
+    for (; k < nv; k += nt) c4[k] = a4[k];
+    for (long e = 4 * nv + t; e < n; e += nt) c[e] = a[e];
 }
 __global__ void k_fill1(float *__restrict__ c, float v, long n) {
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
@@ -147,7 +163,7 @@ void vec_mul(Ctx &c, float *out, const float *a, const float *b, size_t n) {
     INV_HIP(hipGetLastError());
 }
 void vec_copy(Ctx &c, float *out, const float *a, size_t n) {
-    k_copy<<<ew_grid(n), 256, 0, c.stream>>>(out, a, (long)n);
+    k_copy<<<ew_grid((n + 3) / 4), 256, 0, c.stream>>>(out, a, (long)n);
     INV_HIP(hipGetLastError());
 }
 void vec_fill(Ctx &c, float *out, float v, size_t n) {
