@@ -68,10 +68,10 @@ SH_STEP_BYTES = 4 * (2 * 4 + 2)
 #                                 profiles/k_colv_psv_marmousi737x240_16shots.md
 #   c4 two-step 20 x 2, 4 shots:  70.75 B per launch of two steps (algorithmic 68),
 #                                 profiles/k_pair_psv_4096x2048_4shots.md
-#   c2 two-step 16 x 2, 64 shots: 40.36 B per launch of two steps (algorithmic 40),
+#   c2 two-step 20 x 2, 64 shots: 40.37 B per launch of two steps (algorithmic 40),
 #                                 profiles/k_pair_sh_marmousi1474x480_64shots.md
 MEASURED_TRAFFIC_PER_CELL = {("c1", 1, 4, 2, 16): 68.30, ("c4", 2, 20, 2, 4): 70.75,
-                             ("c2", 2, 16, 2, 64): 40.36}
+                             ("c2", 2, 20, 2, 64): 40.37}
 
 
 def log(msg):
@@ -119,9 +119,9 @@ def parse(argv=None):
     ap.add_argument("--graphs", type=int, default=1, help="capture time loops in HIP graphs")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run one plain device copy of this many MiB (counter calibration)")
-    ap.add_argument("--serve-requests", type=int, default=24,
+    ap.add_argument("--serve-requests", type=int, default=48,
                     help="mixed-size serving leg: requests per run (0 = skip)")
-    ap.add_argument("--serve-rate-x", type=float, default=2.0,
+    ap.add_argument("--serve-rate-x", type=float, default=4.0,
                     help="serving leg arrival rate, in multiples of the plain loop's capacity")
     ap.add_argument("--serving-sweep", type=int, default=0,
                     help="run only the serving sweep over this many requests (one JSON line "
