@@ -166,7 +166,8 @@ int inv_obs_from_traces(inv_ctx *ctx, int first, int count);
 
 /* ---- misfit ------------------------------------------------------------- */
 /* Misfit of slots 0..count-1 (shots first..) against resident obs:
- * type 0 waveform (misfit/waveform.h:4-50), 1 envelope (misfit/envelope.h).
+ * type 0 waveform (misfit/waveform.h:4-50), 1 envelope (misfit/envelope.h;
+ * nt <= 2^20, INV_ERR_UNSUPPORTED beyond).
  * Writes the slots' ADSTF and, if chi != NULL, per-trace misfits
  * chi[count][nrec][ncomp] (receiver-major as Misfit::run sums them,
  * misfit/misfit.h:31-40).  *total (FP64) = their sum. */

@@ -203,7 +203,7 @@ std::vector<DevBuf *> ctx_buffers(Ctx &c) {
             &c.rec_list.row, &c.gauss_w, &c.gauss_tmp, &c.gauss_sx, &c.gauss_sz, &c.vx_b,
             &c.vz_b, &c.sxx_b, &c.szz_b, &c.sxz_b, &c.vy_b, &c.sxy_b, &c.szy_b, &c.f_rec_v.ptr,
             &c.f_rec_v.cell, &c.f_rec_v.row, &c.f_rec_t.ptr, &c.f_rec_t.cell, &c.f_rec_t.row,
-            &c.env_w, &c.env_fac, &c.env_scratch, &c.env_blue, &c.bdt,
+            &c.env_w, &c.env_fac, &c.env_scratch, &c.env_blue, &c.env_gbuf, &c.bdt,
             &c.slot_model, &c.xbuf, &c.res_ctl};
 }
 

@@ -184,9 +184,10 @@ struct Ctx {
 
     // envelope misfit: FFT twiddles and radix plan for length nt, or (env_m >
     // 0) for the power-of-two length env_m of a Bluestein transform, whose
-    // chirp and filter spectra live in env_blue
-    DevBuf env_w, env_fac, env_scratch, env_blue;
-    int env_n = -1, env_nfac = 0, env_m = 0;
+    // chirp and filter spectra live in env_blue; env_global: the transforms
+    // run through per-trace buffers in env_gbuf instead of LDS (too long)
+    DevBuf env_w, env_fac, env_scratch, env_blue, env_gbuf;
+    int env_n = -1, env_nfac = 0, env_m = 0, env_global = 0;
 
     // filter
     DevBuf gauss_w, gauss_tmp, gauss_sx, gauss_sz;

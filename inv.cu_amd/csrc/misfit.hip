@@ -126,10 +126,15 @@ double misfit_trace(Ctx &c, int type, const float *syn, const float *obs, float 
 // (radix 4, 2, 3, 5, 7; twiddles from a host FP64 table rounded to FP32).
 // A length with another prime factor (cuFFT takes any length) goes through
 // Bluestein's algorithm: the DFT of length n as a convolution with a chirp,
-// done by power-of-two transforms of length M >= 2n - 1 (bluestein_lds).
+// done by power-of-two transforms of length M >= 2n - 1.  A transform longer
+// than the LDS holds (n or M above ENV_MAXN) runs the same passes out of place
+// between two per-trace buffers in global memory (the G variant), traces in
+// chunks of at most ENV_CHUNK_BYTES of buffers.
 // ---------------------------------------------------------------------------
 constexpr int EB = 1024;               // threads per trace
-constexpr int ENV_MAXN = 16 * EB;      // longest trace (elements per thread <= 16)
+constexpr int ENV_MAXN = 16 * EB;      // longest LDS transform (elements per thread <= 16)
+constexpr int ENV_MAXN_G = 1 << 20;    // longest trace
+constexpr size_t ENV_CHUNK_BYTES = size_t(512) << 20;
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -182,6 +187,66 @@ __device__ __noinline__ void fft_pass(float2 *buf, int n, int ns, const float2 *
     __syncthreads();
 }
 
+// The same pass from src to dst (global memory: no register staging, any n).
+template <int R>
+__device__ __noinline__ void fft_pass_g(const float2 *src, float2 *dst, int n, int ns,
+                                        const float2 *__restrict__ W, bool inv) {
+    const int nb = n / R;
+    const int span = n / (ns * R);
+    for (int j = threadIdx.x; j < nb; j += EB) {
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = src[j + r * nb];
+        const int k = j % ns;
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+            float2 w = W[(k * r * span) % n];
+            if (inv) w.y = -w.y;
+            v[r] = cmul(v[r], w);
+        }
+        const int base = (j / ns) * ns * R + k;
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            float2 acc = v[0];
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                float2 w = W[((q * r) % R) * (n / R)];
+                if (inv) w.y = -w.y;
+                const float2 t = cmul(v[r], w);
+                acc.x += t.x;
+                acc.y += t.y;
+            }
+            dst[base + q * ns] = acc;
+        }
+    }
+    __syncthreads();
+}
+
+// The FFT of buf in place through tmp (both n elements, global memory).
+__device__ void fft_g(float2 *buf, float2 *tmp, int n, const int *__restrict__ fac, int nfac,
+                      const float2 *__restrict__ W, bool inv) {
+    int ns = 1;
+    float2 *src = buf, *dst = tmp;
+    for (int f = 0; f < nfac; f++) {
+        const int R = fac[f];
+        switch (R) {
+            case 4: fft_pass_g<4>(src, dst, n, ns, W, inv); break;
+            case 2: fft_pass_g<2>(src, dst, n, ns, W, inv); break;
+            case 3: fft_pass_g<3>(src, dst, n, ns, W, inv); break;
+            case 5: fft_pass_g<5>(src, dst, n, ns, W, inv); break;
+            default: fft_pass_g<7>(src, dst, n, ns, W, inv); break;
+        }
+        ns *= R;
+        float2 *t = src;
+        src = dst;
+        dst = t;
+    }
+    if (src != buf) {
+        for (int i = threadIdx.x; i < n; i += EB) buf[i] = src[i];
+        __syncthreads();
+    }
+}
+
 __device__ void fft_lds(float2 *buf, int n, const int *__restrict__ fac, int nfac,
                         const float2 *__restrict__ W, bool inv) {
     int ns = 1;
@@ -213,10 +278,19 @@ struct EnvPlan {
     const float2 *chirp, *bf, *bi;
 };
 
-// the DFT of buf[0 .. n-1] in place (buf holds m elements for Bluestein)
-__device__ void dft_lds(float2 *buf, int n, const EnvPlan &p, bool inv) {
+// an FFT of length len in LDS (G false) or through tmp (G true)
+template <bool G>
+__device__ void fft_any(float2 *buf, float2 *tmp, int len, const EnvPlan &p, bool inv) {
+    if (G) fft_g(buf, tmp, len, p.fac, p.nfac, p.W, inv);
+    else fft_lds(buf, len, p.fac, p.nfac, p.W, inv);
+}
+
+// the DFT of buf[0 .. n-1] in place (buf holds m elements for Bluestein;
+// tmp as many, for the G variant)
+template <bool G>
+__device__ void dft_any(float2 *buf, float2 *tmp, int n, const EnvPlan &p, bool inv) {
     if (p.m == 0) {
-        fft_lds(buf, n, p.fac, p.nfac, p.W, inv);
+        fft_any<G>(buf, tmp, n, p, inv);
         return;
     }
     for (int i = threadIdx.x; i < p.m; i += EB) {
@@ -229,11 +303,11 @@ __device__ void dft_lds(float2 *buf, int n, const EnvPlan &p, bool inv) {
         buf[i] = a;
     }
     __syncthreads();
-    fft_lds(buf, p.m, p.fac, p.nfac, p.W, false);
+    fft_any<G>(buf, tmp, p.m, p, false);
     const float2 *B = inv ? p.bi : p.bf;
     for (int i = threadIdx.x; i < p.m; i += EB) buf[i] = cmul(buf[i], B[i]);
     __syncthreads();
-    fft_lds(buf, p.m, p.fac, p.nfac, p.W, true);
+    fft_any<G>(buf, tmp, p.m, p, true);
     for (int i = threadIdx.x; i < n; i += EB) {
         float2 w = p.chirp[i];
         if (inv) w.y = -w.y;
@@ -245,8 +319,9 @@ __device__ void dft_lds(float2 *buf, int n, const EnvPlan &p, bool inv) {
 // analytic signal in place (EnvelopeMisfit::hilbert + _hilbert mask,
 // misfit/envelope.h:4-28,87-93): bin 0 kept, positive bins doubled, the rest
 // zeroed, except that bin n/2+1 of an even length is left untouched.
-__device__ void analytic_lds(float2 *buf, int n, const EnvPlan &p) {
-    dft_lds(buf, n, p, false);
+template <bool G>
+__device__ void analytic(float2 *buf, float2 *tmp, int n, const EnvPlan &p) {
+    dft_any<G>(buf, tmp, n, p, false);
     for (int i = threadIdx.x; i < n; i += EB) {
         if (i == 0) continue;
         float2 h = buf[i];
@@ -260,7 +335,7 @@ __device__ void analytic_lds(float2 *buf, int n, const EnvPlan &p) {
         buf[i] = h;
     }
     __syncthreads();
-    dft_lds(buf, n, p, true);
+    dft_any<G>(buf, tmp, n, p, true);
 }
 
 template <typename T, typename Op>
@@ -276,26 +351,43 @@ __device__ T env_reduce(T v, Op op) {
     return s;  // every thread gets the result
 }
 
+// One workgroup per trace t = t0 + blockIdx.y * gridDim.x + blockIdx.x of
+// the [slot][component][receiver] traces.  G false: the transform buffer in
+// LDS, per-sample intermediates at 3 nt t in scratch.  G true: trace t's
+// buffers at (t - t0) gstride floats in scratch: the transform buffer and
+// its partner (len elements each, len = m or nt), then the intermediates.
+template <bool G>
 __global__ __launch_bounds__(EB) void k_envelope(const float *__restrict__ syn_all,
                                                  const float *__restrict__ obs_all,
                                                  float *__restrict__ adstf_all,
                                                  const EnvPlan plan,
                                                  double *__restrict__ chi,
                                                  float *__restrict__ scratch, int nt, int nrec,
-                                                 int ncomp, float sqrt_dt) {
-    extern __shared__ float2 ebuf[];
-    const int c = blockIdx.x / nrec, ir = blockIdx.x % nrec, b = blockIdx.y;
-    const long base = ((long)b * ncomp + c) * nrec * nt + (long)ir * nt;
+                                                 int ncomp, float sqrt_dt, long t0, long gstride) {
+    extern __shared__ float2 lbuf[];
+    const long t = t0 + (long)blockIdx.y * gridDim.x + blockIdx.x;
+    const int b = (int)(t / ((long)ncomp * nrec));
+    const int rem = (int)(t - (long)b * ncomp * nrec);
+    const int c = rem / nrec, ir = rem % nrec;
+    const long base = t * nt;
     const float *syn = syn_all + base, *obs = obs_all + base;
     float *ad = adstf_all + base;
     const int n = nt;
     const float fn = (float)n;
+    const int len = plan.m ? plan.m : n;
+    float2 *ebuf = lbuf, *tmp = nullptr;
     // per-sample intermediates; element i is only touched by thread i % EB
-    float *esyn = scratch + 3 * base, *hsyn = esyn + n, *etmp = esyn + 2 * n;
+    float *esyn = scratch + 3 * base;
+    if (G) {
+        ebuf = reinterpret_cast<float2 *>(scratch + (t - t0) * gstride);
+        tmp = ebuf + len;
+        esyn = reinterpret_cast<float *>(tmp + len);
+    }
+    float *hsyn = esyn + n, *etmp = esyn + 2 * n;
 
     for (int i = threadIdx.x; i < n; i += EB) ebuf[i] = make_float2(syn[i], 0.0f);  // copyR2C
     __syncthreads();
-    analytic_lds(ebuf, n, plan);
+    analytic<G>(ebuf, tmp, n, plan);
     float amax = 0.0f;
     for (int i = threadIdx.x; i < n; i += EB) {
         const float2 a = ebuf[i];
@@ -307,7 +399,7 @@ __global__ __launch_bounds__(EB) void k_envelope(const float *__restrict__ syn_a
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += EB) ebuf[i] = make_float2(obs[i], 0.0f);
     __syncthreads();
-    analytic_lds(ebuf, n, plan);
+    analytic<G>(ebuf, tmp, n, plan);
     amax = env_reduce<float>(amax, [](float x, float y) { return x > y ? x : y; });
     // max = max(0.05 * amax(E_syn), 0.05) with double literals (envelope.h:70-73)
     float mx = (float)((double)amax * 0.05);
@@ -326,7 +418,7 @@ __global__ __launch_bounds__(EB) void k_envelope(const float *__restrict__ syn_a
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += EB) ebuf[i] = make_float2(hsyn[i], 0.0f);
     __syncthreads();
-    analytic_lds(ebuf, n, plan);
+    analytic<G>(ebuf, tmp, n, plan);
     for (int i = threadIdx.x; i < n; i += EB) {
         const float ersd = ebuf[i].y / fn;
         ad[n - 1 - i] = etmp[i] * syn[i] - ersd;  // prepareEnvelopeSTF
@@ -357,14 +449,20 @@ static void fft_host(std::vector<std::complex<double>> &a) {
     }
 }
 
-// LDS elements one trace's transform needs
-static int env_lds_elems(const Ctx &c) { return c.env_m ? c.env_m : c.env_n; }
+// elements of one trace's transform (LDS elements of the LDS variant)
+static int env_len(const Ctx &c) { return c.env_m ? c.env_m : c.env_n; }
+// floats of one trace's buffers in the global variant: two transform buffers
+// and three nt-sample intermediates, rounded to an even count (float2 bases)
+static long env_gstride(const Ctx &c) {
+    const long f = 4L * env_len(c) + 3L * c.env_n;
+    return (f + 1) & ~1L;
+}
 
 static void envelope_setup(Ctx &c) {
     const int n = c.g.nt;
     if (c.env_n == n) return;
-    INV_REQUIRE(n >= 2 && n <= ENV_MAXN, INV_ERR_UNSUPPORTED,
-                "envelope misfit supports 2 <= nt <= " + std::to_string(ENV_MAXN));
+    INV_REQUIRE(n >= 2 && n <= ENV_MAXN_G, INV_ERR_UNSUPPORTED,
+                "envelope misfit supports 2 <= nt <= " + std::to_string(ENV_MAXN_G));
     auto factor = [](int len) {
         std::vector<int> f;
         for (int p : {4, 2, 3, 5, 7})
@@ -379,16 +477,13 @@ static void envelope_setup(Ctx &c) {
     if (rest != 1) {
         m = 1;
         while (m < 2 * n - 1) m <<= 1;
-        INV_REQUIRE(m <= ENV_MAXN, INV_ERR_UNSUPPORTED,
-                    "envelope misfit: nt = " + std::to_string(n) +
-                        " has a prime factor above 7 and is longer than " +
-                        std::to_string(ENV_MAXN / 2));
         fac = factor(m).first;
     }
     const int len = m ? m : n;
+    const bool global = len > ENV_MAXN;
     const size_t lds = (size_t)len * sizeof(float2);
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_envelope),
+    if (!global && lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_envelope<false>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) {
             (void)hipGetLastError();
@@ -440,6 +535,7 @@ static void envelope_setup(Ctx &c) {
     c.env_nfac = (int)fac.size();
     c.env_m = m;
     c.env_n = n;
+    c.env_global = global ? 1 : 0;
 }
 
 static EnvPlan env_plan(Ctx &c) {
@@ -454,25 +550,47 @@ static EnvPlan env_plan(Ctx &c) {
     return p;
 }
 
+// k_envelope over `traces` traces of the [slot][component][receiver] layout:
+// one launch (LDS variant) or launches of at most ENV_CHUNK_BYTES of
+// per-trace buffers each (global variant)
+static void envelope_launch(Ctx &c, const float *syn, const float *obs, float *adstf,
+                            double *chi, long traces, int nrec, int ncomp) {
+    const EnvPlan plan = env_plan(c);
+    const float sdt = sqrtf(c.g.dt);
+    if (!c.env_global) {
+        c.env_scratch.ensure(3 * (size_t)traces * c.g.nt * sizeof(float));
+        dim3 grid(ncomp * nrec, (unsigned)(traces / ((long)ncomp * nrec)));
+        k_envelope<false><<<grid, EB, (size_t)env_len(c) * sizeof(float2), c.stream>>>(
+            syn, obs, adstf, plan, chi, c.env_scratch.as<float>(), c.g.nt, nrec, ncomp, sdt, 0,
+            0);
+        INV_HIP(hipGetLastError());
+        return;
+    }
+    const long gs = env_gstride(c);
+    const long per = gs * (long)sizeof(float);
+    long chunk = (long)(ENV_CHUNK_BYTES / per);
+    if (chunk < 1) chunk = 1;
+    if (chunk > traces) chunk = traces;
+    c.env_gbuf.ensure((size_t)chunk * per);
+    for (long t0 = 0; t0 < traces; t0 += chunk) {
+        const long k = traces - t0 < chunk ? traces - t0 : chunk;
+        k_envelope<true><<<dim3((unsigned)k, 1), EB, 0, c.stream>>>(
+            syn, obs, adstf, plan, chi, c.env_gbuf.as<float>(), c.g.nt, nrec, ncomp, sdt, t0, gs);
+        INV_HIP(hipGetLastError());
+    }
+}
+
 void envelope_trace(Ctx &c, const float *syn, const float *obs, float *adstf, double *chi_dev) {
     envelope_setup(c);
-    c.env_scratch.ensure(3 * (size_t)c.g.nt * sizeof(float));
-    k_envelope<<<dim3(1, 1), EB, (size_t)env_lds_elems(c) * sizeof(float2), c.stream>>>(
-        syn, obs, adstf, env_plan(c), chi_dev, c.env_scratch.as<float>(), c.g.nt, 1, 1,
-        sqrtf(c.g.dt));
-    INV_HIP(hipGetLastError());
+    envelope_launch(c, syn, obs, adstf, chi_dev, 1, 1, 1);
 }
 
 void misfit_envelope(Ctx &c, int count, int first) {
     envelope_setup(c);
     const long tr = (long)c.ncomp * c.nrec * c.g.nt;
-    c.env_scratch.ensure(3 * (size_t)count * tr * sizeof(float));
-    dim3 grid(c.ncomp * c.nrec, count);
-    k_envelope<<<grid, EB, (size_t)env_lds_elems(c) * sizeof(float2), c.stream>>>(
-        c.out.as<float>(), c.obs.as<float>() + (long)first * tr, c.adstf.as<float>(),
-        env_plan(c), c.chi.as<double>(), c.env_scratch.as<float>(), c.g.nt, c.nrec, c.ncomp,
-        sqrtf(c.g.dt));
-    INV_HIP(hipGetLastError());
+    envelope_launch(c, c.out.as<float>(), c.obs.as<float>() + (long)first * tr,
+                    c.adstf.as<float>(), c.chi.as<double>(), (long)count * c.ncomp * c.nrec,
+                    c.nrec, c.ncomp);
 }
 
 }  // namespace inv

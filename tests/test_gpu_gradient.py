@@ -146,31 +146,40 @@ def test_misfit_envelope_matches_oracle(gpu, nt):
 
 
 def test_envelope_rejects_unsupported_length(gpu):
-    """Bluestein transforms need 2 nt - 1 <= 16384 points of LDS: a prime nt
-    above 8192 is refused (a 7-smooth one up to 16384 is not)."""
-    P = inv.problems.tiny(sh=1, psv=0, nsrc=1, nx=24, nz=20, nrec=2, nt=8209, sfe=1)
+    """Traces longer than 2^20 samples are refused before anything runs."""
+    nt = (1 << 20) + 1
+    P = inv.problems.tiny(sh=1, psv=0, nsrc=1, nx=24, nz=20, nrec=2, nt=nt, sfe=1)
     ctx = P.context(batch=1, adj=0)
-    ctx.forward(0, 1, snapshots=False)
+    d = [ctx.alloc(4 * nt) for _ in range(3)]
     with pytest.raises(inv.invhip.InvError) as e:
-        ctx.misfit(0, 1, kind=1)
+        ctx.misfit_trace(1, d[0], d[1], d[2])
     assert e.value.code == -5
     ctx.close()
 
 
-@pytest.mark.parametrize("nt", [8190, 4099])
+def _envelope_pair(nt, seed):
+    rng = np.random.default_rng(seed)
+    t = np.arange(nt) * 1e-3
+    syn = (np.sin(40 * t) * np.exp(-3 * t) + 0.1 * rng.standard_normal(nt)).astype(np.float32)
+    obs = (np.sin(41 * t) * np.exp(-3 * t) + 0.1 * rng.standard_normal(nt)).astype(np.float32)
+    return syn, obs
+
+
+@pytest.mark.parametrize("nt", [8190, 4099, 8320, 16640, 20000])
 def test_envelope_long_bluestein_trace(gpu, nt):
-    """The longest Bluestein transform (8190 = 2 3^2 5 7 13: length 16384)
-    and a prime length on one device trace against the FP64 oracle."""
+    """Long traces on one device trace against the FP64 oracle: the longest
+    Bluestein transform held in LDS (8190 = 2 3^2 5 7 13: length 16384), a
+    prime length, and transforms run through global memory: Bluestein of
+    length 32768 (8320 = 2^7 5 13) and 65536 (16640 = 2^8 5 13), and the
+    direct mixed-radix transform of 20000 = 2^5 5^4."""
     import pyoracle
     P = inv.problems.tiny(sh=1, psv=0, nsrc=1, nx=24, nz=20, nrec=2, nt=nt, sfe=1)
     ctx = P.context(batch=1, adj=0)
-    rng = np.random.default_rng(nt)
-    t = np.arange(nt) * P.dt
-    syn = (np.sin(40 * t) * np.exp(-3 * t) + 0.1 * rng.standard_normal(nt)).astype(np.float32)
-    obs = (np.sin(41 * t) * np.exp(-3 * t) + 0.1 * rng.standard_normal(nt)).astype(np.float32)
+    syn, obs = _envelope_pair(nt, nt)
     d = [ctx.alloc(4 * nt) for _ in range(3)]
     ctx.h2d(d[0], syn)
     ctx.h2d(d[1], obs)
+    print(f"envelope trace nt={nt}", flush=True)
     chi = ctx.misfit_trace(1, d[0], d[1], d[2])
     x, a = pyoracle.envelope(syn.astype(np.float64), obs.astype(np.float64), P.dt, np.float64)
     e = rel(ctx.d2h(d[2], nt), a)
@@ -178,6 +187,44 @@ def test_envelope_long_bluestein_trace(gpu, nt):
     assert abs(chi - x) <= 1e-4 * x
     assert e < 1e-4
     ctx.close()
+
+
+def test_envelope_global_transforms_in_chunks(gpu):
+    """The envelope misfit of 960 traces of 8320 samples (Bluestein of length
+    32768 through global memory, 860 traces per launch: two launches): the
+    misfit of every trace, and the adjoint sources of the first and last
+    trace of each launch and of both shots, against the FP64 oracle; the
+    total is the sum of the traces'."""
+    import pyoracle
+    nt = 8320
+    P = inv.problems.tiny(sh=0, psv=1, nsrc=2, nx=250, nz=20, nrec=240, nt=nt, sfe=10)
+    tctx = P.context(batch=2, model="true", adj=0)
+    tctx.forward(0, 2, snapshots=False)
+    obs = np.stack([tctx.traces_get(b) for b in range(2)])
+    tctx.close()
+    ctx = P.context(batch=2)
+    for s in range(2):
+        ctx.obs_set(s, obs[s])
+    ctx.forward(0, 2, snapshots=False)
+    print("envelope misfit of 960 traces, nt 8320", flush=True)
+    tot, chi = ctx.misfit(0, 2, kind=1)
+    syn = [ctx.traces_get(s) for s in range(2)]
+    ad = [ctx.adstf_get(s) for s in range(2)]
+    ctx.close()
+    assert abs(tot - chi.sum()) <= 1e-12 * abs(tot)
+    flat = [(s, c, r) for s in range(2) for c in range(2) for r in range(240)]
+    check = {0, 1, 859, 860, 861, 959, 480, 479}
+    worst = 0.0
+    for t, (s, c, r) in enumerate(flat):
+        if t not in check and t % 37:
+            continue
+        x, a = pyoracle.envelope(syn[s][c, r].astype(np.float64),
+                                 obs[s, c, r].astype(np.float64), P.dt, np.float64)
+        assert abs(chi[s, r, c] - x) <= 1e-4 * abs(x) + 1e-12, f"trace {t}"
+        if t in check:
+            worst = max(worst, rel(ad[s][c, r], a))
+    print(f"worst adjoint source rel err vs FP64 {worst:.2e}")
+    assert worst < 1e-4
 
 
 def test_incremental_run_equals_whole_run(gpu):
