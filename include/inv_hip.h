@@ -340,9 +340,15 @@ int inv_set_step_limit(inv_ctx *ctx, int nsteps);
 /* Forward-snapshot storage for the adjoint.  No counterpart in the reference,
  * which keeps FP32 snapshots of every sfe-th step in host memory
  * (solver/fdm.h:403-425,521-560).
- *   format   INV_SNAP_F32, or INV_SNAP_F16 / INV_SNAP_BF16: each snapshot
- *            field is stored as x * 2^15 / max|x| rounded to nearest even and
- *            decoded before use; all arithmetic stays FP32.
+ *   format   INV_SNAP_F32, or a 16-bit format: each row segment of a field
+ *            (the owned columns of one tile of the kernel that writes it) is
+ *            scaled by the power of two 2^e that brings its max-abs into
+ *            [2^14, 2^15), each block of 4 cells by a further 2^d (d in
+ *            0..15, from the block's own max-abs), computed in the launch
+ *            that writes the snapshot and kept in a per-slot exponent table;
+ *            the scaled value is stored as INV_SNAP_I16 (nearest integer,
+ *            |q| <= 32767), INV_SNAP_F16 or INV_SNAP_BF16 (nearest, ties to
+ *            even) and decoded before use; all arithmetic stays FP32.
  *   ckpt     0: every snapshot of the run is kept (nt/sfe slots per shot);
  *            K > 0 (K % sfe == 0, nt % K == 0): the forward keeps the FP32
  *            state every K steps plus the snapshots of its last K steps, and

@@ -256,6 +256,7 @@ class FdmSolver : public Solver {
     inv_ctx *ctx = nullptr;
     int batch = 1;
     int ncomp = 0;
+    bool tune_pending = false;  // extension key autotune
     float xmax = 0, zmax = 0;
 
     ~FdmSolver() override {
@@ -329,6 +330,13 @@ class FdmSolver : public Solver {
         INVH(ctx, inv_grid_set(ctx, &d));
         if (i.count("math")) INVH(ctx, inv_set_math(ctx, i["math"]));
         if (i.count("graphs")) INVH(ctx, inv_set_graphs(ctx, i["graphs"]));
+        // forward-snapshot storage (inv_set_snapshots: 0 FP32, 1 FP16, 2 BF16,
+        // 3 I16; checkpoint interval) and tile autotuning at the first model
+        // load
+        if (i.count("snap_format") || i.count("ckpt"))
+            INVH(ctx, inv_set_snapshots(ctx, i.count("snap_format") ? i["snap_format"] : 0,
+                                        i.count("ckpt") ? i["ckpt"] : 0));
+        tune_pending = i.count("autotune") && i["autotune"];
         // sources: positions (calcIdx, fdm.h:454-455) and wavelets (solver.h:191-200)
         vector<int> sx(nsrc), szi(nsrc);
         vector<float> stfx(nsrc * nt), stfy(nsrc * nt), stfz(nsrc * nt);
@@ -404,6 +412,11 @@ class FdmSolver : public Solver {
         check_npt(n0, n);
         INVH(ctx, inv_model_from_points(ctx, n0, x.data(), z.data(), p1.data(), p2.data(),
                                         pr.data(), xmax, zmax, (int)model_type));
+        if (tune_pending) {
+            tune_pending = false;
+            INVH(ctx, inv_autotune(ctx, 0, batch, (int)std::min<size_t>(nt, 10), nullptr, nullptr,
+                                   nullptr));
+        }
     }
 
     void initKernels() override { INVH(ctx, inv_kernels_zero(ctx)); }

@@ -260,3 +260,30 @@ def test_written_config_runs_the_library_gradient(tmp_path, gpu):
     for c, a, b in zip(("lambda", "mu", "rho"), got, want):
         assert np.abs(b).max() > 0
         assert np.array_equal(a, b), f"k_{c}: rel L2 {rel(a, b):.3e}"
+
+
+@pytest.mark.parametrize("fmt,ckpt,tune", [(3, 0, 0), (1, 100, 0), (0, 0, 1)],
+                         ids=["i16", "f16-ckpt100", "f32-autotune"])
+def test_written_config_snapshot_and_tuning_keys(tmp_path, gpu, fmt, ckpt, tune):
+    """The driver's extension keys snap_format / ckpt (inv_set_snapshots) and
+    autotune (inv_autotune at the first model load): mode 2 exports the
+    kernels the library's gradient gives with the same storage, bit for bit
+    (16-bit snapshots on the default tiles; FP32 snapshots give the same bits
+    on any tile the autotuner picks)."""
+    P = inv.problems.workload("c0", nt=400)[0]
+    root = inv.problems.write_config(P, tmp_path / "c0", mode=2, batch=4, snap_format=fmt,
+                                     ckpt=ckpt, autotune=tune)
+    print(f"c0 written config, snapshots {fmt}, ckpt {ckpt}, autotune {tune}", flush=True)
+    run_cli(root)
+    got = [read_bin(root / "output" / f"proc000000_k_{c}.bin") for c in ("lambda", "mu", "rho")]
+    ctx = P.context(batch=4, model="true")
+    ctx.forward(0, 4, snapshots=False)
+    ctx.obs_from_traces(0, 4)
+    P.load_model(ctx, "init")
+    ctx.set_snapshots(fmt, ckpt)
+    ctx.gradient(P.misfit, P.sigma)
+    want = ctx.kernels_get()
+    ctx.close()
+    for c, a, b in zip(("lambda", "mu", "rho"), got, want):
+        assert np.abs(b).max() > 0
+        assert np.array_equal(a, b), f"k_{c}: rel L2 {rel(a, b):.3e}"
