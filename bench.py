@@ -123,6 +123,9 @@ def parse(argv=None):
                     help="mixed-size serving leg: requests per run (0 = skip)")
     ap.add_argument("--serve-rate-x", type=float, default=4.0,
                     help="serving leg arrival rate, in multiples of the plain loop's capacity")
+    ap.add_argument("--full-gradient", action="store_true",
+                    help="with the 4096x2048 workload: also time one whole gradient of all 256 "
+                         "shots through the command-line driver (about 10 minutes)")
     ap.add_argument("--serving-sweep", type=int, default=0,
                     help="run only the serving sweep over this many requests (one JSON line "
                          "per run) and exit")
@@ -576,6 +579,51 @@ def optimizer_c1(inv, a, iterations=10):
     return rec
 
 
+def full_gradient_c4(inv, a):
+    """The 4096x2048 workload's whole gradient: Misfit::run(kernel=true)
+    (misfit/misfit.h:15-51) over all 256 shots (forward, envelope misfit,
+    adjoint, filter) through the command-line driver's mode 2 on a
+    configuration directory written by problems.write_config.  Three shots
+    per launch: their I16 snapshots (201 GB) and the observed data of all
+    shots (42 GB) fit in HBM together (four shots' 268 GB would not), and a
+    launch of three shots leaves less of its last round of tiles idle than
+    one of two (on 24 of the shots: 32.9 s against 34.5 s).  Tiles are
+    autotuned at the first model load.  The driver times the gradient itself (after setup and the
+    observed-data forwards) and prints a progress line every 16 launches."""
+    import re
+    import subprocess
+    import tempfile
+
+    P = inv.problems.workload("c4", sigma=a.sigma)[0]
+    exe = ROOT / "inv.cu_amd" / "inv.out"
+    with tempfile.TemporaryDirectory(prefix="inv_c4_") as tmp:
+        root = inv.problems.write_config(P, Path(tmp) / "c4", mode=2, batch=3, snap_format=3,
+                                         autotune=1, progress=16)
+        log(f"# c4 full gradient: {P.nsrc} shots through {exe.name}")
+        t0 = time.perf_counter()
+        p = subprocess.Popen([str(exe), str(root)], stdout=subprocess.DEVNULL,
+                             stderr=subprocess.PIPE, text=True)
+        err = []
+        for line in p.stderr:
+            err.append(line)
+            log("#   " + line.rstrip())
+        p.wait(timeout=60)
+        wall = time.perf_counter() - t0
+    m = re.search(r"gradient: ([0-9.eE+-]+) s, misfit (\S+), (\d+) shots", "".join(err))
+    if p.returncode != 0 or not m:
+        log(f"# c4 full gradient failed: rc {p.returncode}")
+        return None
+    secs, shots = float(m.group(1)), int(m.group(3))
+    cells = shots * P.nx * P.nz * P.nt * 2
+    rec = {"shots": shots, "seconds": round(secs, 2), "gcell_updates_per_s": round(cells / secs / 1e9, 2),
+           "gradients_per_hr": round(3600 / secs, 2), "misfit": float(m.group(2)),
+           "wall_s_including_setup": round(wall, 1),
+           "note": "inv.out mode 2 on the written 4096x2048 configuration: all 256 shots, "
+                   "3 per launch, I16 snapshots, envelope misfit, sigma " + str(P.sigma)}
+    log(f"# c4 full gradient: {rec}")
+    return rec
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -640,6 +688,7 @@ def main():
     if not a.no_accuracy:
         acc = {w: accuracy(inv, w) for w in ("c4", "c2")
                if w == a.workload or (w in extras and a.workload == "c4")}
+    full = full_gradient_c4(inv, a) if a.full_gradient and a.workload == "c4" else None
     serving = None
     if a.serve_requests > 0 and a.workload in ("c1", "c4"):
         serving = serving_leg(inv, a.serve_requests, a.serve_rate_x, inv.invhip.INV_SNAP_I16)
@@ -668,6 +717,8 @@ def main():
     out.update({"roofline": head["roofline"], "cpu_baseline": cpu, "plain": plain,
                 "serving": serving, "accuracy": acc, "extra_workloads": extras or None,
                 "optimizer_c1": opt})
+    if full:
+        out["full_gradient_c4"] = full
     if parity:
         out["parity"] = parity
     print(json.dumps(out))

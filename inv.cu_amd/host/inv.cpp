@@ -7,8 +7,12 @@
 //   mode 4  device query
 //   mode 5  model export on the grid
 // Keys beyond the reference's: batch (shots per launch, default 16), math
-// (0 exact, 1 fast), graphs (0/1), device.
+// (0 exact, 1 fast), graphs (0/1), device, snap_format (0 FP32, 1 FP16,
+// 2 BF16, 3 I16), ckpt (checkpoint interval), autotune (0/1), lbfgs_fix.
+// Mode 2 also prints the gradient's wall time and misfit on stderr.
 #include <hip/hip_runtime.h>
+
+#include <chrono>
 
 #include "inv.h"
 
@@ -121,7 +125,15 @@ int run(int argc, const char *argv[]) {
             std::unique_ptr<Filter> filter(module::filter(config.i["filter"]));
             auto misfit = need(module::misfit(config.i["misfit"]), "misfit");
             misfit->init(&config, solver.get(), filter.get());
-            misfit->run(true, true);
+            // the gradient's wall time on stderr (the reference's mode 2
+            // prints nothing; stdout stays as it is)
+            const auto t0 = std::chrono::steady_clock::now();
+            const double f = misfit->run(true, true);
+            if (auto *fdm = dynamic_cast<FdmSolver *>(solver.get())) INVH(fdm->ctx, inv_sync(fdm->ctx));
+            const double secs =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            std::cerr << "gradient: " << secs << " s, misfit " << f << ", " << solver->nsrc
+                      << " shots" << std::endl;
             solver->exportAxis();
             solver->exportKernel();
             checkMemoryUsage(free_memory);

@@ -642,6 +642,7 @@ class Misfit {
 
   public:
     float ref = 1;
+    int progress = 0;  // extension key: a line on stderr every `progress` batches
 
     // Misfit::run (misfit.h:15-51).  Shots run in batches of `batch` per
     // launch; when adjoint wavefield exports are due (snapshot and wae > 0)
@@ -670,6 +671,9 @@ class Misfit {
                 INVH(ctx, inv_misfit(ctx, (int)first, count, type, nullptr, &f));
                 misfit += f;
                 if (kernel) solver->runAdjointBatch((int)first, count);
+                if (progress && (first / solver->batch) % progress == 0)
+                    std::cerr << (kernel ? "gradient" : "misfit") << ": shots " << first + count
+                              << " of " << solver->nsrc << std::endl;
             }
         }
         if (kernel && filter) {
@@ -684,6 +688,7 @@ class Misfit {
     // (trace_file = 1, velocity integrated to displacement when trace_type
     // is 0) or modelled on model_true, then model_init is loaded.
     virtual void init(Config *config, Solver *s, Filter *fl) {
+        progress = config->i.count("progress") ? config->i["progress"] : 0;
         solver = dynamic_cast<FdmSolver *>(s);
         if (!solver) throw std::runtime_error("Misfit needs the FDM solver");
         filter = fl;
@@ -732,6 +737,9 @@ class Misfit {
                 const int count = (int)std::min<size_t>(solver->batch, solver->nsrc - first);
                 INVH(ctx, inv_forward(ctx, (int)first, count, 0));
                 INVH(ctx, inv_obs_from_traces(ctx, (int)first, count));
+                if (progress && (first / solver->batch) % progress == 0)
+                    std::cerr << "traces: shots " << first + count << " of " << solver->nsrc
+                              << std::endl;
             }
         }
         solver->importModel(false);
