@@ -81,7 +81,8 @@ def log(msg):
 def parse(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=8,
+                    help="timed steps (0 with --full-gradient or --lbfgs: only those legs)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c4", choices=["c0", "c1", "c2", "c4"])
     ap.add_argument("--extras", default=None,
@@ -126,6 +127,12 @@ def parse(argv=None):
     ap.add_argument("--full-gradient", action="store_true",
                     help="with the 4096x2048 workload: also time one whole gradient of all 256 "
                          "shots through the command-line driver (about 10 minutes)")
+    ap.add_argument("--lbfgs", type=int, default=0,
+                    help="with the 4096x2048 workload: also run this many L-BFGS iterations "
+                         "over all 256 shots through the command-line driver (about 12 minutes "
+                         "for one)")
+    ap.add_argument("--lbfgs-shots", type=int, default=256,
+                    help="--lbfgs over every (256 / n)-th shot only (n of the 256)")
     ap.add_argument("--serving-sweep", type=int, default=0,
                     help="run only the serving sweep over this many requests (one JSON line "
                          "per run) and exit")
@@ -624,6 +631,64 @@ def full_gradient_c4(inv, a):
     return rec
 
 
+def lbfgs_c4(inv, a, iterations):
+    """The 4096x2048 workload's inversion: `iterations` L-BFGS iterations
+    (Optimizer::run, optimizer/optimizer.h:428-467, with lbfgs.h:18-91's
+    direction, 5 stored pairs, and the backtracking line search of
+    forward-only trials over all 256 shots, or every (256 / n)-th with
+    --lbfgs-shots n) through the command-line driver
+    on a configuration directory written by problems.write_config, three
+    shots per launch with I16 snapshots as in full_gradient_c4.  Lambda only
+    is inverted, as for c1 (the water layer has mu = 0, which an unbounded
+    mu update drives negative).  The driver's own clock (Elapsed seconds)
+    excludes setup and the observed-data forwards."""
+    import re
+    import subprocess
+    import tempfile
+
+    P = inv.problems.workload("c4", sigma=a.sigma)[0]
+    if a.lbfgs_shots < P.nsrc:
+        P = inv.problems.shots(P, np.arange(0, P.nsrc, P.nsrc // a.lbfgs_shots)[:a.lbfgs_shots])
+    exe = ROOT / "inv.cu_amd" / "inv.out"
+    num = r"(-?(?:nan|inf|[0-9.eE+-]+))"
+    with tempfile.TemporaryDirectory(prefix="inv_c4_") as tmp:
+        root = inv.problems.write_config(P.with_(inv_lambda=1, inv_mu=0, inv_rho=0),
+                                         Path(tmp) / "c4", mode=0, optimizer=1, lbfgs_mem=5,
+                                         inv_iteration=iterations, batch=3, snap_format=3,
+                                         autotune=1, progress=16)
+        log(f"# c4 L-BFGS: {iterations} iteration(s) over {P.nsrc} shots through {exe.name}")
+        t0 = time.perf_counter()
+        p = subprocess.Popen([str(exe), str(root)], stdout=subprocess.PIPE,
+                             stderr=subprocess.STDOUT, text=True)
+        lines = []
+        for line in p.stdout:
+            lines.append(line)
+            log("#   " + line.rstrip())
+        p.wait(timeout=60)
+        wall = time.perf_counter() - t0
+    out = "".join(lines)
+    m = re.search(r"Elapsed seconds: ([0-9.eE+-]+)\s+evaluations: (\d+)", out)
+    final = re.search(r"Final misfit: " + num, out)
+    if p.returncode != 0 or not m:
+        log(f"# c4 L-BFGS run failed: rc {p.returncode}")
+        return None
+    secs = float(m.group(1))
+    its = [float(x) for x in re.findall(r"iteration \d+: ([0-9.eE+-]+) s since the start", out)]
+    rec = {"iterations": iterations, "shots": P.nsrc, "seconds": round(secs, 1),
+           "seconds_at_iteration_end": [round(x, 1) for x in its],
+           "seconds_per_iteration": round(secs / iterations, 1),
+           "forward_evaluations": int(m.group(2)),
+           "misfit_after": float(final.group(1)) if final else None,
+           "wall_s_including_setup": round(wall, 1),
+           "note": f"inv.out mode 0, L-BFGS (5 pairs), lambda inverted, {P.nsrc} of the 256 "
+                   f"shots, 3 per launch, I16 snapshots, envelope misfit, sigma {P.sigma}; "
+                   "misfit normalised "
+                   "to 1 at the start; an evaluation is a forward of every shot (a gradient "
+                   "counts 2)"}
+    log(f"# c4 L-BFGS run: {rec}")
+    return rec
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -637,6 +702,15 @@ def main():
         for s in inv.serving.sweep(a.serving_sweep, log=log,
                                    snap_format=inv.invhip.INV_SNAP_I16):
             print(json.dumps(s), flush=True)
+        return
+    if a.steps == 0 and (a.full_gradient or a.lbfgs):
+        # only the whole-workload legs, one JSON line of their records
+        rec = {}
+        if a.full_gradient:
+            rec["full_gradient_c4"] = full_gradient_c4(inv, a)
+        if a.lbfgs:
+            rec["lbfgs_c4"] = lbfgs_c4(inv, a, a.lbfgs)
+        print(json.dumps(rec))
         return
     head, P, gpu_grad = run_workload(inv, a, a.workload, a.steps, a.warmup, headline=True)
     if head["roofline"]:
@@ -689,6 +763,7 @@ def main():
         acc = {w: accuracy(inv, w) for w in ("c4", "c2")
                if w == a.workload or (w in extras and a.workload == "c4")}
     full = full_gradient_c4(inv, a) if a.full_gradient and a.workload == "c4" else None
+    lbfgs = lbfgs_c4(inv, a, a.lbfgs) if a.lbfgs > 0 and a.workload == "c4" else None
     serving = None
     if a.serve_requests > 0 and a.workload in ("c1", "c4"):
         serving = serving_leg(inv, a.serve_requests, a.serve_rate_x, inv.invhip.INV_SNAP_I16)
@@ -719,6 +794,8 @@ def main():
                 "optimizer_c1": opt})
     if full:
         out["full_gradient_c4"] = full
+    if lbfgs:
+        out["lbfgs_c4"] = lbfgs
     if parity:
         out["parity"] = parity
     print(json.dumps(out))

@@ -1092,13 +1092,23 @@ class Optimizer {
             p_copy(m_old, m_new);
             p_copy(p_old, p_new);
             p_copy(g_old, g_new);
-            if (lineSearch(f) < 0) break;
+            const int ls = lineSearch(f);
+            if (misfit->progress)
+                std::cerr << "iteration " << iter + 1 << ": "
+                          << std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                                           time_start)
+                                 .count()
+                          << " s since the start, " << eval_count << " evaluations" << std::endl;
+            if (ls < 0) break;
             solver->exportKernel(iter + 1);
             solver->exportModel(iter + 1);
         }
         const double elapsed =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - time_start).count();
-        std::cout << "\nFinal misfit: " << misfit->run(false) / misfit->ref << std::endl;
+        // evaluated before anything is printed, so that progress lines on
+        // stderr do not split the line
+        const double final_misfit = misfit->run(false) / misfit->ref;
+        std::cout << "\nFinal misfit: " << final_misfit << std::endl;
         std::cout << "Elapsed time: ";
         if (elapsed > 60) {
             const size_t t_min = (size_t)(elapsed / 60);
