@@ -753,8 +753,8 @@ def main():
         else:
             # the filter runs once per step of 4 x 10000 steps: left out of the
             # short samples, where it would dominate
-            cpu = cpu_baseline(P, 1, nt=100, sigma=0)[0]
-            one = cpu_baseline(P, 1, nt=10, sigma=0, threads=1)[0]
+            cpu = cpu_baseline(P, 1, nt=400, sigma=0)[0]
+            one = cpu_baseline(P, 1, nt=40, sigma=0, threads=1)[0]
         cpu.update(info)
         cpu["one_thread"] = {k: one[k] for k in ("value", "cores", "sample")}
         log(f"# cpu baseline {cpu}")
