@@ -70,8 +70,11 @@ SH_STEP_BYTES = 4 * (2 * 4 + 2)
 #                                 profiles/k_pair_psv_4096x2048_4shots.md
 #   c2 two-step 20 x 2, 64 shots: 40.37 B per launch of two steps (algorithmic 40),
 #                                 profiles/k_pair_sh_marmousi1474x480_64shots.md
+#   c4 three-step 12 x 2, 4 shots: 73.85 B per launch of three steps (algorithmic 68),
+#                                 profiles/k_trip_psv_4096x2048_4shots.md
 MEASURED_TRAFFIC_PER_CELL = {("c1", 1, 4, 2, 16): 68.30, ("c4", 2, 20, 2, 4): 70.75,
-                             ("c2", 2, 20, 2, 64): 40.37}
+                             ("c2", 2, 20, 2, 64): 40.37,
+                             ("c4", 3, 12, 2, 4): 73.85}
 
 
 def log(msg):
@@ -116,6 +119,8 @@ def parse(argv=None):
     ap.add_argument("--persistent", type=int, default=-1,
                     help="persistent two-step kernel: 1 on (two-step tile 8 x 4), 0 off, -1 as "
                          "the autotuner chose")
+    ap.add_argument("--triple", type=int, default=-1,
+                    help="three-step tile rows (8 or 12; 0 off; -1 as the autotuner chose)")
     ap.add_argument("--math", type=int, default=1, help="0 exact (IEEE divisions), 1 fast")
     ap.add_argument("--graphs", type=int, default=1, help="capture time loops in HIP graphs")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -181,12 +186,16 @@ def configure_kernel(ctx, a, count, pair=True):
         ctx.set_pair(8, 4)
     if a.persistent >= 0:
         ctx.set_persistent(a.persistent)
+    if a.triple >= 0 and pair and a.variant == 0:
+        ctx.set_triple(a.triple)
     tile = {"variant": v, "rows": tx, "cols_per_lane": cols}
     pr, pv = ctx.get_pair()
     if pr:
         tile["two_step"] = {"rows": pr, "cols_per_lane": pv}
         if ctx.get_persistent():
             tile["two_step"]["persistent"] = True
+    if ctx.get_triple():
+        tile["three_step"] = {"rows": ctx.get_triple(), "cols_per_lane": 2}
     return tile
 
 
@@ -338,7 +347,8 @@ def roofline(ctx, inv, a, name, P, B, tile, fmt, ckpt, step_bytes):
     waves = "SH" if P.sh and not P.psv else "P-SV" if P.psv and not P.sh else "SH+P-SV"
     # a forward graph holds one kind of step launch when every pair of steps
     # is paired (no 16-bit snapshot scales, no checkpoints, nt even) or none
-    uniform = not two or (fmt == H.INV_SNAP_F32 and ckpt == 0 and P.nt % 2 == 0)
+    uniform = not two or (fmt == H.INV_SNAP_F32 and ckpt == 0 and P.nt % 2 == 0
+                          and "three_step" not in tile)
     launch_s, kernel, avg, samples, per_launch_steps = None, None, {}, {}, 1
     if nfr and uniform:
         per_launch_steps = 2 if two else 1
@@ -364,14 +374,24 @@ def roofline(ctx, inv, a, name, P, B, tile, fmt, ckpt, step_bytes):
         ctx.set_step_limit(0)
         nf, tf = ctx.prof_read(H.INV_PROF_STEP)
         npr, tpr = ctx.prof_read(H.INV_PROF_PAIR)
+        ntr, ttr = ctx.prof_read(H.INV_PROF_TRIPLE)
         if npr:
+            avg["two_step_launch"] = round(tpr / npr * 1e3, 2)
+        if nf:
+            avg["one_step_launch"] = round(tf / nf * 1e3, 2)
+        if ntr:
+            avg["three_step_launch"] = round(ttr / ntr * 1e3, 2)
+        samples = {"two_step": npr, "one_step": nf, "three_step": ntr}
+        if ntr and ttr >= tpr:
+            # three-step launches take most of the time: they are the
+            # dominant kernel (68 B per cell cross HBM once per three steps)
+            per_launch_steps = 3
+            launch_s = ttr / ntr / 1e3
+            kernel = f"three-step {waves} kernel k_trip, HIP events on direct launches"
+        elif npr:
             per_launch_steps = 2
             launch_s = tpr / npr / 1e3
             kernel = f"two-step {waves} kernel k_pair, HIP events on direct launches"
-            avg["two_step_launch"] = round(launch_s * 1e6, 2)
-            if nf:
-                avg["one_step_launch"] = round(tf / nf * 1e3, 2)
-            samples = {"two_step": npr, "one_step": nf}
     elif nf:
         launch_s = tf / nf / 1e3
         kernel = f"fused {waves} time step (k_col / k_colv)"
@@ -389,7 +409,7 @@ def roofline(ctx, inv, a, name, P, B, tile, fmt, ckpt, step_bytes):
     if not launch_s:
         return None
     achieved = step_bytes * cells / launch_s / 1e9
-    t = tile.get("two_step") or tile
+    t = (tile.get("three_step") if per_launch_steps == 3 else None) or tile.get("two_step") or tile
     key = (name, per_launch_steps, t["rows"], t["cols_per_lane"], B)
     measured = (MEASURED_TRAFFIC_PER_CELL.get(key)
                 if tile["variant"] == 0 and a.math == 1 else None)
@@ -398,9 +418,9 @@ def roofline(ctx, inv, a, name, P, B, tile, fmt, ckpt, step_bytes):
             "traffic": round(measured * cells) if measured else None,
             "kernel": kernel, "alg_bytes_per_launch": step_bytes * cells,
             "steps_per_launch": per_launch_steps, "avg_us": avg, "samples": samples}
-    if per_launch_steps == 2:
+    if per_launch_steps > 1:
         # the same cell-updates at one step per launch would need this
-        roof["one_step_equivalent_gbps"] = round(2 * achieved, 1)
+        roof["one_step_equivalent_gbps"] = round(per_launch_steps * achieved, 1)
     ni, ti = ctx.prof_read(H.INV_PROF_INTERACT)
     if ni:
         roof["interaction_avg_us"] = round(ti / ni * 1e3, 2)

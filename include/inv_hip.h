@@ -259,7 +259,8 @@ enum inv_prof_kernel {
     INV_PROF_FWD_RUN = 4,   /* replayed forward graphs (count = time steps) */
     INV_PROF_ADJ_RUN = 5,   /* replayed adjoint graphs (count = time steps) */
     INV_PROF_PAIR = 6,      /* two-step launches (count = launches) */
-    INV_PROF_RESIDENT = 7   /* persistent launches (count = time steps) */
+    INV_PROF_RESIDENT = 7,  /* persistent launches (count = time steps) */
+    INV_PROF_TRIPLE = 8     /* three-step launches (count = launches) */
 };
 int inv_prof_enable(inv_ctx *ctx, int every);
 int inv_prof_reset(inv_ctx *ctx);
@@ -284,15 +285,24 @@ int inv_set_tile(inv_ctx *ctx, int tile_x, int cols_per_lane);
  * steps.  tile_x rows x cols_per_lane columns per lane: 4, 8, 12, 16 or 20
  * rows x 2 (116 columns per tile) or 4 or 8 x 4 (240 columns); tile_x = 0 turns it off
  * (default).  Runs use it for every pair of steps that has no checkpoint
- * between them, no kernel accumulation after the first and no 16-bit
- * snapshot scale collected by the first and read by the second; the other
- * steps use the one-step kernel.  Results are identical with and without it.
+ * between them and no kernel accumulation after the first; the other steps
+ * use the one-step kernel.  Results are identical with and without it.
  * Needs nz divisible by cols_per_lane.  No counterpart in the reference (one
  * kernel launch per stencil half-step, solver/fdm.h:534-551). */
 int inv_set_pair(inv_ctx *ctx, int tile_x, int cols_per_lane);
 /* The two-step tile in use (tile_x 0: off), as set by inv_set_pair or chosen
  * by inv_autotune. */
 int inv_get_pair(inv_ctx *ctx, int *tile_x, int *cols_per_lane);
+/* Three time steps per launch (tile_x 8 or 12 rows x 108 columns, two per
+ * lane; 0 off, the default): forward runs use it for every three steps none
+ * of which stores a snapshot, adjoint runs for every three steps with no
+ * kernel accumulation after the first two, both only without checkpoints;
+ * the other steps take the two-step or one-step launches they take without
+ * it.  Results are identical with and without it.  Needs nz even.  No
+ * counterpart in the reference (one launch per stencil half-step,
+ * solver/fdm.h:534-551). */
+int inv_set_triple(inv_ctx *ctx, int tile_x);
+int inv_get_triple(inv_ctx *ctx, int *tile_x);
 /* Persistent two-step kernel (mode 1; 0 off, the default): one launch runs
  * every pair of steps of a forward, or of an adjoint between two kernel
  * accumulations, on tiles of 4 or 8 rows spanning all nz columns, one
@@ -315,7 +325,10 @@ int inv_get_persistent(inv_ctx *ctx, int *mode);
  * selects the fastest (a tile must beat the ones tried before it, in the
  * order above, by 2 %); then, over that tile, each two-step tile of
  * inv_set_pair that nz and the device's LDS allow, kept if 2 % faster than
- * the best so far (inv_get_pair reports it).  The
+ * the best so far (inv_get_pair reports it); then, for grids one tile wide,
+ * the persistent kernel, and over the choice so far three-step tiles of 12
+ * and 8 rows (inv_set_triple; inv_get_triple reports the choice), each kept
+ * if 2 % faster.  The
  * choice is cached per (nx, nz, wave types, math mode) for the process, so
  * later contexts and batch sizes of the same grid take it without timing.
  * Overwrites the slots' wavefields, traces and snapshots.  No counterpart in the

@@ -1262,6 +1262,33 @@ int inv_get_pair(inv_ctx *ctx, int *tile_x, int *cols_per_lane) {
     });
 }
 
+int inv_set_triple(inv_ctx *ctx, int tile_x) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(tile_x == 0 || tile_x == 8 || tile_x == 12, INV_ERR_ARG,
+                    "three-step tile must be 8 or 12 rows (0: off)");
+        INV_REQUIRE(!c.have_grid || tile_x == 0 || c.g.nz % 2 == 0, INV_ERR_UNSUPPORTED,
+                    "the three-step kernel needs nz even");
+        if (tile_x) {
+            int dev = 0, optin = 0;
+            INV_HIP(hipGetDevice(&dev));
+            INV_HIP(hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev));
+            INV_REQUIRE(trip_lds_bytes(true, tile_x) <= optin, INV_ERR_UNSUPPORTED,
+                        "three-step tile needs more LDS per block than the device allows");
+        }
+        sync(c);
+        c.trip_tx = tile_x;
+        invalidate(c);
+        fused_invalidate_lists(c);
+    });
+}
+
+int inv_get_triple(inv_ctx *ctx, int *tile_x) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(tile_x, INV_ERR_ARG, "null output");
+        *tile_x = c.trip_tx;
+    });
+}
+
 // ---- autotuner -------------------------------------------------------------
 // Candidate step kernels and tile shapes, timed on the context's own shape;
 // the best is cached per (grid, slots, wave types, math mode) for the process.
@@ -1270,6 +1297,7 @@ struct TuneChoice {
     int variant, tile, cols;
     int pair_tx = 0, pair_v = 2;  // two-step kernel tile (0 rows: off)
     int persist = 0;              // persistent two-step kernel
+    int trip = 0;                 // three-step kernel tile rows (0: off)
 };
 std::mutex tune_mu;
 std::map<std::string, TuneChoice> tune_cache;
@@ -1281,6 +1309,7 @@ void apply_choice(Ctx &c, const TuneChoice &t) {
     c.pair_tx = t.pair_tx;
     c.pair_v = t.pair_v;
     c.persist = t.persist;
+    c.trip_tx = t.trip;
     invalidate(c);
     fused_invalidate_lists(c);
 }
@@ -1355,10 +1384,10 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                     }
                     if (verbose)
                         fprintf(stderr,
-                                "inv_autotune %s: %d rows x %d cols per lane, two-step %d x %d%s: "
-                                "%.3f ms\n",
+                                "inv_autotune %s: %d rows x %d cols per lane, two-step %d x %d%s, "
+                                "three-step %d: %.3f ms\n",
                                 key.c_str(), t.tile, t.cols, t.pair_tx, t.pair_v,
-                                t.persist ? " persistent" : "", best_t);
+                                t.persist ? " persistent" : "", t.trip, best_t);
                     return best_t;
                 };
                 // one-step tiles, in order of preference (one column per
@@ -1416,6 +1445,27 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                         // not resident at once on this device, or a wait gave
                         // up: not a candidate
                         if (e.code != INV_ERR_UNSUPPORTED && e.code != INV_ERR_HIP) throw;
+                    }
+                }
+                // then three steps per launch over the chosen two-step (or
+                // one-step) tile, which keeps the steps that store
+                // snapshots; taller first
+                if (!best.persist && c.g.nz % 2 == 0 && c.ckpt == 0) {
+                    const TuneChoice two = best;
+                    for (int tx : {12, 8}) {
+                        TuneChoice t = two;
+                        t.trip = tx;
+                        float ms;
+                        try {
+                            ms = timed(t, false);
+                        } catch (const Error &e) {
+                            if (e.code != INV_ERR_UNSUPPORTED) throw;
+                            continue;
+                        }
+                        if (ms < 0.98 * best_ms) {
+                            best_ms = ms;
+                            best = t;
+                        }
                     }
                 }
             } catch (...) {

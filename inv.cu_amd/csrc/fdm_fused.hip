@@ -690,6 +690,7 @@ void fused_invalidate_lists(Ctx &c) {
     c.src_slists.clear();
     c.f_rec_tx = -1;
     c.p_rec_key = -1;
+    c.t_rec_key = -1;
 }
 
 
@@ -789,6 +790,50 @@ static void ensure_pair_rec_lists(Ctx &c) {
     build_tiled(c, c.p_rec_v, 1, cells, rows, c.pair_tx, pair_fz(c), 4, 5);
     build_tiled(c, c.p_rec_t, 1, cells, rows, c.pair_tx, pair_fz(c), 0, 0);
     c.p_rec_key = pair_key(c);
+}
+
+// Three-step kernel geometry: trip_tx rows x trip_strip() columns.  Its
+// lists cover the rows and columns each launch updates: forces and adjoint
+// sources (velocity of the first step, -7 .. +7 around the tile: lo 7, hi 8),
+// explosions (stress of the first step, -6 .. +5: lo 6, hi 6), receivers
+// (owned cells: 0, 0).
+static bool trip_on(const Ctx &c) {
+    return c.trip_tx > 0 && c.variant == INV_VARIANT_FUSED && c.g.nz % 2 == 0;
+}
+static int trip_key(const Ctx &c) { return 600000 + c.trip_tx * 1000 + trip_strip(); }
+
+static PointList &trip_source_list(Ctx &c, int first, int count) {
+    const long long k = list_id(first, count, trip_key(c));
+    auto found = c.src_lists.find(k);
+    if (found != c.src_lists.end()) return found->second;
+    PointList &pl = c.src_lists[k];
+    std::vector<std::vector<int>> cells, rows;
+    source_points(c, first, count, 0, cells, rows);
+    build_tiled(c, pl, count, cells, rows, c.trip_tx, trip_strip(), 7, 8);
+    return pl;
+}
+
+static PointList &trip_stress_source_list(Ctx &c, int first, int count) {
+    const long long k = list_id(first, count, trip_key(c));
+    auto found = c.src_slists.find(k);
+    if (found != c.src_slists.end()) return found->second;
+    PointList &pl = c.src_slists[k];
+    std::vector<std::vector<int>> cells, rows;
+    source_points(c, first, count, 1, cells, rows);
+    build_tiled(c, pl, count, cells, rows, c.trip_tx, trip_strip(), 6, 6);
+    return pl;
+}
+
+static void ensure_trip_rec_lists(Ctx &c) {
+    if (c.t_rec_key == trip_key(c)) return;
+    std::vector<std::vector<int>> cells(1), rows(1);
+    for (int r = 0; r < c.nrec; r++) {
+        cells[0].push_back(c.rec_ix[r] * c.g.nz + c.rec_iz[r]);
+        rows[0].push_back(r);
+    }
+    build_tiled(c, c.t_rec_v, 1, cells, rows, c.trip_tx, trip_strip(), 7, 8);
+    build_tiled(c, c.t_rec_t, 1, cells, rows, c.trip_tx, trip_strip(), 0, 0);
+    c.t_rec_key = trip_key(c);
 }
 
 static void ensure_second_set(Ctx &c) {
@@ -976,6 +1021,13 @@ void fused_prepare(Ctx &c, int kind, int first, int count, int flags) {
             if (c.any_explosive()) pair_stress_source_list(c, first, count);
         }
     }
+    if (trip_on(c)) {
+        ensure_trip_rec_lists(c);
+        if (kind == 1) {
+            trip_source_list(c, first, count);
+            if (c.any_explosive()) trip_stress_source_list(c, first, count);
+        }
+    }
 }
 
 template <bool SH, bool PSV, bool FAST>
@@ -1030,6 +1082,27 @@ static void pair_m(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned bloc
     // SH and P-SV are independent systems: one launch each
     if (c.g.sh) pair_launch<true, false, FAST>(c, a, r2, blocks);
     if (c.g.psv) pair_launch<false, true, FAST>(c, a, r2, blocks);
+}
+
+template <bool FAST>
+static void trip_m(Ctx &c, const StepArgs &a, unsigned blocks) {
+    if (c.g.sh) trip_launch<true, false, FAST>(c, a, blocks);
+    if (c.g.psv) trip_launch<false, true, FAST>(c, a, blocks);
+}
+
+// Steps it, it+1 and it+2 in one launch of the three-step kernel (no
+// snapshot roles).
+static void trip_step(Ctx &c, StepArgs &a, const Fields &F, int &par, int count, int it) {
+    set_state_ptrs(a, F, par);
+    a.it = it;
+    const unsigned blocks = (unsigned)(count * a.tiles_x * a.tiles_z);
+    hipEvent_t ev;
+    const bool p = prof_begin(c, it, &ev);
+    if (c.math == INV_MATH_FAST) trip_m<true>(c, a, blocks);
+    else trip_m<false>(c, a, blocks);
+    INV_HIP(hipGetLastError());
+    if (p) prof_end(c, PROF_TRIP, ev);
+    par ^= 1;
 }
 
 // Steps it and it+1 in one launch of the two-step kernel (roles of step it
@@ -1142,6 +1215,39 @@ static StepArgs pair_args(Ctx &c, const StepArgs &a, int first, int count, bool 
     p.rec_ptr = (record && c.nrec) ? c.p_rec_t.ptr.as<int>() : nullptr;
     p.rec_cell = c.p_rec_t.cell.as<int>();
     p.rec_row = c.p_rec_t.row.as<int>();
+    return p;
+}
+
+// Arguments of the three-step kernel: those of the one-step kernel with the
+// triple geometry's tiles and point lists (built by fused_prepare).
+static StepArgs trip_args(Ctx &c, const StepArgs &a, int first, int count, bool adjoint,
+                          bool record) {
+    StepArgs p = a;
+    p.snap = nullptr;
+    p.sexp = nullptr;
+    p.tiles_x = tiles_x_of(c, c.trip_tx);
+    p.tiles_z = (c.g.nz + trip_strip() - 1) / trip_strip();
+    ensure_trip_rec_lists(c);
+    if (adjoint) {
+        p.inj_ptr = c.nrec ? c.t_rec_v.ptr.as<int>() : nullptr;
+        p.inj_cell = c.t_rec_v.cell.as<int>();
+        p.inj_row = c.t_rec_v.row.as<int>();
+        p.rec_ptr = nullptr;
+        return p;
+    }
+    PointList &sl = trip_source_list(c, first, count);
+    p.inj_ptr = sl.ptr.as<int>();
+    p.inj_cell = sl.cell.as<int>();
+    p.inj_row = sl.row.as<int>();
+    if (a.sinj_ptr) {
+        PointList &xl = trip_stress_source_list(c, first, count);
+        p.sinj_ptr = xl.ptr.as<int>();
+        p.sinj_cell = xl.cell.as<int>();
+        p.sinj_row = xl.row.as<int>();
+    }
+    p.rec_ptr = (record && c.nrec) ? c.t_rec_t.ptr.as<int>() : nullptr;
+    p.rec_cell = c.t_rec_t.cell.as<int>();
+    p.rec_row = c.t_rec_t.row.as<int>();
     return p;
 }
 
@@ -1258,6 +1364,11 @@ void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
     const bool pr = pair_on(c);
     StepArgs a2;
     if (pr) a2 = pair_args(c, a, first, count, false, true);
+    // three steps per launch where none of them stores a snapshot (runs with
+    // checkpoints keep the two-step plan)
+    const bool tr = trip_on(c) && !(sp.on && c.ckpt > 0);
+    StepArgs a3;
+    if (tr) a3 = trip_args(c, a, first, count, false, true);
     if (res_on(c)) {
         // every pair of steps in persistent launches, an odd last step alone
         int it = it0;
@@ -1278,6 +1389,12 @@ void fused_forward(Ctx &c, int first, int count, int flags, int it0, int it1) {
             const int nf = state_list(c, F, c.parity, cur);
             for (int f = 0; f < nf; f++)
                 vec_copy(c, ckpt_field(c, it / c.ckpt, f), cur[f], (size_t)count * c.N);
+        }
+        if (tr && it + 2 < it1 && sp.local(it) < 0 && sp.local(it + 1) < 0 &&
+            sp.local(it + 2) < 0) {
+            trip_step(c, a3, F, c.parity, count, it);
+            it += 3;
+            continue;
         }
         snap_roles(sp, a, it);
         if (pr && it + 1 < it1 && !(sp.on && c.ckpt > 0 && (it + 1) % c.ckpt == 0)) {
@@ -1361,7 +1478,24 @@ void fused_adjoint(Ctx &c, int first, int count, int it0, int it1) {
     StepArgs a2;
     if (pr) a2 = pair_args(c, a, first, count, true, false);
     const bool res = res_on(c) && pr;
+    // three steps per launch where no kernel accumulation follows the first
+    // two (runs without checkpoints); r steps up to and including the next
+    // accumulation split as triples and pairs, never leaving one step alone
+    // (a one-step launch moves as many bytes as a two-step one): 10 = 3+3+2+2
+    const bool three = trip_on(c) && K == 0 && c.g.sfe > 2 && !res;
+    StepArgs a3;
+    if (three) a3 = trip_args(c, a, first, count, true, false);
     for (int it = it0; it < it1;) {
+        const int r = (c.g.sfe - it % c.g.sfe) % c.g.sfe + 1;
+        if (three && it + 2 < it1 && r >= 3 && r != 4) {
+            trip_step(c, a3, F, c.parity, count, it);
+            const int last = it + 2;
+            if (last % c.g.sfe == 0)
+                fdm_interact(c, count, last, last / c.g.sfe, cur_field(c, c.vx, c.vx_b),
+                             cur_field(c, c.vz, c.vz_b), cur_field(c, c.vy, c.vy_b));
+            it = last + 1;
+            continue;
+        }
         const bool two = pr && it + 1 < it1 && it % c.g.sfe != 0;
         if (two && res) {
             // every pair up to the next step that cannot pair (one whose

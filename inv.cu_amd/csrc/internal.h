@@ -60,7 +60,8 @@ constexpr int TILE_X = 4;   // wavefronts per block along x
 // Sampled kernel timing with HIP events on the context stream.
 enum ProfKind {
     PROF_VELOCITY = 0, PROF_STRESS = 1, PROF_INTERACT = 2, PROF_STEP = 3,
-    PROF_FWD_RUN = 4, PROF_ADJ_RUN = 5, PROF_PAIR = 6, PROF_RES = 7, PROF_NKIND = 8
+    PROF_FWD_RUN = 4, PROF_ADJ_RUN = 5, PROF_PAIR = 6, PROF_RES = 7, PROF_TRIP = 8,
+    PROF_NKIND = 9
 };
 struct Prof {
     int every = 0;  // sample every n-th time step (0 = off)
@@ -114,6 +115,8 @@ struct Ctx {
     // two-step kernel (fused variant): tile rows (0 = off) and columns per lane
     int pair_tx = 0;
     int pair_v = 2;
+    // three-step launches (fdm_trip.hip): tile rows 8 or 12, 0 off
+    int trip_tx = 0;
     // persistent two-step kernel (fdm_resident.hip): 0 off, 1 on, 2 on with
     // every wait given up at once (tests the error path); its halo exchange
     // buffer and control words {launch epoch, error}
@@ -168,6 +171,9 @@ struct Ctx {
     // the same for the two-step kernel's tile geometry
     PointList p_rec_v, p_rec_t;
     int p_rec_key = -1;
+    // and for the three-step kernel's
+    PointList t_rec_v, t_rec_t;
+    int t_rec_key = -1;
 
     // captured time loops: one executable graph per (run kind, shots, flags,
     // kernel configuration); `gen` changes whenever buffers or settings they

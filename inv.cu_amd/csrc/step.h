@@ -151,6 +151,20 @@ constexpr int pair_lds_bytes(bool psv, int rows, int V) {
 template <bool SH, bool PSV, bool FAST>
 void pair_launch(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned blocks);
 
+// Three-step kernel (fdm_trip.hip): three time steps per launch on tiles of
+// c.trip_tx = 8 or 12 rows x trip_strip() columns (2 per lane); the stress
+// of step n is read with a halo of 9 rows and trip_halo() columns each side.
+// Launches without snapshot roles only.
+constexpr int trip_halo() { return 10; }
+constexpr int trip_strip() { return 128 - 2 * trip_halo(); }
+// LDS bytes of a three-step tile: stress on rows+18 rows (szz or szy on
+// rows+15) and velocity on rows+15 rows, 128 columns each.
+constexpr int trip_lds_bytes(bool psv, int rows) {
+    return 4 * 128 * ((psv ? 2 : 1) * (rows + 18) + (rows + 15) + (psv ? 2 : 1) * (rows + 15));
+}
+template <bool SH, bool PSV, bool FAST>
+void trip_launch(Ctx &c, const StepArgs &a, unsigned blocks);
+
 // Persistent two-step kernel (fdm_resident.hip): `iters` iterations of two
 // steps per launch on tiles of c.pair_tx = 4 or 8 rows x every column (nz <=
 // 240 with RES_V = 4 columns per lane), one workgroup of RES_NW wavefronts

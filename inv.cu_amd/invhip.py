@@ -20,6 +20,7 @@ INV_SNAP_F32, INV_SNAP_F16, INV_SNAP_BF16, INV_SNAP_I16 = 0, 1, 2, 3
 INV_FWD_SNAPSHOTS = 1
 INV_PROF_VELOCITY, INV_PROF_STRESS, INV_PROF_INTERACT, INV_PROF_STEP = 0, 1, 2, 3
 INV_PROF_FWD_RUN, INV_PROF_ADJ_RUN, INV_PROF_PAIR, INV_PROF_RESIDENT = 4, 5, 6, 7
+INV_PROF_TRIPLE = 8
 INV_VARIANT_FUSED, INV_VARIANT_SPLIT, INV_VARIANT_FUSED_LINEAR = 0, 1, 2
 INV_MATH_EXACT, INV_MATH_FAST = 0, 1
 
@@ -106,6 +107,8 @@ SIGNATURES = {
     "inv_set_tile": [_vp, _i, _i],
     "inv_set_pair": [_vp, _i, _i],
     "inv_get_pair": [_vp, _vp, _vp],
+    "inv_set_triple": [_vp, _i],
+    "inv_get_triple": [_vp, _vp],
     "inv_set_persistent": [_vp, _i],
     "inv_get_persistent": [_vp, _vp],
     "inv_autotune": [_vp, _i, _i, _i, _vp, _vp, _vp],
@@ -486,6 +489,16 @@ class Context:
         r, v = C.c_int(), C.c_int()
         self._chk(self.L.inv_get_pair(self.h, C.byref(r), C.byref(v)))
         return r.value, v.value
+
+    def set_triple(self, tile_x=12):
+        """Three steps per launch with tile_x (8 or 12) rows where the steps
+        allow it (0: off)."""
+        self._chk(self.L.inv_set_triple(self.h, tile_x))
+
+    def get_triple(self):
+        r = C.c_int()
+        self._chk(self.L.inv_get_triple(self.h, C.byref(r)))
+        return r.value
 
     def set_persistent(self, mode=1):
         """Persistent two-step kernel (1), off (0), or on with every wait
