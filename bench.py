@@ -166,12 +166,13 @@ def serving_leg(inv, n, rate_x, snap_format):
             "p99_speedup": round(plain["p99_latency_s"] / batched["p99_latency_s"], 3)}
 
 
-def configure_kernel(ctx, a, count, pair=True):
-    """The step kernels: autotuned (default) or as given; returns the tiles.
-    pair=False keeps one step per launch."""
+def configure_kernel(ctx, a, count, nt, pair=True):
+    """The step kernels: autotuned (default, on up to 30 steps of the nt of
+    the run) or as given; returns the tiles.  pair=False keeps one step per
+    launch."""
     if a.variant == 0 and a.tile_x == 0:
         # a one-step tile and, over it, a two-step tile if one is faster
-        v, tx, cols = ctx.autotune(0, count, steps=20)
+        v, tx, cols = ctx.autotune(0, count, steps=min(30, nt))
     elif a.variant == 0:
         ctx.set_tile(a.tile_x, a.cols)
         v, tx, cols = 0, a.tile_x, a.cols
@@ -446,8 +447,8 @@ def run_workload(inv, a, name, steps, warmup, headline):
     ctx.set_math(a.math)
     ctx.set_graphs(a.graphs)
     ctx.set_snapshots(fmt, ckpt)  # before any run sizes the snapshot storage
-    tile = configure_kernel(ctx, a, B) if headline else configure_kernel(
-        ctx, parse(["--math", str(a.math)]), B)
+    tile = configure_kernel(ctx, a, B, P.nt) if headline else configure_kernel(
+        ctx, parse(["--math", str(a.math)]), B, P.nt)
     log(f"# step kernel tile {tile}")
     # observed data: forward on the true model (Misfit::init, misfit.h:135-148)
     for first in range(0, P.nsrc, B):
@@ -540,7 +541,7 @@ def plain_c1(inv, a):
     launch) on c1: the plain comparison point of the batched path."""
     P = inv.problems.workload("c1", sigma=a.sigma)[0]
     pctx = P.context(batch=1, model="true")
-    configure_kernel(pctx, parse(["--math", str(a.math)]), 1, pair=False)
+    configure_kernel(pctx, parse(["--math", str(a.math)]), 1, P.nt, pair=False)
     pctx.set_math(a.math)
     pctx.set_graphs(False)
     for s in range(P.nsrc):

@@ -326,9 +326,9 @@ int inv_get_persistent(inv_ctx *ctx, int *mode);
  * order above, by 2 %); then, over that tile, each two-step tile of
  * inv_set_pair that nz and the device's LDS allow, kept if 2 % faster than
  * the best so far (inv_get_pair reports it); then, for grids one tile wide,
- * the persistent kernel, and over the choice so far three-step tiles of 12
- * and 8 rows (inv_set_triple; inv_get_triple reports the choice), each kept
- * if 2 % faster.  The
+ * the persistent kernel, and over the choice so far (grids with P-SV waves)
+ * three-step tiles of 12 and 8 rows (inv_set_triple; inv_get_triple reports
+ * the choice), each kept if 2 % faster.  The
  * choice is cached per (nx, nz, wave types, math mode) for the process, so
  * later contexts and batch sizes of the same grid take it without timing.
  * Overwrites the slots' wavefields, traces and snapshots.  No counterpart in the

@@ -1449,8 +1449,10 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                 }
                 // then three steps per launch over the chosen two-step (or
                 // one-step) tile, which keeps the steps that store
-                // snapshots; taller first
-                if (!best.persist && c.g.nz % 2 == 0 && c.ckpt == 0) {
+                // snapshots; taller first.  Grids with P-SV waves only: on
+                // SH alone (1474x480) it gains 1-2 %, within this timing's
+                // noise
+                if (!best.persist && c.g.psv && c.g.nz % 2 == 0 && c.ckpt == 0) {
                     const TuneChoice two = best;
                     for (int tx : {12, 8}) {
                         TuneChoice t = two;
