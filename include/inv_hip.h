@@ -293,8 +293,8 @@ int inv_set_pair(inv_ctx *ctx, int tile_x, int cols_per_lane);
 /* The two-step tile in use (tile_x 0: off), as set by inv_set_pair or chosen
  * by inv_autotune. */
 int inv_get_pair(inv_ctx *ctx, int *tile_x, int *cols_per_lane);
-/* Three time steps per launch (tile_x 8 or 12 rows x 108 columns, two per
- * lane; 0 off, the default): forward runs use it for every three steps none
+/* Three time steps per launch (tile_x 8, 12, 16 or 20 rows x 108 columns,
+ * two per lane; 0 off, the default): forward runs use it for every three steps none
  * of which stores a snapshot, adjoint runs for every three steps with no
  * kernel accumulation after the first two, both only without checkpoints;
  * the other steps take the two-step or one-step launches they take without
@@ -326,9 +326,9 @@ int inv_get_persistent(inv_ctx *ctx, int *mode);
  * order above, by 2 %); then, over that tile, each two-step tile of
  * inv_set_pair that nz and the device's LDS allow, kept if 2 % faster than
  * the best so far (inv_get_pair reports it); then, for grids one tile wide,
- * the persistent kernel, and over the choice so far (grids with P-SV waves)
- * three-step tiles of 12 and 8 rows (inv_set_triple; inv_get_triple reports
- * the choice), each kept if 2 % faster.  The
+ * the persistent kernel, and over the choice so far three-step tiles of 12,
+ * 16, 20 and 8 rows (inv_set_triple; inv_get_triple reports the choice),
+ * each kept if 2 % faster.  The
  * choice is cached per (nx, nz, wave types, math mode) for the process, so
  * later contexts and batch sizes of the same grid take it without timing.
  * Overwrites the slots' wavefields, traces and snapshots.  No counterpart in the

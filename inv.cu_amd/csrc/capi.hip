@@ -1264,15 +1264,16 @@ int inv_get_pair(inv_ctx *ctx, int *tile_x, int *cols_per_lane) {
 
 int inv_set_triple(inv_ctx *ctx, int tile_x) {
     return guard(ctx, [&](Ctx &c) {
-        INV_REQUIRE(tile_x == 0 || tile_x == 8 || tile_x == 12, INV_ERR_ARG,
-                    "three-step tile must be 8 or 12 rows (0: off)");
+        INV_REQUIRE(tile_x == 0 || tile_x == 8 || tile_x == 12 || tile_x == 16 || tile_x == 20,
+                    INV_ERR_ARG, "three-step tile must be 8, 12, 16 or 20 rows (0: off)");
         INV_REQUIRE(!c.have_grid || tile_x == 0 || c.g.nz % 2 == 0, INV_ERR_UNSUPPORTED,
                     "the three-step kernel needs nz even");
         if (tile_x) {
             int dev = 0, optin = 0;
             INV_HIP(hipGetDevice(&dev));
             INV_HIP(hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev));
-            INV_REQUIRE(trip_lds_bytes(true, tile_x) <= optin, INV_ERR_UNSUPPORTED,
+            INV_REQUIRE(trip_lds_bytes(c.have_grid ? c.g.psv != 0 : true, tile_x) <= optin,
+                        INV_ERR_UNSUPPORTED,
                         "three-step tile needs more LDS per block than the device allows");
         }
         sync(c);
@@ -1449,12 +1450,11 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                 }
                 // then three steps per launch over the chosen two-step (or
                 // one-step) tile, which keeps the steps that store
-                // snapshots; taller first.  Grids with P-SV waves only: on
-                // SH alone (1474x480) it gains 1-2 %, within this timing's
-                // noise
-                if (!best.persist && c.g.psv && c.g.nz % 2 == 0 && c.ckpt == 0) {
+                // snapshots (P-SV tiles of 16 and 20 rows fit one per CU and
+                // lose; SH ones fit two or three)
+                if (!best.persist && c.g.nz % 2 == 0 && c.ckpt == 0) {
                     const TuneChoice two = best;
-                    for (int tx : {12, 8}) {
+                    for (int tx : {12, 16, 20, 8}) {
                         TuneChoice t = two;
                         t.trip = tx;
                         float ms;

@@ -265,8 +265,12 @@ void trip_launch(Ctx &c, const StepArgs &a, unsigned blocks) {
 #define INV_TRIP(R)                                                                     \
     (rec ? trip_go(c, k_trip<SH, PSV, R, 8, FAST, true>, 8, lds, a, blocks)             \
          : trip_go(c, k_trip<SH, PSV, R, 8, FAST, false>, 8, lds, a, blocks))
-    if (rows == 12) INV_TRIP(12);
-    else INV_TRIP(8);
+    switch (rows) {
+        case 12: INV_TRIP(12); break;
+        case 16: INV_TRIP(16); break;
+        case 20: INV_TRIP(20); break;
+        default: INV_TRIP(8); break;
+    }
 #undef INV_TRIP
 }
 

@@ -152,7 +152,7 @@ template <bool SH, bool PSV, bool FAST>
 void pair_launch(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned blocks);
 
 // Three-step kernel (fdm_trip.hip): three time steps per launch on tiles of
-// c.trip_tx = 8 or 12 rows x trip_strip() columns (2 per lane); the stress
+// c.trip_tx = 8, 12, 16 or 20 rows x trip_strip() columns (2 per lane); the stress
 // of step n is read with a halo of 9 rows and trip_halo() columns each side.
 // Launches without snapshot roles only.
 constexpr int trip_halo() { return 10; }

@@ -491,8 +491,8 @@ class Context:
         return r.value, v.value
 
     def set_triple(self, tile_x=12):
-        """Three steps per launch with tile_x (8 or 12) rows where the steps
-        allow it (0: off)."""
+        """Three steps per launch with tile_x (8, 12, 16 or 20) rows where the
+        steps allow it (0: off)."""
         self._chk(self.L.inv_set_triple(self.h, tile_x))
 
     def get_triple(self):

@@ -23,18 +23,20 @@ from conftest import inv, oracle_problem, oracle_traces, rel
 pytestmark = pytest.mark.gpu
 
 H = inv.invhip
-TRIPS = [12, 8]
+TRIPS = [12, 8, 16, 20]
 
 
 def edge_problem(sh=1, psv=1, nt=170, explosive=False):
-    """Three-step tiles both ways (108-column strips, 8 or 12 rows), sources
-    and receivers on and next to their edges."""
-    P = inv.problems.tiny(nx=70, nz=256, nt=nt, nsrc=4, nrec=14, sh=sh, psv=psv)
-    P = P.with_(src_ix=np.array([11, 12, 24, 36], np.int32),
+    """Three-step tiles both ways (108-column strips, 8 to 20 rows), sources
+    and receivers on and next to their edges (rows 8, 12, 16, 20, 24, 36,
+    48, 60 start tiles of one height or another)."""
+    P = inv.problems.tiny(nx=70, nz=256, nt=nt, nsrc=4, nrec=22, sh=sh, psv=psv)
+    P = P.with_(src_ix=np.array([11, 16, 24, 40], np.int32),
                 src_iz=np.array([107, 108, 216, 215], np.int32),
-                rec_ix=np.array([4, 7, 8, 12, 23, 24, 35, 36, 47, 48, 59, 60, 66, 69], np.int32),
+                rec_ix=np.array([4, 7, 8, 12, 15, 16, 19, 20, 23, 24, 31, 32, 35, 36, 39, 40, 47,
+                                 48, 59, 60, 66, 69], np.int32),
                 rec_iz=np.array([107, 108, 109, 215, 216, 217, 3, 100, 250, 60, 2, 253, 107,
-                                 108], np.int32))
+                                 108, 1, 254, 216, 107, 109, 215, 120, 0], np.int32))
     if explosive:
         P = P.with_(src_type=np.array([1, 0, 1, 1], np.int32))
     return P
@@ -205,8 +207,8 @@ def test_triple_c4_grid_equals_pair(gpu):
 def test_triple_rejects_what_it_cannot_run(gpu):
     P = edge_problem(sh=0, psv=1, nt=20)
     ctx = P.context(batch=1)
-    with pytest.raises(H.InvError, match="8 or 12 rows"):
-        ctx.set_triple(16)
+    with pytest.raises(H.InvError, match="8, 12, 16 or 20 rows"):
+        ctx.set_triple(24)
     ctx.close()
     P = inv.problems.tiny(nx=40, nz=41, nt=20, nsrc=1, nrec=2, sh=0, psv=1)
     ctx = P.context(batch=1)
