@@ -72,9 +72,11 @@ SH_STEP_BYTES = 4 * (2 * 4 + 2)
 #                                 profiles/k_pair_sh_marmousi1474x480_64shots.md
 #   c4 three-step 12 x 2, 4 shots: 73.85 B per launch of three steps (algorithmic 68),
 #                                 profiles/k_trip_psv_4096x2048_4shots.md
+#   c2 three-step 16 x 2, 64 shots: 40.44 B per launch of three steps (algorithmic 40),
+#                                 profiles/k_trip_sh_marmousi1474x480_64shots.md
 MEASURED_TRAFFIC_PER_CELL = {("c1", 1, 4, 2, 16): 68.30, ("c4", 2, 20, 2, 4): 70.75,
                              ("c2", 2, 20, 2, 64): 40.37,
-                             ("c4", 3, 12, 2, 4): 73.85}
+                             ("c4", 3, 12, 2, 4): 73.85, ("c2", 3, 16, 2, 64): 40.44}
 
 
 def log(msg):
