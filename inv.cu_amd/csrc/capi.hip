@@ -1429,6 +1429,7 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                 // then the persistent kernel (grids one tile wide, whose
                 // tiles all fit the device at once), over the same one-step
                 // tile
+                const TuneChoice two = best;
                 for (int tx : {8, 4}) {
                     if (c.g.nz % RES_V || c.g.nz > pair_strip(RES_V) || c.ckpt) break;
                     TuneChoice t = single;
@@ -1451,9 +1452,9 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                 // then three steps per launch over the chosen two-step (or
                 // one-step) tile, which keeps the steps that store
                 // snapshots (P-SV tiles of 16 and 20 rows fit one per CU and
-                // lose; SH ones fit two or three)
-                if (!best.persist && c.g.nz % 2 == 0 && c.ckpt == 0) {
-                    const TuneChoice two = best;
+                // lose; SH ones fit two or three), against the best so far
+                // (the persistent kernel included)
+                if (c.g.nz % 2 == 0 && c.ckpt == 0) {
                     for (int tx : {12, 16, 20, 8}) {
                         TuneChoice t = two;
                         t.trip = tx;
