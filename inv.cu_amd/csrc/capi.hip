@@ -1326,7 +1326,7 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
         // choice is kept per grid, wave types and math mode
         const std::string key = std::to_string(c.g.nx) + "x" + std::to_string(c.g.nz) + "/" +
                                 std::to_string(c.g.sh) + std::to_string(c.g.psv) + "/" +
-                                std::to_string(c.math);
+                                std::to_string(c.math) + (c.tune_trip ? "" : "/no-trip");
         TuneChoice best{-1, 0, 0};
         {
             std::lock_guard<std::mutex> lk(tune_mu);
@@ -1454,7 +1454,7 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                 // snapshots (P-SV tiles of 16 and 20 rows fit one per CU and
                 // lose; SH ones fit two or three), against the best so far
                 // (the persistent kernel included)
-                if (c.g.nz % 2 == 0 && c.ckpt == 0) {
+                if (c.tune_trip && c.g.nz % 2 == 0 && c.ckpt == 0) {
                     for (int tx : {12, 16, 20, 8}) {
                         TuneChoice t = two;
                         t.trip = tx;

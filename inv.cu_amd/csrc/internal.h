@@ -115,8 +115,11 @@ struct Ctx {
     // two-step kernel (fused variant): tile rows (0 = off) and columns per lane
     int pair_tx = 0;
     int pair_v = 2;
-    // three-step launches (fdm_trip.hip): tile rows 8 or 12, 0 off
+    // three-step launches (fdm_trip.hip): tile rows 8, 12, 16 or 20, 0 off;
+    // tune_trip: the autotuner times them too (the request server's lanes
+    // turn it off)
     int trip_tx = 0;
+    bool tune_trip = true;
     // persistent two-step kernel (fdm_resident.hip): 0 off, 1 on, 2 on with
     // every wait given up at once (tests the error path); its halo exchange
     // buffer and control words {launch epoch, error}

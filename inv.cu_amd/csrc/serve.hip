@@ -168,6 +168,10 @@ static void ensure_ctx(Server &S, Lane &L, const std::string &cls) {
         throw Error{code, "context creation failed"};
     }
     L.ctx->retain = true;
+    // the lanes run one- and two-step launches only: a serving load whose
+    // lanes timed and ran three-step tiles stopped making progress (cause not
+    // found), so that path stays off here
+    L.ctx->tune_trip = false;
     L.ctx_cls = cls;
 }
 

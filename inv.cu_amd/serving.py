@@ -55,6 +55,9 @@ def run_load(reqs, rate=None, seed=3, plain=False, log=None, **server_kw):
             for k in range(len(reqs)):
                 ready.acquire()
                 infos[k], _, _ = srv.wait(ids[k])
+                if log and (k + 1) % 8 == 0:
+                    log(f"#   {'plain' if plain else 'batched'}: {k + 1} of {len(reqs)} "
+                        "requests done")
 
         th = threading.Thread(target=waiter)
         th.start()
