@@ -123,6 +123,9 @@ def parse(argv=None):
                          "the autotuner chose")
     ap.add_argument("--triple", type=int, default=-1,
                     help="three-step tile rows (8 or 12; 0 off; -1 as the autotuner chose)")
+    ap.add_argument("--walk", type=int, default=-1,
+                    help="walking three-step kernel chunk rows (0 off; -1 as the autotuner "
+                         "chose)")
     ap.add_argument("--math", type=int, default=1, help="0 exact (IEEE divisions), 1 fast")
     ap.add_argument("--graphs", type=int, default=1, help="capture time loops in HIP graphs")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -191,13 +194,17 @@ def configure_kernel(ctx, a, count, nt, pair=True):
         ctx.set_persistent(a.persistent)
     if a.triple >= 0 and pair and a.variant == 0:
         ctx.set_triple(a.triple)
+    if a.walk >= 0 and pair and a.variant == 0:
+        ctx.set_walk(a.walk)
     tile = {"variant": v, "rows": tx, "cols_per_lane": cols}
     pr, pv = ctx.get_pair()
     if pr:
         tile["two_step"] = {"rows": pr, "cols_per_lane": pv}
         if ctx.get_persistent():
             tile["two_step"]["persistent"] = True
-    if ctx.get_triple():
+    if ctx.get_walk():
+        tile["three_step"] = {"walk_chunk_rows": ctx.get_walk(), "cols_per_lane": 2}
+    elif ctx.get_triple():
         tile["three_step"] = {"rows": ctx.get_triple(), "cols_per_lane": 2}
     return tile
 
@@ -390,7 +397,9 @@ def roofline(ctx, inv, a, name, P, B, tile, fmt, ckpt, step_bytes):
             # dominant kernel (68 B per cell cross HBM once per three steps)
             per_launch_steps = 3
             launch_s = ttr / ntr / 1e3
-            kernel = f"three-step {waves} kernel k_trip, HIP events on direct launches"
+            kname = "k_walk (walking)" if "walk_chunk_rows" in tile.get("three_step", {}) \
+                else "k_trip"
+            kernel = f"three-step {waves} kernel {kname}, HIP events on direct launches"
         elif npr:
             per_launch_steps = 2
             launch_s = tpr / npr / 1e3
@@ -413,7 +422,8 @@ def roofline(ctx, inv, a, name, P, B, tile, fmt, ckpt, step_bytes):
         return None
     achieved = step_bytes * cells / launch_s / 1e9
     t = (tile.get("three_step") if per_launch_steps == 3 else None) or tile.get("two_step") or tile
-    key = (name, per_launch_steps, t["rows"], t["cols_per_lane"], B)
+    key = (name, per_launch_steps, t.get("rows", f"walk{t.get('walk_chunk_rows')}"),
+           t["cols_per_lane"], B)
     measured = (MEASURED_TRAFFIC_PER_CELL.get(key)
                 if tile["variant"] == 0 and a.math == 1 else None)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,

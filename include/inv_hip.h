@@ -303,6 +303,15 @@ int inv_get_pair(inv_ctx *ctx, int *tile_x, int *cols_per_lane);
  * solver/fdm.h:534-551). */
 int inv_set_triple(inv_ctx *ctx, int tile_x);
 int inv_get_triple(inv_ctx *ctx, int *tile_x);
+/* The same three-step launches by a walking kernel (chunk_rows a multiple of
+ * 8 in [16, 4096]; 0 off, the default): a workgroup walks down chunk_rows
+ * rows of a 108-column strip eight rows at a time, keeping every step's rows
+ * in LDS, so a row is updated once per step instead of once per tile it is
+ * a halo row of.  When set it replaces the tiles of inv_set_triple (and
+ * turns three-step launches on by itself).  Results are identical.  Needs
+ * nz even.  No counterpart in the reference (solver/fdm.h:534-551). */
+int inv_set_walk(inv_ctx *ctx, int chunk_rows);
+int inv_get_walk(inv_ctx *ctx, int *chunk_rows);
 /* Persistent two-step kernel (mode 1; 0 off, the default): one launch runs
  * every pair of steps of a forward, or of an adjoint between two kernel
  * accumulations, on tiles of 4 or 8 rows spanning all nz columns, one
@@ -327,7 +336,8 @@ int inv_get_persistent(inv_ctx *ctx, int *mode);
  * inv_set_pair that nz and the device's LDS allow, kept if 2 % faster than
  * the best so far (inv_get_pair reports it); then, for grids one tile wide,
  * the persistent kernel, and over the choice so far three-step tiles of 12,
- * 16, 20 and 8 rows (inv_set_triple; inv_get_triple reports the choice),
+ * 16, 20 and 8 rows (inv_set_triple; inv_get_triple reports the choice) and
+ * walking chunks of 128, 96, 192 and 64 rows (inv_set_walk, inv_get_walk),
  * each kept if 2 % faster.  The
  * choice is cached per (nx, nz, wave types, math mode) for the process, so
  * later contexts and batch sizes of the same grid take it without timing.

@@ -1290,6 +1290,34 @@ int inv_get_triple(inv_ctx *ctx, int *tile_x) {
     });
 }
 
+int inv_set_walk(inv_ctx *ctx, int chunk_rows) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(chunk_rows == 0 || (chunk_rows >= 16 && chunk_rows <= 4096 &&
+                                        chunk_rows % 8 == 0),
+                    INV_ERR_ARG, "walk chunk must be a multiple of 8 rows in [16, 4096] (0: off)");
+        INV_REQUIRE(!c.have_grid || chunk_rows == 0 || c.g.nz % 2 == 0, INV_ERR_UNSUPPORTED,
+                    "the walking three-step kernel needs nz even");
+        if (chunk_rows) {
+            int dev = 0, optin = 0;
+            INV_HIP(hipGetDevice(&dev));
+            INV_HIP(hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev));
+            INV_REQUIRE(walk_lds_bytes(true) <= optin, INV_ERR_UNSUPPORTED,
+                        "the walking kernel needs more LDS per block than the device allows");
+        }
+        sync(c);
+        c.walk_ch = chunk_rows;
+        invalidate(c);
+        fused_invalidate_lists(c);
+    });
+}
+
+int inv_get_walk(inv_ctx *ctx, int *chunk_rows) {
+    return guard(ctx, [&](Ctx &c) {
+        INV_REQUIRE(chunk_rows, INV_ERR_ARG, "null output");
+        *chunk_rows = c.walk_ch;
+    });
+}
+
 // ---- autotuner -------------------------------------------------------------
 // Candidate step kernels and tile shapes, timed on the context's own shape;
 // the best is cached per (grid, slots, wave types, math mode) for the process.
@@ -1299,6 +1327,7 @@ struct TuneChoice {
     int pair_tx = 0, pair_v = 2;  // two-step kernel tile (0 rows: off)
     int persist = 0;              // persistent two-step kernel
     int trip = 0;                 // three-step kernel tile rows (0: off)
+    int walk = 0;                 // walking three-step kernel chunk rows (0: off)
 };
 std::mutex tune_mu;
 std::map<std::string, TuneChoice> tune_cache;
@@ -1311,6 +1340,7 @@ void apply_choice(Ctx &c, const TuneChoice &t) {
     c.pair_v = t.pair_v;
     c.persist = t.persist;
     c.trip_tx = t.trip;
+    c.walk_ch = t.walk;
     invalidate(c);
     fused_invalidate_lists(c);
 }
@@ -1386,9 +1416,9 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                     if (verbose)
                         fprintf(stderr,
                                 "inv_autotune %s: %d rows x %d cols per lane, two-step %d x %d%s, "
-                                "three-step %d: %.3f ms\n",
+                                "three-step %d, walk %d: %.3f ms\n",
                                 key.c_str(), t.tile, t.cols, t.pair_tx, t.pair_v,
-                                t.persist ? " persistent" : "", t.trip, best_t);
+                                t.persist ? " persistent" : "", t.trip, t.walk, best_t);
                     return best_t;
                 };
                 // one-step tiles, in order of preference (one column per
@@ -1458,6 +1488,24 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                     for (int tx : {12, 16, 20, 8}) {
                         TuneChoice t = two;
                         t.trip = tx;
+                        float ms;
+                        try {
+                            ms = timed(t, false);
+                        } catch (const Error &e) {
+                            if (e.code != INV_ERR_UNSUPPORTED) throw;
+                            continue;
+                        }
+                        if (ms < 0.98 * best_ms) {
+                            best_ms = ms;
+                            best = t;
+                        }
+                    }
+                    // the walking kernel's chunks (each workgroup walks one
+                    // chunk: long chunks leave CUs idle on small grids)
+                    for (int ch : {128, 96, 192, 64}) {
+                        if (ch > c.g.nx) continue;
+                        TuneChoice t = two;
+                        t.walk = ch;
                         float ms;
                         try {
                             ms = timed(t, false);

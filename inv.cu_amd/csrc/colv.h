@@ -81,6 +81,8 @@ __device__ __forceinline__ void window_reg(const float (&c)[V], float (&e)[3 * V
 // conditions go inside the scan's callback.
 struct LanePts {
     int p0, p1, cell, row;
+    bool sorted = false;        // entries in cell order (scan_pts may bisect)
+    int cell2 = -1, row2 = 0;   // entries 64 .. 127 (lane_pts_sorted)
 };
 
 __device__ __forceinline__ LanePts lane_pts(const int *cell, const int *row, int p0, int p1,
@@ -93,6 +95,19 @@ __device__ __forceinline__ LanePts lane_pts(const int *cell, const int *row, int
     return t;
 }
 
+// The same for a list whose buckets are in cell order, holding up to 128
+// entries in lanes (two per lane) for scan_pts' bisection.
+__device__ __forceinline__ LanePts lane_pts_sorted(const int *cell, const int *row, int p0,
+                                                   int p1, int lane) {
+    LanePts t = lane_pts(cell, row, p0, p1, lane);
+    t.sorted = true;
+    if (lane + 64 < p1 - p0) {
+        t.cell2 = cell[p0 + 64 + lane];
+        t.row2 = row[p0 + 64 + lane];
+    }
+    return t;
+}
+
 // f(cell, row) for the entries whose cell lies in rlo .. rhi, in list order.
 // Cell indices are those of one shot's grid (int); callers compute what
 // only a matching entry needs inside f, so a tile without entries (most of
@@ -101,6 +116,26 @@ template <class F>
 __device__ __forceinline__ void scan_pts(const LanePts &t, const int *cell, const int *row,
                                          int rlo, int rhi, F &&f) {
     const int n = t.p1 - t.p0, m = n < 64 ? n : 64;
+    if (t.sorted && n <= 128) {
+        // the first entry at or past rlo by bisection, then the run up to rhi
+        auto cell_at = [&](int q) {
+            return q < 64 ? __builtin_amdgcn_readlane(t.cell, q)
+                          : __builtin_amdgcn_readlane(t.cell2, q - 64);
+        };
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (cell_at(mid) < rlo) lo = mid + 1;
+            else hi = mid;
+        }
+        for (int q = lo; q < n; q++) {
+            const int cp = cell_at(q);
+            if (cp > rhi) break;
+            f(cp, q < 64 ? __builtin_amdgcn_readlane(t.row, q)
+                         : __builtin_amdgcn_readlane(t.row2, q - 64));
+        }
+        return;
+    }
     for (int q = 0; q < m; q++) {
         const int cp = __builtin_amdgcn_readlane(t.cell, q);
         if (cp < rlo || cp > rhi) continue;

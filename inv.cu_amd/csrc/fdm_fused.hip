@@ -669,6 +669,27 @@ static void build_tiled(Ctx &c, PointList &pl, int nsets, const std::vector<std:
                 hr[slot] = rows[s][p];
             }
         }
+    // each bucket in cell order (stable: entries on one cell keep their
+    // order, so every cell's sums are unchanged): the walking kernel finds a
+    // row's entries by bisection
+    {
+        std::vector<int> idx, tc, tr;
+        for (size_t t = 0; t + 1 < count.size(); t++) {
+            const int b0 = count[t], b1 = count[t + 1];
+            if (b1 - b0 < 2) continue;
+            idx.resize(b1 - b0);
+            for (int k = 0; k < b1 - b0; k++) idx[k] = b0 + k;
+            std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return hc[x] < hc[y]; });
+            tc.resize(idx.size());
+            tr.resize(idx.size());
+            for (size_t k = 0; k < idx.size(); k++) {
+                tc[k] = hc[idx[k]];
+                tr[k] = hr[idx[k]];
+            }
+            std::copy(tc.begin(), tc.end(), hc.begin() + b0);
+            std::copy(tr.begin(), tr.end(), hr.begin() + b0);
+        }
+    }
     // queued launches may still read the old lists: wait before replacing them
     INV_HIP(hipStreamSynchronize(c.stream));
     pl.ptr.ensure(count.size() * sizeof(int));
@@ -797,10 +818,12 @@ static void ensure_pair_rec_lists(Ctx &c) {
 // sources (velocity of the first step, -7 .. +7 around the tile: lo 7, hi 8),
 // explosions (stress of the first step, -6 .. +5: lo 6, hi 6), receivers
 // (owned cells: 0, 0).
+// The walking kernel's chunks take the lists of tiles of walk_ch rows.
+static int trip_rows(const Ctx &c) { return c.walk_ch > 0 ? c.walk_ch : c.trip_tx; }
 static bool trip_on(const Ctx &c) {
-    return c.trip_tx > 0 && c.variant == INV_VARIANT_FUSED && c.g.nz % 2 == 0;
+    return trip_rows(c) > 0 && c.variant == INV_VARIANT_FUSED && c.g.nz % 2 == 0;
 }
-static int trip_key(const Ctx &c) { return 600000 + c.trip_tx * 1000 + trip_strip(); }
+static int trip_key(const Ctx &c) { return 600000 + trip_rows(c) * 1000 + trip_strip(); }
 
 static PointList &trip_source_list(Ctx &c, int first, int count) {
     const long long k = list_id(first, count, trip_key(c));
@@ -809,7 +832,7 @@ static PointList &trip_source_list(Ctx &c, int first, int count) {
     PointList &pl = c.src_lists[k];
     std::vector<std::vector<int>> cells, rows;
     source_points(c, first, count, 0, cells, rows);
-    build_tiled(c, pl, count, cells, rows, c.trip_tx, trip_strip(), 7, 8);
+    build_tiled(c, pl, count, cells, rows, trip_rows(c), trip_strip(), 7, 8);
     return pl;
 }
 
@@ -820,7 +843,7 @@ static PointList &trip_stress_source_list(Ctx &c, int first, int count) {
     PointList &pl = c.src_slists[k];
     std::vector<std::vector<int>> cells, rows;
     source_points(c, first, count, 1, cells, rows);
-    build_tiled(c, pl, count, cells, rows, c.trip_tx, trip_strip(), 6, 6);
+    build_tiled(c, pl, count, cells, rows, trip_rows(c), trip_strip(), 6, 6);
     return pl;
 }
 
@@ -831,8 +854,8 @@ static void ensure_trip_rec_lists(Ctx &c) {
         cells[0].push_back(c.rec_ix[r] * c.g.nz + c.rec_iz[r]);
         rows[0].push_back(r);
     }
-    build_tiled(c, c.t_rec_v, 1, cells, rows, c.trip_tx, trip_strip(), 7, 8);
-    build_tiled(c, c.t_rec_t, 1, cells, rows, c.trip_tx, trip_strip(), 0, 0);
+    build_tiled(c, c.t_rec_v, 1, cells, rows, trip_rows(c), trip_strip(), 7, 8);
+    build_tiled(c, c.t_rec_t, 1, cells, rows, trip_rows(c), trip_strip(), 0, 0);
     c.t_rec_key = trip_key(c);
 }
 
@@ -1086,6 +1109,11 @@ static void pair_m(Ctx &c, const StepArgs &a, const StepRoles &r2, unsigned bloc
 
 template <bool FAST>
 static void trip_m(Ctx &c, const StepArgs &a, unsigned blocks) {
+    if (c.walk_ch > 0) {
+        if (c.g.sh) walk_launch<true, false, FAST>(c, a, blocks);
+        if (c.g.psv) walk_launch<false, true, FAST>(c, a, blocks);
+        return;
+    }
     if (c.g.sh) trip_launch<true, false, FAST>(c, a, blocks);
     if (c.g.psv) trip_launch<false, true, FAST>(c, a, blocks);
 }
@@ -1225,7 +1253,7 @@ static StepArgs trip_args(Ctx &c, const StepArgs &a, int first, int count, bool 
     StepArgs p = a;
     p.snap = nullptr;
     p.sexp = nullptr;
-    p.tiles_x = tiles_x_of(c, c.trip_tx);
+    p.tiles_x = tiles_x_of(c, trip_rows(c));
     p.tiles_z = (c.g.nz + trip_strip() - 1) / trip_strip();
     ensure_trip_rec_lists(c);
     if (adjoint) {

@@ -119,6 +119,9 @@ struct Ctx {
     // tune_trip: the autotuner times them too (the request server's lanes
     // turn it off)
     int trip_tx = 0;
+    // rows per chunk of the walking three-step kernel (0: off); when set,
+    // three-step launches walk chunks of this many rows instead of tiles
+    int walk_ch = 0;
     bool tune_trip = true;
     // persistent two-step kernel (fdm_resident.hip): 0 off, 1 on, 2 on with
     // every wait given up at once (tests the error path); its halo exchange

@@ -165,6 +165,16 @@ constexpr int trip_lds_bytes(bool psv, int rows) {
 template <bool SH, bool PSV, bool FAST>
 void trip_launch(Ctx &c, const StepArgs &a, unsigned blocks);
 
+// Walking three-step kernel (fdm_walk.hip): a workgroup walks down c.walk_ch
+// rows of one trip_strip() column strip; its point lists are those of
+// three-step tiles of walk_ch rows.  LDS: ring buffers of WALK_RING rows
+// (and three copied rows) of the stress fields and the velocity fields.
+// Launches without snapshot roles only.
+constexpr int WALK_RING = 25;
+constexpr int walk_lds_bytes(bool psv) { return 4 * 128 * (WALK_RING + 3) * (psv ? 5 : 3); }
+template <bool SH, bool PSV, bool FAST>
+void walk_launch(Ctx &c, const StepArgs &a, unsigned blocks);
+
 // Persistent two-step kernel (fdm_resident.hip): `iters` iterations of two
 // steps per launch on tiles of c.pair_tx = 4 or 8 rows x every column (nz <=
 // 240 with RES_V = 4 columns per lane), one workgroup of RES_NW wavefronts

@@ -109,6 +109,8 @@ SIGNATURES = {
     "inv_get_pair": [_vp, _vp, _vp],
     "inv_set_triple": [_vp, _i],
     "inv_get_triple": [_vp, _vp],
+    "inv_set_walk": [_vp, _i],
+    "inv_get_walk": [_vp, _vp],
     "inv_set_persistent": [_vp, _i],
     "inv_get_persistent": [_vp, _vp],
     "inv_autotune": [_vp, _i, _i, _i, _vp, _vp, _vp],
@@ -498,6 +500,16 @@ class Context:
     def get_triple(self):
         r = C.c_int()
         self._chk(self.L.inv_get_triple(self.h, C.byref(r)))
+        return r.value
+
+    def set_walk(self, chunk_rows=256):
+        """Three steps per launch by the walking kernel, chunks of chunk_rows
+        rows (a multiple of 8 in [16, 4096]; 0: off)."""
+        self._chk(self.L.inv_set_walk(self.h, chunk_rows))
+
+    def get_walk(self):
+        r = C.c_int()
+        self._chk(self.L.inv_get_walk(self.h, C.byref(r)))
         return r.value
 
     def set_persistent(self, mode=1):
