@@ -337,7 +337,9 @@ int inv_get_persistent(inv_ctx *ctx, int *mode);
  * the best so far (inv_get_pair reports it); then, for grids one tile wide,
  * the persistent kernel, and over the choice so far three-step tiles of 12,
  * 16, 20 and 8 rows (inv_set_triple; inv_get_triple reports the choice) and
- * walking chunks of 128, 96, 192 and 64 rows (inv_set_walk, inv_get_walk),
+ * walking chunks of 128, 96, 160, 192 and 64 rows (inv_set_walk,
+ * inv_get_walk; a launch's workgroups are shots x chunks x strips, so the
+ * best chunk depends on the shots per launch timed here),
  * each kept if 2 % faster.  The
  * choice is cached per (nx, nz, wave types, math mode) for the process, so
  * later contexts and batch sizes of the same grid take it without timing.

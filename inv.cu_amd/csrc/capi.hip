@@ -1502,7 +1502,7 @@ int inv_autotune(inv_ctx *ctx, int first, int count, int steps, int *variant, in
                     }
                     // the walking kernel's chunks (each workgroup walks one
                     // chunk: long chunks leave CUs idle on small grids)
-                    for (int ch : {128, 96, 192, 64}) {
+                    for (int ch : {128, 96, 160, 192, 64}) {
                         if (ch > c.g.nx) continue;
                         TuneChoice t = two;
                         t.walk = ch;
