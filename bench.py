@@ -76,10 +76,12 @@ SH_STEP_BYTES = 4 * (2 * 4 + 2)
 #                                 profiles/k_trip_sh_marmousi1474x480_64shots.md
 #   c4 walking three-step, chunks of 128 rows, 4 shots: 75.69 B per launch of three
 #                                 steps (algorithmic 68), profiles/k_walk_psv_4096x2048_4shots.md
+#   c2 walking three-step, chunks of 128 rows, 64 shots: 46.71 B per launch of three
+#                                 steps (algorithmic 40), profiles/k_walk_sh_marmousi1474x480_64shots.md
 MEASURED_TRAFFIC_PER_CELL = {("c1", 1, 4, 2, 16): 68.30, ("c4", 2, 20, 2, 4): 70.75,
                              ("c2", 2, 20, 2, 64): 40.37,
                              ("c4", 3, 12, 2, 4): 73.85, ("c2", 3, 16, 2, 64): 40.44,
-                             ("c4", 3, "walk128", 2, 4): 75.69}
+                             ("c4", 3, "walk128", 2, 4): 75.69, ("c2", 3, "walk128", 2, 64): 46.71}
 
 
 def log(msg):
