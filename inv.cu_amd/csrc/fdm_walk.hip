@@ -257,9 +257,13 @@ __global__ __launch_bounds__(512, 1) void k_walk(const StepArgs a, const int CH)
     // iterations i0 = xa - 24, xa - 16, ...: the first three build the
     // chunk's upper halo (rows below a phase's first row are skipped)
     const int K = (xe - xa) / 8 + 3;
-    load_row(rp_, xa - 16 + w);
+    // iteration 0 computes nothing: its two stress rows (xa-9, xa-8) are
+    // written here, and the loop starts at iteration 1
     load_stress(xa - 15 + w);
-    for (int k = 0; k < K; k++) {
+    put_stress(xa - 15 + w);
+    load_row(rp_, xa - 8 + w);
+    load_stress(xa - 7 + w);
+    for (int k = 1; k < K; k++) {
         const int i0 = xa - 24 + 8 * k;
         const int xo_ = i0 + w, xn = i0 + 8 + w;
         put_stress(i0 + 9 + w);
