@@ -127,7 +127,7 @@ def parse(argv=None):
                     help="persistent two-step kernel: 1 on (two-step tile 8 x 4), 0 off, -1 as "
                          "the autotuner chose")
     ap.add_argument("--triple", type=int, default=-1,
-                    help="three-step tile rows (8 or 12; 0 off; -1 as the autotuner chose)")
+                    help="three-step tile rows (8, 12, 16 or 20; 0 off; -1 as the autotuner chose)")
     ap.add_argument("--walk", type=int, default=-1,
                     help="walking three-step kernel chunk rows (0 off; -1 as the autotuner "
                          "chose)")
